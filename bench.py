@@ -1,0 +1,145 @@
+"""Benchmark of the PDHG hot path on one MI355X.
+
+Workload (the single-GPU configuration the headline metric is quoted on): one
+synthetic LP with m=400,000 rows (half equality, half <=), n=1,200,000
+columns of degree 2 + Poisson(6) (~9.6M nonzeros), N(0,1) values with 1% of
+the columns scaled by 1e3, bounds [0, U[1,100]], feasible and dual feasible by
+construction (seed 20261). A "step" is one PDHG iteration in the plain FP64
+form of the reference loop (A^T y with the fused primal step, then A xbar
+with the fused dual step), everything resident in HBM.
+
+Prints one JSON line: iterations/s over the timed steps, the roofline of the
+dominant kernel (A xbar + dual step) from HIP events on the engine's stream,
+and the reference's own CPU loop (oracle/_ref, compiled from the reference
+sources) timed on a bounded sample of the same instance.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "time-to-1e-6 KKT & PDHG iters/s vs plain FP64 GPU PDHG; % HBM roofline in SpMV"
+HBM_PEAK_GBS = 8000.0
+SEED = 20261
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(lp):
+    """The reference PdhgSolver::solve (oracle/_ref/libref.so, threads=1, its
+    default) on the same instance. Its loop can only stop on the wall-clock
+    deadline, checked every 256 iterations, so the sample is exactly 256
+    iterations: one run with time_limit=0 measures the setup (power iteration
+    and the CSR copy), a second run stops at iteration 256."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Reference, have_ref
+    if not have_ref():
+        return None
+    ref = Reference(lp)
+    _, it0, setup = ref.pdhg_timed(0.0)
+    _, it, total = ref.pdhg_timed(setup * 1.5 + 0.5)
+    if it <= 0:
+        return None
+    return {"value": it / (total - setup), "unit": "PDHG iterations/s", "cores": 1, "kind": "reference",
+            "sample": f"{it} iterations of the reference loop on the full instance "
+                      f"({total - setup:.1f} s after {setup:.1f} s setup), {os.cpu_count()} host cores "
+                      f"available, CPU {cpu_model()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--tile-rows", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is design-only here.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from nirnaya_amd.engine import Engine, MODE_REFERENCE, default_params, gen_mixed_lp
+
+    t0 = time.time()
+    lp = gen_mixed_lp(SEED, 400_000, 1_200_000, 6.0, 0.01, 1e3)
+    log(f"[bench] instance m={lp.m} n={lp.n} nnz={lp.nnz} generated in {time.time() - t0:.1f}s")
+
+    eng = Engine(0)
+    params = default_params(mode=MODE_REFERENCE, max_iters=args.warmup + args.steps + 64, check_termination=0,
+                            tile_nnz=args.tile, tile_rows=args.tile_rows)
+    sess = eng.session(lp, params)
+    log(f"[bench] session ready (upload + power iteration) in {sess.result().info.setup_seconds:.2f}s")
+    sess.run(args.warmup)
+    log(f"[bench] warmup {args.warmup} iterations done")
+    t = sess.time(args.steps)
+    log(f"[bench] timed {args.steps} iterations: {t.total_ms:.1f} ms")
+    sess.close()
+    eng.close()
+
+    it_s = args.steps / (t.total_ms / 1e3)
+    spmv_avg_s = t.spmv_ms / args.steps / 1e3
+    spmtv_avg_s = t.spmtv_ms / args.steps / 1e3
+    achieved = t.spmv_bytes / spmv_avg_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "spmv_cfg1_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] timing the reference CPU loop on a bounded sample ...")
+        cpu = cpu_baseline(lp)
+
+    out = {
+        "metric": METRIC,
+        "value": round(it_s, 2),
+        "unit": "PDHG iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t.total_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "single large LP, plain FP64 PDHG (reference form): m=400000 rows "
+                               "(200000 equality, 200000 <=), n=1200000 columns, degree 2+Poisson(6), "
+                               f"nnz={lp.nnz}, 1% columns x1e3, seed {SEED}",
+                   "m": lp.m, "n": lp.n, "nnz": lp.nnz},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_dual_ref (A xbar + fused dual step)",
+                     "bytes_per_launch": t.spmv_bytes, "avg_launch_us": spmv_avg_s * 1e6},
+        "kernels": {"spmtv_primal_avg_us": spmtv_avg_s * 1e6, "spmtv_bytes_per_launch": t.spmtv_bytes,
+                    "spmtv_gbs": t.spmtv_bytes / spmtv_avg_s / 1e9,
+                    "iteration_bytes": t.iter_bytes,
+                    "iteration_gbs": t.iter_bytes * args.steps / (t.total_ms / 1e3) / 1e9},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

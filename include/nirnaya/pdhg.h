@@ -1,0 +1,142 @@
+/* PDHG engine C ABI: one MI355X, one HIP stream per context, device-resident
+ * LP sessions.
+ *
+ * This ABI replaces the reference's internal operator
+ *     sov::PdhgSolver::solve(const Model&, const Options&, const Deadline&)
+ *         -> PdhgResult                         (src/gpu/pdhg_solver.hpp:14-25)
+ * called from the facade at include/sov/solver.hpp:237. The LP arrives in the
+ * reference's own storage (CSC with int64 column starts, int32 row indices,
+ * double values; src/core/csc_matrix.hpp:18-28) plus the bound and cost
+ * vectors of src/model/model.hpp:21-36, already presolved and scaled by the
+ * caller. Conventions: int return codes (0 = success, -1 = failure with the
+ * context's error string set), no exceptions across the boundary, caller-owned
+ * host buffers, device buffers owned by the context / session.
+ */
+#ifndef NIRNAYA_PDHG_H
+#define NIRNAYA_PDHG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NIR_API __attribute__((visibility("default")))
+
+typedef struct nir_ctx nir_ctx;
+typedef struct nir_session nir_session;
+
+/* min c^T x + obj_offset  s.t.  rowlo <= A x <= rowup,  collo <= x <= colup.
+ * Bounds with |b| >= 1e20 (or non-finite) are absent. */
+typedef struct {
+    int32_t nrow;
+    int32_t ncol;
+    const int64_t* col_start; /* ncol + 1 */
+    const int32_t* row_index; /* nnz, strictly ascending within a column */
+    const double* value;      /* nnz */
+    const double* cost;       /* ncol */
+    const double* collo;      /* ncol */
+    const double* colup;      /* ncol */
+    const double* rowlo;      /* nrow */
+    const double* rowup;      /* nrow */
+    double obj_offset;
+} nir_lp;
+
+enum { NIR_MODE_REFERENCE = 0, NIR_MODE_PDLP = 1 };
+enum { NIR_PREC_FP64 = 0, NIR_PREC_MIXED = 1 };
+
+typedef struct {
+    int32_t mode;              /* NIR_MODE_* */
+    int32_t precision;         /* NIR_PREC_* (reference mode is always FP64) */
+    int64_t max_iters;         /* iteration cap (reference mode: the loop bound K) */
+    double tol;                /* pdlp: relative KKT tolerance */
+    double polish_tol;         /* mixed: FP64 polish target, 0 = no polish */
+    int32_t eval_freq;         /* pdlp: iterations between KKT evaluations */
+    int32_t check_termination; /* reference mode: 1 = stop on the reference rule
+                                  (max primal violation <= 1e-6 at k>20, k%32==0),
+                                  0 = run exactly max_iters iterations */
+    int32_t use_graph;         /* capture iteration blocks in hipGraphs */
+    int32_t verbose;
+    int32_t tile_nnz;          /* SpMV tile: nonzeros per workgroup (0 = default) */
+    int32_t tile_rows;         /* SpMV tile: max rows per workgroup (0 = default) */
+    const double* x0;          /* optional warm start (ncol), NULL = cold */
+    const double* y0;          /* optional warm start (nrow), NULL = cold */
+    const double* unscale_row; /* optional: report KKT in units y*row, (Ax)/row */
+    const double* unscale_col; /* optional: x*col, (c - A^T y)/col */
+    int (*should_stop)(void* user); /* polled every 256 iterations; nonzero = stop */
+    void* stop_user;
+} nir_params;
+
+typedef struct {
+    int32_t status;         /* sov Status code: 0 optimal, 4 time limit, 6 iteration limit,
+                               11 numerical trouble */
+    int32_t restarts;
+    int64_t iterations;     /* iterations completed (reference semantics) */
+    int64_t polish_iterations;
+    double objective;       /* c^T x + obj_offset of the returned x (double-double) */
+    double tau, sigma;      /* step sizes in effect at the end */
+    double primal_weight;
+    double max_primal_viol; /* reference mode: last max row violation of A xbar */
+    double rel_primal, rel_dual, rel_gap; /* last relative KKT terms (pdlp) */
+    double setup_seconds;   /* upload + preprocessing */
+    double solve_seconds;   /* iteration loop */
+} nir_info;
+
+/* Per-kernel timing over a block of iterations, measured with HIP events on
+ * the session's stream around every launch of the two iteration kernels. */
+typedef struct {
+    int64_t iterations;
+    double total_ms;            /* events bracketing the whole block */
+    double spmv_ms;             /* summed durations: A xbar + dual step */
+    double spmtv_ms;            /* summed durations: A^T y + primal step */
+    double spmv_bytes;          /* algorithmic bytes of one A xbar launch */
+    double spmtv_bytes;         /* algorithmic bytes of one A^T y launch */
+    double iter_bytes;          /* algorithmic bytes of one whole iteration */
+} nir_timing;
+
+NIR_API void nir_params_default(nir_params* p);
+
+NIR_API int nir_ctx_create(int device, nir_ctx** out);
+NIR_API void nir_ctx_destroy(nir_ctx* ctx);
+NIR_API const char* nir_ctx_error(const nir_ctx* ctx);
+NIR_API const char* nir_last_error(void); /* errors before a context exists */
+
+NIR_API int nir_session_create(nir_ctx* ctx, const nir_lp* lp, const nir_params* p, nir_session** out);
+/* Runs at most `iters` more iterations (stops earlier on termination). */
+NIR_API int nir_session_run(nir_session* s, int64_t iters);
+/* Runs to termination, the iteration cap or should_stop. */
+NIR_API int nir_session_solve(nir_session* s);
+/* Copies the current iterate (x: ncol, y: nrow; either may be NULL). */
+NIR_API int nir_session_result(nir_session* s, double* x, double* y, nir_info* info);
+NIR_API int nir_session_time(nir_session* s, int64_t iters, nir_timing* out);
+NIR_API void nir_session_destroy(nir_session* s);
+
+/* One-shot: create, solve, copy out, destroy. */
+NIR_API int nir_pdhg_solve(nir_ctx* ctx, const nir_lp* lp, const nir_params* p, double* x, double* y,
+                           nir_info* info);
+
+/* ---- synthetic instances (seeded; generated on the host) ---------------- */
+typedef struct nir_owned_lp nir_owned_lp;
+
+/* m equality rows, n columns with `per_col` distinct uniform rows each, N(0,1)
+ * values, bounds [0, 10], b = A x0 with x0 ~ U[0,1], c = A^T y0 + s. */
+NIR_API nir_owned_lp* nir_gen_equality_lp(uint64_t seed, int32_t m, int32_t n, int32_t per_col);
+/* m rows (first half equality, second half <=), n columns with degree
+ * 2 + Poisson(mean) rows, N(0,1) values, a random `bad_frac` of the columns
+ * multiplied by `bad_scale`, bounds [0, U[1,100]]. */
+NIR_API nir_owned_lp* nir_gen_mixed_lp(uint64_t seed, int32_t m, int32_t n, double mean, double bad_frac,
+                                       double bad_scale);
+/* Multi-period blending/planning LP with T periods and K products. */
+NIR_API nir_owned_lp* nir_gen_staircase_lp(uint64_t seed, int32_t T, int32_t K);
+NIR_API int nir_owned_lp_view(const nir_owned_lp* g, nir_lp* out);
+NIR_API int64_t nir_owned_lp_nnz(const nir_owned_lp* g);
+/* The feasible point used to build the right-hand side (ncol), and the dual
+ * point used to build the costs (nrow). */
+NIR_API const double* nir_owned_lp_x0(const nir_owned_lp* g);
+NIR_API const double* nir_owned_lp_y0(const nir_owned_lp* g);
+NIR_API void nir_owned_lp_free(nir_owned_lp* g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,707 @@
+// PDHG engine: device-resident LP sessions on one MI355X.
+//
+// Reference mode restates src/gpu/pdhg_solver.hpp:25-107 on the device:
+//   power iteration for ||A||_2 (12 rounds, :32-45), x at the bound midpoint
+//   (:47-57), then per iteration
+//     primal kernel : g = A^T y over CSR(A^T) (the CSC arrays of A), fused
+//                     x+ = clamp(x - tau (c - g)), xbar = 2 x+ - x, x = x+
+//                     (:70-78)
+//     dual kernel   : h = A xbar over CSR(A), fused
+//                     y += sigma (clamp(h - y/sigma) - h) and the max row
+//                     violation of h (:82-95)
+//   and the reference's stop rule (k > 20, k % 32 == 0, violation <= 1e-6,
+//   :97-100) decided on the device: the next primal kernel reads the
+//   violation slot and raises a done flag that every later kernel honours.
+// The host only enqueues iteration blocks (hipGraph-captured) and polls the
+// done flag once per 256 iterations, which is also where the deadline is
+// checked (:63).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/nirnaya/pdhg.h"
+#include "pdlp.hpp"
+#include "spmv.hpp"
+
+namespace nir {
+namespace gpu {
+
+struct HipError : std::runtime_error {
+    explicit HipError(const std::string& what) : std::runtime_error(what) {}
+};
+
+#define HIP_CHECK(expr)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            throw HipError(std::string(#expr) + " failed: " + hipGetErrorString(e_));         \
+    } while (0)
+
+// ------------------------------------------------------------------ kernels
+
+struct RefState {
+    int done;      // raised when the stop rule fires
+    int base;      // iteration index of k_lo == 0
+    int iters;     // iterations completed when done was raised
+    int pad;
+    unsigned long long vslot[2];   // max row violation of iteration k in slot k & 1 (bits of a double >= 0)
+};
+
+template <int TILE>
+__global__ __launch_bounds__(kThreads) void k_spmv_store(Csr<double> A, const double* __restrict__ v,
+                                                         double* __restrict__ out) {
+    __shared__ double prod[TILE];
+    stream_rows<TILE>(A, v, prod, blockIdx.x, [&](int r, double s) { out[r] = s; });
+}
+
+template <int TILE>
+__global__ __launch_bounds__(kThreads) void k_primal_ref(Csr<double> AT, const double* __restrict__ y,
+                                                         double* __restrict__ x, double* __restrict__ xbar,
+                                                         const double* __restrict__ c,
+                                                         const double* __restrict__ lo,
+                                                         const double* __restrict__ up, double tau,
+                                                         RefState* st, int k_lo, int check) {
+    __shared__ double prod[TILE];
+    if (st->done) return;
+    const int k = st->base + k_lo;
+    // The reference tests its stop rule at the end of iteration k-1.
+    if (check && k > 21 && ((k - 1) & 31) == 0) {
+        const double V = __longlong_as_double(static_cast<long long>(st->vslot[(k - 1) & 1]));
+        if (V <= 1e-6) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                st->done = 1;
+                st->iters = k;
+            }
+            return;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->vslot[k & 1] = 0ull;
+    stream_rows<TILE>(AT, y, prod, blockIdx.x, [&](int j, double g) {
+        const double xj = x[j];
+        double cand = xj - tau * (c[j] - g);
+        const double l = lo[j], u = up[j];
+        if (finite_bound(l)) cand = amax(l, cand);
+        if (finite_bound(u)) cand = amin(u, cand);
+        xbar[j] = 2.0 * cand - xj;
+        x[j] = cand;
+    });
+}
+
+template <int TILE>
+__global__ __launch_bounds__(kThreads) void k_dual_ref(Csr<double> A, const double* __restrict__ xbar,
+                                                       double* __restrict__ y, const double* __restrict__ rl,
+                                                       const double* __restrict__ ru, double sigma,
+                                                       RefState* st, int k_lo) {
+    __shared__ double prod[TILE];
+    __shared__ double red[kWaves];
+    if (st->done) return;
+    const int k = st->base + k_lo;
+    double vmax = 0.0;
+    stream_rows<TILE>(A, xbar, prod, blockIdx.x, [&](int i, double h) {
+        const double yi = y[i];
+        double r = h - yi / sigma;
+        const double l = rl[i], u = ru[i];
+        const bool lf = finite_bound(l), uf = finite_bound(u);
+        if (lf) r = amax(l, r);
+        if (uf) r = amin(u, r);
+        y[i] = yi + sigma * (r - h);
+        double viol = 0.0;
+        if (lf) viol = amax(viol, l - h);
+        if (uf) viol = amax(viol, h - u);
+        vmax = amax(vmax, viol);
+    });
+    vmax = block_max(vmax, red);
+    if (threadIdx.x == 0 && vmax > 0.0)
+        atomicMax(&st->vslot[k & 1], static_cast<unsigned long long>(__double_as_longlong(vmax)));
+}
+
+__global__ void k_set_base(RefState* st, int base) { st->base = base; }
+__global__ void k_advance(RefState* st, int by) { st->base += by; }
+
+// ------------------------------------------------------------------ host helpers
+
+template <typename T>
+struct DevArray {
+    T* p = nullptr;
+    size_t n = 0;
+    DevArray() = default;
+    DevArray(const DevArray&) = delete;
+    DevArray& operator=(const DevArray&) = delete;
+    ~DevArray() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(count, 1) * sizeof(T)));
+    }
+    void upload(const T* src, size_t count, hipStream_t s) {
+        if (count > n) throw std::runtime_error("upload overflow");
+        if (count) HIP_CHECK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void download(T* dst, size_t count, hipStream_t s) const {
+        if (count > n) throw std::runtime_error("download overflow");
+        if (count) HIP_CHECK(hipMemcpyAsync(dst, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+};
+
+// Row blocks for CSR-stream: <= tile_rows rows and <= tile_nnz nonzeros per
+// block, or one row longer than tile_nnz on its own. Always >= 1 block.
+std::vector<int> partition_rows(const std::vector<int>& rowptr, int tile_nnz, int tile_rows) {
+    const int n = static_cast<int>(rowptr.size()) - 1;
+    std::vector<int> blk{0};
+    int r = 0;
+    while (r < n) {
+        const int s = r;
+        if (rowptr[r + 1] - rowptr[r] > tile_nnz) {
+            ++r;
+        } else {
+            while (r < n && r - s < tile_rows && rowptr[r + 1] - rowptr[s] <= tile_nnz) ++r;
+        }
+        blk.push_back(r);
+    }
+    if (blk.size() == 1) blk.push_back(0);
+    return blk;
+}
+
+template <typename T>
+struct DevCsr {
+    int nrows = 0, ncols = 0, nnz = 0;
+    DevArray<int> rowptr, col, blk;
+    DevArray<T> val;
+    int nblk = 0;
+    Csr<T> view() const { return Csr<T>{nrows, ncols, rowptr.p, col.p, val.p, nblk, blk.p}; }
+};
+
+template <typename T>
+void upload_csr(DevCsr<T>& d, int nrows, int ncols, const std::vector<int>& rowptr, const int* col, const T* val,
+                int tile_nnz, int tile_rows, hipStream_t s) {
+    d.nrows = nrows;
+    d.ncols = ncols;
+    d.nnz = rowptr.back();
+    d.rowptr.alloc(rowptr.size());
+    d.rowptr.upload(rowptr.data(), rowptr.size(), s);
+    d.col.alloc(d.nnz);
+    d.col.upload(col, d.nnz, s);
+    d.val.alloc(d.nnz);
+    d.val.upload(val, d.nnz, s);
+    std::vector<int> blk = partition_rows(rowptr, tile_nnz, tile_rows);
+    d.nblk = static_cast<int>(blk.size()) - 1;
+    d.blk.alloc(blk.size());
+    d.blk.upload(blk.data(), blk.size(), s);
+    HIP_CHECK(hipStreamSynchronize(s));   // host vectors may go out of scope
+}
+
+bool host_finite(double b) { return std::isfinite(b) && std::fabs(b) < 1e20; }
+
+}  // namespace gpu
+}  // namespace nir
+
+using namespace nir::gpu;
+
+// ------------------------------------------------------------------ context
+
+static thread_local std::string g_last_error;
+
+struct nir_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error;
+};
+
+struct nir_session {
+    nir_ctx* ctx = nullptr;
+    hipStream_t s = nullptr;
+    nir_params prm{};
+    int m = 0, n = 0;
+    long long nnz = 0;
+    int tile = 2048, tile_rows = 256;
+    double obj_offset = 0.0;
+    std::vector<double> h_cost;
+
+    // matrices
+    DevCsr<double> A, AT;
+    // vectors
+    DevArray<double> c, lo, up, rl, ru, x, xbar, y;
+    DevArray<RefState> st;
+    RefState* h_poll = nullptr;   // pinned: two poll slots
+
+    // progress
+    long long k_next = 0;     // next iteration index to enqueue
+    long long max_iters = 0;
+    int base_host = 0;
+    bool finished = false;
+    int status = 0;
+    long long iters_done = 0;
+    double tau = 0.0, sigma = 0.0;
+    double setup_seconds = 0.0, solve_seconds = 0.0;
+
+    // graph of 32 reference iterations
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipEvent_t poll_ev[2] = {nullptr, nullptr};
+    int poll_pending[2] = {0, 0};
+    long long poll_serial = 0;
+
+    // PDLP engine state (null in reference mode)
+    std::unique_ptr<nir::gpu::Pdlp, nir::gpu::PdlpDeleter> pdlp;
+
+    ~nir_session();
+};
+
+nir_session::~nir_session() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto& e : poll_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (h_poll) (void)hipHostFree(h_poll);
+    pdlp.reset();
+}
+
+namespace {
+
+void launch_spmv_store(nir_session* S, const DevCsr<double>& M, const double* v, double* out) {
+    switch (S->tile) {
+        case 1024: k_spmv_store<1024><<<M.nblk, kThreads, 0, S->s>>>(M.view(), v, out); break;
+        case 4096: k_spmv_store<4096><<<M.nblk, kThreads, 0, S->s>>>(M.view(), v, out); break;
+        default: k_spmv_store<2048><<<M.nblk, kThreads, 0, S->s>>>(M.view(), v, out); break;
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_primal_ref(nir_session* S, int k_lo) {
+    const int check = S->prm.check_termination ? 1 : 0;
+    auto AT = S->AT.view();
+    switch (S->tile) {
+        case 1024:
+            k_primal_ref<1024><<<AT.nblk, kThreads, 0, S->s>>>(AT, S->y.p, S->x.p, S->xbar.p, S->c.p, S->lo.p, S->up.p, S->tau, S->st.p, k_lo, check);
+            break;
+        case 4096:
+            k_primal_ref<4096><<<AT.nblk, kThreads, 0, S->s>>>(AT, S->y.p, S->x.p, S->xbar.p, S->c.p, S->lo.p, S->up.p, S->tau, S->st.p, k_lo, check);
+            break;
+        default:
+            k_primal_ref<2048><<<AT.nblk, kThreads, 0, S->s>>>(AT, S->y.p, S->x.p, S->xbar.p, S->c.p, S->lo.p, S->up.p, S->tau, S->st.p, k_lo, check);
+            break;
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_dual_ref(nir_session* S, int k_lo) {
+    auto A = S->A.view();
+    switch (S->tile) {
+        case 1024:
+            k_dual_ref<1024><<<A.nblk, kThreads, 0, S->s>>>(A, S->xbar.p, S->y.p, S->rl.p, S->ru.p, S->sigma, S->st.p, k_lo);
+            break;
+        case 4096:
+            k_dual_ref<4096><<<A.nblk, kThreads, 0, S->s>>>(A, S->xbar.p, S->y.p, S->rl.p, S->ru.p, S->sigma, S->st.p, k_lo);
+            break;
+        default:
+            k_dual_ref<2048><<<A.nblk, kThreads, 0, S->s>>>(A, S->xbar.p, S->y.p, S->rl.p, S->ru.p, S->sigma, S->st.p, k_lo);
+            break;
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+void enqueue_ref_iteration(nir_session* S, long long k) {
+    const int k_lo = static_cast<int>(k - S->base_host);
+    launch_primal_ref(S, k_lo);
+    launch_dual_ref(S, k_lo);
+}
+
+void build_ref_graph(nir_session* S) {
+    HIP_CHECK(hipStreamBeginCapture(S->s, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < 32; ++i) {
+        launch_primal_ref(S, i);
+        launch_dual_ref(S, i);
+    }
+    k_advance<<<1, 1, 0, S->s>>>(S->st.p, 32);
+    HIP_CHECK(hipStreamEndCapture(S->s, &S->graph));
+    HIP_CHECK(hipGraphInstantiate(&S->graph_exec, S->graph, nullptr, nullptr, 0));
+}
+
+// Enqueue iterations [k, k + count) of the reference loop.
+void enqueue_ref_block(nir_session* S, long long k, long long count) {
+    const long long end = k + count;
+    while (k < end) {
+        if (S->prm.use_graph && (k % 32) == 0 && end - k >= 32) {
+            if (!S->graph_exec) build_ref_graph(S);
+            if (S->base_host != k) {
+                k_set_base<<<1, 1, 0, S->s>>>(S->st.p, static_cast<int>(k));
+                HIP_CHECK(hipGetLastError());
+                S->base_host = static_cast<int>(k);
+            }
+            HIP_CHECK(hipGraphLaunch(S->graph_exec, S->s));
+            S->base_host += 32;
+            k += 32;
+        } else {
+            enqueue_ref_iteration(S, k);
+            ++k;
+        }
+    }
+}
+
+// Copy the state word to pinned slot `slot` and mark it pending.
+void post_poll(nir_session* S, int slot) {
+    HIP_CHECK(hipMemcpyAsync(&S->h_poll[slot], S->st.p, sizeof(RefState), hipMemcpyDeviceToHost, S->s));
+    HIP_CHECK(hipEventRecord(S->poll_ev[slot], S->s));
+    S->poll_pending[slot] = 1;
+}
+
+bool wait_poll(nir_session* S, int slot) {
+    if (!S->poll_pending[slot]) return false;
+    HIP_CHECK(hipEventSynchronize(S->poll_ev[slot]));
+    S->poll_pending[slot] = 0;
+    return S->h_poll[slot].done != 0;
+}
+
+void finish_ref(nir_session* S) {
+    HIP_CHECK(hipStreamSynchronize(S->s));
+    for (int i = 0; i < 2; ++i) S->poll_pending[i] = 0;
+    RefState h{};
+    HIP_CHECK(hipMemcpy(&h, S->st.p, sizeof(RefState), hipMemcpyDeviceToHost));
+    if (h.done) {
+        S->finished = true;
+        S->iters_done = h.iters;
+        S->status = 0;
+    } else {
+        S->iters_done = S->k_next;
+    }
+}
+
+void run_reference(nir_session* S, long long budget) {
+    auto t0 = std::chrono::steady_clock::now();
+    while (budget > 0 && !S->finished && S->k_next < S->max_iters) {
+        if ((S->k_next % 256) == 0 && S->prm.should_stop && S->prm.should_stop(S->prm.stop_user)) {
+            S->status = 4;   // TimeLimit
+            S->finished = true;
+            break;
+        }
+        const long long room = 256 - (S->k_next % 256);
+        const long long chunk = std::min({budget, room, S->max_iters - S->k_next});
+        enqueue_ref_block(S, S->k_next, chunk);
+        S->k_next += chunk;
+        budget -= chunk;
+        const int slot = static_cast<int>(S->poll_serial++ & 1);
+        post_poll(S, slot);
+        // Keep one block in flight: wait for the previous one only.
+        if (wait_poll(S, slot ^ 1)) break;
+    }
+    finish_ref(S);
+    if (S->k_next >= S->max_iters && !S->finished) S->finished = true;
+    S->solve_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Power iteration of the reference (12 rounds of u <- A^T A u / ||A^T A u||),
+// with the squared norm summed sequentially on the host as the reference does.
+void reference_step_sizes(nir_session* S) {
+    std::vector<double> u(static_cast<size_t>(S->n), 1.0);
+    DevArray<double> du, dv;
+    du.alloc(S->n);
+    dv.alloc(S->m);
+    du.upload(u.data(), u.size(), S->s);
+    double eta = 1.0;
+    for (int p = 0; p < 12; ++p) {
+        launch_spmv_store(S, S->A, du.p, dv.p);
+        launch_spmv_store(S, S->AT, dv.p, du.p);
+        du.download(u.data(), u.size(), S->s);
+        HIP_CHECK(hipStreamSynchronize(S->s));
+        double n2 = 0.0;
+        for (double v : u) n2 += v * v;
+        eta = std::sqrt(std::max(1e-12, n2));
+        for (double& v : u) v /= eta;
+        du.upload(u.data(), u.size(), S->s);
+    }
+    HIP_CHECK(hipStreamSynchronize(S->s));
+    const double smax = std::sqrt(eta);
+    S->tau = 0.85 / std::max(1e-6, smax);
+    S->sigma = 0.85 / std::max(1e-6, smax);
+}
+
+void check_lp(const nir_lp* lp) {
+    if (!lp) throw std::invalid_argument("null LP");
+    if (lp->nrow < 0 || lp->ncol < 0) throw std::invalid_argument("negative dimensions");
+    if (lp->ncol > 0 && (!lp->col_start || !lp->cost || !lp->collo || !lp->colup))
+        throw std::invalid_argument("missing column arrays");
+    if (lp->nrow > 0 && (!lp->rowlo || !lp->rowup)) throw std::invalid_argument("missing row arrays");
+    const int64_t nnz = lp->ncol > 0 ? lp->col_start[lp->ncol] : 0;
+    if (lp->ncol > 0 && lp->col_start[0] != 0) throw std::invalid_argument("col_start[0] != 0");
+    if (nnz < 0 || nnz >= (int64_t(1) << 31) - 1) throw std::invalid_argument("nnz out of int32 range");
+    if (nnz > 0 && (!lp->row_index || !lp->value)) throw std::invalid_argument("missing matrix arrays");
+    for (int32_t j = 0; j < lp->ncol; ++j)
+        if (lp->col_start[j + 1] < lp->col_start[j]) throw std::invalid_argument("col_start not non-decreasing");
+    // Every index is range-checked here: the kernels trust them.
+    for (int64_t p = 0; p < nnz; ++p)
+        if (lp->row_index[p] < 0 || lp->row_index[p] >= lp->nrow) throw std::invalid_argument("row index out of range");
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+NIR_API void nir_params_default(nir_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->mode = NIR_MODE_REFERENCE;
+    p->precision = NIR_PREC_FP64;
+    p->max_iters = 50000;
+    p->tol = 1e-6;
+    p->polish_tol = 0.0;
+    p->eval_freq = 64;
+    p->check_termination = 1;
+    p->use_graph = 1;
+}
+
+NIR_API const char* nir_last_error(void) { return g_last_error.c_str(); }
+
+NIR_API int nir_ctx_create(int device, nir_ctx** out) {
+    if (!out) return -1;
+    *out = nullptr;
+    try {
+        int count = 0;
+        HIP_CHECK(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count) throw std::runtime_error("no HIP device " + std::to_string(device));
+        HIP_CHECK(hipSetDevice(device));
+        auto* c = new nir_ctx;
+        c->device = device;
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            throw HipError(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        }
+        *out = c;
+        return 0;
+    } catch (const std::exception& ex) {
+        g_last_error = ex.what();
+        return -1;
+    }
+}
+
+NIR_API void nir_ctx_destroy(nir_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+NIR_API const char* nir_ctx_error(const nir_ctx* ctx) { return ctx ? ctx->error.c_str() : g_last_error.c_str(); }
+
+NIR_API int nir_session_create(nir_ctx* ctx, const nir_lp* lp, const nir_params* p, nir_session** out) {
+    if (!ctx || !out) return -1;
+    *out = nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    auto S = std::make_unique<nir_session>();
+    try {
+        check_lp(lp);
+        HIP_CHECK(hipSetDevice(ctx->device));
+        S->ctx = ctx;
+        S->s = ctx->stream;
+        if (p) S->prm = *p;
+        else nir_params_default(&S->prm);
+        if (S->prm.mode != NIR_MODE_REFERENCE && S->prm.mode != NIR_MODE_PDLP)
+            throw std::invalid_argument("unknown mode");
+        S->m = lp->nrow;
+        S->n = lp->ncol;
+        S->nnz = lp->ncol > 0 ? lp->col_start[lp->ncol] : 0;
+        S->obj_offset = lp->obj_offset;
+        S->max_iters = std::max<long long>(0, S->prm.max_iters);
+        if (S->prm.tile_nnz == 1024 || S->prm.tile_nnz == 2048 || S->prm.tile_nnz == 4096) S->tile = S->prm.tile_nnz;
+        else if (S->prm.tile_nnz != 0) throw std::invalid_argument("tile_nnz must be 0, 1024, 2048 or 4096");
+        if (S->prm.tile_rows > 0) S->tile_rows = std::min(S->prm.tile_rows, kThreads);
+
+        // CSR(A^T) is the CSC of A; CSR(A) by a counting transpose that keeps
+        // column indices ascending within each row.
+        const int m = S->m, n = S->n;
+        const int nnz = static_cast<int>(S->nnz);
+        std::vector<int> at_ptr(static_cast<size_t>(n) + 1);
+        for (int j = 0; j <= n; ++j) at_ptr[j] = n > 0 ? static_cast<int>(lp->col_start[j]) : 0;
+        if (n == 0) at_ptr.assign(1, 0);
+        std::vector<int> a_ptr(static_cast<size_t>(m) + 1, 0), a_col(static_cast<size_t>(nnz));
+        std::vector<double> a_val(static_cast<size_t>(nnz));
+        for (int p2 = 0; p2 < nnz; ++p2) ++a_ptr[lp->row_index[p2] + 1];
+        for (int i = 0; i < m; ++i) a_ptr[i + 1] += a_ptr[i];
+        {
+            std::vector<int> fill(a_ptr.begin(), a_ptr.end() - 1);
+            for (int j = 0; j < n; ++j)
+                for (int q = at_ptr[j]; q < at_ptr[j + 1]; ++q) {
+                    const int r = fill[lp->row_index[q]]++;
+                    a_col[r] = j;
+                    a_val[r] = lp->value[q];
+                }
+        }
+        upload_csr(S->A, m, n, a_ptr, a_col.data(), a_val.data(), S->tile, S->tile_rows, S->s);
+        upload_csr(S->AT, n, m, at_ptr, lp->row_index, lp->value, S->tile, S->tile_rows, S->s);
+
+        S->h_cost.assign(lp->cost, lp->cost + n);
+        if (S->prm.mode == NIR_MODE_PDLP) {
+            S->pdlp.reset(pdlp_create());
+            pdlp_setup(*S->pdlp, S->s, lp, S->prm, S->A.view(), S->AT.view(), S->tile, S->tile_rows);
+        } else {
+            S->c.alloc(n); S->c.upload(lp->cost, n, S->s);
+            S->lo.alloc(n); S->lo.upload(lp->collo, n, S->s);
+            S->up.alloc(n); S->up.upload(lp->colup, n, S->s);
+            S->rl.alloc(m); S->rl.upload(lp->rowlo, m, S->s);
+            S->ru.alloc(m); S->ru.upload(lp->rowup, m, S->s);
+            // Start point (pdhg_solver.hpp:47-57) unless a warm start is given.
+            std::vector<double> x0(static_cast<size_t>(n), 0.0), y0(static_cast<size_t>(m), 0.0);
+            for (int j = 0; j < n; ++j) {
+                const double l = lp->collo[j], u = lp->colup[j];
+                if (S->prm.x0) x0[j] = S->prm.x0[j];
+                else if (host_finite(l) && host_finite(u)) x0[j] = 0.5 * (l + u);
+                else if (host_finite(l)) x0[j] = std::max(0.0, l);
+                else if (host_finite(u)) x0[j] = std::min(0.0, u);
+            }
+            if (S->prm.y0) y0.assign(S->prm.y0, S->prm.y0 + m);
+            S->x.alloc(n); S->x.upload(x0.data(), n, S->s);
+            S->xbar.alloc(n); S->xbar.upload(x0.data(), n, S->s);
+            S->y.alloc(m); S->y.upload(y0.data(), m, S->s);
+            S->st.alloc(1);
+            RefState init{};
+            S->st.upload(&init, 1, S->s);
+            HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&S->h_poll), 2 * sizeof(RefState), hipHostMallocDefault));
+            std::memset(S->h_poll, 0, 2 * sizeof(RefState));
+            for (auto& e : S->poll_ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            reference_step_sizes(S.get());
+            HIP_CHECK(hipStreamSynchronize(S->s));
+        }
+        S->setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        *out = S.release();
+        return 0;
+    } catch (const std::exception& ex) {
+        ctx->error = ex.what();
+        g_last_error = ex.what();
+        return -1;
+    }
+}
+
+NIR_API int nir_session_run(nir_session* S, int64_t iters) {
+    if (!S) return -1;
+    try {
+        HIP_CHECK(hipSetDevice(S->ctx->device));
+        if (S->pdlp) pdlp_run(*S->pdlp, S->s, S->prm, iters);
+        else run_reference(S, iters);
+        return 0;
+    } catch (const std::exception& ex) {
+        S->ctx->error = ex.what();
+        return -1;
+    }
+}
+
+NIR_API int nir_session_solve(nir_session* S) {
+    if (!S) return -1;
+    return nir_session_run(S, INT64_MAX / 2);
+}
+
+NIR_API int nir_session_result(nir_session* S, double* x, double* y, nir_info* info) {
+    if (!S) return -1;
+    try {
+        HIP_CHECK(hipSetDevice(S->ctx->device));
+        if (S->pdlp) return pdlp_result(*S->pdlp, S->s, x, y, info);
+        std::vector<double> hx(static_cast<size_t>(S->n));
+        S->x.download(hx.data(), hx.size(), S->s);
+        if (y) S->y.download(y, static_cast<size_t>(S->m), S->s);
+        RefState h{};
+        S->st.download(&h, 1, S->s);
+        HIP_CHECK(hipStreamSynchronize(S->s));
+        if (x) std::copy(hx.begin(), hx.end(), x);
+        if (info) {
+            std::memset(info, 0, sizeof(*info));
+            info->status = S->status;
+            info->iterations = S->iters_done;
+            // The reference reports c^T x + offset of the final x in double-double.
+            nir::gpu::DDHost acc(S->obj_offset);
+            for (int j = 0; j < S->n; ++j)
+                if (S->h_cost[j] != 0.0 && hx[j] != 0.0) acc.add_prod(S->h_cost[j], hx[j]);
+            info->objective = acc.value();
+            info->tau = S->tau;
+            info->sigma = S->sigma;
+            info->primal_weight = 1.0;
+            const long long last = S->iters_done - 1;
+            info->max_primal_viol = last >= 0 ? __builtin_bit_cast(double, h.vslot[last & 1]) : 0.0;
+            info->setup_seconds = S->setup_seconds;
+            info->solve_seconds = S->solve_seconds;
+        }
+        return 0;
+    } catch (const std::exception& ex) {
+        S->ctx->error = ex.what();
+        return -1;
+    }
+}
+
+NIR_API int nir_session_time(nir_session* S, int64_t iters, nir_timing* out) {
+    if (!S || !out || iters <= 0) return -1;
+    try {
+        HIP_CHECK(hipSetDevice(S->ctx->device));
+        std::memset(out, 0, sizeof(*out));
+        if (S->pdlp) return pdlp_time(*S->pdlp, S->s, iters, out);
+        if (S->finished) throw std::runtime_error("session already finished");
+        const int m = S->m, n = S->n;
+        const double nnz = static_cast<double>(S->nnz);
+        out->spmv_bytes = 12.0 * nnz + 4.0 * (m + 1) + 8.0 * n + 32.0 * m;
+        out->spmtv_bytes = 12.0 * nnz + 4.0 * (n + 1) + 8.0 * m + 48.0 * n;
+        out->iter_bytes = out->spmv_bytes + out->spmtv_bytes;
+        std::vector<hipEvent_t> ev(static_cast<size_t>(4 * iters + 2));
+        for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+        HIP_CHECK(hipStreamSynchronize(S->s));
+        HIP_CHECK(hipEventRecord(ev[0], S->s));
+        long long k = S->k_next;
+        for (long long t = 0; t < iters; ++t, ++k) {
+            const int k_lo = static_cast<int>(k - S->base_host);
+            HIP_CHECK(hipEventRecord(ev[2 + 4 * t], S->s));
+            launch_primal_ref(S, k_lo);
+            HIP_CHECK(hipEventRecord(ev[3 + 4 * t], S->s));
+            HIP_CHECK(hipEventRecord(ev[4 + 4 * t], S->s));
+            launch_dual_ref(S, k_lo);
+            HIP_CHECK(hipEventRecord(ev[5 + 4 * t], S->s));
+        }
+        HIP_CHECK(hipEventRecord(ev[1], S->s));
+        HIP_CHECK(hipEventSynchronize(ev[1]));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        out->total_ms = ms;
+        for (long long t = 0; t < iters; ++t) {
+            HIP_CHECK(hipEventElapsedTime(&ms, ev[2 + 4 * t], ev[3 + 4 * t]));
+            out->spmtv_ms += ms;
+            HIP_CHECK(hipEventElapsedTime(&ms, ev[4 + 4 * t], ev[5 + 4 * t]));
+            out->spmv_ms += ms;
+        }
+        for (auto& e : ev) HIP_CHECK(hipEventDestroy(e));
+        out->iterations = iters;
+        S->k_next = k;
+        finish_ref(S);
+        return 0;
+    } catch (const std::exception& ex) {
+        S->ctx->error = ex.what();
+        return -1;
+    }
+}
+
+NIR_API void nir_session_destroy(nir_session* S) {
+    if (!S) return;
+    (void)hipSetDevice(S->ctx->device);
+    (void)hipStreamSynchronize(S->s);
+    delete S;
+}
+
+NIR_API int nir_pdhg_solve(nir_ctx* ctx, const nir_lp* lp, const nir_params* p, double* x, double* y,
+                           nir_info* info) {
+    nir_session* S = nullptr;
+    if (nir_session_create(ctx, lp, p, &S) != 0) return -1;
+    int rc = nir_session_solve(S);
+    if (rc == 0) rc = nir_session_result(S, x, y, info);
+    nir_session_destroy(S);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,47 @@
+// Restarted PDHG (PDLP-style) engine interface, used by engine.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "../../../include/nirnaya/pdhg.h"
+#include "spmv.hpp"
+
+namespace nir {
+namespace gpu {
+
+// Host double-double accumulator (two-sum / FMA two-product).
+struct DDHost {
+    double hi = 0.0, lo = 0.0;
+    explicit DDHost(double h = 0.0) : hi(h) {}
+    void add(double bh, double bl) {
+        double s1 = hi + bh, v = s1 - hi, s2 = (hi - (s1 - v)) + (bh - v);
+        double t1 = lo + bl, w = t1 - lo, t2 = (lo - (t1 - w)) + (bl - w);
+        s2 += t1;
+        double u = s1 + s2;
+        s2 = s2 - (u - s1);
+        s1 = u;
+        s2 += t2;
+        hi = s1 + s2;
+        lo = s2 - (hi - s1);
+    }
+    void add_prod(double a, double b) {
+        const double p = a * b;
+        add(p, std::fma(a, b, -p));
+    }
+    double value() const { return hi + lo; }
+};
+
+struct Pdlp;
+struct PdlpDeleter {
+    void operator()(Pdlp* p) const;
+};
+Pdlp* pdlp_create();
+void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm, Csr<double> A, Csr<double> AT,
+                int tile, int tile_rows);
+void pdlp_run(Pdlp& P, hipStream_t s, const nir_params& prm, long long budget);
+int pdlp_result(Pdlp& P, hipStream_t s, double* x, double* y, nir_info* info);
+int pdlp_time(Pdlp& P, hipStream_t s, long long iters, nir_timing* out);
+
+}  // namespace gpu
+}  // namespace nir

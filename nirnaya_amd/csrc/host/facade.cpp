@@ -1,0 +1,245 @@
+#include "facade.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <fstream>
+
+#include "../../../include/nirnaya/pdhg.h"
+
+namespace nir {
+namespace {
+
+int deadline_poll(void* user) { return static_cast<const Deadline*>(user)->expired() ? 1 : 0; }
+
+struct EngineRun {
+    Status status = Status::InternalError;
+    std::vector<double> x, y;
+    long long iterations = 0;
+    nir_info info{};
+    std::string error;
+};
+
+// Hands the presolved, scaled, internal-min LP to the GPU engine; this is the
+// call site of the reference's PdhgSolver::solve (solver.hpp:235-244).
+EngineRun run_engine(const Model& lp, const Options& opt, const Deadline& deadline, const Scaling& sc,
+                     const std::vector<double>* wx, const std::vector<double>* wy) {
+    EngineRun r;
+    nir_lp v{};
+    v.nrow = lp.nrow;
+    v.ncol = lp.ncol;
+    v.col_start = lp.A.start.data();
+    v.row_index = lp.A.index.data();
+    v.value = lp.A.value.data();
+    v.cost = lp.cost.data();
+    v.collo = lp.collo.data();
+    v.colup = lp.colup.data();
+    v.rowlo = lp.rowlo.data();
+    v.rowup = lp.rowup.data();
+    v.obj_offset = lp.obj_offset;
+
+    nir_params p;
+    nir_params_default(&p);
+    const bool pdlp = opt.pdhg.mode == "pdlp";
+    p.mode = pdlp ? NIR_MODE_PDLP : NIR_MODE_REFERENCE;
+    p.precision = (pdlp && opt.pdhg.precision == "mixed") ? NIR_PREC_MIXED : NIR_PREC_FP64;
+    // Reference cap: max(5000, simplex_iter_limit / 10) (pdhg_solver.hpp:60).
+    long long cap = std::max<long long>(5000, opt.simplex_iter_limit / 10);
+    if (opt.pdhg.max_iters > 0) cap = opt.pdhg.max_iters;
+    p.max_iters = cap;
+    p.tol = opt.pdhg.tol;
+    p.polish_tol = opt.pdhg.polish_tol;
+    p.eval_freq = opt.pdhg.eval_freq;
+    p.check_termination = 1;
+    p.use_graph = opt.pdhg.use_graph;
+    p.verbose = opt.pdhg.verbose;
+    p.x0 = (wx && !wx->empty()) ? wx->data() : nullptr;
+    p.y0 = (wy && !wy->empty()) ? wy->data() : nullptr;
+    if (sc.active) {
+        p.unscale_row = sc.row.data();
+        p.unscale_col = sc.col.data();
+    }
+    p.should_stop = &deadline_poll;
+    p.stop_user = const_cast<Deadline*>(&deadline);
+
+    nir_ctx* ctx = nullptr;
+    if (nir_ctx_create(opt.pdhg.device, &ctx) != 0) {
+        r.error = std::string("GPU engine unavailable: ") + nir_last_error();
+        return r;
+    }
+    r.x.assign(static_cast<size_t>(lp.ncol), 0.0);
+    r.y.assign(static_cast<size_t>(lp.nrow), 0.0);
+    const int rc = nir_pdhg_solve(ctx, &v, &p, r.x.data(), r.y.data(), &r.info);
+    if (rc != 0) {
+        r.error = std::string("GPU engine failed: ") + nir_ctx_error(ctx);
+        nir_ctx_destroy(ctx);
+        return r;
+    }
+    nir_ctx_destroy(ctx);
+    r.status = static_cast<Status>(r.info.status);
+    r.iterations = r.info.iterations;
+    return r;
+}
+
+}  // namespace
+
+SolveResult Solver::solve(const Model& user_model) {
+    SolveResult out;
+    stop_flag.store(false, std::memory_order_relaxed);
+    const int L = options.log_level;
+    Deadline deadline;
+    deadline.wall_limit_sec = options.time_limit;
+    deadline.work_limit = options.work_limit;
+    deadline.stop_flag = &stop_flag;
+
+    Model valid = user_model;
+    const Status vs = valid.validate(out.message);
+    if (vs != Status::Optimal) {
+        out.status = vs;
+        out.solve_time_sec = deadline.elapsed();
+        return out;
+    }
+    log_msg(L, 1, "========================================================================\n");
+    log_msg(L, 1, "  NIRNAYA-MI355X | model %s | rows %d | cols %d | nnz %lld | ints %d | qnnz %lld\n",
+            valid.name.c_str(), valid.nrow, valid.ncol, static_cast<long long>(valid.A.nnz()), valid.num_integers(),
+            static_cast<long long>(valid.Q.nnz()));
+    log_msg(L, 1, "========================================================================\n");
+
+    const Model internal = valid.to_internal_min();
+    Presolved pre = presolve(internal, options.presolve, options.presolve_rounds);
+    if (options.presolve > 0 && (pre.rows_removed > 0 || pre.cols_removed > 0 || pre.tightenings > 0))
+        log_msg(L, 1, "Presolve: %d rows, %d cols remaining (removed %d rows, %d cols, %d tightenings)\n",
+                pre.reduced.nrow, pre.reduced.ncol, pre.rows_removed, pre.cols_removed, pre.tightenings);
+    if (pre.solved) {
+        out.status = pre.status;
+        if (pre.status == Status::Optimal) {
+            pre.postsolve({}, out.x);
+            out.has_incumbent = true;
+            out.objective = valid.eval_objective(out.x);
+            out.best_bound = out.objective;
+            out.verification = verify_solution(valid, out.x, out.objective, options.tol.checker_tol);
+        }
+        out.solve_time_sec = deadline.elapsed();
+        return out;
+    }
+
+    Scaling sc;
+    const Model scaled = scale_geometric(pre.reduced, options.scale, sc);
+    std::vector<double> x_red(static_cast<size_t>(pre.reduced.ncol), 0.0);
+    std::vector<double> y_red(static_cast<size_t>(pre.reduced.nrow), 0.0);
+
+    if (scaled.is_qp() || scaled.is_mip()) {
+        out.status = Status::NotSupported;
+        out.message = scaled.is_qp() ? "Quadratic objectives are not handled by the PDHG engine"
+                                     : "Integer variables are not handled by the PDHG engine";
+    } else if (options.method != "pdhg" && options.method != "gpu") {
+        out.status = Status::NotSupported;
+        out.message = "method '" + options.method + "' is not built; use method=pdhg";
+    } else {
+        log_msg(L, 1, "Solving LP via GPU PDHG (%s, %s) on MI355X...\n", options.pdhg.mode.c_str(),
+                options.pdhg.precision.c_str());
+        // Warm start in the reduced, scaled space.
+        std::vector<double> wx, wy;
+        if (start_x.size() == static_cast<size_t>(valid.ncol)) {
+            wx.resize(static_cast<size_t>(scaled.ncol));
+            for (Idx k = 0; k < scaled.ncol; ++k) wx[k] = start_x[pre.col_map[k]] / (sc.active ? sc.col[k] : 1.0);
+        }
+        if (start_y.size() == static_cast<size_t>(valid.nrow)) {
+            const double sgn = valid.sense == Sense::Maximize ? -1.0 : 1.0;
+            wy.resize(static_cast<size_t>(scaled.nrow));
+            for (Idx k = 0; k < scaled.nrow; ++k) wy[k] = sgn * start_y[pre.row_map[k]] / (sc.active ? sc.row[k] : 1.0);
+        }
+        EngineRun er = run_engine(scaled, options, deadline, sc, &wx, &wy);
+        if (!er.error.empty()) {
+            out.status = Status::InternalError;
+            out.message = er.error;
+        } else {
+            out.status = er.status;
+            out.iterations = static_cast<Idx>(std::min<long long>(er.iterations, INT32_MAX));
+            out.engine_iterations = er.iterations + er.info.polish_iterations;
+            out.engine_seconds = er.info.setup_seconds + er.info.solve_seconds;
+            out.rel_primal = er.info.rel_primal;
+            out.rel_dual = er.info.rel_dual;
+            out.rel_gap = er.info.rel_gap;
+            if (out.status == Status::Optimal) {
+                sc.unscale_primal(er.x, x_red);
+                sc.unscale_duals(er.y, y_red);
+                out.has_incumbent = true;
+            }
+        }
+    }
+
+    if (out.has_incumbent) {
+        pre.postsolve(x_red, out.x);
+        valid.A.multiply(out.x, out.row_activity);
+        out.objective = valid.eval_objective(out.x);
+        out.best_bound = out.objective;
+        out.gap = 0.0;
+        const double sgn = valid.sense == Sense::Maximize ? -1.0 : 1.0;
+        out.row_duals.assign(static_cast<size_t>(valid.nrow), 0.0);
+        for (size_t i = 0; i < pre.row_map.size() && i < y_red.size(); ++i) out.row_duals[pre.row_map[i]] = sgn * y_red[i];
+        out.reduced_costs.assign(static_cast<size_t>(valid.ncol), 0.0);
+
+        out.verification = verify_solution(valid, out.x, out.objective, options.tol.checker_tol);
+        if (!out.verification.passed && out.status == Status::Optimal) {
+            if (options.presolve > 0 || options.scale) {
+                Options fo = options;
+                fo.presolve = 0;
+                fo.scale = false;
+                fo.log_level = 0;
+                Solver fb_solver(fo);
+                fb_solver.start_x = start_x;
+                fb_solver.start_y = start_y;
+                SolveResult fb = fb_solver.solve(user_model);
+                if (fb.verification.passed && fb.status == Status::Optimal) {
+                    fb.solve_time_sec = deadline.elapsed();
+                    return fb;
+                }
+            }
+            out.status = Status::NumericalTrouble;
+            out.message = "Verification residual exceeded tolerance on " + out.verification.worst_item;
+        }
+    }
+    out.solve_time_sec = deadline.elapsed();
+
+    log_msg(L, 1, "------------------------------------------------------------------------\n");
+    log_msg(L, 1, "  Status:     %s\n", status_name(out.status));
+    if (!out.message.empty()) log_msg(L, 1, "  Message:    %s\n", out.message.c_str());
+    if (out.has_incumbent) {
+        log_msg(L, 1, "  Objective:  %.12g\n", out.objective);
+        log_msg(L, 1, "  Check (dd): PASS=%s | MaxRowViol=%.2e | MaxBndViol=%.2e | MaxIntViol=%.2e\n",
+                out.verification.passed ? "YES" : "NO", out.verification.max_row_viol, out.verification.max_bound_viol,
+                out.verification.max_int_viol);
+    }
+    log_msg(L, 1, "  Iterations: %d | Time: %.4f s\n", out.iterations, out.solve_time_sec);
+    log_msg(L, 1, "========================================================================\n");
+
+    if (!options.sol_path.empty() && out.has_incumbent)
+        write_sol(options.sol_path, valid, out.status, out.objective, out.gap, out.x);
+    if (!options.stats_json.empty()) write_stats_json(options.stats_json, valid, out);
+    return out;
+}
+
+bool Solver::write_stats_json(const std::string& path, const Model& m, const SolveResult& r) {
+    std::FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) return false;
+    auto fin = [](double v) { return std::isfinite(v) ? v : 0.0; };
+    std::fprintf(f, "{\n");
+    std::fprintf(f, "  \"solver\": \"NIRNAYA-MI355X\",\n");
+    std::fprintf(f, "  \"model\": \"%s\",\n", m.name.c_str());
+    std::fprintf(f, "  \"status\": \"%s\",\n", status_name(r.status));
+    std::fprintf(f, "  \"objective\": %.17g,\n", fin(r.objective));
+    std::fprintf(f, "  \"best_bound\": %.17g,\n", fin(r.best_bound));
+    std::fprintf(f, "  \"gap\": %.17g,\n", fin(r.gap));
+    std::fprintf(f, "  \"iterations\": %d,\n", r.iterations);
+    std::fprintf(f, "  \"nodes\": %d,\n", r.nodes);
+    std::fprintf(f, "  \"cuts_added\": %d,\n", r.cuts_added);
+    std::fprintf(f, "  \"solve_time_sec\": %.9g,\n", r.solve_time_sec);
+    std::fprintf(f, "  \"engine_iterations\": %lld,\n", r.engine_iterations);
+    std::fprintf(f, "  \"engine_seconds\": %.9g,\n", r.engine_seconds);
+    std::fprintf(f, "  \"verified\": %s\n", r.verification.passed ? "true" : "false");
+    std::fprintf(f, "}\n");
+    std::fclose(f);
+    return true;
+}
+
+}  // namespace nir

@@ -1,0 +1,327 @@
+"""ctypes binding of the PDHG engine C ABI (include/nirnaya/pdhg.h) exported by
+bin/libsov.so.
+
+The engine replaces the reference's internal operator
+``PdhgSolver::solve(const Model&, const Options&, const Deadline&)``
+(src/gpu/pdhg_solver.hpp:25); this module lets tests and the benchmark drive it
+directly with numpy arrays. Everything runs in the native library: there is no
+Python or CPU fallback, and a missing library raises ``EngineUnavailable``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_ROOT, "bin", "libsov.so")
+
+MODE_REFERENCE = 0
+MODE_PDLP = 1
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+
+class EngineUnavailable(RuntimeError):
+    """The native library is missing or no GPU can be opened."""
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+_STOP_CB = C.CFUNCTYPE(C.c_int, C.c_void_p)
+
+
+class NirLp(C.Structure):
+    _fields_ = [
+        ("nrow", C.c_int32), ("ncol", C.c_int32),
+        ("col_start", _i64p), ("row_index", _i32p), ("value", _dp),
+        ("cost", _dp), ("collo", _dp), ("colup", _dp), ("rowlo", _dp), ("rowup", _dp),
+        ("obj_offset", C.c_double),
+    ]
+
+
+class NirParams(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32), ("precision", C.c_int32), ("max_iters", C.c_int64),
+        ("tol", C.c_double), ("polish_tol", C.c_double), ("eval_freq", C.c_int32),
+        ("check_termination", C.c_int32), ("use_graph", C.c_int32), ("verbose", C.c_int32),
+        ("tile_nnz", C.c_int32), ("tile_rows", C.c_int32),
+        ("x0", _dp), ("y0", _dp), ("unscale_row", _dp), ("unscale_col", _dp),
+        ("should_stop", _STOP_CB), ("stop_user", C.c_void_p),
+    ]
+
+
+class NirInfo(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("restarts", C.c_int32), ("iterations", C.c_int64),
+        ("polish_iterations", C.c_int64), ("objective", C.c_double), ("tau", C.c_double),
+        ("sigma", C.c_double), ("primal_weight", C.c_double), ("max_primal_viol", C.c_double),
+        ("rel_primal", C.c_double), ("rel_dual", C.c_double), ("rel_gap", C.c_double),
+        ("setup_seconds", C.c_double), ("solve_seconds", C.c_double),
+    ]
+
+
+class NirTiming(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int64), ("total_ms", C.c_double), ("spmv_ms", C.c_double),
+        ("spmtv_ms", C.c_double), ("spmv_bytes", C.c_double), ("spmtv_bytes", C.c_double),
+        ("iter_bytes", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load bin/libsov.so once; raise EngineUnavailable when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise EngineUnavailable(f"{path} is not built (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    lib.nir_params_default.argtypes = [C.POINTER(NirParams)]
+    lib.nir_last_error.restype = C.c_char_p
+    lib.nir_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    lib.nir_ctx_destroy.argtypes = [vp]
+    lib.nir_ctx_error.argtypes = [vp]
+    lib.nir_ctx_error.restype = C.c_char_p
+    lib.nir_session_create.argtypes = [vp, C.POINTER(NirLp), C.POINTER(NirParams), C.POINTER(vp)]
+    lib.nir_session_run.argtypes = [vp, C.c_int64]
+    lib.nir_session_solve.argtypes = [vp]
+    lib.nir_session_result.argtypes = [vp, _dp, _dp, C.POINTER(NirInfo)]
+    lib.nir_session_time.argtypes = [vp, C.c_int64, C.POINTER(NirTiming)]
+    lib.nir_session_destroy.argtypes = [vp]
+    lib.nir_pdhg_solve.argtypes = [vp, C.POINTER(NirLp), C.POINTER(NirParams), _dp, _dp, C.POINTER(NirInfo)]
+    for name in ("nir_gen_equality_lp", "nir_gen_mixed_lp", "nir_gen_staircase_lp"):
+        getattr(lib, name).restype = vp
+    lib.nir_gen_equality_lp.argtypes = [C.c_uint64, C.c_int32, C.c_int32, C.c_int32]
+    lib.nir_gen_mixed_lp.argtypes = [C.c_uint64, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double]
+    lib.nir_gen_staircase_lp.argtypes = [C.c_uint64, C.c_int32, C.c_int32]
+    lib.nir_owned_lp_view.argtypes = [vp, C.POINTER(NirLp)]
+    lib.nir_owned_lp_nnz.argtypes = [vp]
+    lib.nir_owned_lp_nnz.restype = C.c_int64
+    lib.nir_owned_lp_x0.argtypes = [vp]
+    lib.nir_owned_lp_x0.restype = _dp
+    lib.nir_owned_lp_y0.argtypes = [vp]
+    lib.nir_owned_lp_y0.restype = _dp
+    lib.nir_owned_lp_free.argtypes = [vp]
+    _lib = lib
+    return lib
+
+
+def _ptr(a: Optional[np.ndarray], ctype):
+    if a is None:
+        return C.cast(None, C.POINTER(ctype))
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+@dataclass
+class LP:
+    """An LP in the reference's CSC layout (numpy arrays, owned)."""
+    m: int
+    n: int
+    start: np.ndarray   # int64, n + 1
+    index: np.ndarray   # int32, nnz
+    value: np.ndarray   # float64, nnz
+    c: np.ndarray
+    l: np.ndarray
+    u: np.ndarray
+    rl: np.ndarray
+    ru: np.ndarray
+    offset: float = 0.0
+    x0: Optional[np.ndarray] = None   # generator's feasible point, if any
+    y0: Optional[np.ndarray] = None
+
+    @property
+    def nnz(self) -> int:
+        return int(self.start[-1]) if self.n > 0 else 0
+
+    def struct(self) -> NirLp:
+        for name in ("start", "index", "value", "c", "l", "u", "rl", "ru"):
+            a = getattr(self, name)
+            if not a.flags["C_CONTIGUOUS"]:
+                setattr(self, name, np.ascontiguousarray(a))
+        s = NirLp()
+        s.nrow, s.ncol = self.m, self.n
+        s.col_start = _ptr(self.start, C.c_int64)
+        s.row_index = _ptr(self.index, C.c_int32)
+        s.value = _ptr(self.value, C.c_double)
+        s.cost = _ptr(self.c, C.c_double)
+        s.collo = _ptr(self.l, C.c_double)
+        s.colup = _ptr(self.u, C.c_double)
+        s.rowlo = _ptr(self.rl, C.c_double)
+        s.rowup = _ptr(self.ru, C.c_double)
+        s.obj_offset = float(self.offset)
+        return s
+
+    @staticmethod
+    def from_dense(A, c, l, u, rl, ru, offset=0.0) -> "LP":
+        A = np.asarray(A, dtype=np.float64)
+        m, n = A.shape
+        start, index, value = [0], [], []
+        for j in range(n):
+            for i in range(m):
+                if A[i, j] != 0.0:
+                    index.append(i)
+                    value.append(A[i, j])
+            start.append(len(index))
+        f = lambda v: np.ascontiguousarray(np.asarray(v, dtype=np.float64))
+        return LP(m, n, np.asarray(start, np.int64), np.asarray(index, np.int32), f(value) if value else np.zeros(0),
+                  f(c), f(l), f(u), f(rl), f(ru), offset)
+
+    def dense(self) -> np.ndarray:
+        A = np.zeros((self.m, self.n))
+        for j in range(self.n):
+            for p in range(self.start[j], self.start[j + 1]):
+                A[self.index[p], j] = self.value[p]
+        return A
+
+
+def _from_owned(lib, g) -> LP:
+    if not g:
+        raise EngineError("generator rejected its arguments")
+    v = NirLp()
+    lib.nir_owned_lp_view(g, C.byref(v))
+    m, n = v.nrow, v.ncol
+    nnz = lib.nir_owned_lp_nnz(g)
+    arr = lambda p, k, t: np.ctypeslib.as_array(p, shape=(k,)).astype(t, copy=True) if k > 0 else np.zeros(0, t)
+    lp = LP(m, n, arr(v.col_start, n + 1, np.int64), arr(v.row_index, nnz, np.int32), arr(v.value, nnz, np.float64),
+            arr(v.cost, n, np.float64), arr(v.collo, n, np.float64), arr(v.colup, n, np.float64),
+            arr(v.rowlo, m, np.float64), arr(v.rowup, m, np.float64), 0.0,
+            arr(lib.nir_owned_lp_x0(g), n, np.float64), arr(lib.nir_owned_lp_y0(g), m, np.float64))
+    lib.nir_owned_lp_free(g)
+    return lp
+
+
+def gen_equality_lp(seed: int, m: int, n: int, per_col: int = 5) -> LP:
+    lib = load_library()
+    return _from_owned(lib, lib.nir_gen_equality_lp(seed, m, n, per_col))
+
+
+def gen_mixed_lp(seed: int, m: int, n: int, mean: float = 6.0, bad_frac: float = 0.01,
+                 bad_scale: float = 1e3) -> LP:
+    lib = load_library()
+    return _from_owned(lib, lib.nir_gen_mixed_lp(seed, m, n, mean, bad_frac, bad_scale))
+
+
+def gen_staircase_lp(seed: int, T: int, K: int) -> LP:
+    lib = load_library()
+    return _from_owned(lib, lib.nir_gen_staircase_lp(seed, T, K))
+
+
+@dataclass
+class Result:
+    x: np.ndarray
+    y: np.ndarray
+    status: int
+    iterations: int
+    objective: float
+    tau: float
+    sigma: float
+    info: NirInfo
+
+
+def default_params(**kw) -> NirParams:
+    lib = load_library()
+    p = NirParams()
+    lib.nir_params_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+class Engine:
+    """One device context (HIP stream) of the engine."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        if self.lib.nir_ctx_create(device, C.byref(h)) != 0:
+            raise EngineUnavailable("nir_ctx_create failed: " + self.lib.nir_last_error().decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nir_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def error(self) -> str:
+        return self.lib.nir_ctx_error(self.h).decode()
+
+    def session(self, lp: LP, params: Optional[NirParams] = None, x0=None, y0=None) -> "Session":
+        return Session(self, lp, params, x0, y0)
+
+    def solve(self, lp: LP, params: Optional[NirParams] = None, x0=None, y0=None) -> Result:
+        s = self.session(lp, params, x0, y0)
+        try:
+            s.solve()
+            return s.result()
+        finally:
+            s.close()
+
+
+class Session:
+    """An LP resident on the device (nir_session_*)."""
+
+    def __init__(self, eng: Engine, lp: LP, params: Optional[NirParams], x0=None, y0=None):
+        self.eng, self.lib, self.lp = eng, eng.lib, lp
+        self.params = params if params is not None else default_params()
+        self._keep = []
+        if x0 is not None:
+            x0 = np.ascontiguousarray(x0, dtype=np.float64)
+            self._keep.append(x0)
+            self.params.x0 = _ptr(x0, C.c_double)
+        if y0 is not None:
+            y0 = np.ascontiguousarray(y0, dtype=np.float64)
+            self._keep.append(y0)
+            self.params.y0 = _ptr(y0, C.c_double)
+        self._lps = lp.struct()
+        h = C.c_void_p()
+        if self.lib.nir_session_create(eng.h, C.byref(self._lps), C.byref(self.params), C.byref(h)) != 0:
+            raise EngineError("nir_session_create: " + eng.error())
+        self.h = h
+
+    def run(self, iters: int):
+        if self.lib.nir_session_run(self.h, int(iters)) != 0:
+            raise EngineError("nir_session_run: " + self.eng.error())
+
+    def solve(self):
+        if self.lib.nir_session_solve(self.h) != 0:
+            raise EngineError("nir_session_solve: " + self.eng.error())
+
+    def result(self) -> Result:
+        x = np.zeros(self.lp.n)
+        y = np.zeros(self.lp.m)
+        info = NirInfo()
+        if self.lib.nir_session_result(self.h, _ptr(x, C.c_double), _ptr(y, C.c_double), C.byref(info)) != 0:
+            raise EngineError("nir_session_result: " + self.eng.error())
+        return Result(x, y, info.status, info.iterations, info.objective, info.tau, info.sigma, info)
+
+    def time(self, iters: int) -> NirTiming:
+        t = NirTiming()
+        if self.lib.nir_session_time(self.h, int(iters), C.byref(t)) != 0:
+            raise EngineError("nir_session_time: " + self.eng.error())
+        return t
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nir_session_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,70 @@
+/* ORACLE - test infrastructure only.
+ *
+ * A plain-C (C99) restatement of the reference's CPU PDHG path, used as the
+ * checker for the GPU engine. Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it, and only to check or time against
+ * it; the product never calls into this directory.
+ *
+ * Compiled with gcc -O3 -ffp-contract=off (no fused multiply-add), matching
+ * the rounding of the reference's plain x86-64 build (Makefile:2).
+ *
+ * Function -> reference lines it restates -> what pins it
+ *   or_spmv          src/core/csc_matrix.hpp:99-108   KAT unit_tests.cpp:56-60,
+ *                                                     dense brute force, oracle/_ref
+ *   or_spmv_t        src/core/csc_matrix.hpp:111-120  dense brute force, oracle/_ref
+ *   or_scale_geometric src/model/scaling.hpp:14-154   pow2 / equilibrium invariants,
+ *                                                     oracle/_ref factors
+ *   or_pdhg_reference src/gpu/pdhg_solver.hpp:25-107  closed-form step sizes and first
+ *                                                     step of unit_tests.cpp:316-331,
+ *                                                     oracle/_ref iterates at K >= 5000
+ *   or_dd_verify     src/io/sol_io.hpp:25-106         KAT unit_tests.cpp:29-42,
+ *                                                     README.md:84 (2.55e-14)
+ *   or_pdlp          build-defined restarted PDHG     KKT = 0 at known optima,
+ *                                                     scaling invariance, HiGHS optimum
+ */
+#ifndef NIR_ORACLE_H
+#define NIR_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* LP in the reference's CSC layout: min c^T x + off, rl <= A x <= ru, l <= x <= u. */
+typedef struct {
+    int32_t m, n;
+    const int64_t* start;
+    const int32_t* index;
+    const double* value;
+    const double* c;
+    const double* l;
+    const double* u;
+    const double* rl;
+    const double* ru;
+    double offset;
+} or_lp;
+
+void or_spmv(const or_lp* lp, const double* x, double* y);   /* y = A x  (m) */
+void or_spmv_t(const or_lp* lp, const double* y, double* g); /* g = A^T y (n) */
+
+/* Geometric-mean scaling factors; returns 1 when scaling engages. */
+int or_scale_geometric(const or_lp* lp, double* R, double* C);
+
+/* Reference PDHG. Runs at most K iterations; with check != 0 it applies the
+ * reference stop rule. x (n) and y (m) receive the final iterates; the
+ * remaining outputs may be NULL. */
+void or_pdhg_reference(const or_lp* lp, int64_t K, int check, const double* x0, const double* y0, double* x,
+                       double* y, double* tau_out, int64_t* iters_out, double* viol_out);
+
+/* Double-double checker: returns 1 on pass. */
+int or_dd_verify(const or_lp* lp, const double* x, double claimed_obj, double tol, double* max_row_viol,
+                 double* max_bound_viol, double* computed_obj);
+
+/* Relative KKT of (x, y) on the LP (see or_pdlp). out[0..2] = relative primal,
+ * dual, gap; out[3], out[4] = primal and dual objective. */
+void or_kkt(const or_lp* lp, const double* x, const double* y, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

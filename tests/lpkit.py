@@ -1,0 +1,109 @@
+"""Small LPs used across the tests, with where each comes from."""
+import os
+
+import numpy as np
+
+from nirnaya_amd.engine import LP
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+INF = float("inf")
+
+
+def pdhg_unit_lp() -> LP:
+    """tests/unit_tests.cpp:316-331: min 2x1+3x2, x in [0,20]^2,
+    x1+x2 in [4,20], x1+3x2 in [6,30]. Optimum x=(3,1), obj 9, y=(1.5,0.5)."""
+    return LP.from_dense([[1.0, 1.0], [1.0, 3.0]], [2.0, 3.0], [0.0, 0.0], [20.0, 20.0], [4.0, 6.0], [20.0, 30.0])
+
+
+def transport_lp() -> LP:
+    """tools/bench_runner.cpp:23-39: 5x5 assignment LP, cost ((i+1)(j+2) mod 7)+1,
+    rows and columns summing to 1. Optimum 13 (brute force over 5! permutations)."""
+    N = 5
+    A = np.zeros((2 * N, N * N))
+    c = np.zeros(N * N)
+    for i in range(N):
+        for j in range(N):
+            c[i * N + j] = ((i + 1) * (j + 2)) % 7 + 1
+    for i in range(N):
+        for j in range(N):
+            A[2 * i, i * N + j] = 1.0      # row-sum rows come first per i
+            A[2 * i + 1, j * N + i] = 1.0  # then the column-sum row for i
+    ones = np.ones(2 * N)
+    return LP.from_dense(A, c, np.zeros(N * N), np.ones(N * N), ones, ones)
+
+
+def transport_optimum() -> float:
+    import itertools
+    N = 5
+    best = INF
+    for perm in itertools.permutations(range(N)):
+        best = min(best, sum(((i + 1) * (perm[i] + 2)) % 7 + 1 for i in range(N)))
+    return float(best)
+
+
+def read_mps_lp(path):
+    """Minimal free-MPS reader for the fixtures (N/L/G/E rows, COLUMNS, RHS,
+    UP/LO bounds, OBJSENSE MAX). Returns (LP with the file's own costs,
+    sense (+1 min / -1 max), column names)."""
+    rows, kinds, cols, entries, cost, rhs, lo, up = [], {}, [], {}, {}, {}, {}, {}
+    sense, obj, section = 1, None, None
+    with open(path) as f:
+        for line in f:
+            if not line.strip() or line.startswith("*"):
+                continue
+            tok = line.split()
+            if not line[0].isspace():
+                section = tok[0].upper()
+                continue
+            if section == "OBJSENSE":
+                sense = -1 if tok[0].upper().startswith("MAX") else 1
+            elif section == "ROWS":
+                if tok[0].upper() == "N":
+                    obj = tok[1]
+                else:
+                    rows.append(tok[1])
+                    kinds[tok[1]] = tok[0].upper()
+            elif section == "COLUMNS":
+                cn = tok[0]
+                if cn not in cost:
+                    cols.append(cn)
+                    cost[cn] = 0.0
+                for k in range(1, len(tok) - 1, 2):
+                    if tok[k] == obj:
+                        cost[cn] += float(tok[k + 1])
+                    else:
+                        entries[(tok[k], cn)] = float(tok[k + 1])
+            elif section == "RHS":
+                for k in range(len(tok) % 2, len(tok) - 1, 2):
+                    rhs[tok[k]] = float(tok[k + 1])
+            elif section == "BOUNDS":
+                b, cn, v = tok[0].upper(), tok[2], float(tok[3])
+                (up if b == "UP" else lo)[cn] = v
+    m, n = len(rows), len(cols)
+    A = np.zeros((m, n))
+    for (rn, cn), v in entries.items():
+        A[rows.index(rn), cols.index(cn)] = v
+    rl = np.array([rhs.get(r, 0.0) if kinds[r] in ("G", "E") else -INF for r in rows])
+    ru = np.array([rhs.get(r, 0.0) if kinds[r] in ("L", "E") else INF for r in rows])
+    c = np.array([cost[cn] for cn in cols])
+    lp = LP.from_dense(A, c, [lo.get(cn, 0.0) for cn in cols], [up.get(cn, INF) for cn in cols], rl, ru)
+    return lp, sense, cols
+
+
+def read_sol(path):
+    obj, vals = None, {}
+    with open(path) as f:
+        for line in f:
+            tok = line.split()
+            if not tok:
+                continue
+            if tok[0] == "#":
+                if tok[1] == "objective":
+                    obj = float(tok[2])
+                continue
+            vals[tok[0]] = float(tok[1])
+    return obj, vals
+
+
+CRUDE_MPS = os.path.join(GOLDEN, "MRPL_Crude_Blending_LP.mps")
+CRUDE_SOL = os.path.join(GOLDEN, "MRPL_Crude_Blending_LP.sol")

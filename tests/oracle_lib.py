@@ -1,0 +1,178 @@
+"""ctypes access to the oracle (test infrastructure only).
+
+  oracle/liboracle.so  - the C restatement of the reference PDHG path
+  oracle/_ref/libref.so - the reference's own sources compiled by
+                          oracle/Makefile (present only where /root/reference
+                          was available at build time)
+
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, and
+only as the checker or the CPU timing baseline.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+
+
+class OrLp(C.Structure):
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("start", _i64p), ("index", _i32p), ("value", _dp),
+                ("c", _dp), ("l", _dp), ("u", _dp), ("rl", _dp), ("ru", _dp), ("offset", C.c_double)]
+
+
+def _p(a, t=C.c_double):
+    if a is None:
+        return C.cast(None, C.POINTER(t))
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+_oracle = None
+_ref = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_SO):
+            raise FileNotFoundError(ORACLE_SO + " not built")
+        L = C.CDLL(ORACLE_SO)
+        L.or_spmv.argtypes = [C.POINTER(OrLp), _dp, _dp]
+        L.or_spmv_t.argtypes = [C.POINTER(OrLp), _dp, _dp]
+        L.or_scale_geometric.argtypes = [C.POINTER(OrLp), _dp, _dp]
+        L.or_pdhg_reference.argtypes = [C.POINTER(OrLp), C.c_int64, C.c_int, _dp, _dp, _dp, _dp, _dp, _i64p, _dp]
+        L.or_dd_verify.argtypes = [C.POINTER(OrLp), _dp, C.c_double, C.c_double, _dp, _dp, _dp]
+        L.or_kkt.argtypes = [C.POINTER(OrLp), _dp, _dp, _dp]
+        _oracle = L
+    return _oracle
+
+
+def have_ref() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not have_ref():
+            raise FileNotFoundError(REF_SO + " not built")
+        L = C.CDLL(REF_SO)
+        csc = [C.c_int32, C.c_int32, _i64p, _i32p, _dp]
+        lpargs = csc + [_dp] * 5
+        L.ref_pdhg.argtypes = lpargs + [C.c_int32, C.c_int32, _dp, _dp, _i64p, _dp]
+        L.ref_pdhg_timed.argtypes = lpargs + [C.c_double, C.c_int32, _i64p, _dp]
+        L.ref_scale.argtypes = lpargs + [_dp, _dp]
+        L.ref_spmv.argtypes = csc + [_dp, _dp]
+        L.ref_spmv_t.argtypes = csc + [_dp, _dp]
+        L.ref_verify.argtypes = lpargs + [_dp, C.c_double, C.c_double, _dp]
+        _ref = L
+    return _ref
+
+
+class Oracle:
+    """The C oracle bound to one LP (nirnaya_amd.engine.LP)."""
+
+    def __init__(self, lp):
+        self.lp = lp
+        self.s = OrLp(lp.m, lp.n, _p(lp.start, C.c_int64), _p(lp.index, C.c_int32), _p(lp.value), _p(lp.c),
+                      _p(lp.l), _p(lp.u), _p(lp.rl), _p(lp.ru), float(lp.offset))
+        self.L = oracle()
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.lp.m)
+        self.L.or_spmv(C.byref(self.s), _p(x), _p(y))
+        return y
+
+    def spmv_t(self, y):
+        y = np.ascontiguousarray(y, np.float64)
+        g = np.zeros(self.lp.n)
+        self.L.or_spmv_t(C.byref(self.s), _p(y), _p(g))
+        return g
+
+    def scale(self):
+        R = np.zeros(self.lp.m)
+        Cc = np.zeros(self.lp.n)
+        active = self.L.or_scale_geometric(C.byref(self.s), _p(R), _p(Cc))
+        return bool(active), R, Cc
+
+    def pdhg(self, K, check=True, x0=None, y0=None):
+        x = np.zeros(self.lp.n)
+        y = np.zeros(self.lp.m)
+        tau = C.c_double()
+        it = C.c_int64()
+        V = C.c_double()
+        x0 = None if x0 is None else np.ascontiguousarray(x0, np.float64)
+        y0 = None if y0 is None else np.ascontiguousarray(y0, np.float64)
+        self.L.or_pdhg_reference(C.byref(self.s), int(K), 1 if check else 0, _p(x0), _p(y0), _p(x), _p(y),
+                                 C.byref(tau), C.byref(it), C.byref(V))
+        return x, y, tau.value, it.value, V.value
+
+    def verify(self, x, claimed, tol=1e-6):
+        x = np.ascontiguousarray(x, np.float64)
+        rv, bv, obj = C.c_double(), C.c_double(), C.c_double()
+        ok = self.L.or_dd_verify(C.byref(self.s), _p(x), float(claimed), float(tol), C.byref(rv), C.byref(bv),
+                                 C.byref(obj))
+        return bool(ok), rv.value, bv.value, obj.value
+
+    def kkt(self, x, y):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        out = np.zeros(5)
+        self.L.or_kkt(C.byref(self.s), _p(x), _p(y), _p(out))
+        return out
+
+
+class Reference:
+    """The reference's own code (oracle/_ref/libref.so) bound to one LP."""
+
+    def __init__(self, lp):
+        self.lp = lp
+        self.L = ref()
+        self.csc = (lp.m, lp.n, _p(lp.start, C.c_int64), _p(lp.index, C.c_int32), _p(lp.value))
+        self.vecs = (_p(lp.c), _p(lp.l), _p(lp.u), _p(lp.rl), _p(lp.ru))
+
+    def pdhg(self, simplex_iter_limit=500000, threads=1):
+        x = np.zeros(self.lp.n)
+        y = np.zeros(self.lp.m)
+        it = C.c_int64()
+        obj = C.c_double()
+        st = self.L.ref_pdhg(*self.csc, *self.vecs, int(simplex_iter_limit), int(threads), _p(x), _p(y), C.byref(it),
+                             C.byref(obj))
+        return x, y, st, it.value, obj.value
+
+    def pdhg_timed(self, seconds, threads=1):
+        it = C.c_int64()
+        sec = C.c_double()
+        st = self.L.ref_pdhg_timed(*self.csc, *self.vecs, float(seconds), int(threads), C.byref(it), C.byref(sec))
+        return st, it.value, sec.value
+
+    def scale(self):
+        R = np.zeros(self.lp.m)
+        Cc = np.zeros(self.lp.n)
+        active = self.L.ref_scale(*self.csc, *self.vecs, _p(R), _p(Cc))
+        return bool(active), R, Cc
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.lp.m)
+        self.L.ref_spmv(*self.csc, _p(x), _p(y))
+        return y
+
+    def spmv_t(self, y):
+        y = np.ascontiguousarray(y, np.float64)
+        g = np.zeros(self.lp.n)
+        self.L.ref_spmv_t(*self.csc, _p(y), _p(g))
+        return g
+
+    def verify(self, x, claimed, tol=1e-6):
+        x = np.ascontiguousarray(x, np.float64)
+        out = np.zeros(3)
+        ok = self.L.ref_verify(*self.csc, *self.vecs, _p(x), float(claimed), float(tol), _p(out))
+        return bool(ok), out[0], out[1], out[2]

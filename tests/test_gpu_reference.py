@@ -1,0 +1,145 @@
+"""GPU parity of the engine's reference mode against the oracle.
+
+The engine's CSR-stream kernels sum every row in ascending column order from
+0.0 with products rounded before the add (no FMA contraction), exactly as the
+reference's sequential loops do, so iterates are expected to agree bit for bit;
+the acceptance bar is 1e-10 relative (rows longer than one LP tile use a tree
+reduction and are held to that bar only).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from nirnaya_amd.engine import (LP, MODE_REFERENCE, NirParams, _STOP_CB, default_params, gen_equality_lp,
+                                gen_mixed_lp, gen_staircase_lp)
+from oracle_lib import Oracle
+import lpkit
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(1.0, float(np.max(np.abs(b))))) if a.size else 0.0
+
+
+def run_ref(engine, lp, K, check=False, **kw):
+    p = default_params(mode=MODE_REFERENCE, max_iters=K, check_termination=1 if check else 0, **kw)
+    return engine.solve(lp, p)
+
+
+@pytest.mark.parametrize("K", [1, 10, 100, 1000])
+def test_iterates_match_oracle_cfg0(engine, K):
+    lp = gen_equality_lp(20260, 2000, 5000, 5)
+    r = run_ref(engine, lp, K)
+    xo, yo, tau, it, V = Oracle(lp).pdhg(K, check=False)
+    print(f"K={K} rel x {rel(r.x, xo):.3e} y {rel(r.y, yo):.3e} exact={np.array_equal(r.x, xo) and np.array_equal(r.y, yo)}")
+    assert r.iterations == it == K
+    assert r.tau == tau
+    assert rel(r.x, xo) <= 1e-10 and rel(r.y, yo) <= 1e-10
+    assert r.info.max_primal_viol == pytest.approx(V, rel=1e-10)
+
+
+@pytest.mark.parametrize("tile,rows", [(1024, 256), (2048, 64), (4096, 256)])
+@pytest.mark.parametrize("graph", [0, 1])
+def test_tiles_and_graphs_bit_identical(engine, tile, rows, graph):
+    lp = gen_mixed_lp(7, 3000, 9000, 6.0, 0.01, 1e3)
+    r = run_ref(engine, lp, 77, tile_nnz=tile, tile_rows=rows, use_graph=graph)
+    xo, yo, _, _, _ = Oracle(lp).pdhg(77, check=False)
+    assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+
+
+def test_stop_rule_matches_oracle(engine):
+    # Reference stop rule (pdhg_solver.hpp:97-100) decided on the device.
+    for lp in (lpkit.pdhg_unit_lp(), lpkit.transport_lp()):
+        r = run_ref(engine, lp, 50000, check=True)
+        xo, yo, _, it, _ = Oracle(lp).pdhg(50000, check=True)
+        assert r.iterations == it and it < 50000
+        assert rel(r.x, xo) <= 1e-10 and rel(r.y, yo) <= 1e-10
+    # unit_tests.cpp:316-331 bar on the engine itself
+    r = run_ref(engine, lpkit.pdhg_unit_lp(), 50000, check=True)
+    assert r.status == 0 and abs(r.objective - 9.0) <= 1e-2
+
+
+def test_edge_shapes(engine):
+    # Free rows, one-sided and ranged rows, free and one-sided columns, an
+    # empty row, an empty column and a dense row longer than the LP tile.
+    rng = np.random.default_rng(5)
+    m, n = 60, 5000
+    A = np.zeros((m, n))
+    for j in range(n):
+        rows = rng.choice(m - 2, size=3, replace=False) + 1
+        A[rows, j] = rng.standard_normal(3)
+    A[:, 7] = 0.0                       # empty column
+    A[m - 1, :] = rng.standard_normal(n)  # 5000-long row: long-row path
+    A[0, :] = 0.0                       # empty row
+    inf = np.inf
+    l = np.where(rng.random(n) < 0.2, -inf, 0.0)
+    u = np.where(rng.random(n) < 0.3, inf, rng.uniform(1, 5, n))
+    rl = rng.uniform(-1, 0, m)
+    ru = rl + rng.uniform(0, 2, m)
+    rl[rng.random(m) < 0.3] = -inf
+    ru[rng.random(m) < 0.3] = inf
+    rl[5], ru[5] = -inf, inf            # free row
+    lp = LP.from_dense(A, rng.standard_normal(n), l, u, rl, ru)
+    r = run_ref(engine, lp, 300)
+    xo, yo, _, _, _ = Oracle(lp).pdhg(300, check=False)
+    assert rel(r.x, xo) <= 1e-10 and rel(r.y, yo) <= 1e-10
+    # Without the long row every sum keeps the sequential order: bit exact.
+    A[m - 1, :] = 0.0
+    lp2 = LP.from_dense(A, lp.c, l, u, rl, ru)
+    r2 = run_ref(engine, lp2, 300)
+    xo2, yo2, _, _, _ = Oracle(lp2).pdhg(300, check=False)
+    assert np.array_equal(r2.x, xo2) and np.array_equal(r2.y, yo2)
+
+
+def test_warm_start_matches_oracle(engine):
+    lp = gen_staircase_lp(9, 6, 8)
+    x0 = lp.x0 * 0.9
+    y0 = 0.1 * lp.y0
+    p = default_params(mode=MODE_REFERENCE, max_iters=200, check_termination=0)
+    r = engine.solve(lp, p, x0=x0, y0=y0)
+    xo, yo, _, _, _ = Oracle(lp).pdhg(200, check=False, x0=x0, y0=y0)
+    assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+
+
+def test_should_stop_gives_time_limit_on_256_boundary(engine):
+    lp = gen_equality_lp(3, 500, 1200, 5)
+    calls = []
+
+    def stop(_user):
+        calls.append(1)
+        return 1 if len(calls) >= 3 else 0
+
+    cb = _STOP_CB(stop)
+    p = default_params(mode=MODE_REFERENCE, max_iters=100000, check_termination=0)
+    p.should_stop = cb
+    r = engine.solve(lp, p)
+    assert r.status == 4 and r.iterations == 512
+    # and the iterate is exactly the oracle's after 512 iterations
+    xo, yo, _, _, _ = Oracle(lp).pdhg(512, check=False)
+    assert np.array_equal(r.x, xo)
+
+
+def test_session_runs_in_pieces(engine):
+    lp = gen_equality_lp(8, 800, 2000, 5)
+    p = default_params(mode=MODE_REFERENCE, max_iters=1000, check_termination=0)
+    s = engine.session(lp, p)
+    try:
+        for k in (1, 31, 32, 300, 17):
+            s.run(k)
+        r = s.result()
+    finally:
+        s.close()
+    xo, yo, _, _, _ = Oracle(lp).pdhg(381, check=False)
+    assert r.iterations == 381 and np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+
+
+def test_full_size_cfg1_parity(engine):
+    """m=400,000, n=1,200,000, ~9.6M nonzeros: 12 iterations against the oracle."""
+    lp = gen_mixed_lp(20261, 400_000, 1_200_000, 6.0, 0.01, 1e3)
+    assert 9_000_000 < lp.nnz < 10_200_000
+    r = run_ref(engine, lp, 12)
+    xo, yo, tau, _, _ = Oracle(lp).pdhg(12, check=False)
+    assert r.tau == tau
+    assert rel(r.x, xo) <= 1e-10 and rel(r.y, yo) <= 1e-10
