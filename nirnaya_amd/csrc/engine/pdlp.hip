@@ -1,23 +1,1028 @@
+// Restarted PDHG ("pdlp" mode) on the device.
+//
+// Method (build-defined; restated bit for bit by the C oracle's or_pdlp):
+//   * Diagonal preconditioning in the exponent domain: 10 Ruiz passes then one
+//     Pock-Chambolle (alpha = 1) pass. Every factor is a power of two,
+//     2^-floor((e+1)/2) for a row/column measure v in [2^e, 2^(e+1)), so
+//     scaling and unscaling are exact and need no sqrt.
+//   * Constant step: eta = 0.998 / sigma_max(A'), sigma_max from a power
+//     iteration on A'^T A' (deterministic start vector, double-double norms).
+//     tau = eta / omega, sigma = eta * omega for primal weight omega.
+//   * Reflected Halpern iteration (Lu & Yang, "Restarted Halpern PDHG for
+//     linear programming", 2024; reflection as in cuPDLPx, 2025): with T the
+//     PDHG map of the reference loop (pdhg_solver.hpp:70-95) and k the
+//     iterations since the last restart,
+//         z_{k+1} = (k+1)/(k+2) (2 T(z_k) - z_k) + 1/(k+2) z_anchor.
+//     The reflected primal point 2 x+ - x is exactly the extrapolated xbar the
+//     dual step needs, so the anchor costs one extra read per element.
+//   * Every eval_freq iterations (default 64, as in cuPDLP) the kernels keep
+//     T(z) and evaluate in FP64 / double-double: primal residual of A x, dual
+//     residual of c - A^T y, primal and dual objectives (PDLP's relative KKT,
+//     in the caller's units), and the weighted KKT error
+//         KKT_w = sqrt(w ||r_p||^2 + ||r_d||^2 / w + gap^2).
+//     Restart (anchor and iterate := T(z)) when KKT_w <= 0.2 KKT_w(anchor), or
+//     <= 0.8 KKT_w(anchor) while rising, or when the restart length reaches
+//     0.36 of all iterations (cuPDLP's criteria); then
+//     omega <- sqrt(omega * ||dy|| / ||dx||) (PDLP smoothing with theta 1/2).
+//   * All decisions run in a one-workgroup kernel; the host replays a
+//     hipGraph per block of eval_freq iterations and polls a done flag.
+//   * Mixed precision: FP32 matrix values, iterates and accumulation in the
+//     iteration kernels; FP64 matrix and arithmetic for every evaluation,
+//     restart and step decision; then an FP64 tail from the FP32 point (restart
+//     at it) down to the polish tolerance. An FP32 phase that stops improving
+//     hands over to FP64 early.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
 #include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
 
 #include "pdlp.hpp"
 
 namespace nir {
 namespace gpu {
 
+#define PD_CHECK(expr)                                                                                \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ------------------------------------------------------------------ device dd
+
+struct dd {
+    double hi, lo;
+};
+
+__host__ __device__ __forceinline__ dd dd_zero() { return dd{0.0, 0.0}; }
+
+__host__ __device__ __forceinline__ dd dd_add(dd a, dd b) {
+    double s1 = a.hi + b.hi, v = s1 - a.hi;
+    double s2 = (a.hi - (s1 - v)) + (b.hi - v);
+    double t1 = a.lo + b.lo, w = t1 - a.lo;
+    double t2 = (a.lo - (t1 - w)) + (b.lo - w);
+    s2 += t1;
+    double u = s1 + s2;
+    s2 = s2 - (u - s1);
+    s1 = u;
+    s2 += t2;
+    dd r;
+    r.hi = s1 + s2;
+    r.lo = s2 - (r.hi - s1);
+    return r;
+}
+
+__host__ __device__ __forceinline__ dd dd_prod(double a, double b) {
+    const double p = a * b;
+    return dd{p, fma(a, b, -p)};
+}
+
+__host__ __device__ __forceinline__ double dd_val(dd a) { return a.hi + a.lo; }
+
+// Reduce NQ dd accumulators over the workgroup (fixed tree), thread 0 writes.
+template <int NQ>
+__device__ __forceinline__ void block_dd_reduce(dd (&acc)[NQ], dd* sh, dd* out) {
+    const int t = threadIdx.x;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) sh[q * kThreads + t] = acc[q];
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (t < w)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) sh[q * kThreads + t] = dd_add(sh[q * kThreads + t], sh[q * kThreads + t + w]);
+        __syncthreads();
+    }
+    if (t == 0)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) out[q] = sh[q * kThreads];
+}
+
+// 2^-floor((e+1)/2) for v in [2^e, 2^(e+1)); 1 for v == 0 or non-finite.
+__host__ __device__ __forceinline__ double pow2_inv_sqrt(double v) {
+    if (!(v > 0.0) || !isfinite(v)) return 1.0;
+    int e2;
+    (void)frexp(v, &e2);   // v = f 2^e2, f in [0.5, 1)
+    const int e = e2 - 1;
+    const int k = (e + 1) >= 0 ? (e + 1) / 2 : -((-(e + 1) + 1) / 2);   // floor((e+1)/2)
+    return ldexp(1.0, -k);
+}
+
+// ------------------------------------------------------------------ state
+
+struct PdlpState {
+    double omega, tau, sigma, eta;
+    double kkt_anchor, kkt_prev;
+    double rel_p, rel_d, rel_g, pobj, dobj;
+    double best_rel;
+    double tol;
+    double bnorm_o, cnorm_o, offset;
+    long long total;       // iterations completed
+    long long max_iters;
+    int k_base;            // iterations since restart at the start of the block
+    int done;              // 1 = stop iterating
+    int status;            // 0 optimal, 6 iteration limit, 11 numerical trouble, 2 stalled (mixed phase)
+    int restart;           // set by the decision kernel for the copy kernel
+    int restarts;
+    int evals;
+    int since_best;        // evaluations since best_rel halved
+    int watch_stall;       // 1 in the FP32 phase
+    int eval_freq;
+};
+
+// ------------------------------------------------------------------ kernels
+
+template <int TILE, typename T>
+__global__ __launch_bounds__(kThreads) void k_primal_pdlp(Csr<T> AT, const T* __restrict__ y, T* __restrict__ x,
+                                                          T* __restrict__ xbar, const T* __restrict__ xa,
+                                                          T* __restrict__ xp_out, const T* __restrict__ c,
+                                                          const T* __restrict__ lo, const T* __restrict__ up,
+                                                          const PdlpState* __restrict__ st, int k_lo, int save) {
+    __shared__ T prod[TILE];
+    if (st->done) return;
+    const T tau = static_cast<T>(st->tau);
+    const int k = st->k_base + k_lo;
+    const T w = static_cast<T>(static_cast<double>(k + 1) / static_cast<double>(k + 2));
+    const T wc = static_cast<T>(1.0 / static_cast<double>(k + 2));
+    stream_rows<TILE>(AT, y, prod, blockIdx.x, [&](int j, T g) {
+        const T xj = x[j];
+        T cand = xj - tau * (c[j] - g);
+        cand = (lo[j] < cand) ? cand : lo[j];
+        cand = (cand < up[j]) ? cand : up[j];
+        const T xb = T(2) * cand - xj;
+        xbar[j] = xb;
+        x[j] = w * xb + wc * xa[j];
+        if (save) xp_out[j] = cand;
+    });
+}
+
+template <int TILE, typename T>
+__global__ __launch_bounds__(kThreads) void k_dual_pdlp(Csr<T> A, const T* __restrict__ xbar, T* __restrict__ y,
+                                                        const T* __restrict__ ya, T* __restrict__ yp_out,
+                                                        const T* __restrict__ rl, const T* __restrict__ ru,
+                                                        const PdlpState* __restrict__ st, int k_lo, int save) {
+    __shared__ T prod[TILE];
+    if (st->done) return;
+    const T sigma = static_cast<T>(st->sigma);
+    const int k = st->k_base + k_lo;
+    const T w = static_cast<T>(static_cast<double>(k + 1) / static_cast<double>(k + 2));
+    const T wc = static_cast<T>(1.0 / static_cast<double>(k + 2));
+    stream_rows<TILE>(A, xbar, prod, blockIdx.x, [&](int i, T h) {
+        const T yi = y[i];
+        T r = h - yi / sigma;
+        r = (rl[i] < r) ? r : rl[i];
+        r = (r < ru[i]) ? r : ru[i];
+        const T yp = yi + sigma * (r - h);
+        const T yb = T(2) * yp - yi;
+        y[i] = w * yb + wc * ya[i];
+        if (save) yp_out[i] = yp;
+    });
+}
+
+// Column evaluation: lambda = c - A^T yp, its residual against the sign the
+// column bounds allow, the column part of the dual objective, c^T xp and
+// ||xp - x_anchor||^2. Quantities: 0 rd^2, 1 (rd/C)^2, 2 c^T x, 3 dual-obj
+// column terms, 4 ||dx||^2.
+template <int TILE, typename TI>
+__global__ __launch_bounds__(kThreads) void k_eval_cols(Csr<double> AT, const TI* __restrict__ yp,
+                                                        const TI* __restrict__ xp, const TI* __restrict__ xa,
+                                                        const double* __restrict__ c, const double* __restrict__ lo,
+                                                        const double* __restrict__ up,
+                                                        const double* __restrict__ cscale,
+                                                        const PdlpState* __restrict__ st, dd* __restrict__ part,
+                                                        int force) {
+    __shared__ double prod[TILE];
+    __shared__ dd sh[5 * kThreads];
+    if (st->done && !force) return;
+    dd acc[5] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
+    stream_rows<TILE>(AT, yp, prod, blockIdx.x, [&](int j, double g) {
+        const double lam = c[j] - g;
+        const bool lf = lo[j] > -INFINITY, uf = up[j] < INFINITY;
+        double lp = lam;
+        if (!lf) lp = amin(lp, 0.0);
+        if (!uf) lp = amax(lp, 0.0);
+        const double rd = lam - lp;
+        acc[0] = dd_prod(rd, rd);
+        const double rdo = rd / cscale[j];
+        acc[1] = dd_prod(rdo, rdo);
+        const double xj = static_cast<double>(xp[j]);
+        acc[2] = dd_prod(c[j], xj);
+        dd dobj = dd_zero();
+        if (lf && lp > 0.0) dobj = dd_prod(lo[j], lp);
+        if (uf && lp < 0.0) dobj = dd_prod(up[j], lp);
+        acc[3] = dobj;
+        const double dx = xj - static_cast<double>(xa[j]);
+        acc[4] = dd_prod(dx, dx);
+    });
+    block_dd_reduce<5>(acc, sh, part + 5 * blockIdx.x);
+}
+
+// Row evaluation: r_p = A xp - clamp(A xp), sign violations of yp, the row
+// part of the dual objective and ||yp - y_anchor||^2. Quantities: 0 rp^2,
+// 1 (rp/R)^2, 2 yviol^2, 3 (R yviol)^2, 4 dual-obj row terms, 5 ||dy||^2.
+template <int TILE, typename TI>
+__global__ __launch_bounds__(kThreads) void k_eval_rows(Csr<double> A, const TI* __restrict__ xp,
+                                                        const TI* __restrict__ yp, const TI* __restrict__ ya,
+                                                        const double* __restrict__ rl, const double* __restrict__ ru,
+                                                        const double* __restrict__ rscale,
+                                                        const PdlpState* __restrict__ st, dd* __restrict__ part,
+                                                        int force) {
+    __shared__ double prod[TILE];
+    __shared__ dd sh[6 * kThreads];
+    if (st->done && !force) return;
+    dd acc[6] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
+    stream_rows<TILE>(A, xp, prod, blockIdx.x, [&](int i, double h) {
+        const bool lf = rl[i] > -INFINITY, uf = ru[i] < INFINITY;
+        double proj = amax(rl[i], h);
+        proj = amin(ru[i], proj);
+        const double rp = h - proj;
+        acc[0] = dd_prod(rp, rp);
+        const double rpo = rp / rscale[i];
+        acc[1] = dd_prod(rpo, rpo);
+        const double yi = static_cast<double>(yp[i]);
+        double ys = yi;
+        if (!lf) ys = amin(ys, 0.0);
+        if (!uf) ys = amax(ys, 0.0);
+        const double yv = yi - ys;
+        acc[2] = dd_prod(yv, yv);
+        const double yvo = yv * rscale[i];
+        acc[3] = dd_prod(yvo, yvo);
+        dd dobj = dd_zero();
+        if (lf && ys > 0.0) dobj = dd_prod(rl[i], ys);
+        if (uf && ys < 0.0) dobj = dd_prod(ru[i], ys);
+        acc[4] = dobj;
+        const double dy = yi - static_cast<double>(ya[i]);
+        acc[5] = dd_prod(dy, dy);
+    });
+    block_dd_reduce<6>(acc, sh, part + 6 * blockIdx.x);
+}
+
+// Decision kernel (one workgroup): reduce the partials, evaluate KKT,
+// terminate / restart / update the primal weight. mode 0: regular
+// evaluation after a block of eval_freq iterations; mode 1: initial
+// evaluation of the start point (sets the anchor KKT, may terminate).
+__global__ __launch_bounds__(kThreads) void k_decide(PdlpState* st, const dd* __restrict__ pc, int nbc,
+                                                     const dd* __restrict__ pr, int nbr, int mode) {
+    __shared__ dd sh[11 * kThreads];
+    if (st->done && mode == 0) return;
+    dd acc[11];
+    for (int q = 0; q < 11; ++q) acc[q] = dd_zero();
+    for (int b = threadIdx.x; b < nbc; b += kThreads)
+        for (int q = 0; q < 5; ++q) acc[q] = dd_add(acc[q], pc[5 * b + q]);
+    for (int b = threadIdx.x; b < nbr; b += kThreads)
+        for (int q = 0; q < 6; ++q) acc[5 + q] = dd_add(acc[5 + q], pr[6 * b + q]);
+    __shared__ dd tot[11];
+    block_dd_reduce<11>(acc, sh, tot);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+
+    PdlpState& s = *st;
+    const double rd2 = dd_val(dd_add(tot[0], tot[7]));
+    const double rd2o = dd_val(dd_add(tot[1], tot[8]));
+    const double rp2 = dd_val(tot[5]);
+    const double rp2o = dd_val(tot[6]);
+    const double P = dd_val(dd_add(tot[2], dd{s.offset, 0.0}));
+    const double D = dd_val(dd_add(dd_add(tot[3], tot[9]), dd{s.offset, 0.0}));
+    const double dx2 = dd_val(tot[4]);
+    const double dy2 = dd_val(tot[10]);
+    const double gap = fabs(P - D);
+    s.rel_p = sqrt(rp2o) / (1.0 + s.bnorm_o);
+    s.rel_d = sqrt(rd2o) / (1.0 + s.cnorm_o);
+    s.rel_g = gap / (1.0 + fabs(P) + fabs(D));
+    s.pobj = P;
+    s.dobj = D;
+    s.evals += 1;
+    const double worst = amax(s.rel_p, amax(s.rel_d, s.rel_g));
+    if (!isfinite(rd2) || !isfinite(rp2) || !isfinite(P) || !isfinite(D) || !isfinite(dx2) || !isfinite(dy2)) {
+        s.done = 1;
+        s.status = 11;
+        s.restart = 0;
+        return;
+    }
+    const double kkt = sqrt(s.omega * rp2 + rd2 / s.omega + gap * gap);
+    if (mode == 1) {
+        s.kkt_anchor = kkt;
+        s.kkt_prev = kkt;
+        s.best_rel = worst;
+        s.since_best = 0;
+        s.restart = 0;
+        if (worst <= s.tol) {
+            s.done = 1;
+            s.status = 0;
+        }
+        return;
+    }
+    s.total += s.eval_freq;
+    if (worst <= s.tol) {
+        s.done = 1;
+        s.status = 0;
+        s.restart = 0;
+        return;
+    }
+    if (s.watch_stall) {
+        if (worst <= 0.5 * s.best_rel) {
+            s.best_rel = worst;
+            s.since_best = 0;
+        } else if (++s.since_best >= 40) {
+            s.done = 1;
+            s.status = 2;
+            s.restart = 0;
+            return;
+        }
+    }
+    if (s.total >= s.max_iters) {
+        s.done = 1;
+        s.status = 6;
+        s.restart = 0;
+        return;
+    }
+    const int k_new = s.k_base + s.eval_freq;
+    const bool restart = (kkt <= 0.2 * s.kkt_anchor) || (kkt <= 0.8 * s.kkt_anchor && kkt > s.kkt_prev) ||
+                         (static_cast<double>(k_new) >= 0.36 * static_cast<double>(s.total));
+    s.kkt_prev = kkt;
+    if (restart) {
+        if (dx2 > 1e-20 && dy2 > 1e-20) {
+            const double ratio = sqrt(dy2) / sqrt(dx2);
+            s.omega = sqrt(s.omega * ratio);
+            s.tau = s.eta / s.omega;
+            s.sigma = s.eta * s.omega;
+        }
+        s.kkt_anchor = sqrt(s.omega * rp2 + rd2 / s.omega + gap * gap);
+        s.kkt_prev = s.kkt_anchor;
+        s.k_base = 0;
+        s.restarts += 1;
+        s.restart = 1;
+    } else {
+        s.k_base = k_new;
+        s.restart = 0;
+    }
+}
+
+// Restart: iterate and anchor become T(z).
+template <typename T>
+__global__ void k_restart_copy(const PdlpState* __restrict__ st, int n, int m, const T* __restrict__ xp,
+                               const T* __restrict__ yp, T* __restrict__ x, T* __restrict__ xa, T* __restrict__ y,
+                               T* __restrict__ ya, int force) {
+    if (!force && (st->done || !st->restart)) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = gridDim.x * blockDim.x;
+    for (int j = t; j < n; j += stride) {
+        const T v = xp[j];
+        x[j] = v;
+        xa[j] = v;
+    }
+    for (int i = t; i < m; i += stride) {
+        const T v = yp[i];
+        y[i] = v;
+        ya[i] = v;
+    }
+}
+
+// Plain y = M v (power iteration).
+template <int TILE>
+__global__ __launch_bounds__(kThreads) void k_spmv_plain(Csr<double> M, const double* __restrict__ v,
+                                                         double* __restrict__ out) {
+    __shared__ double prod[TILE];
+    stream_rows<TILE>(M, v, prod, blockIdx.x, [&](int r, double s) { out[r] = s; });
+}
+
+// Row measures over a CSR matrix: max |a| (what = 0) or sum |a| (what = 1).
+__global__ void k_row_measure(Csr<double> M, double* out, int what) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= M.nrows) return;
+    double acc = 0.0;
+    for (int p = M.rowptr[r]; p < M.rowptr[r + 1]; ++p) {
+        const double a = fabs(M.val[p]);
+        acc = what == 0 ? amax(acc, a) : acc + a;
+    }
+    out[r] = acc;
+}
+
+__global__ void k_factor(const double* meas, double* f, double* total, int n) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const double g = pow2_inv_sqrt(meas[r]);
+    f[r] = g;
+    total[r] *= g;
+}
+
+// val[p] *= rf[row] * cf[col[p]] for a CSR matrix (all factors powers of two).
+__global__ void k_scale_csr(Csr<double> M, double* val, const double* rf, const double* cf) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= M.nrows) return;
+    const double a = rf[r];
+    for (int p = M.rowptr[r]; p < M.rowptr[r + 1]; ++p) val[p] = val[p] * (a * cf[M.col[p]]);
+}
+
+__global__ void k_to_float(const double* in, float* out, long long n) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    for (long long i = t; i < n; i += stride) out[i] = static_cast<float>(in[i]);
+}
+
+__global__ void k_to_double(const float* in, double* out, long long n) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    for (long long i = t; i < n; i += stride) out[i] = static_cast<double>(in[i]);
+}
+
+// Sum of squares (dd) per block for the power iteration.
+__global__ __launch_bounds__(kThreads) void k_sumsq(const double* v, int n, dd* part) {
+    __shared__ dd sh[kThreads];
+    dd acc[1] = {dd_zero()};
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
+        acc[0] = dd_add(acc[0], dd_prod(v[i], v[i]));
+    block_dd_reduce<1>(acc, sh, part + blockIdx.x);
+}
+
+__global__ void k_scale_vec(double* v, int n, double s) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = v[i] / s;
+}
+
+// ------------------------------------------------------------------ host side
+
+template <typename T>
+struct Buf {
+    T* p = nullptr;
+    size_t n = 0;
+    Buf() = default;
+    Buf(const Buf&) = delete;
+    Buf& operator=(const Buf&) = delete;
+    ~Buf() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(size_t count) {
+        if (p) (void)hipFree(p);
+        n = count;
+        PD_CHECK(hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(count, 1) * sizeof(T)));
+    }
+    void up(const T* src, size_t count, hipStream_t s) {
+        if (count) PD_CHECK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void down(T* dst, size_t count, hipStream_t s) const {
+        if (count) PD_CHECK(hipMemcpyAsync(dst, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+};
+
+template <typename T>
+struct Iterates {
+    Buf<T> x, xbar, xa, xp, y, ya, yp;
+    Buf<T> c, lo, up, rl, ru;
+    void alloc(int n, int m) {
+        x.alloc(n); xbar.alloc(n); xa.alloc(n); xp.alloc(n);
+        y.alloc(m); ya.alloc(m); yp.alloc(m);
+        c.alloc(n); lo.alloc(n); up.alloc(n); rl.alloc(m); ru.alloc(m);
+    }
+};
+
 struct Pdlp {
-    int unused = 0;
+    int n = 0, m = 0, tile = 2048;
+    long long nnz = 0;
+    Csr<double> A{}, AT{};     // FP64, scaled in place
+    Buf<float> a32, at32;      // FP32 copies of the scaled values
+    Buf<double> R, C;          // engine scaling (powers of two)
+    Buf<double> Rtot, Ctot;    // engine x caller scaling, for KKT in the caller's units
+    Iterates<double> d;
+    Iterates<float> f;
+    Buf<dd> part_c, part_r;
+    Buf<PdlpState> st;
+    PdlpState* h_st = nullptr;   // pinned, two poll slots
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int pending[2] = {0, 0};
+    long long serial = 0;
+    bool mixed = false;          // current phase runs FP32 kernels
+    bool polish_done = false;
+    double polish_tol = 0.0, tol = 1e-6;
+    int eval_freq = 64;
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool graph_mixed = false;
+    bool finished = false;
+    int status = 0;
+    long long fp64_iters_at_switch = -1;
+    double setup_s = 0.0, solve_s = 0.0;
+    std::vector<double> h_c;
+    double offset = 0.0;
+    int (*should_stop)(void*) = nullptr;
+    void* stop_user = nullptr;
+    long long polls = 0;
+
+    ~Pdlp() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (h_st) (void)hipHostFree(h_st);
+    }
 };
 
 Pdlp* pdlp_create() { return new Pdlp; }
 void PdlpDeleter::operator()(Pdlp* p) const { delete p; }
 
-void pdlp_setup(Pdlp&, hipStream_t, const nir_lp*, const nir_params&, Csr<double>, Csr<double>, int, int) {
-    throw std::runtime_error("pdlp mode is not available in this build");
+namespace {
+
+int grid_for(long long n) { return static_cast<int>(std::max<long long>(1, std::min<long long>((n + 255) / 256, 8192))); }
+
+template <typename T>
+Csr<T> with_vals(Csr<double> M, const T* v) {
+    return Csr<T>{M.nrows, M.ncols, M.rowptr, M.col, v, M.nblk, M.blk};
 }
-void pdlp_run(Pdlp&, hipStream_t, const nir_params&, long long) {}
-int pdlp_result(Pdlp&, hipStream_t, double*, double*, nir_info*) { return -1; }
-int pdlp_time(Pdlp&, hipStream_t, long long, nir_timing*) { return -1; }
+
+template <typename T>
+void launch_iter(Pdlp& P, hipStream_t s, Iterates<T>& it, Csr<T> A, Csr<T> AT, int k_lo, int save) {
+#define PDLP_ITER(TL)                                                                                              \
+    k_primal_pdlp<TL, T><<<AT.nblk, kThreads, 0, s>>>(AT, it.y.p, it.x.p, it.xbar.p, it.xa.p, it.xp.p, it.c.p,      \
+                                                      it.lo.p, it.up.p, P.st.p, k_lo, save);                       \
+    k_dual_pdlp<TL, T><<<A.nblk, kThreads, 0, s>>>(A, it.xbar.p, it.y.p, it.ya.p, it.yp.p, it.rl.p, it.ru.p, P.st.p, \
+                                                   k_lo, save);
+    switch (P.tile) {
+        case 1024: PDLP_ITER(1024); break;
+        case 4096: PDLP_ITER(4096); break;
+        default: PDLP_ITER(2048); break;
+    }
+#undef PDLP_ITER
+    PD_CHECK(hipGetLastError());
+}
+
+template <typename TI>
+void launch_eval(Pdlp& P, hipStream_t s, Iterates<TI>& it, int mode) {
+    const int force = mode == 1 ? 1 : 0;
+#define PDLP_EVAL(TL)                                                                                              \
+    k_eval_cols<TL, TI><<<P.AT.nblk, kThreads, 0, s>>>(P.AT, it.yp.p, it.xp.p, it.xa.p, P.d.c.p, P.d.lo.p, P.d.up.p, \
+                                                       P.Ctot.p, P.st.p, P.part_c.p, force);                        \
+    k_eval_rows<TL, TI><<<P.A.nblk, kThreads, 0, s>>>(P.A, it.xp.p, it.yp.p, it.ya.p, P.d.rl.p, P.d.ru.p, P.Rtot.p,  \
+                                                      P.st.p, P.part_r.p, force);
+    switch (P.tile) {
+        case 1024: PDLP_EVAL(1024); break;
+        case 4096: PDLP_EVAL(4096); break;
+        default: PDLP_EVAL(2048); break;
+    }
+#undef PDLP_EVAL
+    k_decide<<<1, kThreads, 0, s>>>(P.st.p, P.part_c.p, P.AT.nblk, P.part_r.p, P.A.nblk, mode);
+    if (mode == 0) {
+        k_restart_copy<TI><<<grid_for(std::max(P.n, P.m)), 256, 0, s>>>(P.st.p, P.n, P.m, it.xp.p, it.yp.p, it.x.p,
+                                                                          it.xa.p, it.y.p, it.ya.p, 0);
+    }
+    PD_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void enqueue_block(Pdlp& P, hipStream_t s, Iterates<T>& it, Csr<T> A, Csr<T> AT) {
+    for (int kl = 0; kl < P.eval_freq; ++kl) launch_iter<T>(P, s, it, A, AT, kl, kl == P.eval_freq - 1 ? 1 : 0);
+    launch_eval<T>(P, s, it, 0);
+}
+
+void enqueue(Pdlp& P, hipStream_t s) {
+    if (P.use_graph) {
+        if (!P.gexec || P.graph_mixed != P.mixed) {
+            if (P.gexec) (void)hipGraphExecDestroy(P.gexec);
+            if (P.graph) (void)hipGraphDestroy(P.graph);
+            P.gexec = nullptr;
+            P.graph = nullptr;
+            PD_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            if (P.mixed) enqueue_block<float>(P, s, P.f, with_vals(P.A, P.a32.p), with_vals(P.AT, P.at32.p));
+            else enqueue_block<double>(P, s, P.d, P.A, P.AT);
+            PD_CHECK(hipStreamEndCapture(s, &P.graph));
+            PD_CHECK(hipGraphInstantiate(&P.gexec, P.graph, nullptr, nullptr, 0));
+            P.graph_mixed = P.mixed;
+        }
+        PD_CHECK(hipGraphLaunch(P.gexec, s));
+    } else {
+        if (P.mixed) enqueue_block<float>(P, s, P.f, with_vals(P.A, P.a32.p), with_vals(P.AT, P.at32.p));
+        else enqueue_block<double>(P, s, P.d, P.A, P.AT);
+    }
+}
+
+void post_poll(Pdlp& P, hipStream_t s, int slot) {
+    PD_CHECK(hipMemcpyAsync(&P.h_st[slot], P.st.p, sizeof(PdlpState), hipMemcpyDeviceToHost, s));
+    PD_CHECK(hipEventRecord(P.ev[slot], s));
+    P.pending[slot] = 1;
+}
+
+bool wait_poll(Pdlp& P, int slot) {
+    if (!P.pending[slot]) return false;
+    PD_CHECK(hipEventSynchronize(P.ev[slot]));
+    P.pending[slot] = 0;
+    return P.h_st[slot].done != 0;
+}
+
+PdlpState read_state(Pdlp& P, hipStream_t s) {
+    PdlpState h{};
+    PD_CHECK(hipStreamSynchronize(s));
+    P.pending[0] = P.pending[1] = 0;
+    PD_CHECK(hipMemcpy(&h, P.st.p, sizeof(PdlpState), hipMemcpyDeviceToHost));
+    return h;
+}
+
+void write_state(Pdlp& P, hipStream_t s, const PdlpState& h) {
+    PD_CHECK(hipMemcpyAsync(P.st.p, &h, sizeof(PdlpState), hipMemcpyHostToDevice, s));
+    PD_CHECK(hipStreamSynchronize(s));
+}
+
+// FP32 -> FP64 hand-over: the FP64 phase restarts at T(z) of the FP32 phase.
+void switch_to_fp64(Pdlp& P, hipStream_t s, double new_tol) {
+    const long long n = P.n, m = P.m;
+    k_to_double<<<grid_for(n), 256, 0, s>>>(P.f.xp.p, P.d.xp.p, n);
+    k_to_double<<<grid_for(m), 256, 0, s>>>(P.f.yp.p, P.d.yp.p, m);
+    k_restart_copy<double><<<grid_for(std::max(n, m)), 256, 0, s>>>(P.st.p, P.n, P.m, P.d.xp.p, P.d.yp.p, P.d.x.p,
+                                                                    P.d.xa.p, P.d.y.p, P.d.ya.p, 1);
+    PdlpState h = read_state(P, s);
+    h.done = 0;
+    h.status = 0;
+    h.k_base = 0;
+    h.tol = new_tol;
+    h.watch_stall = 0;
+    write_state(P, s, h);
+    P.mixed = false;
+    P.fp64_iters_at_switch = h.total;
+    // Anchor KKT of the hand-over point, in FP64.
+    launch_eval<double>(P, s, P.d, 1);
+    PD_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm, Csr<double> A, Csr<double> AT,
+                int tile, int tile_rows) {
+    auto t0 = std::chrono::steady_clock::now();
+    P.n = lp->ncol;
+    P.m = lp->nrow;
+    P.nnz = P.n > 0 ? lp->col_start[P.n] : 0;
+    P.A = A;
+    P.AT = AT;
+    P.tile = tile;
+    P.eval_freq = std::max(1, prm.eval_freq);
+    P.use_graph = prm.use_graph != 0;
+    P.tol = prm.tol > 0 ? prm.tol : 1e-6;
+    P.mixed = prm.precision == NIR_PREC_MIXED;
+    P.polish_tol = prm.polish_tol;
+    P.offset = lp->obj_offset;
+    P.should_stop = prm.should_stop;
+    P.stop_user = prm.stop_user;
+    P.h_c.assign(lp->cost, lp->cost + P.n);
+    const int n = P.n, m = P.m;
+
+    // ---- exponent-domain Ruiz (10 passes) + Pock-Chambolle, on the device
+    P.R.alloc(m);
+    P.C.alloc(n);
+    {
+        std::vector<double> ones_m(static_cast<size_t>(m), 1.0), ones_n(static_cast<size_t>(n), 1.0);
+        P.R.up(ones_m.data(), m, s);
+        P.C.up(ones_n.data(), n, s);
+    }
+    Buf<double> mr, mc, fr, fc, onesr, onesc;
+    mr.alloc(m); mc.alloc(n); fr.alloc(m); fc.alloc(n); onesr.alloc(m); onesc.alloc(n);
+    {
+        std::vector<double> ones_m(static_cast<size_t>(m), 1.0), ones_n(static_cast<size_t>(n), 1.0);
+        onesr.up(ones_m.data(), m, s);
+        onesc.up(ones_n.data(), n, s);
+    }
+    double* a_val = const_cast<double*>(A.val);
+    double* at_val = const_cast<double*>(AT.val);
+    const int gm = (m + 255) / 256, gn = (n + 255) / 256;
+    for (int pass = 0; pass < 11; ++pass) {
+        const int what = pass < 10 ? 0 : 1;   // 10 Ruiz (max |a|) passes, then one Pock-Chambolle (sum |a|)
+        if (m > 0) k_row_measure<<<gm, 256, 0, s>>>(A, mr.p, what);
+        if (m > 0) k_factor<<<gm, 256, 0, s>>>(mr.p, fr.p, P.R.p, m);
+        if (m > 0) k_scale_csr<<<gm, 256, 0, s>>>(A, a_val, fr.p, onesc.p);
+        if (n > 0) k_scale_csr<<<gn, 256, 0, s>>>(AT, at_val, onesc.p, fr.p);
+        if (n > 0) k_row_measure<<<gn, 256, 0, s>>>(AT, mc.p, what);
+        if (n > 0) k_factor<<<gn, 256, 0, s>>>(mc.p, fc.p, P.C.p, n);
+        if (n > 0) k_scale_csr<<<gn, 256, 0, s>>>(AT, at_val, fc.p, onesr.p);
+        if (m > 0) k_scale_csr<<<gm, 256, 0, s>>>(A, a_val, onesr.p, fc.p);
+        PD_CHECK(hipGetLastError());
+    }
+    std::vector<double> R(static_cast<size_t>(m)), Cc(static_cast<size_t>(n));
+    P.R.down(R.data(), m, s);
+    P.C.down(Cc.data(), n, s);
+    PD_CHECK(hipStreamSynchronize(s));
+
+    // ---- scaled vectors (absent bounds as +-inf) and caller-unit scales
+    auto finite_b = [](double b) { return std::isfinite(b) && std::fabs(b) < 1e20; };
+    std::vector<double> c(n), lo(n), up(n), rl(m), ru(m), rt(m), ct(n);
+    DDHost bn, cn;
+    for (int j = 0; j < n; ++j) {
+        c[j] = lp->cost[j] * Cc[j];
+        lo[j] = finite_b(lp->collo[j]) ? lp->collo[j] / Cc[j] : -INFINITY;
+        up[j] = finite_b(lp->colup[j]) ? lp->colup[j] / Cc[j] : INFINITY;
+        const double us = prm.unscale_col ? prm.unscale_col[j] : 1.0;
+        ct[j] = Cc[j] / us;   // lambda_caller = lambda' / C  ->  lambda_user = lambda_caller / unscale
+        const double cu = lp->cost[j] / us;
+        cn.add_prod(cu, cu);
+    }
+    for (int i = 0; i < m; ++i) {
+        rl[i] = finite_b(lp->rowlo[i]) ? lp->rowlo[i] * R[i] : -INFINITY;
+        ru[i] = finite_b(lp->rowup[i]) ? lp->rowup[i] * R[i] : INFINITY;
+        const double us = prm.unscale_row ? prm.unscale_row[i] : 1.0;
+        rt[i] = R[i] * us;
+        double b = 0.0;
+        if (finite_b(lp->rowlo[i])) b = std::fabs(lp->rowlo[i]);
+        if (finite_b(lp->rowup[i])) b = std::max(b, std::fabs(lp->rowup[i]));
+        b = b / us;
+        bn.add_prod(b, b);
+    }
+    P.d.alloc(n, m);
+    P.d.c.up(c.data(), n, s);
+    P.d.lo.up(lo.data(), n, s);
+    P.d.up.up(up.data(), n, s);
+    P.d.rl.up(rl.data(), m, s);
+    P.d.ru.up(ru.data(), m, s);
+    P.Rtot.alloc(m);
+    P.Ctot.alloc(n);
+    P.Rtot.up(rt.data(), m, s);
+    P.Ctot.up(ct.data(), n, s);
+
+    // ---- start point: clamp(0) or the warm start, in the scaled space
+    std::vector<double> x0(n), y0(m, 0.0);
+    for (int j = 0; j < n; ++j) {
+        double v = prm.x0 ? prm.x0[j] / Cc[j] : 0.0;
+        v = (lo[j] < v) ? v : lo[j];
+        v = (v < up[j]) ? v : up[j];
+        x0[j] = v;
+    }
+    if (prm.y0)
+        for (int i = 0; i < m; ++i) y0[i] = prm.y0[i] / R[i];
+    for (auto* b : {&P.d.x, &P.d.xa, &P.d.xp, &P.d.xbar}) b->up(x0.data(), n, s);
+    for (auto* b : {&P.d.y, &P.d.ya, &P.d.yp}) b->up(y0.data(), m, s);
+
+    // ---- power iteration on A'^T A'
+    Buf<double> u, v;
+    u.alloc(n);
+    v.alloc(m);
+    {
+        std::vector<double> u0(n);
+        for (int j = 0; j < n; ++j) u0[j] = 1.0 + static_cast<double>((static_cast<unsigned>(j) * 2654435761u) % 1000u) / 1000.0;
+        u.up(u0.data(), n, s);
+    }
+    Buf<dd> sq;
+    const int nsq = std::max(1, std::min(1024, (n + 255) / 256));
+    sq.alloc(nsq);
+    std::vector<dd> hsq(nsq);
+    auto norm_of_u = [&]() {
+        k_sumsq<<<nsq, kThreads, 0, s>>>(u.p, n, sq.p);
+        sq.down(hsq.data(), nsq, s);
+        PD_CHECK(hipStreamSynchronize(s));
+        dd t = dd_zero();
+        for (int b = 0; b < nsq; ++b) t = dd_add(t, hsq[b]);
+        return std::sqrt(dd_val(t));
+    };
+    auto spmv = [&](const Csr<double>& M, const double* in, double* out) {
+        switch (P.tile) {
+            case 1024: k_spmv_plain<1024><<<M.nblk, kThreads, 0, s>>>(M, in, out); break;
+            case 4096: k_spmv_plain<4096><<<M.nblk, kThreads, 0, s>>>(M, in, out); break;
+            default: k_spmv_plain<2048><<<M.nblk, kThreads, 0, s>>>(M, in, out); break;
+        }
+        PD_CHECK(hipGetLastError());
+    };
+    double lam = 0.0;
+    if (n > 0 && m > 0 && P.nnz > 0) {
+        double nu = norm_of_u();
+        if (nu > 0) k_scale_vec<<<gn, 256, 0, s>>>(u.p, n, nu);
+        for (int it = 0; it < 200; ++it) {
+            spmv(A, u.p, v.p);
+            spmv(AT, v.p, u.p);
+            const double nw = norm_of_u();
+            if (!(nw > 0.0) || !std::isfinite(nw)) break;
+            k_scale_vec<<<gn, 256, 0, s>>>(u.p, n, nw);
+            const double prev = lam;
+            lam = nw;
+            if (it > 0 && std::fabs(lam - prev) <= 1e-7 * lam) break;
+        }
+    }
+    const double smax = lam > 0.0 ? std::sqrt(lam) : 1.0;
+    const double eta = 0.998 / smax;
+
+    // ---- primal weight and state
+    DDHost cs, bs;
+    for (int j = 0; j < n; ++j) cs.add_prod(c[j], c[j]);
+    for (int i = 0; i < m; ++i) {
+        double b = 0.0;
+        if (std::isfinite(rl[i])) b = std::fabs(rl[i]);
+        if (std::isfinite(ru[i])) b = std::max(b, std::fabs(ru[i]));
+        bs.add_prod(b, b);
+    }
+    const double cnorm = std::sqrt(cs.value()), bnorm = std::sqrt(bs.value());
+    const double omega = (cnorm > 1e-10 && bnorm > 1e-10) ? cnorm / bnorm : 1.0;
+    PdlpState h{};
+    h.omega = omega;
+    h.eta = eta;
+    h.tau = eta / omega;
+    h.sigma = eta * omega;
+    h.tol = P.mixed ? P.tol : (P.polish_tol > 0 ? std::min(P.tol, P.polish_tol) : P.tol);
+    h.bnorm_o = std::sqrt(bn.value());
+    h.cnorm_o = std::sqrt(cn.value());
+    h.offset = P.offset;
+    h.max_iters = std::max<long long>(1, prm.max_iters);
+    h.eval_freq = P.eval_freq;
+    h.watch_stall = P.mixed ? 1 : 0;
+    P.st.alloc(1);
+    P.st.up(&h, 1, s);
+    PD_CHECK(hipHostMalloc(reinterpret_cast<void**>(&P.h_st), 2 * sizeof(PdlpState), hipHostMallocDefault));
+    std::memset(P.h_st, 0, 2 * sizeof(PdlpState));
+    for (auto& e : P.ev) PD_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    P.part_c.alloc(static_cast<size_t>(5) * AT.nblk);
+    P.part_r.alloc(static_cast<size_t>(6) * A.nblk);
+
+    if (P.mixed) {
+        P.a32.alloc(P.nnz);
+        P.at32.alloc(P.nnz);
+        k_to_float<<<grid_for(P.nnz), 256, 0, s>>>(A.val, P.a32.p, P.nnz);
+        k_to_float<<<grid_for(P.nnz), 256, 0, s>>>(AT.val, P.at32.p, P.nnz);
+        P.f.alloc(n, m);
+        auto conv = [&](Buf<double>& from, Buf<float>& to, long long len) {
+            k_to_float<<<grid_for(len), 256, 0, s>>>(from.p, to.p, len);
+        };
+        conv(P.d.c, P.f.c, n); conv(P.d.lo, P.f.lo, n); conv(P.d.up, P.f.up, n);
+        conv(P.d.rl, P.f.rl, m); conv(P.d.ru, P.f.ru, m);
+        conv(P.d.x, P.f.x, n); conv(P.d.x, P.f.xa, n); conv(P.d.x, P.f.xp, n); conv(P.d.x, P.f.xbar, n);
+        conv(P.d.y, P.f.y, m); conv(P.d.y, P.f.ya, m); conv(P.d.y, P.f.yp, m);
+        PD_CHECK(hipGetLastError());
+        launch_eval<float>(P, s, P.f, 1);
+    } else {
+        launch_eval<double>(P, s, P.d, 1);
+    }
+    PdlpState hs = read_state(P, s);
+    if (hs.done) {
+        P.finished = hs.status != 0 || !(P.mixed && P.polish_tol > 0);
+        if (!P.finished && P.mixed) switch_to_fp64(P, s, P.polish_tol);
+    }
+    P.setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void pdlp_run(Pdlp& P, hipStream_t s, const nir_params& prm, long long budget) {
+    auto t0 = std::chrono::steady_clock::now();
+    long long blocks = (budget + P.eval_freq - 1) / P.eval_freq;
+    const long long poll_every = std::max(1, 256 / P.eval_freq);
+    long long since_poll = 0;
+    while (!P.finished && blocks > 0) {
+        if ((P.polls % poll_every) == 0 && P.should_stop && P.should_stop(P.stop_user)) {
+            PdlpState h = read_state(P, s);
+            P.status = 4;
+            P.finished = true;
+            h.done = 1;
+            h.status = 4;
+            write_state(P, s, h);
+            break;
+        }
+        ++P.polls;
+        enqueue(P, s);
+        --blocks;
+        if (++since_poll >= poll_every || blocks == 0) {
+            since_poll = 0;
+            const int slot = static_cast<int>(P.serial++ & 1);
+            post_poll(P, s, slot);
+            if (wait_poll(P, slot ^ 1) || blocks == 0) {
+                PdlpState h = read_state(P, s);
+                if (h.done) {
+                    if (P.mixed && (h.status == 0 || h.status == 2) && !P.polish_done) {
+                        // FP32 phase met its tolerance (or stalled): FP64 tail.
+                        const double target = P.polish_tol > 0 ? std::min(P.polish_tol, P.tol) : P.tol;
+                        if (h.status == 0 && P.polish_tol <= 0) {
+                            P.finished = true;
+                            P.status = 0;
+                            break;
+                        }
+                        P.polish_done = true;
+                        switch_to_fp64(P, s, target);
+                        PdlpState h2 = read_state(P, s);
+                        if (h2.done) {
+                            P.finished = true;
+                            P.status = h2.status;
+                        }
+                        continue;
+                    }
+                    P.finished = true;
+                    P.status = h.status;
+                    break;
+                }
+            }
+        }
+    }
+    PD_CHECK(hipStreamSynchronize(s));
+    P.solve_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+int pdlp_result(Pdlp& P, hipStream_t s, double* x, double* y, nir_info* info) {
+    PdlpState h = read_state(P, s);
+    std::vector<double> xs(static_cast<size_t>(P.n)), ys(static_cast<size_t>(P.m));
+    if (P.mixed) {
+        std::vector<float> xf(static_cast<size_t>(P.n)), yf(static_cast<size_t>(P.m));
+        P.f.xp.down(xf.data(), P.n, s);
+        P.f.yp.down(yf.data(), P.m, s);
+        PD_CHECK(hipStreamSynchronize(s));
+        for (int j = 0; j < P.n; ++j) xs[j] = xf[j];
+        for (int i = 0; i < P.m; ++i) ys[i] = yf[i];
+    } else {
+        P.d.xp.down(xs.data(), P.n, s);
+        P.d.yp.down(ys.data(), P.m, s);
+    }
+    std::vector<double> R(static_cast<size_t>(P.m)), Cc(static_cast<size_t>(P.n));
+    P.R.down(R.data(), P.m, s);
+    P.C.down(Cc.data(), P.n, s);
+    PD_CHECK(hipStreamSynchronize(s));
+    for (int j = 0; j < P.n; ++j) xs[j] *= Cc[j];
+    for (int i = 0; i < P.m; ++i) ys[i] *= R[i];
+    if (x) std::copy(xs.begin(), xs.end(), x);
+    if (y) std::copy(ys.begin(), ys.end(), y);
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->status = P.finished ? P.status : 6;
+        if (!P.finished && h.done) info->status = h.status;
+        info->iterations = h.total;
+        info->polish_iterations = P.fp64_iters_at_switch >= 0 ? h.total - P.fp64_iters_at_switch : 0;
+        info->restarts = h.restarts;
+        DDHost acc(P.offset);
+        for (int j = 0; j < P.n; ++j)
+            if (P.h_c[j] != 0.0 && xs[j] != 0.0) acc.add_prod(P.h_c[j], xs[j]);
+        info->objective = acc.value();
+        info->tau = h.tau;
+        info->sigma = h.sigma;
+        info->primal_weight = h.omega;
+        info->rel_primal = h.rel_p;
+        info->rel_dual = h.rel_d;
+        info->rel_gap = h.rel_g;
+        info->setup_seconds = P.setup_s;
+        info->solve_seconds = P.solve_s;
+    }
+    return 0;
+}
+
+int pdlp_time(Pdlp& P, hipStream_t s, long long iters, nir_timing* out) {
+    // Per-kernel events around every iteration kernel of eval_freq-blocks.
+    const long long blocks = std::max<long long>(1, iters / P.eval_freq);
+    const int n = P.n, m = P.m;
+    const double nnz = static_cast<double>(P.nnz);
+    const double sv = P.mixed ? 4.0 : 8.0;
+    // A xbar + dual step: values+cols, rowptr, xbar once, y/ya/rl/ru read, y write.
+    out->spmv_bytes = nnz * (sv + 4) + 4.0 * (m + 1) + sv * n + sv * 5 * m;
+    // A^T y + primal step: values+cols, rowptr, y once, x/xa/c/lo/up read, x/xbar write.
+    out->spmtv_bytes = nnz * (sv + 4) + 4.0 * (n + 1) + sv * m + sv * 7 * n;
+    out->iter_bytes = out->spmv_bytes + out->spmtv_bytes;
+    std::vector<hipEvent_t> ev(static_cast<size_t>(4 * blocks * P.eval_freq + 2));
+    for (auto& e : ev) PD_CHECK(hipEventCreate(&e));
+    PD_CHECK(hipStreamSynchronize(s));
+    PD_CHECK(hipEventRecord(ev[0], s));
+    size_t q = 2;
+    for (long long b = 0; b < blocks; ++b) {
+        for (int kl = 0; kl < P.eval_freq; ++kl) {
+            const int save = kl == P.eval_freq - 1 ? 1 : 0;
+            auto one = [&](auto& it, auto Am, auto ATm) {
+                using T = typename std::remove_reference<decltype(*it.x.p)>::type;
+                switch (P.tile) {
+                    case 1024:
+                        PD_CHECK(hipEventRecord(ev[q++], s));
+                        k_primal_pdlp<1024, T><<<ATm.nblk, kThreads, 0, s>>>(ATm, it.y.p, it.x.p, it.xbar.p, it.xa.p, it.xp.p, it.c.p, it.lo.p, it.up.p, P.st.p, kl, save);
+                        PD_CHECK(hipEventRecord(ev[q++], s));
+                        PD_CHECK(hipEventRecord(ev[q++], s));
+                        k_dual_pdlp<1024, T><<<Am.nblk, kThreads, 0, s>>>(Am, it.xbar.p, it.y.p, it.ya.p, it.yp.p, it.rl.p, it.ru.p, P.st.p, kl, save);
+                        PD_CHECK(hipEventRecord(ev[q++], s));
+                        break;
+                    case 4096:
+                        PD_CHECK(hipEventRecord(ev[q++], s));
+                        k_primal_pdlp<4096, T><<<ATm.nblk, kThreads, 0, s>>>(ATm, it.y.p, it.x.p, it.xbar.p, it.xa.p, it.xp.p, it.c.p, it.lo.p, it.up.p, P.st.p, kl, save);
+                        PD_CHECK(hipEventRecord(ev[q++], s));
+                        PD_CHECK(hipEventRecord(ev[q++], s));
+                        k_dual_pdlp<4096, T><<<Am.nblk, kThreads, 0, s>>>(Am, it.xbar.p, it.y.p, it.ya.p, it.yp.p, it.rl.p, it.ru.p, P.st.p, kl, save);
+                        PD_CHECK(hipEventRecord(ev[q++], s));
+                        break;
+                    default:
+                        PD_CHECK(hipEventRecord(ev[q++], s));
+                        k_primal_pdlp<2048, T><<<ATm.nblk, kThreads, 0, s>>>(ATm, it.y.p, it.x.p, it.xbar.p, it.xa.p, it.xp.p, it.c.p, it.lo.p, it.up.p, P.st.p, kl, save);
+                        PD_CHECK(hipEventRecord(ev[q++], s));
+                        PD_CHECK(hipEventRecord(ev[q++], s));
+                        k_dual_pdlp<2048, T><<<Am.nblk, kThreads, 0, s>>>(Am, it.xbar.p, it.y.p, it.ya.p, it.yp.p, it.rl.p, it.ru.p, P.st.p, kl, save);
+                        PD_CHECK(hipEventRecord(ev[q++], s));
+                        break;
+                }
+            };
+            if (P.mixed) one(P.f, with_vals(P.A, P.a32.p), with_vals(P.AT, P.at32.p));
+            else one(P.d, P.A, P.AT);
+        }
+        if (P.mixed) launch_eval<float>(P, s, P.f, 0);
+        else launch_eval<double>(P, s, P.d, 0);
+    }
+    PD_CHECK(hipEventRecord(ev[1], s));
+    PD_CHECK(hipEventSynchronize(ev[1]));
+    float ms = 0.f;
+    PD_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    out->total_ms = ms;
+    for (size_t k = 2; k + 3 < q + 1; k += 4) {
+        PD_CHECK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+        out->spmtv_ms += ms;
+        PD_CHECK(hipEventElapsedTime(&ms, ev[k + 2], ev[k + 3]));
+        out->spmv_ms += ms;
+    }
+    for (auto& e : ev) PD_CHECK(hipEventDestroy(e));
+    out->iterations = blocks * P.eval_freq;
+    return 0;
+}
 
 }  // namespace gpu
 }  // namespace nir

@@ -64,6 +64,27 @@ int or_dd_verify(const or_lp* lp, const double* x, double claimed_obj, double to
  * dual, gap; out[3], out[4] = primal and dual objective. */
 void or_kkt(const or_lp* lp, const double* x, const double* y, double* out);
 
+typedef struct {
+    int64_t max_iters;
+    double tol;
+    int32_t eval_freq;
+    const double* unscale_row; /* may be NULL */
+    const double* unscale_col; /* may be NULL */
+    const double* x0;          /* may be NULL */
+    const double* y0;          /* may be NULL */
+} or_pdlp_params;
+
+typedef struct {
+    int32_t status; /* 0 optimal, 6 iteration limit, 11 numerical trouble */
+    int32_t restarts;
+    int64_t iterations;
+    double omega, eta, rel_p, rel_d, rel_g, pobj, dobj;
+} or_pdlp_info;
+
+/* The engine's pdlp mode in FP64 (see pdhg_oracle.c). x (n) and y (m) are
+ * returned in the LP's own units. */
+void or_pdlp(const or_lp* lp, const or_pdlp_params* p, double* x, double* y, or_pdlp_info* info);
+
 #ifdef __cplusplus
 }
 #endif

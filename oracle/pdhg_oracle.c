@@ -316,3 +316,369 @@ void or_kkt(const or_lp* lp, const double* x, const double* y, double* out) {
     free(ax);
     free(aty);
 }
+
+/* ------------------------------------------------------------------ PDLP */
+/* Restatement of the engine's "pdlp" mode (build-defined, no reference
+ * counterpart): exponent-domain Ruiz (10 passes) + Pock-Chambolle scaling,
+ * power iteration for eta = 0.998 / sigma_max, reflected Halpern PDHG with
+ * KKT restarts every eval_freq iterations and primal weight
+ * omega <- sqrt(omega ||dy|| / ||dx||). Sums of A products are sequential in
+ * ascending index order from 0.0 (as the device's CSR-stream kernels);
+ * norms and objectives are double-double. */
+
+static double pow2_inv_sqrt(double v) {
+    if (!(v > 0.0) || !isfinite(v)) return 1.0;
+    int e2;
+    (void)frexp(v, &e2);
+    const int e = e2 - 1;
+    const int k = (e + 1) >= 0 ? (e + 1) / 2 : -((-(e + 1) + 1) / 2);
+    return ldexp(1.0, -k);
+}
+
+static dd dd_sum(dd a, dd b) {
+    dd r = a;
+    dd_add(&r, b.hi, b.lo);
+    return r;
+}
+
+static dd ddp(double a, double b) {
+    const double p = a * b;
+    dd r = {p, fma(a, b, -p)};
+    return r;
+}
+
+static double ddv(dd a) { return a.hi + a.lo; }
+
+typedef struct {
+    double omega, tau, sigma, eta, kkt_anchor, kkt_prev, rel_p, rel_d, rel_g, pobj, dobj, tol, bnorm_o, cnorm_o, offset;
+    int64_t total, max_iters;
+    int k_base, done, status, restart, restarts, eval_freq;
+} pstate;
+
+typedef struct {
+    int32_t m, n;
+    int32_t *rp, *ci;   /* CSR(A) */
+    double *av;         /* CSR values (scaled) */
+    const int64_t* cs;  /* CSC(A) = CSR(A^T) */
+    const int32_t* ri;
+    double* tv;         /* CSC values (scaled) */
+} pmat;
+
+static double row_dot(const pmat* M, int32_t i, const double* v) {
+    double s = 0.0;
+    for (int32_t p = M->rp[i]; p < M->rp[i + 1]; ++p) s += M->av[p] * v[M->ci[p]];
+    return s;
+}
+
+static double col_dot(const pmat* M, int32_t j, const double* v) {
+    double s = 0.0;
+    for (int64_t p = M->cs[j]; p < M->cs[j + 1]; ++p) s += M->tv[p] * v[M->ri[p]];
+    return s;
+}
+
+/* Evaluation at (xp, yp) with anchor (xa, ya); mode 1 = start point. */
+static void pdlp_eval(const pmat* M, pstate* s, const double* c, const double* lo, const double* up, const double* rl,
+                      const double* ru, const double* rt, const double* ct, const double* xp, const double* yp,
+                      const double* xa, const double* ya, int mode) {
+    dd rd2 = {0, 0}, rd2o = {0, 0}, Pc = {0, 0}, Dc = {0, 0}, dx2 = {0, 0};
+    dd rp2 = {0, 0}, rp2o = {0, 0}, yv2 = {0, 0}, yv2o = {0, 0}, Dr = {0, 0}, dy2 = {0, 0};
+    for (int32_t j = 0; j < M->n; ++j) {
+        const double g = col_dot(M, j, yp);
+        const double lam = c[j] - g;
+        const int lf = lo[j] > -INFINITY, uf = up[j] < INFINITY;
+        double lp = lam;
+        if (!lf) lp = smin(lp, 0.0);
+        if (!uf) lp = smax(lp, 0.0);
+        const double rd = lam - lp;
+        rd2 = dd_sum(rd2, ddp(rd, rd));
+        const double rdo = rd / ct[j];
+        rd2o = dd_sum(rd2o, ddp(rdo, rdo));
+        Pc = dd_sum(Pc, ddp(c[j], xp[j]));
+        if (lf && lp > 0.0) Dc = dd_sum(Dc, ddp(lo[j], lp));
+        if (uf && lp < 0.0) Dc = dd_sum(Dc, ddp(up[j], lp));
+        const double dx = xp[j] - xa[j];
+        dx2 = dd_sum(dx2, ddp(dx, dx));
+    }
+    for (int32_t i = 0; i < M->m; ++i) {
+        const double h = row_dot(M, i, xp);
+        const int lf = rl[i] > -INFINITY, uf = ru[i] < INFINITY;
+        double proj = smax(rl[i], h);
+        proj = smin(ru[i], proj);
+        const double r = h - proj;
+        rp2 = dd_sum(rp2, ddp(r, r));
+        const double ro = r / rt[i];
+        rp2o = dd_sum(rp2o, ddp(ro, ro));
+        double ys = yp[i];
+        if (!lf) ys = smin(ys, 0.0);
+        if (!uf) ys = smax(ys, 0.0);
+        const double yv = yp[i] - ys;
+        yv2 = dd_sum(yv2, ddp(yv, yv));
+        const double yvo = yv * rt[i];
+        yv2o = dd_sum(yv2o, ddp(yvo, yvo));
+        if (lf && ys > 0.0) Dr = dd_sum(Dr, ddp(rl[i], ys));
+        if (uf && ys < 0.0) Dr = dd_sum(Dr, ddp(ru[i], ys));
+        const double dy = yp[i] - ya[i];
+        dy2 = dd_sum(dy2, ddp(dy, dy));
+    }
+    dd off = {s->offset, 0.0};
+    const double vrd2 = ddv(dd_sum(rd2, yv2)), vrd2o = ddv(dd_sum(rd2o, yv2o));
+    const double vrp2 = ddv(rp2), vrp2o = ddv(rp2o);
+    const double P = ddv(dd_sum(Pc, off)), D = ddv(dd_sum(dd_sum(Dc, Dr), off));
+    const double vdx2 = ddv(dx2), vdy2 = ddv(dy2);
+    const double gap = fabs(P - D);
+    s->rel_p = sqrt(vrp2o) / (1.0 + s->bnorm_o);
+    s->rel_d = sqrt(vrd2o) / (1.0 + s->cnorm_o);
+    s->rel_g = gap / (1.0 + fabs(P) + fabs(D));
+    s->pobj = P;
+    s->dobj = D;
+    const double worst = smax(s->rel_p, smax(s->rel_d, s->rel_g));
+    s->restart = 0;
+    if (!isfinite(vrd2) || !isfinite(vrp2) || !isfinite(P) || !isfinite(D) || !isfinite(vdx2) || !isfinite(vdy2)) {
+        s->done = 1;
+        s->status = 11;
+        return;
+    }
+    const double kkt = sqrt(s->omega * vrp2 + vrd2 / s->omega + gap * gap);
+    if (mode == 1) {
+        s->kkt_anchor = kkt;
+        s->kkt_prev = kkt;
+        if (worst <= s->tol) {
+            s->done = 1;
+            s->status = 0;
+        }
+        return;
+    }
+    s->total += s->eval_freq;
+    if (worst <= s->tol) {
+        s->done = 1;
+        s->status = 0;
+        return;
+    }
+    if (s->total >= s->max_iters) {
+        s->done = 1;
+        s->status = 6;
+        return;
+    }
+    const int k_new = s->k_base + s->eval_freq;
+    const int restart = (kkt <= 0.2 * s->kkt_anchor) || (kkt <= 0.8 * s->kkt_anchor && kkt > s->kkt_prev) ||
+                        ((double)k_new >= 0.36 * (double)s->total);
+    s->kkt_prev = kkt;
+    if (restart) {
+        if (vdx2 > 1e-20 && vdy2 > 1e-20) {
+            const double ratio = sqrt(vdy2) / sqrt(vdx2);
+            s->omega = sqrt(s->omega * ratio);
+            s->tau = s->eta / s->omega;
+            s->sigma = s->eta * s->omega;
+        }
+        s->kkt_anchor = sqrt(s->omega * vrp2 + vrd2 / s->omega + gap * gap);
+        s->kkt_prev = s->kkt_anchor;
+        s->k_base = 0;
+        s->restarts += 1;
+        s->restart = 1;
+    } else {
+        s->k_base = k_new;
+    }
+}
+
+void or_pdlp(const or_lp* lp, const or_pdlp_params* prm, double* x_out, double* y_out, or_pdlp_info* info) {
+    const int32_t m = lp->m, n = lp->n;
+    const int64_t nnz = n > 0 ? lp->start[n] : 0;
+    const size_t M1 = (size_t)(m > 0 ? m : 1), N1 = (size_t)(n > 0 ? n : 1), Z1 = (size_t)(nnz > 0 ? nnz : 1);
+    pmat Mt;
+    Mt.m = m;
+    Mt.n = n;
+    Mt.rp = calloc(M1 + 1, sizeof(int32_t));
+    Mt.ci = malloc(Z1 * sizeof(int32_t));
+    Mt.av = malloc(Z1 * sizeof(double));
+    Mt.cs = lp->start;
+    Mt.ri = lp->index;
+    Mt.tv = malloc(Z1 * sizeof(double));
+    for (int64_t p = 0; p < nnz; ++p) {
+        Mt.rp[lp->index[p] + 1]++;
+        Mt.tv[p] = lp->value[p];
+    }
+    for (int32_t i = 0; i < m; ++i) Mt.rp[i + 1] += Mt.rp[i];
+    {
+        int32_t* fill = malloc(M1 * sizeof(int32_t));
+        for (int32_t i = 0; i < m; ++i) fill[i] = Mt.rp[i];
+        for (int32_t j = 0; j < n; ++j)
+            for (int64_t p = lp->start[j]; p < lp->start[j + 1]; ++p) {
+                const int32_t q = fill[lp->index[p]]++;
+                Mt.ci[q] = j;
+                Mt.av[q] = lp->value[p];
+            }
+        free(fill);
+    }
+    /* scaling */
+    double* R = malloc(M1 * sizeof(double));
+    double* C = malloc(N1 * sizeof(double));
+    for (int32_t i = 0; i < m; ++i) R[i] = 1.0;
+    for (int32_t j = 0; j < n; ++j) C[j] = 1.0;
+    double* f = malloc((M1 > N1 ? M1 : N1) * sizeof(double));
+    for (int pass = 0; pass < 11; ++pass) {
+        const int sum = pass >= 10;
+        for (int32_t i = 0; i < m; ++i) {
+            double acc = 0.0;
+            for (int32_t p = Mt.rp[i]; p < Mt.rp[i + 1]; ++p) {
+                const double a = fabs(Mt.av[p]);
+                acc = sum ? acc + a : smax(acc, a);
+            }
+            f[i] = pow2_inv_sqrt(acc);
+            R[i] *= f[i];
+        }
+        for (int32_t i = 0; i < m; ++i)
+            for (int32_t p = Mt.rp[i]; p < Mt.rp[i + 1]; ++p) Mt.av[p] = Mt.av[p] * (f[i] * 1.0);
+        for (int64_t p = 0; p < nnz; ++p) Mt.tv[p] = Mt.tv[p] * (1.0 * f[lp->index[p]]);
+        for (int32_t j = 0; j < n; ++j) {
+            double acc = 0.0;
+            for (int64_t p = lp->start[j]; p < lp->start[j + 1]; ++p) {
+                const double a = fabs(Mt.tv[p]);
+                acc = sum ? acc + a : smax(acc, a);
+            }
+            f[j] = pow2_inv_sqrt(acc);
+            C[j] *= f[j];
+        }
+        for (int32_t j = 0; j < n; ++j)
+            for (int64_t p = lp->start[j]; p < lp->start[j + 1]; ++p) Mt.tv[p] = Mt.tv[p] * (f[j] * 1.0);
+        for (int32_t i = 0; i < m; ++i)
+            for (int32_t p = Mt.rp[i]; p < Mt.rp[i + 1]; ++p) Mt.av[p] = Mt.av[p] * (1.0 * f[Mt.ci[p]]);
+    }
+    free(f);
+    /* vectors in the scaled space */
+    double *c = malloc(N1 * 8), *lo = malloc(N1 * 8), *up = malloc(N1 * 8), *ct = malloc(N1 * 8);
+    double *rl = malloc(M1 * 8), *ru = malloc(M1 * 8), *rt = malloc(M1 * 8);
+    dd bn = {0, 0}, cn = {0, 0};
+    for (int32_t j = 0; j < n; ++j) {
+        c[j] = lp->c[j] * C[j];
+        lo[j] = fin(lp->l[j]) ? lp->l[j] / C[j] : -INFINITY;
+        up[j] = fin(lp->u[j]) ? lp->u[j] / C[j] : INFINITY;
+        const double us = prm->unscale_col ? prm->unscale_col[j] : 1.0;
+        ct[j] = C[j] / us;
+        const double cu = lp->c[j] / us;
+        dd_add_prod(&cn, cu, cu);
+    }
+    for (int32_t i = 0; i < m; ++i) {
+        rl[i] = fin(lp->rl[i]) ? lp->rl[i] * R[i] : -INFINITY;
+        ru[i] = fin(lp->ru[i]) ? lp->ru[i] * R[i] : INFINITY;
+        const double us = prm->unscale_row ? prm->unscale_row[i] : 1.0;
+        rt[i] = R[i] * us;
+        double b = 0.0;
+        if (fin(lp->rl[i])) b = fabs(lp->rl[i]);
+        if (fin(lp->ru[i])) b = smax(b, fabs(lp->ru[i]));
+        b = b / us;
+        dd_add_prod(&bn, b, b);
+    }
+    /* start point */
+    double *x = malloc(N1 * 8), *xa = malloc(N1 * 8), *xp = malloc(N1 * 8), *xb = malloc(N1 * 8);
+    double *y = malloc(M1 * 8), *ya = malloc(M1 * 8), *yp = malloc(M1 * 8);
+    for (int32_t j = 0; j < n; ++j) {
+        double v = prm->x0 ? prm->x0[j] / C[j] : 0.0;
+        v = (lo[j] < v) ? v : lo[j];
+        v = (v < up[j]) ? v : up[j];
+        x[j] = xa[j] = xp[j] = xb[j] = v;
+    }
+    for (int32_t i = 0; i < m; ++i) y[i] = ya[i] = yp[i] = prm->y0 ? prm->y0[i] / R[i] : 0.0;
+    /* power iteration on A'^T A' */
+    double lam = 0.0;
+    if (n > 0 && m > 0 && nnz > 0) {
+        double* u = malloc(N1 * 8);
+        double* v = malloc(M1 * 8);
+        for (int32_t j = 0; j < n; ++j) u[j] = 1.0 + (double)(((unsigned)j * 2654435761u) % 1000u) / 1000.0;
+        dd s2 = {0, 0};
+        for (int32_t j = 0; j < n; ++j) s2 = dd_sum(s2, ddp(u[j], u[j]));
+        double nu = sqrt(ddv(s2));
+        if (nu > 0)
+            for (int32_t j = 0; j < n; ++j) u[j] = u[j] / nu;
+        for (int it = 0; it < 200; ++it) {
+            for (int32_t i = 0; i < m; ++i) v[i] = row_dot(&Mt, i, u);
+            for (int32_t j = 0; j < n; ++j) u[j] = col_dot(&Mt, j, v);
+            dd t = {0, 0};
+            for (int32_t j = 0; j < n; ++j) t = dd_sum(t, ddp(u[j], u[j]));
+            const double nw = sqrt(ddv(t));
+            if (!(nw > 0.0) || !isfinite(nw)) break;
+            for (int32_t j = 0; j < n; ++j) u[j] = u[j] / nw;
+            const double prev = lam;
+            lam = nw;
+            if (it > 0 && fabs(lam - prev) <= 1e-7 * lam) break;
+        }
+        free(u);
+        free(v);
+    }
+    const double smx = lam > 0.0 ? sqrt(lam) : 1.0;
+    pstate s;
+    memset(&s, 0, sizeof(s));
+    s.eta = 0.998 / smx;
+    {
+        dd cs = {0, 0}, bs = {0, 0};
+        for (int32_t j = 0; j < n; ++j) dd_add_prod(&cs, c[j], c[j]);
+        for (int32_t i = 0; i < m; ++i) {
+            double b = 0.0;
+            if (isfinite(rl[i])) b = fabs(rl[i]);
+            if (isfinite(ru[i])) b = smax(b, fabs(ru[i]));
+            dd_add_prod(&bs, b, b);
+        }
+        const double cnorm = sqrt(cs.hi + cs.lo), bnorm = sqrt(bs.hi + bs.lo);
+        s.omega = (cnorm > 1e-10 && bnorm > 1e-10) ? cnorm / bnorm : 1.0;
+    }
+    s.tau = s.eta / s.omega;
+    s.sigma = s.eta * s.omega;
+    s.tol = prm->tol;
+    s.bnorm_o = sqrt(bn.hi + bn.lo);
+    s.cnorm_o = sqrt(cn.hi + cn.lo);
+    s.offset = lp->offset;
+    s.max_iters = prm->max_iters > 0 ? prm->max_iters : 1;
+    s.eval_freq = prm->eval_freq > 0 ? prm->eval_freq : 64;
+    pdlp_eval(&Mt, &s, c, lo, up, rl, ru, rt, ct, xp, yp, xa, ya, 1);
+    while (!s.done) {
+        for (int kl = 0; kl < s.eval_freq; ++kl) {
+            const int k = s.k_base + kl;
+            const double w = (double)(k + 1) / (double)(k + 2);
+            const double wc = 1.0 / (double)(k + 2);
+            const int save = kl == s.eval_freq - 1;
+            for (int32_t j = 0; j < n; ++j) {
+                const double g = col_dot(&Mt, j, y);
+                const double xj = x[j];
+                double cand = xj - s.tau * (c[j] - g);
+                cand = (lo[j] < cand) ? cand : lo[j];
+                cand = (cand < up[j]) ? cand : up[j];
+                const double b = 2.0 * cand - xj;
+                xb[j] = b;
+                x[j] = w * b + wc * xa[j];
+                if (save) xp[j] = cand;
+            }
+            for (int32_t i = 0; i < m; ++i) {
+                const double h = row_dot(&Mt, i, xb);
+                const double yi = y[i];
+                double r = h - yi / s.sigma;
+                r = (rl[i] < r) ? r : rl[i];
+                r = (r < ru[i]) ? r : ru[i];
+                const double ypi = yi + s.sigma * (r - h);
+                const double b = 2.0 * ypi - yi;
+                y[i] = w * b + wc * ya[i];
+                if (save) yp[i] = ypi;
+            }
+        }
+        pdlp_eval(&Mt, &s, c, lo, up, rl, ru, rt, ct, xp, yp, xa, ya, 0);
+        if (s.restart) {
+            for (int32_t j = 0; j < n; ++j) x[j] = xa[j] = xp[j];
+            for (int32_t i = 0; i < m; ++i) y[i] = ya[i] = yp[i];
+        }
+    }
+    for (int32_t j = 0; j < n; ++j) x_out[j] = xp[j] * C[j];
+    for (int32_t i = 0; i < m; ++i) y_out[i] = yp[i] * R[i];
+    if (info) {
+        info->status = s.status;
+        info->iterations = s.total;
+        info->restarts = s.restarts;
+        info->omega = s.omega;
+        info->eta = s.eta;
+        info->rel_p = s.rel_p;
+        info->rel_d = s.rel_d;
+        info->rel_g = s.rel_g;
+        info->pobj = s.pobj;
+        info->dobj = s.dobj;
+    }
+    free(Mt.rp); free(Mt.ci); free(Mt.av); free(Mt.tv); free(R); free(C);
+    free(c); free(lo); free(up); free(ct); free(rl); free(ru); free(rt);
+    free(x); free(xa); free(xp); free(xb); free(y); free(ya); free(yp);
+}

@@ -22,6 +22,17 @@ _i64p = C.POINTER(C.c_int64)
 _i32p = C.POINTER(C.c_int32)
 
 
+class OrPdlpParams(C.Structure):
+    _fields_ = [("max_iters", C.c_int64), ("tol", C.c_double), ("eval_freq", C.c_int32), ("unscale_row", _dp),
+                ("unscale_col", _dp), ("x0", _dp), ("y0", _dp)]
+
+
+class OrPdlpInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("restarts", C.c_int32), ("iterations", C.c_int64), ("omega", C.c_double),
+                ("eta", C.c_double), ("rel_p", C.c_double), ("rel_d", C.c_double), ("rel_g", C.c_double),
+                ("pobj", C.c_double), ("dobj", C.c_double)]
+
+
 class OrLp(C.Structure):
     _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("start", _i64p), ("index", _i32p), ("value", _dp),
                 ("c", _dp), ("l", _dp), ("u", _dp), ("rl", _dp), ("ru", _dp), ("offset", C.c_double)]
@@ -49,6 +60,7 @@ def oracle():
         L.or_pdhg_reference.argtypes = [C.POINTER(OrLp), C.c_int64, C.c_int, _dp, _dp, _dp, _dp, _dp, _i64p, _dp]
         L.or_dd_verify.argtypes = [C.POINTER(OrLp), _dp, C.c_double, C.c_double, _dp, _dp, _dp]
         L.or_kkt.argtypes = [C.POINTER(OrLp), _dp, _dp, _dp]
+        L.or_pdlp.argtypes = [C.POINTER(OrLp), C.POINTER(OrPdlpParams), _dp, _dp, C.POINTER(OrPdlpInfo)]
         _oracle = L
     return _oracle
 
@@ -113,6 +125,15 @@ class Oracle:
         self.L.or_pdhg_reference(C.byref(self.s), int(K), 1 if check else 0, _p(x0), _p(y0), _p(x), _p(y),
                                  C.byref(tau), C.byref(it), C.byref(V))
         return x, y, tau.value, it.value, V.value
+
+    def pdlp(self, max_iters=100000, tol=1e-6, eval_freq=64, unscale_row=None, unscale_col=None, x0=None, y0=None):
+        keep = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (unscale_row, unscale_col, x0, y0)]
+        p = OrPdlpParams(int(max_iters), float(tol), int(eval_freq), *[_p(a) for a in keep])
+        x = np.zeros(self.lp.n)
+        y = np.zeros(self.lp.m)
+        info = OrPdlpInfo()
+        self.L.or_pdlp(C.byref(self.s), C.byref(p), _p(x), _p(y), C.byref(info))
+        return x, y, info
 
     def verify(self, x, claimed, tol=1e-6):
         x = np.ascontiguousarray(x, np.float64)

@@ -1,0 +1,101 @@
+"""GPU parity of the engine's pdlp mode (restarted reflected-Halpern PDHG)
+against the oracle's or_pdlp, and the mixed-precision / polish contract.
+
+FP64: every sum of A products is sequential in ascending index order (as in
+the oracle), norms are double-double and every decision uses only IEEE +, *,
+/ and sqrt, so the device trajectory is expected to be bit-identical to the
+oracle's; the acceptance bar is 1e-10 relative on the returned iterates with
+equal iteration and restart counts.
+Mixed: FP32 iterates and SpMV; the result must meet the tolerance under the
+independent FP64/double-double evaluator, and its error against the FP64
+oracle solution is reported."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from nirnaya_amd.engine import (MODE_PDLP, PREC_MIXED, default_params, gen_equality_lp, gen_mixed_lp,
+                                gen_staircase_lp)
+from oracle_lib import Oracle
+import lpkit
+
+pytestmark = pytest.mark.gpu
+HIGHS = json.load(open(os.path.join(lpkit.GOLDEN, "highs_optima.json")))
+
+CASES = {
+    "unit11": lpkit.pdhg_unit_lp,
+    "transport": lpkit.transport_lp,
+    "staircase_3_6x8": lambda: gen_staircase_lp(3, 6, 8),
+    "equality_20260_2000x5000": lambda: gen_equality_lp(20260, 2000, 5000, 5),
+    "mixed_11_600x1800": lambda: gen_mixed_lp(11, 600, 1800, 6.0, 0.01, 1e3),
+}
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(1.0, float(np.max(np.abs(b))))) if a.size else 0.0
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_fp64_pdlp_matches_oracle(engine, name):
+    lp = CASES[name]()
+    r = engine.solve(lp, default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-6))
+    xo, yo, info = Oracle(lp).pdlp(max_iters=400000, tol=1e-6)
+    print(f"{name}: it {r.iterations}/{info.iterations} restarts {r.info.restarts}/{info.restarts} "
+          f"rel x {rel(r.x, xo):.2e} y {rel(r.y, yo):.2e} exact {np.array_equal(r.x, xo) and np.array_equal(r.y, yo)}")
+    assert r.status == info.status == 0
+    assert r.iterations == info.iterations and r.info.restarts == info.restarts
+    assert rel(r.x, xo) <= 1e-10 and rel(r.y, yo) <= 1e-10
+    k = Oracle(lp).kkt(r.x, r.y)
+    assert max(k[0], k[1], k[2]) <= 1.5e-6
+    if name in HIGHS:
+        assert abs(k[3] - HIGHS[name]) <= 1e-4 * (1 + abs(HIGHS[name]))
+
+
+def test_graph_and_eager_identical(engine):
+    lp = gen_staircase_lp(4, 8, 9)
+    a = engine.solve(lp, default_params(mode=MODE_PDLP, max_iters=100000, use_graph=1))
+    b = engine.solve(lp, default_params(mode=MODE_PDLP, max_iters=100000, use_graph=0))
+    assert a.iterations == b.iterations and np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+
+
+def test_iteration_cap_and_warm_start(engine):
+    lp = gen_equality_lp(20260, 2000, 5000, 5)
+    r = engine.solve(lp, default_params(mode=MODE_PDLP, max_iters=640, tol=1e-9))
+    assert r.status == 6 and r.iterations == 640
+    u = lpkit.pdhg_unit_lp()
+    w = engine.solve(u, default_params(mode=MODE_PDLP, tol=1e-9), x0=np.array([3.0, 1.0]), y0=np.array([1.5, 0.5]))
+    assert w.status == 0 and w.iterations == 0
+
+
+@pytest.mark.parametrize("name", ["equality_20260_2000x5000", "staircase_3_6x8", "mixed_11_600x1800"])
+def test_mixed_precision_meets_tolerance_and_polish(engine, name):
+    lp = CASES[name]()
+    xo, yo, info = Oracle(lp).pdlp(max_iters=400000, tol=1e-9)
+    r = engine.solve(lp, default_params(mode=MODE_PDLP, precision=PREC_MIXED, max_iters=400000, tol=1e-6))
+    k = Oracle(lp).kkt(r.x, r.y)
+    ex = np.linalg.norm(r.x - xo) / max(1e-30, np.linalg.norm(xo))
+    print(f"{name} mixed: status {r.status} it {r.iterations} (fp64 tail {r.info.polish_iterations}) "
+          f"kkt {k[0]:.1e} {k[1]:.1e} {k[2]:.1e} |x-x64|/|x64| {ex:.2e} obj err {abs(k[3]-info.pobj)/(1+abs(info.pobj)):.2e}")
+    assert r.status == 0
+    assert max(k[0], k[1], k[2]) <= 2e-6
+    p = engine.solve(lp, default_params(mode=MODE_PDLP, precision=PREC_MIXED, max_iters=400000, tol=1e-6,
+                                        polish_tol=1e-8))
+    kp = Oracle(lp).kkt(p.x, p.y)
+    print(f"{name} polished: it {p.iterations} (fp64 tail {p.info.polish_iterations}) kkt {kp[0]:.1e} {kp[1]:.1e} {kp[2]:.1e}")
+    assert p.status == 0 and p.info.polish_iterations > 0
+    assert max(kp[0], kp[1], kp[2]) <= 2e-8
+
+
+def test_facade_pdlp_mode_on_fixture(tmp_path):
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sol = tmp_path / "c.sol"
+    rc = subprocess.run([os.path.join(root, "bin", "sovsolve"), lpkit.CRUDE_MPS, "--method=pdhg", "--pdhg-mode=pdlp",
+                         "--tol=1e-9", "--log", "0", "--sol", str(sol)]).returncode
+    assert rc == 0
+    chk = subprocess.run([os.path.join(root, "bin", "checker"), lpkit.CRUDE_MPS, str(sol)], capture_output=True,
+                         text=True)
+    assert chk.returncode == 0, chk.stdout
+    obj, _ = lpkit.read_sol(str(sol))
+    assert obj == pytest.approx(22171.59777424483, rel=1e-6)
