@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run the 256 MiB copy kernel (reference for rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is design-only here.",
@@ -85,6 +87,9 @@ def main():
     log(f"[bench] instance m={lp.m} n={lp.n} nnz={lp.nnz} generated in {time.time() - t0:.1f}s")
 
     eng = Engine(0)
+    if args.calibrate:
+        ms = eng.calibrate_copy(256 << 20)
+        log(f"[bench] calibration copy of 256 MiB: {ms:.3f} ms ({2 * 256 * 1.048576 / ms:.0f} GB/s read+write)")
     params = default_params(mode=MODE_REFERENCE, max_iters=args.warmup + args.steps + 64, check_termination=0,
                             tile_nnz=args.tile, tile_rows=args.tile_rows)
     sess = eng.session(lp, params)

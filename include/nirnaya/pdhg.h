@@ -111,6 +111,10 @@ NIR_API int nir_session_result(nir_session* s, double* x, double* y, nir_info* i
 NIR_API int nir_session_time(nir_session* s, int64_t iters, nir_timing* out);
 NIR_API void nir_session_destroy(nir_session* s);
 
+/* Copies `bytes` (read) into a second buffer (write) with a plain grid-stride
+ * kernel, k_copy_calibrate: the known-size reference for HBM byte counters. */
+NIR_API int nir_calibrate_copy(nir_ctx* ctx, int64_t bytes, double* ms);
+
 /* One-shot: create, solve, copy out, destroy. */
 NIR_API int nir_pdhg_solve(nir_ctx* ctx, const nir_lp* lp, const nir_params* p, double* x, double* y,
                            nir_info* info);

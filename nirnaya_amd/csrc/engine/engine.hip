@@ -123,6 +123,14 @@ __global__ __launch_bounds__(kThreads) void k_dual_ref(Csr<double> A, const doub
 }
 
 __global__ void k_set_base(RefState* st, int base) { st->base = base; }
+
+// Plain grid-stride copy: the calibration kernel for HBM byte counters.
+__global__ __launch_bounds__(256) void k_copy_calibrate(const double4* __restrict__ in, double4* __restrict__ out,
+                                                        long long n) {
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        out[i] = in[i];
+}
 __global__ void k_advance(RefState* st, int by) { st->base += by; }
 
 // ------------------------------------------------------------------ host helpers
@@ -683,6 +691,36 @@ NIR_API int nir_session_time(nir_session* S, int64_t iters, nir_timing* out) {
         return 0;
     } catch (const std::exception& ex) {
         S->ctx->error = ex.what();
+        return -1;
+    }
+}
+
+NIR_API int nir_calibrate_copy(nir_ctx* ctx, int64_t bytes, double* ms) {
+    if (!ctx || bytes <= 0) return -1;
+    try {
+        HIP_CHECK(hipSetDevice(ctx->device));
+        const long long n = bytes / static_cast<long long>(sizeof(double4));
+        DevArray<double4> a, b;
+        a.alloc(n);
+        b.alloc(n);
+        HIP_CHECK(hipMemsetAsync(a.p, 0, n * sizeof(double4), ctx->stream));
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        HIP_CHECK(hipEventRecord(e0, ctx->stream));
+        k_copy_calibrate<<<2048, 256, 0, ctx->stream>>>(a.p, b.p, n);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipEventRecord(e1, ctx->stream));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float t = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+        if (ms) *ms = t;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    } catch (const std::exception& ex) {
+        ctx->error = ex.what();
         return -1;
     }
 }

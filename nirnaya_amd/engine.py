@@ -101,6 +101,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.nir_session_result.argtypes = [vp, _dp, _dp, C.POINTER(NirInfo)]
     lib.nir_session_time.argtypes = [vp, C.c_int64, C.POINTER(NirTiming)]
     lib.nir_session_destroy.argtypes = [vp]
+    lib.nir_calibrate_copy.argtypes = [vp, C.c_int64, _dp]
     lib.nir_pdhg_solve.argtypes = [vp, C.POINTER(NirLp), C.POINTER(NirParams), _dp, _dp, C.POINTER(NirInfo)]
     for name in ("nir_gen_equality_lp", "nir_gen_mixed_lp", "nir_gen_staircase_lp"):
         getattr(lib, name).restype = vp
@@ -262,6 +263,12 @@ class Engine:
 
     def error(self) -> str:
         return self.lib.nir_ctx_error(self.h).decode()
+
+    def calibrate_copy(self, nbytes: int) -> float:
+        ms = C.c_double()
+        if self.lib.nir_calibrate_copy(self.h, int(nbytes), C.byref(ms)) != 0:
+            raise EngineError("nir_calibrate_copy: " + self.error())
+        return ms.value
 
     def session(self, lp: LP, params: Optional[NirParams] = None, x0=None, y0=None) -> "Session":
         return Session(self, lp, params, x0, y0)
