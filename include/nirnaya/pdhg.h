@@ -57,8 +57,10 @@ typedef struct {
                                   0 = run exactly max_iters iterations */
     int32_t use_graph;         /* capture iteration blocks in hipGraphs */
     int32_t verbose;
-    int32_t tile_nnz;          /* SpMV tile: nonzeros per workgroup (0 = default) */
-    int32_t tile_rows;         /* SpMV tile: max rows per workgroup (0 = default) */
+    int32_t tile_nnz;          /* SpMV tile: nonzeros per workgroup (0 = default 2048,
+                                  -1 = autotune tile, rows and segments per matrix) */
+    int32_t tile_rows;         /* SpMV tile: max rows per workgroup (0 = default 256) */
+    int32_t seg_width;         /* column-segment width of the products (0 = one segment) */
     const double* x0;          /* optional warm start (ncol), NULL = cold */
     const double* y0;          /* optional warm start (nrow), NULL = cold */
     const double* unscale_row; /* optional: report KKT in units y*row, (Ax)/row */
@@ -92,6 +94,8 @@ typedef struct {
     double spmv_bytes;          /* algorithmic bytes of one A xbar launch */
     double spmtv_bytes;         /* algorithmic bytes of one A^T y launch */
     double iter_bytes;          /* algorithmic bytes of one whole iteration */
+    int32_t spmv_passes;        /* column-segment passes per A xbar */
+    int32_t spmtv_passes;       /* column-segment passes per A^T y */
 } nir_timing;
 
 NIR_API void nir_params_default(nir_params* p);
@@ -109,6 +113,8 @@ NIR_API int nir_session_solve(nir_session* s);
 /* Copies the current iterate (x: ncol, y: nrow; either may be NULL). */
 NIR_API int nir_session_result(nir_session* s, double* x, double* y, nir_info* info);
 NIR_API int nir_session_time(nir_session* s, int64_t iters, nir_timing* out);
+/* Tile shapes in use: {A tile, A rows, A segments, A^T tile, A^T rows, A^T segments}. */
+NIR_API int nir_session_shape(nir_session* s, int32_t* out6);
 NIR_API void nir_session_destroy(nir_session* s);
 
 /* Copies `bytes` (read) into a second buffer (write) with a plain grid-stride

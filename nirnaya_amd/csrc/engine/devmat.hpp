@@ -1,0 +1,196 @@
+// Device-resident sparse matrices for the CSR-stream kernels: HIP error
+// checking, owning device arrays, the row-block partition and the
+// column-segmented layout (see spmv.hpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "spmv.hpp"
+
+namespace nir {
+namespace gpu {
+
+struct HipError : std::runtime_error {
+    explicit HipError(const std::string& what) : std::runtime_error(what) {}
+};
+
+#define HIP_CHECK(expr)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            throw ::nir::gpu::HipError(std::string(#expr) + " failed: " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+struct DevArray {
+    T* p = nullptr;
+    size_t n = 0;
+    DevArray() = default;
+    DevArray(const DevArray&) = delete;
+    DevArray& operator=(const DevArray&) = delete;
+    ~DevArray() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(count, 1) * sizeof(T)));
+    }
+    void upload(const T* src, size_t count, hipStream_t s) {
+        if (count > n) throw std::runtime_error("upload overflow");
+        if (count) HIP_CHECK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void download(T* dst, size_t count, hipStream_t s) const {
+        if (count > n) throw std::runtime_error("download overflow");
+        if (count) HIP_CHECK(hipMemcpyAsync(dst, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+};
+
+// Row blocks for CSR-stream: <= tile_rows rows and <= tile_nnz nonzeros per
+// block, or one row longer than tile_nnz on its own. Always >= 1 block.
+inline std::vector<int> partition_rows(const int* rowptr, int nrows, int tile_nnz, int tile_rows) {
+    std::vector<int> blk{0};
+    int r = 0;
+    while (r < nrows) {
+        const int s = r;
+        if (rowptr[r + 1] - rowptr[r] > tile_nnz) {
+            ++r;
+        } else {
+            while (r < nrows && r - s < tile_rows && rowptr[r + 1] - rowptr[s] <= tile_nnz) ++r;
+        }
+        blk.push_back(r);
+    }
+    if (blk.size() == 1) blk.push_back(0);
+    return blk;
+}
+
+// Tile shape of one matrix: LDS tile (values), rows per workgroup and the
+// column-segment width (0 = one segment).
+struct TileShape {
+    int tile = 2048;
+    int rows = 256;
+    int seg_width = 0;
+};
+
+// Structure (row pointers per segment, column indices, row blocks) of a
+// segmented CSR matrix; value arrays of either precision are kept beside it.
+struct CsrShape {
+    int nrows = 0, ncols = 0, nnz = 0, nseg = 1;
+    TileShape shape;
+    DevArray<int> rowptr;   // nseg * (nrows + 1), absolute offsets
+    DevArray<int> col;      // nnz, segment-major
+    DevArray<int4> desc;    // concatenated per-segment block descriptors
+    std::vector<int> blk_off, nblk;
+
+    template <typename T>
+    Csr<T> view(const T* val, int s) const {
+        return Csr<T>{nrows, ncols, rowptr.p + static_cast<size_t>(s) * (nrows + 1), col.p, val, nblk[s],
+                      desc.p + blk_off[s]};
+    }
+    template <typename T>
+    Pass<T> pass(const T* val, T* part, int s) const {
+        return Pass<T>{view(val, s), part, s == 0 ? 1 : 0, s == nseg - 1 ? 1 : 0};
+    }
+};
+
+// Upload a CSR matrix (columns ascending within rows) in the segmented
+// layout. `perm` receives, for each stored slot, the input nonzero it holds.
+inline void upload_shape(CsrShape& S, int nrows, int ncols, const std::vector<int>& rowptr, const int* col,
+                         TileShape shape, std::vector<int>& perm, hipStream_t st) {
+    S.nrows = nrows;
+    S.ncols = ncols;
+    S.nnz = rowptr.back();
+    S.shape = shape;
+    const int W = (shape.seg_width > 0 && shape.seg_width < ncols) ? shape.seg_width : std::max(ncols, 1);
+    S.nseg = std::max(1, (ncols + W - 1) / W);
+    const int nseg = S.nseg;
+    std::vector<int> rp(static_cast<size_t>(nseg) * (nrows + 1));
+    perm.assign(static_cast<size_t>(S.nnz), 0);
+    // Count entries per (segment, row), then lay segments out one after another.
+    std::vector<long long> seg_count(nseg, 0);
+    for (int p = 0; p < S.nnz; ++p) ++seg_count[col[p] / W];
+    std::vector<long long> seg_base(nseg + 1, 0);
+    for (int s = 0; s < nseg; ++s) seg_base[s + 1] = seg_base[s] + seg_count[s];
+    std::vector<long long> fill(seg_base.begin(), seg_base.end() - 1);
+    for (int s = 0; s < nseg; ++s) rp[static_cast<size_t>(s) * (nrows + 1)] = static_cast<int>(seg_base[s]);
+    for (int r = 0; r < nrows; ++r) {
+        for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) {
+            const int s = col[p] / W;
+            perm[fill[s]++] = p;
+        }
+        for (int s = 0; s < nseg; ++s) rp[static_cast<size_t>(s) * (nrows + 1) + r + 1] = static_cast<int>(fill[s]);
+    }
+    std::vector<int> c2(static_cast<size_t>(S.nnz));
+    for (int q = 0; q < S.nnz; ++q) c2[q] = col[perm[q]];
+    std::vector<int4> desc_all;
+    S.blk_off.assign(nseg, 0);
+    S.nblk.assign(nseg, 0);
+    for (int s = 0; s < nseg; ++s) {
+        const int* srp = rp.data() + static_cast<size_t>(s) * (nrows + 1);
+        std::vector<int> b = partition_rows(srp, nrows, shape.tile, std::min(shape.rows, kThreads));
+        S.blk_off[s] = static_cast<int>(desc_all.size());
+        S.nblk[s] = static_cast<int>(b.size()) - 1;
+        for (size_t k = 0; k + 1 < b.size(); ++k) {
+            int4 d;
+            d.x = b[k];
+            d.y = b[k + 1];
+            d.z = srp[b[k]];
+            d.w = srp[b[k + 1]];
+            desc_all.push_back(d);
+        }
+    }
+    S.rowptr.alloc(rp.size());
+    S.rowptr.upload(rp.data(), rp.size(), st);
+    S.col.alloc(c2.size());
+    S.col.upload(c2.data(), c2.size(), st);
+    S.desc.alloc(desc_all.size());
+    S.desc.upload(desc_all.data(), desc_all.size(), st);
+    HIP_CHECK(hipStreamSynchronize(st));
+}
+
+// A matrix on the device: segmented structure plus its FP64 values and the
+// running-sum buffer of the segment passes.
+struct DevMatrix {
+    CsrShape shape;
+    DevArray<double> val;
+    DevArray<double> part;
+};
+
+inline void upload_matrix(DevMatrix& M, int nrows, int ncols, const std::vector<int>& rowptr, const int* col,
+                          const double* val, TileShape ts, hipStream_t s) {
+    std::vector<int> perm;
+    upload_shape(M.shape, nrows, ncols, rowptr, col, ts, perm, s);
+    std::vector<double> v(perm.size());
+    for (size_t q = 0; q < perm.size(); ++q) v[q] = val[perm[q]];
+    M.val.alloc(v.size());
+    M.val.upload(v.data(), v.size(), s);
+    M.part.alloc(static_cast<size_t>(std::max(nrows, 1)));
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace gpu
+}  // namespace nir
+
+// Instantiate a tile-templated launch for the matrix's LDS tile size.
+#define TILE_DISPATCH(tile, ...)     \
+    switch (tile) {                  \
+        case 1024: {                 \
+            constexpr int TL = 1024; \
+            __VA_ARGS__;             \
+        } break;                     \
+        case 4096: {                 \
+            constexpr int TL = 4096; \
+            __VA_ARGS__;             \
+        } break;                     \
+        default: {                   \
+            constexpr int TL = 2048; \
+            __VA_ARGS__;             \
+        } break;                     \
+    }

@@ -14,35 +14,28 @@
 //   violation slot and raises a done flag that every later kernel honours.
 // The host only enqueues iteration blocks (hipGraph-captured) and polls the
 // done flag once per 256 iterations, which is also where the deadline is
-// checked (:63).
+// checked (:63). Each product may run as several column-segment passes
+// (spmv.hpp); the fused epilogue runs in the last pass.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../../include/nirnaya/pdhg.h"
+#include "devmat.hpp"
 #include "pdlp.hpp"
-#include "spmv.hpp"
 
 namespace nir {
 namespace gpu {
-
-struct HipError : std::runtime_error {
-    explicit HipError(const std::string& what) : std::runtime_error(what) {}
-};
-
-#define HIP_CHECK(expr)                                                                        \
-    do {                                                                                       \
-        hipError_t e_ = (expr);                                                                \
-        if (e_ != hipSuccess)                                                                  \
-            throw HipError(std::string(#expr) + " failed: " + hipGetErrorString(e_));         \
-    } while (0)
 
 // ------------------------------------------------------------------ kernels
 
@@ -55,59 +48,74 @@ struct RefState {
 };
 
 template <int TILE>
-__global__ __launch_bounds__(kThreads) void k_spmv_store(Csr<double> A, const double* __restrict__ v,
+__global__ __launch_bounds__(kThreads) void k_spmv_store(Pass<double> P, const double* __restrict__ v,
                                                          double* __restrict__ out) {
-    __shared__ double prod[TILE];
-    stream_rows<TILE>(A, v, prod, blockIdx.x, [&](int r, double s) { out[r] = s; });
+    __shared__ double prod[lds_slots(TILE)];
+    stream_rows<TILE>(P, v, prod, blockIdx.x, [&](int r, double s) { out[r] = s; });
 }
 
 template <int TILE>
-__global__ __launch_bounds__(kThreads) void k_primal_ref(Csr<double> AT, const double* __restrict__ y,
+__global__ __launch_bounds__(kThreads) void k_primal_ref(Pass<double> P, const double* __restrict__ y,
                                                          double* __restrict__ x, double* __restrict__ xbar,
                                                          const double* __restrict__ c,
                                                          const double* __restrict__ lo,
                                                          const double* __restrict__ up, double tau,
                                                          RefState* st, int k_lo, int check) {
-    __shared__ double prod[TILE];
+    __shared__ double prod[lds_slots(TILE)];
     if (st->done) return;
     const int k = st->base + k_lo;
-    // The reference tests its stop rule at the end of iteration k-1.
-    if (check && k > 21 && ((k - 1) & 31) == 0) {
-        const double V = __longlong_as_double(static_cast<long long>(st->vslot[(k - 1) & 1]));
-        if (V <= 1e-6) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
-                st->done = 1;
-                st->iters = k;
+    if (P.first) {
+        // The reference tests its stop rule at the end of iteration k-1.
+        if (check && k > 21 && ((k - 1) & 31) == 0) {
+            const double V = __longlong_as_double(static_cast<long long>(st->vslot[(k - 1) & 1]));
+            if (V <= 1e-6) {
+                if (blockIdx.x == 0 && threadIdx.x == 0) {
+                    st->done = 1;
+                    st->iters = k;
+                }
+                return;
             }
-            return;
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->vslot[k & 1] = 0ull;
-    stream_rows<TILE>(AT, y, prod, blockIdx.x, [&](int j, double g) {
-        const double xj = x[j];
-        double cand = xj - tau * (c[j] - g);
-        const double l = lo[j], u = up[j];
-        if (finite_bound(l)) cand = amax(l, cand);
-        if (finite_bound(u)) cand = amin(u, cand);
+    struct Ops {
+        double x, c, l, u;
+    };
+    const auto pre = [&](int j) {
+        Ops o{0.0, 0.0, 0.0, 0.0};
+        if (P.last) o = Ops{x[j], c[j], lo[j], up[j]};
+        return o;
+    };
+    stream_rows<TILE>(P, y, prod, blockIdx.x, pre, [&](int j, double g, const Ops& o) {
+        const double xj = o.x;
+        double cand = xj - tau * (o.c - g);
+        if (finite_bound(o.l)) cand = amax(o.l, cand);
+        if (finite_bound(o.u)) cand = amin(o.u, cand);
         xbar[j] = 2.0 * cand - xj;
         x[j] = cand;
     });
 }
 
 template <int TILE>
-__global__ __launch_bounds__(kThreads) void k_dual_ref(Csr<double> A, const double* __restrict__ xbar,
+__global__ __launch_bounds__(kThreads) void k_dual_ref(Pass<double> P, const double* __restrict__ xbar,
                                                        double* __restrict__ y, const double* __restrict__ rl,
                                                        const double* __restrict__ ru, double sigma,
-                                                       RefState* st, int k_lo) {
-    __shared__ double prod[TILE];
+                                                       const RefState* st, double* __restrict__ vpart) {
+    __shared__ double prod[lds_slots(TILE)];
     __shared__ double red[kWaves];
     if (st->done) return;
-    const int k = st->base + k_lo;
     double vmax = 0.0;
-    stream_rows<TILE>(A, xbar, prod, blockIdx.x, [&](int i, double h) {
-        const double yi = y[i];
+    struct Ops {
+        double y, l, u;
+    };
+    const auto pre = [&](int i) {
+        Ops o{0.0, 0.0, 0.0};
+        if (P.last) o = Ops{y[i], rl[i], ru[i]};
+        return o;
+    };
+    stream_rows<TILE>(P, xbar, prod, blockIdx.x, pre, [&](int i, double h, const Ops& o) {
+        const double yi = o.y;
         double r = h - yi / sigma;
-        const double l = rl[i], u = ru[i];
+        const double l = o.l, u = o.u;
         const bool lf = finite_bound(l), uf = finite_bound(u);
         if (lf) r = amax(l, r);
         if (uf) r = amin(u, r);
@@ -117,12 +125,29 @@ __global__ __launch_bounds__(kThreads) void k_dual_ref(Csr<double> A, const doub
         if (uf) viol = amax(viol, h - u);
         vmax = amax(vmax, viol);
     });
-    vmax = block_max(vmax, red);
-    if (threadIdx.x == 0 && vmax > 0.0)
-        atomicMax(&st->vslot[k & 1], static_cast<unsigned long long>(__double_as_longlong(vmax)));
+    if (P.last) {
+        // One plain store per block; the maximum over blocks is only formed on
+        // the iterations the stop rule reads it (k_vmax_reduce).
+        vmax = block_max(vmax, red);
+        if (threadIdx.x == 0) vpart[blockIdx.x] = vmax;
+    }
+}
+
+// Max row violation of iteration k = base + k_lo over the dual kernel's
+// per-block maxima, into slot k & 1 (one workgroup, fixed order).
+__global__ __launch_bounds__(kThreads) void k_vmax_reduce(const double* __restrict__ vpart, int nb, RefState* st,
+                                                          int k_lo) {
+    __shared__ double red[kWaves];
+    if (st->done) return;
+    const int k = st->base + k_lo;
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kThreads) v = amax(v, vpart[b]);
+    v = block_max(v, red);
+    if (threadIdx.x == 0) st->vslot[k & 1] = static_cast<unsigned long long>(__double_as_longlong(v));
 }
 
 __global__ void k_set_base(RefState* st, int base) { st->base = base; }
+__global__ void k_advance(RefState* st, int by) { st->base += by; }
 
 // Plain grid-stride copy: the calibration kernel for HBM byte counters.
 __global__ __launch_bounds__(256) void k_copy_calibrate(const double4* __restrict__ in, double4* __restrict__ out,
@@ -131,86 +156,68 @@ __global__ __launch_bounds__(256) void k_copy_calibrate(const double4* __restric
     for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
         out[i] = in[i];
 }
-__global__ void k_advance(RefState* st, int by) { st->base += by; }
-
-// ------------------------------------------------------------------ host helpers
-
-template <typename T>
-struct DevArray {
-    T* p = nullptr;
-    size_t n = 0;
-    DevArray() = default;
-    DevArray(const DevArray&) = delete;
-    DevArray& operator=(const DevArray&) = delete;
-    ~DevArray() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    void alloc(size_t count) {
-        release();
-        n = count;
-        HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(count, 1) * sizeof(T)));
-    }
-    void upload(const T* src, size_t count, hipStream_t s) {
-        if (count > n) throw std::runtime_error("upload overflow");
-        if (count) HIP_CHECK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
-    }
-    void download(T* dst, size_t count, hipStream_t s) const {
-        if (count > n) throw std::runtime_error("download overflow");
-        if (count) HIP_CHECK(hipMemcpyAsync(dst, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
-    }
-};
-
-// Row blocks for CSR-stream: <= tile_rows rows and <= tile_nnz nonzeros per
-// block, or one row longer than tile_nnz on its own. Always >= 1 block.
-std::vector<int> partition_rows(const std::vector<int>& rowptr, int tile_nnz, int tile_rows) {
-    const int n = static_cast<int>(rowptr.size()) - 1;
-    std::vector<int> blk{0};
-    int r = 0;
-    while (r < n) {
-        const int s = r;
-        if (rowptr[r + 1] - rowptr[r] > tile_nnz) {
-            ++r;
-        } else {
-            while (r < n && r - s < tile_rows && rowptr[r + 1] - rowptr[s] <= tile_nnz) ++r;
-        }
-        blk.push_back(r);
-    }
-    if (blk.size() == 1) blk.push_back(0);
-    return blk;
-}
-
-template <typename T>
-struct DevCsr {
-    int nrows = 0, ncols = 0, nnz = 0;
-    DevArray<int> rowptr, col, blk;
-    DevArray<T> val;
-    int nblk = 0;
-    Csr<T> view() const { return Csr<T>{nrows, ncols, rowptr.p, col.p, val.p, nblk, blk.p}; }
-};
-
-template <typename T>
-void upload_csr(DevCsr<T>& d, int nrows, int ncols, const std::vector<int>& rowptr, const int* col, const T* val,
-                int tile_nnz, int tile_rows, hipStream_t s) {
-    d.nrows = nrows;
-    d.ncols = ncols;
-    d.nnz = rowptr.back();
-    d.rowptr.alloc(rowptr.size());
-    d.rowptr.upload(rowptr.data(), rowptr.size(), s);
-    d.col.alloc(d.nnz);
-    d.col.upload(col, d.nnz, s);
-    d.val.alloc(d.nnz);
-    d.val.upload(val, d.nnz, s);
-    std::vector<int> blk = partition_rows(rowptr, tile_nnz, tile_rows);
-    d.nblk = static_cast<int>(blk.size()) - 1;
-    d.blk.alloc(blk.size());
-    d.blk.upload(blk.data(), blk.size(), s);
-    HIP_CHECK(hipStreamSynchronize(s));   // host vectors may go out of scope
-}
 
 bool host_finite(double b) { return std::isfinite(b) && std::fabs(b) < 1e20; }
+
+void spmv_store(const DevMatrix& M, const double* v, double* out, hipStream_t s) {
+    for (int p = 0; p < M.shape.nseg; ++p) {
+        const Pass<double> P = M.shape.pass(M.val.p, M.part.p, p);
+        TILE_DISPATCH(M.shape.shape.tile, k_spmv_store<TL><<<P.M.nblk, kThreads, 0, s>>>(P, v, out));
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+// Autotuner: time y = M v for every candidate tile shape on the real matrix
+// and keep the fastest. Results are cached per matrix signature; the choice
+// never changes results except for rows longer than a tile.
+TileShape autotune(int nrows, int ncols, const std::vector<int>& rowptr, const int* col, const double* val,
+                   hipStream_t s) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int>, TileShape> cache;
+    const auto key = std::make_tuple(nrows, ncols, rowptr.back());
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    std::vector<TileShape> cands;
+    for (int w : {0, 1 << 18, 1 << 17})
+        if (w == 0 || w < ncols)
+            for (int t : {2048, 4096})
+                for (int r : {128, 256}) cands.push_back(TileShape{t, r, w});
+    DevArray<double> v, out;
+    v.alloc(std::max(ncols, 1));
+    out.alloc(std::max(nrows, 1));
+    {
+        std::vector<double> ones(static_cast<size_t>(std::max(ncols, 1)), 1.0);
+        v.upload(ones.data(), ones.size(), s);
+    }
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    TileShape best = cands.front();
+    float best_ms = 1e30f;
+    for (const TileShape& c : cands) {
+        DevMatrix M;
+        upload_matrix(M, nrows, ncols, rowptr, col, val, c, s);
+        for (int w = 0; w < 2; ++w) spmv_store(M, v.p, out.p, s);
+        HIP_CHECK(hipEventRecord(e0, s));
+        for (int w = 0; w < 8; ++w) spmv_store(M, v.p, out.p, s);
+        HIP_CHECK(hipEventRecord(e1, s));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::lock_guard<std::mutex> lock(mu);
+    cache[key] = best;
+    return best;
+}
 
 }  // namespace gpu
 }  // namespace nir
@@ -233,15 +240,15 @@ struct nir_session {
     nir_params prm{};
     int m = 0, n = 0;
     long long nnz = 0;
-    int tile = 2048, tile_rows = 256;
     double obj_offset = 0.0;
     std::vector<double> h_cost;
 
-    // matrices
-    DevCsr<double> A, AT;
+    // matrices: CSR(A) for A xbar, CSR(A^T) = CSC(A) for A^T y
+    DevMatrix A, AT;
     // vectors
     DevArray<double> c, lo, up, rl, ru, x, xbar, y;
     DevArray<RefState> st;
+    DevArray<double> vpart;       // per-block max row violation of the last dual kernel
     RefState* h_poll = nullptr;   // pinned: two poll slots
 
     // progress
@@ -278,59 +285,46 @@ nir_session::~nir_session() {
 
 namespace {
 
-void launch_spmv_store(nir_session* S, const DevCsr<double>& M, const double* v, double* out) {
-    switch (S->tile) {
-        case 1024: k_spmv_store<1024><<<M.nblk, kThreads, 0, S->s>>>(M.view(), v, out); break;
-        case 4096: k_spmv_store<4096><<<M.nblk, kThreads, 0, S->s>>>(M.view(), v, out); break;
-        default: k_spmv_store<2048><<<M.nblk, kThreads, 0, S->s>>>(M.view(), v, out); break;
-    }
-    HIP_CHECK(hipGetLastError());
-}
-
 void launch_primal_ref(nir_session* S, int k_lo) {
     const int check = S->prm.check_termination ? 1 : 0;
-    auto AT = S->AT.view();
-    switch (S->tile) {
-        case 1024:
-            k_primal_ref<1024><<<AT.nblk, kThreads, 0, S->s>>>(AT, S->y.p, S->x.p, S->xbar.p, S->c.p, S->lo.p, S->up.p, S->tau, S->st.p, k_lo, check);
-            break;
-        case 4096:
-            k_primal_ref<4096><<<AT.nblk, kThreads, 0, S->s>>>(AT, S->y.p, S->x.p, S->xbar.p, S->c.p, S->lo.p, S->up.p, S->tau, S->st.p, k_lo, check);
-            break;
-        default:
-            k_primal_ref<2048><<<AT.nblk, kThreads, 0, S->s>>>(AT, S->y.p, S->x.p, S->xbar.p, S->c.p, S->lo.p, S->up.p, S->tau, S->st.p, k_lo, check);
-            break;
+    const DevMatrix& M = S->AT;
+    for (int p = 0; p < M.shape.nseg; ++p) {
+        const Pass<double> P = M.shape.pass(M.val.p, M.part.p, p);
+        TILE_DISPATCH(M.shape.shape.tile,
+                      k_primal_ref<TL><<<P.M.nblk, kThreads, 0, S->s>>>(P, S->y.p, S->x.p, S->xbar.p, S->c.p,
+                                                                         S->lo.p, S->up.p, S->tau, S->st.p, k_lo,
+                                                                         check));
     }
     HIP_CHECK(hipGetLastError());
 }
 
-void launch_dual_ref(nir_session* S, int k_lo) {
-    auto A = S->A.view();
-    switch (S->tile) {
-        case 1024:
-            k_dual_ref<1024><<<A.nblk, kThreads, 0, S->s>>>(A, S->xbar.p, S->y.p, S->rl.p, S->ru.p, S->sigma, S->st.p, k_lo);
-            break;
-        case 4096:
-            k_dual_ref<4096><<<A.nblk, kThreads, 0, S->s>>>(A, S->xbar.p, S->y.p, S->rl.p, S->ru.p, S->sigma, S->st.p, k_lo);
-            break;
-        default:
-            k_dual_ref<2048><<<A.nblk, kThreads, 0, S->s>>>(A, S->xbar.p, S->y.p, S->rl.p, S->ru.p, S->sigma, S->st.p, k_lo);
-            break;
+// k is the global iteration index (used only to place the violation reduction).
+void launch_dual_ref(nir_session* S, int k_lo, long long k) {
+    const DevMatrix& M = S->A;
+    for (int p = 0; p < M.shape.nseg; ++p) {
+        const Pass<double> P = M.shape.pass(M.val.p, M.part.p, p);
+        TILE_DISPATCH(M.shape.shape.tile,
+                      k_dual_ref<TL><<<P.M.nblk, kThreads, 0, S->s>>>(P, S->xbar.p, S->y.p, S->rl.p, S->ru.p,
+                                                                       S->sigma, S->st.p, S->vpart.p));
     }
+    // The stop rule reads the violation of iterations with k % 32 == 0.
+    if (S->prm.check_termination && (k % 32) == 0)
+        k_vmax_reduce<<<1, kThreads, 0, S->s>>>(S->vpart.p, M.shape.nblk[M.shape.nseg - 1], S->st.p, k_lo);
     HIP_CHECK(hipGetLastError());
 }
 
 void enqueue_ref_iteration(nir_session* S, long long k) {
     const int k_lo = static_cast<int>(k - S->base_host);
     launch_primal_ref(S, k_lo);
-    launch_dual_ref(S, k_lo);
+    launch_dual_ref(S, k_lo, k);
 }
 
+// One graph replays 32 iterations starting at a multiple of 32.
 void build_ref_graph(nir_session* S) {
     HIP_CHECK(hipStreamBeginCapture(S->s, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < 32; ++i) {
         launch_primal_ref(S, i);
-        launch_dual_ref(S, i);
+        launch_dual_ref(S, i, i);
     }
     k_advance<<<1, 1, 0, S->s>>>(S->st.p, 32);
     HIP_CHECK(hipStreamEndCapture(S->s, &S->graph));
@@ -419,8 +413,8 @@ void reference_step_sizes(nir_session* S) {
     du.upload(u.data(), u.size(), S->s);
     double eta = 1.0;
     for (int p = 0; p < 12; ++p) {
-        launch_spmv_store(S, S->A, du.p, dv.p);
-        launch_spmv_store(S, S->AT, dv.p, du.p);
+        spmv_store(S->A, du.p, dv.p, S->s);
+        spmv_store(S->AT, dv.p, du.p, S->s);
         du.download(u.data(), u.size(), S->s);
         HIP_CHECK(hipStreamSynchronize(S->s));
         double n2 = 0.0;
@@ -450,6 +444,16 @@ void check_lp(const nir_lp* lp) {
     // Every index is range-checked here: the kernels trust them.
     for (int64_t p = 0; p < nnz; ++p)
         if (lp->row_index[p] < 0 || lp->row_index[p] >= lp->nrow) throw std::invalid_argument("row index out of range");
+}
+
+TileShape shape_from_params(const nir_params& p) {
+    TileShape t;
+    if (p.tile_nnz == 1024 || p.tile_nnz == 2048 || p.tile_nnz == 4096) t.tile = p.tile_nnz;
+    else if (p.tile_nnz != 0 && p.tile_nnz != -1) throw std::invalid_argument("tile_nnz must be -1, 0, 1024, 2048 or 4096");
+    if (p.tile_rows > 0) t.rows = std::min(p.tile_rows, kThreads);
+    if (p.seg_width < 0) throw std::invalid_argument("seg_width must be >= 0");
+    t.seg_width = p.seg_width;
+    return t;
 }
 
 }  // namespace
@@ -524,9 +528,7 @@ NIR_API int nir_session_create(nir_ctx* ctx, const nir_lp* lp, const nir_params*
         S->nnz = lp->ncol > 0 ? lp->col_start[lp->ncol] : 0;
         S->obj_offset = lp->obj_offset;
         S->max_iters = std::max<long long>(0, S->prm.max_iters);
-        if (S->prm.tile_nnz == 1024 || S->prm.tile_nnz == 2048 || S->prm.tile_nnz == 4096) S->tile = S->prm.tile_nnz;
-        else if (S->prm.tile_nnz != 0) throw std::invalid_argument("tile_nnz must be 0, 1024, 2048 or 4096");
-        if (S->prm.tile_rows > 0) S->tile_rows = std::min(S->prm.tile_rows, kThreads);
+        const TileShape base = shape_from_params(S->prm);
 
         // CSR(A^T) is the CSC of A; CSR(A) by a counting transpose that keeps
         // column indices ascending within each row.
@@ -537,7 +539,7 @@ NIR_API int nir_session_create(nir_ctx* ctx, const nir_lp* lp, const nir_params*
         if (n == 0) at_ptr.assign(1, 0);
         std::vector<int> a_ptr(static_cast<size_t>(m) + 1, 0), a_col(static_cast<size_t>(nnz));
         std::vector<double> a_val(static_cast<size_t>(nnz));
-        for (int p2 = 0; p2 < nnz; ++p2) ++a_ptr[lp->row_index[p2] + 1];
+        for (int q = 0; q < nnz; ++q) ++a_ptr[lp->row_index[q] + 1];
         for (int i = 0; i < m; ++i) a_ptr[i + 1] += a_ptr[i];
         {
             std::vector<int> fill(a_ptr.begin(), a_ptr.end() - 1);
@@ -548,13 +550,15 @@ NIR_API int nir_session_create(nir_ctx* ctx, const nir_lp* lp, const nir_params*
                     a_val[r] = lp->value[q];
                 }
         }
-        upload_csr(S->A, m, n, a_ptr, a_col.data(), a_val.data(), S->tile, S->tile_rows, S->s);
-        upload_csr(S->AT, n, m, at_ptr, lp->row_index, lp->value, S->tile, S->tile_rows, S->s);
+        const TileShape ta = S->prm.tile_nnz == -1 ? autotune(m, n, a_ptr, a_col.data(), a_val.data(), S->s) : base;
+        const TileShape tt = S->prm.tile_nnz == -1 ? autotune(n, m, at_ptr, lp->row_index, lp->value, S->s) : base;
+        upload_matrix(S->A, m, n, a_ptr, a_col.data(), a_val.data(), ta, S->s);
+        upload_matrix(S->AT, n, m, at_ptr, lp->row_index, lp->value, tt, S->s);
 
         S->h_cost.assign(lp->cost, lp->cost + n);
         if (S->prm.mode == NIR_MODE_PDLP) {
             S->pdlp.reset(pdlp_create());
-            pdlp_setup(*S->pdlp, S->s, lp, S->prm, S->A.view(), S->AT.view(), S->tile, S->tile_rows);
+            pdlp_setup(*S->pdlp, S->s, lp, S->prm, &S->A, &S->AT);
         } else {
             S->c.alloc(n); S->c.upload(lp->cost, n, S->s);
             S->lo.alloc(n); S->lo.upload(lp->collo, n, S->s);
@@ -574,6 +578,11 @@ NIR_API int nir_session_create(nir_ctx* ctx, const nir_lp* lp, const nir_params*
             S->x.alloc(n); S->x.upload(x0.data(), n, S->s);
             S->xbar.alloc(n); S->xbar.upload(x0.data(), n, S->s);
             S->y.alloc(m); S->y.upload(y0.data(), m, S->s);
+            S->vpart.alloc(static_cast<size_t>(S->A.shape.nblk[S->A.shape.nseg - 1]));
+            {
+                std::vector<double> zeros(S->vpart.n, 0.0);
+                S->vpart.upload(zeros.data(), zeros.size(), S->s);
+            }
             S->st.alloc(1);
             RefState init{};
             S->st.upload(&init, 1, S->s);
@@ -619,8 +628,8 @@ NIR_API int nir_session_result(nir_session* S, double* x, double* y, nir_info* i
         std::vector<double> hx(static_cast<size_t>(S->n));
         S->x.download(hx.data(), hx.size(), S->s);
         if (y) S->y.download(y, static_cast<size_t>(S->m), S->s);
-        RefState h{};
-        S->st.download(&h, 1, S->s);
+        std::vector<double> vp(S->vpart.n);
+        S->vpart.download(vp.data(), vp.size(), S->s);
         HIP_CHECK(hipStreamSynchronize(S->s));
         if (x) std::copy(hx.begin(), hx.end(), x);
         if (info) {
@@ -628,15 +637,16 @@ NIR_API int nir_session_result(nir_session* S, double* x, double* y, nir_info* i
             info->status = S->status;
             info->iterations = S->iters_done;
             // The reference reports c^T x + offset of the final x in double-double.
-            nir::gpu::DDHost acc(S->obj_offset);
+            DDHost acc(S->obj_offset);
             for (int j = 0; j < S->n; ++j)
                 if (S->h_cost[j] != 0.0 && hx[j] != 0.0) acc.add_prod(S->h_cost[j], hx[j]);
             info->objective = acc.value();
             info->tau = S->tau;
             info->sigma = S->sigma;
             info->primal_weight = 1.0;
-            const long long last = S->iters_done - 1;
-            info->max_primal_viol = last >= 0 ? __builtin_bit_cast(double, h.vslot[last & 1]) : 0.0;
+            double vmax = 0.0;   // of the last iteration's A xbar
+            for (double v : vp) vmax = std::max(vmax, v);
+            info->max_primal_viol = S->iters_done > 0 ? vmax : 0.0;
             info->setup_seconds = S->setup_seconds;
             info->solve_seconds = S->solve_seconds;
         }
@@ -670,7 +680,7 @@ NIR_API int nir_session_time(nir_session* S, int64_t iters, nir_timing* out) {
             launch_primal_ref(S, k_lo);
             HIP_CHECK(hipEventRecord(ev[3 + 4 * t], S->s));
             HIP_CHECK(hipEventRecord(ev[4 + 4 * t], S->s));
-            launch_dual_ref(S, k_lo);
+            launch_dual_ref(S, k_lo, k);
             HIP_CHECK(hipEventRecord(ev[5 + 4 * t], S->s));
         }
         HIP_CHECK(hipEventRecord(ev[1], S->s));
@@ -686,6 +696,8 @@ NIR_API int nir_session_time(nir_session* S, int64_t iters, nir_timing* out) {
         }
         for (auto& e : ev) HIP_CHECK(hipEventDestroy(e));
         out->iterations = iters;
+        out->spmv_passes = S->A.shape.nseg;
+        out->spmtv_passes = S->AT.shape.nseg;
         S->k_next = k;
         finish_ref(S);
         return 0;
@@ -693,6 +705,17 @@ NIR_API int nir_session_time(nir_session* S, int64_t iters, nir_timing* out) {
         S->ctx->error = ex.what();
         return -1;
     }
+}
+
+NIR_API int nir_session_shape(nir_session* S, int32_t* out6) {
+    if (!S || !out6) return -1;
+    out6[0] = S->A.shape.shape.tile;
+    out6[1] = S->A.shape.shape.rows;
+    out6[2] = S->A.shape.nseg;
+    out6[3] = S->AT.shape.shape.tile;
+    out6[4] = S->AT.shape.shape.rows;
+    out6[5] = S->AT.shape.nseg;
+    return 0;
 }
 
 NIR_API int nir_calibrate_copy(nir_ctx* ctx, int64_t bytes, double* ms) {

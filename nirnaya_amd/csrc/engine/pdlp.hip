@@ -29,8 +29,9 @@
 //   * Mixed precision: FP32 matrix values, iterates and accumulation in the
 //     iteration kernels; FP64 matrix and arithmetic for every evaluation,
 //     restart and step decision; then an FP64 tail from the FP32 point (restart
-//     at it) down to the polish tolerance. An FP32 phase that stops improving
-//     hands over to FP64 early.
+//     at it) down to the polish tolerance. An FP32 phase whose relative KKT
+//     has not improved by 5% for max(100, evaluations/3) evaluations hands
+//     over to FP64 early.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,6 +43,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "devmat.hpp"
 #include "pdlp.hpp"
 
 namespace nir {
@@ -130,7 +132,7 @@ struct PdlpState {
     int restart;           // set by the decision kernel for the copy kernel
     int restarts;
     int evals;
-    int since_best;        // evaluations since best_rel halved
+    int since_best;        // evaluations since best_rel last improved by 5%
     int watch_stall;       // 1 in the FP32 phase
     int eval_freq;
 };
@@ -138,48 +140,64 @@ struct PdlpState {
 // ------------------------------------------------------------------ kernels
 
 template <int TILE, typename T>
-__global__ __launch_bounds__(kThreads) void k_primal_pdlp(Csr<T> AT, const T* __restrict__ y, T* __restrict__ x,
+__global__ __launch_bounds__(kThreads) void k_primal_pdlp(Pass<T> P, const T* __restrict__ y, T* __restrict__ x,
                                                           T* __restrict__ xbar, const T* __restrict__ xa,
                                                           T* __restrict__ xp_out, const T* __restrict__ c,
                                                           const T* __restrict__ lo, const T* __restrict__ up,
                                                           const PdlpState* __restrict__ st, int k_lo, int save) {
-    __shared__ T prod[TILE];
+    __shared__ T prod[lds_slots(TILE)];
     if (st->done) return;
     const T tau = static_cast<T>(st->tau);
     const int k = st->k_base + k_lo;
     const T w = static_cast<T>(static_cast<double>(k + 1) / static_cast<double>(k + 2));
     const T wc = static_cast<T>(1.0 / static_cast<double>(k + 2));
-    stream_rows<TILE>(AT, y, prod, blockIdx.x, [&](int j, T g) {
-        const T xj = x[j];
-        T cand = xj - tau * (c[j] - g);
-        cand = (lo[j] < cand) ? cand : lo[j];
-        cand = (cand < up[j]) ? cand : up[j];
+    struct Ops {
+        T x, c, l, u, a;
+    };
+    const auto pre = [&](int j) {
+        Ops o{T(0), T(0), T(0), T(0), T(0)};
+        if (P.last) o = Ops{x[j], c[j], lo[j], up[j], xa[j]};
+        return o;
+    };
+    stream_rows<TILE>(P, y, prod, blockIdx.x, pre, [&](int j, T g, const Ops& o) {
+        const T xj = o.x;
+        T cand = xj - tau * (o.c - g);
+        cand = (o.l < cand) ? cand : o.l;
+        cand = (cand < o.u) ? cand : o.u;
         const T xb = T(2) * cand - xj;
         xbar[j] = xb;
-        x[j] = w * xb + wc * xa[j];
+        x[j] = w * xb + wc * o.a;
         if (save) xp_out[j] = cand;
     });
 }
 
 template <int TILE, typename T>
-__global__ __launch_bounds__(kThreads) void k_dual_pdlp(Csr<T> A, const T* __restrict__ xbar, T* __restrict__ y,
+__global__ __launch_bounds__(kThreads) void k_dual_pdlp(Pass<T> P, const T* __restrict__ xbar, T* __restrict__ y,
                                                         const T* __restrict__ ya, T* __restrict__ yp_out,
                                                         const T* __restrict__ rl, const T* __restrict__ ru,
                                                         const PdlpState* __restrict__ st, int k_lo, int save) {
-    __shared__ T prod[TILE];
+    __shared__ T prod[lds_slots(TILE)];
     if (st->done) return;
     const T sigma = static_cast<T>(st->sigma);
     const int k = st->k_base + k_lo;
     const T w = static_cast<T>(static_cast<double>(k + 1) / static_cast<double>(k + 2));
     const T wc = static_cast<T>(1.0 / static_cast<double>(k + 2));
-    stream_rows<TILE>(A, xbar, prod, blockIdx.x, [&](int i, T h) {
-        const T yi = y[i];
+    struct Ops {
+        T y, l, u, a;
+    };
+    const auto pre = [&](int i) {
+        Ops o{T(0), T(0), T(0), T(0)};
+        if (P.last) o = Ops{y[i], rl[i], ru[i], ya[i]};
+        return o;
+    };
+    stream_rows<TILE>(P, xbar, prod, blockIdx.x, pre, [&](int i, T h, const Ops& o) {
+        const T yi = o.y;
         T r = h - yi / sigma;
-        r = (rl[i] < r) ? r : rl[i];
-        r = (r < ru[i]) ? r : ru[i];
+        r = (o.l < r) ? r : o.l;
+        r = (r < o.u) ? r : o.u;
         const T yp = yi + sigma * (r - h);
         const T yb = T(2) * yp - yi;
-        y[i] = w * yb + wc * ya[i];
+        y[i] = w * yb + wc * o.a;
         if (save) yp_out[i] = yp;
     });
 }
@@ -189,18 +207,18 @@ __global__ __launch_bounds__(kThreads) void k_dual_pdlp(Csr<T> A, const T* __res
 // ||xp - x_anchor||^2. Quantities: 0 rd^2, 1 (rd/C)^2, 2 c^T x, 3 dual-obj
 // column terms, 4 ||dx||^2.
 template <int TILE, typename TI>
-__global__ __launch_bounds__(kThreads) void k_eval_cols(Csr<double> AT, const TI* __restrict__ yp,
+__global__ __launch_bounds__(kThreads) void k_eval_cols(Pass<double> P, const TI* __restrict__ yp,
                                                         const TI* __restrict__ xp, const TI* __restrict__ xa,
                                                         const double* __restrict__ c, const double* __restrict__ lo,
                                                         const double* __restrict__ up,
                                                         const double* __restrict__ cscale,
                                                         const PdlpState* __restrict__ st, dd* __restrict__ part,
                                                         int force) {
-    __shared__ double prod[TILE];
+    __shared__ double prod[lds_slots(TILE)];
     __shared__ dd sh[5 * kThreads];
     if (st->done && !force) return;
     dd acc[5] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
-    stream_rows<TILE>(AT, yp, prod, blockIdx.x, [&](int j, double g) {
+    stream_rows<TILE>(P, yp, prod, blockIdx.x, [&](int j, double g) {
         const double lam = c[j] - g;
         const bool lf = lo[j] > -INFINITY, uf = up[j] < INFINITY;
         double lp = lam;
@@ -219,24 +237,24 @@ __global__ __launch_bounds__(kThreads) void k_eval_cols(Csr<double> AT, const TI
         const double dx = xj - static_cast<double>(xa[j]);
         acc[4] = dd_prod(dx, dx);
     });
-    block_dd_reduce<5>(acc, sh, part + 5 * blockIdx.x);
+    if (P.last) block_dd_reduce<5>(acc, sh, part + 5 * blockIdx.x);
 }
 
 // Row evaluation: r_p = A xp - clamp(A xp), sign violations of yp, the row
 // part of the dual objective and ||yp - y_anchor||^2. Quantities: 0 rp^2,
 // 1 (rp/R)^2, 2 yviol^2, 3 (R yviol)^2, 4 dual-obj row terms, 5 ||dy||^2.
 template <int TILE, typename TI>
-__global__ __launch_bounds__(kThreads) void k_eval_rows(Csr<double> A, const TI* __restrict__ xp,
+__global__ __launch_bounds__(kThreads) void k_eval_rows(Pass<double> P, const TI* __restrict__ xp,
                                                         const TI* __restrict__ yp, const TI* __restrict__ ya,
                                                         const double* __restrict__ rl, const double* __restrict__ ru,
                                                         const double* __restrict__ rscale,
                                                         const PdlpState* __restrict__ st, dd* __restrict__ part,
                                                         int force) {
-    __shared__ double prod[TILE];
+    __shared__ double prod[lds_slots(TILE)];
     __shared__ dd sh[6 * kThreads];
     if (st->done && !force) return;
     dd acc[6] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
-    stream_rows<TILE>(A, xp, prod, blockIdx.x, [&](int i, double h) {
+    stream_rows<TILE>(P, xp, prod, blockIdx.x, [&](int i, double h) {
         const bool lf = rl[i] > -INFINITY, uf = ru[i] < INFINITY;
         double proj = amax(rl[i], h);
         proj = amin(ru[i], proj);
@@ -259,7 +277,7 @@ __global__ __launch_bounds__(kThreads) void k_eval_rows(Csr<double> A, const TI*
         const double dy = yi - static_cast<double>(ya[i]);
         acc[5] = dd_prod(dy, dy);
     });
-    block_dd_reduce<6>(acc, sh, part + 6 * blockIdx.x);
+    if (P.last) block_dd_reduce<6>(acc, sh, part + 6 * blockIdx.x);
 }
 
 // Decision kernel (one workgroup): reduce the partials, evaluate KKT,
@@ -325,10 +343,12 @@ __global__ __launch_bounds__(kThreads) void k_decide(PdlpState* st, const dd* __
         return;
     }
     if (s.watch_stall) {
-        if (worst <= 0.5 * s.best_rel) {
+        // FP32 phase: hand over to FP64 once the best relative KKT has not
+        // improved by 5% for max(100, evals/3) evaluations.
+        if (worst <= 0.95 * s.best_rel) {
             s.best_rel = worst;
             s.since_best = 0;
-        } else if (++s.since_best >= 40) {
+        } else if (++s.since_best >= (s.evals / 3 > 100 ? s.evals / 3 : 100)) {
             s.done = 1;
             s.status = 2;
             s.restart = 0;
@@ -385,20 +405,25 @@ __global__ void k_restart_copy(const PdlpState* __restrict__ st, int n, int m, c
 
 // Plain y = M v (power iteration).
 template <int TILE>
-__global__ __launch_bounds__(kThreads) void k_spmv_plain(Csr<double> M, const double* __restrict__ v,
+__global__ __launch_bounds__(kThreads) void k_spmv_plain(Pass<double> P, const double* __restrict__ v,
                                                          double* __restrict__ out) {
-    __shared__ double prod[TILE];
-    stream_rows<TILE>(M, v, prod, blockIdx.x, [&](int r, double s) { out[r] = s; });
+    __shared__ double prod[lds_slots(TILE)];
+    stream_rows<TILE>(P, v, prod, blockIdx.x, [&](int r, double s) { out[r] = s; });
 }
 
-// Row measures over a CSR matrix: max |a| (what = 0) or sum |a| (what = 1).
-__global__ void k_row_measure(Csr<double> M, double* out, int what) {
+// Row measures over a segmented CSR matrix, segments in column order: max |a|
+// (what = 0) or sum |a| (what = 1).
+__global__ void k_row_measure(const int* __restrict__ rowptr, int nseg, int nrows, const double* __restrict__ val,
+                              double* out, int what) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= M.nrows) return;
+    if (r >= nrows) return;
     double acc = 0.0;
-    for (int p = M.rowptr[r]; p < M.rowptr[r + 1]; ++p) {
-        const double a = fabs(M.val[p]);
-        acc = what == 0 ? amax(acc, a) : acc + a;
+    for (int s = 0; s < nseg; ++s) {
+        const int* rp = rowptr + static_cast<size_t>(s) * (nrows + 1);
+        for (int p = rp[r]; p < rp[r + 1]; ++p) {
+            const double a = fabs(val[p]);
+            acc = what == 0 ? amax(acc, a) : acc + a;
+        }
     }
     out[r] = acc;
 }
@@ -411,12 +436,16 @@ __global__ void k_factor(const double* meas, double* f, double* total, int n) {
     total[r] *= g;
 }
 
-// val[p] *= rf[row] * cf[col[p]] for a CSR matrix (all factors powers of two).
-__global__ void k_scale_csr(Csr<double> M, double* val, const double* rf, const double* cf) {
+// val[p] *= rf[row] * cf[col[p]] over every segment (all factors powers of two).
+__global__ void k_scale_csr(const int* __restrict__ rowptr, int nseg, int nrows, const int* __restrict__ col,
+                            double* val, const double* rf, const double* cf) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= M.nrows) return;
+    if (r >= nrows) return;
     const double a = rf[r];
-    for (int p = M.rowptr[r]; p < M.rowptr[r + 1]; ++p) val[p] = val[p] * (a * cf[M.col[p]]);
+    for (int s = 0; s < nseg; ++s) {
+        const int* rp = rowptr + static_cast<size_t>(s) * (nrows + 1);
+        for (int p = rp[r]; p < rp[r + 1]; ++p) val[p] = val[p] * (a * cf[col[p]]);
+    }
 }
 
 __global__ void k_to_float(const double* in, float* out, long long n) {
@@ -481,11 +510,17 @@ struct Iterates {
     }
 };
 
+// FP32 copy of a matrix's values (same segmented structure as the FP64 one).
+struct Values32 {
+    Buf<float> val, part;
+};
+
 struct Pdlp {
-    int n = 0, m = 0, tile = 2048;
+    int n = 0, m = 0;
     long long nnz = 0;
-    Csr<double> A{}, AT{};     // FP64, scaled in place
-    Buf<float> a32, at32;      // FP32 copies of the scaled values
+    DevMatrix* A = nullptr;    // FP64, scaled in place (owned by the session)
+    DevMatrix* AT = nullptr;
+    Values32 a32, at32;        // FP32 copies of the scaled values (mixed mode)
     Buf<double> R, C;          // engine scaling (powers of two)
     Buf<double> Rtot, Ctot;    // engine x caller scaling, for KKT in the caller's units
     Iterates<double> d;
@@ -530,42 +565,70 @@ namespace {
 
 int grid_for(long long n) { return static_cast<int>(std::max<long long>(1, std::min<long long>((n + 255) / 256, 8192))); }
 
+// Pass `s` of a matrix product in precision T.
 template <typename T>
-Csr<T> with_vals(Csr<double> M, const T* v) {
-    return Csr<T>{M.nrows, M.ncols, M.rowptr, M.col, v, M.nblk, M.blk};
+Pass<T> pass_of(const DevMatrix& M, const Values32& v32, int s);
+template <>
+Pass<double> pass_of<double>(const DevMatrix& M, const Values32&, int s) {
+    return M.shape.pass(M.val.p, M.part.p, s);
+}
+template <>
+Pass<float> pass_of<float>(const DevMatrix& M, const Values32& v32, int s) {
+    return M.shape.pass(static_cast<const float*>(v32.val.p), v32.part.p, s);
 }
 
 template <typename T>
-void launch_iter(Pdlp& P, hipStream_t s, Iterates<T>& it, Csr<T> A, Csr<T> AT, int k_lo, int save) {
-#define PDLP_ITER(TL)                                                                                              \
-    k_primal_pdlp<TL, T><<<AT.nblk, kThreads, 0, s>>>(AT, it.y.p, it.x.p, it.xbar.p, it.xa.p, it.xp.p, it.c.p,      \
-                                                      it.lo.p, it.up.p, P.st.p, k_lo, save);                       \
-    k_dual_pdlp<TL, T><<<A.nblk, kThreads, 0, s>>>(A, it.xbar.p, it.y.p, it.ya.p, it.yp.p, it.rl.p, it.ru.p, P.st.p, \
-                                                   k_lo, save);
-    switch (P.tile) {
-        case 1024: PDLP_ITER(1024); break;
-        case 4096: PDLP_ITER(4096); break;
-        default: PDLP_ITER(2048); break;
+void launch_primal(Pdlp& P, hipStream_t s, Iterates<T>& it, int k_lo, int save) {
+    const DevMatrix& M = *P.AT;
+    for (int q = 0; q < M.shape.nseg; ++q) {
+        const Pass<T> ps = pass_of<T>(M, P.at32, q);
+        TILE_DISPATCH(M.shape.shape.tile,
+                      k_primal_pdlp<TL, T><<<ps.M.nblk, kThreads, 0, s>>>(ps, it.y.p, it.x.p, it.xbar.p, it.xa.p,
+                                                                          it.xp.p, it.c.p, it.lo.p, it.up.p, P.st.p,
+                                                                          k_lo, save));
     }
-#undef PDLP_ITER
+}
+
+template <typename T>
+void launch_dual(Pdlp& P, hipStream_t s, Iterates<T>& it, int k_lo, int save) {
+    const DevMatrix& M = *P.A;
+    for (int q = 0; q < M.shape.nseg; ++q) {
+        const Pass<T> ps = pass_of<T>(M, P.a32, q);
+        TILE_DISPATCH(M.shape.shape.tile,
+                      k_dual_pdlp<TL, T><<<ps.M.nblk, kThreads, 0, s>>>(ps, it.xbar.p, it.y.p, it.ya.p, it.yp.p,
+                                                                        it.rl.p, it.ru.p, P.st.p, k_lo, save));
+    }
+}
+
+template <typename T>
+void launch_iter(Pdlp& P, hipStream_t s, Iterates<T>& it, int k_lo, int save) {
+    launch_primal<T>(P, s, it, k_lo, save);
+    launch_dual<T>(P, s, it, k_lo, save);
     PD_CHECK(hipGetLastError());
 }
 
 template <typename TI>
 void launch_eval(Pdlp& P, hipStream_t s, Iterates<TI>& it, int mode) {
     const int force = mode == 1 ? 1 : 0;
-#define PDLP_EVAL(TL)                                                                                              \
-    k_eval_cols<TL, TI><<<P.AT.nblk, kThreads, 0, s>>>(P.AT, it.yp.p, it.xp.p, it.xa.p, P.d.c.p, P.d.lo.p, P.d.up.p, \
-                                                       P.Ctot.p, P.st.p, P.part_c.p, force);                        \
-    k_eval_rows<TL, TI><<<P.A.nblk, kThreads, 0, s>>>(P.A, it.xp.p, it.yp.p, it.ya.p, P.d.rl.p, P.d.ru.p, P.Rtot.p,  \
-                                                      P.st.p, P.part_r.p, force);
-    switch (P.tile) {
-        case 1024: PDLP_EVAL(1024); break;
-        case 4096: PDLP_EVAL(4096); break;
-        default: PDLP_EVAL(2048); break;
+    const DevMatrix& AT = *P.AT;
+    const DevMatrix& A = *P.A;
+    for (int q = 0; q < AT.shape.nseg; ++q) {
+        const Pass<double> ps = AT.shape.pass(AT.val.p, AT.part.p, q);
+        TILE_DISPATCH(AT.shape.shape.tile,
+                      k_eval_cols<TL, TI><<<ps.M.nblk, kThreads, 0, s>>>(ps, it.yp.p, it.xp.p, it.xa.p, P.d.c.p,
+                                                                         P.d.lo.p, P.d.up.p, P.Ctot.p, P.st.p,
+                                                                         P.part_c.p, force));
     }
-#undef PDLP_EVAL
-    k_decide<<<1, kThreads, 0, s>>>(P.st.p, P.part_c.p, P.AT.nblk, P.part_r.p, P.A.nblk, mode);
+    for (int q = 0; q < A.shape.nseg; ++q) {
+        const Pass<double> ps = A.shape.pass(A.val.p, A.part.p, q);
+        TILE_DISPATCH(A.shape.shape.tile,
+                      k_eval_rows<TL, TI><<<ps.M.nblk, kThreads, 0, s>>>(ps, it.xp.p, it.yp.p, it.ya.p, P.d.rl.p,
+                                                                         P.d.ru.p, P.Rtot.p, P.st.p, P.part_r.p,
+                                                                         force));
+    }
+    const int nbc = AT.shape.nblk[AT.shape.nseg - 1];
+    const int nbr = A.shape.nblk[A.shape.nseg - 1];
+    k_decide<<<1, kThreads, 0, s>>>(P.st.p, P.part_c.p, nbc, P.part_r.p, nbr, mode);
     if (mode == 0) {
         k_restart_copy<TI><<<grid_for(std::max(P.n, P.m)), 256, 0, s>>>(P.st.p, P.n, P.m, it.xp.p, it.yp.p, it.x.p,
                                                                           it.xa.p, it.y.p, it.ya.p, 0);
@@ -574,8 +637,8 @@ void launch_eval(Pdlp& P, hipStream_t s, Iterates<TI>& it, int mode) {
 }
 
 template <typename T>
-void enqueue_block(Pdlp& P, hipStream_t s, Iterates<T>& it, Csr<T> A, Csr<T> AT) {
-    for (int kl = 0; kl < P.eval_freq; ++kl) launch_iter<T>(P, s, it, A, AT, kl, kl == P.eval_freq - 1 ? 1 : 0);
+void enqueue_block(Pdlp& P, hipStream_t s, Iterates<T>& it) {
+    for (int kl = 0; kl < P.eval_freq; ++kl) launch_iter<T>(P, s, it, kl, kl == P.eval_freq - 1 ? 1 : 0);
     launch_eval<T>(P, s, it, 0);
 }
 
@@ -587,16 +650,16 @@ void enqueue(Pdlp& P, hipStream_t s) {
             P.gexec = nullptr;
             P.graph = nullptr;
             PD_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            if (P.mixed) enqueue_block<float>(P, s, P.f, with_vals(P.A, P.a32.p), with_vals(P.AT, P.at32.p));
-            else enqueue_block<double>(P, s, P.d, P.A, P.AT);
+            if (P.mixed) enqueue_block<float>(P, s, P.f);
+            else enqueue_block<double>(P, s, P.d);
             PD_CHECK(hipStreamEndCapture(s, &P.graph));
             PD_CHECK(hipGraphInstantiate(&P.gexec, P.graph, nullptr, nullptr, 0));
             P.graph_mixed = P.mixed;
         }
         PD_CHECK(hipGraphLaunch(P.gexec, s));
     } else {
-        if (P.mixed) enqueue_block<float>(P, s, P.f, with_vals(P.A, P.a32.p), with_vals(P.AT, P.at32.p));
-        else enqueue_block<double>(P, s, P.d, P.A, P.AT);
+        if (P.mixed) enqueue_block<float>(P, s, P.f);
+        else enqueue_block<double>(P, s, P.d);
     }
 }
 
@@ -647,17 +710,23 @@ void switch_to_fp64(Pdlp& P, hipStream_t s, double new_tol) {
     PD_CHECK(hipStreamSynchronize(s));
 }
 
+void spmv_plain(const DevMatrix& M, const double* in, double* out, hipStream_t s) {
+    for (int q = 0; q < M.shape.nseg; ++q) {
+        const Pass<double> ps = M.shape.pass(M.val.p, M.part.p, q);
+        TILE_DISPATCH(M.shape.shape.tile, k_spmv_plain<TL><<<ps.M.nblk, kThreads, 0, s>>>(ps, in, out));
+    }
+    PD_CHECK(hipGetLastError());
+}
+
 }  // namespace
 
-void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm, Csr<double> A, Csr<double> AT,
-                int tile, int tile_rows) {
+void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm, DevMatrix* Am, DevMatrix* ATm) {
     auto t0 = std::chrono::steady_clock::now();
     P.n = lp->ncol;
     P.m = lp->nrow;
     P.nnz = P.n > 0 ? lp->col_start[P.n] : 0;
-    P.A = A;
-    P.AT = AT;
-    P.tile = tile;
+    P.A = Am;
+    P.AT = ATm;
     P.eval_freq = std::max(1, prm.eval_freq);
     P.use_graph = prm.use_graph != 0;
     P.tol = prm.tol > 0 ? prm.tol : 1e-6;
@@ -672,31 +741,34 @@ void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm,
     // ---- exponent-domain Ruiz (10 passes) + Pock-Chambolle, on the device
     P.R.alloc(m);
     P.C.alloc(n);
-    {
-        std::vector<double> ones_m(static_cast<size_t>(m), 1.0), ones_n(static_cast<size_t>(n), 1.0);
-        P.R.up(ones_m.data(), m, s);
-        P.C.up(ones_n.data(), n, s);
-    }
     Buf<double> mr, mc, fr, fc, onesr, onesc;
     mr.alloc(m); mc.alloc(n); fr.alloc(m); fc.alloc(n); onesr.alloc(m); onesc.alloc(n);
     {
         std::vector<double> ones_m(static_cast<size_t>(m), 1.0), ones_n(static_cast<size_t>(n), 1.0);
+        P.R.up(ones_m.data(), m, s);
+        P.C.up(ones_n.data(), n, s);
         onesr.up(ones_m.data(), m, s);
         onesc.up(ones_n.data(), n, s);
     }
-    double* a_val = const_cast<double*>(A.val);
-    double* at_val = const_cast<double*>(AT.val);
+    const CsrShape& SA = Am->shape;
+    const CsrShape& ST = ATm->shape;
+    double* a_val = Am->val.p;
+    double* at_val = ATm->val.p;
     const int gm = (m + 255) / 256, gn = (n + 255) / 256;
     for (int pass = 0; pass < 11; ++pass) {
         const int what = pass < 10 ? 0 : 1;   // 10 Ruiz (max |a|) passes, then one Pock-Chambolle (sum |a|)
-        if (m > 0) k_row_measure<<<gm, 256, 0, s>>>(A, mr.p, what);
-        if (m > 0) k_factor<<<gm, 256, 0, s>>>(mr.p, fr.p, P.R.p, m);
-        if (m > 0) k_scale_csr<<<gm, 256, 0, s>>>(A, a_val, fr.p, onesc.p);
-        if (n > 0) k_scale_csr<<<gn, 256, 0, s>>>(AT, at_val, onesc.p, fr.p);
-        if (n > 0) k_row_measure<<<gn, 256, 0, s>>>(AT, mc.p, what);
-        if (n > 0) k_factor<<<gn, 256, 0, s>>>(mc.p, fc.p, P.C.p, n);
-        if (n > 0) k_scale_csr<<<gn, 256, 0, s>>>(AT, at_val, fc.p, onesr.p);
-        if (m > 0) k_scale_csr<<<gm, 256, 0, s>>>(A, a_val, onesr.p, fc.p);
+        if (m > 0) {
+            k_row_measure<<<gm, 256, 0, s>>>(SA.rowptr.p, SA.nseg, m, a_val, mr.p, what);
+            k_factor<<<gm, 256, 0, s>>>(mr.p, fr.p, P.R.p, m);
+            k_scale_csr<<<gm, 256, 0, s>>>(SA.rowptr.p, SA.nseg, m, SA.col.p, a_val, fr.p, onesc.p);
+        }
+        if (n > 0) {
+            k_scale_csr<<<gn, 256, 0, s>>>(ST.rowptr.p, ST.nseg, n, ST.col.p, at_val, onesc.p, fr.p);
+            k_row_measure<<<gn, 256, 0, s>>>(ST.rowptr.p, ST.nseg, n, at_val, mc.p, what);
+            k_factor<<<gn, 256, 0, s>>>(mc.p, fc.p, P.C.p, n);
+            k_scale_csr<<<gn, 256, 0, s>>>(ST.rowptr.p, ST.nseg, n, ST.col.p, at_val, fc.p, onesr.p);
+        }
+        if (m > 0) k_scale_csr<<<gm, 256, 0, s>>>(SA.rowptr.p, SA.nseg, m, SA.col.p, a_val, onesr.p, fc.p);
         PD_CHECK(hipGetLastError());
     }
     std::vector<double> R(static_cast<size_t>(m)), Cc(static_cast<size_t>(n));
@@ -773,21 +845,13 @@ void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm,
         for (int b = 0; b < nsq; ++b) t = dd_add(t, hsq[b]);
         return std::sqrt(dd_val(t));
     };
-    auto spmv = [&](const Csr<double>& M, const double* in, double* out) {
-        switch (P.tile) {
-            case 1024: k_spmv_plain<1024><<<M.nblk, kThreads, 0, s>>>(M, in, out); break;
-            case 4096: k_spmv_plain<4096><<<M.nblk, kThreads, 0, s>>>(M, in, out); break;
-            default: k_spmv_plain<2048><<<M.nblk, kThreads, 0, s>>>(M, in, out); break;
-        }
-        PD_CHECK(hipGetLastError());
-    };
     double lam = 0.0;
     if (n > 0 && m > 0 && P.nnz > 0) {
         double nu = norm_of_u();
         if (nu > 0) k_scale_vec<<<gn, 256, 0, s>>>(u.p, n, nu);
         for (int it = 0; it < 200; ++it) {
-            spmv(A, u.p, v.p);
-            spmv(AT, v.p, u.p);
+            spmv_plain(*Am, u.p, v.p, s);
+            spmv_plain(*ATm, v.p, u.p, s);
             const double nw = norm_of_u();
             if (!(nw > 0.0) || !std::isfinite(nw)) break;
             k_scale_vec<<<gn, 256, 0, s>>>(u.p, n, nw);
@@ -827,14 +891,16 @@ void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm,
     PD_CHECK(hipHostMalloc(reinterpret_cast<void**>(&P.h_st), 2 * sizeof(PdlpState), hipHostMallocDefault));
     std::memset(P.h_st, 0, 2 * sizeof(PdlpState));
     for (auto& e : P.ev) PD_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    P.part_c.alloc(static_cast<size_t>(5) * AT.nblk);
-    P.part_r.alloc(static_cast<size_t>(6) * A.nblk);
+    P.part_c.alloc(static_cast<size_t>(5) * ST.nblk[ST.nseg - 1]);
+    P.part_r.alloc(static_cast<size_t>(6) * SA.nblk[SA.nseg - 1]);
 
     if (P.mixed) {
-        P.a32.alloc(P.nnz);
-        P.at32.alloc(P.nnz);
-        k_to_float<<<grid_for(P.nnz), 256, 0, s>>>(A.val, P.a32.p, P.nnz);
-        k_to_float<<<grid_for(P.nnz), 256, 0, s>>>(AT.val, P.at32.p, P.nnz);
+        P.a32.val.alloc(P.nnz);
+        P.at32.val.alloc(P.nnz);
+        P.a32.part.alloc(std::max(m, 1));
+        P.at32.part.alloc(std::max(n, 1));
+        k_to_float<<<grid_for(P.nnz), 256, 0, s>>>(a_val, P.a32.val.p, P.nnz);
+        k_to_float<<<grid_for(P.nnz), 256, 0, s>>>(at_val, P.at32.val.p, P.nnz);
         P.f.alloc(n, m);
         auto conv = [&](Buf<double>& from, Buf<float>& to, long long len) {
             k_to_float<<<grid_for(len), 256, 0, s>>>(from.p, to.p, len);
@@ -851,7 +917,16 @@ void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm,
     PdlpState hs = read_state(P, s);
     if (hs.done) {
         P.finished = hs.status != 0 || !(P.mixed && P.polish_tol > 0);
-        if (!P.finished && P.mixed) switch_to_fp64(P, s, P.polish_tol);
+        P.status = hs.status;
+        if (!P.finished && P.mixed) {
+            P.polish_done = true;
+            switch_to_fp64(P, s, P.polish_tol);
+            PdlpState h2 = read_state(P, s);
+            if (h2.done) {
+                P.finished = true;
+                P.status = h2.status;
+            }
+        }
     }
     P.setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -933,8 +1008,7 @@ int pdlp_result(Pdlp& P, hipStream_t s, double* x, double* y, nir_info* info) {
     if (y) std::copy(ys.begin(), ys.end(), y);
     if (info) {
         std::memset(info, 0, sizeof(*info));
-        info->status = P.finished ? P.status : 6;
-        if (!P.finished && h.done) info->status = h.status;
+        info->status = P.finished ? P.status : (h.done ? h.status : 6);
         info->iterations = h.total;
         info->polish_iterations = P.fp64_iters_at_switch >= 0 ? h.total - P.fp64_iters_at_switch : 0;
         info->restarts = h.restarts;
@@ -955,7 +1029,8 @@ int pdlp_result(Pdlp& P, hipStream_t s, double* x, double* y, nir_info* info) {
 }
 
 int pdlp_time(Pdlp& P, hipStream_t s, long long iters, nir_timing* out) {
-    // Per-kernel events around every iteration kernel of eval_freq-blocks.
+    // Events around every iteration kernel of eval_freq-blocks (all segment
+    // passes of a product between one pair of events).
     const long long blocks = std::max<long long>(1, iters / P.eval_freq);
     const int n = P.n, m = P.m;
     const double nnz = static_cast<double>(P.nnz);
@@ -965,6 +1040,8 @@ int pdlp_time(Pdlp& P, hipStream_t s, long long iters, nir_timing* out) {
     // A^T y + primal step: values+cols, rowptr, y once, x/xa/c/lo/up read, x/xbar write.
     out->spmtv_bytes = nnz * (sv + 4) + 4.0 * (n + 1) + sv * m + sv * 7 * n;
     out->iter_bytes = out->spmv_bytes + out->spmtv_bytes;
+    out->spmv_passes = P.A->shape.nseg;
+    out->spmtv_passes = P.AT->shape.nseg;
     std::vector<hipEvent_t> ev(static_cast<size_t>(4 * blocks * P.eval_freq + 2));
     for (auto& e : ev) PD_CHECK(hipEventCreate(&e));
     PD_CHECK(hipStreamSynchronize(s));
@@ -973,37 +1050,14 @@ int pdlp_time(Pdlp& P, hipStream_t s, long long iters, nir_timing* out) {
     for (long long b = 0; b < blocks; ++b) {
         for (int kl = 0; kl < P.eval_freq; ++kl) {
             const int save = kl == P.eval_freq - 1 ? 1 : 0;
-            auto one = [&](auto& it, auto Am, auto ATm) {
-                using T = typename std::remove_reference<decltype(*it.x.p)>::type;
-                switch (P.tile) {
-                    case 1024:
-                        PD_CHECK(hipEventRecord(ev[q++], s));
-                        k_primal_pdlp<1024, T><<<ATm.nblk, kThreads, 0, s>>>(ATm, it.y.p, it.x.p, it.xbar.p, it.xa.p, it.xp.p, it.c.p, it.lo.p, it.up.p, P.st.p, kl, save);
-                        PD_CHECK(hipEventRecord(ev[q++], s));
-                        PD_CHECK(hipEventRecord(ev[q++], s));
-                        k_dual_pdlp<1024, T><<<Am.nblk, kThreads, 0, s>>>(Am, it.xbar.p, it.y.p, it.ya.p, it.yp.p, it.rl.p, it.ru.p, P.st.p, kl, save);
-                        PD_CHECK(hipEventRecord(ev[q++], s));
-                        break;
-                    case 4096:
-                        PD_CHECK(hipEventRecord(ev[q++], s));
-                        k_primal_pdlp<4096, T><<<ATm.nblk, kThreads, 0, s>>>(ATm, it.y.p, it.x.p, it.xbar.p, it.xa.p, it.xp.p, it.c.p, it.lo.p, it.up.p, P.st.p, kl, save);
-                        PD_CHECK(hipEventRecord(ev[q++], s));
-                        PD_CHECK(hipEventRecord(ev[q++], s));
-                        k_dual_pdlp<4096, T><<<Am.nblk, kThreads, 0, s>>>(Am, it.xbar.p, it.y.p, it.ya.p, it.yp.p, it.rl.p, it.ru.p, P.st.p, kl, save);
-                        PD_CHECK(hipEventRecord(ev[q++], s));
-                        break;
-                    default:
-                        PD_CHECK(hipEventRecord(ev[q++], s));
-                        k_primal_pdlp<2048, T><<<ATm.nblk, kThreads, 0, s>>>(ATm, it.y.p, it.x.p, it.xbar.p, it.xa.p, it.xp.p, it.c.p, it.lo.p, it.up.p, P.st.p, kl, save);
-                        PD_CHECK(hipEventRecord(ev[q++], s));
-                        PD_CHECK(hipEventRecord(ev[q++], s));
-                        k_dual_pdlp<2048, T><<<Am.nblk, kThreads, 0, s>>>(Am, it.xbar.p, it.y.p, it.ya.p, it.yp.p, it.rl.p, it.ru.p, P.st.p, kl, save);
-                        PD_CHECK(hipEventRecord(ev[q++], s));
-                        break;
-                }
-            };
-            if (P.mixed) one(P.f, with_vals(P.A, P.a32.p), with_vals(P.AT, P.at32.p));
-            else one(P.d, P.A, P.AT);
+            PD_CHECK(hipEventRecord(ev[q++], s));
+            if (P.mixed) launch_primal<float>(P, s, P.f, kl, save);
+            else launch_primal<double>(P, s, P.d, kl, save);
+            PD_CHECK(hipEventRecord(ev[q++], s));
+            PD_CHECK(hipEventRecord(ev[q++], s));
+            if (P.mixed) launch_dual<float>(P, s, P.f, kl, save);
+            else launch_dual<double>(P, s, P.d, kl, save);
+            PD_CHECK(hipEventRecord(ev[q++], s));
         }
         if (P.mixed) launch_eval<float>(P, s, P.f, 0);
         else launch_eval<double>(P, s, P.d, 0);
@@ -1013,7 +1067,7 @@ int pdlp_time(Pdlp& P, hipStream_t s, long long iters, nir_timing* out) {
     float ms = 0.f;
     PD_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
     out->total_ms = ms;
-    for (size_t k = 2; k + 3 < q + 1; k += 4) {
+    for (size_t k = 2; k + 3 < q; k += 4) {
         PD_CHECK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
         out->spmtv_ms += ms;
         PD_CHECK(hipEventElapsedTime(&ms, ev[k + 2], ev[k + 3]));

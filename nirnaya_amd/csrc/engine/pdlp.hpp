@@ -37,8 +37,8 @@ struct PdlpDeleter {
     void operator()(Pdlp* p) const;
 };
 Pdlp* pdlp_create();
-void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm, Csr<double> A, Csr<double> AT,
-                int tile, int tile_rows);
+struct DevMatrix;
+void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm, DevMatrix* A, DevMatrix* AT);
 void pdlp_run(Pdlp& P, hipStream_t s, const nir_params& prm, long long budget);
 int pdlp_result(Pdlp& P, hipStream_t s, double* x, double* y, nir_info* info);
 int pdlp_time(Pdlp& P, hipStream_t s, long long iters, nir_timing* out);

@@ -8,10 +8,19 @@
 //      index and value arrays, gather the dense operand, and park the products
 //      in LDS;
 //   2. one lane per row sums its products from LDS in ascending column order,
-//      starting from 0.0 - the same order and the same rounding as a
-//      sequential CSR/CSC dot product, so results are bit-identical to it;
+//      starting from the row's running sum (0.0 on the first pass) - the same
+//      order and the same rounding as a sequential CSR/CSC dot product, so
+//      results are bit-identical to it;
 //   3. the caller's epilogue consumes the row sum (fused vector update).
 // Rows longer than TILE use strided lane partials and a fixed-order LDS tree.
+//
+// Column segments: a matrix may be split by column ranges into S segments
+// (each a CSR over all rows, stored segment-major in one value/index array).
+// A product then runs S passes; pass s gathers only from the s-th slice of
+// the dense operand, which stays resident in each XCD's L2, and carries the
+// running row sums in a partial buffer. Because segments are taken in
+// ascending column order and each pass continues the same sequential sum, the
+// result is still bit-identical to the single-pass product.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -25,32 +34,63 @@ template <typename T>
 struct Csr {
     int nrows;
     int ncols;
-    const int* rowptr;   // nrows + 1
+    const int* rowptr;   // nrows + 1 (absolute offsets into col / val)
     const int* col;      // nnz
     const T* val;        // nnz
     int nblk;            // number of row blocks (>= 1)
-    const int* blk;      // nblk + 1 row boundaries
+    const int4* desc;    // per block {first row, end row, first nonzero, end nonzero}
+};
+
+// One pass of a (possibly segmented) product: which segment, and where the
+// running row sums live between passes.
+template <typename T>
+struct Pass {
+    Csr<T> M;
+    T* part;     // running row sums (nrows), unused when first && last
+    int first;   // start sums at 0.0
+    int last;    // hand the completed sum to the epilogue
 };
 
 __device__ __forceinline__ bool finite_bound(double b) { return __builtin_isfinite(b) && fabs(b) < 1e20; }
 __device__ __forceinline__ double amax(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double amin(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ float amaxf(float a, float b) { return (a < b) ? b : a; }
-__device__ __forceinline__ float aminf(float a, float b) { return (b < a) ? b : a; }
 
 template <typename T>
 __device__ __forceinline__ T load_stream(const T* p) {
     return __builtin_nontemporal_load(p);
 }
 
-// Row sums of A * v for the block `b`; calls on_row(row, sum) once per row.
-template <int TILE, typename T, typename V, typename F>
-__device__ __forceinline__ void stream_rows(const Csr<T>& A, const V* __restrict__ v, T* prod, int b, F&& on_row) {
+// LDS product tile layout: one padding slot after every 32 products, so lanes
+// summing rows that start ~24 products apart hit different banks.
+__host__ __device__ constexpr int lds_slots(int tile) { return tile + tile / 32; }
+__device__ __forceinline__ int lds_at(int q) { return q + (q >> 5); }
+
+// No epilogue operands to prefetch.
+struct NoPre {};
+struct PreNone {
+    __device__ __forceinline__ NoPre operator()(int) const { return NoPre{}; }
+};
+
+// Row sums of A * v for block `b`. On the last pass on_row(row, sum, pre) is
+// called once per row, where pre = prefetch(row) was issued before the tile's
+// barrier so the epilogue operands arrive while the products are summed; on
+// earlier passes the sums are stored to P.part.
+//
+// The tile loop is branch-free: lanes past the block's nonzeros load slot 0
+// (every array is allocated with at least one element), then use column 0
+// and value 0 instead, gather v[0] (the dense operand also has at least one
+// element), and store into LDS slots the row phase never reads; so all loads
+// and gathers of a lane are in flight together.
+template <int TILE, typename T, typename V, typename Pre, typename F>
+__device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restrict__ v, T* prod, int b,
+                                            Pre&& prefetch, F&& on_row) {
     static_assert(TILE % kThreads == 0, "tile must be a multiple of the workgroup");
+    const Csr<T>& A = P.M;
     const int tid = threadIdx.x;
-    const int r0 = A.blk[b], r1 = A.blk[b + 1];
-    const int p0 = A.rowptr[r0], p1 = A.rowptr[r1];
+    const int4 d = A.desc[b];   // one load instead of blk -> rowptr chained loads
+    const int r0 = d.x, r1 = d.y, p0 = d.z, p1 = d.w;
     const int cnt = p1 - p0;
+    if (r1 == r0) return;   // empty matrix: the one block has no rows
     if (cnt > TILE) {
         // One long row: lane partials then a fixed-order tree.
         T s = T(0);
@@ -61,33 +101,62 @@ __device__ __forceinline__ void stream_rows(const Csr<T>& A, const V* __restrict
             if (tid < w) prod[tid] += prod[tid + w];
             __syncthreads();
         }
-        if (tid == 0) on_row(r0, prod[0]);
+        if (tid == 0) {
+            const T init = P.first ? T(0) : P.part[r0];
+            const T sum = init + prod[0];
+            if (P.last) on_row(r0, sum, prefetch(r0));
+            else P.part[r0] = sum;
+        }
         return;
     }
+    // Row bookkeeping and epilogue operands first: their latency overlaps the tile.
+    const int r = r0 + tid;
+    const bool has_row = r < r1;
+    const int rr = has_row ? r : r0;
+    const int ra = A.rowptr[rr] - p0, re = A.rowptr[rr + 1] - p0;
+    const T init = (P.first || !has_row) ? T(0) : P.part[rr];
+    const auto pre = prefetch(rr);
+
     constexpr int PER = TILE / kThreads;
     int ci[PER];
     T vi[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         const int q = tid + u * kThreads;
-        if (q < cnt) {
-            ci[u] = load_stream(A.col + p0 + q);
-            vi[u] = load_stream(A.val + p0 + q);
-        }
+        const bool in = q < cnt;
+        const int slot = in ? p0 + q : 0;
+        const int cl = load_stream(A.col + slot);
+        const T vl = load_stream(A.val + slot);
+        ci[u] = in ? cl : 0;
+        vi[u] = in ? vl : T(0);
     }
+    T g[PER];
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int q = tid + u * kThreads;
-        if (q < cnt) prod[q] = vi[u] * static_cast<T>(v[ci[u]]);
-    }
+    for (int u = 0; u < PER; ++u) g[u] = static_cast<T>(v[ci[u]]);
+#pragma unroll
+    for (int u = 0; u < PER; ++u) prod[lds_at(tid + u * kThreads)] = vi[u] * g[u];
     __syncthreads();
-    const int r = r0 + tid;
-    if (r < r1) {
-        const int a = A.rowptr[r] - p0, e = A.rowptr[r + 1] - p0;
-        T s = T(0);
-        for (int q = a; q < e; ++q) s += prod[q];
-        on_row(r, s);
+    if (has_row) {
+        // Sequential sum in column order; four LDS reads in flight per step.
+        T s = init;
+        int q = ra;
+        for (; q + 4 <= re; q += 4) {
+            const T a0 = prod[lds_at(q)], a1 = prod[lds_at(q + 1)], a2 = prod[lds_at(q + 2)], a3 = prod[lds_at(q + 3)];
+            s += a0;
+            s += a1;
+            s += a2;
+            s += a3;
+        }
+        for (; q < re; ++q) s += prod[lds_at(q)];
+        if (P.last) on_row(r, s, pre);
+        else P.part[r] = s;
     }
+}
+
+// Same without epilogue prefetch.
+template <int TILE, typename T, typename V, typename F>
+__device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restrict__ v, T* prod, int b, F&& on_row) {
+    stream_rows<TILE>(P, v, prod, b, PreNone{}, [&](int r, T s, NoPre) { on_row(r, s); });
 }
 
 // Max of nonnegative values over the workgroup; result valid in thread 0.

@@ -53,7 +53,7 @@ class NirParams(C.Structure):
         ("mode", C.c_int32), ("precision", C.c_int32), ("max_iters", C.c_int64),
         ("tol", C.c_double), ("polish_tol", C.c_double), ("eval_freq", C.c_int32),
         ("check_termination", C.c_int32), ("use_graph", C.c_int32), ("verbose", C.c_int32),
-        ("tile_nnz", C.c_int32), ("tile_rows", C.c_int32),
+        ("tile_nnz", C.c_int32), ("tile_rows", C.c_int32), ("seg_width", C.c_int32),
         ("x0", _dp), ("y0", _dp), ("unscale_row", _dp), ("unscale_col", _dp),
         ("should_stop", _STOP_CB), ("stop_user", C.c_void_p),
     ]
@@ -73,7 +73,7 @@ class NirTiming(C.Structure):
     _fields_ = [
         ("iterations", C.c_int64), ("total_ms", C.c_double), ("spmv_ms", C.c_double),
         ("spmtv_ms", C.c_double), ("spmv_bytes", C.c_double), ("spmtv_bytes", C.c_double),
-        ("iter_bytes", C.c_double),
+        ("iter_bytes", C.c_double), ("spmv_passes", C.c_int32), ("spmtv_passes", C.c_int32),
     ]
 
 
@@ -101,6 +101,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.nir_session_result.argtypes = [vp, _dp, _dp, C.POINTER(NirInfo)]
     lib.nir_session_time.argtypes = [vp, C.c_int64, C.POINTER(NirTiming)]
     lib.nir_session_destroy.argtypes = [vp]
+    lib.nir_session_shape.argtypes = [vp, _i32p]
     lib.nir_calibrate_copy.argtypes = [vp, C.c_int64, _dp]
     lib.nir_pdhg_solve.argtypes = [vp, C.POINTER(NirLp), C.POINTER(NirParams), _dp, _dp, C.POINTER(NirInfo)]
     for name in ("nir_gen_equality_lp", "nir_gen_mixed_lp", "nir_gen_staircase_lp"):
@@ -324,6 +325,12 @@ class Session:
         if self.lib.nir_session_time(self.h, int(iters), C.byref(t)) != 0:
             raise EngineError("nir_session_time: " + self.eng.error())
         return t
+
+    def shape(self):
+        """(A tile, A rows, A segments, A^T tile, A^T rows, A^T segments)."""
+        out = (C.c_int32 * 6)()
+        self.lib.nir_session_shape(self.h, out)
+        return tuple(out)
 
     def close(self):
         if getattr(self, "h", None):

@@ -52,6 +52,13 @@ def test_fp64_pdlp_matches_oracle(engine, name):
         assert abs(k[3] - HIGHS[name]) <= 1e-4 * (1 + abs(HIGHS[name]))
 
 
+def test_segmented_pdlp_matches_oracle(engine):
+    lp = gen_staircase_lp(3, 6, 8)
+    r = engine.solve(lp, default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-6, seg_width=37, tile_rows=64))
+    xo, yo, info = Oracle(lp).pdlp(max_iters=400000, tol=1e-6)
+    assert r.iterations == info.iterations and np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+
+
 def test_graph_and_eager_identical(engine):
     lp = gen_staircase_lp(4, 8, 9)
     a = engine.solve(lp, default_params(mode=MODE_PDLP, max_iters=100000, use_graph=1))

@@ -49,6 +49,31 @@ def test_tiles_and_graphs_bit_identical(engine, tile, rows, graph):
     assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
 
 
+@pytest.mark.parametrize("seg", [1000, 4096, 1 << 17])
+def test_column_segments_bit_identical(engine, seg):
+    # Segment passes carry the running row sums, so the order of every sum is
+    # unchanged: still bit-identical to the oracle.
+    lp = gen_mixed_lp(17, 6000, 18000, 6.0, 0.01, 1e3)
+    r = run_ref(engine, lp, 50, seg_width=seg, tile_rows=128)
+    xo, yo, _, _, _ = Oracle(lp).pdhg(50, check=False)
+    assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+
+
+def test_autotuned_shapes_bit_identical(engine):
+    lp = gen_equality_lp(20260, 2000, 5000, 5)
+    p = default_params(mode=MODE_REFERENCE, max_iters=100, check_termination=0, tile_nnz=-1)
+    s = engine.session(lp, p)
+    try:
+        s.run(100)
+        r = s.result()
+        shape = s.shape()
+    finally:
+        s.close()
+    print("autotuned (A tile, rows, segs, AT tile, rows, segs):", shape)
+    xo, yo, _, _, _ = Oracle(lp).pdhg(100, check=False)
+    assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+
+
 def test_stop_rule_matches_oracle(engine):
     # Reference stop rule (pdhg_solver.hpp:97-100) decided on the device.
     for lp in (lpkit.pdhg_unit_lp(), lpkit.transport_lp()):
