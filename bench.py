@@ -70,7 +70,9 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--tile", type=int, default=-1, help="LDS tile (-1: autotune tile, rows and segments)")
+    ap.add_argument("--seg", type=int, default=0, help="column-segment width of A xbar when not autotuning")
+    ap.add_argument("--seg-t", type=int, default=-1, help="column-segment width of A^T y (-1: as --seg)")
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the 256 MiB copy kernel (reference for rocprofv3 byte counters)")
@@ -90,14 +92,23 @@ def main():
     if args.calibrate:
         ms = eng.calibrate_copy(256 << 20)
         log(f"[bench] calibration copy of 256 MiB: {ms:.3f} ms ({2 * 256 * 1.048576 / ms:.0f} GB/s read+write)")
-    params = default_params(mode=MODE_REFERENCE, max_iters=args.warmup + args.steps + 64, check_termination=0,
-                            tile_nnz=args.tile, tile_rows=args.tile_rows)
+    params = default_params(mode=MODE_REFERENCE, max_iters=args.warmup + 2 * args.steps + 64, check_termination=0,
+                            tile_nnz=args.tile, tile_rows=args.tile_rows, seg_width=args.seg,
+                            seg_width_t=args.seg_t)
     sess = eng.session(lp, params)
-    log(f"[bench] session ready (upload + power iteration) in {sess.result().info.setup_seconds:.2f}s")
+    shape = sess.shape()
+    log(f"[bench] session ready (upload, tile autotune, power iteration) in {sess.result().info.setup_seconds:.2f}s; "
+        f"shape A x: tile {shape[0]} rows {shape[1]} segments {shape[2]}, A^T y: tile {shape[3]} rows {shape[4]} "
+        f"segments {shape[5]}")
     sess.run(args.warmup)
     log(f"[bench] warmup {args.warmup} iterations done")
+    # Two sweeps of `steps` iterations: the production path (graph replays)
+    # for the throughput, then the same kernels with events around every
+    # dispatch for the per-kernel durations.
     t = sess.time(args.steps)
-    log(f"[bench] timed {args.steps} iterations: {t.total_ms:.1f} ms")
+    log(f"[bench] timed {args.steps} iterations: {t.total_ms:.1f} ms; per-dispatch sweep: A xbar "
+        f"{t.spmv_ms / args.steps * 1e3:.1f} us in {t.spmv_passes} passes, A^T y {t.spmtv_ms / args.steps * 1e3:.1f} us "
+        f"in {t.spmtv_passes} passes")
     sess.close()
     eng.close()
 
@@ -109,7 +120,9 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "spmv_cfg1_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tr = json.load(f)
+        if tr.get("segments") == t.spmv_passes:
+            traffic = tr.get("hbm_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -135,9 +148,13 @@ def main():
                    "m": lp.m, "n": lp.n, "nnz": lp.nnz},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_dual_ref (A xbar + fused dual step)",
-                     "bytes_per_launch": t.spmv_bytes, "avg_launch_us": spmv_avg_s * 1e6},
-        "kernels": {"spmtv_primal_avg_us": spmtv_avg_s * 1e6, "spmtv_bytes_per_launch": t.spmtv_bytes,
+                     "kernel": "k_dual_ref (A xbar + fused dual step); one launch = its column-segment passes",
+                     "bytes_per_launch": t.spmv_bytes, "avg_launch_us": spmv_avg_s * 1e6,
+                     "dispatches_per_launch": t.spmv_passes,
+                     "avg_dispatch_us": spmv_avg_s * 1e6 / t.spmv_passes,
+                     "shape": {"tile": shape[0], "rows": shape[1], "segments": shape[2]}},
+        "kernels": {"spmtv_primal_avg_us": spmtv_avg_s * 1e6, "spmtv_dispatches_per_launch": t.spmtv_passes,
+                    "spmtv_bytes_per_launch": t.spmtv_bytes,
                     "spmtv_gbs": t.spmtv_bytes / spmtv_avg_s / 1e9,
                     "iteration_bytes": t.iter_bytes,
                     "iteration_gbs": t.iter_bytes * args.steps / (t.total_ms / 1e3) / 1e9},

@@ -60,7 +60,8 @@ typedef struct {
     int32_t tile_nnz;          /* SpMV tile: nonzeros per workgroup (0 = default 2048,
                                   -1 = autotune tile, rows and segments per matrix) */
     int32_t tile_rows;         /* SpMV tile: max rows per workgroup (0 = default 256) */
-    int32_t seg_width;         /* column-segment width of the products (0 = one segment) */
+    int32_t seg_width;         /* column-segment width of the A xbar products (0 = one segment) */
+    int32_t seg_width_t;       /* the same for the A^T y products (-1 = seg_width) */
     const double* x0;          /* optional warm start (ncol), NULL = cold */
     const double* y0;          /* optional warm start (nrow), NULL = cold */
     const double* unscale_row; /* optional: report KKT in units y*row, (Ax)/row */
@@ -84,11 +85,15 @@ typedef struct {
     double solve_seconds;   /* iteration loop */
 } nir_info;
 
-/* Per-kernel timing over a block of iterations, measured with HIP events on
- * the session's stream around every launch of the two iteration kernels. */
+/* Timing of `iterations` iterations, with HIP events on the session's stream.
+ * Reference mode runs two sweeps of that many iterations: the production path
+ * (32-iteration graph replays) between one event pair gives total_ms; then the
+ * same kernels launched one by one with an event pair around each product
+ * (all its column-segment passes) give spmv_ms / spmtv_ms. Pdlp mode runs one
+ * sweep with an event pair around each product. */
 typedef struct {
     int64_t iterations;
-    double total_ms;            /* events bracketing the whole block */
+    double total_ms;            /* events bracketing the production sweep */
     double spmv_ms;             /* summed durations: A xbar + dual step */
     double spmtv_ms;            /* summed durations: A^T y + primal step */
     double spmv_bytes;          /* algorithmic bytes of one A xbar launch */
@@ -125,6 +130,40 @@ NIR_API int nir_calibrate_copy(nir_ctx* ctx, int64_t bytes, double* ms);
 NIR_API int nir_pdhg_solve(nir_ctx* ctx, const nir_lp* lp, const nir_params* p, double* x, double* y,
                            nir_info* info);
 
+/* ---- batched solves: many LPs in persistent launches --------------------
+ * No counterpart in the reference (it solves one model per call); this is
+ * the batch entry point of the engine ABI. Every instance runs pdlp mode with
+ * the shared parameters (mode must be NIR_MODE_PDLP; no warm start or
+ * unscaling vectors) and gets the same iterates, iteration and restart counts
+ * as nir_pdhg_solve on it alone. */
+typedef struct nir_batch nir_batch;
+typedef struct {
+    int32_t threads;          /* workgroup size of the persistent kernel: 256, 512 or 1024;
+                                 0 = autotune (timed on the batch after setup) */
+    int32_t quantum;          /* evaluation blocks an instance runs per work-list ticket (0 = 16) */
+    int32_t rounds_per_graph; /* persistent launches captured per hipGraph replay (0 = 4) */
+    int32_t workgroups;       /* persistent grid (0 = what fits on the device) */
+} nir_batch_opts;
+typedef struct {
+    int32_t count, threads, workgroups, quantum;
+    int64_t launches;         /* persistent launches of the iteration kernel */
+    int64_t total_iterations; /* summed over instances */
+    double nnz_iterations;    /* sum over instances of iterations x nnz */
+    double total_nnz;
+    double setup_seconds, solve_seconds;
+} nir_batch_stats;
+NIR_API void nir_batch_opts_default(nir_batch_opts* o);
+/* Uploads and prepares every instance (scaling, power iteration, start point
+ * evaluation) on the device; `lps` holds `count` pointers. */
+NIR_API int nir_batch_create(nir_ctx* ctx, const nir_lp* const* lps, int32_t count, const nir_params* p,
+                             const nir_batch_opts* o, nir_batch** out);
+/* Runs every instance to its tolerance, iteration cap or should_stop (polled
+ * between graph replays; unfinished instances then report status 4). */
+NIR_API int nir_batch_solve(nir_batch* b);
+NIR_API int nir_batch_result(nir_batch* b, int32_t i, double* x, double* y, nir_info* info);
+NIR_API int nir_batch_get_stats(nir_batch* b, nir_batch_stats* out);
+NIR_API void nir_batch_destroy(nir_batch* b);
+
 /* ---- synthetic instances (seeded; generated on the host) ---------------- */
 typedef struct nir_owned_lp nir_owned_lp;
 
@@ -138,6 +177,9 @@ NIR_API nir_owned_lp* nir_gen_mixed_lp(uint64_t seed, int32_t m, int32_t n, doub
                                        double bad_scale);
 /* Multi-period blending/planning LP with T periods and K products. */
 NIR_API nir_owned_lp* nir_gen_staircase_lp(uint64_t seed, int32_t T, int32_t K);
+/* Periods T ~ U{4..48} and products K ~ U{5..40} of the planning-family
+ * member with this seed (the batch of uneven sizes). */
+NIR_API void nir_gen_planning_dims(uint64_t seed, int32_t* T, int32_t* K);
 NIR_API int nir_owned_lp_view(const nir_owned_lp* g, nir_lp* out);
 NIR_API int64_t nir_owned_lp_nnz(const nir_owned_lp* g);
 /* The feasible point used to build the right-hand side (ncol), and the dual

@@ -1,0 +1,1108 @@
+// Batched restarted PDHG: many LPs of uneven size in persistent launches.
+//
+// Layout in HBM: every instance's CSR(A) and CSR(A^T) (int32 local indices,
+// FP64 values scaled in place, FP32 copies in mixed mode) and its vectors are
+// concatenated; an instance table holds the offsets. One workgroup owns one
+// instance at a time and runs whole PDHG iterations on it with workgroup
+// barriers between the primal and the dual half (an instance of the
+// planning family fits comfortably in one CU's share of L2).
+//
+// Work queue: a list of unfinished instances, largest first. A launch of the
+// persistent kernel hands them out by an atomic ticket; the workgroup runs up
+// to `quantum` evaluation blocks (eval_freq iterations + FP64 evaluation +
+// restart decision each) on its instance, writes the instance's state back
+// and takes the next ticket; it exits when the list is exhausted, so every
+// launch drains. A one-workgroup kernel then compacts the list (finished
+// instances leave it) and re-arms the ticket counter. The host replays a
+// hipGraph of several (run, compact) rounds and polls the list length.
+//
+// Every per-instance computation uses the single-LP engine's arithmetic
+// (pdlp_core.hpp): sequential row sums in ascending index order, the same
+// scaling, power iteration, step sizes, Halpern updates, evaluation terms
+// and decisions; norms are double-double. So each instance's iterates,
+// iteration and restart counts equal what nir_pdhg_solve returns for it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/nirnaya/pdhg.h"
+#include "ctx.hpp"
+#include "devmat.hpp"
+#include "lpcheck.hpp"
+#include "pdlp.hpp"
+#include "pdlp_core.hpp"
+
+namespace nir {
+namespace gpu {
+namespace batch {
+
+// Offsets of one instance in the concatenated arrays.
+struct Inst {
+    int m, n, nnz, pad;
+    long long arp;    // CSR(A) row pointers (m + 1)
+    long long trp;    // CSR(A^T) row pointers (n + 1)
+    long long nz;     // nonzeros (both matrices)
+    long long xo;     // column vectors
+    long long yo;     // row vectors
+};
+
+// Per-instance control beyond the iteration state.
+struct Ctl {
+    int fp32;          // 1 while the instance iterates in FP32 (mixed mode)
+    int finished;
+    int status;
+    int polish_done;
+    long long fp64_at; // iterations completed at the FP64 hand-over (-1: none)
+};
+
+template <typename T>
+struct Vec {
+    T *x, *xbar, *xa, *xp, *y, *ya, *yp;
+    T *c, *lo, *up, *rl, *ru;
+    T *aval, *tval;
+};
+
+struct View {
+    const Inst* inst;
+    const int *arow, *acol, *trow, *tcol;
+    Vec<double> d;
+    Vec<float> f;
+    double *R, *C, *hs;
+    const double *cost, *collo, *colup, *rowlo, *rowup, *offset;
+    PdlpState* st;
+    Ctl* ctl;
+    int* list;
+    int* count;
+    int* head;
+    long long* launches;
+    int mixed;
+    int eval_freq;
+    int quantum;
+    double tol, polish_tol;
+    long long max_iters;
+};
+
+// Instance-local pointers of one precision.
+template <typename T>
+struct Local {
+    const int *arow, *acol, *trow, *tcol;
+    T *aval, *tval;
+    T *x, *xbar, *xa, *xp, *y, *ya, *yp;
+    T *c, *lo, *up, *rl, *ru;
+};
+
+template <typename T>
+__device__ __forceinline__ Local<T> local(const View& V, const Vec<T>& v, const Inst& I) {
+    Local<T> L;
+    L.arow = V.arow + I.arp;
+    L.acol = V.acol + I.nz;
+    L.trow = V.trow + I.trp;
+    L.tcol = V.tcol + I.nz;
+    L.aval = v.aval + I.nz;
+    L.tval = v.tval + I.nz;
+    L.x = v.x + I.xo; L.xbar = v.xbar + I.xo; L.xa = v.xa + I.xo; L.xp = v.xp + I.xo;
+    L.c = v.c + I.xo; L.lo = v.lo + I.xo; L.up = v.up + I.xo;
+    L.y = v.y + I.yo; L.ya = v.ya + I.yo; L.yp = v.yp + I.yo;
+    L.rl = v.rl + I.yo; L.ru = v.ru + I.yo;
+    return L;
+}
+
+// Sequential dot product of row r with v, ascending index order, products
+// rounded before the add: the sum order of the single-LP CSR-stream kernels.
+template <typename T, typename V>
+__device__ __forceinline__ T row_dot(const int* __restrict__ rp, const int* __restrict__ col,
+                                     const T* __restrict__ val, const V* __restrict__ v, int r) {
+    T s = T(0);
+    const int e = rp[r + 1];
+    for (int p = rp[r]; p < e; ++p) s += val[p] * static_cast<T>(v[col[p]]);
+    return s;
+}
+
+// dd sums of NQ per-thread accumulators over the workgroup (wave butterflies,
+// then the waves in order); every thread receives the totals in `tot`.
+template <int NQ, int BT>
+__device__ __forceinline__ void wg_dd_sum(dd (&acc)[NQ], dd* sh, dd* tot) {
+    constexpr int NW = BT / 64;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc[q] = dd_add(acc[q], dd{__shfl_xor(acc[q].hi, o), __shfl_xor(acc[q].lo, o)});
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) sh[q * NW + w] = acc[q];
+    __syncthreads();
+    if (threadIdx.x < NQ) {
+        dd t = dd_zero();
+        for (int k = 0; k < NW; ++k) t = dd_add(t, sh[threadIdx.x * NW + k]);
+        tot[threadIdx.x] = t;
+    }
+    __syncthreads();
+}
+
+// Workgroup-shared scratch of the persistent kernels.
+template <int BT>
+struct Shared {
+    PdlpState s;
+    Ctl ctl;
+    dd red[11 * (BT / 64)];
+    dd tot[11];
+    double scalar;
+    int item;
+};
+
+// ------------------------------------------------------------------ evaluation
+
+// FP64 evaluation of T(z) (xp, yp in precision TI) and the decision.
+template <typename TI, int BT>
+__device__ void wg_eval(const View& V, const Inst& I, const Vec<TI>& it, Shared<BT>& sh, int mode) {
+    const Local<double> D = local(V, V.d, I);
+    const TI* xp = it.xp + I.xo;
+    const TI* xa = it.xa + I.xo;
+    const TI* yp = it.yp + I.yo;
+    const TI* ya = it.ya + I.yo;
+    const double* C = V.C + I.xo;
+    const double* R = V.R + I.yo;
+    // Products first (xbar is dead between blocks and holds A^T yp), then the
+    // terms: two light loops instead of one that keeps gathers and eleven
+    // double-double accumulators live together.
+    double* gs = V.d.xbar + I.xo;
+    double* hs = V.hs + I.yo;
+    for (int j = threadIdx.x; j < I.n; j += BT) gs[j] = row_dot<double>(D.trow, D.tcol, D.tval, yp, j);
+    for (int i = threadIdx.x; i < I.m; i += BT) hs[i] = row_dot<double>(D.arow, D.acol, D.aval, xp, i);
+    {
+        dd acc[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) acc[q] = dd_zero();
+        for (int j = threadIdx.x; j < I.n; j += BT) {
+            dd t[5];
+            col_terms(gs[j], D.c[j], D.lo[j], D.up[j], C[j], static_cast<double>(xp[j]), static_cast<double>(xa[j]),
+                      t);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) acc[q] = dd_add(acc[q], t[q]);
+        }
+        wg_dd_sum<5, BT>(acc, sh.red, sh.tot);
+    }
+    {
+        dd acc[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) acc[q] = dd_zero();
+        for (int i = threadIdx.x; i < I.m; i += BT) {
+            dd t[6];
+            row_terms(hs[i], D.rl[i], D.ru[i], R[i], static_cast<double>(yp[i]), static_cast<double>(ya[i]), t);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) acc[q] = dd_add(acc[q], t[q]);
+        }
+        wg_dd_sum<6, BT>(acc, sh.red, sh.tot + 5);
+    }
+    if (threadIdx.x == 0) pdlp_decide(sh.s, sh.tot, mode);
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------ iteration
+
+template <typename T, int BT>
+__device__ void wg_iteration(const Local<T>& L, const Inst& I, const PdlpState& s, int k_lo, int save) {
+    const Halpern<T> hw = halpern<T>(s.k_base + k_lo);
+    const T tau = static_cast<T>(s.tau);
+    for (int j = threadIdx.x; j < I.n; j += BT) {
+        const T g = row_dot<T>(L.trow, L.tcol, L.tval, L.y, j);
+        const PrimalOut<T> r = primal_point<T>(L.x[j], g, L.c[j], L.lo[j], L.up[j], L.xa[j], tau, hw);
+        L.xbar[j] = r.xb;
+        L.x[j] = r.next;
+        if (save) L.xp[j] = r.cand;
+    }
+    __syncthreads();
+    const T sigma = static_cast<T>(s.sigma);
+    for (int i = threadIdx.x; i < I.m; i += BT) {
+        const T h = row_dot<T>(L.arow, L.acol, L.aval, L.xbar, i);
+        const DualOut<T> r = dual_point<T>(L.y[i], h, L.rl[i], L.ru[i], L.ya[i], sigma, hw);
+        L.y[i] = r.next;
+        if (save) L.yp[i] = r.yp;
+    }
+    __syncthreads();
+}
+
+// Restart: iterate and anchor become T(z).
+template <typename T, int BT>
+__device__ __forceinline__ void wg_restart_copy(const Local<T>& L, const Inst& I) {
+    for (int j = threadIdx.x; j < I.n; j += BT) {
+        const T v = L.xp[j];
+        L.x[j] = v;
+        L.xa[j] = v;
+    }
+    for (int i = threadIdx.x; i < I.m; i += BT) {
+        const T v = L.yp[i];
+        L.y[i] = v;
+        L.ya[i] = v;
+    }
+    __syncthreads();
+}
+
+// FP32 -> FP64 hand-over: the FP64 phase restarts at T(z) of the FP32 phase.
+template <int BT>
+__device__ void wg_switch_to_fp64(const View& V, const Inst& I, Shared<BT>& sh, double new_tol) {
+    const Local<double> D = local(V, V.d, I);
+    const float* fxp = V.f.xp + I.xo;
+    const float* fyp = V.f.yp + I.yo;
+    for (int j = threadIdx.x; j < I.n; j += BT) D.xp[j] = static_cast<double>(fxp[j]);
+    for (int i = threadIdx.x; i < I.m; i += BT) D.yp[i] = static_cast<double>(fyp[i]);
+    __syncthreads();
+    wg_restart_copy<double, BT>(D, I);
+    if (threadIdx.x == 0) {
+        sh.s.done = 0;
+        sh.s.status = 0;
+        sh.s.k_base = 0;
+        sh.s.tol = new_tol;
+        sh.s.watch_stall = 0;
+        sh.ctl.fp32 = 0;
+        sh.ctl.fp64_at = sh.s.total;
+    }
+    __syncthreads();
+    wg_eval<double, BT>(V, I, V.d, sh, 1);
+}
+
+// What follows a decision that set done (the host loop of the single-LP
+// engine, made per instance): finish, or hand the FP32 phase over to FP64.
+template <int BT>
+__device__ void wg_after_done(const View& V, const Inst& I, Shared<BT>& sh) {
+    const int st = sh.s.status;
+    if (sh.ctl.fp32 && (st == 0 || st == 2) && !sh.ctl.polish_done) {
+        const double target = V.polish_tol > 0 ? fmin(V.polish_tol, V.tol) : V.tol;
+        if (st == 0 && V.polish_tol <= 0) {
+            if (threadIdx.x == 0) {
+                sh.ctl.finished = 1;
+                sh.ctl.status = 0;
+            }
+            __syncthreads();
+            return;
+        }
+        if (threadIdx.x == 0) sh.ctl.polish_done = 1;
+        __syncthreads();
+        wg_switch_to_fp64<BT>(V, I, sh, target);
+        if (threadIdx.x == 0 && sh.s.done) {
+            sh.ctl.finished = 1;
+            sh.ctl.status = sh.s.status;
+        }
+        __syncthreads();
+        return;
+    }
+    if (threadIdx.x == 0) {
+        sh.ctl.finished = 1;
+        sh.ctl.status = st;
+    }
+    __syncthreads();
+}
+
+// One evaluation block: eval_freq iterations, evaluation, restart copy.
+template <typename T, int BT>
+__device__ void wg_block(const View& V, const Inst& I, const Vec<T>& v, Shared<BT>& sh) {
+    const Local<T> L = local(V, v, I);
+    for (int kl = 0; kl < V.eval_freq; ++kl) wg_iteration<T, BT>(L, I, sh.s, kl, kl == V.eval_freq - 1 ? 1 : 0);
+    wg_eval<T, BT>(V, I, v, sh, 0);
+    if (sh.s.restart) wg_restart_copy<T, BT>(L, I);
+    if (sh.s.done) wg_after_done<BT>(V, I, sh);
+}
+
+// Take the next ticket of the work list; -1 when the list is exhausted.
+template <int BT>
+__device__ __forceinline__ int wg_pop(const View& V, Shared<BT>& sh) {
+    if (threadIdx.x == 0) {
+        const int t = atomicAdd(V.head, 1);
+        sh.item = t < *V.count ? V.list[t] : -1;
+    }
+    __syncthreads();
+    const int i = __builtin_amdgcn_readfirstlane(sh.item);   // uniform: instance pointers stay scalar
+    __syncthreads();
+    return i;
+}
+
+// 4 waves per SIMD: the register budget of the iteration loop (the rare
+// evaluation code spills a few dwords).
+template <int BT>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_batch_run(View V) {
+    __shared__ Shared<BT> sh;
+    for (;;) {
+        const int i = wg_pop<BT>(V, sh);
+        if (i < 0) return;
+        const Inst I = V.inst[i];
+        if (threadIdx.x == 0) {
+            sh.s = V.st[i];
+            sh.ctl = V.ctl[i];
+        }
+        __syncthreads();
+        for (int b = 0; b < V.quantum && !sh.ctl.finished; ++b) {
+            if (sh.ctl.fp32) wg_block<float, BT>(V, I, V.f, sh);
+            else wg_block<double, BT>(V, I, V.d, sh);
+        }
+        if (threadIdx.x == 0) {
+            V.st[i] = sh.s;
+            V.ctl[i] = sh.ctl;
+        }
+        __syncthreads();
+    }
+}
+
+// Drop finished instances from the list (order kept) and re-arm the tickets.
+__global__ void k_batch_compact(View V) {
+    if (threadIdx.x != 0) return;
+    const int cnt = *V.count;
+    int k = 0;
+    for (int t = 0; t < cnt; ++t) {
+        const int i = V.list[t];
+        if (!V.ctl[i].finished) V.list[k++] = i;
+    }
+    *V.count = k;
+    *V.head = 0;
+    *V.launches += 1;
+}
+
+// ------------------------------------------------------------------ setup
+
+// Sum of squares (double-double) of v over the workgroup.
+template <int BT>
+__device__ double wg_norm(const double* v, int n, Shared<BT>& sh) {
+    dd acc[1] = {dd_zero()};
+    for (int j = threadIdx.x; j < n; j += BT) acc[0] = dd_add(acc[0], dd_prod(v[j], v[j]));
+    wg_dd_sum<1, BT>(acc, sh.red, sh.tot);
+    return sqrt(dd_val(sh.tot[0]));
+}
+
+// Row measure: max |a| (what 0) or sum |a| (what 1), ascending index order.
+__device__ __forceinline__ double row_measure(const int* rp, const double* val, int r, int what) {
+    double acc = 0.0;
+    for (int p = rp[r]; p < rp[r + 1]; ++p) {
+        const double a = fabs(val[p]);
+        acc = what == 0 ? amax(acc, a) : acc + a;
+    }
+    return acc;
+}
+
+// Scaling (10 Ruiz passes, one Pock-Chambolle pass; powers of two), scaled
+// vectors, start point, power iteration, primal weight, FP32 copies and the
+// evaluation of the start point, for one instance.
+template <int BT>
+__device__ void wg_setup(const View& V, int i, Shared<BT>& sh) {
+    const Inst I = V.inst[i];
+    const int m = I.m, n = I.n, tid = threadIdx.x;
+    const Local<double> D = local(V, V.d, I);
+    double* aval = D.aval;
+    double* tval = D.tval;
+    double* R = V.R + I.yo;
+    double* C = V.C + I.xo;
+    double* fr = D.ya;   // scratch until the start point is written
+    double* fc = D.xa;
+    for (int r = tid; r < m; r += BT) R[r] = 1.0;
+    for (int j = tid; j < n; j += BT) C[j] = 1.0;
+    __syncthreads();
+    for (int pass = 0; pass < 11; ++pass) {
+        const int what = pass < 10 ? 0 : 1;
+        if (m > 0) {
+            for (int r = tid; r < m; r += BT) {
+                const double g = pow2_inv_sqrt(row_measure(D.arow, aval, r, what));
+                fr[r] = g;
+                R[r] *= g;
+            }
+            __syncthreads();
+            for (int r = tid; r < m; r += BT)
+                for (int p = D.arow[r]; p < D.arow[r + 1]; ++p) aval[p] = aval[p] * (fr[r] * 1.0);
+        }
+        if (n > 0) {
+            for (int j = tid; j < n; j += BT)
+                for (int p = D.trow[j]; p < D.trow[j + 1]; ++p) tval[p] = tval[p] * (1.0 * fr[D.tcol[p]]);
+            __syncthreads();
+            for (int j = tid; j < n; j += BT) {
+                const double g = pow2_inv_sqrt(row_measure(D.trow, tval, j, what));
+                fc[j] = g;
+                C[j] *= g;
+            }
+            __syncthreads();
+            for (int j = tid; j < n; j += BT)
+                for (int p = D.trow[j]; p < D.trow[j + 1]; ++p) tval[p] = tval[p] * (fc[j] * 1.0);
+        }
+        if (m > 0)
+            for (int r = tid; r < m; r += BT)
+                for (int p = D.arow[r]; p < D.arow[r + 1]; ++p) aval[p] = aval[p] * (1.0 * fc[D.acol[p]]);
+        __syncthreads();
+    }
+
+    // Scaled vectors (absent bounds as +-inf) and the norms.
+    const double* cost = V.cost + I.xo;
+    const double* collo = V.collo + I.xo;
+    const double* colup = V.colup + I.xo;
+    const double* rowlo = V.rowlo + I.yo;
+    const double* rowup = V.rowup + I.yo;
+    dd acc[4] = {dd_zero(), dd_zero(), dd_zero(), dd_zero()};   // caller |c|^2, |b|^2; scaled |c|^2, |b|^2
+    for (int j = tid; j < n; j += BT) {
+        const double cj = cost[j] * C[j];
+        D.c[j] = cj;
+        D.lo[j] = finite_bound(collo[j]) ? collo[j] / C[j] : -INFINITY;
+        D.up[j] = finite_bound(colup[j]) ? colup[j] / C[j] : INFINITY;
+        acc[0] = dd_add(acc[0], dd_prod(cost[j], cost[j]));
+        acc[2] = dd_add(acc[2], dd_prod(cj, cj));
+    }
+    for (int r = tid; r < m; r += BT) {
+        const double rl = finite_bound(rowlo[r]) ? rowlo[r] * R[r] : -INFINITY;
+        const double ru = finite_bound(rowup[r]) ? rowup[r] * R[r] : INFINITY;
+        D.rl[r] = rl;
+        D.ru[r] = ru;
+        double b = 0.0;
+        if (finite_bound(rowlo[r])) b = fabs(rowlo[r]);
+        if (finite_bound(rowup[r])) b = amax(b, fabs(rowup[r]));
+        acc[1] = dd_add(acc[1], dd_prod(b, b));
+        double bs = 0.0;
+        if (isfinite(rl)) bs = fabs(rl);
+        if (isfinite(ru)) bs = amax(bs, fabs(ru));
+        acc[3] = dd_add(acc[3], dd_prod(bs, bs));
+    }
+    wg_dd_sum<4, BT>(acc, sh.red, sh.tot);
+    const double cnorm_o = sqrt(dd_val(sh.tot[0])), bnorm_o = sqrt(dd_val(sh.tot[1]));
+    const double cnorm = sqrt(dd_val(sh.tot[2])), bnorm = sqrt(dd_val(sh.tot[3]));
+    __syncthreads();
+
+    // Power iteration on A'^T A' (u in xbar, v in yp as scratch).
+    double lam = 0.0;
+    if (n > 0 && m > 0 && I.nnz > 0) {
+        double* u = D.xbar;
+        double* v = D.yp;
+        for (int j = tid; j < n; j += BT)
+            u[j] = 1.0 + static_cast<double>((static_cast<unsigned>(j) * 2654435761u) % 1000u) / 1000.0;
+        __syncthreads();
+        const double nu = wg_norm<BT>(u, n, sh);
+        if (nu > 0) {
+            for (int j = tid; j < n; j += BT) u[j] = u[j] / nu;
+            __syncthreads();
+        }
+        for (int it = 0; it < 200; ++it) {
+            for (int r = tid; r < m; r += BT) v[r] = row_dot<double>(D.arow, D.acol, aval, u, r);
+            __syncthreads();
+            for (int j = tid; j < n; j += BT) u[j] = row_dot<double>(D.trow, D.tcol, tval, v, j);
+            __syncthreads();
+            const double nw = wg_norm<BT>(u, n, sh);
+            if (!(nw > 0.0) || !isfinite(nw)) break;
+            for (int j = tid; j < n; j += BT) u[j] = u[j] / nw;
+            __syncthreads();
+            const double prev = lam;
+            lam = nw;
+            if (it > 0 && fabs(lam - prev) <= 1e-7 * lam) break;
+        }
+    }
+    const double smax = lam > 0.0 ? sqrt(lam) : 1.0;
+    const double eta = 0.998 / smax;
+
+    // Start point: 0 clamped into the bounds; y = 0.
+    for (int j = tid; j < n; j += BT) {
+        double v = 0.0;
+        v = (D.lo[j] < v) ? v : D.lo[j];
+        v = (v < D.up[j]) ? v : D.up[j];
+        D.x[j] = v;
+        D.xa[j] = v;
+        D.xp[j] = v;
+        D.xbar[j] = v;
+    }
+    for (int r = tid; r < m; r += BT) {
+        D.y[r] = 0.0;
+        D.ya[r] = 0.0;
+        D.yp[r] = 0.0;
+    }
+    if (tid == 0) {
+        PdlpState h{};
+        h.omega = (cnorm > 1e-10 && bnorm > 1e-10) ? cnorm / bnorm : 1.0;
+        h.eta = eta;
+        h.tau = eta / h.omega;
+        h.sigma = eta * h.omega;
+        h.tol = V.mixed ? V.tol : (V.polish_tol > 0 ? fmin(V.tol, V.polish_tol) : V.tol);
+        h.bnorm_o = bnorm_o;
+        h.cnorm_o = cnorm_o;
+        h.offset = V.offset[i];
+        h.max_iters = V.max_iters;
+        h.eval_freq = V.eval_freq;
+        h.watch_stall = V.mixed ? 1 : 0;
+        sh.s = h;
+        Ctl c{};
+        c.fp32 = V.mixed;
+        c.fp64_at = -1;
+        sh.ctl = c;
+    }
+    __syncthreads();
+
+    if (V.mixed) {
+        const Local<float> F = local(V, V.f, I);
+        for (int p = tid; p < I.nnz; p += BT) {
+            F.aval[p] = static_cast<float>(aval[p]);
+            F.tval[p] = static_cast<float>(tval[p]);
+        }
+        for (int j = tid; j < n; j += BT) {
+            F.c[j] = static_cast<float>(D.c[j]);
+            F.lo[j] = static_cast<float>(D.lo[j]);
+            F.up[j] = static_cast<float>(D.up[j]);
+            const float xv = static_cast<float>(D.x[j]);
+            F.x[j] = xv;
+            F.xa[j] = xv;
+            F.xp[j] = xv;
+            F.xbar[j] = xv;
+        }
+        for (int r = tid; r < m; r += BT) {
+            F.rl[r] = static_cast<float>(D.rl[r]);
+            F.ru[r] = static_cast<float>(D.ru[r]);
+            const float yv = static_cast<float>(D.y[r]);
+            F.y[r] = yv;
+            F.ya[r] = yv;
+            F.yp[r] = yv;
+        }
+        __syncthreads();
+        wg_eval<float, BT>(V, I, V.f, sh, 1);
+    } else {
+        wg_eval<double, BT>(V, I, V.d, sh, 1);
+    }
+    // The single-LP setup's handling of a start point that already meets the
+    // tolerance (finish, or polish from it in FP64).
+    if (sh.s.done) {
+        const int st = sh.s.status;
+        const bool fin = st != 0 || !(V.mixed && V.polish_tol > 0);
+        if (threadIdx.x == 0) {
+            sh.ctl.finished = fin ? 1 : 0;
+            sh.ctl.status = st;
+        }
+        __syncthreads();
+        if (!fin && V.mixed) {
+            if (threadIdx.x == 0) sh.ctl.polish_done = 1;
+            __syncthreads();
+            wg_switch_to_fp64<BT>(V, I, sh, V.polish_tol);
+            if (threadIdx.x == 0 && sh.s.done) {
+                sh.ctl.finished = 1;
+                sh.ctl.status = sh.s.status;
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0) {
+        V.st[i] = sh.s;
+        V.ctl[i] = sh.ctl;
+    }
+    __syncthreads();
+}
+
+template <int BT>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_batch_setup(View V) {
+    __shared__ Shared<BT> sh;
+    for (;;) {
+        const int i = wg_pop<BT>(V, sh);
+        if (i < 0) return;
+        wg_setup<BT>(V, i, sh);
+    }
+}
+
+// ------------------------------------------------------------------ host
+
+template <typename T>
+struct HostVec {
+    DevArray<T> x, xbar, xa, xp, y, ya, yp, c, lo, up, rl, ru, aval, tval;
+    void alloc(size_t nx, size_t ny, size_t nz) {
+        for (auto* a : {&x, &xbar, &xa, &xp, &c, &lo, &up}) a->alloc(nx);
+        for (auto* a : {&y, &ya, &yp, &rl, &ru}) a->alloc(ny);
+        aval.alloc(nz);
+        tval.alloc(nz);
+    }
+    Vec<T> view() {
+        return Vec<T>{x.p, xbar.p, xa.p, xp.p, y.p, ya.p, yp.p, c.p, lo.p, up.p, rl.p, ru.p, aval.p, tval.p};
+    }
+    // Arrays the iteration changes (autotune snapshots).
+    std::vector<DevArray<T>*> mutable_arrays() { return {&x, &xbar, &xa, &xp, &y, &ya, &yp}; }
+};
+
+}  // namespace batch
+}  // namespace gpu
+}  // namespace nir
+
+using namespace nir::gpu;
+using namespace nir::gpu::batch;
+
+struct nir_batch {
+    nir_ctx* ctx = nullptr;
+    hipStream_t s = nullptr;
+    nir_params prm{};
+    nir_batch_opts opts{};
+    int count = 0;
+    std::vector<Inst> h_inst;
+    std::vector<std::vector<double>> h_cost;   // for the objective of the result
+    std::vector<double> h_offset;
+    DevArray<Inst> inst;
+    DevArray<int> arow, acol, trow, tcol;
+    HostVec<double> d;
+    HostVec<float> f;
+    DevArray<double> R, C, hs, cost, collo, colup, rowlo, rowup, offset;
+    DevArray<PdlpState> st;
+    DevArray<Ctl> ctl;
+    DevArray<int> list, count_d, head;
+    DevArray<long long> launches;
+    int* h_poll = nullptr;   // pinned: list length after the last graph replay (two slots)
+    hipEvent_t poll_ev[2] = {nullptr, nullptr};
+    int threads = 512, workgroups = 0;
+    bool solved = false, stopped = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double setup_s = 0.0, solve_s = 0.0, total_nnz = 0.0;
+    long long total_x = 0, total_y = 0, total_nz = 0;
+
+    View view() {
+        View V{};
+        V.inst = inst.p;
+        V.arow = arow.p;
+        V.acol = acol.p;
+        V.trow = trow.p;
+        V.tcol = tcol.p;
+        V.d = d.view();
+        V.f = prm.precision == NIR_PREC_MIXED ? f.view() : Vec<float>{};
+        V.R = R.p;
+        V.C = C.p;
+        V.hs = hs.p;
+        V.cost = cost.p;
+        V.collo = collo.p;
+        V.colup = colup.p;
+        V.rowlo = rowlo.p;
+        V.rowup = rowup.p;
+        V.offset = offset.p;
+        V.st = st.p;
+        V.ctl = ctl.p;
+        V.list = list.p;
+        V.count = count_d.p;
+        V.head = head.p;
+        V.launches = launches.p;
+        V.mixed = prm.precision == NIR_PREC_MIXED ? 1 : 0;
+        V.eval_freq = std::max(1, prm.eval_freq);
+        V.quantum = opts.quantum > 0 ? opts.quantum : 16;
+        V.tol = prm.tol > 0 ? prm.tol : 1e-6;
+        V.polish_tol = prm.polish_tol;
+        V.max_iters = std::max<long long>(1, prm.max_iters);
+        return V;
+    }
+    ~nir_batch() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto& e : poll_ev)
+            if (e) (void)hipEventDestroy(e);
+        if (h_poll) (void)hipHostFree(h_poll);
+    }
+};
+
+namespace {
+
+void launch_run(nir_batch* B, const View& V, int threads) {
+    switch (threads) {
+        case 256: k_batch_run<256><<<B->workgroups, 256, 0, B->s>>>(V); break;
+        case 1024: k_batch_run<1024><<<B->workgroups, 1024, 0, B->s>>>(V); break;
+        default: k_batch_run<512><<<B->workgroups, 512, 0, B->s>>>(V); break;
+    }
+    k_batch_compact<<<1, 64, 0, B->s>>>(V);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_setup(nir_batch* B, const View& V, int threads) {
+    switch (threads) {
+        case 256: k_batch_setup<256><<<B->workgroups, 256, 0, B->s>>>(V); break;
+        case 1024: k_batch_setup<1024><<<B->workgroups, 1024, 0, B->s>>>(V); break;
+        default: k_batch_setup<512><<<B->workgroups, 512, 0, B->s>>>(V); break;
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+int compute_units(int device) {
+    int cu = 0;
+    HIP_CHECK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device));
+    return std::max(1, cu);
+}
+
+// Autotuner: time one quantum-1 launch of every workgroup size from the
+// post-setup state, restoring the state after each; keep the fastest. The
+// choice never changes results (each instance's arithmetic is the same).
+int autotune_threads(nir_batch* B, int cu) {
+    std::vector<int> cands = {256, 512, 1024};
+    View V = B->view();
+    V.quantum = 1;
+    std::vector<DevArray<double>*> dm = B->d.mutable_arrays();
+    std::vector<DevArray<float>*> fm;
+    if (V.mixed) fm = B->f.mutable_arrays();
+    // Snapshot.
+    std::vector<std::unique_ptr<DevArray<double>>> sd;
+    std::vector<std::unique_ptr<DevArray<float>>> sf;
+    auto copy = [&](void* dst, const void* src, size_t bytes) {
+        if (bytes) HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, B->s));
+    };
+    for (auto* a : dm) {
+        sd.emplace_back(new DevArray<double>());
+        sd.back()->alloc(a->n);
+        copy(sd.back()->p, a->p, a->n * sizeof(double));
+    }
+    for (auto* a : fm) {
+        sf.emplace_back(new DevArray<float>());
+        sf.back()->alloc(a->n);
+        copy(sf.back()->p, a->p, a->n * sizeof(float));
+    }
+    DevArray<PdlpState> st0;
+    DevArray<Ctl> ctl0;
+    DevArray<int> list0, cnt0;
+    st0.alloc(B->count);
+    ctl0.alloc(B->count);
+    list0.alloc(B->count);
+    cnt0.alloc(1);
+    copy(st0.p, B->st.p, B->count * sizeof(PdlpState));
+    copy(ctl0.p, B->ctl.p, B->count * sizeof(Ctl));
+    copy(list0.p, B->list.p, B->count * sizeof(int));
+    copy(cnt0.p, B->count_d.p, sizeof(int));
+    auto restore = [&]() {
+        for (size_t k = 0; k < dm.size(); ++k) copy(dm[k]->p, sd[k]->p, dm[k]->n * sizeof(double));
+        for (size_t k = 0; k < fm.size(); ++k) copy(fm[k]->p, sf[k]->p, fm[k]->n * sizeof(float));
+        copy(B->st.p, st0.p, B->count * sizeof(PdlpState));
+        copy(B->ctl.p, ctl0.p, B->count * sizeof(Ctl));
+        copy(B->list.p, list0.p, B->count * sizeof(int));
+        copy(B->count_d.p, cnt0.p, sizeof(int));
+        const int zero = 0;
+        HIP_CHECK(hipMemcpyAsync(B->head.p, &zero, sizeof(int), hipMemcpyHostToDevice, B->s));
+        HIP_CHECK(hipStreamSynchronize(B->s));
+    };
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    int best = 512;
+    float best_ms = 1e30f;
+    const int saved_wg = B->workgroups;
+    for (int t : cands) {
+        B->workgroups = std::max(1, std::min(B->count, cu * (2048 / t)));
+        HIP_CHECK(hipEventRecord(e0, B->s));
+        launch_run(B, V, t);
+        HIP_CHECK(hipEventRecord(e1, B->s));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+        restore();
+    }
+    B->workgroups = saved_wg;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+}
+
+void build_graph(nir_batch* B, int rounds) {
+    const View V = B->view();
+    HIP_CHECK(hipStreamBeginCapture(B->s, hipStreamCaptureModeThreadLocal));
+    for (int r = 0; r < rounds; ++r) launch_run(B, V, B->threads);
+    HIP_CHECK(hipStreamEndCapture(B->s, &B->graph));
+    HIP_CHECK(hipGraphInstantiate(&B->gexec, B->graph, nullptr, nullptr, 0));
+}
+
+}  // namespace
+
+extern "C" {
+
+NIR_API void nir_batch_opts_default(nir_batch_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+}
+
+NIR_API int nir_batch_create(nir_ctx* ctx, const nir_lp* const* lps, int32_t count, const nir_params* p,
+                             const nir_batch_opts* o, nir_batch** out) {
+    if (!ctx || !out) return -1;
+    *out = nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    auto B = std::make_unique<nir_batch>();
+    try {
+        if (count <= 0 || !lps) throw std::invalid_argument("empty batch");
+        if (!p) throw std::invalid_argument("batch needs parameters");
+        B->prm = *p;
+        if (o) B->opts = *o;
+        if (B->prm.mode != NIR_MODE_PDLP) throw std::invalid_argument("batched solves run pdlp mode only");
+        if (B->prm.x0 || B->prm.y0 || B->prm.unscale_row || B->prm.unscale_col)
+            throw std::invalid_argument("batched solves take no warm start or unscaling vectors");
+        if (B->opts.threads != 0 && B->opts.threads != 256 && B->opts.threads != 512 && B->opts.threads != 1024)
+            throw std::invalid_argument("batch threads must be 0, 256, 512 or 1024");
+        if (B->opts.quantum < 0 || B->opts.rounds_per_graph < 0 || B->opts.workgroups < 0)
+            throw std::invalid_argument("negative batch option");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        B->ctx = ctx;
+        B->s = ctx->stream;
+        B->count = count;
+        const bool mixed = B->prm.precision == NIR_PREC_MIXED;
+
+        // Host layout: offsets, CSR(A) by counting transpose, CSR(A^T) = CSC.
+        B->h_inst.resize(count);
+        B->h_cost.resize(count);
+        B->h_offset.resize(count);
+        long long xo = 0, yo = 0, nz = 0, arp = 0, trp = 0;
+        for (int i = 0; i < count; ++i) {
+            const nir_lp* lp = lps[i];
+            check_lp(lp);
+            Inst& I = B->h_inst[i];
+            I.m = lp->nrow;
+            I.n = lp->ncol;
+            I.nnz = lp->ncol > 0 ? static_cast<int>(lp->col_start[lp->ncol]) : 0;
+            I.pad = 0;
+            I.arp = arp;
+            I.trp = trp;
+            I.nz = nz;
+            I.xo = xo;
+            I.yo = yo;
+            arp += I.m + 1;
+            trp += I.n + 1;
+            nz += I.nnz;
+            xo += I.n;
+            yo += I.m;
+            B->h_cost[i].assign(lp->cost, lp->cost + lp->ncol);
+            B->h_offset[i] = lp->obj_offset;
+            B->total_nnz += I.nnz;
+        }
+        B->total_x = xo;
+        B->total_y = yo;
+        B->total_nz = nz;
+        std::vector<int> arow(arp), acol(nz), trow(trp), tcol(nz);
+        std::vector<double> aval(nz), tval(nz), cost(xo), collo(xo), colup(xo), rowlo(yo), rowup(yo);
+        {
+            std::vector<int> ptr, col;
+            std::vector<double> val;
+            for (int i = 0; i < count; ++i) {
+                const nir_lp* lp = lps[i];
+                const Inst& I = B->h_inst[i];
+                csr_from_csc(lp, ptr, col, val);
+                std::copy(ptr.begin(), ptr.end(), arow.begin() + I.arp);
+                std::copy(col.begin(), col.end(), acol.begin() + I.nz);
+                std::copy(val.begin(), val.end(), aval.begin() + I.nz);
+                for (int j = 0; j <= I.n; ++j) trow[I.trp + j] = I.n > 0 ? static_cast<int>(lp->col_start[j]) : 0;
+                std::copy(lp->row_index, lp->row_index + I.nnz, tcol.begin() + I.nz);
+                std::copy(lp->value, lp->value + I.nnz, tval.begin() + I.nz);
+                std::copy(lp->cost, lp->cost + I.n, cost.begin() + I.xo);
+                std::copy(lp->collo, lp->collo + I.n, collo.begin() + I.xo);
+                std::copy(lp->colup, lp->colup + I.n, colup.begin() + I.xo);
+                std::copy(lp->rowlo, lp->rowlo + I.m, rowlo.begin() + I.yo);
+                std::copy(lp->rowup, lp->rowup + I.m, rowup.begin() + I.yo);
+            }
+        }
+        hipStream_t s = B->s;
+        auto up_i = [&](DevArray<int>& a, const std::vector<int>& v) { a.alloc(v.size()); a.upload(v.data(), v.size(), s); };
+        auto up_d = [&](DevArray<double>& a, const std::vector<double>& v) {
+            a.alloc(v.size());
+            a.upload(v.data(), v.size(), s);
+        };
+        B->inst.alloc(count);
+        B->inst.upload(B->h_inst.data(), count, s);
+        up_i(B->arow, arow);
+        up_i(B->acol, acol);
+        up_i(B->trow, trow);
+        up_i(B->tcol, tcol);
+        up_d(B->cost, cost);
+        up_d(B->collo, collo);
+        up_d(B->colup, colup);
+        up_d(B->rowlo, rowlo);
+        up_d(B->rowup, rowup);
+        up_d(B->offset, B->h_offset);
+        B->d.alloc(xo, yo, nz);
+        B->d.aval.upload(aval.data(), nz, s);
+        B->d.tval.upload(tval.data(), nz, s);
+        if (mixed) B->f.alloc(xo, yo, nz);
+        B->R.alloc(yo);
+        B->C.alloc(xo);
+        B->hs.alloc(yo);
+        B->st.alloc(count);
+        B->ctl.alloc(count);
+        // Work list: largest instances first.
+        std::vector<int> order(count);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            const Inst& A = B->h_inst[a];
+            const Inst& Bb = B->h_inst[b];
+            return static_cast<long long>(A.nnz) + A.m + A.n > static_cast<long long>(Bb.nnz) + Bb.m + Bb.n;
+        });
+        up_i(B->list, order);
+        up_i(B->count_d, std::vector<int>{count});
+        up_i(B->head, std::vector<int>{0});
+        B->launches.alloc(1);
+        {
+            const long long zero = 0;
+            B->launches.upload(&zero, 1, s);
+        }
+        HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&B->h_poll), 2 * sizeof(int), hipHostMallocDefault));
+        B->h_poll[0] = B->h_poll[1] = count;
+        for (auto& e : B->poll_ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+
+        const int cu = compute_units(ctx->device);
+        // Setup on 512-lane workgroups, then the compaction of the list.
+        B->workgroups = B->opts.workgroups > 0 ? B->opts.workgroups : std::max(1, std::min(count, 4 * cu));
+        View V = B->view();
+        launch_setup(B.get(), V, 512);
+        k_batch_compact<<<1, 64, 0, s>>>(V);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(s));
+        B->threads = B->opts.threads > 0 ? B->opts.threads : autotune_threads(B.get(), cu);
+        if (B->opts.workgroups <= 0) B->workgroups = std::max(1, std::min(count, cu * (2048 / B->threads)));
+        {
+            const long long zero = 0;
+            B->launches.upload(&zero, 1, s);
+        }
+        HIP_CHECK(hipStreamSynchronize(s));
+        B->setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        *out = B.release();
+        return 0;
+    } catch (const std::exception& ex) {
+        ctx->error = ex.what();
+        return -1;
+    }
+}
+
+NIR_API int nir_batch_solve(nir_batch* B) {
+    if (!B) return -1;
+    try {
+        HIP_CHECK(hipSetDevice(B->ctx->device));
+        auto t0 = std::chrono::steady_clock::now();
+        const int rounds = B->opts.rounds_per_graph > 0 ? B->opts.rounds_per_graph : 4;
+        const View V = B->view();
+        long long serial = 0;
+        int pending[2] = {0, 0};
+        HIP_CHECK(hipMemcpy(&B->h_poll[0], B->count_d.p, sizeof(int), hipMemcpyDeviceToHost));
+        int remaining = B->h_poll[0];
+        // Every launch runs `quantum` blocks of each listed instance and the
+        // iteration cap bounds every instance, so the list must be empty
+        // after this many replays (two more for the one in flight).
+        const View V0 = B->view();
+        const long long launches_bound =
+            (V0.max_iters + static_cast<long long>(V0.eval_freq) * V0.quantum - 1) /
+                (static_cast<long long>(V0.eval_freq) * V0.quantum) + 2;
+        const long long replay_bound = (launches_bound + rounds - 1) / rounds + 2;
+        long long replays = 0;
+        while (remaining > 0) {
+            if (++replays > replay_bound) throw std::runtime_error("batch did not drain within its iteration bound");
+            if (B->prm.should_stop && B->prm.should_stop(B->prm.stop_user)) {
+                B->stopped = true;
+                break;
+            }
+            if (B->prm.use_graph) {
+                if (!B->gexec) build_graph(B, rounds);
+                HIP_CHECK(hipGraphLaunch(B->gexec, B->s));
+            } else {
+                for (int r = 0; r < rounds; ++r) launch_run(B, V, B->threads);
+            }
+            const int slot = static_cast<int>(serial++ & 1);
+            HIP_CHECK(hipMemcpyAsync(&B->h_poll[slot], B->count_d.p, sizeof(int), hipMemcpyDeviceToHost, B->s));
+            HIP_CHECK(hipEventRecord(B->poll_ev[slot], B->s));
+            pending[slot] = 1;
+            // Keep one replay in flight: wait for the previous one.
+            const int prev = slot ^ 1;
+            if (pending[prev]) {
+                HIP_CHECK(hipEventSynchronize(B->poll_ev[prev]));
+                pending[prev] = 0;
+                remaining = B->h_poll[prev];
+            }
+        }
+        HIP_CHECK(hipStreamSynchronize(B->s));
+        B->solved = true;
+        B->solve_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return 0;
+    } catch (const std::exception& ex) {
+        B->ctx->error = ex.what();
+        return -1;
+    }
+}
+
+NIR_API int nir_batch_result(nir_batch* B, int32_t i, double* x, double* y, nir_info* info) {
+    if (!B || i < 0 || i >= B->count) return -1;
+    try {
+        HIP_CHECK(hipSetDevice(B->ctx->device));
+        const Inst& I = B->h_inst[i];
+        PdlpState h{};
+        Ctl c{};
+        HIP_CHECK(hipMemcpy(&h, B->st.p + i, sizeof(h), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(&c, B->ctl.p + i, sizeof(c), hipMemcpyDeviceToHost));
+        std::vector<double> xs(static_cast<size_t>(I.n)), ys(static_cast<size_t>(I.m));
+        if (c.fp32) {
+            std::vector<float> xf(static_cast<size_t>(I.n)), yf(static_cast<size_t>(I.m));
+            if (I.n) HIP_CHECK(hipMemcpy(xf.data(), B->f.xp.p + I.xo, I.n * sizeof(float), hipMemcpyDeviceToHost));
+            if (I.m) HIP_CHECK(hipMemcpy(yf.data(), B->f.yp.p + I.yo, I.m * sizeof(float), hipMemcpyDeviceToHost));
+            for (int j = 0; j < I.n; ++j) xs[j] = xf[j];
+            for (int r = 0; r < I.m; ++r) ys[r] = yf[r];
+        } else {
+            if (I.n) HIP_CHECK(hipMemcpy(xs.data(), B->d.xp.p + I.xo, I.n * sizeof(double), hipMemcpyDeviceToHost));
+            if (I.m) HIP_CHECK(hipMemcpy(ys.data(), B->d.yp.p + I.yo, I.m * sizeof(double), hipMemcpyDeviceToHost));
+        }
+        std::vector<double> R(static_cast<size_t>(I.m)), Cc(static_cast<size_t>(I.n));
+        if (I.m) HIP_CHECK(hipMemcpy(R.data(), B->R.p + I.yo, I.m * sizeof(double), hipMemcpyDeviceToHost));
+        if (I.n) HIP_CHECK(hipMemcpy(Cc.data(), B->C.p + I.xo, I.n * sizeof(double), hipMemcpyDeviceToHost));
+        for (int j = 0; j < I.n; ++j) xs[j] *= Cc[j];
+        for (int r = 0; r < I.m; ++r) ys[r] *= R[r];
+        if (x) std::copy(xs.begin(), xs.end(), x);
+        if (y) std::copy(ys.begin(), ys.end(), y);
+        if (info) {
+            std::memset(info, 0, sizeof(*info));
+            info->status = c.finished ? c.status : (B->stopped ? 4 : 6);
+            info->iterations = h.total;
+            info->polish_iterations = c.fp64_at >= 0 ? h.total - c.fp64_at : 0;
+            info->restarts = h.restarts;
+            DDHost acc(B->h_offset[i]);
+            const std::vector<double>& cost = B->h_cost[i];
+            for (int j = 0; j < I.n; ++j)
+                if (cost[j] != 0.0 && xs[j] != 0.0) acc.add_prod(cost[j], xs[j]);
+            info->objective = acc.value();
+            info->tau = h.tau;
+            info->sigma = h.sigma;
+            info->primal_weight = h.omega;
+            info->rel_primal = h.rel_p;
+            info->rel_dual = h.rel_d;
+            info->rel_gap = h.rel_g;
+            info->setup_seconds = B->setup_s;
+            info->solve_seconds = B->solve_s;
+        }
+        return 0;
+    } catch (const std::exception& ex) {
+        B->ctx->error = ex.what();
+        return -1;
+    }
+}
+
+NIR_API int nir_batch_get_stats(nir_batch* B, nir_batch_stats* out) {
+    if (!B || !out) return -1;
+    try {
+        HIP_CHECK(hipSetDevice(B->ctx->device));
+        std::memset(out, 0, sizeof(*out));
+        out->count = B->count;
+        out->threads = B->threads;
+        out->workgroups = B->workgroups;
+        out->quantum = B->opts.quantum > 0 ? B->opts.quantum : 16;
+        long long launches = 0;
+        HIP_CHECK(hipMemcpy(&launches, B->launches.p, sizeof(launches), hipMemcpyDeviceToHost));
+        out->launches = launches;
+        std::vector<PdlpState> h(static_cast<size_t>(B->count));
+        HIP_CHECK(hipMemcpy(h.data(), B->st.p, B->count * sizeof(PdlpState), hipMemcpyDeviceToHost));
+        double nzit = 0.0;
+        long long it = 0;
+        for (int i = 0; i < B->count; ++i) {
+            it += h[i].total;
+            nzit += static_cast<double>(h[i].total) * B->h_inst[i].nnz;
+        }
+        out->total_iterations = it;
+        out->nnz_iterations = nzit;
+        out->total_nnz = B->total_nnz;
+        out->setup_seconds = B->setup_s;
+        out->solve_seconds = B->solve_s;
+        return 0;
+    } catch (const std::exception& ex) {
+        B->ctx->error = ex.what();
+        return -1;
+    }
+}
+
+NIR_API void nir_batch_destroy(nir_batch* B) {
+    if (!B) return;
+    (void)hipSetDevice(B->ctx->device);
+    delete B;
+}
+
+}  // extern "C"

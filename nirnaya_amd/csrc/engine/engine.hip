@@ -32,6 +32,8 @@
 
 #include "../../../include/nirnaya/pdhg.h"
 #include "devmat.hpp"
+#include "ctx.hpp"
+#include "lpcheck.hpp"
 #include "pdlp.hpp"
 
 namespace nir {
@@ -228,11 +230,6 @@ using namespace nir::gpu;
 
 static thread_local std::string g_last_error;
 
-struct nir_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string error;
-};
 
 struct nir_session {
     nir_ctx* ctx = nullptr;
@@ -429,30 +426,14 @@ void reference_step_sizes(nir_session* S) {
     S->sigma = 0.85 / std::max(1e-6, smax);
 }
 
-void check_lp(const nir_lp* lp) {
-    if (!lp) throw std::invalid_argument("null LP");
-    if (lp->nrow < 0 || lp->ncol < 0) throw std::invalid_argument("negative dimensions");
-    if (lp->ncol > 0 && (!lp->col_start || !lp->cost || !lp->collo || !lp->colup))
-        throw std::invalid_argument("missing column arrays");
-    if (lp->nrow > 0 && (!lp->rowlo || !lp->rowup)) throw std::invalid_argument("missing row arrays");
-    const int64_t nnz = lp->ncol > 0 ? lp->col_start[lp->ncol] : 0;
-    if (lp->ncol > 0 && lp->col_start[0] != 0) throw std::invalid_argument("col_start[0] != 0");
-    if (nnz < 0 || nnz >= (int64_t(1) << 31) - 1) throw std::invalid_argument("nnz out of int32 range");
-    if (nnz > 0 && (!lp->row_index || !lp->value)) throw std::invalid_argument("missing matrix arrays");
-    for (int32_t j = 0; j < lp->ncol; ++j)
-        if (lp->col_start[j + 1] < lp->col_start[j]) throw std::invalid_argument("col_start not non-decreasing");
-    // Every index is range-checked here: the kernels trust them.
-    for (int64_t p = 0; p < nnz; ++p)
-        if (lp->row_index[p] < 0 || lp->row_index[p] >= lp->nrow) throw std::invalid_argument("row index out of range");
-}
-
-TileShape shape_from_params(const nir_params& p) {
+TileShape shape_from_params(const nir_params& p, bool transpose) {
     TileShape t;
     if (p.tile_nnz == 1024 || p.tile_nnz == 2048 || p.tile_nnz == 4096) t.tile = p.tile_nnz;
     else if (p.tile_nnz != 0 && p.tile_nnz != -1) throw std::invalid_argument("tile_nnz must be -1, 0, 1024, 2048 or 4096");
     if (p.tile_rows > 0) t.rows = std::min(p.tile_rows, kThreads);
     if (p.seg_width < 0) throw std::invalid_argument("seg_width must be >= 0");
-    t.seg_width = p.seg_width;
+    if (p.seg_width_t < -1) throw std::invalid_argument("seg_width_t must be >= -1");
+    t.seg_width = (transpose && p.seg_width_t >= 0) ? p.seg_width_t : p.seg_width;
     return t;
 }
 
@@ -473,6 +454,7 @@ NIR_API void nir_params_default(nir_params* p) {
     p->eval_freq = 64;
     p->check_termination = 1;
     p->use_graph = 1;
+    p->seg_width_t = -1;
 }
 
 NIR_API const char* nir_last_error(void) { return g_last_error.c_str(); }
@@ -528,30 +510,20 @@ NIR_API int nir_session_create(nir_ctx* ctx, const nir_lp* lp, const nir_params*
         S->nnz = lp->ncol > 0 ? lp->col_start[lp->ncol] : 0;
         S->obj_offset = lp->obj_offset;
         S->max_iters = std::max<long long>(0, S->prm.max_iters);
-        const TileShape base = shape_from_params(S->prm);
+        const TileShape base = shape_from_params(S->prm, false);
+        const TileShape base_t = shape_from_params(S->prm, true);
 
         // CSR(A^T) is the CSC of A; CSR(A) by a counting transpose that keeps
         // column indices ascending within each row.
         const int m = S->m, n = S->n;
-        const int nnz = static_cast<int>(S->nnz);
         std::vector<int> at_ptr(static_cast<size_t>(n) + 1);
         for (int j = 0; j <= n; ++j) at_ptr[j] = n > 0 ? static_cast<int>(lp->col_start[j]) : 0;
         if (n == 0) at_ptr.assign(1, 0);
-        std::vector<int> a_ptr(static_cast<size_t>(m) + 1, 0), a_col(static_cast<size_t>(nnz));
-        std::vector<double> a_val(static_cast<size_t>(nnz));
-        for (int q = 0; q < nnz; ++q) ++a_ptr[lp->row_index[q] + 1];
-        for (int i = 0; i < m; ++i) a_ptr[i + 1] += a_ptr[i];
-        {
-            std::vector<int> fill(a_ptr.begin(), a_ptr.end() - 1);
-            for (int j = 0; j < n; ++j)
-                for (int q = at_ptr[j]; q < at_ptr[j + 1]; ++q) {
-                    const int r = fill[lp->row_index[q]]++;
-                    a_col[r] = j;
-                    a_val[r] = lp->value[q];
-                }
-        }
+        std::vector<int> a_ptr, a_col;
+        std::vector<double> a_val;
+        csr_from_csc(lp, a_ptr, a_col, a_val);
         const TileShape ta = S->prm.tile_nnz == -1 ? autotune(m, n, a_ptr, a_col.data(), a_val.data(), S->s) : base;
-        const TileShape tt = S->prm.tile_nnz == -1 ? autotune(n, m, at_ptr, lp->row_index, lp->value, S->s) : base;
+        const TileShape tt = S->prm.tile_nnz == -1 ? autotune(n, m, at_ptr, lp->row_index, lp->value, S->s) : base_t;
         upload_matrix(S->A, m, n, a_ptr, a_col.data(), a_val.data(), ta, S->s);
         upload_matrix(S->AT, n, m, at_ptr, lp->row_index, lp->value, tt, S->s);
 
@@ -669,30 +641,50 @@ NIR_API int nir_session_time(nir_session* S, int64_t iters, nir_timing* out) {
         out->spmv_bytes = 12.0 * nnz + 4.0 * (m + 1) + 8.0 * n + 32.0 * m;
         out->spmtv_bytes = 12.0 * nnz + 4.0 * (n + 1) + 8.0 * m + 48.0 * n;
         out->iter_bytes = out->spmv_bytes + out->spmtv_bytes;
-        std::vector<hipEvent_t> ev(static_cast<size_t>(4 * iters + 2));
-        for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+        // Sweep 1, the production path (32-iteration graph replays), between
+        // one pair of events: total_ms.
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
         HIP_CHECK(hipStreamSynchronize(S->s));
-        HIP_CHECK(hipEventRecord(ev[0], S->s));
-        long long k = S->k_next;
-        for (long long t = 0; t < iters; ++t, ++k) {
-            const int k_lo = static_cast<int>(k - S->base_host);
-            HIP_CHECK(hipEventRecord(ev[2 + 4 * t], S->s));
-            launch_primal_ref(S, k_lo);
-            HIP_CHECK(hipEventRecord(ev[3 + 4 * t], S->s));
-            HIP_CHECK(hipEventRecord(ev[4 + 4 * t], S->s));
-            launch_dual_ref(S, k_lo, k);
-            HIP_CHECK(hipEventRecord(ev[5 + 4 * t], S->s));
-        }
-        HIP_CHECK(hipEventRecord(ev[1], S->s));
-        HIP_CHECK(hipEventSynchronize(ev[1]));
+        HIP_CHECK(hipEventRecord(e0, S->s));
+        enqueue_ref_block(S, S->k_next, iters);
+        S->k_next += iters;
+        HIP_CHECK(hipEventRecord(e1, S->s));
+        HIP_CHECK(hipEventSynchronize(e1));
         float ms = 0.f;
-        HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
         out->total_ms = ms;
-        for (long long t = 0; t < iters; ++t) {
-            HIP_CHECK(hipEventElapsedTime(&ms, ev[2 + 4 * t], ev[3 + 4 * t]));
-            out->spmtv_ms += ms;
-            HIP_CHECK(hipEventElapsedTime(&ms, ev[4 + 4 * t], ev[5 + 4 * t]));
-            out->spmv_ms += ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        // Sweep 2, as many iterations launched one by one with an event pair
+        // around each product (all its segment passes): spmv_ms and spmtv_ms
+        // are the summed product durations of this sweep. The events skip the
+        // system-scope fence, which only host visibility of memory needs.
+        const long long chunk = 64;
+        std::vector<hipEvent_t> ev(static_cast<size_t>(4 * chunk));
+        for (auto& e : ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+        long long k = S->k_next;
+        for (long long t0 = 0; t0 < iters; t0 += chunk) {
+            const long long cnt = std::min(chunk, iters - t0);
+            for (long long t = 0; t < cnt; ++t, ++k) {
+                hipEvent_t* e = ev.data() + 4 * t;
+                const int k_lo = static_cast<int>(k - S->base_host);
+                HIP_CHECK(hipEventRecord(e[0], S->s));
+                launch_primal_ref(S, k_lo);
+                HIP_CHECK(hipEventRecord(e[1], S->s));
+                HIP_CHECK(hipEventRecord(e[2], S->s));
+                launch_dual_ref(S, k_lo, k);
+                HIP_CHECK(hipEventRecord(e[3], S->s));
+            }
+            HIP_CHECK(hipStreamSynchronize(S->s));
+            for (long long t = 0; t < cnt; ++t) {
+                hipEvent_t* e = ev.data() + 4 * t;
+                HIP_CHECK(hipEventElapsedTime(&ms, e[0], e[1]));
+                out->spmtv_ms += ms;
+                HIP_CHECK(hipEventElapsedTime(&ms, e[2], e[3]));
+                out->spmv_ms += ms;
+            }
         }
         for (auto& e : ev) HIP_CHECK(hipEventDestroy(e));
         out->iterations = iters;

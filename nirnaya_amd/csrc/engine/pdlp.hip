@@ -45,6 +45,7 @@
 
 #include "devmat.hpp"
 #include "pdlp.hpp"
+#include "pdlp_core.hpp"
 
 namespace nir {
 namespace gpu {
@@ -55,36 +56,7 @@ namespace gpu {
         if (e_ != hipSuccess) throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-// ------------------------------------------------------------------ device dd
-
-struct dd {
-    double hi, lo;
-};
-
-__host__ __device__ __forceinline__ dd dd_zero() { return dd{0.0, 0.0}; }
-
-__host__ __device__ __forceinline__ dd dd_add(dd a, dd b) {
-    double s1 = a.hi + b.hi, v = s1 - a.hi;
-    double s2 = (a.hi - (s1 - v)) + (b.hi - v);
-    double t1 = a.lo + b.lo, w = t1 - a.lo;
-    double t2 = (a.lo - (t1 - w)) + (b.lo - w);
-    s2 += t1;
-    double u = s1 + s2;
-    s2 = s2 - (u - s1);
-    s1 = u;
-    s2 += t2;
-    dd r;
-    r.hi = s1 + s2;
-    r.lo = s2 - (r.hi - s1);
-    return r;
-}
-
-__host__ __device__ __forceinline__ dd dd_prod(double a, double b) {
-    const double p = a * b;
-    return dd{p, fma(a, b, -p)};
-}
-
-__host__ __device__ __forceinline__ double dd_val(dd a) { return a.hi + a.lo; }
+// ------------------------------------------------------------------ reductions
 
 // Reduce NQ dd accumulators over the workgroup (fixed tree), thread 0 writes.
 template <int NQ>
@@ -105,38 +77,6 @@ __device__ __forceinline__ void block_dd_reduce(dd (&acc)[NQ], dd* sh, dd* out) 
         for (int q = 0; q < NQ; ++q) out[q] = sh[q * kThreads];
 }
 
-// 2^-floor((e+1)/2) for v in [2^e, 2^(e+1)); 1 for v == 0 or non-finite.
-__host__ __device__ __forceinline__ double pow2_inv_sqrt(double v) {
-    if (!(v > 0.0) || !isfinite(v)) return 1.0;
-    int e2;
-    (void)frexp(v, &e2);   // v = f 2^e2, f in [0.5, 1)
-    const int e = e2 - 1;
-    const int k = (e + 1) >= 0 ? (e + 1) / 2 : -((-(e + 1) + 1) / 2);   // floor((e+1)/2)
-    return ldexp(1.0, -k);
-}
-
-// ------------------------------------------------------------------ state
-
-struct PdlpState {
-    double omega, tau, sigma, eta;
-    double kkt_anchor, kkt_prev;
-    double rel_p, rel_d, rel_g, pobj, dobj;
-    double best_rel;
-    double tol;
-    double bnorm_o, cnorm_o, offset;
-    long long total;       // iterations completed
-    long long max_iters;
-    int k_base;            // iterations since restart at the start of the block
-    int done;              // 1 = stop iterating
-    int status;            // 0 optimal, 6 iteration limit, 11 numerical trouble, 2 stalled (mixed phase)
-    int restart;           // set by the decision kernel for the copy kernel
-    int restarts;
-    int evals;
-    int since_best;        // evaluations since best_rel last improved by 5%
-    int watch_stall;       // 1 in the FP32 phase
-    int eval_freq;
-};
-
 // ------------------------------------------------------------------ kernels
 
 template <int TILE, typename T>
@@ -148,9 +88,7 @@ __global__ __launch_bounds__(kThreads) void k_primal_pdlp(Pass<T> P, const T* __
     __shared__ T prod[lds_slots(TILE)];
     if (st->done) return;
     const T tau = static_cast<T>(st->tau);
-    const int k = st->k_base + k_lo;
-    const T w = static_cast<T>(static_cast<double>(k + 1) / static_cast<double>(k + 2));
-    const T wc = static_cast<T>(1.0 / static_cast<double>(k + 2));
+    const Halpern<T> hw = halpern<T>(st->k_base + k_lo);
     struct Ops {
         T x, c, l, u, a;
     };
@@ -160,14 +98,10 @@ __global__ __launch_bounds__(kThreads) void k_primal_pdlp(Pass<T> P, const T* __
         return o;
     };
     stream_rows<TILE>(P, y, prod, blockIdx.x, pre, [&](int j, T g, const Ops& o) {
-        const T xj = o.x;
-        T cand = xj - tau * (o.c - g);
-        cand = (o.l < cand) ? cand : o.l;
-        cand = (cand < o.u) ? cand : o.u;
-        const T xb = T(2) * cand - xj;
-        xbar[j] = xb;
-        x[j] = w * xb + wc * o.a;
-        if (save) xp_out[j] = cand;
+        const PrimalOut<T> r = primal_point<T>(o.x, g, o.c, o.l, o.u, o.a, tau, hw);
+        xbar[j] = r.xb;
+        x[j] = r.next;
+        if (save) xp_out[j] = r.cand;
     });
 }
 
@@ -179,9 +113,7 @@ __global__ __launch_bounds__(kThreads) void k_dual_pdlp(Pass<T> P, const T* __re
     __shared__ T prod[lds_slots(TILE)];
     if (st->done) return;
     const T sigma = static_cast<T>(st->sigma);
-    const int k = st->k_base + k_lo;
-    const T w = static_cast<T>(static_cast<double>(k + 1) / static_cast<double>(k + 2));
-    const T wc = static_cast<T>(1.0 / static_cast<double>(k + 2));
+    const Halpern<T> hw = halpern<T>(st->k_base + k_lo);
     struct Ops {
         T y, l, u, a;
     };
@@ -191,14 +123,9 @@ __global__ __launch_bounds__(kThreads) void k_dual_pdlp(Pass<T> P, const T* __re
         return o;
     };
     stream_rows<TILE>(P, xbar, prod, blockIdx.x, pre, [&](int i, T h, const Ops& o) {
-        const T yi = o.y;
-        T r = h - yi / sigma;
-        r = (o.l < r) ? r : o.l;
-        r = (r < o.u) ? r : o.u;
-        const T yp = yi + sigma * (r - h);
-        const T yb = T(2) * yp - yi;
-        y[i] = w * yb + wc * o.a;
-        if (save) yp_out[i] = yp;
+        const DualOut<T> r = dual_point<T>(o.y, h, o.l, o.u, o.a, sigma, hw);
+        y[i] = r.next;
+        if (save) yp_out[i] = r.yp;
     });
 }
 
@@ -219,23 +146,7 @@ __global__ __launch_bounds__(kThreads) void k_eval_cols(Pass<double> P, const TI
     if (st->done && !force) return;
     dd acc[5] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
     stream_rows<TILE>(P, yp, prod, blockIdx.x, [&](int j, double g) {
-        const double lam = c[j] - g;
-        const bool lf = lo[j] > -INFINITY, uf = up[j] < INFINITY;
-        double lp = lam;
-        if (!lf) lp = amin(lp, 0.0);
-        if (!uf) lp = amax(lp, 0.0);
-        const double rd = lam - lp;
-        acc[0] = dd_prod(rd, rd);
-        const double rdo = rd / cscale[j];
-        acc[1] = dd_prod(rdo, rdo);
-        const double xj = static_cast<double>(xp[j]);
-        acc[2] = dd_prod(c[j], xj);
-        dd dobj = dd_zero();
-        if (lf && lp > 0.0) dobj = dd_prod(lo[j], lp);
-        if (uf && lp < 0.0) dobj = dd_prod(up[j], lp);
-        acc[3] = dobj;
-        const double dx = xj - static_cast<double>(xa[j]);
-        acc[4] = dd_prod(dx, dx);
+        col_terms(g, c[j], lo[j], up[j], cscale[j], static_cast<double>(xp[j]), static_cast<double>(xa[j]), acc);
     });
     if (P.last) block_dd_reduce<5>(acc, sh, part + 5 * blockIdx.x);
 }
@@ -255,27 +166,7 @@ __global__ __launch_bounds__(kThreads) void k_eval_rows(Pass<double> P, const TI
     if (st->done && !force) return;
     dd acc[6] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
     stream_rows<TILE>(P, xp, prod, blockIdx.x, [&](int i, double h) {
-        const bool lf = rl[i] > -INFINITY, uf = ru[i] < INFINITY;
-        double proj = amax(rl[i], h);
-        proj = amin(ru[i], proj);
-        const double rp = h - proj;
-        acc[0] = dd_prod(rp, rp);
-        const double rpo = rp / rscale[i];
-        acc[1] = dd_prod(rpo, rpo);
-        const double yi = static_cast<double>(yp[i]);
-        double ys = yi;
-        if (!lf) ys = amin(ys, 0.0);
-        if (!uf) ys = amax(ys, 0.0);
-        const double yv = yi - ys;
-        acc[2] = dd_prod(yv, yv);
-        const double yvo = yv * rscale[i];
-        acc[3] = dd_prod(yvo, yvo);
-        dd dobj = dd_zero();
-        if (lf && ys > 0.0) dobj = dd_prod(rl[i], ys);
-        if (uf && ys < 0.0) dobj = dd_prod(ru[i], ys);
-        acc[4] = dobj;
-        const double dy = yi - static_cast<double>(ya[i]);
-        acc[5] = dd_prod(dy, dy);
+        row_terms(h, rl[i], ru[i], rscale[i], static_cast<double>(yp[i]), static_cast<double>(ya[i]), acc);
     });
     if (P.last) block_dd_reduce<6>(acc, sh, part + 6 * blockIdx.x);
 }
@@ -298,89 +189,7 @@ __global__ __launch_bounds__(kThreads) void k_decide(PdlpState* st, const dd* __
     block_dd_reduce<11>(acc, sh, tot);
     __syncthreads();
     if (threadIdx.x != 0) return;
-
-    PdlpState& s = *st;
-    const double rd2 = dd_val(dd_add(tot[0], tot[7]));
-    const double rd2o = dd_val(dd_add(tot[1], tot[8]));
-    const double rp2 = dd_val(tot[5]);
-    const double rp2o = dd_val(tot[6]);
-    const double P = dd_val(dd_add(tot[2], dd{s.offset, 0.0}));
-    const double D = dd_val(dd_add(dd_add(tot[3], tot[9]), dd{s.offset, 0.0}));
-    const double dx2 = dd_val(tot[4]);
-    const double dy2 = dd_val(tot[10]);
-    const double gap = fabs(P - D);
-    s.rel_p = sqrt(rp2o) / (1.0 + s.bnorm_o);
-    s.rel_d = sqrt(rd2o) / (1.0 + s.cnorm_o);
-    s.rel_g = gap / (1.0 + fabs(P) + fabs(D));
-    s.pobj = P;
-    s.dobj = D;
-    s.evals += 1;
-    const double worst = amax(s.rel_p, amax(s.rel_d, s.rel_g));
-    if (!isfinite(rd2) || !isfinite(rp2) || !isfinite(P) || !isfinite(D) || !isfinite(dx2) || !isfinite(dy2)) {
-        s.done = 1;
-        s.status = 11;
-        s.restart = 0;
-        return;
-    }
-    const double kkt = sqrt(s.omega * rp2 + rd2 / s.omega + gap * gap);
-    if (mode == 1) {
-        s.kkt_anchor = kkt;
-        s.kkt_prev = kkt;
-        s.best_rel = worst;
-        s.since_best = 0;
-        s.restart = 0;
-        if (worst <= s.tol) {
-            s.done = 1;
-            s.status = 0;
-        }
-        return;
-    }
-    s.total += s.eval_freq;
-    if (worst <= s.tol) {
-        s.done = 1;
-        s.status = 0;
-        s.restart = 0;
-        return;
-    }
-    if (s.watch_stall) {
-        // FP32 phase: hand over to FP64 once the best relative KKT has not
-        // improved by 5% for max(100, evals/3) evaluations.
-        if (worst <= 0.95 * s.best_rel) {
-            s.best_rel = worst;
-            s.since_best = 0;
-        } else if (++s.since_best >= (s.evals / 3 > 100 ? s.evals / 3 : 100)) {
-            s.done = 1;
-            s.status = 2;
-            s.restart = 0;
-            return;
-        }
-    }
-    if (s.total >= s.max_iters) {
-        s.done = 1;
-        s.status = 6;
-        s.restart = 0;
-        return;
-    }
-    const int k_new = s.k_base + s.eval_freq;
-    const bool restart = (kkt <= 0.2 * s.kkt_anchor) || (kkt <= 0.8 * s.kkt_anchor && kkt > s.kkt_prev) ||
-                         (static_cast<double>(k_new) >= 0.36 * static_cast<double>(s.total));
-    s.kkt_prev = kkt;
-    if (restart) {
-        if (dx2 > 1e-20 && dy2 > 1e-20) {
-            const double ratio = sqrt(dy2) / sqrt(dx2);
-            s.omega = sqrt(s.omega * ratio);
-            s.tau = s.eta / s.omega;
-            s.sigma = s.eta * s.omega;
-        }
-        s.kkt_anchor = sqrt(s.omega * rp2 + rd2 / s.omega + gap * gap);
-        s.kkt_prev = s.kkt_anchor;
-        s.k_base = 0;
-        s.restarts += 1;
-        s.restart = 1;
-    } else {
-        s.k_base = k_new;
-        s.restart = 0;
-    }
+    pdlp_decide(*st, tot, mode);
 }
 
 // Restart: iterate and anchor become T(z).

@@ -1,0 +1,263 @@
+// Arithmetic of the restarted PDHG ("pdlp" mode) shared by the single-LP
+// kernels (pdlp.hip) and the batched persistent kernel (batch.hip): the same
+// inline functions produce every iterate, evaluation term and decision in
+// both, so a batched instance follows the single-LP trajectory exactly.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "spmv.hpp"
+
+namespace nir {
+namespace gpu {
+
+// ------------------------------------------------------------------ double-double
+
+struct dd {
+    double hi, lo;
+};
+
+__host__ __device__ __forceinline__ dd dd_zero() { return dd{0.0, 0.0}; }
+
+// Sum of two dd values (two-sum on both parts, then renormalise).
+__host__ __device__ __forceinline__ dd dd_add(dd a, dd b) {
+    double s1 = a.hi + b.hi, v = s1 - a.hi;
+    double s2 = (a.hi - (s1 - v)) + (b.hi - v);
+    double t1 = a.lo + b.lo, w = t1 - a.lo;
+    double t2 = (a.lo - (t1 - w)) + (b.lo - w);
+    s2 += t1;
+    double u = s1 + s2;
+    s2 = s2 - (u - s1);
+    s1 = u;
+    s2 += t2;
+    dd r;
+    r.hi = s1 + s2;
+    r.lo = s2 - (r.hi - s1);
+    return r;
+}
+
+// Exact product a*b as a dd (FMA two-product).
+__host__ __device__ __forceinline__ dd dd_prod(double a, double b) {
+    const double p = a * b;
+    return dd{p, fma(a, b, -p)};
+}
+
+__host__ __device__ __forceinline__ double dd_val(dd a) { return a.hi + a.lo; }
+
+// 2^-floor((e+1)/2) for v in [2^e, 2^(e+1)); 1 for v == 0 or non-finite.
+__host__ __device__ __forceinline__ double pow2_inv_sqrt(double v) {
+    if (!(v > 0.0) || !isfinite(v)) return 1.0;
+    int e2;
+    (void)frexp(v, &e2);   // v = f 2^e2, f in [0.5, 1)
+    const int e = e2 - 1;
+    const int k = (e + 1) >= 0 ? (e + 1) / 2 : -((-(e + 1) + 1) / 2);   // floor((e+1)/2)
+    return ldexp(1.0, -k);
+}
+
+// ------------------------------------------------------------------ state
+
+struct PdlpState {
+    double omega, tau, sigma, eta;
+    double kkt_anchor, kkt_prev;
+    double rel_p, rel_d, rel_g, pobj, dobj;
+    double best_rel;
+    double tol;
+    double bnorm_o, cnorm_o, offset;
+    long long total;       // iterations completed
+    long long max_iters;
+    int k_base;            // iterations since restart at the start of the block
+    int done;              // 1 = stop iterating
+    int status;            // 0 optimal, 6 iteration limit, 11 numerical trouble, 2 stalled (mixed phase)
+    int restart;           // set by the decision for the restart copy
+    int restarts;
+    int evals;
+    int since_best;        // evaluations since best_rel last improved by 5%
+    int watch_stall;       // 1 in the FP32 phase
+    int eval_freq;
+};
+
+// ------------------------------------------------------------------ iteration
+
+// Halpern weights of inner iteration k (iterations since the last restart):
+// z+ = w (2 T(z) - z) + wc z_anchor.
+template <typename T>
+struct Halpern {
+    T w, wc;
+};
+template <typename T>
+__device__ __forceinline__ Halpern<T> halpern(int k) {
+    return Halpern<T>{static_cast<T>(static_cast<double>(k + 1) / static_cast<double>(k + 2)),
+                      static_cast<T>(1.0 / static_cast<double>(k + 2))};
+}
+
+// Projected primal step at column j from g = (A^T y)_j: T(z)_x = cand and the
+// reflected point xb = 2 cand - x (the extrapolated point of the dual step).
+template <typename T>
+struct PrimalOut {
+    T cand, xb, next;
+};
+template <typename T>
+__device__ __forceinline__ PrimalOut<T> primal_point(T xj, T g, T c, T l, T u, T xa, T tau, Halpern<T> h) {
+    T cand = xj - tau * (c - g);
+    cand = (l < cand) ? cand : l;
+    cand = (cand < u) ? cand : u;
+    const T xb = T(2) * cand - xj;
+    return PrimalOut<T>{cand, xb, h.w * xb + h.wc * xa};
+}
+
+// Projected dual step at row i from hx = (A xb)_i.
+template <typename T>
+struct DualOut {
+    T yp, next;
+};
+template <typename T>
+__device__ __forceinline__ DualOut<T> dual_point(T yi, T hx, T l, T u, T ya, T sigma, Halpern<T> h) {
+    T r = hx - yi / sigma;
+    r = (l < r) ? r : l;
+    r = (r < u) ? r : u;
+    const T yp = yi + sigma * (r - hx);
+    const T yb = T(2) * yp - yi;
+    return DualOut<T>{yp, h.w * yb + h.wc * ya};
+}
+
+// ------------------------------------------------------------------ evaluation
+
+// Column terms at T(z) with g = (A^T yp)_j: 0 rd^2, 1 (rd/C)^2, 2 c x,
+// 3 dual-objective column term, 4 (x - x_anchor)^2.
+__device__ __forceinline__ void col_terms(double g, double c, double lo, double up, double cscale, double xj,
+                                          double xaj, dd (&t)[5]) {
+    const double lam = c - g;
+    const bool lf = lo > -INFINITY, uf = up < INFINITY;
+    double lp = lam;
+    if (!lf) lp = amin(lp, 0.0);
+    if (!uf) lp = amax(lp, 0.0);
+    const double rd = lam - lp;
+    t[0] = dd_prod(rd, rd);
+    const double rdo = rd / cscale;
+    t[1] = dd_prod(rdo, rdo);
+    t[2] = dd_prod(c, xj);
+    dd dobj = dd_zero();
+    if (lf && lp > 0.0) dobj = dd_prod(lo, lp);
+    if (uf && lp < 0.0) dobj = dd_prod(up, lp);
+    t[3] = dobj;
+    const double dx = xj - xaj;
+    t[4] = dd_prod(dx, dx);
+}
+
+// Row terms at T(z) with h = (A xp)_i: 0 rp^2, 1 (rp/R)^2, 2 yviol^2,
+// 3 (R yviol)^2, 4 dual-objective row term, 5 (y - y_anchor)^2.
+__device__ __forceinline__ void row_terms(double h, double rl, double ru, double rscale, double yi, double yai,
+                                          dd (&t)[6]) {
+    const bool lf = rl > -INFINITY, uf = ru < INFINITY;
+    double proj = amax(rl, h);
+    proj = amin(ru, proj);
+    const double rp = h - proj;
+    t[0] = dd_prod(rp, rp);
+    const double rpo = rp / rscale;
+    t[1] = dd_prod(rpo, rpo);
+    double ys = yi;
+    if (!lf) ys = amin(ys, 0.0);
+    if (!uf) ys = amax(ys, 0.0);
+    const double yv = yi - ys;
+    t[2] = dd_prod(yv, yv);
+    const double yvo = yv * rscale;
+    t[3] = dd_prod(yvo, yvo);
+    dd dobj = dd_zero();
+    if (lf && ys > 0.0) dobj = dd_prod(rl, ys);
+    if (uf && ys < 0.0) dobj = dd_prod(ru, ys);
+    t[4] = dobj;
+    const double dy = yi - yai;
+    t[5] = dd_prod(dy, dy);
+}
+
+// ------------------------------------------------------------------ decision
+
+// KKT evaluation and the restart / primal-weight / termination decision from
+// the 11 reduced terms (5 column terms, then 6 row terms). mode 0: regular
+// evaluation after a block of eval_freq iterations; mode 1: evaluation of the
+// start point (sets the anchor KKT, may terminate).
+__device__ __forceinline__ void pdlp_decide(PdlpState& s, const dd* tot, int mode) {
+    const double rd2 = dd_val(dd_add(tot[0], tot[7]));
+    const double rd2o = dd_val(dd_add(tot[1], tot[8]));
+    const double rp2 = dd_val(tot[5]);
+    const double rp2o = dd_val(tot[6]);
+    const double P = dd_val(dd_add(tot[2], dd{s.offset, 0.0}));
+    const double D = dd_val(dd_add(dd_add(tot[3], tot[9]), dd{s.offset, 0.0}));
+    const double dx2 = dd_val(tot[4]);
+    const double dy2 = dd_val(tot[10]);
+    const double gap = fabs(P - D);
+    s.rel_p = sqrt(rp2o) / (1.0 + s.bnorm_o);
+    s.rel_d = sqrt(rd2o) / (1.0 + s.cnorm_o);
+    s.rel_g = gap / (1.0 + fabs(P) + fabs(D));
+    s.pobj = P;
+    s.dobj = D;
+    s.evals += 1;
+    const double worst = amax(s.rel_p, amax(s.rel_d, s.rel_g));
+    if (!isfinite(rd2) || !isfinite(rp2) || !isfinite(P) || !isfinite(D) || !isfinite(dx2) || !isfinite(dy2)) {
+        s.done = 1;
+        s.status = 11;
+        s.restart = 0;
+        return;
+    }
+    const double kkt = sqrt(s.omega * rp2 + rd2 / s.omega + gap * gap);
+    if (mode == 1) {
+        s.kkt_anchor = kkt;
+        s.kkt_prev = kkt;
+        s.best_rel = worst;
+        s.since_best = 0;
+        s.restart = 0;
+        if (worst <= s.tol) {
+            s.done = 1;
+            s.status = 0;
+        }
+        return;
+    }
+    s.total += s.eval_freq;
+    if (worst <= s.tol) {
+        s.done = 1;
+        s.status = 0;
+        s.restart = 0;
+        return;
+    }
+    if (s.watch_stall) {
+        // FP32 phase: hand over to FP64 once the best relative KKT has not
+        // improved by 5% for max(100, evals/3) evaluations.
+        if (worst <= 0.95 * s.best_rel) {
+            s.best_rel = worst;
+            s.since_best = 0;
+        } else if (++s.since_best >= (s.evals / 3 > 100 ? s.evals / 3 : 100)) {
+            s.done = 1;
+            s.status = 2;
+            s.restart = 0;
+            return;
+        }
+    }
+    if (s.total >= s.max_iters) {
+        s.done = 1;
+        s.status = 6;
+        s.restart = 0;
+        return;
+    }
+    const int k_new = s.k_base + s.eval_freq;
+    const bool restart = (kkt <= 0.2 * s.kkt_anchor) || (kkt <= 0.8 * s.kkt_anchor && kkt > s.kkt_prev) ||
+                         (static_cast<double>(k_new) >= 0.36 * static_cast<double>(s.total));
+    s.kkt_prev = kkt;
+    if (restart) {
+        if (dx2 > 1e-20 && dy2 > 1e-20) {
+            const double ratio = sqrt(dy2) / sqrt(dx2);
+            s.omega = sqrt(s.omega * ratio);
+            s.tau = s.eta / s.omega;
+            s.sigma = s.eta * s.omega;
+        }
+        s.kkt_anchor = sqrt(s.omega * rp2 + rd2 / s.omega + gap * gap);
+        s.kkt_prev = s.kkt_anchor;
+        s.k_base = 0;
+        s.restarts += 1;
+        s.restart = 1;
+    } else {
+        s.k_base = k_new;
+        s.restart = 0;
+    }
+}
+
+}  // namespace gpu
+}  // namespace nir

@@ -211,6 +211,16 @@ NIR_API nir_owned_lp* nir_gen_mixed_lp(uint64_t seed, int32_t m, int32_t n, doub
 // (equality, links consecutive periods), a yield row and S-1 spec rows over
 // (p, r_q) (<=), and per period one capacity row over all products' p (the
 // longest rows). Coefficients U[0.1,2] with 10% sign flips.
+NIR_API void nir_gen_planning_dims(uint64_t seed, int32_t* T, int32_t* K) {
+    // A stream of its own (seed mixed with a constant) so the dimensions do
+    // not reuse the instance's coefficient stream.
+    Rng rng(seed ^ 0x5bd1e9955bd1e995ULL);
+    const int32_t t = 4 + static_cast<int32_t>(rng.below(45));   // U{4..48}
+    const int32_t k = 5 + static_cast<int32_t>(rng.below(36));   // U{5..40}
+    if (T) *T = t;
+    if (K) *K = k;
+}
+
 NIR_API nir_owned_lp* nir_gen_staircase_lp(uint64_t seed, int32_t T, int32_t K) {
     if (T <= 0 || K <= 0) return nullptr;
     constexpr int Q = 4;       // components per product

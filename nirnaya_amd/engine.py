@@ -54,6 +54,7 @@ class NirParams(C.Structure):
         ("tol", C.c_double), ("polish_tol", C.c_double), ("eval_freq", C.c_int32),
         ("check_termination", C.c_int32), ("use_graph", C.c_int32), ("verbose", C.c_int32),
         ("tile_nnz", C.c_int32), ("tile_rows", C.c_int32), ("seg_width", C.c_int32),
+        ("seg_width_t", C.c_int32),
         ("x0", _dp), ("y0", _dp), ("unscale_row", _dp), ("unscale_col", _dp),
         ("should_stop", _STOP_CB), ("stop_user", C.c_void_p),
     ]
@@ -74,6 +75,19 @@ class NirTiming(C.Structure):
         ("iterations", C.c_int64), ("total_ms", C.c_double), ("spmv_ms", C.c_double),
         ("spmtv_ms", C.c_double), ("spmv_bytes", C.c_double), ("spmtv_bytes", C.c_double),
         ("iter_bytes", C.c_double), ("spmv_passes", C.c_int32), ("spmtv_passes", C.c_int32),
+    ]
+
+
+class NirBatchOpts(C.Structure):
+    _fields_ = [("threads", C.c_int32), ("quantum", C.c_int32), ("rounds_per_graph", C.c_int32),
+                ("workgroups", C.c_int32)]
+
+
+class NirBatchStats(C.Structure):
+    _fields_ = [
+        ("count", C.c_int32), ("threads", C.c_int32), ("workgroups", C.c_int32), ("quantum", C.c_int32),
+        ("launches", C.c_int64), ("total_iterations", C.c_int64), ("nnz_iterations", C.c_double),
+        ("total_nnz", C.c_double), ("setup_seconds", C.c_double), ("solve_seconds", C.c_double),
     ]
 
 
@@ -106,6 +120,14 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.nir_pdhg_solve.argtypes = [vp, C.POINTER(NirLp), C.POINTER(NirParams), _dp, _dp, C.POINTER(NirInfo)]
     for name in ("nir_gen_equality_lp", "nir_gen_mixed_lp", "nir_gen_staircase_lp"):
         getattr(lib, name).restype = vp
+    lib.nir_batch_opts_default.argtypes = [C.POINTER(NirBatchOpts)]
+    lib.nir_batch_create.argtypes = [vp, C.POINTER(C.POINTER(NirLp)), C.c_int32, C.POINTER(NirParams),
+                                     C.POINTER(NirBatchOpts), C.POINTER(vp)]
+    lib.nir_batch_solve.argtypes = [vp]
+    lib.nir_batch_result.argtypes = [vp, C.c_int32, _dp, _dp, C.POINTER(NirInfo)]
+    lib.nir_batch_get_stats.argtypes = [vp, C.POINTER(NirBatchStats)]
+    lib.nir_batch_destroy.argtypes = [vp]
+    lib.nir_gen_planning_dims.argtypes = [C.c_uint64, _i32p, _i32p]
     lib.nir_gen_equality_lp.argtypes = [C.c_uint64, C.c_int32, C.c_int32, C.c_int32]
     lib.nir_gen_mixed_lp.argtypes = [C.c_uint64, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double]
     lib.nir_gen_staircase_lp.argtypes = [C.c_uint64, C.c_int32, C.c_int32]
@@ -221,6 +243,20 @@ def gen_staircase_lp(seed: int, T: int, K: int) -> LP:
     return _from_owned(lib, lib.nir_gen_staircase_lp(seed, T, K))
 
 
+def planning_dims(seed: int):
+    """(T, K) of the planning-family member with this seed: T ~ U{4..48}, K ~ U{5..40}."""
+    lib = load_library()
+    T, K = C.c_int32(), C.c_int32()
+    lib.nir_gen_planning_dims(seed, C.byref(T), C.byref(K))
+    return T.value, K.value
+
+
+def gen_planning_lp(seed: int) -> LP:
+    """Planning-family member: a staircase LP of planning_dims(seed)."""
+    T, K = planning_dims(seed)
+    return gen_staircase_lp(seed, T, K)
+
+
 @dataclass
 class Result:
     x: np.ndarray
@@ -270,6 +306,10 @@ class Engine:
         if self.lib.nir_calibrate_copy(self.h, int(nbytes), C.byref(ms)) != 0:
             raise EngineError("nir_calibrate_copy: " + self.error())
         return ms.value
+
+    def batch(self, lps, params: NirParams, threads: int = 0, quantum: int = 0, rounds_per_graph: int = 0,
+              workgroups: int = 0) -> "Batch":
+        return Batch(self, lps, params, threads, quantum, rounds_per_graph, workgroups)
 
     def session(self, lp: LP, params: Optional[NirParams] = None, x0=None, y0=None) -> "Session":
         return Session(self, lp, params, x0, y0)
@@ -335,6 +375,52 @@ class Session:
     def close(self):
         if getattr(self, "h", None):
             self.lib.nir_session_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Batch:
+    """Many LPs resident on the device, solved by the persistent kernel (nir_batch_*)."""
+
+    def __init__(self, eng: Engine, lps, params: NirParams, threads=0, quantum=0, rounds_per_graph=0, workgroups=0):
+        self.eng, self.lib, self.lps = eng, eng.lib, list(lps)
+        self.params = params
+        self._structs = [lp.struct() for lp in self.lps]
+        arr = (C.POINTER(NirLp) * len(self._structs))(*[C.pointer(s) for s in self._structs])
+        opts = NirBatchOpts()
+        self.lib.nir_batch_opts_default(C.byref(opts))
+        opts.threads, opts.quantum, opts.rounds_per_graph, opts.workgroups = threads, quantum, rounds_per_graph, \
+            workgroups
+        h = C.c_void_p()
+        if self.lib.nir_batch_create(eng.h, arr, len(self._structs), C.byref(params), C.byref(opts),
+                                     C.byref(h)) != 0:
+            raise EngineError("nir_batch_create: " + eng.error())
+        self.h = h
+
+    def solve(self):
+        if self.lib.nir_batch_solve(self.h) != 0:
+            raise EngineError("nir_batch_solve: " + self.eng.error())
+
+    def result(self, i: int) -> Result:
+        lp = self.lps[i]
+        x = np.zeros(lp.n)
+        y = np.zeros(lp.m)
+        info = NirInfo()
+        if self.lib.nir_batch_result(self.h, i, _ptr(x, C.c_double), _ptr(y, C.c_double), C.byref(info)) != 0:
+            raise EngineError("nir_batch_result: " + self.eng.error())
+        return Result(x, y, info.status, info.iterations, info.objective, info.tau, info.sigma, info)
+
+    def stats(self) -> NirBatchStats:
+        st = NirBatchStats()
+        if self.lib.nir_batch_get_stats(self.h, C.byref(st)) != 0:
+            raise EngineError("nir_batch_get_stats: " + self.eng.error())
+        return st
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nir_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

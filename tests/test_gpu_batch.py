@@ -1,0 +1,96 @@
+"""GPU parity of the batched persistent-kernel engine (nir_batch_*) against the
+single-LP engine (nir_pdhg_solve) and the oracle.
+
+Each instance in a batch must return exactly what the single-LP engine
+returns for it alone: the same x and y bit for bit, the same iteration,
+restart and FP64-tail counts and the same status, in FP64 and in mixed
+precision (with and without the FP64 polish), for every workgroup size,
+work-list quantum and graph setting. The single-LP FP64 trajectory is itself
+bit-identical to the oracle's or_pdlp, which one case checks directly."""
+import numpy as np
+import pytest
+
+from nirnaya_amd.engine import (MODE_PDLP, PREC_MIXED, default_params, gen_equality_lp, gen_planning_lp,
+                                gen_staircase_lp, planning_dims)
+from oracle_lib import Oracle
+import lpkit
+
+pytestmark = pytest.mark.gpu
+
+
+def small_batch():
+    """Uneven sizes, different structures, one LP solved at its start point."""
+    lps = [gen_staircase_lp(20263000 + k, T, K) for k, (T, K) in
+           enumerate([(4, 5), (6, 8), (9, 12), (12, 7), (5, 30), (3, 6)])]
+    lps += [lpkit.pdhg_unit_lp(), lpkit.transport_lp(), gen_equality_lp(31, 300, 700, 5)]
+    return lps
+
+
+def same(a, b):
+    return (np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y) and a.status == b.status
+            and a.iterations == b.iterations and a.info.restarts == b.info.restarts
+            and a.info.polish_iterations == b.info.polish_iterations)
+
+
+def check_batch(engine, lps, params, **opts):
+    singles = [engine.solve(lp, params) for lp in lps]
+    b = engine.batch(lps, params, **opts)
+    try:
+        b.solve()
+        res = [b.result(i) for i in range(len(lps))]
+        st = b.stats()
+    finally:
+        b.close()
+    for i, (r, s) in enumerate(zip(res, singles)):
+        print(f"instance {i}: m={lps[i].m} n={lps[i].n} status {r.status}/{s.status} it {r.iterations}/"
+              f"{s.iterations} restarts {r.info.restarts}/{s.info.restarts} tail {r.info.polish_iterations}/"
+              f"{s.info.polish_iterations} exact {np.array_equal(r.x, s.x) and np.array_equal(r.y, s.y)}")
+        assert same(r, s), f"instance {i} differs from the single-LP engine"
+    print(f"threads {st.threads} workgroups {st.workgroups} launches {st.launches} iterations {st.total_iterations}")
+    return res, st
+
+
+@pytest.mark.parametrize("threads", [0, 256, 1024])
+def test_batch_fp64_matches_single(engine, threads):
+    params = default_params(mode=MODE_PDLP, max_iters=200000, tol=1e-6)
+    res, st = check_batch(engine, small_batch(), params, threads=threads)
+    assert all(r.status == 0 for r in res)
+    assert st.launches >= 1
+
+
+def test_batch_mixed_and_polish_match_single(engine):
+    lps = small_batch()
+    check_batch(engine, lps, default_params(mode=MODE_PDLP, precision=PREC_MIXED, max_iters=200000, tol=1e-6))
+    res, _ = check_batch(engine, lps, default_params(mode=MODE_PDLP, precision=PREC_MIXED, max_iters=200000,
+                                                     tol=1e-6, polish_tol=1e-8))
+    for lp, r in zip(lps, res):
+        k = Oracle(lp).kkt(r.x, r.y)
+        assert r.status == 0 and max(k[0], k[1], k[2]) <= 2e-8
+
+
+def test_batch_small_quantum_eager_and_iteration_cap(engine):
+    # Many launches (one block per ticket, one launch per replay), no graphs,
+    # and a cap that stops some instances at status 6.
+    lps = small_batch()
+    params = default_params(mode=MODE_PDLP, max_iters=3000, tol=1e-7, use_graph=0)
+    res, st = check_batch(engine, lps, params, quantum=1, rounds_per_graph=1)
+    assert any(r.status == 6 for r in res) and st.launches > 10
+
+
+def test_batch_instance_matches_oracle(engine):
+    lp = gen_staircase_lp(20263002, 9, 12)
+    b = engine.batch([lpkit.transport_lp(), lp], default_params(mode=MODE_PDLP, max_iters=200000, tol=1e-6))
+    try:
+        b.solve()
+        r = b.result(1)
+    finally:
+        b.close()
+    xo, yo, info = Oracle(lp).pdlp(max_iters=200000, tol=1e-6)
+    assert r.iterations == info.iterations and np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+
+
+def test_planning_family_members_match_single(engine):
+    # Members of the 512-instance planning batch (seeds 20263*1000 + i).
+    lps = [gen_planning_lp(20263 * 1000 + i) for i in (1, 4, 9)]
+    print("dims", [planning_dims(20263 * 1000 + i) for i in (1, 4, 9)])
+    check_batch(engine, lps, default_params(mode=MODE_PDLP, precision=PREC_MIXED, max_iters=400000, tol=1e-6))
