@@ -272,10 +272,12 @@ __device__ void wg_switch_to_fp64(const View& V, const Inst& I, Shared<BT>& sh, 
 
 // What follows a decision that set done (the host loop of the single-LP
 // engine, made per instance): finish, or hand the FP32 phase over to FP64.
+// st, fp32 and polish_done are every thread's own copies of the shared
+// state, taken before a barrier: thread 0 rewrites that state below, so no
+// thread may branch on the shared copy after this point.
 template <int BT>
-__device__ void wg_after_done(const View& V, const Inst& I, Shared<BT>& sh) {
-    const int st = sh.s.status;
-    if (sh.ctl.fp32 && (st == 0 || st == 2) && !sh.ctl.polish_done) {
+__device__ void wg_after_done(const View& V, const Inst& I, Shared<BT>& sh, int st, int fp32, int polish_done) {
+    if (fp32 && (st == 0 || st == 2) && !polish_done) {
         const double target = V.polish_tol > 0 ? fmin(V.polish_tol, V.tol) : V.tol;
         if (st == 0 && V.polish_tol <= 0) {
             if (threadIdx.x == 0) {
@@ -308,8 +310,13 @@ __device__ void wg_block(const View& V, const Inst& I, const Vec<T>& v, Shared<B
     const Local<T> L = local(V, v, I);
     for (int kl = 0; kl < V.eval_freq; ++kl) wg_iteration<T, BT>(L, I, sh.s, kl, kl == V.eval_freq - 1 ? 1 : 0);
     wg_eval<T, BT>(V, I, v, sh, 0);
-    if (sh.s.restart) wg_restart_copy<T, BT>(L, I);
-    if (sh.s.done) wg_after_done<BT>(V, I, sh);
+    // Decisions into registers, then a barrier: every thread branches on the
+    // same values although thread 0 rewrites the shared state afterwards.
+    const int restart = sh.s.restart, done = sh.s.done, st = sh.s.status;
+    const int fp32 = sh.ctl.fp32, polish_done = sh.ctl.polish_done;
+    __syncthreads();
+    if (restart) wg_restart_copy<T, BT>(L, I);
+    if (done) wg_after_done<BT>(V, I, sh, st, fp32, polish_done);
 }
 
 // Take the next ticket of the work list; -1 when the list is exhausted.
@@ -339,8 +346,11 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4, 8))) void
             sh.ctl = V.ctl[i];
         }
         __syncthreads();
-        for (int b = 0; b < V.quantum && !sh.ctl.finished; ++b) {
-            if (sh.ctl.fp32) wg_block<float, BT>(V, I, V.f, sh);
+        for (int b = 0; b < V.quantum; ++b) {
+            const int finished = sh.ctl.finished, fp32 = sh.ctl.fp32;
+            __syncthreads();
+            if (finished) break;
+            if (fp32) wg_block<float, BT>(V, I, V.f, sh);
             else wg_block<double, BT>(V, I, V.d, sh);
         }
         if (threadIdx.x == 0) {
@@ -565,8 +575,9 @@ __device__ void wg_setup(const View& V, int i, Shared<BT>& sh) {
     }
     // The single-LP setup's handling of a start point that already meets the
     // tolerance (finish, or polish from it in FP64).
-    if (sh.s.done) {
-        const int st = sh.s.status;
+    const int done0 = sh.s.done, st = sh.s.status;
+    __syncthreads();
+    if (done0) {
         const bool fin = st != 0 || !(V.mixed && V.polish_tol > 0);
         if (threadIdx.x == 0) {
             sh.ctl.finished = fin ? 1 : 0;
