@@ -164,6 +164,38 @@ NIR_API int nir_batch_result(nir_batch* b, int32_t i, double* x, double* y, nir_
 NIR_API int nir_batch_get_stats(nir_batch* b, nir_batch_stats* out);
 NIR_API void nir_batch_destroy(nir_batch* b);
 
+/* ---- learned warm start -------------------------------------------------
+ * No counterpart in the reference. A small MLP (n_in -> hidden -> hidden ->
+ * n_out, GELU, bf16 MFMA GEMMs with FP32 accumulation, Adam) trained on the
+ * device on solutions the engine produced, and the FP64 kernels of the
+ * proper orthogonal decomposition of solution snapshots. Widths are
+ * multiples of 64; parameters are FP32 in the order W1 [hidden][n_in], b1,
+ * W2 [hidden][hidden], b2, W3 [n_out][hidden], b3. */
+typedef struct nir_mlp nir_mlp;
+NIR_API int nir_mlp_create(nir_ctx* ctx, int32_t n_in, int32_t hidden, int32_t n_out, int32_t max_rows,
+                           uint64_t seed, nir_mlp** out);
+NIR_API int64_t nir_mlp_num_params(const nir_mlp* m);
+NIR_API int nir_mlp_get_params(nir_mlp* m, float* out);
+NIR_API int nir_mlp_set_params(nir_mlp* m, const float* in);
+/* out[rows][n_out] = MLP(x[rows][n_in]). */
+NIR_API int nir_mlp_forward(nir_mlp* m, const float* x, int32_t rows, float* out);
+/* Mean squared error over rows x n_out against target and its gradient with
+ * respect to every parameter (rows <= max_rows). */
+NIR_API int nir_mlp_gradient(nir_mlp* m, const float* x, const float* target, int32_t rows, float* grad,
+                             double* loss);
+NIR_API int nir_mlp_set_data(nir_mlp* m, const float* x, const float* target, int32_t rows);
+NIR_API int nir_mlp_train(nir_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t seed, float* epoch_loss);
+NIR_API int nir_mlp_train_stats(nir_mlp* m, int64_t* rows, double* ms, double* gemm_flops);
+NIR_API void nir_mlp_destroy(nir_mlp* m);
+/* One device GEMM C[M][N] = A B^T of bf16-rounded inputs (A [M][K] or, with
+ * transpose_a, [K][M]; B [N][K] or [K][N]); M, N multiples of 64, K of 32. */
+NIR_API int nir_mlp_gemm_check(nir_ctx* ctx, int32_t M, int32_t N, int32_t K, const float* A, const float* B,
+                               int32_t transpose_a, int32_t transpose_b, float* C);
+/* POD: G = X^T X and Y = X V for snapshots X [len][N] (row-major). */
+NIR_API int nir_pod_gram(nir_ctx* ctx, const double* X, int32_t len, int32_t N, double* G);
+NIR_API int nir_pod_project(nir_ctx* ctx, const double* X, int32_t len, int32_t N, const double* V, int32_t R,
+                            double* Y);
+
 /* ---- synthetic instances (seeded; generated on the host) ---------------- */
 typedef struct nir_owned_lp nir_owned_lp;
 

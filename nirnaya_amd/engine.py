@@ -128,6 +128,21 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.nir_batch_get_stats.argtypes = [vp, C.POINTER(NirBatchStats)]
     lib.nir_batch_destroy.argtypes = [vp]
     lib.nir_gen_planning_dims.argtypes = [C.c_uint64, _i32p, _i32p]
+    _fp = C.POINTER(C.c_float)
+    lib.nir_mlp_create.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(vp)]
+    lib.nir_mlp_num_params.argtypes = [vp]
+    lib.nir_mlp_num_params.restype = C.c_int64
+    lib.nir_mlp_get_params.argtypes = [vp, _fp]
+    lib.nir_mlp_set_params.argtypes = [vp, _fp]
+    lib.nir_mlp_forward.argtypes = [vp, _fp, C.c_int32, _fp]
+    lib.nir_mlp_gradient.argtypes = [vp, _fp, _fp, C.c_int32, _fp, _dp]
+    lib.nir_mlp_set_data.argtypes = [vp, _fp, _fp, C.c_int32]
+    lib.nir_mlp_train.argtypes = [vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _fp]
+    lib.nir_mlp_train_stats.argtypes = [vp, C.POINTER(C.c_int64), _dp, _dp]
+    lib.nir_mlp_destroy.argtypes = [vp]
+    lib.nir_mlp_gemm_check.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, _fp, _fp, C.c_int32, C.c_int32, _fp]
+    lib.nir_pod_gram.argtypes = [vp, _dp, C.c_int32, C.c_int32, _dp]
+    lib.nir_pod_project.argtypes = [vp, _dp, C.c_int32, C.c_int32, _dp, C.c_int32, _dp]
     lib.nir_gen_equality_lp.argtypes = [C.c_uint64, C.c_int32, C.c_int32, C.c_int32]
     lib.nir_gen_mixed_lp.argtypes = [C.c_uint64, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double]
     lib.nir_gen_staircase_lp.argtypes = [C.c_uint64, C.c_int32, C.c_int32]
@@ -307,6 +322,42 @@ class Engine:
             raise EngineError("nir_calibrate_copy: " + self.error())
         return ms.value
 
+    def gemm_check(self, A: np.ndarray, B: np.ndarray, transpose_a=False, transpose_b=False) -> np.ndarray:
+        """C = A' B'^T on the device with bf16-rounded inputs, where A' is A ([M][K]) or A^T when A is
+        given [K][M], and B' is B ([N][K]) or B^T when B is given [K][N]."""
+        A = np.ascontiguousarray(A, dtype=np.float32)
+        B = np.ascontiguousarray(B, dtype=np.float32)
+        M = A.shape[1] if transpose_a else A.shape[0]
+        K = A.shape[0] if transpose_a else A.shape[1]
+        N = B.shape[1] if transpose_b else B.shape[0]
+        out = np.zeros((M, N), dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        if self.lib.nir_mlp_gemm_check(self.h, M, N, K, A.ctypes.data_as(fp), B.ctypes.data_as(fp),
+                                       int(transpose_a), int(transpose_b), out.ctypes.data_as(fp)) != 0:
+            raise EngineError("nir_mlp_gemm_check: " + self.error())
+        return out
+
+    def pod_gram(self, X: np.ndarray) -> np.ndarray:
+        """X^T X for X [len][N] on the device (FP64)."""
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        G = np.zeros((X.shape[1], X.shape[1]))
+        if self.lib.nir_pod_gram(self.h, _ptr(X, C.c_double), X.shape[0], X.shape[1], _ptr(G, C.c_double)) != 0:
+            raise EngineError("nir_pod_gram: " + self.error())
+        return G
+
+    def pod_project(self, X: np.ndarray, V: np.ndarray) -> np.ndarray:
+        """X V for X [len][N], V [N][R] on the device (FP64)."""
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        Y = np.zeros((X.shape[0], V.shape[1]))
+        if self.lib.nir_pod_project(self.h, _ptr(X, C.c_double), X.shape[0], X.shape[1], _ptr(V, C.c_double),
+                                    V.shape[1], _ptr(Y, C.c_double)) != 0:
+            raise EngineError("nir_pod_project: " + self.error())
+        return Y
+
+    def mlp(self, n_in: int, hidden: int, n_out: int, max_rows: int, seed: int = 0) -> "Mlp":
+        return Mlp(self, n_in, hidden, n_out, max_rows, seed)
+
     def batch(self, lps, params: NirParams, threads: int = 0, quantum: int = 0, rounds_per_graph: int = 0,
               workgroups: int = 0) -> "Batch":
         return Batch(self, lps, params, threads, quantum, rounds_per_graph, workgroups)
@@ -421,6 +472,74 @@ class Batch:
     def close(self):
         if getattr(self, "h", None):
             self.lib.nir_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Mlp:
+    """The warm-start network on the device (nir_mlp_*): bf16 MFMA forward and backward, Adam."""
+
+    def __init__(self, eng: Engine, n_in: int, hidden: int, n_out: int, max_rows: int, seed: int = 0):
+        self.eng, self.lib = eng, eng.lib
+        self.n_in, self.hidden, self.n_out = n_in, hidden, n_out
+        h = C.c_void_p()
+        if self.lib.nir_mlp_create(eng.h, n_in, hidden, n_out, max_rows, seed, C.byref(h)) != 0:
+            raise EngineError("nir_mlp_create: " + eng.error())
+        self.h = h
+        self.count = int(self.lib.nir_mlp_num_params(h))
+
+    @staticmethod
+    def _f(a):
+        return np.ascontiguousarray(a, dtype=np.float32)
+
+    def _fp(self, a):
+        return a.ctypes.data_as(C.POINTER(C.c_float))
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.count, dtype=np.float32)
+        if self.lib.nir_mlp_get_params(self.h, self._fp(out)) != 0:
+            raise EngineError("nir_mlp_get_params: " + self.eng.error())
+        return out
+
+    def set_params(self, p: np.ndarray):
+        p = self._f(p)
+        if p.size != self.count or self.lib.nir_mlp_set_params(self.h, self._fp(p)) != 0:
+            raise EngineError("nir_mlp_set_params: " + self.eng.error())
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        x = self._f(x)
+        out = np.zeros((x.shape[0], self.n_out), dtype=np.float32)
+        if self.lib.nir_mlp_forward(self.h, self._fp(x), x.shape[0], self._fp(out)) != 0:
+            raise EngineError("nir_mlp_forward: " + self.eng.error())
+        return out
+
+    def gradient(self, x: np.ndarray, t: np.ndarray):
+        x, t = self._f(x), self._f(t)
+        g = np.zeros(self.count, dtype=np.float32)
+        loss = C.c_double()
+        if self.lib.nir_mlp_gradient(self.h, self._fp(x), self._fp(t), x.shape[0], self._fp(g), C.byref(loss)) != 0:
+            raise EngineError("nir_mlp_gradient: " + self.eng.error())
+        return loss.value, g
+
+    def train(self, x: np.ndarray, t: np.ndarray, epochs: int, batch: int, lr: float, seed: int = 0) -> np.ndarray:
+        x, t = self._f(x), self._f(t)
+        if self.lib.nir_mlp_set_data(self.h, self._fp(x), self._fp(t), x.shape[0]) != 0:
+            raise EngineError("nir_mlp_set_data: " + self.eng.error())
+        losses = np.zeros(max(epochs, 1), dtype=np.float32)
+        if self.lib.nir_mlp_train(self.h, epochs, batch, lr, seed, self._fp(losses)) != 0:
+            raise EngineError("nir_mlp_train: " + self.eng.error())
+        return losses[:epochs]
+
+    def train_stats(self):
+        rows, ms, fl = C.c_int64(), C.c_double(), C.c_double()
+        self.lib.nir_mlp_train_stats(self.h, C.byref(rows), C.byref(ms), C.byref(fl))
+        return rows.value, ms.value, fl.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nir_mlp_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -1,0 +1,191 @@
+"""Learned warm start for parametric LP families.
+
+A family keeps A fixed and moves the right-hand side and the costs with a
+parameter vector theta in R^64:
+    b(theta) = b0 * (1 + 0.1 tanh(B theta)),   c(theta) = c0 + 0.05 C theta,
+with fixed seeded N(0,1) matrices B [m][64] and C [n][64] and A, b0, c0 the
+equality LP of nir_gen_equality_lp.
+
+The warm start is a proper orthogonal decomposition (POD) of solution
+snapshots - mean-centred, rank <= 256 for x and for y, computed from the
+snapshot Gram matrix on the device (method of snapshots) - and an MLP
+64 -> 512 -> 512 -> 512 (GELU, bf16 MFMA with FP32 accumulation, trained on
+the device by the engine's own forward/backward kernels) that predicts the
+POD coefficients of (x, y) from theta. Training data are solves the PDHG
+engine itself produced. Every warm-started solve must reach the same KKT
+tolerance as a cold solve; if it does not, the cold solve is the fallback and
+its result is returned.
+
+The reference has no learned component; this module has no counterpart there.
+"""
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from .engine import LP, MODE_PDLP, Engine, default_params, gen_equality_lp
+
+THETA_DIM = 64
+
+
+class ParametricFamily:
+    """Fixed-A family of equality LPs (see the module docstring)."""
+
+    def __init__(self, seed: int, m: int, n: int, per_col: int = 5, dim: int = THETA_DIM):
+        self.seed, self.m, self.n, self.dim = seed, m, n, dim
+        self.base = gen_equality_lp(seed, m, n, per_col)
+        rng = np.random.default_rng(seed)
+        self.B = rng.standard_normal((m, dim))
+        self.C = rng.standard_normal((n, dim))
+        self.b0 = self.base.rl.copy()
+        self.c0 = self.base.c.copy()
+
+    def thetas(self, count: int, seed: int) -> np.ndarray:
+        return np.random.default_rng(seed).standard_normal((count, self.dim))
+
+    def instance(self, theta: np.ndarray) -> LP:
+        b = self.b0 * (1.0 + 0.1 * np.tanh(self.B @ theta))
+        c = self.c0 + 0.05 * (self.C @ theta)
+        base = self.base
+        return LP(self.m, self.n, base.start, base.index, base.value, np.ascontiguousarray(c), base.l, base.u,
+                  np.ascontiguousarray(b), b.copy())
+
+
+class Pod:
+    """Mean-centred POD of snapshots S [N][len]: modes [len][r] with orthonormal
+    columns, singular values, and the coefficients of the training snapshots."""
+
+    def __init__(self, engine: Engine, S: np.ndarray, rank: int, rel_floor: float = 1e-12):
+        S = np.asarray(S, dtype=np.float64)
+        self.mean = S.mean(axis=0)
+        X = np.ascontiguousarray((S - self.mean).T)           # [len][N]
+        G = engine.pod_gram(X)                                  # X^T X on the device
+        lam, V = np.linalg.eigh(G)
+        order = np.argsort(lam)[::-1]
+        lam, V = lam[order], V[:, order]
+        keep = int(np.sum(lam > rel_floor * max(lam[0], 1e-300))) if lam.size else 0
+        r = max(1, min(rank, keep))
+        self.sing = np.sqrt(np.maximum(lam[:r], 0.0))
+        self.modes = engine.pod_project(X, V[:, :r] / self.sing)   # X V S^-1 on the device
+        self.train_coef = V[:, :r] * self.sing                   # [N][r] = modes^T (s - mean)
+        self.rank = r
+
+    def encode(self, s: np.ndarray) -> np.ndarray:
+        return self.modes.T @ (np.asarray(s, dtype=np.float64) - self.mean)
+
+    def decode(self, alpha: np.ndarray) -> np.ndarray:
+        return self.mean + self.modes @ np.asarray(alpha, dtype=np.float64)
+
+
+def _pad64(k: int) -> int:
+    return (k + 63) // 64 * 64
+
+
+class WarmStart:
+    """theta -> (x0, y0) through the MLP and the two POD bases."""
+
+    def __init__(self, engine: Engine, pod_x: Pod, pod_y: Pod, hidden: int = 512, seed: int = 0,
+                 max_rows: int = 512, dim: int = THETA_DIM):
+        self.engine, self.pod_x, self.pod_y, self.dim = engine, pod_x, pod_y, dim
+        self.n_coef = pod_x.rank + pod_y.rank
+        self.n_out = _pad64(self.n_coef)
+        self.net = engine.mlp(dim, hidden, self.n_out, max_rows, seed)
+        self.mu = np.zeros(self.n_out)
+        self.sd = np.ones(self.n_out)
+
+    def fit(self, thetas: np.ndarray, epochs: int = 200, batch: int = 256, lr: float = 1e-3, seed: int = 0):
+        """Train on the POD coefficients of the training snapshots (z-scored)."""
+        alpha = np.concatenate([self.pod_x.train_coef, self.pod_y.train_coef], axis=1)
+        self.mu[:self.n_coef] = alpha.mean(axis=0)
+        self.sd[:self.n_coef] = np.maximum(alpha.std(axis=0), 1e-12)
+        target = np.zeros((alpha.shape[0], self.n_out), dtype=np.float32)
+        target[:, :self.n_coef] = (alpha - self.mu[:self.n_coef]) / self.sd[:self.n_coef]
+        return self.net.train(thetas.astype(np.float32), target, epochs, batch, lr, seed)
+
+    def predict(self, theta: np.ndarray):
+        out = self.net.forward(np.asarray(theta, dtype=np.float32).reshape(1, -1))[0].astype(np.float64)
+        alpha = out * self.sd + self.mu
+        rx = self.pod_x.rank
+        return self.pod_x.decode(alpha[:rx]), self.pod_y.decode(alpha[rx:self.n_coef])
+
+
+@dataclass
+class SolveRecord:
+    status: int
+    iterations: int
+    seconds: float
+    fallback: bool = False
+
+
+@dataclass
+class PipelineReport:
+    n_train: int
+    n_solved: int
+    rank_x: int
+    rank_y: int
+    train_loss: List[float]
+    cold: List[SolveRecord] = field(default_factory=list)
+    warm: List[SolveRecord] = field(default_factory=list)
+    train_solve_seconds: float = 0.0
+    pod_seconds: float = 0.0
+    train_seconds: float = 0.0
+
+
+def solve_warm(engine: Engine, lp: LP, params, x0, y0):
+    """Warm-started solve with cold start as the fallback (and the judge)."""
+    import time
+    t0 = time.perf_counter()
+    r = engine.solve(lp, params, x0=x0, y0=y0)
+    dt = time.perf_counter() - t0
+    if r.status == 0:
+        return r, SolveRecord(r.status, r.iterations, dt)
+    t1 = time.perf_counter()
+    c = engine.solve(lp, params)
+    return c, SolveRecord(c.status, r.iterations + c.iterations, dt + time.perf_counter() - t1, fallback=True)
+
+
+def run_pipeline(engine: Engine, family: ParametricFamily, n_train: int, n_test: int, rank: int = 256,
+                 epochs: int = 200, batch: int = 256, lr: float = 1e-3, tol: float = 1e-6, precision: int = 0,
+                 max_iters: int = 400000, seed: int = 7, log=None) -> PipelineReport:
+    """Generate training solves with the batched engine, fit the POD bases and
+    the MLP on the device, then solve held-out thetas cold and warm."""
+    import time
+    say = log or (lambda *_: None)
+    params = default_params(mode=MODE_PDLP, precision=precision, max_iters=max_iters, tol=tol)
+    th_train = family.thetas(n_train, seed)
+    th_test = family.thetas(n_test, seed + 1)
+    t0 = time.perf_counter()
+    lps = [family.instance(t) for t in th_train]
+    b = engine.batch(lps, params)
+    try:
+        b.solve()
+        res = [b.result(i) for i in range(n_train)]
+    finally:
+        b.close()
+    t_solve = time.perf_counter() - t0
+    ok = [i for i, r in enumerate(res) if r.status == 0]   # screen out failed / infeasible thetas
+    say(f"training solves: {len(ok)}/{n_train} reached tolerance in {t_solve:.1f}s")
+    t0 = time.perf_counter()
+    pod_x = Pod(engine, np.stack([res[i].x for i in ok]), rank)
+    pod_y = Pod(engine, np.stack([res[i].y for i in ok]), rank)
+    t_pod = time.perf_counter() - t0
+    ws = WarmStart(engine, pod_x, pod_y, max_rows=max(64, batch))
+    t0 = time.perf_counter()
+    losses = ws.fit(th_train[ok], epochs=epochs, batch=batch, lr=lr, seed=seed)
+    t_fit = time.perf_counter() - t0
+    say(f"POD ranks x {pod_x.rank} y {pod_y.rank} ({t_pod:.1f}s); MLP loss {losses[0]:.3e} -> {losses[-1]:.3e} "
+        f"({t_fit:.1f}s)")
+    rep = PipelineReport(n_train, len(ok), pod_x.rank, pod_y.rank, [float(v) for v in losses],
+                         train_solve_seconds=t_solve, pod_seconds=t_pod, train_seconds=t_fit)
+    for t in th_test:
+        lp = family.instance(t)
+        t0 = time.perf_counter()
+        c = engine.solve(lp, params)
+        rep.cold.append(SolveRecord(c.status, c.iterations, time.perf_counter() - t0))
+        x0, y0 = ws.predict(t)
+        _, w = solve_warm(engine, lp, params, x0, y0)
+        rep.warm.append(w)
+        say(f"held-out: cold {c.iterations} it ({rep.cold[-1].seconds:.3f}s) warm {w.iterations} it "
+            f"({w.seconds:.3f}s){' fallback' if w.fallback else ''}")
+    ws.net.close()
+    return rep
