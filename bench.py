@@ -64,6 +64,66 @@ def cpu_baseline(lp):
                       f"available, CPU {cpu_model()}"}
 
 
+def planning512(args):
+    """Batched workload: 512 planning LPs of uneven size (T ~ U{4..48}
+    periods x K ~ U{5..40} products, seeds 20263*1000 + i), restarted PDHG to
+    relative KKT 1e-6. Times the batched persistent engine (setup + solve) in
+    FP64 and in mixed precision, and the FP64 single-LP engine one LP at a
+    time on every 32nd instance; the one-at-a-time total is that sample's
+    time per iteration times the exact iteration counts of all 512 (the batch
+    FP64 trajectory of each instance is the single-LP one, bit for bit)."""
+    import numpy as np
+    from nirnaya_amd.engine import Engine, MODE_PDLP, PREC_MIXED, default_params, gen_planning_lp
+    t0 = time.time()
+    lps = [gen_planning_lp(20263 * 1000 + i) for i in range(512)]
+    ms = np.array([lp.m for lp in lps])
+    log(f"[bench] 512 planning LPs generated in {time.time() - t0:.1f}s, m {ms.min()}..{ms.max()}, "
+        f"nnz {sum(lp.nnz for lp in lps)}")
+    eng = Engine(0)
+    out = {}
+    for name, prec in (("fp64", 0), ("mixed", PREC_MIXED)):
+        p = default_params(mode=MODE_PDLP, precision=prec, max_iters=400000, tol=1e-6)
+        t0 = time.perf_counter()
+        b = eng.batch(lps, p)
+        t1 = time.perf_counter()
+        b.solve()
+        t2 = time.perf_counter()
+        res = [b.result(i) for i in range(len(lps))]
+        st = b.stats()
+        b.close()
+        out[name] = {"setup_s": t1 - t0, "solve_s": t2 - t1, "total_s": t2 - t0,
+                     "optimal": int(sum(r.status == 0 for r in res)), "iterations": int(st.total_iterations),
+                     "threads": st.threads, "launches": int(st.launches),
+                     "nnz_iterations_per_s": st.nnz_iterations / (t2 - t1)}
+        if name == "fp64":
+            its = np.array([r.iterations for r in res])
+        log(f"[bench] batch {name}: {out[name]}")
+    p = default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-6)
+    samp = list(range(0, 512, 32))
+    ts, si = 0.0, 0
+    for i in samp:
+        t0 = time.perf_counter()
+        r = eng.solve(lps[i], p)
+        ts += time.perf_counter() - t0
+        si += r.iterations
+        assert r.iterations == its[i]
+    one = ts / si * its.sum()
+    eng.close()
+    hist, edges = np.histogram(ms, bins=[0, 1000, 2000, 5000, 10000, 20000])
+    line = {
+        "metric": "time-to-1e-6 relative KKT, 512 planning LPs (batched persistent engine)",
+        "value": round(out["fp64"]["total_s"], 3), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "512 multi-period planning LPs, T~U{4..48} x K~U{5..40}, seeds 20263*1000+i",
+                   "rows_histogram": {f"{int(a)}-{int(b)}": int(h) for a, b, h in zip(edges[:-1], edges[1:], hist)}},
+        "batch_fp64": out["fp64"], "batch_mixed": out["mixed"],
+        "one_at_a_time_fp64_s": round(one, 3),
+        "one_at_a_time_sample": f"{len(samp)} instances (every 32nd), {ts:.2f}s for {si} iterations",
+        "speedup_vs_one_at_a_time": round(one / out["fp64"]["total_s"], 2),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,11 +136,16 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the 256 MiB copy kernel (reference for rocprofv3 byte counters)")
+    ap.add_argument("--workload", default="large-lp", choices=["large-lp", "planning512"],
+                    help="large-lp: the headline line (default); planning512: the batched workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is design-only here.",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "planning512":
+        planning512(args)
+        return
 
     from nirnaya_amd.engine import Engine, MODE_REFERENCE, default_params, gen_mixed_lp
 

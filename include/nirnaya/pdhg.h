@@ -68,6 +68,9 @@ typedef struct {
     const double* unscale_col; /* optional: x*col, (c - A^T y)/col */
     int (*should_stop)(void* user); /* polled every 256 iterations; nonzero = stop */
     void* stop_user;
+    double primal_weight;      /* pdlp: initial primal weight in the engine's scaled space
+                                  (0 = ||c|| / ||b|| of the scaled LP); a warm start can carry
+                                  the weight its family's solves ended with */
 } nir_params;
 
 typedef struct {
@@ -157,6 +160,12 @@ NIR_API void nir_batch_opts_default(nir_batch_opts* o);
  * evaluation) on the device; `lps` holds `count` pointers. */
 NIR_API int nir_batch_create(nir_ctx* ctx, const nir_lp* const* lps, int32_t count, const nir_params* p,
                              const nir_batch_opts* o, nir_batch** out);
+/* The same with per-instance warm starts: x0s / y0s hold `count` pointers
+ * (ncol / nrow values each; a NULL array or entry = cold start), read with
+ * the single-LP meaning of nir_params.x0 / y0. */
+NIR_API int nir_batch_create_warm(nir_ctx* ctx, const nir_lp* const* lps, int32_t count,
+                                  const double* const* x0s, const double* const* y0s, const nir_params* p,
+                                  const nir_batch_opts* o, nir_batch** out);
 /* Runs every instance to its tolerance, iteration cap or should_stop (polled
  * between graph replays; unfinished instances then report status 4). */
 NIR_API int nir_batch_solve(nir_batch* b);

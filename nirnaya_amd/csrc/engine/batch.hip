@@ -87,7 +87,12 @@ struct View {
     int eval_freq;
     int quantum;
     double tol, polish_tol;
+    double omega0;     // initial primal weight (0: from the norms)
     long long max_iters;
+    const double* x0;  // warm starts (concatenated like the vectors), with
+    const double* y0;  // per-instance flags; null when no instance has one
+    const int* has_x0;
+    const int* has_y0;
 };
 
 // Instance-local pointers of one precision.
@@ -508,9 +513,11 @@ __device__ void wg_setup(const View& V, int i, Shared<BT>& sh) {
     const double smax = lam > 0.0 ? sqrt(lam) : 1.0;
     const double eta = 0.998 / smax;
 
-    // Start point: 0 clamped into the bounds; y = 0.
+    // Start point: the warm start (in caller units) or 0, clamped into the
+    // bounds; y = the warm start or 0.
+    const bool wx = V.x0 && V.has_x0[i], wy = V.y0 && V.has_y0[i];
     for (int j = tid; j < n; j += BT) {
-        double v = 0.0;
+        double v = wx ? V.x0[I.xo + j] / C[j] : 0.0;
         v = (D.lo[j] < v) ? v : D.lo[j];
         v = (v < D.up[j]) ? v : D.up[j];
         D.x[j] = v;
@@ -519,13 +526,14 @@ __device__ void wg_setup(const View& V, int i, Shared<BT>& sh) {
         D.xbar[j] = v;
     }
     for (int r = tid; r < m; r += BT) {
-        D.y[r] = 0.0;
-        D.ya[r] = 0.0;
-        D.yp[r] = 0.0;
+        const double v = wy ? V.y0[I.yo + r] / R[r] : 0.0;
+        D.y[r] = v;
+        D.ya[r] = v;
+        D.yp[r] = v;
     }
     if (tid == 0) {
         PdlpState h{};
-        h.omega = (cnorm > 1e-10 && bnorm > 1e-10) ? cnorm / bnorm : 1.0;
+        h.omega = V.omega0 > 0 ? V.omega0 : ((cnorm > 1e-10 && bnorm > 1e-10) ? cnorm / bnorm : 1.0);
         h.eta = eta;
         h.tau = eta / h.omega;
         h.sigma = eta * h.omega;
@@ -650,7 +658,9 @@ struct nir_batch {
     DevArray<int> arow, acol, trow, tcol;
     HostVec<double> d;
     HostVec<float> f;
-    DevArray<double> R, C, hs, cost, collo, colup, rowlo, rowup, offset;
+    DevArray<double> R, C, hs, cost, collo, colup, rowlo, rowup, offset, x0, y0;
+    DevArray<int> has_x0, has_y0;
+    bool warm = false;
     DevArray<PdlpState> st;
     DevArray<Ctl> ctl;
     DevArray<int> list, count_d, head;
@@ -693,7 +703,12 @@ struct nir_batch {
         V.quantum = opts.quantum > 0 ? opts.quantum : 16;
         V.tol = prm.tol > 0 ? prm.tol : 1e-6;
         V.polish_tol = prm.polish_tol;
+        V.omega0 = prm.primal_weight > 0 ? prm.primal_weight : 0.0;
         V.max_iters = std::max<long long>(1, prm.max_iters);
+        V.x0 = warm ? x0.p : nullptr;
+        V.y0 = warm ? y0.p : nullptr;
+        V.has_x0 = warm ? has_x0.p : nullptr;
+        V.has_y0 = warm ? has_y0.p : nullptr;
         return V;
     }
     ~nir_batch() {
@@ -825,6 +840,12 @@ NIR_API void nir_batch_opts_default(nir_batch_opts* o) {
 
 NIR_API int nir_batch_create(nir_ctx* ctx, const nir_lp* const* lps, int32_t count, const nir_params* p,
                              const nir_batch_opts* o, nir_batch** out) {
+    return nir_batch_create_warm(ctx, lps, count, nullptr, nullptr, p, o, out);
+}
+
+NIR_API int nir_batch_create_warm(nir_ctx* ctx, const nir_lp* const* lps, int32_t count,
+                                  const double* const* x0s, const double* const* y0s, const nir_params* p,
+                                  const nir_batch_opts* o, nir_batch** out) {
     if (!ctx || !out) return -1;
     *out = nullptr;
     auto t0 = std::chrono::steady_clock::now();
@@ -836,7 +857,7 @@ NIR_API int nir_batch_create(nir_ctx* ctx, const nir_lp* const* lps, int32_t cou
         if (o) B->opts = *o;
         if (B->prm.mode != NIR_MODE_PDLP) throw std::invalid_argument("batched solves run pdlp mode only");
         if (B->prm.x0 || B->prm.y0 || B->prm.unscale_row || B->prm.unscale_col)
-            throw std::invalid_argument("batched solves take no warm start or unscaling vectors");
+            throw std::invalid_argument("batched solves take warm starts per instance and no unscaling vectors");
         if (B->opts.threads != 0 && B->opts.threads != 256 && B->opts.threads != 512 && B->opts.threads != 1024)
             throw std::invalid_argument("batch threads must be 0, 256, 512 or 1024");
         if (B->opts.quantum < 0 || B->opts.rounds_per_graph < 0 || B->opts.workgroups < 0)
@@ -917,6 +938,27 @@ NIR_API int nir_batch_create(nir_ctx* ctx, const nir_lp* const* lps, int32_t cou
         up_d(B->rowlo, rowlo);
         up_d(B->rowup, rowup);
         up_d(B->offset, B->h_offset);
+        if (x0s || y0s) {
+            // Warm starts, concatenated like the column / row vectors.
+            B->warm = true;
+            std::vector<double> hx(xo, 0.0), hy(yo, 0.0);
+            std::vector<int> fx(count, 0), fy(count, 0);
+            for (int i = 0; i < count; ++i) {
+                const Inst& I = B->h_inst[i];
+                if (x0s && x0s[i]) {
+                    std::copy(x0s[i], x0s[i] + I.n, hx.begin() + I.xo);
+                    fx[i] = 1;
+                }
+                if (y0s && y0s[i]) {
+                    std::copy(y0s[i], y0s[i] + I.m, hy.begin() + I.yo);
+                    fy[i] = 1;
+                }
+            }
+            up_d(B->x0, hx);
+            up_d(B->y0, hy);
+            up_i(B->has_x0, fx);
+            up_i(B->has_y0, fy);
+        }
         B->d.alloc(xo, yo, nz);
         B->d.aval.upload(aval.data(), nz, s);
         B->d.tval.upload(tval.data(), nz, s);

@@ -682,7 +682,8 @@ void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm,
         bs.add_prod(b, b);
     }
     const double cnorm = std::sqrt(cs.value()), bnorm = std::sqrt(bs.value());
-    const double omega = (cnorm > 1e-10 && bnorm > 1e-10) ? cnorm / bnorm : 1.0;
+    const double omega = prm.primal_weight > 0 ? prm.primal_weight
+                                               : ((cnorm > 1e-10 && bnorm > 1e-10) ? cnorm / bnorm : 1.0);
     PdlpState h{};
     h.omega = omega;
     h.eta = eta;

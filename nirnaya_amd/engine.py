@@ -56,7 +56,7 @@ class NirParams(C.Structure):
         ("tile_nnz", C.c_int32), ("tile_rows", C.c_int32), ("seg_width", C.c_int32),
         ("seg_width_t", C.c_int32),
         ("x0", _dp), ("y0", _dp), ("unscale_row", _dp), ("unscale_col", _dp),
-        ("should_stop", _STOP_CB), ("stop_user", C.c_void_p),
+        ("should_stop", _STOP_CB), ("stop_user", C.c_void_p), ("primal_weight", C.c_double),
     ]
 
 
@@ -123,6 +123,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.nir_batch_opts_default.argtypes = [C.POINTER(NirBatchOpts)]
     lib.nir_batch_create.argtypes = [vp, C.POINTER(C.POINTER(NirLp)), C.c_int32, C.POINTER(NirParams),
                                      C.POINTER(NirBatchOpts), C.POINTER(vp)]
+    lib.nir_batch_create_warm.argtypes = [vp, C.POINTER(C.POINTER(NirLp)), C.c_int32, C.POINTER(_dp),
+                                          C.POINTER(_dp), C.POINTER(NirParams), C.POINTER(NirBatchOpts),
+                                          C.POINTER(vp)]
     lib.nir_batch_solve.argtypes = [vp]
     lib.nir_batch_result.argtypes = [vp, C.c_int32, _dp, _dp, C.POINTER(NirInfo)]
     lib.nir_batch_get_stats.argtypes = [vp, C.POINTER(NirBatchStats)]
@@ -359,8 +362,8 @@ class Engine:
         return Mlp(self, n_in, hidden, n_out, max_rows, seed)
 
     def batch(self, lps, params: NirParams, threads: int = 0, quantum: int = 0, rounds_per_graph: int = 0,
-              workgroups: int = 0) -> "Batch":
-        return Batch(self, lps, params, threads, quantum, rounds_per_graph, workgroups)
+              workgroups: int = 0, x0s=None, y0s=None) -> "Batch":
+        return Batch(self, lps, params, threads, quantum, rounds_per_graph, workgroups, x0s, y0s)
 
     def session(self, lp: LP, params: Optional[NirParams] = None, x0=None, y0=None) -> "Session":
         return Session(self, lp, params, x0, y0)
@@ -379,8 +382,10 @@ class Session:
 
     def __init__(self, eng: Engine, lp: LP, params: Optional[NirParams], x0=None, y0=None):
         self.eng, self.lib, self.lp = eng, eng.lib, lp
-        self.params = params if params is not None else default_params()
-        self._keep = []
+        # A private copy: the warm-start pointers set below must not leak into
+        # the caller's parameters (and from there into later sessions).
+        self.params = NirParams.from_buffer_copy(params) if params is not None else default_params()
+        self._keep = [params]
         if x0 is not None:
             x0 = np.ascontiguousarray(x0, dtype=np.float64)
             self._keep.append(x0)
@@ -435,7 +440,8 @@ class Session:
 class Batch:
     """Many LPs resident on the device, solved by the persistent kernel (nir_batch_*)."""
 
-    def __init__(self, eng: Engine, lps, params: NirParams, threads=0, quantum=0, rounds_per_graph=0, workgroups=0):
+    def __init__(self, eng: Engine, lps, params: NirParams, threads=0, quantum=0, rounds_per_graph=0, workgroups=0,
+                 x0s=None, y0s=None):
         self.eng, self.lib, self.lps = eng, eng.lib, list(lps)
         self.params = params
         self._structs = [lp.struct() for lp in self.lps]
@@ -445,8 +451,22 @@ class Batch:
         opts.threads, opts.quantum, opts.rounds_per_graph, opts.workgroups = threads, quantum, rounds_per_graph, \
             workgroups
         h = C.c_void_p()
-        if self.lib.nir_batch_create(eng.h, arr, len(self._structs), C.byref(params), C.byref(opts),
-                                     C.byref(h)) != 0:
+        if x0s is None and y0s is None:
+            rc = self.lib.nir_batch_create(eng.h, arr, len(self._structs), C.byref(params), C.byref(opts), C.byref(h))
+        else:
+            self._keep = []
+
+            def ptrs(vs):
+                out = (_dp * len(self._structs))()
+                for i, v in enumerate(vs if vs is not None else [None] * len(self._structs)):
+                    if v is not None:
+                        a = np.ascontiguousarray(v, dtype=np.float64)
+                        self._keep.append(a)
+                        out[i] = a.ctypes.data_as(_dp)
+                return out
+            rc = self.lib.nir_batch_create_warm(eng.h, arr, len(self._structs), ptrs(x0s), ptrs(y0s),
+                                                C.byref(params), C.byref(opts), C.byref(h))
+        if rc != 0:
             raise EngineError("nir_batch_create: " + eng.error())
         self.h = h
 

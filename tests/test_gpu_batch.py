@@ -94,3 +94,26 @@ def test_planning_family_members_match_single(engine):
     lps = [gen_planning_lp(20263 * 1000 + i) for i in (1, 4, 9)]
     print("dims", [planning_dims(20263 * 1000 + i) for i in (1, 4, 9)])
     check_batch(engine, lps, default_params(mode=MODE_PDLP, precision=PREC_MIXED, max_iters=400000, tol=1e-6))
+
+
+def test_batch_warm_starts_and_primal_weight_match_single(engine):
+    lps = [gen_staircase_lp(20263000 + k, T, K) for k, (T, K) in enumerate([(4, 5), (6, 8), (9, 12)])]
+    lps.append(gen_equality_lp(31, 300, 700, 5))
+    rng = np.random.default_rng(4)
+    x0s = [lp.x0 * (1 + 0.1 * rng.standard_normal(lp.n)) for lp in lps]
+    y0s = [lp.y0 * 0.5 for lp in lps]
+    x0s[1] = None   # cold x, warm y
+    y0s[2] = None   # warm x, cold y
+    for prec in (0, 1):
+        params = default_params(mode=MODE_PDLP, precision=prec, max_iters=200000, tol=1e-6, primal_weight=0.7)
+        b = engine.batch(lps, params, x0s=x0s, y0s=y0s)
+        try:
+            b.solve()
+            res = [b.result(i) for i in range(len(lps))]
+        finally:
+            b.close()
+        for i, lp in enumerate(lps):
+            s = engine.solve(lp, params, x0=x0s[i], y0=y0s[i])
+            print(f"prec {prec} instance {i}: it {res[i].iterations}/{s.iterations} omega0 0.7 "
+                  f"exact {np.array_equal(res[i].x, s.x) and np.array_equal(res[i].y, s.y)}")
+            assert same(res[i], s)
