@@ -31,6 +31,7 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../../include/nirnaya/pdhg.h"
@@ -76,6 +77,7 @@ struct View {
     Vec<double> d;
     Vec<float> f;
     double *R, *C, *hs;
+    double *HA, *GA;   // FP32 phase: A X and A^T Y of the FP64 anchor (X, Y) = (d.xa, d.ya)
     const double *cost, *collo, *colup, *rowlo, *rowup, *offset;
     PdlpState* st;
     Ctl* ctl;
@@ -183,14 +185,23 @@ __device__ void wg_eval(const View& V, const Inst& I, const Vec<TI>& it, Shared<
     double* hs = V.hs + I.yo;
     for (int j = threadIdx.x; j < I.n; j += BT) gs[j] = row_dot<double>(D.trow, D.tcol, D.tval, yp, j);
     for (int i = threadIdx.x; i < I.m; i += BT) hs[i] = row_dot<double>(D.arow, D.acol, D.aval, xp, i);
+    // FP32 iterates are deltas from the FP64 anchor: T(z) = anchor + delta,
+    // A^T yp = GA + A^T dyp, A xp = HA + A dxp.
+    constexpr bool kDelta = std::is_same<TI, float>::value;
+    const double* GA = V.GA + I.xo;
+    const double* HA = V.HA + I.yo;
     {
         dd acc[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q) acc[q] = dd_zero();
         for (int j = threadIdx.x; j < I.n; j += BT) {
             dd t[5];
-            col_terms(gs[j], D.c[j], D.lo[j], D.up[j], C[j], static_cast<double>(xp[j]), static_cast<double>(xa[j]),
-                      t);
+            if (kDelta)
+                col_terms(GA[j] + gs[j], D.c[j], D.lo[j], D.up[j], C[j], D.xa[j] + static_cast<double>(xp[j]),
+                          D.xa[j], t);
+            else
+                col_terms(gs[j], D.c[j], D.lo[j], D.up[j], C[j], static_cast<double>(xp[j]),
+                          static_cast<double>(xa[j]), t);
 #pragma unroll
             for (int q = 0; q < 5; ++q) acc[q] = dd_add(acc[q], t[q]);
         }
@@ -202,7 +213,10 @@ __device__ void wg_eval(const View& V, const Inst& I, const Vec<TI>& it, Shared<
         for (int q = 0; q < 6; ++q) acc[q] = dd_zero();
         for (int i = threadIdx.x; i < I.m; i += BT) {
             dd t[6];
-            row_terms(hs[i], D.rl[i], D.ru[i], R[i], static_cast<double>(yp[i]), static_cast<double>(ya[i]), t);
+            if (kDelta)
+                row_terms(HA[i] + hs[i], D.rl[i], D.ru[i], R[i], D.ya[i] + static_cast<double>(yp[i]), D.ya[i], t);
+            else
+                row_terms(hs[i], D.rl[i], D.ru[i], R[i], static_cast<double>(yp[i]), static_cast<double>(ya[i]), t);
 #pragma unroll
             for (int q = 0; q < 6; ++q) acc[q] = dd_add(acc[q], t[q]);
         }
@@ -218,9 +232,10 @@ template <typename T, int BT>
 __device__ void wg_iteration(const Local<T>& L, const Inst& I, const PdlpState& s, int k_lo, int save) {
     const Halpern<T> hw = halpern<T>(s.k_base + k_lo);
     const T tau = static_cast<T>(s.tau);
+    constexpr bool kDelta = std::is_same<T, float>::value;   // FP32: re-centred form, zero anchor delta
     for (int j = threadIdx.x; j < I.n; j += BT) {
         const T g = row_dot<T>(L.trow, L.tcol, L.tval, L.y, j);
-        const PrimalOut<T> r = primal_point<T>(L.x[j], g, L.c[j], L.lo[j], L.up[j], L.xa[j], tau, hw);
+        const PrimalOut<T> r = primal_point<T>(L.x[j], g, L.c[j], L.lo[j], L.up[j], kDelta ? T(0) : L.xa[j], tau, hw);
         L.xbar[j] = r.xb;
         L.x[j] = r.next;
         if (save) L.xp[j] = r.cand;
@@ -229,7 +244,8 @@ __device__ void wg_iteration(const Local<T>& L, const Inst& I, const PdlpState& 
     const T sigma = static_cast<T>(s.sigma);
     for (int i = threadIdx.x; i < I.m; i += BT) {
         const T h = row_dot<T>(L.arow, L.acol, L.aval, L.xbar, i);
-        const DualOut<T> r = dual_point<T>(L.y[i], h, L.rl[i], L.ru[i], L.ya[i], sigma, hw);
+        const DualOut<T> r = kDelta ? dual_point_delta<T>(L.y[i], h, L.rl[i], L.ru[i], L.ya[i], sigma, hw)
+                                    : dual_point<T>(L.y[i], h, L.rl[i], L.ru[i], L.ya[i], sigma, hw);
         L.y[i] = r.next;
         if (save) L.yp[i] = r.yp;
     }
@@ -252,14 +268,49 @@ __device__ __forceinline__ void wg_restart_copy(const Local<T>& L, const Inst& I
     __syncthreads();
 }
 
+// Re-centre the FP32 phase: the anchor moves to T(z) (skipped at setup,
+// `force`), A X and A^T Y are recomputed in FP64 and the FP32 constants are
+// re-based on the anchor with zero deltas (the single-LP recenter sequence).
+template <int BT>
+__device__ void wg_recenter(const View& V, const Inst& I, const PdlpState& s, int force) {
+    const Local<double> D = local(V, V.d, I);
+    const Local<float> F = local(V, V.f, I);
+    double* HA = V.HA + I.yo;
+    double* GA = V.GA + I.xo;
+    if (!force) {
+        for (int j = threadIdx.x; j < I.n; j += BT) D.xa[j] = D.xa[j] + static_cast<double>(F.xp[j]);
+        for (int i = threadIdx.x; i < I.m; i += BT) D.ya[i] = D.ya[i] + static_cast<double>(F.yp[i]);
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < I.m; i += BT) HA[i] = row_dot<double>(D.arow, D.acol, D.aval, D.xa, i);
+    for (int j = threadIdx.x; j < I.n; j += BT) GA[j] = row_dot<double>(D.trow, D.tcol, D.tval, D.ya, j);
+    __syncthreads();
+    const double sigma = s.sigma;
+    for (int j = threadIdx.x; j < I.n; j += BT) {
+        F.c[j] = static_cast<float>(D.c[j] - GA[j]);
+        F.lo[j] = static_cast<float>(D.lo[j] - D.xa[j]);
+        F.up[j] = static_cast<float>(D.up[j] - D.xa[j]);
+        F.x[j] = 0.0f;
+        F.xp[j] = 0.0f;
+    }
+    for (int i = threadIdx.x; i < I.m; i += BT) {
+        F.rl[i] = static_cast<float>(D.rl[i] - HA[i]);
+        F.ru[i] = static_cast<float>(D.ru[i] - HA[i]);
+        F.ya[i] = static_cast<float>(-D.ya[i] / sigma);
+        F.y[i] = 0.0f;
+        F.yp[i] = 0.0f;
+    }
+    __syncthreads();
+}
+
 // FP32 -> FP64 hand-over: the FP64 phase restarts at T(z) of the FP32 phase.
 template <int BT>
 __device__ void wg_switch_to_fp64(const View& V, const Inst& I, Shared<BT>& sh, double new_tol) {
     const Local<double> D = local(V, V.d, I);
     const float* fxp = V.f.xp + I.xo;
     const float* fyp = V.f.yp + I.yo;
-    for (int j = threadIdx.x; j < I.n; j += BT) D.xp[j] = static_cast<double>(fxp[j]);
-    for (int i = threadIdx.x; i < I.m; i += BT) D.yp[i] = static_cast<double>(fyp[i]);
+    for (int j = threadIdx.x; j < I.n; j += BT) D.xp[j] = D.xa[j] + static_cast<double>(fxp[j]);
+    for (int i = threadIdx.x; i < I.m; i += BT) D.yp[i] = D.ya[i] + static_cast<double>(fyp[i]);
     __syncthreads();
     wg_restart_copy<double, BT>(D, I);
     if (threadIdx.x == 0) {
@@ -320,7 +371,10 @@ __device__ void wg_block(const View& V, const Inst& I, const Vec<T>& v, Shared<B
     const int restart = sh.s.restart, done = sh.s.done, st = sh.s.status;
     const int fp32 = sh.ctl.fp32, polish_done = sh.ctl.polish_done;
     __syncthreads();
-    if (restart) wg_restart_copy<T, BT>(L, I);
+    if (restart) {
+        if (std::is_same<T, float>::value) wg_recenter<BT>(V, I, sh.s, 0);
+        else wg_restart_copy<T, BT>(L, I);
+    }
     if (done) wg_after_done<BT>(V, I, sh, st, fp32, polish_done);
 }
 
@@ -558,24 +612,10 @@ __device__ void wg_setup(const View& V, int i, Shared<BT>& sh) {
             F.aval[p] = static_cast<float>(aval[p]);
             F.tval[p] = static_cast<float>(tval[p]);
         }
-        for (int j = tid; j < n; j += BT) {
-            F.c[j] = static_cast<float>(D.c[j]);
-            F.lo[j] = static_cast<float>(D.lo[j]);
-            F.up[j] = static_cast<float>(D.up[j]);
-            const float xv = static_cast<float>(D.x[j]);
-            F.x[j] = xv;
-            F.xa[j] = xv;
-            F.xp[j] = xv;
-            F.xbar[j] = xv;
-        }
-        for (int r = tid; r < m; r += BT) {
-            F.rl[r] = static_cast<float>(D.rl[r]);
-            F.ru[r] = static_cast<float>(D.ru[r]);
-            const float yv = static_cast<float>(D.y[r]);
-            F.y[r] = yv;
-            F.ya[r] = yv;
-            F.yp[r] = yv;
-        }
+        __syncthreads();
+        // The anchor is the start point (D.xa / D.ya); re-base the FP32
+        // constants on it with zero deltas.
+        wg_recenter<BT>(V, I, sh.s, 1);
         __syncthreads();
         wg_eval<float, BT>(V, I, V.f, sh, 1);
     } else {
@@ -634,8 +674,9 @@ struct HostVec {
     Vec<T> view() {
         return Vec<T>{x.p, xbar.p, xa.p, xp.p, y.p, ya.p, yp.p, c.p, lo.p, up.p, rl.p, ru.p, aval.p, tval.p};
     }
-    // Arrays the iteration changes (autotune snapshots).
-    std::vector<DevArray<T>*> mutable_arrays() { return {&x, &xbar, &xa, &xp, &y, &ya, &yp}; }
+    // Arrays the iteration changes (autotune snapshots): iterates, and in the
+    // re-centred FP32 phase also the re-based constants.
+    std::vector<DevArray<T>*> mutable_arrays() { return {&x, &xbar, &xa, &xp, &y, &ya, &yp, &c, &lo, &up, &rl, &ru}; }
 };
 
 }  // namespace batch
@@ -658,7 +699,7 @@ struct nir_batch {
     DevArray<int> arow, acol, trow, tcol;
     HostVec<double> d;
     HostVec<float> f;
-    DevArray<double> R, C, hs, cost, collo, colup, rowlo, rowup, offset, x0, y0;
+    DevArray<double> R, C, hs, HA, GA, cost, collo, colup, rowlo, rowup, offset, x0, y0;
     DevArray<int> has_x0, has_y0;
     bool warm = false;
     DevArray<PdlpState> st;
@@ -686,6 +727,8 @@ struct nir_batch {
         V.R = R.p;
         V.C = C.p;
         V.hs = hs.p;
+        V.HA = HA.p;
+        V.GA = GA.p;
         V.cost = cost.p;
         V.collo = collo.p;
         V.colup = colup.p;
@@ -756,7 +799,11 @@ int autotune_threads(nir_batch* B, int cu) {
     V.quantum = 1;
     std::vector<DevArray<double>*> dm = B->d.mutable_arrays();
     std::vector<DevArray<float>*> fm;
-    if (V.mixed) fm = B->f.mutable_arrays();
+    if (V.mixed) {
+        fm = B->f.mutable_arrays();
+        dm.push_back(&B->HA);   // anchor products, rewritten at every FP32 restart
+        dm.push_back(&B->GA);
+    }
     // Snapshot.
     std::vector<std::unique_ptr<DevArray<double>>> sd;
     std::vector<std::unique_ptr<DevArray<float>>> sf;
@@ -966,6 +1013,10 @@ NIR_API int nir_batch_create_warm(nir_ctx* ctx, const nir_lp* const* lps, int32_
         B->R.alloc(yo);
         B->C.alloc(xo);
         B->hs.alloc(yo);
+        if (mixed) {
+            B->HA.alloc(yo);
+            B->GA.alloc(xo);
+        }
         B->st.alloc(count);
         B->ctl.alloc(count);
         // Work list: largest instances first.
@@ -1077,11 +1128,14 @@ NIR_API int nir_batch_result(nir_batch* B, int32_t i, double* x, double* y, nir_
         HIP_CHECK(hipMemcpy(&c, B->ctl.p + i, sizeof(c), hipMemcpyDeviceToHost));
         std::vector<double> xs(static_cast<size_t>(I.n)), ys(static_cast<size_t>(I.m));
         if (c.fp32) {
+            // T(z) of the FP32 phase: FP64 anchor + FP32 delta.
             std::vector<float> xf(static_cast<size_t>(I.n)), yf(static_cast<size_t>(I.m));
             if (I.n) HIP_CHECK(hipMemcpy(xf.data(), B->f.xp.p + I.xo, I.n * sizeof(float), hipMemcpyDeviceToHost));
             if (I.m) HIP_CHECK(hipMemcpy(yf.data(), B->f.yp.p + I.yo, I.m * sizeof(float), hipMemcpyDeviceToHost));
-            for (int j = 0; j < I.n; ++j) xs[j] = xf[j];
-            for (int r = 0; r < I.m; ++r) ys[r] = yf[r];
+            if (I.n) HIP_CHECK(hipMemcpy(xs.data(), B->d.xa.p + I.xo, I.n * sizeof(double), hipMemcpyDeviceToHost));
+            if (I.m) HIP_CHECK(hipMemcpy(ys.data(), B->d.ya.p + I.yo, I.m * sizeof(double), hipMemcpyDeviceToHost));
+            for (int j = 0; j < I.n; ++j) xs[j] = xs[j] + static_cast<double>(xf[j]);
+            for (int r = 0; r < I.m; ++r) ys[r] = ys[r] + static_cast<double>(yf[r]);
         } else {
             if (I.n) HIP_CHECK(hipMemcpy(xs.data(), B->d.xp.p + I.xo, I.n * sizeof(double), hipMemcpyDeviceToHost));
             if (I.m) HIP_CHECK(hipMemcpy(ys.data(), B->d.yp.p + I.yo, I.m * sizeof(double), hipMemcpyDeviceToHost));

@@ -92,9 +92,12 @@ __global__ __launch_bounds__(kThreads) void k_primal_pdlp(Pass<T> P, const T* __
     struct Ops {
         T x, c, l, u, a;
     };
+    // FP32 runs the re-centred form: the anchor is the FP64 point the deltas
+    // are taken from, its delta is 0 and no anchor is read.
+    constexpr bool kDelta = std::is_same<T, float>::value;
     const auto pre = [&](int j) {
         Ops o{T(0), T(0), T(0), T(0), T(0)};
-        if (P.last) o = Ops{x[j], c[j], lo[j], up[j], xa[j]};
+        if (P.last) o = Ops{x[j], c[j], lo[j], up[j], kDelta ? T(0) : xa[j]};
         return o;
     };
     stream_rows<TILE>(P, y, prod, blockIdx.x, pre, [&](int j, T g, const Ops& o) {
@@ -122,8 +125,11 @@ __global__ __launch_bounds__(kThreads) void k_dual_pdlp(Pass<T> P, const T* __re
         if (P.last) o = Ops{y[i], rl[i], ru[i], ya[i]};
         return o;
     };
+    // FP32 runs the re-centred form; `ya` then holds sa = -Y / sigma.
+    constexpr bool kDelta = std::is_same<T, float>::value;
     stream_rows<TILE>(P, xbar, prod, blockIdx.x, pre, [&](int i, T h, const Ops& o) {
-        const DualOut<T> r = dual_point<T>(o.y, h, o.l, o.u, o.a, sigma, hw);
+        const DualOut<T> r = kDelta ? dual_point_delta<T>(o.y, h, o.l, o.u, o.a, sigma, hw)
+                                    : dual_point<T>(o.y, h, o.l, o.u, o.a, sigma, hw);
         y[i] = r.next;
         if (save) yp_out[i] = r.yp;
     });
@@ -139,14 +145,22 @@ __global__ __launch_bounds__(kThreads) void k_eval_cols(Pass<double> P, const TI
                                                         const double* __restrict__ c, const double* __restrict__ lo,
                                                         const double* __restrict__ up,
                                                         const double* __restrict__ cscale,
+                                                        const double* __restrict__ anchor,
+                                                        const double* __restrict__ base,
                                                         const PdlpState* __restrict__ st, dd* __restrict__ part,
                                                         int force) {
     __shared__ double prod[lds_slots(TILE)];
     __shared__ dd sh[5 * kThreads];
     if (st->done && !force) return;
     dd acc[5] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
+    // Re-centred iterates (anchor != null): T(z) = anchor + delta and
+    // A^T yp = base + A^T dyp.
     stream_rows<TILE>(P, yp, prod, blockIdx.x, [&](int j, double g) {
-        col_terms(g, c[j], lo[j], up[j], cscale[j], static_cast<double>(xp[j]), static_cast<double>(xa[j]), acc);
+        if (anchor)
+            col_terms(base[j] + g, c[j], lo[j], up[j], cscale[j], anchor[j] + static_cast<double>(xp[j]), anchor[j],
+                      acc);
+        else
+            col_terms(g, c[j], lo[j], up[j], cscale[j], static_cast<double>(xp[j]), static_cast<double>(xa[j]), acc);
     });
     if (P.last) block_dd_reduce<5>(acc, sh, part + 5 * blockIdx.x);
 }
@@ -159,6 +173,8 @@ __global__ __launch_bounds__(kThreads) void k_eval_rows(Pass<double> P, const TI
                                                         const TI* __restrict__ yp, const TI* __restrict__ ya,
                                                         const double* __restrict__ rl, const double* __restrict__ ru,
                                                         const double* __restrict__ rscale,
+                                                        const double* __restrict__ anchor,
+                                                        const double* __restrict__ base,
                                                         const PdlpState* __restrict__ st, dd* __restrict__ part,
                                                         int force) {
     __shared__ double prod[lds_slots(TILE)];
@@ -166,7 +182,10 @@ __global__ __launch_bounds__(kThreads) void k_eval_rows(Pass<double> P, const TI
     if (st->done && !force) return;
     dd acc[6] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
     stream_rows<TILE>(P, xp, prod, blockIdx.x, [&](int i, double h) {
-        row_terms(h, rl[i], ru[i], rscale[i], static_cast<double>(yp[i]), static_cast<double>(ya[i]), acc);
+        if (anchor)
+            row_terms(base[i] + h, rl[i], ru[i], rscale[i], anchor[i] + static_cast<double>(yp[i]), anchor[i], acc);
+        else
+            row_terms(h, rl[i], ru[i], rscale[i], static_cast<double>(yp[i]), static_cast<double>(ya[i]), acc);
     });
     if (P.last) block_dd_reduce<6>(acc, sh, part + 6 * blockIdx.x);
 }
@@ -218,6 +237,61 @@ __global__ __launch_bounds__(kThreads) void k_spmv_plain(Pass<double> P, const d
                                                          double* __restrict__ out) {
     __shared__ double prod[lds_slots(TILE)];
     stream_rows<TILE>(P, v, prod, blockIdx.x, [&](int r, double s) { out[r] = s; });
+}
+
+// y = M v only when the decision restarted (or `force`): the FP64 anchor
+// products of the re-centred FP32 phase.
+template <int TILE>
+__global__ __launch_bounds__(kThreads) void k_spmv_on_restart(Pass<double> P, const double* __restrict__ v,
+                                                              double* __restrict__ out,
+                                                              const PdlpState* __restrict__ st, int force) {
+    __shared__ double prod[lds_slots(TILE)];
+    if (!force && (st->done || !st->restart)) return;
+    stream_rows<TILE>(P, v, prod, blockIdx.x, [&](int r, double s) { out[r] = s; });
+}
+
+// Re-centring, step 1: the anchor moves to T(z) = anchor + delta (FP64).
+__global__ void k_anchor_move(const PdlpState* __restrict__ st, int n, int m, double* X, double* Y,
+                              const float* __restrict__ dxp, const float* __restrict__ dyp, int force) {
+    if (!force && (st->done || !st->restart)) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    for (int j = t; j < n; j += stride) X[j] = X[j] + static_cast<double>(dxp[j]);
+    for (int i = t; i < m; i += stride) Y[i] = Y[i] + static_cast<double>(dyp[i]);
+}
+
+// Re-centring, step 3 (after HA = A X, GA = A^T Y): FP32 constants re-based
+// on the anchor and zero deltas.
+__global__ void k_recenter(const PdlpState* __restrict__ st, int n, int m, const double* __restrict__ c,
+                           const double* __restrict__ lo, const double* __restrict__ up,
+                           const double* __restrict__ rl, const double* __restrict__ ru,
+                           const double* __restrict__ X, const double* __restrict__ Y,
+                           const double* __restrict__ HA, const double* __restrict__ GA, float* rc, float* dl,
+                           float* du, float* drl, float* dru, float* sa, float* dx, float* dxp, float* dy, float* dyp,
+                           int force) {
+    if (!force && (st->done || !st->restart)) return;
+    const double sigma = st->sigma;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    for (int j = t; j < n; j += stride) {
+        rc[j] = static_cast<float>(c[j] - GA[j]);
+        dl[j] = static_cast<float>(lo[j] - X[j]);
+        du[j] = static_cast<float>(up[j] - X[j]);
+        dx[j] = 0.0f;
+        dxp[j] = 0.0f;
+    }
+    for (int i = t; i < m; i += stride) {
+        drl[i] = static_cast<float>(rl[i] - HA[i]);
+        dru[i] = static_cast<float>(ru[i] - HA[i]);
+        sa[i] = static_cast<float>(-Y[i] / sigma);
+        dy[i] = 0.0f;
+        dyp[i] = 0.0f;
+    }
+}
+
+// out = anchor + delta (FP64), the FP32 phase's point in FP64.
+__global__ void k_delta_to_double(const double* anchor, const float* delta, double* out, long long n) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    for (long long i = t; i < n; i += stride) out[i] = anchor[i] + static_cast<double>(delta[i]);
 }
 
 // Row measures over a segmented CSR matrix, segments in column order: max |a|
@@ -332,8 +406,9 @@ struct Pdlp {
     Values32 a32, at32;        // FP32 copies of the scaled values (mixed mode)
     Buf<double> R, C;          // engine scaling (powers of two)
     Buf<double> Rtot, Ctot;    // engine x caller scaling, for KKT in the caller's units
-    Iterates<double> d;
-    Iterates<float> f;
+    Iterates<double> d;        // FP32 phase: d.xa / d.ya are the FP64 anchor X, Y
+    Iterates<float> f;         // FP32 phase: deltas and re-based constants (f.ya = -Y / sigma)
+    Buf<double> HA, GA;        // FP32 phase: A X and A^T Y of the anchor
     Buf<dd> part_c, part_r;
     Buf<PdlpState> st;
     PdlpState* h_st = nullptr;   // pinned, two poll slots
@@ -416,31 +491,67 @@ void launch_iter(Pdlp& P, hipStream_t s, Iterates<T>& it, int k_lo, int save) {
     PD_CHECK(hipGetLastError());
 }
 
+// Re-centre the FP32 phase on its anchor: move the anchor to T(z) (unless
+// `force`, used at setup where the deltas are zero), recompute A X and A^T Y
+// in FP64 and re-base the FP32 constants. Every kernel returns at once unless
+// the decision restarted, so the sequence can sit in every captured block.
+void recenter(Pdlp& P, hipStream_t s, int force) {
+    const int n = P.n, m = P.m;
+    if (!force)
+        k_anchor_move<<<grid_for(std::max(n, m)), 256, 0, s>>>(P.st.p, n, m, P.d.xa.p, P.d.ya.p, P.f.xp.p, P.f.yp.p,
+                                                               0);
+    for (int q = 0; q < P.A->shape.nseg; ++q) {
+        const Pass<double> ps = P.A->shape.pass(P.A->val.p, P.A->part.p, q);
+        TILE_DISPATCH(P.A->shape.shape.tile,
+                      k_spmv_on_restart<TL><<<ps.M.nblk, kThreads, 0, s>>>(ps, P.d.xa.p, P.HA.p, P.st.p, force));
+    }
+    for (int q = 0; q < P.AT->shape.nseg; ++q) {
+        const Pass<double> ps = P.AT->shape.pass(P.AT->val.p, P.AT->part.p, q);
+        TILE_DISPATCH(P.AT->shape.shape.tile,
+                      k_spmv_on_restart<TL><<<ps.M.nblk, kThreads, 0, s>>>(ps, P.d.ya.p, P.GA.p, P.st.p, force));
+    }
+    k_recenter<<<grid_for(std::max(n, m)), 256, 0, s>>>(P.st.p, n, m, P.d.c.p, P.d.lo.p, P.d.up.p, P.d.rl.p, P.d.ru.p,
+                                                        P.d.xa.p, P.d.ya.p, P.HA.p, P.GA.p, P.f.c.p, P.f.lo.p,
+                                                        P.f.up.p, P.f.rl.p, P.f.ru.p, P.f.ya.p, P.f.x.p, P.f.xp.p,
+                                                        P.f.y.p, P.f.yp.p, force);
+    PD_CHECK(hipGetLastError());
+}
+
 template <typename TI>
 void launch_eval(Pdlp& P, hipStream_t s, Iterates<TI>& it, int mode) {
     const int force = mode == 1 ? 1 : 0;
     const DevMatrix& AT = *P.AT;
     const DevMatrix& A = *P.A;
+    // FP32 iterates are deltas from the FP64 anchor (X, Y) with products
+    // GA = A^T Y, HA = A X.
+    constexpr bool kDelta = std::is_same<TI, float>::value;
+    const double* xanc = kDelta ? P.d.xa.p : nullptr;
+    const double* yanc = kDelta ? P.d.ya.p : nullptr;
+    const double* gbase = kDelta ? P.GA.p : nullptr;
+    const double* hbase = kDelta ? P.HA.p : nullptr;
     for (int q = 0; q < AT.shape.nseg; ++q) {
         const Pass<double> ps = AT.shape.pass(AT.val.p, AT.part.p, q);
         TILE_DISPATCH(AT.shape.shape.tile,
                       k_eval_cols<TL, TI><<<ps.M.nblk, kThreads, 0, s>>>(ps, it.yp.p, it.xp.p, it.xa.p, P.d.c.p,
-                                                                         P.d.lo.p, P.d.up.p, P.Ctot.p, P.st.p,
-                                                                         P.part_c.p, force));
+                                                                         P.d.lo.p, P.d.up.p, P.Ctot.p, xanc, gbase,
+                                                                         P.st.p, P.part_c.p, force));
     }
     for (int q = 0; q < A.shape.nseg; ++q) {
         const Pass<double> ps = A.shape.pass(A.val.p, A.part.p, q);
         TILE_DISPATCH(A.shape.shape.tile,
                       k_eval_rows<TL, TI><<<ps.M.nblk, kThreads, 0, s>>>(ps, it.xp.p, it.yp.p, it.ya.p, P.d.rl.p,
-                                                                         P.d.ru.p, P.Rtot.p, P.st.p, P.part_r.p,
-                                                                         force));
+                                                                         P.d.ru.p, P.Rtot.p, yanc, hbase, P.st.p,
+                                                                         P.part_r.p, force));
     }
     const int nbc = AT.shape.nblk[AT.shape.nseg - 1];
     const int nbr = A.shape.nblk[A.shape.nseg - 1];
     k_decide<<<1, kThreads, 0, s>>>(P.st.p, P.part_c.p, nbc, P.part_r.p, nbr, mode);
     if (mode == 0) {
-        k_restart_copy<TI><<<grid_for(std::max(P.n, P.m)), 256, 0, s>>>(P.st.p, P.n, P.m, it.xp.p, it.yp.p, it.x.p,
-                                                                          it.xa.p, it.y.p, it.ya.p, 0);
+        if (kDelta)
+            recenter(P, s, 0);
+        else
+            k_restart_copy<TI><<<grid_for(std::max(P.n, P.m)), 256, 0, s>>>(P.st.p, P.n, P.m, it.xp.p, it.yp.p,
+                                                                              it.x.p, it.xa.p, it.y.p, it.ya.p, 0);
     }
     PD_CHECK(hipGetLastError());
 }
@@ -501,8 +612,8 @@ void write_state(Pdlp& P, hipStream_t s, const PdlpState& h) {
 // FP32 -> FP64 hand-over: the FP64 phase restarts at T(z) of the FP32 phase.
 void switch_to_fp64(Pdlp& P, hipStream_t s, double new_tol) {
     const long long n = P.n, m = P.m;
-    k_to_double<<<grid_for(n), 256, 0, s>>>(P.f.xp.p, P.d.xp.p, n);
-    k_to_double<<<grid_for(m), 256, 0, s>>>(P.f.yp.p, P.d.yp.p, m);
+    k_delta_to_double<<<grid_for(n), 256, 0, s>>>(P.d.xa.p, P.f.xp.p, P.d.xp.p, n);
+    k_delta_to_double<<<grid_for(m), 256, 0, s>>>(P.d.ya.p, P.f.yp.p, P.d.yp.p, m);
     k_restart_copy<double><<<grid_for(std::max(n, m)), 256, 0, s>>>(P.st.p, P.n, P.m, P.d.xp.p, P.d.yp.p, P.d.x.p,
                                                                     P.d.xa.p, P.d.y.p, P.d.ya.p, 1);
     PdlpState h = read_state(P, s);
@@ -712,14 +823,11 @@ void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm,
         k_to_float<<<grid_for(P.nnz), 256, 0, s>>>(a_val, P.a32.val.p, P.nnz);
         k_to_float<<<grid_for(P.nnz), 256, 0, s>>>(at_val, P.at32.val.p, P.nnz);
         P.f.alloc(n, m);
-        auto conv = [&](Buf<double>& from, Buf<float>& to, long long len) {
-            k_to_float<<<grid_for(len), 256, 0, s>>>(from.p, to.p, len);
-        };
-        conv(P.d.c, P.f.c, n); conv(P.d.lo, P.f.lo, n); conv(P.d.up, P.f.up, n);
-        conv(P.d.rl, P.f.rl, m); conv(P.d.ru, P.f.ru, m);
-        conv(P.d.x, P.f.x, n); conv(P.d.x, P.f.xa, n); conv(P.d.x, P.f.xp, n); conv(P.d.x, P.f.xbar, n);
-        conv(P.d.y, P.f.y, m); conv(P.d.y, P.f.ya, m); conv(P.d.y, P.f.yp, m);
-        PD_CHECK(hipGetLastError());
+        P.HA.alloc(m);
+        P.GA.alloc(n);
+        // The anchor is the start point (d.xa / d.ya); deltas start at zero
+        // (k_recenter writes them together with the re-based constants).
+        recenter(P, s, 1);
         launch_eval<float>(P, s, P.f, 1);
     } else {
         launch_eval<double>(P, s, P.d, 1);
@@ -798,12 +906,15 @@ int pdlp_result(Pdlp& P, hipStream_t s, double* x, double* y, nir_info* info) {
     PdlpState h = read_state(P, s);
     std::vector<double> xs(static_cast<size_t>(P.n)), ys(static_cast<size_t>(P.m));
     if (P.mixed) {
+        // T(z) of the FP32 phase: anchor + delta in FP64.
         std::vector<float> xf(static_cast<size_t>(P.n)), yf(static_cast<size_t>(P.m));
         P.f.xp.down(xf.data(), P.n, s);
         P.f.yp.down(yf.data(), P.m, s);
+        P.d.xa.down(xs.data(), P.n, s);
+        P.d.ya.down(ys.data(), P.m, s);
         PD_CHECK(hipStreamSynchronize(s));
-        for (int j = 0; j < P.n; ++j) xs[j] = xf[j];
-        for (int i = 0; i < P.m; ++i) ys[i] = yf[i];
+        for (int j = 0; j < P.n; ++j) xs[j] = xs[j] + static_cast<double>(xf[j]);
+        for (int i = 0; i < P.m; ++i) ys[i] = ys[i] + static_cast<double>(yf[i]);
     } else {
         P.d.xp.down(xs.data(), P.n, s);
         P.d.yp.down(ys.data(), P.m, s);
@@ -847,8 +958,9 @@ int pdlp_time(Pdlp& P, hipStream_t s, long long iters, nir_timing* out) {
     const double sv = P.mixed ? 4.0 : 8.0;
     // A xbar + dual step: values+cols, rowptr, xbar once, y/ya/rl/ru read, y write.
     out->spmv_bytes = nnz * (sv + 4) + 4.0 * (m + 1) + sv * n + sv * 5 * m;
-    // A^T y + primal step: values+cols, rowptr, y once, x/xa/c/lo/up read, x/xbar write.
-    out->spmtv_bytes = nnz * (sv + 4) + 4.0 * (n + 1) + sv * m + sv * 7 * n;
+    // A^T y + primal step: values+cols, rowptr, y once, x/xa/c/lo/up read, x/xbar write
+    // (the re-centred FP32 form reads no anchor).
+    out->spmtv_bytes = nnz * (sv + 4) + 4.0 * (n + 1) + sv * m + sv * (P.mixed ? 6 : 7) * n;
     out->iter_bytes = out->spmv_bytes + out->spmtv_bytes;
     out->spmv_passes = P.A->shape.nseg;
     out->spmtv_passes = P.AT->shape.nseg;

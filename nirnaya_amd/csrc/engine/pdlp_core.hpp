@@ -119,6 +119,28 @@ __device__ __forceinline__ DualOut<T> dual_point(T yi, T hx, T l, T u, T ya, T s
     return DualOut<T>{yp, h.w * yb + h.wc * ya};
 }
 
+// Re-centred FP32 iteration (mixed precision). The iterate is an FP64 anchor
+// (X, Y) plus FP32 deltas (dx, dy); the anchor is also the Halpern anchor, so
+// its delta is 0 and the Halpern step is next = w * (2 T(dz) - dz). At every
+// restart the anchor moves to T(z) in FP64 and the FP32 constants are
+// re-based on it: rc = c - A^T Y, [dl, du] = [l - X, u - X],
+// [drl, dru] = [rl - A X, ru - A X], sa = -Y / sigma. The FP32 quantities then
+// scale with the distance to the anchor, not with the solution, so FP32
+// rounding shrinks as the iterates converge.
+//
+// Primal: primal_point with c = rc, bounds [dl, du] and a zero anchor.
+// Dual, from h = (A dxb)_i:
+//     r = clamp(h - dy/sigma + sa, drl, dru);  dyp = dy + sigma (r - h).
+template <typename T>
+__device__ __forceinline__ DualOut<T> dual_point_delta(T dy, T h, T drl, T dru, T sa, T sigma, Halpern<T> hw) {
+    T r = h - dy / sigma + sa;
+    r = (drl < r) ? r : drl;
+    r = (r < dru) ? r : dru;
+    const T yp = dy + sigma * (r - h);
+    const T yb = T(2) * yp - dy;
+    return DualOut<T>{yp, hw.w * yb};
+}
+
 // ------------------------------------------------------------------ evaluation
 
 // Column terms at T(z) with g = (A^T yp)_j: 0 rd^2, 1 (rd/C)^2, 2 c x,
