@@ -124,6 +124,96 @@ def planning512(args):
     print(json.dumps(line), flush=True)
 
 
+def warmstart_workload(args):
+    """Learned warm start on a parametric family (fixed A; b, c move with
+    theta in R^64): training solves by the batched engine, POD bases and the
+    64->512->512->512 bf16 MLP fitted on the device, then the held-out thetas
+    solved cold and warm-started. Reports iterations and time to relative KKT
+    1e-6 for (i) the FP64 single-LP engine one LP at a time, cold (the plain
+    baseline; timed on a sample, scaled by exact iteration counts), (ii) the
+    batched mixed-precision engine, cold, (iii) the same, warm-started."""
+    import numpy as np
+    from nirnaya_amd.engine import Engine, MODE_PDLP, PREC_MIXED, default_params
+    from nirnaya_amd.warmstart import ParametricFamily, Pod, WarmStart
+    m, n = args.ws_m, args.ws_n
+    fam = ParametricFamily(20264, m, n, 5)
+    eng = Engine(0)
+    mixed = default_params(mode=MODE_PDLP, precision=PREC_MIXED, max_iters=400000, tol=1e-6)
+    th_train = fam.thetas(args.ws_train, 1)
+    th_test = fam.thetas(args.ws_test, 2)
+
+    def run_batch(thetas, params, x0s=None, y0s=None):
+        t0 = time.perf_counter()
+        b = eng.batch([fam.instance(t) for t in thetas], params, x0s=x0s, y0s=y0s)
+        b.solve()
+        dt = time.perf_counter() - t0
+        res = [b.result(i) for i in range(len(thetas))]
+        b.close()
+        return res, dt
+
+    res, t_train = run_batch(th_train, mixed)
+    ok = [i for i, r in enumerate(res) if r.status == 0]
+    log(f"[bench] {len(ok)}/{len(res)} training solves optimal in {t_train:.1f}s")
+    t0 = time.perf_counter()
+    pod_x = Pod(eng, np.stack([res[i].x for i in ok]), 256)
+    pod_y = Pod(eng, np.stack([res[i].y for i in ok]), 256)
+    t_pod = time.perf_counter() - t0
+    ws = WarmStart(eng, pod_x, pod_y, hidden=512, max_rows=256)
+    t0 = time.perf_counter()
+    losses = ws.fit(th_train[ok], epochs=args.ws_epochs, batch=256, lr=1e-3, seed=3)
+    t_fit = time.perf_counter() - t0
+    rows, ms, flops = ws.net.train_stats()
+    log(f"[bench] POD ranks {pod_x.rank}/{pod_y.rank} ({t_pod:.1f}s); MLP loss {losses[0]:.3e} -> {losses[-1]:.3e} "
+        f"in {t_fit:.1f}s ({flops / (ms * 1e-3) / 1e12:.2f} TFLOP/s in the training GEMMs)")
+    t0 = time.perf_counter()
+    preds = [ws.predict(t) for t in th_test]
+    t_pred = time.perf_counter() - t0
+    cold, t_cold = run_batch(th_test, mixed)
+    warm, t_warm = run_batch(th_test, mixed, [p[0] for p in preds], [p[1] for p in preds])
+    # Cold start is the fallback of a warm solve that misses the tolerance.
+    fallback = [i for i, r in enumerate(warm) if r.status != 0]
+    if fallback:
+        redo, t_redo = run_batch(th_test[fallback], mixed)
+        t_warm += t_redo
+        for k, i in enumerate(fallback):
+            warm[i] = redo[k]
+    plain = default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-6)
+    samp = list(range(0, len(th_test), max(1, len(th_test) // 16)))
+    ts, si, its64 = 0.0, 0, []
+    for i in samp:
+        t0 = time.perf_counter()
+        r = eng.solve(fam.instance(th_test[i]), plain)
+        ts += time.perf_counter() - t0
+        si += r.iterations
+        its64.append(r.iterations)
+    plain_per_lp = ts / len(samp)
+    ci = np.array([r.iterations for r in cold])
+    wi = np.array([r.iterations for r in warm])
+    ws.net.close()
+    eng.close()
+    line = {
+        "metric": "time-to-1e-6 relative KKT on held-out parametric LPs, learned warm start",
+        "value": round(t_warm + t_pred, 3), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"parametric equality LP family m={m} n={n} (5 nnz per column), theta in R^64, "
+                               f"{args.ws_train} training solves, {args.ws_test} held-out",
+                   "pod_rank_x": pod_x.rank, "pod_rank_y": pod_y.rank, "mlp": "64-512-512-512 bf16 GELU"},
+        "held_out_iterations": {"cold_median": float(np.median(ci)), "warm_median": float(np.median(wi)),
+                                "cold_mean": float(ci.mean()), "warm_mean": float(wi.mean()),
+                                "saved_fraction_mean": float(1 - wi.sum() / ci.sum()),
+                                "fallbacks": len(fallback)},
+        "time_s": {"plain_fp64_one_at_a_time": round(plain_per_lp * len(th_test), 3),
+                   "batched_mixed_cold": round(t_cold, 3), "batched_mixed_warm": round(t_warm, 3),
+                   "prediction": round(t_pred, 3), "training_solves": round(t_train, 3),
+                   "pod": round(t_pod, 3), "mlp_training": round(t_fit, 3)},
+        "speedup_vs_plain_fp64": round(plain_per_lp * len(th_test) / (t_warm + t_pred), 2),
+        "plain_sample": f"{len(samp)} held-out LPs solved one at a time, {ts:.2f}s, iterations median "
+                        f"{float(np.median(its64))}",
+        "mlp_train_tflops": round(flops / (ms * 1e-3) / 1e12, 3),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,11 +223,18 @@ def main():
     ap.add_argument("--tile", type=int, default=-1, help="LDS tile (-1: autotune tile, rows and segments)")
     ap.add_argument("--seg", type=int, default=0, help="column-segment width of A xbar when not autotuning")
     ap.add_argument("--seg-t", type=int, default=-1, help="column-segment width of A^T y (-1: as --seg)")
+    ap.add_argument("--fused", type=int, default=0, help="walk all segments in one launch when not autotuning")
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the 256 MiB copy kernel (reference for rocprofv3 byte counters)")
-    ap.add_argument("--workload", default="large-lp", choices=["large-lp", "planning512"],
-                    help="large-lp: the headline line (default); planning512: the batched workload")
+    ap.add_argument("--workload", default="large-lp", choices=["large-lp", "planning512", "warmstart"],
+                    help="large-lp: the headline line (default); planning512: the batched workload; "
+                         "warmstart: the learned warm start on a parametric family")
+    ap.add_argument("--ws-m", type=int, default=5000)
+    ap.add_argument("--ws-n", type=int, default=12000)
+    ap.add_argument("--ws-train", type=int, default=2048)
+    ap.add_argument("--ws-test", type=int, default=128)
+    ap.add_argument("--ws-epochs", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is design-only here.",
@@ -145,6 +242,9 @@ def main():
         sys.exit(2)
     if args.workload == "planning512":
         planning512(args)
+        return
+    if args.workload == "warmstart":
+        warmstart_workload(args)
         return
 
     from nirnaya_amd.engine import Engine, MODE_REFERENCE, default_params, gen_mixed_lp
@@ -159,12 +259,12 @@ def main():
         log(f"[bench] calibration copy of 256 MiB: {ms:.3f} ms ({2 * 256 * 1.048576 / ms:.0f} GB/s read+write)")
     params = default_params(mode=MODE_REFERENCE, max_iters=args.warmup + 2 * args.steps + 64, check_termination=0,
                             tile_nnz=args.tile, tile_rows=args.tile_rows, seg_width=args.seg,
-                            seg_width_t=args.seg_t)
+                            seg_width_t=args.seg_t, seg_fused=args.fused)
     sess = eng.session(lp, params)
     shape = sess.shape()
     log(f"[bench] session ready (upload, tile autotune, power iteration) in {sess.result().info.setup_seconds:.2f}s; "
-        f"shape A x: tile {shape[0]} rows {shape[1]} segments {shape[2]}, A^T y: tile {shape[3]} rows {shape[4]} "
-        f"segments {shape[5]}")
+        f"shape A x: tile {shape[0]} rows {shape[1]} segments {shape[2]} fused {shape[3]}, A^T y: tile {shape[4]} "
+        f"rows {shape[5]} segments {shape[6]} fused {shape[7]}")
     sess.run(args.warmup)
     log(f"[bench] warmup {args.warmup} iterations done")
     # Two sweeps of `steps` iterations: the production path (graph replays)
@@ -217,7 +317,7 @@ def main():
                      "bytes_per_launch": t.spmv_bytes, "avg_launch_us": spmv_avg_s * 1e6,
                      "dispatches_per_launch": t.spmv_passes,
                      "avg_dispatch_us": spmv_avg_s * 1e6 / t.spmv_passes,
-                     "shape": {"tile": shape[0], "rows": shape[1], "segments": shape[2]}},
+                     "shape": {"tile": shape[0], "rows": shape[1], "segments": shape[2], "fused": shape[3]}},
         "kernels": {"spmtv_primal_avg_us": spmtv_avg_s * 1e6, "spmtv_dispatches_per_launch": t.spmtv_passes,
                     "spmtv_bytes_per_launch": t.spmtv_bytes,
                     "spmtv_gbs": t.spmtv_bytes / spmtv_avg_s / 1e9,

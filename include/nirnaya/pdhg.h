@@ -71,6 +71,9 @@ typedef struct {
     double primal_weight;      /* pdlp: initial primal weight in the engine's scaled space
                                   (0 = ||c|| / ||b|| of the scaled LP); a warm start can carry
                                   the weight its family's solves ended with */
+    int32_t seg_fused;         /* reference mode, segmented products: walk all segments in one
+                                  launch with register running sums (1) or one launch per
+                                  segment (0); the autotuner chooses when tile_nnz = -1 */
 } nir_params;
 
 typedef struct {
@@ -102,11 +105,15 @@ typedef struct {
     double spmv_bytes;          /* algorithmic bytes of one A xbar launch */
     double spmtv_bytes;         /* algorithmic bytes of one A^T y launch */
     double iter_bytes;          /* algorithmic bytes of one whole iteration */
-    int32_t spmv_passes;        /* column-segment passes per A xbar */
-    int32_t spmtv_passes;       /* column-segment passes per A^T y */
+    int32_t spmv_passes;        /* kernel dispatches per A xbar (segment passes, 1 when fused) */
+    int32_t spmtv_passes;       /* kernel dispatches per A^T y */
 } nir_timing;
 
 NIR_API void nir_params_default(nir_params* p);
+/* Layout check for bindings: {sizeof, offsetof(last field)} of nir_lp,
+ * nir_params, nir_info, nir_timing, nir_batch_opts, nir_batch_stats (12
+ * values); returns the count, writes them when cap allows. */
+NIR_API int nir_abi_layout(int64_t* out, int32_t cap);
 
 NIR_API int nir_ctx_create(int device, nir_ctx** out);
 NIR_API void nir_ctx_destroy(nir_ctx* ctx);
@@ -121,8 +128,9 @@ NIR_API int nir_session_solve(nir_session* s);
 /* Copies the current iterate (x: ncol, y: nrow; either may be NULL). */
 NIR_API int nir_session_result(nir_session* s, double* x, double* y, nir_info* info);
 NIR_API int nir_session_time(nir_session* s, int64_t iters, nir_timing* out);
-/* Tile shapes in use: {A tile, A rows, A segments, A^T tile, A^T rows, A^T segments}. */
-NIR_API int nir_session_shape(nir_session* s, int32_t* out6);
+/* Tile shapes in use: {A tile, A rows, A segments, A fused, A^T tile, A^T rows,
+ * A^T segments, A^T fused} (fused = all segments walked in one launch). */
+NIR_API int nir_session_shape(nir_session* s, int32_t* out8);
 NIR_API void nir_session_destroy(nir_session* s);
 
 /* Copies `bytes` (read) into a second buffer (write) with a plain grid-stride

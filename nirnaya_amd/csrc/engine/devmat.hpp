@@ -77,6 +77,7 @@ struct TileShape {
     int tile = 2048;
     int rows = 256;
     int seg_width = 0;
+    int fused = 0;   // walk all segments in one launch (SegCsr) when the rows allow it
 };
 
 // Structure (row pointers per segment, column indices, row blocks) of a
@@ -88,6 +89,10 @@ struct CsrShape {
     DevArray<int> col;      // nnz, segment-major
     DevArray<int4> desc;    // concatenated per-segment block descriptors
     std::vector<int> blk_off, nblk;
+    // Fused-segment blocks (SegCsr): row ranges and per-segment nonzero ranges.
+    DevArray<int2> frows, fnz;
+    int fnblk = 0;
+    bool fused_ok = false;
 
     template <typename T>
     Csr<T> view(const T* val, int s) const {
@@ -98,7 +103,41 @@ struct CsrShape {
     Pass<T> pass(const T* val, T* part, int s) const {
         return Pass<T>{view(val, s), part, s == 0 ? 1 : 0, s == nseg - 1 ? 1 : 0};
     }
+    template <typename T>
+    SegCsr<T> fused(const T* val) const {
+        return SegCsr<T>{nrows, nseg, rowptr.p, col.p, val, fnblk, frows.p, fnz.p};
+    }
+    // Blocks whose per-block results (e.g. violation maxima) the last launch writes.
+    int result_blocks() const { return fused_ok ? fnblk : nblk[nseg - 1]; }
 };
+
+// Fused-segment partition: consecutive rows, at most `rows_max`, whose
+// nonzeros in every segment fit one tile. Fails (returns false) when a single
+// row has more than `tile` nonzeros in some segment.
+inline bool partition_fused(const std::vector<int>& rp, int nrows, int nseg, int tile, int rows_max,
+                            std::vector<int2>& rows, std::vector<int2>& nz) {
+    rows.clear();
+    nz.clear();
+    auto seg_ptr = [&](int s, int r) { return rp[static_cast<size_t>(s) * (nrows + 1) + r]; };
+    int r = 0;
+    while (r < nrows) {
+        const int s0 = r;
+        while (r < nrows && r - s0 < rows_max) {
+            bool fits = true;
+            for (int s = 0; s < nseg && fits; ++s) fits = seg_ptr(s, r + 1) - seg_ptr(s, s0) <= tile;
+            if (!fits) break;
+            ++r;
+        }
+        if (r == s0) return false;
+        rows.push_back(make_int2(s0, r));
+        for (int s = 0; s < nseg; ++s) nz.push_back(make_int2(seg_ptr(s, s0), seg_ptr(s, r)));
+    }
+    if (rows.empty()) {   // empty matrix: one block without rows
+        rows.push_back(make_int2(0, 0));
+        for (int s = 0; s < nseg; ++s) nz.push_back(make_int2(0, 0));
+    }
+    return true;
+}
 
 // Upload a CSR matrix (columns ascending within rows) in the segmented
 // layout. `perm` receives, for each stored slot, the input nonzero it holds.
@@ -152,6 +191,19 @@ inline void upload_shape(CsrShape& S, int nrows, int ncols, const std::vector<in
     S.col.upload(c2.data(), c2.size(), st);
     S.desc.alloc(desc_all.size());
     S.desc.upload(desc_all.data(), desc_all.size(), st);
+    S.fused_ok = false;
+    S.fnblk = 0;
+    if (shape.fused && nseg > 1) {
+        std::vector<int2> fr, fz;
+        if (partition_fused(rp, nrows, nseg, shape.tile, std::min(shape.rows, kThreads), fr, fz)) {
+            S.fused_ok = true;
+            S.fnblk = static_cast<int>(fr.size());
+            S.frows.alloc(fr.size());
+            S.frows.upload(fr.data(), fr.size(), st);
+            S.fnz.alloc(fz.size());
+            S.fnz.upload(fz.data(), fz.size(), st);
+        }
+    }
     HIP_CHECK(hipStreamSynchronize(st));
 }
 

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -49,15 +50,15 @@ struct RefState {
     unsigned long long vslot[2];   // max row violation of iteration k in slot k & 1 (bits of a double >= 0)
 };
 
-template <int TILE>
-__global__ __launch_bounds__(kThreads) void k_spmv_store(Pass<double> P, const double* __restrict__ v,
+template <int TILE, typename MV>
+__global__ __launch_bounds__(kThreads) void k_spmv_store(MV P, const double* __restrict__ v,
                                                          double* __restrict__ out) {
     __shared__ double prod[lds_slots(TILE)];
     stream_rows<TILE>(P, v, prod, blockIdx.x, [&](int r, double s) { out[r] = s; });
 }
 
-template <int TILE>
-__global__ __launch_bounds__(kThreads) void k_primal_ref(Pass<double> P, const double* __restrict__ y,
+template <int TILE, typename MV>
+__global__ __launch_bounds__(kThreads) void k_primal_ref(MV P, const double* __restrict__ y,
                                                          double* __restrict__ x, double* __restrict__ xbar,
                                                          const double* __restrict__ c,
                                                          const double* __restrict__ lo,
@@ -97,8 +98,8 @@ __global__ __launch_bounds__(kThreads) void k_primal_ref(Pass<double> P, const d
     });
 }
 
-template <int TILE>
-__global__ __launch_bounds__(kThreads) void k_dual_ref(Pass<double> P, const double* __restrict__ xbar,
+template <int TILE, typename MV>
+__global__ __launch_bounds__(kThreads) void k_dual_ref(MV P, const double* __restrict__ xbar,
                                                        double* __restrict__ y, const double* __restrict__ rl,
                                                        const double* __restrict__ ru, double sigma,
                                                        const RefState* st, double* __restrict__ vpart) {
@@ -162,15 +163,20 @@ __global__ __launch_bounds__(256) void k_copy_calibrate(const double4* __restric
 bool host_finite(double b) { return std::isfinite(b) && std::fabs(b) < 1e20; }
 
 void spmv_store(const DevMatrix& M, const double* v, double* out, hipStream_t s) {
-    for (int p = 0; p < M.shape.nseg; ++p) {
-        const Pass<double> P = M.shape.pass(M.val.p, M.part.p, p);
-        TILE_DISPATCH(M.shape.shape.tile, k_spmv_store<TL><<<P.M.nblk, kThreads, 0, s>>>(P, v, out));
+    if (M.shape.fused_ok) {
+        const SegCsr<double> F = M.shape.fused(M.val.p);
+        TILE_DISPATCH(M.shape.shape.tile, k_spmv_store<TL><<<F.nblk, kThreads, 0, s>>>(F, v, out));
+    } else {
+        for (int p = 0; p < M.shape.nseg; ++p) {
+            const Pass<double> P = M.shape.pass(M.val.p, M.part.p, p);
+            TILE_DISPATCH(M.shape.shape.tile, k_spmv_store<TL><<<P.M.nblk, kThreads, 0, s>>>(P, v, out));
+        }
     }
     HIP_CHECK(hipGetLastError());
 }
 
-// Autotuner: time y = M v for every candidate tile shape on the real matrix
-// and keep the fastest. Results are cached per matrix signature; the choice
+// Autotuner: time y = M v for every candidate layout (segment width, tile,
+// fused or per-segment launches) on the real matrix and keep the fastest. Results are cached per matrix signature; the choice
 // never changes results except for rows longer than a tile.
 TileShape autotune(int nrows, int ncols, const std::vector<int>& rowptr, const int* col, const double* val,
                    hipStream_t s) {
@@ -183,10 +189,11 @@ TileShape autotune(int nrows, int ncols, const std::vector<int>& rowptr, const i
         if (it != cache.end()) return it->second;
     }
     std::vector<TileShape> cands;
-    for (int w : {0, 1 << 18, 1 << 17})
+    for (int w : {0, 1 << 19, 1 << 18, 1 << 17})
         if (w == 0 || w < ncols)
-            for (int t : {2048, 4096})
-                for (int r : {128, 256}) cands.push_back(TileShape{t, r, w});
+            for (int t : {1024, 2048, 4096})
+                for (int f : {0, 1})
+                    if (f == 0 || w > 0) cands.push_back(TileShape{t, 256, w, f});
     DevArray<double> v, out;
     v.alloc(std::max(ncols, 1));
     out.alloc(std::max(nrows, 1));
@@ -285,6 +292,14 @@ namespace {
 void launch_primal_ref(nir_session* S, int k_lo) {
     const int check = S->prm.check_termination ? 1 : 0;
     const DevMatrix& M = S->AT;
+    if (M.shape.fused_ok) {
+        const SegCsr<double> F = M.shape.fused(M.val.p);
+        TILE_DISPATCH(M.shape.shape.tile,
+                      k_primal_ref<TL><<<F.nblk, kThreads, 0, S->s>>>(F, S->y.p, S->x.p, S->xbar.p, S->c.p, S->lo.p,
+                                                                     S->up.p, S->tau, S->st.p, k_lo, check));
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
     for (int p = 0; p < M.shape.nseg; ++p) {
         const Pass<double> P = M.shape.pass(M.val.p, M.part.p, p);
         TILE_DISPATCH(M.shape.shape.tile,
@@ -298,15 +313,22 @@ void launch_primal_ref(nir_session* S, int k_lo) {
 // k is the global iteration index (used only to place the violation reduction).
 void launch_dual_ref(nir_session* S, int k_lo, long long k) {
     const DevMatrix& M = S->A;
+    if (M.shape.fused_ok) {
+        const SegCsr<double> F = M.shape.fused(M.val.p);
+        TILE_DISPATCH(M.shape.shape.tile,
+                      k_dual_ref<TL><<<F.nblk, kThreads, 0, S->s>>>(F, S->xbar.p, S->y.p, S->rl.p, S->ru.p,
+                                                                   S->sigma, S->st.p, S->vpart.p));
+    } else {
     for (int p = 0; p < M.shape.nseg; ++p) {
         const Pass<double> P = M.shape.pass(M.val.p, M.part.p, p);
         TILE_DISPATCH(M.shape.shape.tile,
                       k_dual_ref<TL><<<P.M.nblk, kThreads, 0, S->s>>>(P, S->xbar.p, S->y.p, S->rl.p, S->ru.p,
                                                                        S->sigma, S->st.p, S->vpart.p));
     }
+    }
     // The stop rule reads the violation of iterations with k % 32 == 0.
     if (S->prm.check_termination && (k % 32) == 0)
-        k_vmax_reduce<<<1, kThreads, 0, S->s>>>(S->vpart.p, M.shape.nblk[M.shape.nseg - 1], S->st.p, k_lo);
+        k_vmax_reduce<<<1, kThreads, 0, S->s>>>(S->vpart.p, M.shape.result_blocks(), S->st.p, k_lo);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -434,6 +456,7 @@ TileShape shape_from_params(const nir_params& p, bool transpose) {
     if (p.seg_width < 0) throw std::invalid_argument("seg_width must be >= 0");
     if (p.seg_width_t < -1) throw std::invalid_argument("seg_width_t must be >= -1");
     t.seg_width = (transpose && p.seg_width_t >= 0) ? p.seg_width_t : p.seg_width;
+    t.fused = p.seg_fused ? 1 : 0;
     return t;
 }
 
@@ -458,6 +481,21 @@ NIR_API void nir_params_default(nir_params* p) {
 }
 
 NIR_API const char* nir_last_error(void) { return g_last_error.c_str(); }
+
+NIR_API int nir_abi_layout(int64_t* out, int32_t cap) {
+    const int64_t v[] = {
+        static_cast<int64_t>(sizeof(nir_lp)),          static_cast<int64_t>(offsetof(nir_lp, obj_offset)),
+        static_cast<int64_t>(sizeof(nir_params)),      static_cast<int64_t>(offsetof(nir_params, seg_fused)),
+        static_cast<int64_t>(sizeof(nir_info)),        static_cast<int64_t>(offsetof(nir_info, solve_seconds)),
+        static_cast<int64_t>(sizeof(nir_timing)),      static_cast<int64_t>(offsetof(nir_timing, spmtv_passes)),
+        static_cast<int64_t>(sizeof(nir_batch_opts)),  static_cast<int64_t>(offsetof(nir_batch_opts, workgroups)),
+        static_cast<int64_t>(sizeof(nir_batch_stats)), static_cast<int64_t>(offsetof(nir_batch_stats, solve_seconds)),
+    };
+    const int32_t n = static_cast<int32_t>(sizeof(v) / sizeof(v[0]));
+    if (!out || cap < n) return n;
+    for (int32_t i = 0; i < n; ++i) out[i] = v[i];
+    return n;
+}
 
 NIR_API int nir_ctx_create(int device, nir_ctx** out) {
     if (!out) return -1;
@@ -550,7 +588,7 @@ NIR_API int nir_session_create(nir_ctx* ctx, const nir_lp* lp, const nir_params*
             S->x.alloc(n); S->x.upload(x0.data(), n, S->s);
             S->xbar.alloc(n); S->xbar.upload(x0.data(), n, S->s);
             S->y.alloc(m); S->y.upload(y0.data(), m, S->s);
-            S->vpart.alloc(static_cast<size_t>(S->A.shape.nblk[S->A.shape.nseg - 1]));
+            S->vpart.alloc(static_cast<size_t>(S->A.shape.result_blocks()));
             {
                 std::vector<double> zeros(S->vpart.n, 0.0);
                 S->vpart.upload(zeros.data(), zeros.size(), S->s);
@@ -688,8 +726,8 @@ NIR_API int nir_session_time(nir_session* S, int64_t iters, nir_timing* out) {
         }
         for (auto& e : ev) HIP_CHECK(hipEventDestroy(e));
         out->iterations = iters;
-        out->spmv_passes = S->A.shape.nseg;
-        out->spmtv_passes = S->AT.shape.nseg;
+        out->spmv_passes = S->A.shape.fused_ok ? 1 : S->A.shape.nseg;
+        out->spmtv_passes = S->AT.shape.fused_ok ? 1 : S->AT.shape.nseg;
         S->k_next = k;
         finish_ref(S);
         return 0;
@@ -699,14 +737,16 @@ NIR_API int nir_session_time(nir_session* S, int64_t iters, nir_timing* out) {
     }
 }
 
-NIR_API int nir_session_shape(nir_session* S, int32_t* out6) {
-    if (!S || !out6) return -1;
-    out6[0] = S->A.shape.shape.tile;
-    out6[1] = S->A.shape.shape.rows;
-    out6[2] = S->A.shape.nseg;
-    out6[3] = S->AT.shape.shape.tile;
-    out6[4] = S->AT.shape.shape.rows;
-    out6[5] = S->AT.shape.nseg;
+NIR_API int nir_session_shape(nir_session* S, int32_t* out8) {
+    if (!S || !out8) return -1;
+    out8[0] = S->A.shape.shape.tile;
+    out8[1] = S->A.shape.shape.rows;
+    out8[2] = S->A.shape.nseg;
+    out8[3] = S->A.shape.fused_ok ? 1 : 0;
+    out8[4] = S->AT.shape.shape.tile;
+    out8[5] = S->AT.shape.shape.rows;
+    out8[6] = S->AT.shape.nseg;
+    out8[7] = S->AT.shape.fused_ok ? 1 : 0;
     return 0;
 }
 

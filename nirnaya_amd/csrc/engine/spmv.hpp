@@ -159,6 +159,97 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
     stream_rows<TILE>(P, v, prod, b, PreNone{}, [&](int r, T s, NoPre) { on_row(r, s); });
 }
 
+// ------------------------------------------------------------------ fused segments
+
+// All column segments of a product in one launch: block b owns rows
+// [rows[b].x, rows[b].y) (at most 256, one lane per row) and walks the
+// segments in ascending column order; its nonzeros of segment s are
+// nz[b * nseg + s] = [p0, p1), at most TILE of them. Each lane keeps its
+// row's running sum in a register across segments, so no partial sums go
+// through memory, and the order of every sum is still ascending columns (bit
+// for bit the single-pass result). All blocks of a launch are resident at
+// once and walk the segments nearly in step, so the gathered vector's working
+// set stays about one segment wide.
+template <typename T>
+struct SegCsr {
+    int nrows;
+    int nseg;
+    const int* rowptr;   // nseg * (nrows + 1), absolute offsets
+    const int* col;
+    const T* val;
+    int nblk;
+    const int2* rows;    // nblk
+    const int2* nz;      // nblk * nseg
+    static constexpr int first = 1, last = 1;
+};
+
+// Row sums of A * v for block b over every segment; on_row(row, sum, pre) as
+// in stream_rows. The next segment's index and value loads are issued before
+// the current segment's sums are read from LDS.
+template <int TILE, typename T, typename V, typename Pre, typename F>
+__device__ __forceinline__ void stream_rows(const SegCsr<T>& A, const V* __restrict__ v, T* prod, int b,
+                                            Pre&& prefetch, F&& on_row) {
+    static_assert(TILE % kThreads == 0, "tile must be a multiple of the workgroup");
+    constexpr int PER = TILE / kThreads;
+    const int tid = threadIdx.x;
+    const int2 rr2 = A.rows[b];
+    const int r0 = rr2.x, r1 = rr2.y;
+    if (r1 == r0) return;
+    const int r = r0 + tid;
+    const bool has_row = r < r1;
+    const int rr = has_row ? r : r0;
+    const auto pre = prefetch(rr);
+    const int2* nzb = A.nz + static_cast<size_t>(b) * A.nseg;
+    int ci[PER];
+    T vi[PER];
+    auto load = [&](int seg, int (&c)[PER], T (&w)[PER]) {
+        const int2 pz = nzb[seg];
+        const int cnt = pz.y - pz.x;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int q = tid + u * kThreads;
+            const bool in = q < cnt;
+            const int slot = in ? pz.x + q : 0;
+            const int cl = load_stream(A.col + slot);
+            const T vl = load_stream(A.val + slot);
+            c[u] = in ? cl : 0;
+            w[u] = in ? vl : T(0);
+        }
+    };
+    load(0, ci, vi);
+    T s = T(0);
+    for (int seg = 0; seg < A.nseg; ++seg) {
+        const int* rp = A.rowptr + static_cast<size_t>(seg) * (A.nrows + 1);
+        const int p0 = nzb[seg].x;
+        const int ra = rp[rr] - p0, re = rp[rr + 1] - p0;
+        T g[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) g[u] = static_cast<T>(v[ci[u]]);
+#pragma unroll
+        for (int u = 0; u < PER; ++u) prod[lds_at(tid + u * kThreads)] = vi[u] * g[u];
+        __syncthreads();
+        if (seg + 1 < A.nseg) load(seg + 1, ci, vi);
+        if (has_row) {
+            int q = ra;
+            for (; q + 4 <= re; q += 4) {
+                const T a0 = prod[lds_at(q)], a1 = prod[lds_at(q + 1)], a2 = prod[lds_at(q + 2)], a3 = prod[lds_at(q + 3)];
+                s += a0;
+                s += a1;
+                s += a2;
+                s += a3;
+            }
+            for (; q < re; ++q) s += prod[lds_at(q)];
+        }
+        __syncthreads();
+    }
+    if (has_row) on_row(r, s, pre);
+}
+
+template <int TILE, typename T, typename V, typename F>
+__device__ __forceinline__ void stream_rows(const SegCsr<T>& A, const V* __restrict__ v, T* prod, int b, F&& on_row) {
+    stream_rows<TILE>(A, v, prod, b, PreNone{}, [&](int r, T s, NoPre) { on_row(r, s); });
+}
+
 // Max of nonnegative values over the workgroup; result valid in thread 0.
 __device__ __forceinline__ double block_max(double v, double* red) {
 #pragma unroll

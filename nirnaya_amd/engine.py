@@ -57,6 +57,7 @@ class NirParams(C.Structure):
         ("seg_width_t", C.c_int32),
         ("x0", _dp), ("y0", _dp), ("unscale_row", _dp), ("unscale_col", _dp),
         ("should_stop", _STOP_CB), ("stop_user", C.c_void_p), ("primal_weight", C.c_double),
+        ("seg_fused", C.c_int32),
     ]
 
 
@@ -423,8 +424,8 @@ class Session:
         return t
 
     def shape(self):
-        """(A tile, A rows, A segments, A^T tile, A^T rows, A^T segments)."""
-        out = (C.c_int32 * 6)()
+        """(A tile, A rows, A segments, A fused, A^T tile, A^T rows, A^T segments, A^T fused)."""
+        out = (C.c_int32 * 8)()
         self.lib.nir_session_shape(self.h, out)
         return tuple(out)
 

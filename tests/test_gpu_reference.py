@@ -49,13 +49,33 @@ def test_tiles_and_graphs_bit_identical(engine, tile, rows, graph):
     assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
 
 
+@pytest.mark.parametrize("fused", [0, 1])
 @pytest.mark.parametrize("seg", [1000, 4096, 1 << 17])
-def test_column_segments_bit_identical(engine, seg):
-    # Segment passes carry the running row sums, so the order of every sum is
-    # unchanged: still bit-identical to the oracle.
+def test_column_segments_bit_identical(engine, seg, fused):
+    # Segment passes carry the running row sums (in memory between launches,
+    # or in registers when all segments are walked in one launch), so the
+    # order of every sum is unchanged: still bit-identical to the oracle.
     lp = gen_mixed_lp(17, 6000, 18000, 6.0, 0.01, 1e3)
-    r = run_ref(engine, lp, 50, seg_width=seg, tile_rows=128)
-    xo, yo, _, _, _ = Oracle(lp).pdhg(50, check=False)
+    r = run_ref(engine, lp, 50, seg_width=seg, tile_rows=128, seg_fused=fused)
+    xo, yo, _, it, V = Oracle(lp).pdhg(50, check=False)
+    assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+    assert r.info.max_primal_viol == pytest.approx(V, rel=1e-12)
+
+
+def test_fused_segments_full_size_cfg1(engine):
+    """m=400,000, n=1,200,000: 5 segments walked in one launch, 12 iterations."""
+    lp = gen_mixed_lp(20261, 400_000, 1_200_000, 6.0, 0.01, 1e3)
+    p = default_params(mode=MODE_REFERENCE, max_iters=12, check_termination=0, seg_width=1 << 18, seg_width_t=0,
+                       seg_fused=1)
+    s = engine.session(lp, p)
+    try:
+        s.solve()
+        r = s.result()
+        shape = s.shape()
+    finally:
+        s.close()
+    assert shape[2] == 5 and shape[3] == 1
+    xo, yo, _, _, _ = Oracle(lp).pdhg(12, check=False)
     assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
 
 

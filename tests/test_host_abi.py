@@ -188,3 +188,21 @@ def test_cli_flag_forms_reach_the_option(tmp_path):
                        capture_output=True)
         txt = stats.read_text()
         assert '"status": "not_supported"' not in txt, (form, txt)
+
+
+def test_python_struct_layouts_match_the_header():
+    """Every ctypes mirror of an engine struct has the C size and the C offset
+    of its last field (a stray or missing field shifts everything after it)."""
+    import ctypes as C
+    from nirnaya_amd.engine import (NirBatchOpts, NirBatchStats, NirInfo, NirLp, NirParams, NirTiming,
+                                    load_library)
+    lib = load_library()
+    lib.nir_abi_layout.argtypes = [C.POINTER(C.c_int64), C.c_int32]
+    n = lib.nir_abi_layout(None, 0)
+    out = (C.c_int64 * n)()
+    assert lib.nir_abi_layout(out, n) == n == 12
+    mirrors = [(NirLp, "obj_offset"), (NirParams, "seg_fused"), (NirInfo, "solve_seconds"),
+               (NirTiming, "spmtv_passes"), (NirBatchOpts, "workgroups"), (NirBatchStats, "solve_seconds")]
+    for k, (cls, last) in enumerate(mirrors):
+        assert C.sizeof(cls) == out[2 * k], cls.__name__
+        assert getattr(cls, last).offset == out[2 * k + 1], cls.__name__
