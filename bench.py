@@ -169,6 +169,12 @@ def warmstart_workload(args):
     preds = [ws.predict(t) for t in th_test]
     t_pred = time.perf_counter() - t0
     cold, t_cold = run_batch(th_test, mixed)
+    # Prediction quality on the held-out solutions: error of the prediction
+    # and of the POD mean alone, relative to the solution's distance from the
+    # mean.
+    ex = [np.linalg.norm(p[0] - r.x) / max(np.linalg.norm(r.x - pod_x.mean), 1e-30) for p, r in zip(preds, cold)]
+    ey = [np.linalg.norm(p[1] - r.y) / max(np.linalg.norm(r.y - pod_y.mean), 1e-30) for p, r in zip(preds, cold)]
+    log(f"[bench] held-out prediction error / distance from the mean: x {np.median(ex):.3f}, y {np.median(ey):.3f}")
     warm, t_warm = run_batch(th_test, mixed, [p[0] for p in preds], [p[1] for p in preds])
     # Cold start is the fallback of a warm solve that misses the tolerance.
     fallback = [i for i, r in enumerate(warm) if r.status != 0]
@@ -202,6 +208,9 @@ def warmstart_workload(args):
                                 "cold_mean": float(ci.mean()), "warm_mean": float(wi.mean()),
                                 "saved_fraction_mean": float(1 - wi.sum() / ci.sum()),
                                 "fallbacks": len(fallback)},
+        "held_out_prediction_error": {"x_median": float(np.median(ex)), "y_median": float(np.median(ey)),
+                                      "note": "||prediction - solution|| / ||solution - POD mean||"},
+        "mlp_loss": {"first_epoch": float(losses[0]), "last_epoch": float(losses[-1])},
         "time_s": {"plain_fp64_one_at_a_time": round(plain_per_lp * len(th_test), 3),
                    "batched_mixed_cold": round(t_cold, 3), "batched_mixed_warm": round(t_warm, 3),
                    "prediction": round(t_pred, 3), "training_solves": round(t_train, 3),
@@ -234,7 +243,7 @@ def main():
     ap.add_argument("--ws-n", type=int, default=12000)
     ap.add_argument("--ws-train", type=int, default=2048)
     ap.add_argument("--ws-test", type=int, default=128)
-    ap.add_argument("--ws-epochs", type=int, default=200)
+    ap.add_argument("--ws-epochs", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is design-only here.",

@@ -94,10 +94,18 @@ class WarmStart:
         self.sd = np.ones(self.n_out)
 
     def fit(self, thetas: np.ndarray, epochs: int = 200, batch: int = 256, lr: float = 1e-3, seed: int = 0):
-        """Train on the POD coefficients of the training snapshots (z-scored)."""
+        """Train on the POD coefficients of the training snapshots, centred and
+        divided by one scale for the x block and one for the y block: the
+        squared error then weighs each mode by its share of the snapshot
+        variance (the POD modes are orthonormal, so it is the squared error of
+        the reconstructed x and y up to those two scales) instead of giving
+        the noisy trailing modes the weight of the leading ones."""
         alpha = np.concatenate([self.pod_x.train_coef, self.pod_y.train_coef], axis=1)
         self.mu[:self.n_coef] = alpha.mean(axis=0)
-        self.sd[:self.n_coef] = np.maximum(alpha.std(axis=0), 1e-12)
+        rx = self.pod_x.rank
+        cx, cy = alpha[:, :rx] - self.mu[:rx], alpha[:, rx:] - self.mu[rx:self.n_coef]
+        self.sd[:rx] = max(float(np.sqrt(np.mean(np.sum(cx * cx, axis=1)) / rx)), 1e-12)
+        self.sd[rx:self.n_coef] = max(float(np.sqrt(np.mean(np.sum(cy * cy, axis=1)) / max(1, cy.shape[1]))), 1e-12)
         target = np.zeros((alpha.shape[0], self.n_out), dtype=np.float32)
         target[:, :self.n_coef] = (alpha - self.mu[:self.n_coef]) / self.sd[:self.n_coef]
         return self.net.train(thetas.astype(np.float32), target, epochs, batch, lr, seed)
