@@ -117,3 +117,57 @@ def test_batch_warm_starts_and_primal_weight_match_single(engine):
             print(f"prec {prec} instance {i}: it {res[i].iterations}/{s.iterations} omega0 0.7 "
                   f"exact {np.array_equal(res[i].x, s.x) and np.array_equal(res[i].y, s.y)}")
             assert same(res[i], s)
+
+
+def test_batch_edge_shapes_match_single(engine):
+    # Free and one-sided rows, ranged rows, an empty row, an empty column,
+    # free and one-sided columns; an LP without nonzeros; a 1x1 LP.
+    from nirnaya_amd.engine import LP
+    rng = np.random.default_rng(12)
+    m, n = 40, 90
+    A = np.zeros((m, n))
+    for j in range(n):
+        rows = rng.choice(m - 1, size=3, replace=False) + 1
+        A[rows, j] = rng.standard_normal(3)
+    A[:, 5] = 0.0          # empty column
+    A[0, :] = 0.0          # empty row
+    inf = np.inf
+    lo = np.where(rng.random(n) < 0.2, -inf, 0.0)
+    up = np.where(rng.random(n) < 0.3, inf, rng.uniform(1, 5, n))
+    rl = rng.uniform(-1, 0, m)
+    ru = rl + rng.uniform(0, 2, m)
+    rl[rng.random(m) < 0.3] = -inf
+    ru[rng.random(m) < 0.3] = inf
+    rl[7], ru[7] = -inf, inf
+    c = rng.standard_normal(n)
+    edge = LP.from_dense(A, c, lo, up, rl, ru)
+    empty = LP.from_dense(np.zeros((3, 4)), np.array([1.0, -1.0, 0.0, 2.0]), np.zeros(4), np.ones(4),
+                          np.full(3, -1.0), np.full(3, 1.0))
+    one = LP.from_dense(np.array([[2.0]]), np.array([1.0]), np.array([0.0]), np.array([10.0]),
+                        np.array([4.0]), np.array([4.0]))
+    lps = [edge, empty, one, gen_staircase_lp(20263000, 4, 5)]
+    for prec in (0, 1):
+        params = default_params(mode=MODE_PDLP, precision=prec, max_iters=20000, tol=1e-6)
+        check_batch(engine, lps, params)
+
+
+def test_batch_should_stop_reports_time_limit(engine):
+    from nirnaya_amd.engine import _STOP_CB
+    calls = []
+
+    def stop(_user):
+        calls.append(1)
+        return 1 if len(calls) >= 2 else 0
+
+    cb = _STOP_CB(stop)
+    lps = [gen_planning_lp(20263 * 1000 + i) for i in (2, 3)]
+    params = default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-9)
+    params.should_stop = cb
+    b = engine.batch(lps, params, quantum=1, rounds_per_graph=1)
+    try:
+        b.solve()
+        res = [b.result(i) for i in range(len(lps))]
+    finally:
+        b.close()
+    assert all(r.status == 4 for r in res)
+    assert all(0 < r.iterations < 400000 for r in res)
