@@ -158,7 +158,8 @@ def warmstart_workload(args):
     pod_x = Pod(eng, np.stack([res[i].x for i in ok]), 256)
     pod_y = Pod(eng, np.stack([res[i].y for i in ok]), 256)
     t_pod = time.perf_counter() - t0
-    ws = WarmStart(eng, pod_x, pod_y, hidden=512, max_rows=256)
+    ws = WarmStart(eng, pod_x, pod_y, hidden=512, max_rows=256,
+                   primal_weight=float(np.median([res[i].info.primal_weight for i in ok])))
     t0 = time.perf_counter()
     losses = ws.fit(th_train[ok], epochs=args.ws_epochs, batch=256, lr=1e-3, seed=3)
     t_fit = time.perf_counter() - t0
@@ -175,7 +176,9 @@ def warmstart_workload(args):
     ex = [np.linalg.norm(p[0] - r.x) / max(np.linalg.norm(r.x - pod_x.mean), 1e-30) for p, r in zip(preds, cold)]
     ey = [np.linalg.norm(p[1] - r.y) / max(np.linalg.norm(r.y - pod_y.mean), 1e-30) for p, r in zip(preds, cold)]
     log(f"[bench] held-out prediction error / distance from the mean: x {np.median(ex):.3f}, y {np.median(ey):.3f}")
-    warm, t_warm = run_batch(th_test, mixed, [p[0] for p in preds], [p[1] for p in preds])
+    from nirnaya_amd.warmstart import warm_params
+    warm, t_warm = run_batch(th_test, warm_params(mixed, ws.primal_weight), [p[0] for p in preds],
+                             [p[1] for p in preds])
     # Cold start is the fallback of a warm solve that misses the tolerance.
     fallback = [i for i, r in enumerate(warm) if r.status != 0]
     if fallback:
@@ -208,6 +211,7 @@ def warmstart_workload(args):
                                 "cold_mean": float(ci.mean()), "warm_mean": float(wi.mean()),
                                 "saved_fraction_mean": float(1 - wi.sum() / ci.sum()),
                                 "fallbacks": len(fallback)},
+        "warm_primal_weight": ws.primal_weight,
         "held_out_prediction_error": {"x_median": float(np.median(ex)), "y_median": float(np.median(ey)),
                                       "note": "||prediction - solution|| / ||solution - POD mean||"},
         "mlp_loss": {"first_epoch": float(losses[0]), "last_epoch": float(losses[-1])},

@@ -2,7 +2,8 @@
 
 A family keeps A fixed and moves the right-hand side and the costs with a
 parameter vector theta in R^64:
-    b(theta) = b0 * (1 + 0.1 tanh(B theta)),   c(theta) = c0 + 0.05 C theta,
+    b(theta) = b0 * (1 + 0.1 tanh(B theta)),   c(theta) = c0 + 0.05 C theta
+(amplitudes 0.1 and 0.05 by default),
 with fixed seeded N(0,1) matrices B [m][64] and C [n][64] and A, b0, c0 the
 equality LP of nir_gen_equality_lp.
 
@@ -31,8 +32,10 @@ THETA_DIM = 64
 class ParametricFamily:
     """Fixed-A family of equality LPs (see the module docstring)."""
 
-    def __init__(self, seed: int, m: int, n: int, per_col: int = 5, dim: int = THETA_DIM):
+    def __init__(self, seed: int, m: int, n: int, per_col: int = 5, dim: int = THETA_DIM, b_amp: float = 0.1,
+                 c_amp: float = 0.05):
         self.seed, self.m, self.n, self.dim = seed, m, n, dim
+        self.b_amp, self.c_amp = b_amp, c_amp
         self.base = gen_equality_lp(seed, m, n, per_col)
         rng = np.random.default_rng(seed)
         self.B = rng.standard_normal((m, dim))
@@ -44,8 +47,8 @@ class ParametricFamily:
         return np.random.default_rng(seed).standard_normal((count, self.dim))
 
     def instance(self, theta: np.ndarray) -> LP:
-        b = self.b0 * (1.0 + 0.1 * np.tanh(self.B @ theta))
-        c = self.c0 + 0.05 * (self.C @ theta)
+        b = self.b0 * (1.0 + self.b_amp * np.tanh(self.B @ theta))
+        c = self.c0 + self.c_amp * (self.C @ theta)
         base = self.base
         return LP(self.m, self.n, base.start, base.index, base.value, np.ascontiguousarray(c), base.l, base.u,
                   np.ascontiguousarray(b), b.copy())
@@ -85,8 +88,11 @@ class WarmStart:
     """theta -> (x0, y0) through the MLP and the two POD bases."""
 
     def __init__(self, engine: Engine, pod_x: Pod, pod_y: Pod, hidden: int = 512, seed: int = 0,
-                 max_rows: int = 512, dim: int = THETA_DIM):
+                 max_rows: int = 512, dim: int = THETA_DIM, primal_weight: float = 0.0):
         self.engine, self.pod_x, self.pod_y, self.dim = engine, pod_x, pod_y, dim
+        # The primal weight the family's training solves ended with (median),
+        # carried into warm solves with the predicted point; 0 = engine default.
+        self.primal_weight = primal_weight
         self.n_coef = pod_x.rank + pod_y.rank
         self.n_out = _pad64(self.n_coef)
         self.net = engine.mlp(dim, hidden, self.n_out, max_rows, seed)
@@ -139,11 +145,21 @@ class PipelineReport:
     train_seconds: float = 0.0
 
 
-def solve_warm(engine: Engine, lp: LP, params, x0, y0):
-    """Warm-started solve with cold start as the fallback (and the judge)."""
+def warm_params(params, primal_weight: float):
+    """A copy of `params` with the initial primal weight of a warm solve."""
+    from .engine import NirParams
+    p = NirParams.from_buffer_copy(params)
+    if primal_weight > 0:
+        p.primal_weight = primal_weight
+    return p
+
+
+def solve_warm(engine: Engine, lp: LP, params, x0, y0, primal_weight: float = 0.0):
+    """Warm-started solve with cold start (the caller's parameters) as the
+    fallback and the judge."""
     import time
     t0 = time.perf_counter()
-    r = engine.solve(lp, params, x0=x0, y0=y0)
+    r = engine.solve(lp, warm_params(params, primal_weight), x0=x0, y0=y0)
     dt = time.perf_counter() - t0
     if r.status == 0:
         return r, SolveRecord(r.status, r.iterations, dt)
@@ -177,7 +193,8 @@ def run_pipeline(engine: Engine, family: ParametricFamily, n_train: int, n_test:
     pod_x = Pod(engine, np.stack([res[i].x for i in ok]), rank)
     pod_y = Pod(engine, np.stack([res[i].y for i in ok]), rank)
     t_pod = time.perf_counter() - t0
-    ws = WarmStart(engine, pod_x, pod_y, max_rows=max(64, batch))
+    ws = WarmStart(engine, pod_x, pod_y, max_rows=max(64, batch),
+                   primal_weight=float(np.median([res[i].info.primal_weight for i in ok])))
     t0 = time.perf_counter()
     losses = ws.fit(th_train[ok], epochs=epochs, batch=batch, lr=lr, seed=seed)
     t_fit = time.perf_counter() - t0
@@ -191,7 +208,7 @@ def run_pipeline(engine: Engine, family: ParametricFamily, n_train: int, n_test:
         c = engine.solve(lp, params)
         rep.cold.append(SolveRecord(c.status, c.iterations, time.perf_counter() - t0))
         x0, y0 = ws.predict(t)
-        _, w = solve_warm(engine, lp, params, x0, y0)
+        _, w = solve_warm(engine, lp, params, x0, y0, ws.primal_weight)
         rep.warm.append(w)
         say(f"held-out: cold {c.iterations} it ({rep.cold[-1].seconds:.3f}s) warm {w.iterations} it "
             f"({w.seconds:.3f}s){' fallback' if w.fallback else ''}")
