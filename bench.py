@@ -10,8 +10,13 @@ with the fused dual step), everything resident in HBM.
 
 Prints one JSON line: iterations/s over the timed steps, the roofline of the
 dominant kernel (A xbar + dual step) from HIP events on the engine's stream,
-and the reference's own CPU loop (oracle/_ref, compiled from the reference
-sources) timed on a bounded sample of the same instance.
+the time to relative KKT 1e-6 of the restarted engine on the same instance in
+plain FP64 and in mixed precision, and the reference's own CPU loop
+(oracle/_ref, compiled from the reference sources) timed on a bounded sample
+of the same instance.
+
+Other workloads: --workload planning512 (512 LPs in the batched persistent
+engine) and --workload warmstart (learned warm start on a parametric family).
 """
 import argparse
 import json
@@ -62,6 +67,30 @@ def cpu_baseline(lp):
             "sample": f"{it} iterations of the reference loop on the full instance "
                       f"({total - setup:.1f} s after {setup:.1f} s setup), {os.cpu_count()} host cores "
                       f"available, CPU {cpu_model()}"}
+
+
+def time_to_tolerance(eng_factory, lp):
+    """Time to relative KKT 1e-6 on the same instance with the restarted
+    engine: plain FP64 (the baseline the north star compares against) and
+    re-centred mixed precision. Both include the session setup (upload,
+    layout autotune, scaling, power iteration) in `total_s`; `solve_s` is the
+    iteration loop with everything resident."""
+    from nirnaya_amd.engine import MODE_PDLP, PREC_MIXED, default_params
+    out = {}
+    eng = eng_factory()
+    for name, prec in (("fp64", 0), ("mixed", PREC_MIXED)):
+        p = default_params(mode=MODE_PDLP, precision=prec, max_iters=2_000_000, tol=1e-6)
+        t0 = time.perf_counter()
+        r = eng.solve(lp, p)
+        total = time.perf_counter() - t0
+        i = r.info
+        out[name] = {"status": r.status, "iterations": r.iterations, "fp64_tail_iterations": i.polish_iterations,
+                     "restarts": i.restarts, "solve_s": round(i.solve_seconds, 3), "total_s": round(total, 3),
+                     "rel_kkt": [i.rel_primal, i.rel_dual, i.rel_gap], "objective": r.objective}
+        log(f"[bench] time to 1e-6 ({name}): {out[name]}")
+    eng.close()
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["solve_s"] / out["mixed"]["solve_s"], 3)
+    return out
 
 
 def planning512(args):
@@ -233,6 +262,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ttt", action="store_true", help="skip the time-to-1e-6 solves (pdlp mode, FP64 and mixed)")
     ap.add_argument("--tile", type=int, default=-1, help="LDS tile (-1: autotune tile, rows and segments)")
     ap.add_argument("--seg", type=int, default=0, help="column-segment width of A xbar when not autotuning")
     ap.add_argument("--seg-t", type=int, default=-1, help="column-segment width of A^T y (-1: as --seg)")
@@ -302,6 +332,10 @@ def main():
         if tr.get("segments") == t.spmv_passes:
             traffic = tr.get("hbm_bytes_per_launch")
 
+    ttt = None
+    if not args.no_ttt:
+        ttt = time_to_tolerance(eng_factory=lambda: Engine(0), lp=lp)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] timing the reference CPU loop on a bounded sample ...")
@@ -336,6 +370,7 @@ def main():
                     "spmtv_gbs": t.spmtv_bytes / spmtv_avg_s / 1e9,
                     "iteration_bytes": t.iter_bytes,
                     "iteration_gbs": t.iter_bytes * args.steps / (t.total_ms / 1e3) / 1e9},
+        "time_to_1e-6": ttt,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

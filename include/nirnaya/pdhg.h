@@ -57,8 +57,9 @@ typedef struct {
                                   0 = run exactly max_iters iterations */
     int32_t use_graph;         /* capture iteration blocks in hipGraphs */
     int32_t verbose;
-    int32_t tile_nnz;          /* SpMV tile: nonzeros per workgroup (0 = default 2048,
-                                  -1 = autotune tile, rows and segments per matrix) */
+    int32_t tile_nnz;          /* SpMV tile: nonzeros per workgroup (1024, 2048 or 4096;
+                                  -1 = autotune tile, rows and segments per matrix; 0 = autotune
+                                  when nnz >= 2^21 and seg_width is 0, else 2048) */
     int32_t tile_rows;         /* SpMV tile: max rows per workgroup (0 = default 256) */
     int32_t seg_width;         /* column-segment width of the A xbar products (0 = one segment) */
     int32_t seg_width_t;       /* the same for the A^T y products (-1 = seg_width) */

@@ -50,11 +50,10 @@ struct RefState {
     unsigned long long vslot[2];   // max row violation of iteration k in slot k & 1 (bits of a double >= 0)
 };
 
-template <int TILE, typename MV>
-__global__ __launch_bounds__(kThreads) void k_spmv_store(MV P, const double* __restrict__ v,
-                                                         double* __restrict__ out) {
-    __shared__ double prod[lds_slots(TILE)];
-    stream_rows<TILE>(P, v, prod, blockIdx.x, [&](int r, double s) { out[r] = s; });
+template <int TILE, typename MV, typename T>
+__global__ __launch_bounds__(kThreads) void k_spmv_store(MV P, const T* __restrict__ v, T* __restrict__ out) {
+    __shared__ T prod[lds_slots(TILE)];
+    stream_rows<TILE>(P, v, prod, blockIdx.x, [&](int r, T s) { out[r] = s; });
 }
 
 template <int TILE, typename MV>
@@ -162,27 +161,64 @@ __global__ __launch_bounds__(256) void k_copy_calibrate(const double4* __restric
 
 bool host_finite(double b) { return std::isfinite(b) && std::fabs(b) < 1e20; }
 
-void spmv_store(const DevMatrix& M, const double* v, double* out, hipStream_t s) {
-    if (M.shape.fused_ok) {
-        const SegCsr<double> F = M.shape.fused(M.val.p);
-        TILE_DISPATCH(M.shape.shape.tile, k_spmv_store<TL><<<F.nblk, kThreads, 0, s>>>(F, v, out));
+// y = M v over a segmented shape with values `val` (and running sums `part`).
+template <typename T>
+void spmv_store(const CsrShape& S, const T* val, T* part, const T* v, T* out, hipStream_t s) {
+    if (S.fused_ok) {
+        const SegCsr<T> F = S.fused(val);
+        TILE_DISPATCH(S.shape.tile, k_spmv_store<TL><<<F.nblk, kThreads, 0, s>>>(F, v, out));
     } else {
-        for (int p = 0; p < M.shape.nseg; ++p) {
-            const Pass<double> P = M.shape.pass(M.val.p, M.part.p, p);
-            TILE_DISPATCH(M.shape.shape.tile, k_spmv_store<TL><<<P.M.nblk, kThreads, 0, s>>>(P, v, out));
+        for (int p = 0; p < S.nseg; ++p) {
+            const Pass<T> P = S.pass(val, part, p);
+            TILE_DISPATCH(S.shape.tile, k_spmv_store<TL><<<P.M.nblk, kThreads, 0, s>>>(P, v, out));
         }
     }
     HIP_CHECK(hipGetLastError());
 }
 
+void spmv_store(const DevMatrix& M, const double* v, double* out, hipStream_t s) {
+    spmv_store<double>(M.shape, M.val.p, M.part.p, v, out, s);
+}
+
+// Time 8 products y = M v in the layout `c`, values in precision T.
+template <typename T>
+float time_layout(int nrows, int ncols, const std::vector<int>& rowptr, const int* col, const double* val,
+                  const TileShape& c, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    CsrShape S;
+    std::vector<int> perm;
+    upload_shape(S, nrows, ncols, rowptr, col, c, perm, s);
+    std::vector<T> hv(perm.size());
+    for (size_t q = 0; q < perm.size(); ++q) hv[q] = static_cast<T>(val[perm[q]]);
+    DevArray<T> dval, part, v, out;
+    dval.alloc(hv.size());
+    dval.upload(hv.data(), hv.size(), s);
+    part.alloc(static_cast<size_t>(std::max(nrows, 1)));
+    v.alloc(static_cast<size_t>(std::max(ncols, 1)));
+    out.alloc(static_cast<size_t>(std::max(nrows, 1)));
+    {
+        std::vector<T> ones(static_cast<size_t>(std::max(ncols, 1)), T(1));
+        v.upload(ones.data(), ones.size(), s);
+    }
+    for (int w = 0; w < 2; ++w) spmv_store<T>(S, dval.p, part.p, v.p, out.p, s);
+    HIP_CHECK(hipEventRecord(e0, s));
+    for (int w = 0; w < 8; ++w) spmv_store<T>(S, dval.p, part.p, v.p, out.p, s);
+    HIP_CHECK(hipEventRecord(e1, s));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    return ms;
+}
+
 // Autotuner: time y = M v for every candidate layout (segment width, tile,
-// fused or per-segment launches) on the real matrix and keep the fastest. Results are cached per matrix signature; the choice
-// never changes results except for rows longer than a tile.
+// fused or per-segment launches) on the real matrix, in the precision the
+// iterations will run in (FP32 for mixed precision), and keep the fastest.
+// Results are cached per matrix signature and precision; the choice never
+// changes results except for rows longer than a tile.
 TileShape autotune(int nrows, int ncols, const std::vector<int>& rowptr, const int* col, const double* val,
-                   hipStream_t s) {
+                   hipStream_t s, bool fp32, bool allow_fused) {
     static std::mutex mu;
-    static std::map<std::tuple<int, int, int>, TileShape> cache;
-    const auto key = std::make_tuple(nrows, ncols, rowptr.back());
+    static std::map<std::tuple<int, int, int, int>, TileShape> cache;
+    const auto key = std::make_tuple(nrows, ncols, rowptr.back(), (fp32 ? 1 : 0) + (allow_fused ? 2 : 0));
     {
         std::lock_guard<std::mutex> lock(mu);
         auto it = cache.find(key);
@@ -193,29 +229,15 @@ TileShape autotune(int nrows, int ncols, const std::vector<int>& rowptr, const i
         if (w == 0 || w < ncols)
             for (int t : {1024, 2048, 4096})
                 for (int f : {0, 1})
-                    if (f == 0 || w > 0) cands.push_back(TileShape{t, 256, w, f});
-    DevArray<double> v, out;
-    v.alloc(std::max(ncols, 1));
-    out.alloc(std::max(nrows, 1));
-    {
-        std::vector<double> ones(static_cast<size_t>(std::max(ncols, 1)), 1.0);
-        v.upload(ones.data(), ones.size(), s);
-    }
+                    if (f == 0 || (w > 0 && allow_fused)) cands.push_back(TileShape{t, 256, w, f});
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     TileShape best = cands.front();
     float best_ms = 1e30f;
     for (const TileShape& c : cands) {
-        DevMatrix M;
-        upload_matrix(M, nrows, ncols, rowptr, col, val, c, s);
-        for (int w = 0; w < 2; ++w) spmv_store(M, v.p, out.p, s);
-        HIP_CHECK(hipEventRecord(e0, s));
-        for (int w = 0; w < 8; ++w) spmv_store(M, v.p, out.p, s);
-        HIP_CHECK(hipEventRecord(e1, s));
-        HIP_CHECK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        const float ms = fp32 ? time_layout<float>(nrows, ncols, rowptr, col, val, c, s, e0, e1)
+                              : time_layout<double>(nrows, ncols, rowptr, col, val, c, s, e0, e1);
         if (ms < best_ms) {
             best_ms = ms;
             best = c;
@@ -448,6 +470,9 @@ void reference_step_sizes(nir_session* S) {
     S->sigma = 0.85 / std::max(1e-6, smax);
 }
 
+// Nonzeros from which tile_nnz = 0 autotunes the layout.
+constexpr long long kAutotuneNnz = 1ll << 21;
+
 TileShape shape_from_params(const nir_params& p, bool transpose) {
     TileShape t;
     if (p.tile_nnz == 1024 || p.tile_nnz == 2048 || p.tile_nnz == 4096) t.tile = p.tile_nnz;
@@ -560,8 +585,17 @@ NIR_API int nir_session_create(nir_ctx* ctx, const nir_lp* lp, const nir_params*
         std::vector<int> a_ptr, a_col;
         std::vector<double> a_val;
         csr_from_csc(lp, a_ptr, a_col, a_val);
-        const TileShape ta = S->prm.tile_nnz == -1 ? autotune(m, n, a_ptr, a_col.data(), a_val.data(), S->s) : base;
-        const TileShape tt = S->prm.tile_nnz == -1 ? autotune(n, m, at_ptr, lp->row_index, lp->value, S->s) : base_t;
+        // tile_nnz = -1 autotunes; 0 autotunes only large matrices (where a
+        // better layout repays the autotuner's few hundred milliseconds) and
+        // keeps the default layout otherwise.
+        const bool tune = S->prm.tile_nnz == -1 ||
+                          (S->prm.tile_nnz == 0 && S->prm.seg_width == 0 && S->nnz >= kAutotuneNnz);
+        // pdlp mode launches one kernel per segment (no fused layout) and
+        // iterates in FP32 in mixed precision: time what will run.
+        const bool pdlp = S->prm.mode == NIR_MODE_PDLP;
+        const bool fp32 = pdlp && S->prm.precision == NIR_PREC_MIXED;
+        const TileShape ta = tune ? autotune(m, n, a_ptr, a_col.data(), a_val.data(), S->s, fp32, !pdlp) : base;
+        const TileShape tt = tune ? autotune(n, m, at_ptr, lp->row_index, lp->value, S->s, fp32, !pdlp) : base_t;
         upload_matrix(S->A, m, n, a_ptr, a_col.data(), a_val.data(), ta, S->s);
         upload_matrix(S->AT, n, m, at_ptr, lp->row_index, lp->value, tt, S->s);
 
