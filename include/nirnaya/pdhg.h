@@ -112,8 +112,9 @@ typedef struct {
 
 NIR_API void nir_params_default(nir_params* p);
 /* Layout check for bindings: {sizeof, offsetof(last field)} of nir_lp,
- * nir_params, nir_info, nir_timing, nir_batch_opts, nir_batch_stats (12
- * values); returns the count, writes them when cap allows. */
+ * nir_params, nir_info, nir_timing, nir_batch_opts, nir_batch_stats,
+ * nir_verify_report (14 values); returns the count, writes them when cap
+ * allows. */
 NIR_API int nir_abi_layout(int64_t* out, int32_t cap);
 
 NIR_API int nir_ctx_create(int device, nir_ctx** out);
@@ -213,6 +214,29 @@ NIR_API int nir_mlp_gemm_check(nir_ctx* ctx, int32_t M, int32_t N, int32_t K, co
 NIR_API int nir_pod_gram(nir_ctx* ctx, const double* X, int32_t len, int32_t N, double* G);
 NIR_API int nir_pod_project(nir_ctx* ctx, const double* X, int32_t len, int32_t N, const double* V, int32_t R,
                             double* Y);
+
+/* ---- double-double solution certifier on the device -----------------------
+ * Replaces the host pass of sov::verify_solution (src/io/sol_io.hpp:25-106)
+ * for a continuous LP and Model::eval_objective (src/model/model.hpp:73-98):
+ * relative bound and row violations (row activities in double-double,
+ * ascending columns, zeros skipped - the host checker's values bit for bit),
+ * the double-double objective and its relative error against claimed_obj.
+ * passed = no non-finite x, both violations <= tol, objective error <= 1e-5.
+ * Worst indices are the first index attaining the maximum (-1 if none). */
+typedef struct {
+    double max_bound_viol;
+    double max_row_viol;
+    double computed_obj;
+    double obj_rel_err;
+    int32_t worst_col;       /* column of max_bound_viol (-1 if 0) */
+    int32_t worst_col_upper; /* 1 if it is the upper bound */
+    int32_t worst_row;       /* row of max_row_viol (-1 if 0) */
+    int32_t worst_row_upper; /* 1 if it is the row's upper bound */
+    int32_t nonfinite_col;   /* first column with a NaN/Inf value, -1 if none */
+    int32_t passed;
+} nir_verify_report;
+NIR_API int nir_verify_solution(nir_ctx* ctx, const nir_lp* lp, const double* x, double claimed_obj, double tol,
+                                nir_verify_report* out);
 
 /* ---- synthetic instances (seeded; generated on the host) ---------------- */
 typedef struct nir_owned_lp nir_owned_lp;

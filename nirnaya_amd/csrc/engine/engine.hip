@@ -515,6 +515,7 @@ NIR_API int nir_abi_layout(int64_t* out, int32_t cap) {
         static_cast<int64_t>(sizeof(nir_timing)),      static_cast<int64_t>(offsetof(nir_timing, spmtv_passes)),
         static_cast<int64_t>(sizeof(nir_batch_opts)),  static_cast<int64_t>(offsetof(nir_batch_opts, workgroups)),
         static_cast<int64_t>(sizeof(nir_batch_stats)), static_cast<int64_t>(offsetof(nir_batch_stats, solve_seconds)),
+        static_cast<int64_t>(sizeof(nir_verify_report)), static_cast<int64_t>(offsetof(nir_verify_report, passed)),
     };
     const int32_t n = static_cast<int32_t>(sizeof(v) / sizeof(v[0]));
     if (!out || cap < n) return n;

@@ -1,0 +1,212 @@
+// Double-double solution certifier on the device: the checks of the
+// reference's verify_solution (src/io/sol_io.hpp:25-106) for a continuous LP
+// and its objective evaluation (src/model/model.hpp:73-98), with the same
+// arithmetic as the host checker:
+//   * bound violation max(0, l - x) / (1 + |l|) and max(0, x - u) / (1 + |u|)
+//     for finite bounds (|b| < 1e20);
+//   * row activity r_i = sum_j a_ij x_j in double-double, accumulated in
+//     ascending column order and skipping x_j == 0 (two-product + two-sum, the
+//     host DD::add_prod), row violation relative to 1 + |bound|;
+//   * objective offset + sum c_j x_j in double-double (skipping zeros), and
+//     its relative difference from the claimed objective;
+//   * pass iff both violations <= tol and the objective error <= 1e-5.
+// One lane per row walks its CSR row sequentially, so each activity is the
+// host's value bit for bit; maxima keep the first (index, lower before upper)
+// attaining them, as the host's strict running maximum does.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/nirnaya/pdhg.h"
+#include "ctx.hpp"
+#include "devmat.hpp"
+#include "lpcheck.hpp"
+#include "pdlp.hpp"
+#include "pdlp_core.hpp"
+
+namespace nir {
+namespace gpu {
+namespace verify {
+
+struct MaxAt {
+    double v;
+    int at;
+};
+
+__device__ __forceinline__ MaxAt better(MaxAt a, MaxAt b) {
+    // larger violation wins; on ties the smaller index (the host's first)
+    if (b.v > a.v || (b.v == a.v && b.at >= 0 && (a.at < 0 || b.at < a.at))) return b;
+    return a;
+}
+
+__device__ __forceinline__ MaxAt block_best(MaxAt m, MaxAt* sh) {
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] = better(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+// Row activities and row violations: one lane per row.
+__global__ __launch_bounds__(256) void k_verify_rows(int m, const int* __restrict__ rp, const int* __restrict__ col,
+                                                     const double* __restrict__ val, const double* __restrict__ x,
+                                                     const double* __restrict__ rl, const double* __restrict__ ru,
+                                                     MaxAt* part) {
+    __shared__ MaxAt sh[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    MaxAt best{0.0, -1};
+    if (i < m) {
+        dd act = dd_zero();
+        for (int p = rp[i]; p < rp[i + 1]; ++p) {
+            const double xj = x[col[p]];
+            if (xj != 0.0) act = dd_add(act, dd_prod(val[p], xj));
+        }
+        const double r = dd_val(act);
+        // `at` = 2 index + side (0 lower, 1 upper): the host checks the lower
+        // bound first and keeps the first strict maximum.
+        if (finite_bound(rl[i])) best = better(best, MaxAt{amax(0.0, rl[i] - r) / (1.0 + fabs(rl[i])), 2 * i});
+        if (finite_bound(ru[i])) best = better(best, MaxAt{amax(0.0, r - ru[i]) / (1.0 + fabs(ru[i])), 2 * i + 1});
+    }
+    const MaxAt b = block_best(best, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = b;
+}
+
+// Bound violations, non-finite entries and the objective's dd partials.
+__global__ __launch_bounds__(256) void k_verify_cols(int n, const double* __restrict__ x,
+                                                     const double* __restrict__ lo, const double* __restrict__ up,
+                                                     const double* __restrict__ c, MaxAt* part, dd* obj_part,
+                                                     int* nonfinite) {
+    __shared__ MaxAt sh[256];
+    __shared__ dd so[256];
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    MaxAt best{0.0, -1};
+    dd o = dd_zero();
+    if (j < n) {
+        const double v = x[j];
+        if (!isfinite(v)) {
+            atomicMin(nonfinite, j);
+        } else {
+            if (finite_bound(lo[j])) best = better(best, MaxAt{amax(0.0, lo[j] - v) / (1.0 + fabs(lo[j])), 2 * j});
+            if (finite_bound(up[j])) best = better(best, MaxAt{amax(0.0, v - up[j]) / (1.0 + fabs(up[j])), 2 * j + 1});
+            if (c[j] != 0.0 && v != 0.0) o = dd_prod(c[j], v);
+        }
+    }
+    const MaxAt b = block_best(best, sh);
+    so[threadIdx.x] = o;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) so[threadIdx.x] = dd_add(so[threadIdx.x], so[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = b;
+        obj_part[blockIdx.x] = so[0];
+    }
+}
+
+}  // namespace verify
+}  // namespace gpu
+}  // namespace nir
+
+using namespace nir::gpu;
+using namespace nir::gpu::verify;
+
+extern "C" {
+
+NIR_API int nir_verify_solution(nir_ctx* ctx, const nir_lp* lp, const double* x, double claimed_obj, double tol,
+                                nir_verify_report* out) {
+    if (!ctx || !out) return -1;
+    try {
+        check_lp(lp);
+        if (lp->ncol > 0 && !x) throw std::invalid_argument("missing solution vector");
+        if (lp->nrow >= (1 << 30) || lp->ncol >= (1 << 30)) throw std::invalid_argument("LP too large to certify");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        const int m = lp->nrow, n = lp->ncol;
+        std::vector<int> ptr, col;
+        std::vector<double> val;
+        csr_from_csc(lp, ptr, col, val);
+        DevArray<int> dptr, dcol;
+        DevArray<double> dval, dx, drl, dru, dlo, dup, dc;
+        dptr.alloc(ptr.size());
+        dptr.upload(ptr.data(), ptr.size(), s);
+        dcol.alloc(col.size());
+        dcol.upload(col.data(), col.size(), s);
+        dval.alloc(val.size());
+        dval.upload(val.data(), val.size(), s);
+        dx.alloc(n);
+        dx.upload(x, n, s);
+        dlo.alloc(n);
+        dlo.upload(lp->collo, n, s);
+        dup.alloc(n);
+        dup.upload(lp->colup, n, s);
+        dc.alloc(n);
+        dc.upload(lp->cost, n, s);
+        drl.alloc(m);
+        drl.upload(lp->rowlo, m, s);
+        dru.alloc(m);
+        dru.upload(lp->rowup, m, s);
+        const int gr = std::max(1, (m + 255) / 256), gc = std::max(1, (n + 255) / 256);
+        DevArray<MaxAt> prow, pcol;
+        DevArray<dd> pobj;
+        DevArray<int> nonfinite;
+        prow.alloc(gr);
+        pcol.alloc(gc);
+        pobj.alloc(gc);
+        nonfinite.alloc(1);
+        const int big = 0x7fffffff;
+        nonfinite.upload(&big, 1, s);
+        k_verify_rows<<<gr, 256, 0, s>>>(m, dptr.p, dcol.p, dval.p, dx.p, drl.p, dru.p, prow.p);
+        k_verify_cols<<<gc, 256, 0, s>>>(n, dx.p, dlo.p, dup.p, dc.p, pcol.p, pobj.p, nonfinite.p);
+        HIP_CHECK(hipGetLastError());
+        std::vector<MaxAt> hr(gr), hc(gc);
+        std::vector<dd> ho(gc);
+        int nf = big;
+        prow.download(hr.data(), gr, s);
+        pcol.download(hc.data(), gc, s);
+        pobj.download(ho.data(), gc, s);
+        nonfinite.download(&nf, 1, s);
+        HIP_CHECK(hipStreamSynchronize(s));
+        std::memset(out, 0, sizeof(*out));
+        out->worst_row = -1;
+        out->worst_col = -1;
+        out->nonfinite_col = nf == big ? -1 : nf;
+        // Blocks hold consecutive indices, so taking the first strict maximum
+        // over blocks in order keeps the smallest index among equal maxima.
+        for (int b = 0; b < gr && m > 0; ++b)
+            if (hr[b].at >= 0 && hr[b].v > out->max_row_viol) {
+                out->max_row_viol = hr[b].v;
+                out->worst_row = hr[b].at / 2;
+                out->worst_row_upper = hr[b].at & 1;
+            }
+        for (int b = 0; b < gc && n > 0; ++b)
+            if (hc[b].at >= 0 && hc[b].v > out->max_bound_viol) {
+                out->max_bound_viol = hc[b].v;
+                out->worst_col = hc[b].at / 2;
+                out->worst_col_upper = hc[b].at & 1;
+            }
+        DDHost acc(lp->obj_offset);
+        for (int b = 0; b < gc && n > 0; ++b) acc.add(ho[b].hi, ho[b].lo);
+        out->computed_obj = acc.value();
+        out->obj_rel_err = std::isfinite(claimed_obj)
+                               ? std::fabs(out->computed_obj - claimed_obj) / (1.0 + std::fabs(out->computed_obj))
+                               : 0.0;
+        out->passed = out->nonfinite_col < 0 && out->max_bound_viol <= tol && out->max_row_viol <= tol &&
+                              out->obj_rel_err <= 1e-5
+                          ? 1
+                          : 0;
+        return 0;
+    } catch (const std::exception& ex) {
+        ctx->error = ex.what();
+        return -1;
+    }
+}
+
+}  // extern "C"

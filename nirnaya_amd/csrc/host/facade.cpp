@@ -80,6 +80,60 @@ EngineRun run_engine(const Model& lp, const Options& opt, const Deadline& deadli
     return r;
 }
 
+// Large continuous LPs are certified on the device (nir_verify_solution: the
+// same double-double activities and rules as verify_solution, names filled in
+// here); smaller or integer models on the host.
+constexpr long long kDeviceVerifyNnz = 1ll << 20;
+
+CheckReport certify(const Model& m, const std::vector<double>& x, double claimed, double tol, int device) {
+    if (m.A.nnz() < kDeviceVerifyNnz || m.num_integers() > 0 || static_cast<Idx>(x.size()) != m.ncol)
+        return verify_solution(m, x, claimed, tol);
+    CheckReport rep;
+    nir_lp v{};
+    v.nrow = m.nrow;
+    v.ncol = m.ncol;
+    v.col_start = m.A.start.data();
+    v.row_index = m.A.index.data();
+    v.value = m.A.value.data();
+    v.cost = m.cost.data();
+    v.collo = m.collo.data();
+    v.colup = m.colup.data();
+    v.rowlo = m.rowlo.data();
+    v.rowup = m.rowup.data();
+    v.obj_offset = m.obj_offset;
+    nir_ctx* ctx = nullptr;
+    nir_verify_report r{};
+    if (nir_ctx_create(device, &ctx) != 0) {
+        rep.passed = false;
+        rep.worst_item = std::string("device certifier unavailable: ") + nir_last_error();
+        return rep;
+    }
+    const int rc = nir_verify_solution(ctx, &v, x.data(), claimed, tol, &r);
+    if (rc != 0) {
+        rep.passed = false;
+        rep.worst_item = std::string("device certifier failed: ") + nir_ctx_error(ctx);
+        nir_ctx_destroy(ctx);
+        return rep;
+    }
+    nir_ctx_destroy(ctx);
+    if (r.nonfinite_col >= 0) {
+        rep.passed = false;
+        rep.worst_item = "NaN/Inf in variable " + m.col_names[r.nonfinite_col];
+        return rep;
+    }
+    rep.max_bound_viol = r.max_bound_viol;
+    rep.max_row_viol = r.max_row_viol;
+    rep.computed_obj = r.computed_obj;
+    rep.obj_rel_err = r.obj_rel_err;
+    rep.passed = r.passed != 0;
+    // The host checker names the worst violation above tol, rows after columns.
+    if (r.max_row_viol > tol && r.worst_row >= 0)
+        rep.worst_item = (r.worst_row_upper ? "Row upper bound on " : "Row lower bound on ") + m.row_names[r.worst_row];
+    else if (r.max_bound_viol > tol && r.worst_col >= 0)
+        rep.worst_item = (r.worst_col_upper ? "Upper bound on " : "Lower bound on ") + m.col_names[r.worst_col];
+    return rep;
+}
+
 }  // namespace
 
 SolveResult Solver::solve(const Model& user_model) {
@@ -179,7 +233,7 @@ SolveResult Solver::solve(const Model& user_model) {
         for (size_t i = 0; i < pre.row_map.size() && i < y_red.size(); ++i) out.row_duals[pre.row_map[i]] = sgn * y_red[i];
         out.reduced_costs.assign(static_cast<size_t>(valid.ncol), 0.0);
 
-        out.verification = verify_solution(valid, out.x, out.objective, options.tol.checker_tol);
+        out.verification = certify(valid, out.x, out.objective, options.tol.checker_tol, options.pdhg.device);
         if (!out.verification.passed && out.status == Status::Optimal) {
             if (options.presolve > 0 || options.scale) {
                 Options fo = options;

@@ -92,6 +92,13 @@ class NirBatchStats(C.Structure):
     ]
 
 
+class NirVerifyReport(C.Structure):
+    _fields_ = [("max_bound_viol", C.c_double), ("max_row_viol", C.c_double), ("computed_obj", C.c_double),
+                ("obj_rel_err", C.c_double), ("worst_col", C.c_int32), ("worst_col_upper", C.c_int32),
+                ("worst_row", C.c_int32), ("worst_row_upper", C.c_int32), ("nonfinite_col", C.c_int32),
+                ("passed", C.c_int32)]
+
+
 _lib = None
 
 
@@ -145,6 +152,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.nir_mlp_train_stats.argtypes = [vp, C.POINTER(C.c_int64), _dp, _dp]
     lib.nir_mlp_destroy.argtypes = [vp]
     lib.nir_mlp_gemm_check.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, _fp, _fp, C.c_int32, C.c_int32, _fp]
+    lib.nir_verify_solution.argtypes = [vp, C.POINTER(NirLp), _dp, C.c_double, C.c_double,
+                                        C.POINTER(NirVerifyReport)]
     lib.nir_pod_gram.argtypes = [vp, _dp, C.c_int32, C.c_int32, _dp]
     lib.nir_pod_project.argtypes = [vp, _dp, C.c_int32, C.c_int32, _dp, C.c_int32, _dp]
     lib.nir_gen_equality_lp.argtypes = [C.c_uint64, C.c_int32, C.c_int32, C.c_int32]
@@ -340,6 +349,16 @@ class Engine:
                                        int(transpose_a), int(transpose_b), out.ctypes.data_as(fp)) != 0:
             raise EngineError("nir_mlp_gemm_check: " + self.error())
         return out
+
+    def verify(self, lp: LP, x: np.ndarray, claimed_obj: float, tol: float = 1e-6) -> NirVerifyReport:
+        """Double-double certificate of x on the device (nir_verify_solution)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        s = lp.struct()
+        rep = NirVerifyReport()
+        if self.lib.nir_verify_solution(self.h, C.byref(s), _ptr(x, C.c_double), float(claimed_obj), float(tol),
+                                        C.byref(rep)) != 0:
+            raise EngineError("nir_verify_solution: " + self.error())
+        return rep
 
     def pod_gram(self, X: np.ndarray) -> np.ndarray:
         """X^T X for X [len][N] on the device (FP64)."""

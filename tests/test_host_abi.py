@@ -195,14 +195,15 @@ def test_python_struct_layouts_match_the_header():
     of its last field (a stray or missing field shifts everything after it)."""
     import ctypes as C
     from nirnaya_amd.engine import (NirBatchOpts, NirBatchStats, NirInfo, NirLp, NirParams, NirTiming,
-                                    load_library)
+                                    NirVerifyReport, load_library)
     lib = load_library()
     lib.nir_abi_layout.argtypes = [C.POINTER(C.c_int64), C.c_int32]
     n = lib.nir_abi_layout(None, 0)
     out = (C.c_int64 * n)()
-    assert lib.nir_abi_layout(out, n) == n == 12
+    assert lib.nir_abi_layout(out, n) == n == 14
     mirrors = [(NirLp, "obj_offset"), (NirParams, "seg_fused"), (NirInfo, "solve_seconds"),
-               (NirTiming, "spmtv_passes"), (NirBatchOpts, "workgroups"), (NirBatchStats, "solve_seconds")]
+               (NirTiming, "spmtv_passes"), (NirBatchOpts, "workgroups"), (NirBatchStats, "solve_seconds"),
+               (NirVerifyReport, "passed")]
     for k, (cls, last) in enumerate(mirrors):
         assert C.sizeof(cls) == out[2 * k], cls.__name__
         assert getattr(cls, last).offset == out[2 * k + 1], cls.__name__
