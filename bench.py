@@ -63,10 +63,19 @@ def cpu_baseline(lp):
     _, it, total = ref.pdhg_timed(setup * 1.5 + 0.5)
     if it <= 0:
         return None
-    return {"value": it / (total - setup), "unit": "PDHG iterations/s", "cores": 1, "kind": "reference",
-            "sample": f"{it} iterations of the reference loop on the full instance "
-                      f"({total - setup:.1f} s after {setup:.1f} s setup), {os.cpu_count()} host cores "
-                      f"available, CPU {cpu_model()}"}
+    out = {"value": it / (total - setup), "unit": "PDHG iterations/s", "cores": 1, "kind": "reference",
+           "sample": f"{it} iterations of the reference loop on the full instance "
+                     f"({total - setup:.1f} s after {setup:.1f} s setup), {os.cpu_count()} host cores "
+                     f"available, CPU {cpu_model()}"}
+    # The reference's own parallel setting (Options::threads = host cores):
+    # only its primal update runs on the thread pool (pdhg_solver.hpp:71).
+    # The box allots 16 cores to a GPU job whatever os.cpu_count() reports.
+    nth = max(1, min(16, os.cpu_count() or 1))
+    _, itp, totp = ref.pdhg_timed(setup * 1.5 + 0.5, threads=nth)
+    if itp > 0:
+        out["parallel"] = {"value": itp / (totp - setup), "threads": nth,
+                           "sample": f"{itp} iterations, {totp - setup:.1f} s after setup"}
+    return out
 
 
 def time_to_tolerance(eng_factory, lp):
@@ -329,8 +338,11 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
-        if tr.get("segments") == t.spmv_passes:
-            traffic = tr.get("hbm_bytes_per_launch")
+        # Measured traffic of the layout this run uses (the autotuner may pick
+        # another one on another box; then there is no measurement to quote).
+        for e in tr.get("layouts", []):
+            if e.get("tile") == shape[0] and e.get("segments") == shape[2] and not shape[3]:
+                traffic = e.get("hbm_bytes_per_launch")
 
     ttt = None
     if not args.no_ttt:
