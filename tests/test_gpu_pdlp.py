@@ -106,3 +106,37 @@ def test_facade_pdlp_mode_on_fixture(tmp_path):
     assert chk.returncode == 0, chk.stdout
     obj, _ = lpkit.read_sol(str(sol))
     assert obj == pytest.approx(22171.59777424483, rel=1e-6)
+
+
+@pytest.mark.parametrize("polish", [0, 1])
+def test_facade_mixed_precision_on_fixture(tmp_path, polish):
+    # CLI -> facade (presolve, scaling, engine in mixed precision, unscale,
+    # postsolve, dd certificate) on the reference's crude-blending LP.
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sol = tmp_path / "m.sol"
+    args = [os.path.join(root, "bin", "sovsolve"), lpkit.CRUDE_MPS, "--method=pdhg", "--pdhg-mode=pdlp",
+            "--precision=mixed", "--tol=1e-7", "--log", "0", "--sol", str(sol)]
+    if polish:
+        args += ["--set", "pdhg_polish_tol=1e-9"]
+    assert subprocess.run(args).returncode == 0
+    chk = subprocess.run([os.path.join(root, "bin", "checker"), lpkit.CRUDE_MPS, str(sol)], capture_output=True,
+                         text=True)
+    assert chk.returncode == 0, chk.stdout
+    obj, _ = lpkit.read_sol(str(sol))
+    assert obj == pytest.approx(22171.59777424483, rel=1e-5)
+
+
+def test_python_wrapper_mixed_precision():
+    from nirnaya_amd.nirnaya import NirnayaModel
+    m = NirnayaModel("a5")
+    a = m.add_var(2.0, 0.0, 1e20)
+    b = m.add_var(3.0, 0.0, 1e20)
+    m.add_constr([a, b], [1.0, 1.0], 4.0, 1e20)
+    m.add_constr([a, b], [1.0, 3.0], 6.0, 1e20)
+    for k, v in (("method", "pdhg"), ("pdhg_mode", "pdlp"), ("pdhg_precision", "mixed"), ("pdhg_tol", "1e-9")):
+        m.set_option(k, v)
+    m.optimize()
+    assert m.Status == 0
+    assert m.ObjVal == pytest.approx(9.0, abs=1e-6)
+    assert np.allclose(m.X, [3.0, 1.0], atol=1e-5)
