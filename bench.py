@@ -10,8 +10,9 @@ with the fused dual step), everything resident in HBM.
 
 Prints one JSON line: iterations/s over the timed steps, the roofline of the
 dominant kernel (A xbar + dual step) from HIP events on the engine's stream,
-the time to relative KKT 1e-6 of the restarted engine on the same instance in
-plain FP64 and in mixed precision, and the reference's own CPU loop
+the time to relative KKT 1e-6 on the same instance of the restarted engine in
+FP64 and in mixed precision and of the plain FP64 PDHG (capped; a speedup over
+it is then a lower bound), and the reference's own CPU loop
 (oracle/_ref, compiled from the reference sources) timed on a bounded sample
 of the same instance.
 
@@ -78,12 +79,56 @@ def cpu_baseline(lp):
     return out
 
 
-def time_to_tolerance(eng_factory, lp):
-    """Time to relative KKT 1e-6 on the same instance with the restarted
-    engine: plain FP64 (the baseline the north star compares against) and
-    re-centred mixed precision. Both include the session setup (upload,
-    layout autotune, scaling, power iteration) in `total_s`; `solve_s` is the
-    iteration loop with everything resident."""
+def plain_to_tolerance(eng, lp, cap_s, tol=1e-6, chunk=8192):
+    """The plain FP64 PDHG on the GPU (the reference's loop, reference mode,
+    autotuned layout) run in chunks of `chunk` iterations, each followed by a
+    device evaluation of the relative KKT of its iterate, until the KKT
+    reaches `tol`, the reference's own stop rule ends the loop, or `cap_s`
+    seconds of iteration time have passed. `solve_s` counts the iteration
+    chunks only (the evaluations are not part of the plain loop)."""
+    import numpy as np
+    from nirnaya_amd.engine import MODE_REFERENCE, default_params
+    p = default_params(mode=MODE_REFERENCE, max_iters=10 ** 12, check_termination=1, tile_nnz=-1)
+    s = eng.session(lp, p)
+    solve = 0.0
+    trace = []
+    k = np.zeros(5)
+    reached = stopped = False
+    asked = 0
+    try:
+        while solve < cap_s:
+            t0 = time.perf_counter()
+            s.run(chunk)
+            solve += time.perf_counter() - t0
+            asked += chunk
+            k = s.kkt()
+            info = s.result().info
+            reached = max(k[:3]) <= tol
+            stopped = info.iterations < asked   # the reference's own stop rule ended the loop
+            if not trace or solve - trace[-1][1] >= 10.0 or reached or stopped:
+                trace.append([int(info.iterations), round(solve, 2), float(max(k[:3]))])
+                log(f"[bench] plain FP64 PDHG: {trace[-1][0]} iterations, {trace[-1][1]} s, relative KKT "
+                    f"{trace[-1][2]:.3e}")
+            if reached or stopped:
+                break
+        info = s.result().info
+    finally:
+        s.close()
+    out = {"reached": bool(reached), "iterations": int(info.iterations), "solve_s": round(solve, 3),
+           "stopped_by_reference_rule": bool(stopped), "status": int(info.status), "rel_kkt": [float(v) for v in k[:3]], "trace_it_s_kkt": trace}
+    log(f"[bench] plain FP64 PDHG toward 1e-6: {out}")
+    return out
+
+
+def time_to_tolerance(eng_factory, lp, plain_cap_s):
+    """Time to relative KKT 1e-6 on the same instance: the restarted engine in
+    FP64 and in re-centred mixed precision, and the plain FP64 PDHG (the
+    reference's loop on the GPU, the baseline the metric names). The
+    restarted solves include the session setup (upload, layout autotune,
+    scaling, power iteration) in `total_s`; `solve_s` is the iteration loop
+    with everything resident. The plain PDHG stops at `plain_cap_s` of
+    iteration time; if it has not reached the tolerance by then, the speedup
+    over it is a lower bound."""
     from nirnaya_amd.engine import MODE_PDLP, PREC_MIXED, default_params
     out = {}
     eng = eng_factory()
@@ -97,8 +142,14 @@ def time_to_tolerance(eng_factory, lp):
                      "restarts": i.restarts, "solve_s": round(i.solve_seconds, 3), "total_s": round(total, 3),
                      "rel_kkt": [i.rel_primal, i.rel_dual, i.rel_gap], "objective": r.objective}
         log(f"[bench] time to 1e-6 ({name}): {out[name]}")
-    eng.close()
     out["speedup_mixed_vs_fp64"] = round(out["fp64"]["solve_s"] / out["mixed"]["solve_s"], 3)
+    if plain_cap_s > 0:
+        pl = plain_to_tolerance(eng, lp, plain_cap_s)
+        out["plain_fp64"] = pl
+        out["speedup_mixed_vs_plain_fp64"] = round(pl["solve_s"] / out["mixed"]["solve_s"], 3)
+        out["speedup_fp64_vs_plain_fp64"] = round(pl["solve_s"] / out["fp64"]["solve_s"], 3)
+        out["speedup_vs_plain_is_lower_bound"] = not pl["reached"]
+    eng.close()
     return out
 
 
@@ -272,6 +323,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ttt", action="store_true", help="skip the time-to-1e-6 solves (pdlp mode, FP64 and mixed)")
+    ap.add_argument("--plain-cap", type=float, default=100.0,
+                    help="seconds of iteration time for the plain FP64 PDHG toward 1e-6 (0 = skip that leg)")
     ap.add_argument("--tile", type=int, default=-1, help="LDS tile (-1: autotune tile, rows and segments)")
     ap.add_argument("--seg", type=int, default=0, help="column-segment width of A xbar when not autotuning")
     ap.add_argument("--seg-t", type=int, default=-1, help="column-segment width of A^T y (-1: as --seg)")
@@ -346,7 +399,7 @@ def main():
 
     ttt = None
     if not args.no_ttt:
-        ttt = time_to_tolerance(eng_factory=lambda: Engine(0), lp=lp)
+        ttt = time_to_tolerance(eng_factory=lambda: Engine(0), lp=lp, plain_cap_s=args.plain_cap)
 
     cpu = None
     if not args.no_cpu_baseline:

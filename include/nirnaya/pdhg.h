@@ -133,6 +133,16 @@ NIR_API int nir_session_time(nir_session* s, int64_t iters, nir_timing* out);
 /* Tile shapes in use: {A tile, A rows, A segments, A fused, A^T tile, A^T rows,
  * A^T segments, A^T fused} (fused = all segments walked in one launch). */
 NIR_API int nir_session_shape(nir_session* s, int32_t* out8);
+/* Relative KKT error of a reference-mode session's current iterate (x, y) on
+ * its LP, evaluated on the device with double-double products and sums:
+ * out5 = {relative primal, relative dual, relative gap, primal objective, dual
+ * objective}, the definitions the pdlp mode reports in nir_info. The
+ * reference loop itself stops only on its row-violation rule
+ * (src/gpu/pdhg_solver.hpp:97-100); this lets a caller time the plain PDHG to
+ * a KKT tolerance by running it in chunks (nir_session_run) between
+ * evaluations. No counterpart in the reference. pdlp sessions: error (their
+ * terms are in nir_info). */
+NIR_API int nir_session_kkt(nir_session* s, double* out5);
 NIR_API void nir_session_destroy(nir_session* s);
 
 /* Copies `bytes` (read) into a second buffer (write) with a plain grid-stride

@@ -36,6 +36,7 @@
 #include "ctx.hpp"
 #include "lpcheck.hpp"
 #include "pdlp.hpp"
+#include "verify.hpp"
 
 namespace nir {
 namespace gpu {
@@ -653,6 +654,21 @@ NIR_API int nir_session_run(nir_session* S, int64_t iters) {
         HIP_CHECK(hipSetDevice(S->ctx->device));
         if (S->pdlp) pdlp_run(*S->pdlp, S->s, S->prm, iters);
         else run_reference(S, iters);
+        return 0;
+    } catch (const std::exception& ex) {
+        S->ctx->error = ex.what();
+        return -1;
+    }
+}
+
+NIR_API int nir_session_kkt(nir_session* S, double* out5) {
+    if (!S) return -1;
+    try {
+        if (!out5) throw std::invalid_argument("missing output");
+        if (S->pdlp) throw std::invalid_argument("pdlp sessions report their relative KKT terms in nir_info");
+        HIP_CHECK(hipSetDevice(S->ctx->device));
+        kkt_device(S->s, S->A.shape, S->A.val.p, S->AT.shape, S->AT.val.p, S->x.p, S->y.p, S->c.p, S->lo.p,
+                   S->up.p, S->rl.p, S->ru.p, S->obj_offset, out5);
         return 0;
     } catch (const std::exception& ex) {
         S->ctx->error = ex.what();

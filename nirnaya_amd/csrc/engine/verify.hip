@@ -13,6 +13,9 @@
 // One lane per row walks its CSR row sequentially, so each activity is the
 // host's value bit for bit; maxima keep the first (index, lower before upper)
 // attaining them, as the host's strict running maximum does.
+//
+// The same file evaluates the relative KKT error of a point (kkt_device) for
+// reference-mode sessions, with the definitions of the restarted engine.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,6 +31,7 @@
 #include "lpcheck.hpp"
 #include "pdlp.hpp"
 #include "pdlp_core.hpp"
+#include "verify.hpp"
 
 namespace nir {
 namespace gpu {
@@ -111,7 +115,135 @@ __global__ __launch_bounds__(256) void k_verify_cols(int n, const double* __rest
     }
 }
 
+// ------------------------------------------------------------------ relative KKT
+
+// sum_p val[p] v[col[p]] over row i of a segmented CSR, in ascending column
+// order (segment by segment), in double-double.
+__device__ __forceinline__ dd row_dot_dd(int nrows, int nseg, const int* __restrict__ rp,
+                                         const int* __restrict__ col, const double* __restrict__ val,
+                                         const double* __restrict__ v, int i) {
+    dd acc = dd_zero();
+    for (int s = 0; s < nseg; ++s) {
+        const int* r = rp + static_cast<size_t>(s) * (nrows + 1);
+        for (int p = r[i]; p < r[i + 1]; ++p) acc = dd_add(acc, dd_prod(val[p], v[col[p]]));
+    }
+    return acc;
+}
+
+// Fixed-order tree over the block of four dd terms per lane; lane 0 writes
+// the block's four sums to part[4 b ..].
+__device__ __forceinline__ void block_sum4(dd (&t)[4], dd (*sh)[256], dd* part) {
+    for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] = t[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] = dd_add(sh[k][threadIdx.x], sh[k][threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) part[4 * blockIdx.x + k] = sh[k][0];
+}
+
+// Row terms: 0 primal residual^2, 1 ||b||^2 term, 2 dual sign violation^2,
+// 3 dual-objective row term.
+__global__ __launch_bounds__(256) void k_kkt_rows(int m, int nseg, const int* __restrict__ rp,
+                                                  const int* __restrict__ col, const double* __restrict__ val,
+                                                  const double* __restrict__ x, const double* __restrict__ y,
+                                                  const double* __restrict__ rl, const double* __restrict__ ru,
+                                                  dd* part) {
+    __shared__ dd sh[4][256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    dd t[4] = {dd_zero(), dd_zero(), dd_zero(), dd_zero()};
+    if (i < m) {
+        const double ax = dd_val(row_dot_dd(m, nseg, rp, col, val, x, i));
+        const double l = rl[i], u = ru[i];
+        const bool lf = finite_bound(l), uf = finite_bound(u);
+        double proj = ax;
+        if (lf) proj = amax(l, proj);
+        if (uf) proj = amin(u, proj);
+        const double r = ax - proj;
+        t[0] = dd_prod(r, r);
+        double b = 0.0;
+        if (lf) b = fabs(l);
+        if (uf) b = amax(b, fabs(u));
+        t[1] = dd_prod(b, b);
+        double yp = y[i];
+        if (!lf) yp = amin(yp, 0.0);
+        if (!uf) yp = amax(yp, 0.0);
+        const double yv = y[i] - yp;
+        t[2] = dd_prod(yv, yv);
+        if (lf && yp > 0.0) t[3] = dd_prod(l, yp);
+        if (uf && yp < 0.0) t[3] = dd_add(t[3], dd_prod(u, yp));
+    }
+    block_sum4(t, sh, part);
+}
+
+// Column terms: 0 dual residual^2, 1 ||c||^2 term, 2 primal objective term,
+// 3 dual-objective column term.
+__global__ __launch_bounds__(256) void k_kkt_cols(int n, int nseg, const int* __restrict__ rp,
+                                                  const int* __restrict__ col, const double* __restrict__ val,
+                                                  const double* __restrict__ y, const double* __restrict__ x,
+                                                  const double* __restrict__ c, const double* __restrict__ lo,
+                                                  const double* __restrict__ up, dd* part) {
+    __shared__ dd sh[4][256];
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    dd t[4] = {dd_zero(), dd_zero(), dd_zero(), dd_zero()};
+    if (j < n) {
+        const double aty = dd_val(row_dot_dd(n, nseg, rp, col, val, y, j));
+        const double l = lo[j], u = up[j], cj = c[j], xj = x[j];
+        const bool lf = finite_bound(l), uf = finite_bound(u);
+        const double lam = cj - aty;
+        double lp = lam;
+        if (!lf) lp = amin(lp, 0.0);
+        if (!uf) lp = amax(lp, 0.0);
+        const double r = lam - lp;
+        t[0] = dd_prod(r, r);
+        t[1] = dd_prod(cj, cj);
+        if (xj != 0.0) t[2] = dd_prod(cj, xj);
+        if (lf && lp > 0.0) t[3] = dd_prod(l, lp);
+        if (uf && lp < 0.0) t[3] = dd_add(t[3], dd_prod(u, lp));
+    }
+    block_sum4(t, sh, part);
+}
+
 }  // namespace verify
+
+void kkt_device(hipStream_t s, const CsrShape& A, const double* aval, const CsrShape& AT, const double* atval,
+                const double* x, const double* y, const double* c, const double* lo, const double* up,
+                const double* rl, const double* ru, double offset, double* out) {
+    const int m = A.nrows, n = AT.nrows;
+    const int gr = std::max(1, (m + 255) / 256), gc = std::max(1, (n + 255) / 256);
+    DevArray<dd> prow, pcol;
+    prow.alloc(static_cast<size_t>(4) * gr);
+    pcol.alloc(static_cast<size_t>(4) * gc);
+    verify::k_kkt_rows<<<gr, 256, 0, s>>>(m, A.nseg, A.rowptr.p, A.col.p, aval, x, y, rl, ru, prow.p);
+    verify::k_kkt_cols<<<gc, 256, 0, s>>>(n, AT.nseg, AT.rowptr.p, AT.col.p, atval, y, x, c, lo, up, pcol.p);
+    HIP_CHECK(hipGetLastError());
+    std::vector<dd> hr(static_cast<size_t>(4) * gr), hc(static_cast<size_t>(4) * gc);
+    prow.download(hr.data(), hr.size(), s);
+    pcol.download(hc.data(), hc.size(), s);
+    HIP_CHECK(hipStreamSynchronize(s));
+    DDHost rp, bn, rd, cn, P(offset), D(offset);
+    for (int b = 0; b < gr; ++b) {
+        rp.add(hr[4 * b].hi, hr[4 * b].lo);
+        bn.add(hr[4 * b + 1].hi, hr[4 * b + 1].lo);
+        rd.add(hr[4 * b + 2].hi, hr[4 * b + 2].lo);
+        D.add(hr[4 * b + 3].hi, hr[4 * b + 3].lo);
+    }
+    for (int b = 0; b < gc; ++b) {
+        rd.add(hc[4 * b].hi, hc[4 * b].lo);
+        cn.add(hc[4 * b + 1].hi, hc[4 * b + 1].lo);
+        P.add(hc[4 * b + 2].hi, hc[4 * b + 2].lo);
+        D.add(hc[4 * b + 3].hi, hc[4 * b + 3].lo);
+    }
+    const double pv = P.value(), dv = D.value();
+    out[0] = std::sqrt(rp.value()) / (1.0 + std::sqrt(bn.value()));
+    out[1] = std::sqrt(rd.value()) / (1.0 + std::sqrt(cn.value()));
+    out[2] = std::fabs(pv - dv) / (1.0 + std::fabs(pv) + std::fabs(dv));
+    out[3] = pv;
+    out[4] = dv;
+}
+
 }  // namespace gpu
 }  // namespace nir
 

@@ -124,6 +124,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.nir_session_time.argtypes = [vp, C.c_int64, C.POINTER(NirTiming)]
     lib.nir_session_destroy.argtypes = [vp]
     lib.nir_session_shape.argtypes = [vp, _i32p]
+    lib.nir_session_kkt.argtypes = [vp, _dp]
     lib.nir_calibrate_copy.argtypes = [vp, C.c_int64, _dp]
     lib.nir_pdhg_solve.argtypes = [vp, C.POINTER(NirLp), C.POINTER(NirParams), _dp, _dp, C.POINTER(NirInfo)]
     for name in ("nir_gen_equality_lp", "nir_gen_mixed_lp", "nir_gen_staircase_lp"):
@@ -447,6 +448,14 @@ class Session:
         out = (C.c_int32 * 8)()
         self.lib.nir_session_shape(self.h, out)
         return tuple(out)
+
+    def kkt(self) -> np.ndarray:
+        """Relative KKT of a reference-mode session's current iterate, on the
+        device: [rel primal, rel dual, rel gap, primal objective, dual objective]."""
+        out = np.zeros(5)
+        if self.lib.nir_session_kkt(self.h, out.ctypes.data_as(C.POINTER(C.c_double))) != 0:
+            raise EngineError("nir_session_kkt: " + self.eng.error())
+        return out
 
     def close(self):
         if getattr(self, "h", None):

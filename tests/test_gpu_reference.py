@@ -180,6 +180,47 @@ def test_session_runs_in_pieces(engine):
     assert r.iterations == 381 and np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
 
 
+@pytest.mark.parametrize("seg", [0, 700])
+def test_session_kkt_matches_oracle(engine, seg):
+    """Relative KKT of the plain PDHG's iterate, evaluated on the device,
+    against the oracle's evaluation of its own (bit-identical) iterate. The
+    per-element terms are the same double-double arithmetic; only the order
+    of the block sums differs, so the terms agree to rounding of the sums."""
+    lp = gen_mixed_lp(23, 1500, 4500, 6.0, 0.01, 1e3)
+    lp.ru[::7] = np.inf                       # some rows free above, so y may carry sign violations
+    lp.l[::11] = -np.inf                      # free-below columns give dual residuals
+    p = default_params(mode=MODE_REFERENCE, max_iters=10_000, check_termination=0, seg_width=seg)
+    s = engine.session(lp, p)
+    o = Oracle(lp)
+    try:
+        xo = yo = None
+        done = 0
+        for k in (0, 250, 1750):
+            s.run(k)
+            done += k
+            got = s.kkt()
+            r = s.result()
+            xo, yo, _, _, _ = o.pdhg(done, check=False) if done else (r.x, r.y, 0, 0, 0)
+            assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+            want = o.kkt(xo, yo)
+            print(f"K={done} device {got[:3]} oracle {want[:3]}")
+            np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-15)
+            assert got[0] > 0 and got[2] > 0
+    finally:
+        s.close()
+
+
+def test_session_kkt_rejects_pdlp_sessions(engine):
+    from nirnaya_amd.engine import EngineError, MODE_PDLP
+    lp = gen_equality_lp(8, 200, 500, 5)
+    s = engine.session(lp, default_params(mode=MODE_PDLP, max_iters=100))
+    try:
+        with pytest.raises(EngineError):
+            s.kkt()
+    finally:
+        s.close()
+
+
 def test_full_size_cfg1_parity(engine):
     """m=400,000, n=1,200,000, ~9.6M nonzeros: 12 iterations against the oracle."""
     lp = gen_mixed_lp(20261, 400_000, 1_200_000, 6.0, 0.01, 1e3)
