@@ -379,6 +379,10 @@ def main():
     log(f"[bench] timed {args.steps} iterations: {t.total_ms:.1f} ms; per-dispatch sweep: A xbar "
         f"{t.spmv_ms / args.steps * 1e3:.1f} us in {t.spmv_passes} passes, A^T y {t.spmtv_ms / args.steps * 1e3:.1f} us "
         f"in {t.spmtv_passes} passes")
+    # Access floors: the same layouts' value/index streams and operand gathers
+    # with nothing else (no row sums, no epilogue).
+    floor_a, floor_t = sess.gather_floor(0), sess.gather_floor(1)
+    log(f"[bench] access floor: A xbar {floor_a:.1f} us, A^T y {floor_t:.1f} us")
     sess.close()
     eng.close()
 
@@ -429,8 +433,11 @@ def main():
                      "bytes_per_launch": t.spmv_bytes, "avg_launch_us": spmv_avg_s * 1e6,
                      "dispatches_per_launch": t.spmv_passes,
                      "avg_dispatch_us": spmv_avg_s * 1e6 / t.spmv_passes,
-                     "shape": {"tile": shape[0], "rows": shape[1], "segments": shape[2], "fused": shape[3]}},
+                     "shape": {"tile": shape[0], "rows": shape[1], "segments": shape[2], "fused": shape[3]},
+                     "access_floor_us": round(floor_a, 2),
+                     "frac_of_access_floor": round(floor_a / (spmv_avg_s * 1e6), 4)},
         "kernels": {"spmtv_primal_avg_us": spmtv_avg_s * 1e6, "spmtv_dispatches_per_launch": t.spmtv_passes,
+                    "spmtv_access_floor_us": round(floor_t, 2),
                     "spmtv_bytes_per_launch": t.spmtv_bytes,
                     "spmtv_gbs": t.spmtv_bytes / spmtv_avg_s / 1e9,
                     "iteration_bytes": t.iter_bytes,

@@ -143,6 +143,15 @@ NIR_API int nir_session_shape(nir_session* s, int32_t* out8);
  * evaluations. No counterpart in the reference. pdlp sessions: error (their
  * terms are in nir_info). */
 NIR_API int nir_session_kkt(nir_session* s, double* out5);
+/* Access floor of a reference-mode session's product (which = 0: A xbar,
+ * 1: A^T y): microseconds per product of a kernel that makes only the
+ * product's memory accesses over the session's own layout - each stored
+ * nonzero's value and column index streamed, its operand entry gathered once,
+ * one launch per column segment - averaged over `reps` products. The ratio
+ * to the product's own duration (nir_session_time) shows what the row
+ * structure, the segment passes and the fused epilogue add. No counterpart
+ * in the reference. */
+NIR_API int nir_session_gather_floor(nir_session* s, int32_t which, int32_t reps, double* us);
 NIR_API void nir_session_destroy(nir_session* s);
 
 /* Copies `bytes` (read) into a second buffer (write) with a plain grid-stride

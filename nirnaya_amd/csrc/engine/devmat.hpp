@@ -89,6 +89,7 @@ struct CsrShape {
     DevArray<int> col;      // nnz, segment-major
     DevArray<int4> desc;    // concatenated per-segment block descriptors
     std::vector<int> blk_off, nblk;
+    std::vector<long long> seg_nz;   // nseg + 1: segment s holds slots [seg_nz[s], seg_nz[s + 1])
     // Fused-segment blocks (SegCsr): row ranges and per-segment nonzero ranges.
     DevArray<int2> frows, fnz;
     int fnblk = 0;
@@ -157,6 +158,7 @@ inline void upload_shape(CsrShape& S, int nrows, int ncols, const std::vector<in
     for (int p = 0; p < S.nnz; ++p) ++seg_count[col[p] / W];
     std::vector<long long> seg_base(nseg + 1, 0);
     for (int s = 0; s < nseg; ++s) seg_base[s + 1] = seg_base[s] + seg_count[s];
+    S.seg_nz = seg_base;
     std::vector<long long> fill(seg_base.begin(), seg_base.end() - 1);
     for (int s = 0; s < nseg; ++s) rp[static_cast<size_t>(s) * (nrows + 1)] = static_cast<int>(seg_base[s]);
     for (int r = 0; r < nrows; ++r) {

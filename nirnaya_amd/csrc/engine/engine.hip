@@ -160,6 +160,21 @@ __global__ __launch_bounds__(256) void k_copy_calibrate(const double4* __restric
         out[i] = in[i];
 }
 
+// Access floor of a product: every stored nonzero of slots [begin, end) reads
+// its value and column index and gathers v[col] once, with no row structure,
+// no LDS and no epilogue - the memory accesses any CSR product over this
+// layout must make, at the rate the chip sustains for them. Per-lane sums go
+// to out so the loads stay live.
+__global__ __launch_bounds__(256) void k_gather_floor(const int* __restrict__ col, const double* __restrict__ val,
+                                                      const double* __restrict__ v, long long begin, long long end,
+                                                      double* __restrict__ out) {
+    double s = 0.0;
+    const long long stride = static_cast<long long>(gridDim.x) * 256;
+    for (long long p = begin + static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; p < end; p += stride)
+        s += load_stream(val + p) * v[load_stream(col + p)];
+    out[static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x] = s;
+}
+
 bool host_finite(double b) { return std::isfinite(b) && std::fabs(b) < 1e20; }
 
 // y = M v over a segmented shape with values `val` (and running sums `part`).
@@ -827,6 +842,45 @@ NIR_API int nir_calibrate_copy(nir_ctx* ctx, int64_t bytes, double* ms) {
         return 0;
     } catch (const std::exception& ex) {
         ctx->error = ex.what();
+        return -1;
+    }
+}
+
+NIR_API int nir_session_gather_floor(nir_session* S, int32_t which, int32_t reps, double* us) {
+    if (!S) return -1;
+    try {
+        if (S->pdlp) throw std::invalid_argument("the access floor is measured on reference-mode sessions");
+        if (which != 0 && which != 1) throw std::invalid_argument("which must be 0 (A xbar) or 1 (A^T y)");
+        if (reps < 1 || !us) throw std::invalid_argument("reps must be >= 1 and us non-null");
+        HIP_CHECK(hipSetDevice(S->ctx->device));
+        const DevMatrix& M = which == 0 ? S->A : S->AT;
+        const double* v = which == 0 ? S->xbar.p : S->y.p;
+        constexpr int kBlocks = 2048;
+        DevArray<double> out;
+        out.alloc(static_cast<size_t>(kBlocks) * 256);
+        auto launch = [&]() {
+            // One launch per column segment, as the product makes its passes.
+            for (int g = 0; g < M.shape.nseg; ++g)
+                k_gather_floor<<<kBlocks, 256, 0, S->s>>>(M.shape.col.p, M.val.p, v, M.shape.seg_nz[g],
+                                                          M.shape.seg_nz[g + 1], out.p);
+            HIP_CHECK(hipGetLastError());
+        };
+        launch();   // warm caches and code
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        HIP_CHECK(hipEventRecord(e0, S->s));
+        for (int r = 0; r < reps; ++r) launch();
+        HIP_CHECK(hipEventRecord(e1, S->s));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float t = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *us = static_cast<double>(t) * 1e3 / reps;
+        return 0;
+    } catch (const std::exception& ex) {
+        S->ctx->error = ex.what();
         return -1;
     }
 }

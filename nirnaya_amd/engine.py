@@ -125,6 +125,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.nir_session_destroy.argtypes = [vp]
     lib.nir_session_shape.argtypes = [vp, _i32p]
     lib.nir_session_kkt.argtypes = [vp, _dp]
+    lib.nir_session_gather_floor.argtypes = [vp, C.c_int32, C.c_int32, _dp]
     lib.nir_calibrate_copy.argtypes = [vp, C.c_int64, _dp]
     lib.nir_pdhg_solve.argtypes = [vp, C.POINTER(NirLp), C.POINTER(NirParams), _dp, _dp, C.POINTER(NirInfo)]
     for name in ("nir_gen_equality_lp", "nir_gen_mixed_lp", "nir_gen_staircase_lp"):
@@ -456,6 +457,14 @@ class Session:
         if self.lib.nir_session_kkt(self.h, out.ctypes.data_as(C.POINTER(C.c_double))) != 0:
             raise EngineError("nir_session_kkt: " + self.eng.error())
         return out
+
+    def gather_floor(self, which: int, reps: int = 20) -> float:
+        """Microseconds per product (0: A xbar, 1: A^T y) of the access-only
+        kernel over this session's layout (reference mode)."""
+        us = C.c_double()
+        if self.lib.nir_session_gather_floor(self.h, int(which), int(reps), C.byref(us)) != 0:
+            raise EngineError("nir_session_gather_floor: " + self.eng.error())
+        return us.value
 
     def close(self):
         if getattr(self, "h", None):

@@ -221,6 +221,25 @@ def test_session_kkt_rejects_pdlp_sessions(engine):
         s.close()
 
 
+def test_gather_floor_measures_both_products(engine):
+    lp = gen_mixed_lp(5, 3000, 9000, 6.0, 0.01, 1e3)
+    s = engine.session(lp, default_params(mode=MODE_REFERENCE, max_iters=100, check_termination=0,
+                                          seg_width=2000))
+    try:
+        s.run(10)
+        before = s.result()
+        fa, ft = s.gather_floor(0, 5), s.gather_floor(1, 5)
+        after = s.result()
+        assert 0 < fa < 1e5 and 0 < ft < 1e5
+        # it only reads the session's arrays
+        assert np.array_equal(before.x, after.x) and np.array_equal(before.y, after.y)
+        from nirnaya_amd.engine import EngineError
+        with pytest.raises(EngineError):
+            s.gather_floor(2, 5)
+    finally:
+        s.close()
+
+
 def test_full_size_cfg1_parity(engine):
     """m=400,000, n=1,200,000, ~9.6M nonzeros: 12 iterations against the oracle."""
     lp = gen_mixed_lp(20261, 400_000, 1_200_000, 6.0, 0.01, 1e3)
