@@ -281,6 +281,14 @@ NIR_API const double* nir_owned_lp_x0(const nir_owned_lp* g);
 NIR_API const double* nir_owned_lp_y0(const nir_owned_lp* g);
 NIR_API void nir_owned_lp_free(nir_owned_lp* g);
 
+/* Binary LP container (.nlpb, the format of sov_write_lpb) for engine-level
+ * LPs: write a canonical CSC LP (min form, continuous), or read one back as an
+ * owned LP (a maximisation is returned in min form; quadratic or integer
+ * models are rejected). Errors: -1 / NULL with nir_last_error() set. No
+ * counterpart in the reference. */
+NIR_API int nir_lp_write_lpb(const nir_lp* lp, const char* path);
+NIR_API nir_owned_lp* nir_lp_read_lpb(const char* path);
+
 #ifdef __cplusplus
 }
 #endif

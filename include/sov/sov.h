@@ -43,6 +43,12 @@ SOV_API int sov_set_quad(sov_model* model, int nnz, const int* i, const int* j, 
 /* reference sov.h:21-22 */
 SOV_API int sov_read_mps(sov_model* model, const char* path);
 SOV_API int sov_write_mps(sov_model* model, const char* path);
+/* Binary LP container (.nlpb): the model's compressed-column arrays with a
+ * checksum, loaded without text parsing. No counterpart in the reference; it
+ * sits beside sov_read_mps / sov_write_mps and carries the same model.
+ * sov_read_lpb rejects truncated, corrupt or non-canonical files. */
+SOV_API int sov_read_lpb(sov_model* model, const char* path);
+SOV_API int sov_write_lpb(sov_model* model, const char* path);
 
 /* reference sov.h:24-26 */
 SOV_API int sov_set_option_string(sov_model* model, const char* name, const char* value);

@@ -523,6 +523,14 @@ NIR_API void nir_params_default(nir_params* p) {
 
 NIR_API const char* nir_last_error(void) { return g_last_error.c_str(); }
 
+}  // extern "C"
+
+namespace nir {
+void record_last_error(const std::string& what) { g_last_error = what; }
+}  // namespace nir
+
+extern "C" {
+
 NIR_API int nir_abi_layout(int64_t* out, int32_t cap) {
     const int64_t v[] = {
         static_cast<int64_t>(sizeof(nir_lp)),          static_cast<int64_t>(offsetof(nir_lp, obj_offset)),

@@ -131,6 +131,34 @@ SOV_API int sov_write_mps(sov_model* model, const char* path) {
     }
 }
 
+SOV_API int sov_read_lpb(sov_model* model, const char* path) {
+    if (!model || !path) return -1;
+    std::string err;
+    const nir::Status st = nir::read_lpb(path, model->model, err);
+    if (st != nir::Status::Optimal) {
+        model->last_error = err;
+        return -1;
+    }
+    model->finalized = true;
+    return 0;
+}
+
+SOV_API int sov_write_lpb(sov_model* model, const char* path) {
+    if (!model || !path) return -1;
+    try {
+        model->ensure_finalized();
+        std::string err;
+        if (!nir::write_lpb(path, model->model, err)) {
+            model->last_error = err;
+            return -1;
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        model->last_error = e.what();
+        return -1;
+    }
+}
+
 SOV_API int sov_set_option_string(sov_model* model, const char* name, const char* value) {
     if (!model || !name || !value) return -1;
     return model->solver.options.set_by_name(name, value) ? 0 : -1;

@@ -22,7 +22,7 @@ int main(int argc, char** argv) {
     }
     nir::Model model;
     std::string err;
-    if (nir::read_mps(argv[1], model, err) != nir::Status::Optimal) {
+    if (nir::read_model(argv[1], model, err) != nir::Status::Optimal) {
         std::fprintf(stderr, "FAIL: Could not read model %s: %s\n", argv[1], err.c_str());
         return 1;
     }

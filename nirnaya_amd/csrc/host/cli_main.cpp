@@ -1,4 +1,4 @@
-// sovsolve <model.mps> [options] - command line front end of the facade.
+// sovsolve <model.mps|model.nlpb> [options] - command line front end of the facade.
 // Flags and exit codes follow src/driver/cli_main.cpp:35-98 and
 // src/core/status.hpp:48-70. Every "--flag value" is also accepted as
 // "--flag=value" (so --method=pdhg works), and "--set key=value" reaches any
@@ -12,7 +12,7 @@ namespace {
 
 void usage() {
     std::printf(
-        "Usage: sovsolve <model.mps> [options]\n"
+        "Usage: sovsolve <model.mps|model.nlpb> [options]\n"
         "  --method <m>              pdhg | gpu (GPU PDHG engine)\n"
         "  --pdhg-mode <m>           reference | pdlp\n"
         "  --precision <p>           fp64 | mixed (pdlp mode)\n"
@@ -28,6 +28,8 @@ void usage() {
         "  --deterministic           accepted for compatibility\n"
         "  --tolerance-preset <p>    strict | default | relaxed\n"
         "  --set <name>=<value>      set any option by name\n"
+        "  --write-lpb <path>        write the model as a binary LP (.nlpb) and exit\n"
+        "  --write-mps <path>        write the model as MPS and exit\n"
         "  --version                 print the version\n");
 }
 
@@ -39,7 +41,7 @@ int main(int argc, char** argv) {
         return 4;
     }
     nir::Options opt;
-    std::string path;
+    std::string path, lpb_out, mps_out;
     try {
         for (int i = 1; i < argc; ++i) {
             std::string arg = argv[i];
@@ -75,6 +77,10 @@ int main(int argc, char** argv) {
                 opt.scale = false;
             } else if (arg == "--deterministic") {
                 opt.deterministic = true;
+            } else if (arg == "--write-lpb" && value(v)) {
+                lpb_out = v;
+            } else if (arg == "--write-mps" && value(v)) {
+                mps_out = v;
             } else if (arg == "--sol" && value(v)) {
                 opt.sol_path = v;
             } else if (arg == "--stats-json" && value(v)) {
@@ -126,10 +132,24 @@ int main(int argc, char** argv) {
     }
     nir::Model model;
     std::string err;
-    const nir::Status rs = nir::read_mps(path, model, err);
+    const nir::Status rs = nir::read_model(path, model, err);
     if (rs != nir::Status::Optimal) {
         std::fprintf(stderr, "Input Error reading %s: %s\n", path.c_str(), err.c_str());
         return nir::status_exit_code(rs);
+    }
+    if (!lpb_out.empty() || !mps_out.empty()) {
+        if (!lpb_out.empty() && !nir::write_lpb(lpb_out, model, err)) {
+            std::fprintf(stderr, "Error writing %s: %s\n", lpb_out.c_str(), err.c_str());
+            return 4;
+        }
+        if (!mps_out.empty() && !nir::write_mps(mps_out, model)) {
+            std::fprintf(stderr, "Error writing %s\n", mps_out.c_str());
+            return 4;
+        }
+        std::printf("Wrote %s%s%s (%d rows, %d columns, %lld nonzeros)\n", lpb_out.c_str(),
+                    (!lpb_out.empty() && !mps_out.empty()) ? " and " : "", mps_out.c_str(), model.nrow, model.ncol,
+                    static_cast<long long>(model.A.nnz()));
+        return 0;
     }
     nir::Solver solver(opt);
     const nir::SolveResult res = solver.solve(model);

@@ -8,11 +8,20 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
 #include <unordered_set>
 #include <vector>
 
 #include "../../../include/nirnaya/pdhg.h"
 #include "model.hpp"
+#include "prep.hpp"
+
+namespace nir {
+// The engine's per-thread error text, read by nir_last_error() (engine.hip).
+void record_last_error(const std::string& what);
+}  // namespace nir
 
 namespace nir {
 namespace {
@@ -99,9 +108,10 @@ private:
 
 struct nir_owned_lp {
     int32_t m = 0, n = 0;
+    double obj_offset = 0.0;
     std::vector<int64_t> start;
     std::vector<int32_t> index;
-    std::vector<double> value, cost, collo, colup, rowlo, rowup, x0, y0;
+    std::vector<double> value, cost, collo, colup, rowlo, rowup, x0, y0;   // x0, y0 empty when unknown
 };
 
 namespace {
@@ -311,13 +321,83 @@ NIR_API int nir_owned_lp_view(const nir_owned_lp* g, nir_lp* out) {
     out->colup = g->colup.data();
     out->rowlo = g->rowlo.data();
     out->rowup = g->rowup.data();
-    out->obj_offset = 0.0;
+    out->obj_offset = g->obj_offset;
     return 0;
 }
 
 NIR_API int64_t nir_owned_lp_nnz(const nir_owned_lp* g) { return g ? g->start.back() : -1; }
-NIR_API const double* nir_owned_lp_x0(const nir_owned_lp* g) { return g ? g->x0.data() : nullptr; }
-NIR_API const double* nir_owned_lp_y0(const nir_owned_lp* g) { return g ? g->y0.data() : nullptr; }
+NIR_API const double* nir_owned_lp_x0(const nir_owned_lp* g) {
+    return (g && !g->x0.empty()) ? g->x0.data() : nullptr;
+}
+NIR_API const double* nir_owned_lp_y0(const nir_owned_lp* g) {
+    return (g && !g->y0.empty()) ? g->y0.data() : nullptr;
+}
+
+NIR_API int nir_lp_write_lpb(const nir_lp* lp, const char* path) {
+    try {
+        if (!lp || !path) throw std::invalid_argument("missing LP or path");
+        if (lp->nrow < 0 || lp->ncol < 0) throw std::invalid_argument("negative dimensions");
+        nir::Model m;
+        m.nrow = lp->nrow;
+        m.ncol = lp->ncol;
+        const size_t n = static_cast<size_t>(lp->ncol), mr = static_cast<size_t>(lp->nrow);
+        if ((n && (!lp->col_start || !lp->cost || !lp->collo || !lp->colup)) || (mr && (!lp->rowlo || !lp->rowup)))
+            throw std::invalid_argument("missing LP array");
+        m.A.nrow = lp->nrow;
+        m.A.ncol = lp->ncol;
+        m.A.start.assign(lp->col_start ? lp->col_start : nullptr, lp->col_start ? lp->col_start + n + 1 : nullptr);
+        if (m.A.start.empty()) m.A.start.assign(1, 0);
+        const size_t nnz = static_cast<size_t>(m.A.start.back());
+        if (nnz && (!lp->row_index || !lp->value)) throw std::invalid_argument("missing matrix arrays");
+        m.A.index.assign(lp->row_index, lp->row_index + nnz);
+        m.A.value.assign(lp->value, lp->value + nnz);
+        const std::string defect = m.A.start.front() == 0 ? m.A.check() : std::string("start[0] != 0");
+        if (!defect.empty()) throw std::invalid_argument("matrix not canonical: " + defect);
+        m.cost.assign(lp->cost, lp->cost + n);
+        m.collo.assign(lp->collo, lp->collo + n);
+        m.colup.assign(lp->colup, lp->colup + n);
+        m.rowlo.assign(lp->rowlo, lp->rowlo + mr);
+        m.rowup.assign(lp->rowup, lp->rowup + mr);
+        m.vartype.assign(n, nir::VarType::Continuous);
+        m.obj_offset = lp->obj_offset;
+        std::string err;
+        if (!nir::write_lpb(path, m, err)) throw std::runtime_error(err);
+        return 0;
+    } catch (const std::exception& ex) {
+        nir::record_last_error(ex.what());
+        return -1;
+    }
+}
+
+NIR_API nir_owned_lp* nir_lp_read_lpb(const char* path) {
+    try {
+        if (!path) throw std::invalid_argument("missing path");
+        nir::Model m;
+        std::string err;
+        if (nir::read_lpb(path, m, err) != nir::Status::Optimal) throw std::runtime_error(err);
+        if (m.is_qp()) throw std::invalid_argument("the file holds a quadratic objective");
+        if (m.is_mip()) throw std::invalid_argument("the file holds integer variables");
+        auto g = std::make_unique<nir_owned_lp>();
+        g->m = m.nrow;
+        g->n = m.ncol;
+        // min form: a maximisation is stored as min -c^T x - offset (src/model/model.hpp:73-110)
+        const double sgn = m.sense == nir::Sense::Maximize ? -1.0 : 1.0;
+        g->obj_offset = sgn * m.obj_offset;
+        g->start = std::move(m.A.start);
+        g->index = std::move(m.A.index);
+        g->value = std::move(m.A.value);
+        g->cost = std::move(m.cost);
+        for (double& c : g->cost) c *= sgn;
+        g->collo = std::move(m.collo);
+        g->colup = std::move(m.colup);
+        g->rowlo = std::move(m.rowlo);
+        g->rowup = std::move(m.rowup);
+        return g.release();
+    } catch (const std::exception& ex) {
+        nir::record_last_error(ex.what());
+        return nullptr;
+    }
+}
 NIR_API void nir_owned_lp_free(nir_owned_lp* g) { delete g; }
 
 }  // extern "C"

@@ -69,4 +69,12 @@ bool read_sol(const std::string& path, const Model& m, Status& st, double& obj,
 Status read_mps(const std::string& path, Model& out, std::string& err);
 bool write_mps(const std::string& path, const Model& m);
 
+// Binary LP container (".nlpb", lpb.cpp): the model's arrays as stored, with
+// a checksum; the reader validates sizes, checksum and canonical form.
+bool is_lpb_file(const std::string& path);
+bool write_lpb(const std::string& path, const Model& m, std::string& err);
+Status read_lpb(const std::string& path, Model& out, std::string& err);
+// read_lpb for binary files (recognised by their magic), read_mps otherwise.
+Status read_model(const std::string& path, Model& out, std::string& err);
+
 }  // namespace nir

@@ -169,6 +169,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.nir_owned_lp_y0.argtypes = [vp]
     lib.nir_owned_lp_y0.restype = _dp
     lib.nir_owned_lp_free.argtypes = [vp]
+    lib.nir_lp_write_lpb.argtypes = [C.POINTER(NirLp), C.c_char_p]
+    lib.nir_lp_read_lpb.argtypes = [C.c_char_p]
+    lib.nir_lp_read_lpb.restype = vp
     _lib = lib
     return lib
 
@@ -241,20 +244,38 @@ class LP:
         return A
 
 
-def _from_owned(lib, g) -> LP:
+def _from_owned(lib, g, what="generator rejected its arguments") -> LP:
     if not g:
-        raise EngineError("generator rejected its arguments")
+        raise EngineError(what)
     v = NirLp()
     lib.nir_owned_lp_view(g, C.byref(v))
     m, n = v.nrow, v.ncol
     nnz = lib.nir_owned_lp_nnz(g)
     arr = lambda p, k, t: np.ctypeslib.as_array(p, shape=(k,)).astype(t, copy=True) if k > 0 else np.zeros(0, t)
+    x0p, y0p = lib.nir_owned_lp_x0(g), lib.nir_owned_lp_y0(g)
     lp = LP(m, n, arr(v.col_start, n + 1, np.int64), arr(v.row_index, nnz, np.int32), arr(v.value, nnz, np.float64),
             arr(v.cost, n, np.float64), arr(v.collo, n, np.float64), arr(v.colup, n, np.float64),
-            arr(v.rowlo, m, np.float64), arr(v.rowup, m, np.float64), 0.0,
-            arr(lib.nir_owned_lp_x0(g), n, np.float64), arr(lib.nir_owned_lp_y0(g), m, np.float64))
+            arr(v.rowlo, m, np.float64), arr(v.rowup, m, np.float64), float(v.obj_offset),
+            arr(x0p, n, np.float64) if x0p else None, arr(y0p, m, np.float64) if y0p else None)
     lib.nir_owned_lp_free(g)
     return lp
+
+
+def write_lp_binary(lp: LP, path: str) -> None:
+    """Write an engine LP as a binary LP container (.nlpb)."""
+    lib = load_library()
+    v = lp.struct()
+    if lib.nir_lp_write_lpb(C.byref(v), path.encode("utf-8")) != 0:
+        raise EngineError("nir_lp_write_lpb: " + lib.nir_last_error().decode())
+
+
+def read_lp_binary(path: str) -> LP:
+    """Read a binary LP container (.nlpb) as an engine LP (min form)."""
+    lib = load_library()
+    g = lib.nir_lp_read_lpb(path.encode("utf-8"))
+    if not g:
+        raise EngineError("nir_lp_read_lpb: " + lib.nir_last_error().decode())
+    return _from_owned(lib, g)
 
 
 def gen_equality_lp(seed: int, m: int, n: int, per_col: int = 5) -> LP:

@@ -6,7 +6,8 @@ Same class, methods, properties and defaults as the reference wrapper
 and ``Status`` / ``ObjVal`` / ``ObjBound`` / ``X``. Set
 ``set_option("method", "pdhg")`` to solve on the MI355X PDHG engine.
 Additions: ``Y`` (row duals), ``Iterations``, ``read_mps``, ``write_mps``,
-``set_start`` (warm start) and ``error_message``.
+``read_lpb`` / ``write_lpb`` (binary LP container), ``set_start`` (warm
+start) and ``error_message``.
 """
 import ctypes
 import os
@@ -75,6 +76,10 @@ class NirnayaModel:
             L.sov_set_start.argtypes = [vp, dp, dp]
             L.sov_get_num_cols.argtypes = [vp]
             L.sov_get_num_rows.argtypes = [vp]
+            L.sov_read_lpb.argtypes = [vp, ctypes.c_char_p]
+            L.sov_read_lpb.restype = i
+            L.sov_write_lpb.argtypes = [vp, ctypes.c_char_p]
+            L.sov_write_lpb.restype = i
 
     def add_var(self, cost: float = 0.0, lb: float = 0.0, ub: float = 1e20, vtype: int = 0, name: str = "") -> int:
         idx = self._lib.sov_add_col(self._model, float(cost), float(lb), float(ub), int(vtype), name.encode("utf-8"))
@@ -107,6 +112,16 @@ class NirnayaModel:
 
     def write_mps(self, path: str) -> int:
         return self._lib.sov_write_mps(self._model, path.encode("utf-8"))
+
+    def read_lpb(self, path: str) -> int:
+        rc = self._lib.sov_read_lpb(self._model, path.encode("utf-8"))
+        if rc == 0 and self._has_ext:
+            self._num_cols = self._lib.sov_get_num_cols(self._model)
+            self._num_rows = self._lib.sov_get_num_rows(self._model)
+        return rc
+
+    def write_lpb(self, path: str) -> int:
+        return self._lib.sov_write_lpb(self._model, path.encode("utf-8"))
 
     def set_start(self, x: Optional[Sequence[float]] = None, y: Optional[Sequence[float]] = None) -> int:
         xa = (ctypes.c_double * len(x))(*x) if x is not None else None

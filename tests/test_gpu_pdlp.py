@@ -108,6 +108,24 @@ def test_facade_pdlp_mode_on_fixture(tmp_path):
     assert obj == pytest.approx(22171.59777424483, rel=1e-6)
 
 
+def test_facade_reads_binary_container(tmp_path):
+    # The same solve from the binary LP container gives the same solution
+    # file, bit for bit, as from the MPS text.
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sovsolve = os.path.join(root, "bin", "sovsolve")
+    lpb = str(tmp_path / "crude.nlpb")
+    assert subprocess.run([sovsolve, lpkit.CRUDE_MPS, "--write-lpb", lpb], capture_output=True).returncode == 0
+    sols = []
+    for src in (lpkit.CRUDE_MPS, lpb):
+        sol = tmp_path / (os.path.basename(src) + ".sol")
+        rc = subprocess.run([sovsolve, src, "--method=pdhg", "--pdhg-mode=pdlp", "--tol=1e-9", "--log", "0", "--sol",
+                             str(sol)]).returncode
+        assert rc == 0
+        sols.append(sol.read_text())
+    assert sols[0] == sols[1]
+
+
 @pytest.mark.parametrize("polish", [0, 1])
 def test_facade_mixed_precision_on_fixture(tmp_path, polish):
     # CLI -> facade (presolve, scaling, engine in mixed precision, unscale,
