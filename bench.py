@@ -359,9 +359,18 @@ def main():
     log(f"[bench] instance m={lp.m} n={lp.n} nnz={lp.nnz} generated in {time.time() - t0:.1f}s")
 
     eng = Engine(0)
+    calibration = None
     if args.calibrate:
         ms = eng.calibrate_copy(256 << 20)
         log(f"[bench] calibration copy of 256 MiB: {ms:.3f} ms ({2 * 256 * 1.048576 / ms:.0f} GB/s read+write)")
+        # Random FP64 gathers at the instance's nonzero count, by operand size.
+        calibration = {"copy_256MiB_ms": ms, "gather_rate_G_per_s": {}}
+        for length in (1_200_000, 400_000, 240_000, 100_000, 4_000):
+            g = lp.nnz / eng.calibrate_gather(lp.nnz, length, with_values=False) / 1e3
+            gv = lp.nnz / eng.calibrate_gather(lp.nnz, length, with_values=True) / 1e3
+            calibration["gather_rate_G_per_s"][str(length * 8)] = {"gather": round(g, 1), "value_x_gather": round(gv, 1)}
+            log(f"[bench] random FP64 gathers, operand {length * 8 / 1e6:.2f} MB: {g:.0f} G/s alone, "
+                f"{gv:.0f} G/s with the value stream")
     params = default_params(mode=MODE_REFERENCE, max_iters=args.warmup + 2 * args.steps + 64, check_termination=0,
                             tile_nnz=args.tile, tile_rows=args.tile_rows, seg_width=args.seg,
                             seg_width_t=args.seg_t, seg_fused=args.fused)
@@ -443,6 +452,7 @@ def main():
                     "iteration_bytes": t.iter_bytes,
                     "iteration_gbs": t.iter_bytes * args.steps / (t.total_ms / 1e3) / 1e9},
         "time_to_1e-6": ttt,
+        "calibration": calibration,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -157,6 +157,13 @@ NIR_API void nir_session_destroy(nir_session* s);
 /* Copies `bytes` (read) into a second buffer (write) with a plain grid-stride
  * kernel, k_copy_calibrate: the known-size reference for HBM byte counters. */
 NIR_API int nir_calibrate_copy(nir_ctx* ctx, int64_t bytes, double* ms);
+/* Random-gather rate: microseconds per launch of a grid-stride kernel
+ * (`blocks` x 256 lanes) that gathers an FP64 operand of `len` entries at n
+ * uniform random int32 indices (streamed), optionally times a streamed FP64
+ * value each; average of 10 launches after 2 warm ones. Measures the
+ * access pattern of a sparse product over an unstructured matrix. */
+NIR_API int nir_calibrate_gather(nir_ctx* ctx, int64_t n, int32_t len, int32_t with_values, int32_t blocks,
+                                 double* us);
 
 /* One-shot: create, solve, copy out, destroy. */
 NIR_API int nir_pdhg_solve(nir_ctx* ctx, const nir_lp* lp, const nir_params* p, double* x, double* y,

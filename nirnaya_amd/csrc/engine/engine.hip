@@ -175,6 +175,39 @@ __global__ __launch_bounds__(256) void k_gather_floor(const int* __restrict__ co
     out[static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x] = s;
 }
 
+// Random-gather rate probe: n uniform random indices into an operand of len
+// entries (splitmix64 of the position), values in [0, 1).
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_fill_gather_probe(int* __restrict__ idx, double* __restrict__ val,
+                                                           double* __restrict__ x, long long n, int len) {
+    const long long stride = static_cast<long long>(gridDim.x) * 256;
+    for (long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
+        const unsigned long long h = mix64(static_cast<unsigned long long>(i));
+        idx[i] = static_cast<int>(h % static_cast<unsigned long long>(len));
+        val[i] = static_cast<double>(h >> 11) * 0x1.0p-53;
+        if (i < len) x[i] = val[i];
+    }
+}
+
+template <int WITH_VALUES>
+__global__ __launch_bounds__(256) void k_gather_probe(const int* __restrict__ idx, const double* __restrict__ val,
+                                                      const double* __restrict__ x, long long n,
+                                                      double* __restrict__ out) {
+    double s = 0.0;
+    const long long stride = static_cast<long long>(gridDim.x) * 256;
+    for (long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
+        const double g = x[load_stream(idx + i)];
+        s += WITH_VALUES ? load_stream(val + i) * g : g;
+    }
+    out[static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x] = s;
+}
+
 bool host_finite(double b) { return std::isfinite(b) && std::fabs(b) < 1e20; }
 
 // y = M v over a segmented shape with values `val` (and running sums `part`).
@@ -889,6 +922,48 @@ NIR_API int nir_session_gather_floor(nir_session* S, int32_t which, int32_t reps
         return 0;
     } catch (const std::exception& ex) {
         S->ctx->error = ex.what();
+        return -1;
+    }
+}
+
+NIR_API int nir_calibrate_gather(nir_ctx* ctx, int64_t n, int32_t len, int32_t with_values, int32_t blocks,
+                                 double* us) {
+    if (!ctx) return -1;
+    try {
+        if (n < 1 || len < 1 || len > n || blocks < 1 || blocks > 65536 || !us)
+            throw std::invalid_argument("need 1 <= len <= n, 1 <= blocks <= 65536 and an output");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        DevArray<int> idx;
+        DevArray<double> val, x, out;
+        idx.alloc(static_cast<size_t>(n));
+        val.alloc(static_cast<size_t>(n));
+        x.alloc(static_cast<size_t>(len));
+        out.alloc(static_cast<size_t>(blocks) * 256);
+        k_fill_gather_probe<<<2048, 256, 0, s>>>(idx.p, val.p, x.p, n, len);
+        auto launch = [&]() {
+            if (with_values) k_gather_probe<1><<<blocks, 256, 0, s>>>(idx.p, val.p, x.p, n, out.p);
+            else k_gather_probe<0><<<blocks, 256, 0, s>>>(idx.p, val.p, x.p, n, out.p);
+        };
+        launch();
+        launch();
+        HIP_CHECK(hipGetLastError());
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        constexpr int kReps = 10;
+        HIP_CHECK(hipEventRecord(e0, s));
+        for (int r = 0; r < kReps; ++r) launch();
+        HIP_CHECK(hipEventRecord(e1, s));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float t = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *us = static_cast<double>(t) * 1e3 / kReps;
+        return 0;
+    } catch (const std::exception& ex) {
+        ctx->error = ex.what();
         return -1;
     }
 }

@@ -127,6 +127,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.nir_session_kkt.argtypes = [vp, _dp]
     lib.nir_session_gather_floor.argtypes = [vp, C.c_int32, C.c_int32, _dp]
     lib.nir_calibrate_copy.argtypes = [vp, C.c_int64, _dp]
+    lib.nir_calibrate_gather.argtypes = [vp, C.c_int64, C.c_int32, C.c_int32, C.c_int32, _dp]
     lib.nir_pdhg_solve.argtypes = [vp, C.POINTER(NirLp), C.POINTER(NirParams), _dp, _dp, C.POINTER(NirInfo)]
     for name in ("nir_gen_equality_lp", "nir_gen_mixed_lp", "nir_gen_staircase_lp"):
         getattr(lib, name).restype = vp
@@ -357,6 +358,15 @@ class Engine:
         if self.lib.nir_calibrate_copy(self.h, int(nbytes), C.byref(ms)) != 0:
             raise EngineError("nir_calibrate_copy: " + self.error())
         return ms.value
+
+    def calibrate_gather(self, n: int, length: int, with_values: bool = True, blocks: int = 2048) -> float:
+        """Microseconds per launch gathering an FP64 operand of `length` entries at
+        n uniform random indices (times a streamed value each when with_values)."""
+        us = C.c_double()
+        if self.lib.nir_calibrate_gather(self.h, int(n), int(length), 1 if with_values else 0, int(blocks),
+                                         C.byref(us)) != 0:
+            raise EngineError("nir_calibrate_gather: " + self.error())
+        return us.value
 
     def gemm_check(self, A: np.ndarray, B: np.ndarray, transpose_a=False, transpose_b=False) -> np.ndarray:
         """C = A' B'^T on the device with bf16-rounded inputs, where A' is A ([M][K]) or A^T when A is
