@@ -364,10 +364,13 @@ def main():
         ms = eng.calibrate_copy(256 << 20)
         log(f"[bench] calibration copy of 256 MiB: {ms:.3f} ms ({2 * 256 * 1.048576 / ms:.0f} GB/s read+write)")
         # Random FP64 gathers at the instance's nonzero count, by operand size.
-        calibration = {"copy_256MiB_ms": ms, "gather_rate_G_per_s": {}}
+        st_us = eng.calibrate_gather(lp.nnz, 1, with_values=2)
+        calibration = {"copy_256MiB_ms": ms, "gather_rate_G_per_s": {},
+                       "index_value_stream_us": st_us, "index_value_stream_GBs": lp.nnz * 12 / st_us / 1e3}
+        log(f"[bench] index + value streams alone: {st_us:.2f} us ({lp.nnz * 12 / st_us / 1e3:.0f} GB/s)")
         for length in (1_200_000, 400_000, 240_000, 100_000, 4_000):
-            g = lp.nnz / eng.calibrate_gather(lp.nnz, length, with_values=False) / 1e3
-            gv = lp.nnz / eng.calibrate_gather(lp.nnz, length, with_values=True) / 1e3
+            g = lp.nnz / eng.calibrate_gather(lp.nnz, length, with_values=0) / 1e3
+            gv = lp.nnz / eng.calibrate_gather(lp.nnz, length, with_values=1) / 1e3
             calibration["gather_rate_G_per_s"][str(length * 8)] = {"gather": round(g, 1), "value_x_gather": round(gv, 1)}
             log(f"[bench] random FP64 gathers, operand {length * 8 / 1e6:.2f} MB: {g:.0f} G/s alone, "
                 f"{gv:.0f} G/s with the value stream")

@@ -159,9 +159,10 @@ NIR_API void nir_session_destroy(nir_session* s);
 NIR_API int nir_calibrate_copy(nir_ctx* ctx, int64_t bytes, double* ms);
 /* Random-gather rate: microseconds per launch of a grid-stride kernel
  * (`blocks` x 256 lanes) that gathers an FP64 operand of `len` entries at n
- * uniform random int32 indices (streamed), optionally times a streamed FP64
- * value each; average of 10 launches after 2 warm ones. Measures the
- * access pattern of a sparse product over an unstructured matrix. */
+ * uniform random int32 indices (streamed); with_values 1 also multiplies
+ * each by a streamed FP64 value, 2 streams the indices and values without
+ * gathering; average of 10 launches after 2 warm ones. Measures the access
+ * pattern of a sparse product over an unstructured matrix. */
 NIR_API int nir_calibrate_gather(nir_ctx* ctx, int64_t n, int32_t len, int32_t with_values, int32_t blocks,
                                  double* us);
 

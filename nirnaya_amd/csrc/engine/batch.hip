@@ -360,6 +360,42 @@ __device__ void wg_after_done(const View& V, const Inst& I, Shared<BT>& sh, int 
     __syncthreads();
 }
 
+// Infeasibility certificate of a restart (terms and decision in
+// pdlp_core.hpp), taken before the anchor moves. The step between the anchors
+// is formed as the single-LP engine forms it after the move: T(z) - anchor in
+// FP64, (X + dxp) - X in the FP32 phase. Returns the same code for every
+// thread (0, kStPrimalInfeasible or kStDualInfeasible).
+template <typename T, int BT>
+__device__ int wg_certificate(const View& V, const Inst& I, Shared<BT>& sh) {
+    const Local<double> D = local(V, V.d, I);
+    const float* fxp = V.f.xp + I.xo;
+    const float* fyp = V.f.yp + I.yo;
+    constexpr bool kF = std::is_same<T, float>::value;
+    auto sx = [&](int j) -> double {
+        return kF ? (D.xa[j] + static_cast<double>(fxp[j])) - D.xa[j] : D.xp[j] - D.xa[j];
+    };
+    auto sy = [&](int i) -> double {
+        return kF ? (D.ya[i] + static_cast<double>(fyp[i])) - D.ya[i] : D.yp[i] - D.ya[i];
+    };
+    dd acc[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) acc[q] = dd_zero();
+    for (int j = threadIdx.x; j < I.n; j += BT) {
+        dd g = dd_zero();
+        for (int p = D.trow[j]; p < D.trow[j + 1]; ++p) g = dd_add(g, dd_prod(D.tval[p], sy(D.tcol[p])));
+        cert_col_terms(-dd_val(g), D.c[j], D.lo[j], D.up[j], sx(j), acc);
+    }
+    for (int i = threadIdx.x; i < I.m; i += BT) {
+        dd h = dd_zero();
+        for (int p = D.arow[i]; p < D.arow[i + 1]; ++p) h = dd_add(h, dd_prod(D.aval[p], sx(D.acol[p])));
+        cert_row_terms(dd_val(h), D.rl[i], D.ru[i], sy(i), acc);
+    }
+    wg_dd_sum<6, BT>(acc, sh.red, sh.tot);
+    const int code = cert_decide(sh.tot);
+    __syncthreads();
+    return code;
+}
+
 // One evaluation block: eval_freq iterations, evaluation, restart copy.
 template <typename T, int BT>
 __device__ void wg_block(const View& V, const Inst& I, const Vec<T>& v, Shared<BT>& sh) {
@@ -372,8 +408,18 @@ __device__ void wg_block(const View& V, const Inst& I, const Vec<T>& v, Shared<B
     const int fp32 = sh.ctl.fp32, polish_done = sh.ctl.polish_done;
     __syncthreads();
     if (restart) {
+        const int cert = wg_certificate<T, BT>(V, I, sh);
         if (std::is_same<T, float>::value) wg_recenter<BT>(V, I, sh.s, 0);
         else wg_restart_copy<T, BT>(L, I);
+        if (cert) {
+            if (threadIdx.x == 0) {
+                sh.s.done = 1;
+                sh.s.status = cert;
+                sh.s.restart = 0;
+            }
+            __syncthreads();
+            wg_after_done<BT>(V, I, sh, cert, fp32, polish_done);
+        }
     }
     if (done) wg_after_done<BT>(V, I, sh, st, fp32, polish_done);
 }
@@ -1149,7 +1195,9 @@ NIR_API int nir_batch_result(nir_batch* B, int32_t i, double* x, double* y, nir_
         if (y) std::copy(ys.begin(), ys.end(), y);
         if (info) {
             std::memset(info, 0, sizeof(*info));
-            info->status = c.finished ? c.status : (B->stopped ? 4 : 6);
+            PdlpState fin = h;
+            fin.status = c.status;
+            info->status = c.finished ? sov_status(fin) : (B->stopped ? 4 : 6);
             info->iterations = h.total;
             info->polish_iterations = c.fp64_at >= 0 ? h.total - c.fp64_at : 0;
             info->restarts = h.restarts;

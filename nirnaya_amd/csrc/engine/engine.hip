@@ -202,8 +202,12 @@ __global__ __launch_bounds__(256) void k_gather_probe(const int* __restrict__ id
     double s = 0.0;
     const long long stride = static_cast<long long>(gridDim.x) * 256;
     for (long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
-        const double g = x[load_stream(idx + i)];
-        s += WITH_VALUES ? load_stream(val + i) * g : g;
+        if (WITH_VALUES == 2) {   // streams only: the index is used as a number, nothing gathered
+            s += load_stream(val + i) + static_cast<double>(load_stream(idx + i));
+        } else {
+            const double g = x[load_stream(idx + i)];
+            s += WITH_VALUES ? load_stream(val + i) * g : g;
+        }
     }
     out[static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x] = s;
 }
@@ -930,8 +934,8 @@ NIR_API int nir_calibrate_gather(nir_ctx* ctx, int64_t n, int32_t len, int32_t w
                                  double* us) {
     if (!ctx) return -1;
     try {
-        if (n < 1 || len < 1 || len > n || blocks < 1 || blocks > 65536 || !us)
-            throw std::invalid_argument("need 1 <= len <= n, 1 <= blocks <= 65536 and an output");
+        if (n < 1 || len < 1 || len > n || blocks < 1 || blocks > 65536 || !us || with_values < 0 || with_values > 2)
+            throw std::invalid_argument("need 1 <= len <= n, 1 <= blocks <= 65536, mode 0..2 and an output");
         HIP_CHECK(hipSetDevice(ctx->device));
         hipStream_t s = ctx->stream;
         DevArray<int> idx;
@@ -942,7 +946,8 @@ NIR_API int nir_calibrate_gather(nir_ctx* ctx, int64_t n, int32_t len, int32_t w
         out.alloc(static_cast<size_t>(blocks) * 256);
         k_fill_gather_probe<<<2048, 256, 0, s>>>(idx.p, val.p, x.p, n, len);
         auto launch = [&]() {
-            if (with_values) k_gather_probe<1><<<blocks, 256, 0, s>>>(idx.p, val.p, x.p, n, out.p);
+            if (with_values == 2) k_gather_probe<2><<<blocks, 256, 0, s>>>(idx.p, val.p, x.p, n, out.p);
+            else if (with_values) k_gather_probe<1><<<blocks, 256, 0, s>>>(idx.p, val.p, x.p, n, out.p);
             else k_gather_probe<0><<<blocks, 256, 0, s>>>(idx.p, val.p, x.p, n, out.p);
         };
         launch();

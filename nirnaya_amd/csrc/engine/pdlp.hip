@@ -231,6 +231,84 @@ __global__ void k_restart_copy(const PdlpState* __restrict__ st, int n, int m, c
     }
 }
 
+// ------------------------------------------------------------------ infeasibility certificates
+
+// The step between the anchors of a restart is tested as a ray (the terms and
+// the decision are in pdlp_core.hpp). Only the kernels' early exits run when
+// the evaluation did not restart.
+__device__ __forceinline__ bool cert_active(const PdlpState* st) { return !st->done && st->restart; }
+
+// Before the anchor moves: keep it.
+__global__ void k_cert_save(const PdlpState* __restrict__ st, int n, int m, const double* __restrict__ X,
+                            const double* __restrict__ Y, double* __restrict__ qx, double* __restrict__ qy) {
+    if (!cert_active(st)) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    for (int j = t; j < n; j += stride) qx[j] = X[j];
+    for (int i = t; i < m; i += stride) qy[i] = Y[i];
+}
+
+// After it moved: d = new anchor - old anchor.
+__global__ void k_cert_diff(const PdlpState* __restrict__ st, int n, int m, const double* __restrict__ X,
+                            const double* __restrict__ Y, const double* __restrict__ qx,
+                            const double* __restrict__ qy, double* __restrict__ dx, double* __restrict__ dy) {
+    if (!cert_active(st)) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    for (int j = t; j < n; j += stride) dx[j] = X[j] - qx[j];
+    for (int i = t; i < m; i += stride) dy[i] = Y[i] - qy[i];
+}
+
+// Column terms (one lane per column, (A^T dy)_j over CSR(A^T)).
+__global__ __launch_bounds__(kThreads) void k_cert_cols(const PdlpState* __restrict__ st, int n, int nseg,
+                                                        const int* __restrict__ rp, const int* __restrict__ col,
+                                                        const double* __restrict__ val,
+                                                        const double* __restrict__ dy, const double* __restrict__ dx,
+                                                        const double* __restrict__ c, const double* __restrict__ lo,
+                                                        const double* __restrict__ up, dd* __restrict__ part) {
+    __shared__ dd sh[6 * kThreads];
+    if (!cert_active(st)) return;
+    dd acc[6] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
+    const int j = blockIdx.x * kThreads + threadIdx.x;
+    if (j < n) cert_col_terms(-dd_val(row_dot_dd(n, nseg, rp, col, val, dy, j)), c[j], lo[j], up[j], dx[j], acc);
+    block_dd_reduce<6>(acc, sh, part + 6 * blockIdx.x);
+}
+
+// Row terms (one lane per row, (A dx)_i over CSR(A)).
+__global__ __launch_bounds__(kThreads) void k_cert_rows(const PdlpState* __restrict__ st, int m, int nseg,
+                                                        const int* __restrict__ rp, const int* __restrict__ col,
+                                                        const double* __restrict__ val,
+                                                        const double* __restrict__ dx, const double* __restrict__ dy,
+                                                        const double* __restrict__ rl, const double* __restrict__ ru,
+                                                        dd* __restrict__ part) {
+    __shared__ dd sh[6 * kThreads];
+    if (!cert_active(st)) return;
+    dd acc[6] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < m) cert_row_terms(dd_val(row_dot_dd(m, nseg, rp, col, val, dx, i)), rl[i], ru[i], dy[i], acc);
+    block_dd_reduce<6>(acc, sh, part + 6 * blockIdx.x);
+}
+
+// Reduce both partial sets and decide (one workgroup).
+__global__ __launch_bounds__(kThreads) void k_cert_decide(PdlpState* st, const dd* __restrict__ pc, int nbc,
+                                                          const dd* __restrict__ pr, int nbr) {
+    __shared__ dd sh[6 * kThreads];
+    __shared__ dd tot[6];
+    if (!cert_active(st)) return;
+    dd acc[6] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
+    for (int b = threadIdx.x; b < nbc; b += kThreads)
+        for (int q = 0; q < 6; ++q) acc[q] = dd_add(acc[q], pc[6 * b + q]);
+    for (int b = threadIdx.x; b < nbr; b += kThreads)
+        for (int q = 0; q < 6; ++q) acc[q] = dd_add(acc[q], pr[6 * b + q]);
+    block_dd_reduce<6>(acc, sh, tot);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const int code = cert_decide(tot);
+    if (code) {
+        st->done = 1;
+        st->status = code;
+        st->restart = 0;
+    }
+}
+
 // Plain y = M v (power iteration).
 template <int TILE>
 __global__ __launch_bounds__(kThreads) void k_spmv_plain(Pass<double> P, const double* __restrict__ v,
@@ -409,6 +487,9 @@ struct Pdlp {
     Iterates<double> d;        // FP32 phase: d.xa / d.ya are the FP64 anchor X, Y
     Iterates<float> f;         // FP32 phase: deltas and re-based constants (f.ya = -Y / sigma)
     Buf<double> HA, GA;        // FP32 phase: A X and A^T Y of the anchor
+    Buf<double> qx, qy;        // certificate: the anchor before the restart
+    Buf<double> rx, ry;        // certificate: the step between the two anchors
+    Buf<dd> cert_c, cert_r;    // certificate: per-block partial sums
     Buf<dd> part_c, part_r;
     Buf<PdlpState> st;
     PdlpState* h_st = nullptr;   // pinned, two poll slots
@@ -517,6 +598,22 @@ void recenter(Pdlp& P, hipStream_t s, int force) {
     PD_CHECK(hipGetLastError());
 }
 
+// Certificate test of the step between the anchors (kernels exit early unless
+// the decision just restarted).
+void launch_certificate(Pdlp& P, hipStream_t s) {
+    const int n = P.n, m = P.m;
+    k_cert_diff<<<grid_for(std::max(n, m)), 256, 0, s>>>(P.st.p, n, m, P.d.xa.p, P.d.ya.p, P.qx.p, P.qy.p, P.rx.p,
+                                                         P.ry.p);
+    const int gc = std::max(1, (n + kThreads - 1) / kThreads), gr = std::max(1, (m + kThreads - 1) / kThreads);
+    const CsrShape& AT = P.AT->shape;
+    const CsrShape& A = P.A->shape;
+    k_cert_cols<<<gc, kThreads, 0, s>>>(P.st.p, n, AT.nseg, AT.rowptr.p, AT.col.p, P.AT->val.p, P.ry.p, P.rx.p,
+                                        P.d.c.p, P.d.lo.p, P.d.up.p, P.cert_c.p);
+    k_cert_rows<<<gr, kThreads, 0, s>>>(P.st.p, m, A.nseg, A.rowptr.p, A.col.p, P.A->val.p, P.rx.p, P.ry.p, P.d.rl.p,
+                                        P.d.ru.p, P.cert_r.p);
+    k_cert_decide<<<1, kThreads, 0, s>>>(P.st.p, P.cert_c.p, gc, P.cert_r.p, gr);
+}
+
 template <typename TI>
 void launch_eval(Pdlp& P, hipStream_t s, Iterates<TI>& it, int mode) {
     const int force = mode == 1 ? 1 : 0;
@@ -547,11 +644,15 @@ void launch_eval(Pdlp& P, hipStream_t s, Iterates<TI>& it, int mode) {
     const int nbr = A.shape.nblk[A.shape.nseg - 1];
     k_decide<<<1, kThreads, 0, s>>>(P.st.p, P.part_c.p, nbc, P.part_r.p, nbr, mode);
     if (mode == 0) {
+        // The FP64 anchor is d.xa / d.ya in both phases.
+        const int gv = grid_for(std::max(P.n, P.m));
+        k_cert_save<<<gv, 256, 0, s>>>(P.st.p, P.n, P.m, P.d.xa.p, P.d.ya.p, P.qx.p, P.qy.p);
         if (kDelta)
             recenter(P, s, 0);
         else
-            k_restart_copy<TI><<<grid_for(std::max(P.n, P.m)), 256, 0, s>>>(P.st.p, P.n, P.m, it.xp.p, it.yp.p,
-                                                                              it.x.p, it.xa.p, it.y.p, it.ya.p, 0);
+            k_restart_copy<TI><<<gv, 256, 0, s>>>(P.st.p, P.n, P.m, it.xp.p, it.yp.p, it.x.p, it.xa.p, it.y.p,
+                                                  it.ya.p, 0);
+        launch_certificate(P, s);
     }
     PD_CHECK(hipGetLastError());
 }
@@ -814,6 +915,12 @@ void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm,
     for (auto& e : P.ev) PD_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     P.part_c.alloc(static_cast<size_t>(5) * ST.nblk[ST.nseg - 1]);
     P.part_r.alloc(static_cast<size_t>(6) * SA.nblk[SA.nseg - 1]);
+    P.qx.alloc(n);
+    P.qy.alloc(m);
+    P.rx.alloc(n);
+    P.ry.alloc(m);
+    P.cert_c.alloc(static_cast<size_t>(6) * std::max(1, (n + kThreads - 1) / kThreads));
+    P.cert_r.alloc(static_cast<size_t>(6) * std::max(1, (m + kThreads - 1) / kThreads));
 
     if (P.mixed) {
         P.a32.val.alloc(P.nnz);
@@ -887,12 +994,12 @@ void pdlp_run(Pdlp& P, hipStream_t s, const nir_params& prm, long long budget) {
                         PdlpState h2 = read_state(P, s);
                         if (h2.done) {
                             P.finished = true;
-                            P.status = h2.status;
+                            P.status = sov_status(h2);
                         }
                         continue;
                     }
                     P.finished = true;
-                    P.status = h.status;
+                    P.status = sov_status(h);
                     break;
                 }
             }
@@ -929,7 +1036,7 @@ int pdlp_result(Pdlp& P, hipStream_t s, double* x, double* y, nir_info* info) {
     if (y) std::copy(ys.begin(), ys.end(), y);
     if (info) {
         std::memset(info, 0, sizeof(*info));
-        info->status = P.finished ? P.status : (h.done ? h.status : 6);
+        info->status = P.finished ? P.status : (h.done ? sov_status(h) : 6);
         info->iterations = h.total;
         info->polish_iterations = P.fp64_iters_at_switch >= 0 ? h.total - P.fp64_iters_at_switch : 0;
         info->restarts = h.restarts;

@@ -43,6 +43,20 @@ __host__ __device__ __forceinline__ dd dd_prod(double a, double b) {
 
 __host__ __device__ __forceinline__ double dd_val(dd a) { return a.hi + a.lo; }
 
+// sum_p val[p] v[col[p]] over row i of a segmented CSR (row pointers
+// nseg * (nrows + 1)), in ascending column order (segment by segment), in
+// double-double.
+__device__ __forceinline__ dd row_dot_dd(int nrows, int nseg, const int* __restrict__ rp,
+                                         const int* __restrict__ col, const double* __restrict__ val,
+                                         const double* __restrict__ v, int i) {
+    dd acc = dd_zero();
+    for (int s = 0; s < nseg; ++s) {
+        const int* r = rp + static_cast<size_t>(s) * (nrows + 1);
+        for (int p = r[i]; p < r[i + 1]; ++p) acc = dd_add(acc, dd_prod(val[p], v[col[p]]));
+    }
+    return acc;
+}
+
 // 2^-floor((e+1)/2) for v in [2^e, 2^(e+1)); 1 for v == 0 or non-finite.
 __host__ __device__ __forceinline__ double pow2_inv_sqrt(double v) {
     if (!(v > 0.0) || !isfinite(v)) return 1.0;
@@ -279,6 +293,89 @@ __device__ __forceinline__ void pdlp_decide(PdlpState& s, const dd* tot, int mod
         s.k_base = k_new;
         s.restart = 0;
     }
+}
+
+// ------------------------------------------------------------------ infeasibility certificates
+
+// On an infeasible or unbounded LP the restarted iterates drift along a ray,
+// so at every restart the step between the old and the new anchor, d = (dx,
+// dy), is tested as a certificate in the engine's scaled space:
+//   dual ray dy (primal infeasible): lam = -A^T dy; violations lam - proj(lam)
+//     for infinite column bounds and dy_i > 0 / < 0 for infinite rl_i / ru_i;
+//     objective D = sum rl_i dy_i [dy_i > 0] + ru_i dy_i [dy_i < 0]
+//                 + sum l_j lp_j [lp_j > 0] + u_j lp_j [lp_j < 0] > 0;
+//   primal ray dx (dual infeasible): the parts of A dx and dx outside the
+//     recession cones of the rows and bounds; objective -c^T dx > 0.
+// A dx and A^T dy are double-double dot products over the FP64 matrices and
+// every sum is double-double. A ray passes when its objective exceeds 1e-12
+// of the sum of its terms' magnitudes and ||violations||_2 <= kCertEps times
+// the objective.
+constexpr double kCertEps = 1e-8;
+constexpr int kStPrimalInfeasible = 21;   // PdlpState::status: a dual ray passed
+constexpr int kStDualInfeasible = 22;     // PdlpState::status: a primal ray passed
+
+// Column j with lam = -(A^T dy)_j; slots: 0 dual violation^2, 1 D, 2 |D
+// terms|, 3 c^T dx, 4 |c| |dx|, 5 primal violation^2.
+__device__ __forceinline__ void cert_col_terms(double lam, double c, double lo, double up, double dx, dd (&acc)[6]) {
+    const bool lf = lo > -INFINITY, uf = up < INFINITY;
+    double lp = lam;
+    if (!lf) lp = amin(lp, 0.0);
+    if (!uf) lp = amax(lp, 0.0);
+    const double r = lam - lp;
+    acc[0] = dd_add(acc[0], dd_prod(r, r));
+    if (lf && lp > 0.0) {
+        acc[1] = dd_add(acc[1], dd_prod(lo, lp));
+        acc[2] = dd_add(acc[2], dd_prod(fabs(lo), lp));
+    }
+    if (uf && lp < 0.0) {
+        acc[1] = dd_add(acc[1], dd_prod(up, lp));
+        acc[2] = dd_add(acc[2], dd_prod(fabs(up), -lp));
+    }
+    acc[3] = dd_add(acc[3], dd_prod(c, dx));
+    acc[4] = dd_add(acc[4], dd_prod(fabs(c), fabs(dx)));
+    double v = 0.0;
+    if (lf && dx < 0.0) v = -dx;
+    if (uf && dx > 0.0) v = dx;
+    acc[5] = dd_add(acc[5], dd_prod(v, v));
+}
+
+// Row i with h = (A dx)_i, same slots (3 and 4 untouched).
+__device__ __forceinline__ void cert_row_terms(double h, double rl, double ru, double dy, dd (&acc)[6]) {
+    const bool lf = rl > -INFINITY, uf = ru < INFINITY;
+    double v = 0.0;
+    if (lf && h < 0.0) v = -h;
+    if (uf && h > 0.0) v = h;
+    acc[5] = dd_add(acc[5], dd_prod(v, v));
+    double sv = 0.0;
+    if (!lf && dy > 0.0) sv = dy;
+    if (!uf && dy < 0.0) sv = -dy;
+    acc[0] = dd_add(acc[0], dd_prod(sv, sv));
+    if (lf && dy > 0.0) {
+        acc[1] = dd_add(acc[1], dd_prod(rl, dy));
+        acc[2] = dd_add(acc[2], dd_prod(fabs(rl), dy));
+    }
+    if (uf && dy < 0.0) {
+        acc[1] = dd_add(acc[1], dd_prod(ru, dy));
+        acc[2] = dd_add(acc[2], dd_prod(fabs(ru), -dy));
+    }
+}
+
+// Decision from the six totals: kStPrimalInfeasible, kStDualInfeasible or 0.
+__device__ __forceinline__ int cert_decide(const dd* tot) {
+    const double D = dd_val(tot[1]), Pn = -dd_val(tot[3]);
+    if (D > 0.0 && D > 1e-12 * dd_val(tot[2]) && sqrt(dd_val(tot[0])) <= kCertEps * D) return kStPrimalInfeasible;
+    if (Pn > 0.0 && Pn > 1e-12 * dd_val(tot[4]) && sqrt(dd_val(tot[5])) <= kCertEps * Pn) return kStDualInfeasible;
+    return 0;
+}
+
+// Status of a finished state in the reference's codes (src/core/status.hpp:7-24):
+// a dual ray certifies infeasibility; a primal ray certifies dual
+// infeasibility - unbounded when the last evaluated point met the primal
+// tolerance, infeasible-or-unbounded otherwise.
+__host__ __device__ inline int sov_status(const PdlpState& h) {
+    if (h.status == kStPrimalInfeasible) return 1;
+    if (h.status == kStDualInfeasible) return h.rel_p <= h.tol ? 2 : 3;
+    return h.status;
 }
 
 }  // namespace gpu

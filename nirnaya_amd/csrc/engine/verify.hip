@@ -117,19 +117,6 @@ __global__ __launch_bounds__(256) void k_verify_cols(int n, const double* __rest
 
 // ------------------------------------------------------------------ relative KKT
 
-// sum_p val[p] v[col[p]] over row i of a segmented CSR, in ascending column
-// order (segment by segment), in double-double.
-__device__ __forceinline__ dd row_dot_dd(int nrows, int nseg, const int* __restrict__ rp,
-                                         const int* __restrict__ col, const double* __restrict__ val,
-                                         const double* __restrict__ v, int i) {
-    dd acc = dd_zero();
-    for (int s = 0; s < nseg; ++s) {
-        const int* r = rp + static_cast<size_t>(s) * (nrows + 1);
-        for (int p = r[i]; p < r[i + 1]; ++p) acc = dd_add(acc, dd_prod(val[p], v[col[p]]));
-    }
-    return acc;
-}
-
 // Fixed-order tree over the block of four dd terms per lane; lane 0 writes
 // the block's four sums to part[4 b ..].
 __device__ __forceinline__ void block_sum4(dd (&t)[4], dd (*sh)[256], dd* part) {

@@ -359,11 +359,12 @@ class Engine:
             raise EngineError("nir_calibrate_copy: " + self.error())
         return ms.value
 
-    def calibrate_gather(self, n: int, length: int, with_values: bool = True, blocks: int = 2048) -> float:
+    def calibrate_gather(self, n: int, length: int, with_values: int = 1, blocks: int = 2048) -> float:
         """Microseconds per launch gathering an FP64 operand of `length` entries at
-        n uniform random indices (times a streamed value each when with_values)."""
+        n uniform random indices (with_values 1: times a streamed value each;
+        2: the index and value streams only, no gather)."""
         us = C.c_double()
-        if self.lib.nir_calibrate_gather(self.h, int(n), int(length), 1 if with_values else 0, int(blocks),
+        if self.lib.nir_calibrate_gather(self.h, int(n), int(length), int(with_values), int(blocks),
                                          C.byref(us)) != 0:
             raise EngineError("nir_calibrate_gather: " + self.error())
         return us.value

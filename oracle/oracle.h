@@ -20,7 +20,10 @@
  *   or_dd_verify     src/io/sol_io.hpp:25-106         KAT unit_tests.cpp:29-42,
  *                                                     README.md:84 (2.55e-14)
  *   or_pdlp          build-defined restarted PDHG     KKT = 0 at known optima,
- *                                                     scaling invariance, HiGHS optimum
+ *                                                     scaling invariance, HiGHS optimum;
+ *                    + ray certificates at restarts   status 21 / 22 exactly on LPs
+ *                                                     scipy's HiGHS finds infeasible /
+ *                                                     unbounded, 0 on feasible ones
  */
 #ifndef NIR_ORACLE_H
 #define NIR_ORACLE_H
@@ -75,7 +78,9 @@ typedef struct {
 } or_pdlp_params;
 
 typedef struct {
-    int32_t status; /* 0 optimal, 6 iteration limit, 11 numerical trouble */
+    int32_t status; /* 0 optimal, 6 iteration limit, 11 numerical trouble,
+                       21 a dual ray certifies infeasibility,
+                       22 a primal ray certifies dual infeasibility */
     int32_t restarts;
     int64_t iterations;
     double omega, eta, rel_p, rel_d, rel_g, pobj, dobj;

@@ -3,6 +3,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -480,6 +481,95 @@ static void pdlp_eval(const pmat* M, pstate* s, const double* c, const double* l
     }
 }
 
+/* Infeasibility certificate test of a candidate ray (dx, dy) in the scaled
+ * space (build-defined, after the PDLP infeasibility-detection idea: on an
+ * infeasible or unbounded LP the restarted iterates drift along a ray, so the
+ * difference of consecutive restart points is tested as one).
+ *   Dual ray dy (certifies primal infeasibility): lam = -A^T dy; violations
+ *   are lam - proj(lam) for infinite column bounds and dy_i > 0 / dy_i < 0 for
+ *   infinite rl_i / ru_i; the ray objective
+ *   D = sum_i (rl_i dy_i [dy_i > 0] + ru_i dy_i [dy_i < 0])
+ *     + sum_j (l_j lp_j [lp_j > 0] + u_j lp_j [lp_j < 0])
+ *   must be positive.
+ *   Primal ray dx (certifies dual infeasibility): violations are the parts of
+ *   A dx and dx outside the recession cones of the rows and bounds; -c^T dx
+ *   must be positive.
+ * A dx and A^T dy are double-double dot products in ascending index order and
+ * every sum is double-double, so an exact zero stays zero to ~1e-32. A ray
+ * passes when its objective exceeds 1e-12 of the sum of its terms' magnitudes
+ * and ||violations||_2 <= eps * objective. Returns 21 (dual ray passes), 22
+ * (primal ray passes) or 0; ratios[0..1] = violation / objective (inf when
+ * the objective test fails). */
+static double row_dot_dd(const pmat* M, int32_t i, const double* v) {
+    dd s = {0, 0};
+    for (int32_t p = M->rp[i]; p < M->rp[i + 1]; ++p) dd_add_prod(&s, M->av[p], v[M->ci[p]]);
+    return s.hi + s.lo;
+}
+
+static double col_dot_dd(const pmat* M, int32_t j, const double* v) {
+    dd s = {0, 0};
+    for (int64_t p = M->cs[j]; p < M->cs[j + 1]; ++p) dd_add_prod(&s, M->tv[p], v[M->ri[p]]);
+    return s.hi + s.lo;
+}
+
+static int cert_check(const pmat* M, const double* c, const double* lo, const double* up, const double* rl,
+                      const double* ru, const double* dx, const double* dy, double eps, double* ratios) {
+    dd pobj = {0, 0}, pabs = {0, 0}, pviol = {0, 0}, dobj = {0, 0}, dabs = {0, 0}, dviol = {0, 0};
+    for (int32_t j = 0; j < M->n; ++j) {
+        const double lam = -col_dot_dd(M, j, dy);
+        const int lf = lo[j] > -INFINITY, uf = up[j] < INFINITY;
+        double lp = lam;
+        if (!lf) lp = smin(lp, 0.0);
+        if (!uf) lp = smax(lp, 0.0);
+        const double r = lam - lp;
+        dd_add_prod(&dviol, r, r);
+        if (lf && lp > 0.0) {
+            dd_add_prod(&dobj, lo[j], lp);
+            dd_add_prod(&dabs, fabs(lo[j]), lp);
+        }
+        if (uf && lp < 0.0) {
+            dd_add_prod(&dobj, up[j], lp);
+            dd_add_prod(&dabs, fabs(up[j]), -lp);
+        }
+        dd_add_prod(&pobj, c[j], dx[j]);
+        dd_add_prod(&pabs, fabs(c[j]), fabs(dx[j]));
+        double v = 0.0;
+        if (lf && dx[j] < 0.0) v = -dx[j];
+        if (uf && dx[j] > 0.0) v = dx[j];
+        dd_add_prod(&pviol, v, v);
+    }
+    for (int32_t i = 0; i < M->m; ++i) {
+        const double h = row_dot_dd(M, i, dx);
+        const int lf = rl[i] > -INFINITY, uf = ru[i] < INFINITY;
+        double v = 0.0;
+        if (lf && h < 0.0) v = -h;
+        if (uf && h > 0.0) v = h;
+        dd_add_prod(&pviol, v, v);
+        double sv = 0.0;
+        if (!lf && dy[i] > 0.0) sv = dy[i];
+        if (!uf && dy[i] < 0.0) sv = -dy[i];
+        dd_add_prod(&dviol, sv, sv);
+        if (lf && dy[i] > 0.0) {
+            dd_add_prod(&dobj, rl[i], dy[i]);
+            dd_add_prod(&dabs, fabs(rl[i]), dy[i]);
+        }
+        if (uf && dy[i] < 0.0) {
+            dd_add_prod(&dobj, ru[i], dy[i]);
+            dd_add_prod(&dabs, fabs(ru[i]), -dy[i]);
+        }
+    }
+    const double D = ddv(dobj), Pn = -ddv(pobj);
+    const double rd = (D > 0.0 && D > 1e-12 * ddv(dabs)) ? sqrt(ddv(dviol)) / D : INFINITY;
+    const double rp = (Pn > 0.0 && Pn > 1e-12 * ddv(pabs)) ? sqrt(ddv(pviol)) / Pn : INFINITY;
+    if (ratios) {
+        ratios[0] = rd;
+        ratios[1] = rp;
+    }
+    if (rd <= eps) return 21;
+    if (rp <= eps) return 22;
+    return 0;
+}
+
 void or_pdlp(const or_lp* lp, const or_pdlp_params* prm, double* x_out, double* y_out, or_pdlp_info* info) {
     const int32_t m = lp->m, n = lp->n;
     const int64_t nnz = n > 0 ? lp->start[n] : 0;
@@ -571,6 +661,7 @@ void or_pdlp(const or_lp* lp, const or_pdlp_params* prm, double* x_out, double* 
     /* start point */
     double *x = malloc(N1 * 8), *xa = malloc(N1 * 8), *xp = malloc(N1 * 8), *xb = malloc(N1 * 8);
     double *y = malloc(M1 * 8), *ya = malloc(M1 * 8), *yp = malloc(M1 * 8);
+    double *dxr = malloc(N1 * 8), *dyr = malloc(M1 * 8);
     for (int32_t j = 0; j < n; ++j) {
         double v = prm->x0 ? prm->x0[j] / C[j] : 0.0;
         v = (lo[j] < v) ? v : lo[j];
@@ -660,8 +751,19 @@ void or_pdlp(const or_lp* lp, const or_pdlp_params* prm, double* x_out, double* 
         }
         pdlp_eval(&Mt, &s, c, lo, up, rl, ru, rt, ct, xp, yp, xa, ya, 0);
         if (s.restart) {
+            for (int32_t j = 0; j < n; ++j) dxr[j] = xp[j] - xa[j];
+            for (int32_t i = 0; i < m; ++i) dyr[i] = yp[i] - ya[i];
             for (int32_t j = 0; j < n; ++j) x[j] = xa[j] = xp[j];
             for (int32_t i = 0; i < m; ++i) y[i] = ya[i] = yp[i];
+            double ratios[2];
+            const int cs = cert_check(&Mt, c, lo, up, rl, ru, dxr, dyr, 1e-8, ratios);
+            if (getenv("OR_CERT_DEBUG"))
+                fprintf(stderr, "restart %d at %lld: dual-ray ratio %.3e primal-ray ratio %.3e\n", s.restarts,
+                        (long long)s.total, ratios[0], ratios[1]);
+            if (cs) {
+                s.done = 1;
+                s.status = cs;
+            }
         }
     }
     for (int32_t j = 0; j < n; ++j) x_out[j] = xp[j] * C[j];
@@ -680,5 +782,5 @@ void or_pdlp(const or_lp* lp, const or_pdlp_params* prm, double* x_out, double* 
     }
     free(Mt.rp); free(Mt.ci); free(Mt.av); free(Mt.tv); free(R); free(C);
     free(c); free(lo); free(up); free(ct); free(rl); free(ru); free(rt);
-    free(x); free(xa); free(xp); free(xb); free(y); free(ya); free(yp);
+    free(x); free(xa); free(xp); free(xb); free(y); free(ya); free(yp); free(dxr); free(dyr);
 }

@@ -41,6 +41,41 @@ def transport_optimum() -> float:
     return float(best)
 
 
+def _dense(lp):
+    A = np.zeros((lp.m, lp.n))
+    for j in range(lp.n):
+        A[lp.index[lp.start[j]:lp.start[j + 1]], j] = lp.value[lp.start[j]:lp.start[j + 1]]
+    return A
+
+
+def infeasible_lps():
+    """LPs without an optimum, infeasible or unbounded by construction, with
+    the kind of certificate that proves it: 'dual' (a Farkas ray y: the LP is
+    infeasible) or 'primal' (a ray x of decreasing cost: no dual solution)."""
+    from nirnaya_amd.engine import gen_equality_lp
+    out = {}
+    # x1 + x2 >= 5 and x1 + x2 <= 3 on [0, 10]^2: y = (1, -1) gives 5 - 3 > 0.
+    out["two_rows_contradict"] = ("dual", LP.from_dense([[1.0, 1.0], [1.0, 1.0]], [1.0, 1.0], [0.0, 0.0],
+                                                        [10.0, 10.0], [5.0, -INF], [INF, 3.0]))
+    # min -x1, x1 - x2 <= 1, x >= 0: x = (1, 1) keeps feasibility and lowers the cost.
+    out["ray_of_descent"] = ("primal", LP.from_dense([[1.0, -1.0]], [-1.0, 0.0], [0.0, 0.0], [INF, INF], [-INF],
+                                                     [1.0]))
+    # an equality LP with its first row repeated at right-hand side + 1
+    base = gen_equality_lp(5, 200, 500, 5)
+    A = _dense(base)
+    out["repeated_row_shifted"] = ("dual", LP.from_dense(np.vstack([A, A[:1]]), base.c, base.l, base.u,
+                                                         np.append(base.rl, base.rl[0] + 1.0),
+                                                         np.append(base.ru, base.ru[0] + 1.0)))
+    # an equality LP with a column and its negative, both unbounded above, the
+    # first with cost -1: their difference is a free direction of descent
+    base = gen_equality_lp(6, 200, 500, 5)
+    A = _dense(base)
+    out["free_direction"] = ("primal", LP.from_dense(np.hstack([A, A[:, :1], -A[:, :1]]),
+                                                     np.append(base.c, [-1.0, 0.0]), np.append(base.l, [0.0, 0.0]),
+                                                     np.append(base.u, [INF, INF]), base.rl, base.ru))
+    return out
+
+
 def read_mps_lp(path):
     """Minimal free-MPS reader for the fixtures (N/L/G/E rows, COLUMNS, RHS,
     UP/LO bounds, OBJSENSE MAX). Returns (LP with the file's own costs,

@@ -171,3 +171,18 @@ def test_batch_should_stop_reports_time_limit(engine):
         b.close()
     assert all(r.status == 4 for r in res)
     assert all(0 < r.iterations < 400000 for r in res)
+
+
+def test_batch_certifies_infeasible_members(engine):
+    # Infeasible and unbounded members end with the single-LP engine's
+    # certificate status and iterate; the feasible ones beside them are solved.
+    import lpkit
+    bad = [lp for _, lp in sorted(lpkit.infeasible_lps().items())]
+    lps = bad + [lpkit.transport_lp(), gen_staircase_lp(20263001, 5, 6)]
+    for prec in (0, 1):
+        params = default_params(mode=MODE_PDLP, precision=prec, max_iters=50000, tol=1e-6)
+        res, _ = check_batch(engine, lps, params)
+        kinds = [kind for _, (kind, _) in sorted(lpkit.infeasible_lps().items())]
+        for r, kind in zip(res, kinds):
+            assert r.status == 1 if kind == "dual" else r.status in (2, 3)
+        assert [r.status for r in res[len(bad):]] == [0, 0]

@@ -158,3 +158,39 @@ def test_python_wrapper_mixed_precision():
     assert m.Status == 0
     assert m.ObjVal == pytest.approx(9.0, abs=1e-6)
     assert np.allclose(m.X, [3.0, 1.0], atol=1e-5)
+
+
+def _sov_status(info):
+    """The oracle's ray codes in the reference's status codes."""
+    if info.status == 21:
+        return 1
+    if info.status == 22:
+        return 2 if info.rel_p <= 1e-6 else 3
+    return info.status
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("name", sorted(lpkit.infeasible_lps()))
+def test_infeasibility_certificates(engine, name, prec):
+    kind, lp = lpkit.infeasible_lps()[name]
+    p = default_params(mode=MODE_PDLP, precision=PREC_MIXED if prec == "mixed" else 0, max_iters=200000, tol=1e-6)
+    r = engine.solve(lp, p)
+    print(f"{name} {prec}: status {r.status} after {r.iterations} iterations, {r.info.restarts} restarts")
+    assert r.status == 1 if kind == "dual" else r.status in (2, 3)
+    if prec == "fp64":
+        # the same trajectory and the same decision as the oracle
+        _, _, info = Oracle(lp).pdlp(max_iters=200000, tol=1e-6)
+        assert (r.iterations, r.info.restarts, r.status) == (info.iterations, info.restarts, _sov_status(info))
+
+
+def test_python_wrapper_reports_infeasible():
+    from nirnaya_amd.nirnaya import NirnayaModel
+    m = NirnayaModel("infeasible")
+    a = m.add_var(1.0, 0.0, 10.0)
+    b = m.add_var(1.0, 0.0, 10.0)
+    m.add_constr([a, b], [1.0, 1.0], 5.0, 1e20)
+    m.add_constr([a, b], [1.0, 1.0], -1e20, 3.0)
+    for k, v in (("method", "pdhg"), ("pdhg_mode", "pdlp"), ("presolve", "0")):
+        m.set_option(k, v)
+    m.optimize()
+    assert m.Status == 1

@@ -65,3 +65,38 @@ def test_pdlp_oracle_units_follow_unscale_vectors():
     k = Oracle(lp).kkt(xs * Cs, ys * R)
     assert max(k[0], k[1], k[2]) <= 1.5e-8
     assert info.rel_p == pytest.approx(k[0], rel=1e-3, abs=1e-14)
+
+
+def _linprog_status(lp):
+    """scipy's HiGHS on the same LP: 0 optimal, 2 infeasible, 3 unbounded."""
+    from scipy.optimize import linprog
+    A = lpkit._dense(lp)
+    fin = lambda v: np.isfinite(v) & (np.abs(v) < 1e20)
+    ub_rows = [i for i in range(lp.m) if fin(lp.ru[i])]
+    lb_rows = [i for i in range(lp.m) if fin(lp.rl[i])]
+    A_ub = np.vstack([A[ub_rows], -A[lb_rows]])
+    b_ub = np.concatenate([lp.ru[ub_rows], -lp.rl[lb_rows]])
+    bounds = [(lp.l[j] if fin(lp.l[j]) else None, lp.u[j] if fin(lp.u[j]) else None) for j in range(lp.n)]
+    return linprog(lp.c, A_ub=A_ub, b_ub=b_ub, bounds=bounds, method="highs").status
+
+
+@pytest.mark.parametrize("name", sorted(lpkit.infeasible_lps()))
+def test_pdlp_oracle_certifies_infeasible_and_unbounded(name):
+    # A dual ray (status 21) must come only from LPs scipy's HiGHS finds
+    # infeasible, a primal ray (22) only from unbounded ones; a sign error in
+    # either ray objective or a dropped violation term breaks the pairing.
+    kind, lp = lpkit.infeasible_lps()[name]
+    x, y, info = Oracle(lp).pdlp(max_iters=200000, tol=1e-6)
+    want = {"dual": (21, 2), "primal": (22, 3)}[kind]
+    assert info.status == want[0], (name, info.status, info.iterations)
+    assert _linprog_status(lp) == want[1]
+    assert info.iterations < 20000
+
+
+@pytest.mark.parametrize("name", ["transport", "unit11", "staircase"])
+def test_pdlp_oracle_no_certificate_on_feasible_lps(name):
+    lp = {"transport": lpkit.transport_lp, "unit11": lpkit.pdhg_unit_lp,
+          "staircase": lambda: gen_staircase_lp(3, 6, 8)}[name]()
+    assert _linprog_status(lp) == 0
+    _, _, info = Oracle(lp).pdlp(max_iters=400000, tol=1e-9)
+    assert info.status == 0
