@@ -177,7 +177,7 @@ def test_batch_certifies_infeasible_members(engine):
     # Infeasible and unbounded members end with the single-LP engine's
     # certificate status and iterate; the feasible ones beside them are solved.
     import lpkit
-    bad = [lp for _, lp in sorted(lpkit.infeasible_lps().items())]
+    bad = [lp for _, (_, lp) in sorted(lpkit.infeasible_lps().items())]
     lps = bad + [lpkit.transport_lp(), gen_staircase_lp(20263001, 5, 6)]
     for prec in (0, 1):
         params = default_params(mode=MODE_PDLP, precision=prec, max_iters=50000, tol=1e-6)
