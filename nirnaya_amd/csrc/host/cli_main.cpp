@@ -30,6 +30,7 @@ void usage() {
         "  --set <name>=<value>      set any option by name\n"
         "  --write-lpb <path>        write the model as a binary LP (.nlpb) and exit\n"
         "  --write-mps <path>        write the model as MPS and exit\n"
+        "  --presolve-report         run validation and presolve only, print what they removed\n"
         "  --version                 print the version\n");
 }
 
@@ -42,6 +43,7 @@ int main(int argc, char** argv) {
     }
     nir::Options opt;
     std::string path, lpb_out, mps_out;
+    bool presolve_report = false;
     try {
         for (int i = 1; i < argc; ++i) {
             std::string arg = argv[i];
@@ -77,6 +79,8 @@ int main(int argc, char** argv) {
                 opt.scale = false;
             } else if (arg == "--deterministic") {
                 opt.deterministic = true;
+            } else if (arg == "--presolve-report") {
+                presolve_report = true;
             } else if (arg == "--write-lpb" && value(v)) {
                 lpb_out = v;
             } else if (arg == "--write-mps" && value(v)) {
@@ -149,6 +153,21 @@ int main(int argc, char** argv) {
         std::printf("Wrote %s%s%s (%d rows, %d columns, %lld nonzeros)\n", lpb_out.c_str(),
                     (!lpb_out.empty() && !mps_out.empty()) ? " and " : "", mps_out.c_str(), model.nrow, model.ncol,
                     static_cast<long long>(model.A.nnz()));
+        return 0;
+    }
+    if (presolve_report) {
+        // The facade's steps before the engine (solver.hpp:156-187 order):
+        // validate, internal min form, presolve.
+        std::string msg;
+        const nir::Status vs = model.validate(msg);
+        if (vs != nir::Status::Optimal) {
+            std::printf("validation: %s (%s)\n", nir::status_name(vs), msg.c_str());
+            return nir::status_exit_code(vs);
+        }
+        const nir::Presolved pre = nir::presolve(model.to_internal_min(), opt.presolve, opt.presolve_rounds);
+        std::printf("presolve level %d: rows %d -> %d, cols %d -> %d, removed %d rows, %d cols, %d tightenings%s\n",
+                    opt.presolve, model.nrow, pre.reduced.nrow, model.ncol, pre.reduced.ncol, pre.rows_removed,
+                    pre.cols_removed, pre.tightenings, pre.solved ? ", solved" : "");
         return 0;
     }
     nir::Solver solver(opt);

@@ -39,6 +39,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <string>
 
 #include "prep.hpp"
 
@@ -332,6 +333,13 @@ Status read_lpb(const std::string& path, Model& out, std::string& err) {
         m.row_names.resize(static_cast<size_t>(m.nrow));
         for (Idx j = 0; ok && j < m.ncol; ++j) ok = c.str(m.col_names[j], err);
         for (Idx i = 0; ok && i < m.nrow; ++i) ok = c.str(m.row_names[i], err);
+    }
+    if (ok && !(h.flags & kNames)) {
+        // Every model carries names (the builder's defaults, model.cpp: "C<j>", "R<i>").
+        m.col_names.reserve(static_cast<size_t>(m.ncol));
+        m.row_names.reserve(static_cast<size_t>(m.nrow));
+        for (Idx j = 0; j < m.ncol; ++j) m.col_names.push_back("C" + std::to_string(j));
+        for (Idx i = 0; i < m.nrow; ++i) m.row_names.push_back("R" + std::to_string(i));
     }
     if (ok && c.at() != size) {
         err = "trailing bytes after the last section";
