@@ -16,8 +16,9 @@ it is then a lower bound), and the reference's own CPU loop
 (oracle/_ref, compiled from the reference sources) timed on a bounded sample
 of the same instance.
 
-Other workloads: --workload planning512 (512 LPs in the batched persistent
-engine) and --workload warmstart (learned warm start on a parametric family).
+Other workloads: --workload mixed (mixed precision on the same instance),
+--workload planning512 (512 LPs in the batched persistent engine) and
+--workload warmstart (learned warm start on a parametric family).
 """
 import argparse
 import json
@@ -213,6 +214,82 @@ def planning512(args):
     print(json.dumps(line), flush=True)
 
 
+def mixed_workload(args):
+    """Mixed precision on the 400k x 1.2M instance: the restarted engine's FP32
+    phase (FP32 values, iterates and accumulation in the iteration kernels;
+    FP64 evaluations, restarts, primal weight and objective). `value` is its
+    iteration rate over `steps` iterations of the production path (graph
+    replays, evaluation blocks included); the roofline is the FP32 A xbar
+    kernel's, from the events sweep. Then both precisions are solved to
+    relative KKT 1e-6 and the mixed solution's error against the FP64 one is
+    reported (the FP64 engine's trajectory is the oracle's, bit for bit, in
+    the parity tests)."""
+    import numpy as np
+    from nirnaya_amd.engine import Engine, MODE_PDLP, PREC_MIXED, default_params, gen_mixed_lp
+    t0 = time.time()
+    lp = gen_mixed_lp(SEED, 400_000, 1_200_000, 6.0, 0.01, 1e3)
+    log(f"[bench] instance m={lp.m} n={lp.n} nnz={lp.nnz} generated in {time.time() - t0:.1f}s")
+    eng = Engine(0)
+    # tol far below reach: the timed iterations stay in the FP32 phase
+    p = default_params(mode=MODE_PDLP, precision=PREC_MIXED, max_iters=10 ** 9, tol=1e-14, tile_nnz=-1)
+    sess = eng.session(lp, p)
+    shape = sess.shape()
+    log(f"[bench] mixed session ready in {sess.result().info.setup_seconds:.2f}s; shape A x: tile {shape[0]} "
+        f"segments {shape[2]}, A^T y: tile {shape[4]} segments {shape[6]}")
+    sess.run(args.warmup)
+    t0 = time.perf_counter()
+    sess.run(args.steps)
+    wall = time.perf_counter() - t0
+    t = sess.time(args.steps)
+    sess.close()
+    it_s = args.steps / wall
+    spmv_avg_s = t.spmv_ms / t.iterations / 1e3
+    spmtv_avg_s = t.spmtv_ms / t.iterations / 1e3
+    achieved = t.spmv_bytes / spmv_avg_s / 1e9
+    log(f"[bench] FP32 phase: {it_s:.0f} it/s; A xbar {spmv_avg_s * 1e6:.1f} us ({achieved:.0f} GB/s), "
+        f"A^T y {spmtv_avg_s * 1e6:.1f} us")
+    sol = {}
+    ttt = {}
+    for name, prec in (() if args.no_ttt else (("fp64", 0), ("mixed", PREC_MIXED))):
+        q = default_params(mode=MODE_PDLP, precision=prec, max_iters=2_000_000, tol=1e-6)
+        t0 = time.perf_counter()
+        r = eng.solve(lp, q)
+        sol[name] = r
+        ttt[name] = {"status": r.status, "iterations": r.iterations, "solve_s": round(r.info.solve_seconds, 3),
+                     "total_s": round(time.perf_counter() - t0, 3),
+                     "rel_kkt": [r.info.rel_primal, r.info.rel_dual, r.info.rel_gap], "objective": r.objective}
+        log(f"[bench] to 1e-6 ({name}): {ttt[name]}")
+    eng.close()
+    err = None
+    if sol:
+        a, b = sol["mixed"], sol["fp64"]
+        err = {"x_rel_l2": float(np.linalg.norm(a.x - b.x) / max(np.linalg.norm(b.x), 1e-300)),
+               "y_rel_l2": float(np.linalg.norm(a.y - b.y) / max(np.linalg.norm(b.y), 1e-300)),
+               "objective_rel": abs(a.objective - b.objective) / (1.0 + abs(b.objective))}
+        ttt["speedup_mixed_vs_fp64"] = round(ttt["fp64"]["solve_s"] / ttt["mixed"]["solve_s"], 3)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(lp)
+    line = {
+        "metric": METRIC, "value": round(it_s, 2), "unit": "PDHG iterations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (iterations) + f64 (evaluations)",
+        "data": "synthetic",
+        "config": {"workload": "mixed precision on the single large LP: m=400000, n=1200000, "
+                               f"nnz={lp.nnz}, seed {SEED}, restarted PDHG FP32 phase",
+                   "m": lp.m, "n": lp.n, "nnz": lp.nnz},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_dual_pdlp<float> (FP32 A xbar + dual step); one launch = its segment passes",
+                     "bytes_per_launch": t.spmv_bytes, "avg_launch_us": spmv_avg_s * 1e6,
+                     "dispatches_per_launch": t.spmv_passes},
+        "kernels": {"spmtv_primal_avg_us": spmtv_avg_s * 1e6, "spmtv_bytes_per_launch": t.spmtv_bytes,
+                    "iteration_bytes": t.iter_bytes},
+        "time_to_1e-6": ttt or None,
+        "mixed_error_vs_fp64": err,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def warmstart_workload(args):
     """Learned warm start on a parametric family (fixed A; b, c move with
     theta in R^64): training solves by the batched engine, POD bases and the
@@ -332,9 +409,10 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the 256 MiB copy kernel (reference for rocprofv3 byte counters)")
-    ap.add_argument("--workload", default="large-lp", choices=["large-lp", "planning512", "warmstart"],
-                    help="large-lp: the headline line (default); planning512: the batched workload; "
-                         "warmstart: the learned warm start on a parametric family")
+    ap.add_argument("--workload", default="large-lp", choices=["large-lp", "mixed", "planning512", "warmstart"],
+                    help="large-lp: the headline line (default); mixed: mixed precision on the same instance; "
+                         "planning512: the batched workload; warmstart: the learned warm start on a parametric "
+                         "family")
     ap.add_argument("--ws-m", type=int, default=5000)
     ap.add_argument("--ws-n", type=int, default=12000)
     ap.add_argument("--ws-train", type=int, default=2048)
@@ -350,6 +428,9 @@ def main():
         return
     if args.workload == "warmstart":
         warmstart_workload(args)
+        return
+    if args.workload == "mixed":
+        mixed_workload(args)
         return
 
     from nirnaya_amd.engine import Engine, MODE_REFERENCE, default_params, gen_mixed_lp
