@@ -16,7 +16,9 @@ it is then a lower bound), and the reference's own CPU loop
 (oracle/_ref, compiled from the reference sources) timed on a bounded sample
 of the same instance.
 
-Other workloads: --workload mixed (mixed precision on the same instance),
+Other workloads: --workload small-lp (the reference's CPU-runnable 2,000 x
+5,000 case, GPU and CPU times to 1e-6), --workload mixed (mixed precision on
+the same instance),
 --workload planning512 (512 LPs in the batched persistent engine) and
 --workload warmstart (learned warm start on a parametric family).
 """
@@ -210,6 +212,97 @@ def planning512(args):
         "one_at_a_time_fp64_s": round(one, 3),
         "one_at_a_time_sample": f"{len(samp)} instances (every 32nd), {ts:.2f}s for {si} iterations",
         "speedup_vs_one_at_a_time": round(one / out["fp64"]["total_s"], 2),
+    }
+    print(json.dumps(line), flush=True)
+
+
+def small_lp_workload(args):
+    """The reference's CPU-runnable case: the 2,000 x 5,000 equality LP (5
+    nonzeros per column at uniform rows, N(0,1) values, seed 20260), time to
+    relative KKT 1e-6.
+    GPU: plain FP64 GPU PDHG - the engine in FP64, one instance, cold start,
+    with its restarts (the baseline the metric names) -, the same in mixed
+    precision, and the unrestarted reference form (its relative KKT evaluated
+    on the device every 64 iterations, timer paused, capped at 50,000
+    iterations, the reference's default limit).
+    CPU (the GPU box's host, one thread): the reference loop's restatement in
+    the oracle run the same way (KKT every 64 iterations, timer paused, cap
+    50,000), the restarted FP64 oracle, and the reference's own loop
+    (oracle/_ref) iterations/s at threads=1 and at the host's cores."""
+    import numpy as np
+    from nirnaya_amd.engine import Engine, MODE_PDLP, MODE_REFERENCE, PREC_MIXED, default_params, gen_equality_lp
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle, Reference, have_ref
+    lp = gen_equality_lp(20260, 2000, 5000, 5)
+    tol, cap, chunk = 1e-6, 50_000, 64
+    eng = Engine(0)
+    gpu = {}
+    for name, prec in (("fp64", 0), ("mixed", PREC_MIXED)):
+        q = default_params(mode=MODE_PDLP, precision=prec, max_iters=2_000_000, tol=tol)
+        eng.solve(lp, q)   # warm the code paths
+        t0 = time.perf_counter()
+        r = eng.solve(lp, q)
+        gpu[name] = {"status": r.status, "iterations": r.iterations, "solve_s": round(r.info.solve_seconds, 5),
+                     "total_s": round(time.perf_counter() - t0, 5),
+                     "rel_kkt": [r.info.rel_primal, r.info.rel_dual, r.info.rel_gap]}
+        log(f"[bench] GPU {name} to 1e-6: {gpu[name]}")
+    # reference form on the GPU, KKT on the device between chunks
+    s = eng.session(lp, default_params(mode=MODE_REFERENCE, max_iters=10 ** 12, check_termination=0))
+    solve, it, k = 0.0, 0, np.ones(5)
+    while it < cap and max(k[:3]) > tol:
+        step = min(chunk, cap - it)
+        t0 = time.perf_counter()
+        s.run(step)
+        solve += time.perf_counter() - t0
+        it += step
+        k = s.kkt()
+    s.close()
+    eng.close()
+    gpu["reference_form"] = {"reached": bool(max(k[:3]) <= tol), "iterations": it, "solve_s": round(solve, 4),
+                             "rel_kkt": [float(v) for v in k[:3]]}
+    log(f"[bench] GPU reference form: {gpu['reference_form']}")
+    # CPU: the reference loop's restatement, chunked with paused KKT checks
+    o = Oracle(lp)
+    x, y, tau, _, _ = o.pdhg(0, check=False)
+    solve, it, k = 0.0, 0, np.ones(5)
+    while it < cap and max(k[:3]) > tol:
+        step = min(chunk, cap - it)
+        t0 = time.perf_counter()
+        o.steps(step, tau, x, y)
+        solve += time.perf_counter() - t0
+        it += step
+        k = o.kkt(x, y)
+    cpu_ref_form = {"reached": bool(max(k[:3]) <= tol), "iterations": it, "solve_s": round(solve, 3),
+                    "rel_kkt": [float(v) for v in k[:3]], "kind": "port", "cores": 1}
+    log(f"[bench] CPU reference form (oracle): {cpu_ref_form}")
+    t0 = time.perf_counter()
+    _, _, info = o.pdlp(max_iters=2_000_000, tol=tol)
+    cpu_pdlp = {"status": info.status, "iterations": info.iterations, "total_s": round(time.perf_counter() - t0, 3),
+                "kind": "port", "cores": 1}
+    log(f"[bench] CPU restarted FP64 (oracle): {cpu_pdlp}")
+    cpu = None
+    if have_ref():
+        ref = Reference(lp)
+        nth = max(1, min(16, os.cpu_count() or 1))
+        rates = {}
+        for th in (1, nth):
+            _, it0, setup = ref.pdhg_timed(0.0, threads=th)
+            _, it1, total = ref.pdhg_timed(setup + 1.0, threads=th)
+            rates[th] = {"iterations": it1, "seconds": round(total - setup, 3),
+                         "value": it1 / max(total - setup, 1e-9)}
+        cpu = {"value": rates[1]["value"], "unit": "PDHG iterations/s", "cores": 1, "kind": "reference",
+               "sample": f"{rates[1]['iterations']} iterations of the reference loop in {rates[1]['seconds']} s "
+                         f"after setup, CPU {cpu_model()}",
+               "parallel": {"threads": nth, **rates[nth]},
+               "reference_form_time_to_1e-6": cpu_ref_form, "restarted_fp64_time_to_1e-6": cpu_pdlp}
+    line = {
+        "metric": "time to relative KKT 1e-6, 2,000 x 5,000 equality LP", "value": gpu["fp64"]["solve_s"],
+        "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "2,000 equality rows x 5,000 columns, 5 nonzeros per column at uniform rows, "
+                               "N(0,1) values, bounds [0,10], seed 20260", "m": lp.m, "n": lp.n, "nnz": lp.nnz},
+        "gpu": gpu,
+        "speedup_gpu_fp64_vs_cpu_restarted": round(cpu_pdlp["total_s"] / gpu["fp64"]["total_s"], 2),
+        "cpu_baseline": cpu or {"reference_form_time_to_1e-6": cpu_ref_form, "restarted_fp64_time_to_1e-6": cpu_pdlp},
     }
     print(json.dumps(line), flush=True)
 
@@ -409,8 +502,9 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the 256 MiB copy kernel (reference for rocprofv3 byte counters)")
-    ap.add_argument("--workload", default="large-lp", choices=["large-lp", "mixed", "planning512", "warmstart"],
-                    help="large-lp: the headline line (default); mixed: mixed precision on the same instance; "
+    ap.add_argument("--workload", default="large-lp", choices=["large-lp", "small-lp", "mixed", "planning512", "warmstart"],
+                    help="large-lp: the headline line (default); small-lp: the reference's CPU-runnable "
+                         "2,000 x 5,000 case; mixed: mixed precision on the large instance; "
                          "planning512: the batched workload; warmstart: the learned warm start on a parametric "
                          "family")
     ap.add_argument("--ws-m", type=int, default=5000)
@@ -431,6 +525,9 @@ def main():
         return
     if args.workload == "mixed":
         mixed_workload(args)
+        return
+    if args.workload == "small-lp":
+        small_lp_workload(args)
         return
 
     from nirnaya_amd.engine import Engine, MODE_REFERENCE, default_params, gen_mixed_lp

@@ -17,6 +17,7 @@
  *   or_pdhg_reference src/gpu/pdhg_solver.hpp:25-107  closed-form step sizes and first
  *                                                     step of unit_tests.cpp:316-331,
  *                                                     oracle/_ref iterates at K >= 5000
+ *   or_pdhg_steps    the loop of :59-101 in chunks    chunked run = uninterrupted run
  *   or_dd_verify     src/io/sol_io.hpp:25-106         KAT unit_tests.cpp:29-42,
  *                                                     README.md:84 (2.55e-14)
  *   or_pdlp          build-defined restarted PDHG     KKT = 0 at known optima,
@@ -58,6 +59,11 @@ int or_scale_geometric(const or_lp* lp, double* R, double* C);
  * remaining outputs may be NULL. */
 void or_pdhg_reference(const or_lp* lp, int64_t K, int check, const double* x0, const double* y0, double* x,
                        double* y, double* tau_out, int64_t* iters_out, double* viol_out);
+
+/* K iterations of the reference loop (no stop rule) from (x, y) in place,
+ * with the given step sizes: continuing a run in chunks gives the iterates of
+ * one uninterrupted run (the loop keeps no state beyond x and y). */
+void or_pdhg_steps(const or_lp* lp, int64_t K, double tau, double sigma, double* x, double* y);
 
 /* Double-double checker: returns 1 on pass. */
 int or_dd_verify(const or_lp* lp, const double* x, double claimed_obj, double tol, double* max_row_viol,

@@ -113,6 +113,44 @@ int or_scale_geometric(const or_lp* lp, double* Rout, double* Cout) {
 }
 
 /* pdhg_solver.hpp:25-107 */
+/* The loop of pdhg_solver.hpp:59-101 from (x, y): K iterations (fewer when
+ * check != 0 and the stop rule fires). Returns the last max row violation. */
+static double ref_iterations(const or_lp* lp, int64_t K, int check, double tau, double sigma, double* x, double* y,
+                             double* g, double* xbar, double* h, int64_t* iters_out) {
+    const int32_t m = lp->m, n = lp->n;
+    int64_t iters = 0;
+    double V = 0.0;
+    for (int64_t k = 0; k < K; ++k) {
+        iters = k + 1;
+        /* :70-78 primal step */
+        or_spmv_t(lp, y, g);
+        for (int32_t j = 0; j < n; ++j) {
+            double cand = x[j] - tau * (lp->c[j] - g[j]);
+            if (fin(lp->l[j])) cand = smax(lp->l[j], cand);
+            if (fin(lp->u[j])) cand = smin(lp->u[j], cand);
+            xbar[j] = 2.0 * cand - x[j];
+            x[j] = cand;
+        }
+        /* :82-95 dual step and max primal violation */
+        or_spmv(lp, xbar, h);
+        V = 0.0;
+        for (int32_t i = 0; i < m; ++i) {
+            double r = h[i] - y[i] / sigma;
+            if (fin(lp->rl[i])) r = smax(lp->rl[i], r);
+            if (fin(lp->ru[i])) r = smin(lp->ru[i], r);
+            y[i] += sigma * (r - h[i]);
+            double viol = 0.0;
+            if (fin(lp->rl[i])) viol = smax(viol, lp->rl[i] - h[i]);
+            if (fin(lp->ru[i])) viol = smax(viol, h[i] - lp->ru[i]);
+            V = smax(V, viol);
+        }
+        /* :97-100 stop rule */
+        if (check && k > 20 && (k & 31) == 0 && V <= 1e-6) break;
+    }
+    *iters_out = iters;
+    return V;
+}
+
 void or_pdhg_reference(const or_lp* lp, int64_t K, int check, const double* x0, const double* y0, double* x,
                        double* y, double* tau_out, int64_t* iters_out, double* viol_out) {
     const int32_t m = lp->m, n = lp->n;
@@ -149,39 +187,24 @@ void or_pdhg_reference(const or_lp* lp, int64_t K, int check, const double* x0, 
     for (int32_t i = 0; i < m; ++i) y[i] = y0 ? y0[i] : 0.0;
 
     int64_t iters = 0;
-    double V = 0.0;
-    for (int64_t k = 0; k < K; ++k) {
-        iters = k + 1;
-        /* :70-78 primal step */
-        or_spmv_t(lp, y, g);
-        for (int32_t j = 0; j < n; ++j) {
-            double cand = x[j] - tau * (lp->c[j] - g[j]);
-            if (fin(lp->l[j])) cand = smax(lp->l[j], cand);
-            if (fin(lp->u[j])) cand = smin(lp->u[j], cand);
-            xbar[j] = 2.0 * cand - x[j];
-            x[j] = cand;
-        }
-        /* :82-95 dual step and max primal violation */
-        or_spmv(lp, xbar, h);
-        V = 0.0;
-        for (int32_t i = 0; i < m; ++i) {
-            double r = h[i] - y[i] / sigma;
-            if (fin(lp->rl[i])) r = smax(lp->rl[i], r);
-            if (fin(lp->ru[i])) r = smin(lp->ru[i], r);
-            y[i] += sigma * (r - h[i]);
-            double viol = 0.0;
-            if (fin(lp->rl[i])) viol = smax(viol, lp->rl[i] - h[i]);
-            if (fin(lp->ru[i])) viol = smax(viol, h[i] - lp->ru[i]);
-            V = smax(V, viol);
-        }
-        /* :97-100 stop rule */
-        if (check && k > 20 && (k & 31) == 0 && V <= 1e-6) break;
-    }
+    const double V = ref_iterations(lp, K, check, tau, sigma, x, y, g, xbar, h, &iters);
     if (tau_out) *tau_out = tau;
     if (iters_out) *iters_out = iters;
     if (viol_out) *viol_out = V;
     free(u);
     free(v);
+    free(g);
+    free(xbar);
+    free(h);
+}
+
+void or_pdhg_steps(const or_lp* lp, int64_t K, double tau, double sigma, double* x, double* y) {
+    const int32_t m = lp->m, n = lp->n;
+    double* g = malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double* xbar = malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double* h = malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+    int64_t iters = 0;
+    (void)ref_iterations(lp, K, 0, tau, sigma, x, y, g, xbar, h, &iters);
     free(g);
     free(xbar);
     free(h);

@@ -58,6 +58,7 @@ def oracle():
         L.or_spmv_t.argtypes = [C.POINTER(OrLp), _dp, _dp]
         L.or_scale_geometric.argtypes = [C.POINTER(OrLp), _dp, _dp]
         L.or_pdhg_reference.argtypes = [C.POINTER(OrLp), C.c_int64, C.c_int, _dp, _dp, _dp, _dp, _dp, _i64p, _dp]
+        L.or_pdhg_steps.argtypes = [C.POINTER(OrLp), C.c_int64, C.c_double, C.c_double, _dp, _dp]
         L.or_dd_verify.argtypes = [C.POINTER(OrLp), _dp, C.c_double, C.c_double, _dp, _dp, _dp]
         L.or_kkt.argtypes = [C.POINTER(OrLp), _dp, _dp, _dp]
         L.or_pdlp.argtypes = [C.POINTER(OrLp), C.POINTER(OrPdlpParams), _dp, _dp, C.POINTER(OrPdlpInfo)]
@@ -125,6 +126,10 @@ class Oracle:
         self.L.or_pdhg_reference(C.byref(self.s), int(K), 1 if check else 0, _p(x0), _p(y0), _p(x), _p(y),
                                  C.byref(tau), C.byref(it), C.byref(V))
         return x, y, tau.value, it.value, V.value
+
+    def steps(self, K, tau, x, y):
+        """K reference iterations from (x, y) in place (step sizes tau = sigma)."""
+        self.L.or_pdhg_steps(C.byref(self.s), int(K), float(tau), float(tau), _p(x), _p(y))
 
     def pdlp(self, max_iters=100000, tol=1e-6, eval_freq=64, unscale_row=None, unscale_col=None, x0=None, y0=None):
         keep = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (unscale_row, unscale_col, x0, y0)]

@@ -176,3 +176,16 @@ def test_golden_reference_iterates():
     x, y, _, it, _ = Oracle(lp).pdhg(5000, check=True)
     assert it == int(g["iters"])
     assert np.array_equal(x, g["x"]) and np.array_equal(y, g["y"])
+
+
+def test_chunked_reference_steps_equal_one_run():
+    # The reference loop keeps no state beyond (x, y): 64-iteration chunks
+    # from the start point give the uninterrupted run's iterates bit for bit.
+    lp = gen_equality_lp(20260, 2000, 5000, 5)
+    o = Oracle(lp)
+    x1, y1, tau, _, _ = o.pdhg(640, check=False)
+    x, y, tau0, _, _ = o.pdhg(0, check=False)
+    assert tau0 == tau
+    for _ in range(10):
+        o.steps(64, tau, x, y)
+    assert np.array_equal(x, x1) and np.array_equal(y, y1)
