@@ -7,6 +7,14 @@
 // barriers between the primal and the dual half (an instance of the
 // planning family fits comfortably in one CU's share of L2).
 //
+// Operand staging: the products gather y (for A^T y) and xbar (for A xbar)
+// at random. When the largest instance's max(m, n) FP64 entries fit in one
+// workgroup's LDS (up to 152 KB on gfx950), the 1024-lane kernel keeps the
+// operand being gathered in LDS - y on entry, xbar after the primal half, y
+// again after the dual half, each copied in by the lanes that wrote it - so
+// the random reads hit LDS instead of the vector memory path; the value and
+// index streams stay coalesced global loads.
+//
 // Work queue: a list of unfinished instances, largest first. A launch of the
 // persistent kernel hands them out by an atomic ticket; the workgroup runs up
 // to `quantum` evaluation blocks (eval_freq iterations + FP64 evaluation +
@@ -91,6 +99,7 @@ struct View {
     double tol, polish_tol;
     double omega0;     // initial primal weight (0: from the norms)
     long long max_iters;
+    int stage_bytes;   // dynamic LDS of the launch for staged operands (0: none)
     const double* x0;  // warm starts (concatenated like the vectors), with
     const double* y0;  // per-instance flags; null when no instance has one
     const int* has_x0;
@@ -252,6 +261,48 @@ __device__ void wg_iteration(const Local<T>& L, const Inst& I, const PdlpState& 
     __syncthreads();
 }
 
+// The launch's dynamic LDS, where an instance's gathered operand is staged.
+template <typename T>
+__device__ __forceinline__ T* stage_base() {
+    extern __shared__ double batch_stage[];
+    return reinterpret_cast<T*>(batch_stage);
+}
+
+// The same iteration with the gathered operands in LDS: y (m entries) for
+// A^T y, then xbar (n entries) for A xbar, each copied in by the lanes that
+// wrote it. The products read the same values in the same order, so the
+// iterates are bit-identical to the unstaged form; the random accesses go to
+// LDS instead of the vector memory path. `stage` holds y on entry and on
+// return.
+template <typename T, int BT>
+__device__ __forceinline__ void wg_iteration_staged(const Local<T>& L, const Inst& I, const PdlpState& s, int k_lo,
+                                                    int save, T* stage) {
+    const Halpern<T> hw = halpern<T>(s.k_base + k_lo);
+    const T tau = static_cast<T>(s.tau);
+    constexpr bool kDelta = std::is_same<T, float>::value;
+    for (int j = threadIdx.x; j < I.n; j += BT) {
+        const T g = row_dot<T>(L.trow, L.tcol, L.tval, stage, j);
+        const PrimalOut<T> r = primal_point<T>(L.x[j], g, L.c[j], L.lo[j], L.up[j], kDelta ? T(0) : L.xa[j], tau, hw);
+        L.xbar[j] = r.xb;
+        L.x[j] = r.next;
+        if (save) L.xp[j] = r.cand;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < I.n; j += BT) stage[j] = L.xbar[j];
+    __syncthreads();
+    const T sigma = static_cast<T>(s.sigma);
+    for (int i = threadIdx.x; i < I.m; i += BT) {
+        const T h = row_dot<T>(L.arow, L.acol, L.aval, stage, i);
+        const DualOut<T> r = kDelta ? dual_point_delta<T>(L.y[i], h, L.rl[i], L.ru[i], L.ya[i], sigma, hw)
+                                    : dual_point<T>(L.y[i], h, L.rl[i], L.ru[i], L.ya[i], sigma, hw);
+        L.y[i] = r.next;
+        if (save) L.yp[i] = r.yp;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < I.m; i += BT) stage[i] = L.y[i];
+    __syncthreads();
+}
+
 // Restart: iterate and anchor become T(z).
 template <typename T, int BT>
 __device__ __forceinline__ void wg_restart_copy(const Local<T>& L, const Inst& I) {
@@ -397,10 +448,20 @@ __device__ int wg_certificate(const View& V, const Inst& I, Shared<BT>& sh) {
 }
 
 // One evaluation block: eval_freq iterations, evaluation, restart copy.
-template <typename T, int BT>
+// STAGED: the launch stages every instance's gathered operands in LDS (the
+// host launches it only when the largest instance fits).
+template <typename T, int BT, bool STAGED>
 __device__ void wg_block(const View& V, const Inst& I, const Vec<T>& v, Shared<BT>& sh) {
     const Local<T> L = local(V, v, I);
-    for (int kl = 0; kl < V.eval_freq; ++kl) wg_iteration<T, BT>(L, I, sh.s, kl, kl == V.eval_freq - 1 ? 1 : 0);
+    if (STAGED) {
+        T* stage = stage_base<T>();
+        for (int i = threadIdx.x; i < I.m; i += BT) stage[i] = L.y[i];
+        __syncthreads();
+        for (int kl = 0; kl < V.eval_freq; ++kl)
+            wg_iteration_staged<T, BT>(L, I, sh.s, kl, kl == V.eval_freq - 1 ? 1 : 0, stage);
+    } else {
+        for (int kl = 0; kl < V.eval_freq; ++kl) wg_iteration<T, BT>(L, I, sh.s, kl, kl == V.eval_freq - 1 ? 1 : 0);
+    }
     wg_eval<T, BT>(V, I, v, sh, 0);
     // Decisions into registers, then a barrier: every thread branches on the
     // same values although thread 0 rewrites the shared state afterwards.
@@ -439,7 +500,7 @@ __device__ __forceinline__ int wg_pop(const View& V, Shared<BT>& sh) {
 
 // 4 waves per SIMD: the register budget of the iteration loop (the rare
 // evaluation code spills a few dwords).
-template <int BT>
+template <int BT, bool STAGED>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_batch_run(View V) {
     __shared__ Shared<BT> sh;
     for (;;) {
@@ -455,8 +516,8 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4, 8))) void
             const int finished = sh.ctl.finished, fp32 = sh.ctl.fp32;
             __syncthreads();
             if (finished) break;
-            if (fp32) wg_block<float, BT>(V, I, V.f, sh);
-            else wg_block<double, BT>(V, I, V.d, sh);
+            if (fp32) wg_block<float, BT, STAGED>(V, I, V.f, sh);
+            else wg_block<double, BT, STAGED>(V, I, V.d, sh);
         }
         if (threadIdx.x == 0) {
             V.st[i] = sh.s;
@@ -755,6 +816,7 @@ struct nir_batch {
     int* h_poll = nullptr;   // pinned: list length after the last graph replay (two slots)
     hipEvent_t poll_ev[2] = {nullptr, nullptr};
     int threads = 512, workgroups = 0;
+    int stage_bytes = 0;   // dynamic LDS of the 1024-lane run kernel (largest staged operand)
     bool solved = false, stopped = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -811,11 +873,18 @@ struct nir_batch {
 
 namespace {
 
-void launch_run(nir_batch* B, const View& V, int threads) {
+void launch_run(nir_batch* B, const View& V0, int threads) {
+    // Operand staging in LDS with the 1024-lane workgroups (one per CU),
+    // when the largest instance fits.
+    View V = V0;
+    V.stage_bytes = threads == 1024 ? B->stage_bytes : 0;
     switch (threads) {
-        case 256: k_batch_run<256><<<B->workgroups, 256, 0, B->s>>>(V); break;
-        case 1024: k_batch_run<1024><<<B->workgroups, 1024, 0, B->s>>>(V); break;
-        default: k_batch_run<512><<<B->workgroups, 512, 0, B->s>>>(V); break;
+        case 256: k_batch_run<256, false><<<B->workgroups, 256, 0, B->s>>>(V); break;
+        case 1024:
+            if (V.stage_bytes > 0) k_batch_run<1024, true><<<B->workgroups, 1024, V.stage_bytes, B->s>>>(V);
+            else k_batch_run<1024, false><<<B->workgroups, 1024, 0, B->s>>>(V);
+            break;
+        default: k_batch_run<512, false><<<B->workgroups, 512, 0, B->s>>>(V); break;
     }
     k_batch_compact<<<1, 64, 0, B->s>>>(V);
     HIP_CHECK(hipGetLastError());
@@ -1093,6 +1162,22 @@ NIR_API int nir_batch_create_warm(nir_ctx* ctx, const nir_lp* const* lps, int32_
         k_batch_compact<<<1, 64, 0, s>>>(V);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(s));
+        // LDS for the staged operands of the 1024-lane kernel: the largest
+        // max(m, n) in FP64, when it fits beside the workgroup's static
+        // scratch; otherwise no staging and every instance gathers from
+        // global memory.
+        {
+            constexpr long long kStageCap = 152 * 1024;
+            long long need = 0;
+            for (const Inst& I : B->h_inst) need = std::max<long long>(need, std::max(I.m, I.n));
+            B->stage_bytes = need * 8 <= kStageCap ? static_cast<int>(std::max<long long>(need * 8, 8)) : 0;
+            if (B->stage_bytes > 0 &&
+                hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_run<1024, true>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, B->stage_bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                B->stage_bytes = 0;
+            }
+        }
         B->threads = B->opts.threads > 0 ? B->opts.threads : autotune_threads(B.get(), cu);
         if (B->opts.workgroups <= 0) B->workgroups = std::max(1, std::min(count, cu * (2048 / B->threads)));
         {
