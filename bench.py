@@ -324,6 +324,9 @@ def mixed_workload(args):
     lp = gen_mixed_lp(SEED, 400_000, 1_200_000, 6.0, 0.01, 1e3)
     log(f"[bench] instance m={lp.m} n={lp.n} nnz={lp.nnz} generated in {time.time() - t0:.1f}s")
     eng = Engine(0)
+    if args.calibrate:
+        ms = eng.calibrate_copy(256 << 20)
+        log(f"[bench] calibration copy of 256 MiB: {ms:.3f} ms")
     # tol far below reach: the timed iterations stay in the FP32 phase
     p = default_params(mode=MODE_PDLP, precision=PREC_MIXED, max_iters=10 ** 9, tol=1e-14, tile_nnz=-1)
     sess = eng.session(lp, p)
@@ -362,6 +365,13 @@ def mixed_workload(args):
                "objective_rel": abs(a.objective - b.objective) / (1.0 + abs(b.objective))}
         ttt["speedup_mixed_vs_fp64"] = round(ttt["fp64"]["solve_s"] / ttt["mixed"]["solve_s"], 3)
     cpu = None if args.no_cpu_baseline else cpu_baseline(lp)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "spmv_cfg2_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            for e in json.load(f).get("layouts", []):
+                if e.get("tile") == shape[0] and e.get("segments") == shape[2]:
+                    traffic = e.get("hbm_bytes_per_launch")
     line = {
         "metric": METRIC, "value": round(it_s, 2), "unit": "PDHG iterations/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
@@ -371,7 +381,7 @@ def mixed_workload(args):
                                f"nnz={lp.nnz}, seed {SEED}, restarted PDHG FP32 phase",
                    "m": lp.m, "n": lp.n, "nnz": lp.nnz},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_dual_pdlp<float> (FP32 A xbar + dual step); one launch = its segment passes",
                      "bytes_per_launch": t.spmv_bytes, "avg_launch_us": spmv_avg_s * 1e6,
                      "dispatches_per_launch": t.spmv_passes},
