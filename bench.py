@@ -226,7 +226,7 @@ def small_lp_workload(args):
     precision, and the unrestarted reference form (its relative KKT evaluated
     on the device every 64 iterations, timer paused, capped at 50,000
     iterations, the reference's default limit).
-    CPU (the GPU box's host, one thread): the reference loop's restatement in
+    CPU (the host of the GPU, one thread): the reference loop's restatement in
     the oracle run the same way (KKT every 64 iterations, timer paused, cap
     50,000), the restarted FP64 oracle, and the reference's own loop
     (oracle/_ref) iterations/s at threads=1 and at the host's cores."""
@@ -328,7 +328,8 @@ def mixed_workload(args):
         ms = eng.calibrate_copy(256 << 20)
         log(f"[bench] calibration copy of 256 MiB: {ms:.3f} ms")
     # tol far below reach: the timed iterations stay in the FP32 phase
-    p = default_params(mode=MODE_PDLP, precision=PREC_MIXED, max_iters=10 ** 9, tol=1e-14, tile_nnz=-1)
+    p = default_params(mode=MODE_PDLP, precision=PREC_MIXED, max_iters=10 ** 9, tol=1e-14, tile_nnz=args.tile,
+                       tile_rows=args.tile_rows, seg_width=args.seg, seg_width_t=args.seg_t)
     sess = eng.session(lp, p)
     shape = sess.shape()
     log(f"[bench] mixed session ready in {sess.result().info.setup_seconds:.2f}s; shape A x: tile {shape[0]} "

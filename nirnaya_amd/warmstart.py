@@ -155,8 +155,8 @@ def warm_params(params, primal_weight: float):
 
 
 def solve_warm(engine: Engine, lp: LP, params, x0, y0, primal_weight: float = 0.0):
-    """Warm-started solve with cold start (the caller's parameters) as the
-    fallback and the judge."""
+    """Warm-started solve; if it misses the tolerance, the cold solve (the
+    caller's parameters) runs and its result is returned."""
     import time
     t0 = time.perf_counter()
     r = engine.solve(lp, warm_params(params, primal_weight), x0=x0, y0=y0)

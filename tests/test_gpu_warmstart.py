@@ -4,7 +4,7 @@ Training solves come from the batched engine; the POD bases and the MLP are
 fitted on the device; held-out parameters are solved cold and warm-started
 (predicted point + the primal weight the training solves ended with). Every
 warm solve must reach the tolerance (cold start is the fallback and the
-judge).
+reference point for the iterations saved).
 
 What the network gains depends on how predictable the family's solutions are
 from theta. On a family whose parameters move b and c by 0.1 % / 0.05 % the
