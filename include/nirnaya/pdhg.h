@@ -180,7 +180,9 @@ typedef struct nir_batch nir_batch;
 typedef struct {
     int32_t threads;          /* workgroup size of the persistent kernel: 256, 512 or 1024;
                                  0 = autotune (timed on the batch after setup) */
-    int32_t quantum;          /* evaluation blocks an instance runs per work-list ticket (0 = 16) */
+    int32_t quantum;          /* evaluation blocks an instance runs per work-list ticket (0 = 1024:
+                                 most instances finish on their first ticket; should_stop is
+                                 polled between launches, so a smaller quantum stops sooner) */
     int32_t rounds_per_graph; /* persistent launches captured per hipGraph replay (0 = 4) */
     int32_t workgroups;       /* persistent grid (0 = what fits on the device) */
 } nir_batch_opts;

@@ -63,6 +63,13 @@ struct Inst {
     long long yo;     // row vectors
 };
 
+// Evaluation blocks per work-list ticket by default: large enough that most
+// instances run to completion on their first ticket, so the largest-first
+// list schedules whole solves longest first (time-slicing instances with
+// short tickets measured 30 % slower on 512 planning LPs: 2.81 s at 16 blocks,
+// 1.96 s at 4096).
+constexpr int kDefaultQuantum = 1024;
+
 // Per-instance control beyond the iteration state.
 struct Ctl {
     int fp32;          // 1 while the instance iterates in FP32 (mixed mode)
@@ -851,7 +858,7 @@ struct nir_batch {
         V.launches = launches.p;
         V.mixed = prm.precision == NIR_PREC_MIXED ? 1 : 0;
         V.eval_freq = std::max(1, prm.eval_freq);
-        V.quantum = opts.quantum > 0 ? opts.quantum : 16;
+        V.quantum = opts.quantum > 0 ? opts.quantum : kDefaultQuantum;
         V.tol = prm.tol > 0 ? prm.tol : 1e-6;
         V.polish_tol = prm.polish_tol;
         V.omega0 = prm.primal_weight > 0 ? prm.primal_weight : 0.0;
@@ -1315,7 +1322,7 @@ NIR_API int nir_batch_get_stats(nir_batch* B, nir_batch_stats* out) {
         out->count = B->count;
         out->threads = B->threads;
         out->workgroups = B->workgroups;
-        out->quantum = B->opts.quantum > 0 ? B->opts.quantum : 16;
+        out->quantum = B->opts.quantum > 0 ? B->opts.quantum : kDefaultQuantum;
         long long launches = 0;
         HIP_CHECK(hipMemcpy(&launches, B->launches.p, sizeof(launches), hipMemcpyDeviceToHost));
         out->launches = launches;
