@@ -132,6 +132,7 @@ bool Options::set_by_name(const std::string& key, const std::string& val) {
         }
         if (key == "pdhg_tol") { pdhg.tol = parse_f64(val); return true; }
         if (key == "pdhg_polish_tol") { pdhg.polish_tol = parse_f64(val); return true; }
+        if (key == "pdhg_host_scaling") { pdhg.host_scaling = parse_i32(val); return true; }
         if (key == "pdhg_max_iters") { pdhg.max_iters = parse_i64(val); return true; }
         if (key == "pdhg_eval_freq") {
             int f = parse_i32(val);

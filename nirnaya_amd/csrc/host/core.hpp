@@ -75,6 +75,9 @@ struct PdhgOptions {
     int device = 0;
     int use_graph = 1;                // capture iteration blocks in hipGraphs
     int verbose = 0;
+    // pdlp mode: also apply the reference's geometric pre-scaling before the
+    // engine's own Ruiz + Pock-Chambolle scaling (0 = engine scaling only).
+    int host_scaling = 0;
 };
 
 struct Options {

@@ -103,3 +103,23 @@ def test_checker_certifies_returned_solution(tmp_path):
     else:
         # No incumbent: the reference pipeline reports NumericalTrouble (5) or a limit.
         assert rc in (1, 5)
+
+
+def test_facade_feasibility_polish_makes_the_certificate_pass(tmp_path):
+    # A loose relative KKT tolerance (1e-4, l2 over all rows) leaves single
+    # rows violated beyond the certificate's 1e-6; the pdlp path then polishes
+    # primal feasibility from the engine's point and certifies that point.
+    import re
+    import subprocess
+    from nirnaya_amd.engine import gen_mixed_lp, write_lp_binary
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    path = str(tmp_path / "m.nlpb")
+    write_lp_binary(gen_mixed_lp(31, 20000, 60000, 6.0, 0.01, 1e3), path)
+    r = subprocess.run([os.path.join(root, "bin", "sovsolve"), path, "--method=pdhg", "--pdhg-mode=pdlp", "--tol=1e-4",
+                        "--log", "2"], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Feasibility polish" in r.stdout
+    assert re.search(r"PASS=YES", r.stdout)
+    m = re.search(r"max row violation ([0-9.e+-]+) -> ([0-9.e+-]+)", r.stdout)
+    assert m and float(m.group(1)) > 1e-6 >= float(m.group(2))
