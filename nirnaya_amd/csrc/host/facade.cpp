@@ -257,8 +257,9 @@ SolveResult Solver::solve(const Model& user_model) {
             // pdlp mode: a relative KKT tolerance (l2 norms over all rows and
             // columns) does not bound each row's violation, which the
             // certificate below checks against checker_tol. When it would
-            // fail, continue from the engine's point and primal weight with
-            // the tolerance tightened tenfold, at most twice.
+            // fail: a primal feasibility polish from the engine's point; if
+            // that fails too, continue from the point and primal weight with
+            // the tolerance tightened tenfold (at most twice).
             if (eo.pdhg.mode != "pdlp") break;
             std::vector<double> xo;
             pre.postsolve(x_red, xo);
