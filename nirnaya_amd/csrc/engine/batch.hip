@@ -83,12 +83,14 @@ template <typename T>
 struct Vec {
     T *x, *xbar, *xa, *xp, *y, *ya, *yp;
     T *c, *lo, *up, *rl, *ru;
-    T *aval, *tval;
+    T *aval, *tval;   // CSR values (FP64 only; scaled in place at setup)
+    T *sa, *st;       // the same values wave-sliced (see slice_dot)
 };
 
 struct View {
     const Inst* inst;
     const int *arow, *acol, *trow, *tcol;
+    int *sacol, *stcol;   // wave-sliced column indices of A and A^T (written at setup)
     Vec<double> d;
     Vec<float> f;
     double *R, *C, *hs;
@@ -117,7 +119,8 @@ struct View {
 template <typename T>
 struct Local {
     const int *arow, *acol, *trow, *tcol;
-    T *aval, *tval;
+    const int *sacol, *stcol;
+    T *aval, *tval, *sa, *st;
     T *x, *xbar, *xa, *xp, *y, *ya, *yp;
     T *c, *lo, *up, *rl, *ru;
 };
@@ -129,8 +132,12 @@ __device__ __forceinline__ Local<T> local(const View& V, const Vec<T>& v, const 
     L.acol = V.acol + I.nz;
     L.trow = V.trow + I.trp;
     L.tcol = V.tcol + I.nz;
+    L.sacol = V.sacol + I.nz;
+    L.stcol = V.stcol + I.nz;
     L.aval = v.aval + I.nz;
     L.tval = v.tval + I.nz;
+    L.sa = v.sa + I.nz;
+    L.st = v.st + I.nz;
     L.x = v.x + I.xo; L.xbar = v.xbar + I.xo; L.xa = v.xa + I.xo; L.xp = v.xp + I.xo;
     L.c = v.c + I.xo; L.lo = v.lo + I.xo; L.up = v.up + I.xo;
     L.y = v.y + I.yo; L.ya = v.ya + I.yo; L.yp = v.yp + I.yo;
@@ -138,15 +145,93 @@ __device__ __forceinline__ Local<T> local(const View& V, const Vec<T>& v, const 
     return L;
 }
 
-// Sequential dot product of row r with v, ascending index order, products
-// rounded before the add: the sum order of the single-LP CSR-stream kernels.
+// Wave-sliced copies of the CSR matrices for the row loops. An instance's
+// rows are cut into slices of 64: the rows one wave takes in every row loop
+// of this file (row = threadIdx.x + k * BT, BT a multiple of 64). A slice
+// keeps the CSR range of its rows but orders it by position within the row
+// first: the first nonzeros of all its nonempty rows in row order, then the
+// second nonzeros of the rows longer than one, and so on. At step t a wave
+// loads one contiguous run of values and indices instead of 64 strided ones
+// (with one lane per CSR row, each lane's loads are a stream of their own),
+// and every row still adds its products in CSR order, one rounded product at
+// a time: the sequential row sums of the single-LP CSR-stream kernels, bit
+// for bit. A call must come from a full row loop: the wave's active lanes
+// are exactly the slice's rows below the row count, lane 0 holding the
+// slice's first row.
+__device__ __forceinline__ int lane_rank(unsigned long long mask) {
+    return static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(mask >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0u)));
+}
+
 template <typename T, typename V>
-__device__ __forceinline__ T row_dot(const int* __restrict__ rp, const int* __restrict__ col,
-                                     const T* __restrict__ val, const V* __restrict__ v, int r) {
+__device__ __forceinline__ T slice_dot(const int* __restrict__ rp, const int* __restrict__ scol,
+                                       const T* __restrict__ sval, const V* __restrict__ v, int r) {
+    const int start = rp[r];
+    const int len = rp[r + 1] - start;
+    int base = __builtin_amdgcn_readfirstlane(start);   // the slice's CSR start
     T s = T(0);
-    const int e = rp[r + 1];
-    for (int p = rp[r]; p < e; ++p) s += val[p] * static_cast<T>(v[col[p]]);
+    // U steps per round: one workgroup owns an instance, so the loop is
+    // bound by memory latency, not bandwidth; all U steps' loads and then
+    // all U gathers are issued before the first product is added. Eight in
+    // FP32 (most rows of the planning and staircase families take one
+    // round), four in FP64, where eight value and operand pairs cost more
+    // registers than the extra requests in flight gain. Lanes past their
+    // row's end load from `base` (in bounds: step t has an active lane) and
+    // keep their sum by a select.
+    constexpr int U = sizeof(T) == 4 ? 8 : 4;
+    for (int t = 0;; t += U) {
+        unsigned long long mk[U];
+        mk[0] = __ballot(t < len);
+        if (mk[0] == 0ull) break;
+#pragma unroll
+        for (int u = 1; u < U; ++u) mk[u] = __ballot(t + u < len);
+        int p[U];
+        int b = base;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            p[u] = t + u < len ? b + lane_rank(mk[u]) : base;
+            b += __popcll(mk[u]);
+        }
+        T a[U];
+        int c[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            a[u] = sval[p[u]];
+            c[u] = scol[p[u]];
+        }
+        T g[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) g[u] = static_cast<T>(v[c[u]]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const T su = s + a[u] * g[u];
+            s = t + u < len ? su : s;
+        }
+        base = b;
+    }
     return s;
+}
+
+// Writes the wave-sliced copy of a CSR matrix (rows x ?, row pointers rp):
+// the positions slice_dot reads, by the same traversal.
+template <int BT, typename T>
+__device__ void wg_slice(const int* __restrict__ rp, const int* __restrict__ col, const T* __restrict__ val,
+                         int rows, int* __restrict__ scol, T* __restrict__ sval) {
+    for (int r = threadIdx.x; r < rows; r += BT) {
+        const int start = rp[r];
+        const int len = rp[r + 1] - start;
+        int base = __builtin_amdgcn_readfirstlane(start);
+        for (int t = 0;; ++t) {
+            const unsigned long long mask = __ballot(t < len);
+            if (mask == 0ull) break;
+            if (t < len) {
+                const int p = base + lane_rank(mask);
+                scol[p] = col[start + t];
+                sval[p] = val[start + t];
+            }
+            base += __popcll(mask);
+        }
+    }
 }
 
 // dd sums of NQ per-thread accumulators over the workgroup (wave butterflies,
@@ -199,8 +284,8 @@ __device__ void wg_eval(const View& V, const Inst& I, const Vec<TI>& it, Shared<
     // double-double accumulators live together.
     double* gs = V.d.xbar + I.xo;
     double* hs = V.hs + I.yo;
-    for (int j = threadIdx.x; j < I.n; j += BT) gs[j] = row_dot<double>(D.trow, D.tcol, D.tval, yp, j);
-    for (int i = threadIdx.x; i < I.m; i += BT) hs[i] = row_dot<double>(D.arow, D.acol, D.aval, xp, i);
+    for (int j = threadIdx.x; j < I.n; j += BT) gs[j] = slice_dot<double>(D.trow, D.stcol, D.st, yp, j);
+    for (int i = threadIdx.x; i < I.m; i += BT) hs[i] = slice_dot<double>(D.arow, D.sacol, D.sa, xp, i);
     // FP32 iterates are deltas from the FP64 anchor: T(z) = anchor + delta,
     // A^T yp = GA + A^T dyp, A xp = HA + A dxp.
     constexpr bool kDelta = std::is_same<TI, float>::value;
@@ -250,8 +335,9 @@ __device__ void wg_iteration(const Local<T>& L, const Inst& I, const PdlpState& 
     const T tau = static_cast<T>(s.tau);
     constexpr bool kDelta = std::is_same<T, float>::value;   // FP32: re-centred form, zero anchor delta
     for (int j = threadIdx.x; j < I.n; j += BT) {
-        const T g = row_dot<T>(L.trow, L.tcol, L.tval, L.y, j);
-        const PrimalOut<T> r = primal_point<T>(L.x[j], g, L.c[j], L.lo[j], L.up[j], kDelta ? T(0) : L.xa[j], tau, hw);
+        const T xj = L.x[j], cj = L.c[j], lj = L.lo[j], uj = L.up[j], aj = kDelta ? T(0) : L.xa[j];
+        const T g = slice_dot<T>(L.trow, L.stcol, L.st, L.y, j);
+        const PrimalOut<T> r = primal_point<T>(xj, g, cj, lj, uj, aj, tau, hw);
         L.xbar[j] = r.xb;
         L.x[j] = r.next;
         if (save) L.xp[j] = r.cand;
@@ -259,9 +345,10 @@ __device__ void wg_iteration(const Local<T>& L, const Inst& I, const PdlpState& 
     __syncthreads();
     const T sigma = static_cast<T>(s.sigma);
     for (int i = threadIdx.x; i < I.m; i += BT) {
-        const T h = row_dot<T>(L.arow, L.acol, L.aval, L.xbar, i);
-        const DualOut<T> r = kDelta ? dual_point_delta<T>(L.y[i], h, L.rl[i], L.ru[i], L.ya[i], sigma, hw)
-                                    : dual_point<T>(L.y[i], h, L.rl[i], L.ru[i], L.ya[i], sigma, hw);
+        const T yi = L.y[i], rli = L.rl[i], rui = L.ru[i], ai = L.ya[i];
+        const T h = slice_dot<T>(L.arow, L.sacol, L.sa, L.xbar, i);
+        const DualOut<T> r = kDelta ? dual_point_delta<T>(yi, h, rli, rui, ai, sigma, hw)
+                                    : dual_point<T>(yi, h, rli, rui, ai, sigma, hw);
         L.y[i] = r.next;
         if (save) L.yp[i] = r.yp;
     }
@@ -288,8 +375,9 @@ __device__ __forceinline__ void wg_iteration_staged(const Local<T>& L, const Ins
     const T tau = static_cast<T>(s.tau);
     constexpr bool kDelta = std::is_same<T, float>::value;
     for (int j = threadIdx.x; j < I.n; j += BT) {
-        const T g = row_dot<T>(L.trow, L.tcol, L.tval, stage, j);
-        const PrimalOut<T> r = primal_point<T>(L.x[j], g, L.c[j], L.lo[j], L.up[j], kDelta ? T(0) : L.xa[j], tau, hw);
+        const T xj = L.x[j], cj = L.c[j], lj = L.lo[j], uj = L.up[j], aj = kDelta ? T(0) : L.xa[j];
+        const T g = slice_dot<T>(L.trow, L.stcol, L.st, stage, j);
+        const PrimalOut<T> r = primal_point<T>(xj, g, cj, lj, uj, aj, tau, hw);
         L.xbar[j] = r.xb;
         L.x[j] = r.next;
         if (save) L.xp[j] = r.cand;
@@ -299,9 +387,10 @@ __device__ __forceinline__ void wg_iteration_staged(const Local<T>& L, const Ins
     __syncthreads();
     const T sigma = static_cast<T>(s.sigma);
     for (int i = threadIdx.x; i < I.m; i += BT) {
-        const T h = row_dot<T>(L.arow, L.acol, L.aval, stage, i);
-        const DualOut<T> r = kDelta ? dual_point_delta<T>(L.y[i], h, L.rl[i], L.ru[i], L.ya[i], sigma, hw)
-                                    : dual_point<T>(L.y[i], h, L.rl[i], L.ru[i], L.ya[i], sigma, hw);
+        const T yi = L.y[i], rli = L.rl[i], rui = L.ru[i], ai = L.ya[i];
+        const T h = slice_dot<T>(L.arow, L.sacol, L.sa, stage, i);
+        const DualOut<T> r = kDelta ? dual_point_delta<T>(yi, h, rli, rui, ai, sigma, hw)
+                                    : dual_point<T>(yi, h, rli, rui, ai, sigma, hw);
         L.y[i] = r.next;
         if (save) L.yp[i] = r.yp;
     }
@@ -340,8 +429,8 @@ __device__ void wg_recenter(const View& V, const Inst& I, const PdlpState& s, in
         for (int i = threadIdx.x; i < I.m; i += BT) D.ya[i] = D.ya[i] + static_cast<double>(F.yp[i]);
         __syncthreads();
     }
-    for (int i = threadIdx.x; i < I.m; i += BT) HA[i] = row_dot<double>(D.arow, D.acol, D.aval, D.xa, i);
-    for (int j = threadIdx.x; j < I.n; j += BT) GA[j] = row_dot<double>(D.trow, D.tcol, D.tval, D.ya, j);
+    for (int i = threadIdx.x; i < I.m; i += BT) HA[i] = slice_dot<double>(D.arow, D.sacol, D.sa, D.xa, i);
+    for (int j = threadIdx.x; j < I.n; j += BT) GA[j] = slice_dot<double>(D.trow, D.stcol, D.st, D.ya, j);
     __syncthreads();
     const double sigma = s.sigma;
     for (int j = threadIdx.x; j < I.n; j += BT) {
@@ -616,6 +705,10 @@ __device__ void wg_setup(const View& V, int i, Shared<BT>& sh) {
                 for (int p = D.arow[r]; p < D.arow[r + 1]; ++p) aval[p] = aval[p] * (1.0 * fc[D.acol[p]]);
         __syncthreads();
     }
+    // Wave-sliced copies of the scaled matrices for the row loops.
+    wg_slice<BT>(D.arow, D.acol, aval, m, V.sacol + I.nz, D.sa);
+    wg_slice<BT>(D.trow, D.tcol, tval, n, V.stcol + I.nz, D.st);
+    __syncthreads();
 
     // Scaled vectors (absent bounds as +-inf) and the norms.
     const double* cost = V.cost + I.xo;
@@ -665,9 +758,9 @@ __device__ void wg_setup(const View& V, int i, Shared<BT>& sh) {
             __syncthreads();
         }
         for (int it = 0; it < 200; ++it) {
-            for (int r = tid; r < m; r += BT) v[r] = row_dot<double>(D.arow, D.acol, aval, u, r);
+            for (int r = tid; r < m; r += BT) v[r] = slice_dot<double>(D.arow, D.sacol, D.sa, u, r);
             __syncthreads();
-            for (int j = tid; j < n; j += BT) u[j] = row_dot<double>(D.trow, D.tcol, tval, v, j);
+            for (int j = tid; j < n; j += BT) u[j] = slice_dot<double>(D.trow, D.stcol, D.st, v, j);
             __syncthreads();
             const double nw = wg_norm<BT>(u, n, sh);
             if (!(nw > 0.0) || !isfinite(nw)) break;
@@ -723,8 +816,8 @@ __device__ void wg_setup(const View& V, int i, Shared<BT>& sh) {
     if (V.mixed) {
         const Local<float> F = local(V, V.f, I);
         for (int p = tid; p < I.nnz; p += BT) {
-            F.aval[p] = static_cast<float>(aval[p]);
-            F.tval[p] = static_cast<float>(tval[p]);
+            F.sa[p] = static_cast<float>(D.sa[p]);
+            F.st[p] = static_cast<float>(D.st[p]);
         }
         __syncthreads();
         // The anchor is the start point (D.xa / D.ya); re-base the FP32
@@ -778,15 +871,20 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4, 8))) void
 
 template <typename T>
 struct HostVec {
-    DevArray<T> x, xbar, xa, xp, y, ya, yp, c, lo, up, rl, ru, aval, tval;
-    void alloc(size_t nx, size_t ny, size_t nz) {
+    DevArray<T> x, xbar, xa, xp, y, ya, yp, c, lo, up, rl, ru, aval, tval, sa, st;
+    // csr: also the CSR value arrays (the FP64 set: scaling and certificates).
+    void alloc(size_t nx, size_t ny, size_t nz, bool csr) {
         for (auto* a : {&x, &xbar, &xa, &xp, &c, &lo, &up}) a->alloc(nx);
         for (auto* a : {&y, &ya, &yp, &rl, &ru}) a->alloc(ny);
-        aval.alloc(nz);
-        tval.alloc(nz);
+        if (csr) {
+            aval.alloc(nz);
+            tval.alloc(nz);
+        }
+        sa.alloc(nz);
+        st.alloc(nz);
     }
     Vec<T> view() {
-        return Vec<T>{x.p, xbar.p, xa.p, xp.p, y.p, ya.p, yp.p, c.p, lo.p, up.p, rl.p, ru.p, aval.p, tval.p};
+        return Vec<T>{x.p, xbar.p, xa.p, xp.p, y.p, ya.p, yp.p, c.p, lo.p, up.p, rl.p, ru.p, aval.p, tval.p, sa.p, st.p};
     }
     // Arrays the iteration changes (autotune snapshots): iterates, and in the
     // re-centred FP32 phase also the re-based constants.
@@ -810,7 +908,7 @@ struct nir_batch {
     std::vector<std::vector<double>> h_cost;   // for the objective of the result
     std::vector<double> h_offset;
     DevArray<Inst> inst;
-    DevArray<int> arow, acol, trow, tcol;
+    DevArray<int> arow, acol, trow, tcol, sacol, stcol;
     HostVec<double> d;
     HostVec<float> f;
     DevArray<double> R, C, hs, HA, GA, cost, collo, colup, rowlo, rowup, offset, x0, y0;
@@ -837,6 +935,8 @@ struct nir_batch {
         V.acol = acol.p;
         V.trow = trow.p;
         V.tcol = tcol.p;
+        V.sacol = sacol.p;
+        V.stcol = stcol.p;
         V.d = d.view();
         V.f = prm.precision == NIR_PREC_MIXED ? f.view() : Vec<float>{};
         V.R = R.p;
@@ -1128,10 +1228,12 @@ NIR_API int nir_batch_create_warm(nir_ctx* ctx, const nir_lp* const* lps, int32_
             up_i(B->has_x0, fx);
             up_i(B->has_y0, fy);
         }
-        B->d.alloc(xo, yo, nz);
+        B->d.alloc(xo, yo, nz, true);
         B->d.aval.upload(aval.data(), nz, s);
         B->d.tval.upload(tval.data(), nz, s);
-        if (mixed) B->f.alloc(xo, yo, nz);
+        if (mixed) B->f.alloc(xo, yo, nz, false);
+        B->sacol.alloc(nz);   // written by the setup kernel
+        B->stcol.alloc(nz);
         B->R.alloc(yo);
         B->C.alloc(xo);
         B->hs.alloc(yo);
