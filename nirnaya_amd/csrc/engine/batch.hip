@@ -10,10 +10,15 @@
 // Operand staging: the products gather y (for A^T y) and xbar (for A xbar)
 // at random. When the largest instance's max(m, n) FP64 entries fit in one
 // workgroup's LDS (up to 152 KB on gfx950), the 1024-lane kernel keeps the
-// operand being gathered in LDS - y on entry, xbar after the primal half, y
-// again after the dual half, each copied in by the lanes that wrote it - so
-// the random reads hit LDS instead of the vector memory path; the value and
-// index streams stay coalesced global loads.
+// operand being gathered in LDS, so the random reads hit LDS instead of the
+// vector memory path: an instance whose m + n entries fit (in the precision
+// of the phase) keeps y and xbar in two regions that the lanes computing
+// them write directly; a larger one keeps one region - y on entry, xbar
+// after the primal half, y again after the dual half, each copied in from
+// global memory after a barrier.
+//
+// Matrix streams: the row loops read wave-sliced copies of A and A^T (see
+// slice_dot), so a wave's value and index loads are contiguous runs.
 //
 // Work queue: a list of unfinished instances, largest first. A launch of the
 // persistent kernel hands them out by an atomic ticket; the workgroup runs up
@@ -362,41 +367,66 @@ __device__ __forceinline__ T* stage_base() {
     return reinterpret_cast<T*>(batch_stage);
 }
 
-// The same iteration with the gathered operands in LDS: y (m entries) for
-// A^T y, then xbar (n entries) for A xbar, each copied in by the lanes that
-// wrote it. The products read the same values in the same order, so the
-// iterates are bit-identical to the unstaged form; the random accesses go to
-// LDS instead of the vector memory path. `stage` holds y on entry and on
-// return.
+// LDS copy of `count` entries written earlier by other lanes, four loads in
+// flight per lane before the LDS writes (the copy is latency-bound).
+template <typename T, int BT>
+__device__ __forceinline__ void stage_copy(T* dst, const T* __restrict__ src, int count) {
+    for (int k = threadIdx.x; k < count; k += 4 * BT) {
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = k + u * BT < count ? src[k + u * BT] : T(0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k + u * BT < count) dst[k + u * BT] = v[u];
+    }
+}
+
+// The same iteration with the gathered operands in LDS. The products read
+// the same values in the same order, so the iterates are bit-identical to
+// the unstaged form; the random accesses go to LDS instead of the vector
+// memory path. `stage` holds y on entry and on return.
+//   two regions (m + n entries fit): y at stage[0, m), xbar at stage[m, m+n);
+//     each lane writes the xbar and y entries it computes straight into
+//     LDS (nothing reads the region a half writes until the barrier after
+//     it), so an iteration has two barriers and no copies;
+//   one region (max(m, n) entries): y, then xbar after the primal half, then
+//     y again, each copied in from global memory after a barrier.
 template <typename T, int BT>
 __device__ __forceinline__ void wg_iteration_staged(const Local<T>& L, const Inst& I, const PdlpState& s, int k_lo,
-                                                    int save, T* stage) {
+                                                    int save, T* stage, bool two) {
     const Halpern<T> hw = halpern<T>(s.k_base + k_lo);
     const T tau = static_cast<T>(s.tau);
     constexpr bool kDelta = std::is_same<T, float>::value;
+    T* sx = two ? stage + I.m : stage;
     for (int j = threadIdx.x; j < I.n; j += BT) {
         const T xj = L.x[j], cj = L.c[j], lj = L.lo[j], uj = L.up[j], aj = kDelta ? T(0) : L.xa[j];
         const T g = slice_dot<T>(L.trow, L.stcol, L.st, stage, j);
         const PrimalOut<T> r = primal_point<T>(xj, g, cj, lj, uj, aj, tau, hw);
-        L.xbar[j] = r.xb;
+        if (two) sx[j] = r.xb;
+        else L.xbar[j] = r.xb;
         L.x[j] = r.next;
         if (save) L.xp[j] = r.cand;
     }
     __syncthreads();
-    for (int j = threadIdx.x; j < I.n; j += BT) stage[j] = L.xbar[j];
-    __syncthreads();
+    if (!two) {
+        stage_copy<T, BT>(stage, L.xbar, I.n);
+        __syncthreads();
+    }
     const T sigma = static_cast<T>(s.sigma);
     for (int i = threadIdx.x; i < I.m; i += BT) {
         const T yi = L.y[i], rli = L.rl[i], rui = L.ru[i], ai = L.ya[i];
-        const T h = slice_dot<T>(L.arow, L.sacol, L.sa, stage, i);
+        const T h = slice_dot<T>(L.arow, L.sacol, L.sa, sx, i);
         const DualOut<T> r = kDelta ? dual_point_delta<T>(yi, h, rli, rui, ai, sigma, hw)
                                     : dual_point<T>(yi, h, rli, rui, ai, sigma, hw);
         L.y[i] = r.next;
+        if (two) stage[i] = r.next;
         if (save) L.yp[i] = r.yp;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < I.m; i += BT) stage[i] = L.y[i];
-    __syncthreads();
+    if (!two) {
+        stage_copy<T, BT>(stage, L.y, I.m);
+        __syncthreads();
+    }
 }
 
 // Restart: iterate and anchor become T(z).
@@ -551,10 +581,11 @@ __device__ void wg_block(const View& V, const Inst& I, const Vec<T>& v, Shared<B
     const Local<T> L = local(V, v, I);
     if (STAGED) {
         T* stage = stage_base<T>();
-        for (int i = threadIdx.x; i < I.m; i += BT) stage[i] = L.y[i];
+        const bool two = static_cast<long long>(I.m + I.n) * static_cast<long long>(sizeof(T)) <= V.stage_bytes;
+        stage_copy<T, BT>(stage, L.y, I.m);
         __syncthreads();
         for (int kl = 0; kl < V.eval_freq; ++kl)
-            wg_iteration_staged<T, BT>(L, I, sh.s, kl, kl == V.eval_freq - 1 ? 1 : 0, stage);
+            wg_iteration_staged<T, BT>(L, I, sh.s, kl, kl == V.eval_freq - 1 ? 1 : 0, stage, two);
     } else {
         for (int kl = 0; kl < V.eval_freq; ++kl) wg_iteration<T, BT>(L, I, sh.s, kl, kl == V.eval_freq - 1 ? 1 : 0);
     }
@@ -1271,15 +1302,21 @@ NIR_API int nir_batch_create_warm(nir_ctx* ctx, const nir_lp* const* lps, int32_
         k_batch_compact<<<1, 64, 0, s>>>(V);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(s));
-        // LDS for the staged operands of the 1024-lane kernel: the largest
-        // max(m, n) in FP64, when it fits beside the workgroup's static
+        // LDS for the staged operands of the 1024-lane kernel, when the
+        // largest max(m, n) in FP64 fits beside the workgroup's static
         // scratch; otherwise no staging and every instance gathers from
         // global memory.
         {
             constexpr long long kStageCap = 152 * 1024;
-            long long need = 0;
-            for (const Inst& I : B->h_inst) need = std::max<long long>(need, std::max(I.m, I.n));
-            B->stage_bytes = need * 8 <= kStageCap ? static_cast<int>(std::max<long long>(need * 8, 8)) : 0;
+            long long need = 0, both = 0;
+            for (const Inst& I : B->h_inst) {
+                need = std::max<long long>(need, std::max(I.m, I.n));
+                both = std::max<long long>(both, static_cast<long long>(I.m) + I.n);
+            }
+            // One region of max(m, n) must fit every instance; instances
+            // whose m + n entries also fit keep y and xbar apart.
+            B->stage_bytes =
+                need * 8 <= kStageCap ? static_cast<int>(std::max<long long>(std::min(both * 8, kStageCap), 8)) : 0;
             if (B->stage_bytes > 0 &&
                 hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_run<1024, true>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, B->stage_bytes) != hipSuccess) {

@@ -96,6 +96,21 @@ def test_planning_family_members_match_single(engine):
     check_batch(engine, lps, default_params(mode=MODE_PDLP, precision=PREC_MIXED, max_iters=400000, tol=1e-6))
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_lds_staging_regions_match_single(engine, prec):
+    # 1024-lane workgroups with operands in LDS. The 48 x 40 staircase
+    # (m + n = 30,768) holds y and xbar in one region copied between the
+    # halves in FP64 and in two regions in FP32; the 20 x 30 one (m + n =
+    # 9,620) and the small ones use two regions in both precisions. Rows of
+    # length up to 40 (capacity rows) take several rounds of the sliced
+    # products.
+    lps = [gen_staircase_lp(20263100, 48, 40), gen_staircase_lp(20263101, 20, 30), gen_staircase_lp(20263102, 3, 6),
+           lpkit.transport_lp()]
+    params = default_params(mode=MODE_PDLP, precision=prec, max_iters=400000, tol=1e-6)
+    res, st = check_batch(engine, lps, params, threads=1024)
+    assert st.threads == 1024 and all(r.status == 0 for r in res)
+
+
 def test_batch_warm_starts_and_primal_weight_match_single(engine):
     lps = [gen_staircase_lp(20263000 + k, T, K) for k, (T, K) in enumerate([(4, 5), (6, 8), (9, 12)])]
     lps.append(gen_equality_lp(31, 300, 700, 5))
