@@ -191,6 +191,20 @@ def planning512(args):
         if name == "fp64":
             its = np.array([r.iterations for r in res])
         log(f"[bench] batch {name}: {out[name]}")
+    # The critical path: the instance with the most nonzero-iterations, alone
+    # in a batch (one workgroup), against the whole batch's solve time.
+    work = its * np.array([lp.nnz for lp in lps], dtype=np.float64)
+    big = int(np.argmax(work))
+    for name, prec in (("fp64", 0), ("mixed", PREC_MIXED)):
+        b = eng.batch([lps[big]], default_params(mode=MODE_PDLP, precision=prec, max_iters=400000, tol=1e-6))
+        t0 = time.perf_counter()
+        b.solve()
+        out[name]["largest_alone_s"] = round(time.perf_counter() - t0, 3)
+        b.close()
+    out["largest"] = {"instance": big, "m": int(lps[big].m), "n": int(lps[big].n), "nnz": int(lps[big].nnz),
+                      "fp64_iterations": int(its[big]), "work_share": round(float(work[big] / work.sum()), 4)}
+    log(f"[bench] largest instance: {out['largest']}, alone fp64 {out['fp64']['largest_alone_s']} s, "
+        f"mixed {out['mixed']['largest_alone_s']} s")
     p = default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-6)
     samp = [] if args.batch_only else list(range(0, 512, 32))
     ts, si = 0.0, 0
@@ -209,7 +223,7 @@ def planning512(args):
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "512 multi-period planning LPs, T~U{4..48} x K~U{5..40}, seeds 20263*1000+i",
                    "rows_histogram": {f"{int(a)}-{int(b)}": int(h) for a, b, h in zip(edges[:-1], edges[1:], hist)}},
-        "batch_fp64": out["fp64"], "batch_mixed": out["mixed"],
+        "batch_fp64": out["fp64"], "batch_mixed": out["mixed"], "largest_instance": out["largest"],
         "one_at_a_time_fp64_s": round(one, 3) if one else None,
         "one_at_a_time_sample": f"{len(samp)} instances (every 32nd), {ts:.2f}s for {si} iterations",
         "speedup_vs_one_at_a_time": round(one / out["fp64"]["total_s"], 2) if one else None,

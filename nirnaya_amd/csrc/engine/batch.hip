@@ -168,6 +168,14 @@ __device__ __forceinline__ int lane_rank(unsigned long long mask) {
                                                       __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0u)));
 }
 
+// base[i] addressed by a 32-bit byte offset from a uniform base (the
+// scalar-base form of the load, without 64-bit address arithmetic per lane;
+// per-instance arrays are below 2^28 entries, checked at batch creation).
+template <typename T>
+__device__ __forceinline__ T at32(const T* __restrict__ base, unsigned i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * static_cast<unsigned>(sizeof(T)));
+}
+
 template <typename T, typename V>
 __device__ __forceinline__ T slice_dot(const int* __restrict__ rp, const int* __restrict__ scol,
                                        const T* __restrict__ sval, const V* __restrict__ v, int r) {
@@ -201,12 +209,12 @@ __device__ __forceinline__ T slice_dot(const int* __restrict__ rp, const int* __
         int c[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            a[u] = sval[p[u]];
-            c[u] = scol[p[u]];
+            a[u] = at32(sval, static_cast<unsigned>(p[u]));
+            c[u] = at32(scol, static_cast<unsigned>(p[u]));
         }
         T g[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) g[u] = static_cast<T>(v[c[u]]);
+        for (int u = 0; u < U; ++u) g[u] = static_cast<T>(at32(v, static_cast<unsigned>(c[u])));
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const T su = s + a[u] * g[u];
@@ -1176,6 +1184,10 @@ NIR_API int nir_batch_create_warm(nir_ctx* ctx, const nir_lp* const* lps, int32_
         for (int i = 0; i < count; ++i) {
             const nir_lp* lp = lps[i];
             check_lp(lp);
+            constexpr long long kMaxEntries = 1LL << 28;   // 32-bit byte offsets in the device loops
+            if (lp->nrow >= kMaxEntries || lp->ncol >= kMaxEntries ||
+                (lp->ncol > 0 && lp->col_start[lp->ncol] >= kMaxEntries))
+                throw std::invalid_argument("batch instances are limited to 2^28 rows, columns and nonzeros each");
             Inst& I = B->h_inst[i];
             I.m = lp->nrow;
             I.n = lp->ncol;
