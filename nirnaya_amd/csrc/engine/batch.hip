@@ -2,7 +2,11 @@
 //
 // Layout in HBM: every instance's CSR(A) and CSR(A^T) (int32 local indices,
 // FP64 values scaled in place, FP32 copies in mixed mode) and its vectors are
-// concatenated; an instance table holds the offsets. One workgroup owns one
+// concatenated; an instance table holds the offsets. An instance's rows are
+// stored by decreasing length (a stable order; each row keeps its entries in
+// CSR order and A^T its entries in place, relabelled), so that the 64-row
+// slices of the row loops hold rows of like length; row-space inputs and
+// results go through the order on the host. One workgroup owns one
 // instance at a time and runs whole PDHG iterations on it with workgroup
 // barriers between the primal and the dual half (an instance of the
 // planning family fits comfortably in one CU's share of L2).
@@ -30,20 +34,25 @@
 // hipGraph of several (run, compact) rounds and polls the list length.
 //
 // Every per-instance computation uses the single-LP engine's arithmetic
-// (pdlp_core.hpp): sequential row sums in ascending index order, the same
-// scaling, power iteration, step sizes, Halpern updates, evaluation terms
-// and decisions; norms are double-double. So each instance's iterates,
-// iteration and restart counts equal what nir_pdhg_solve returns for it.
+// (pdlp_core.hpp): sequential row and column sums in the caller's index
+// order, the same scaling, power iteration, step sizes, Halpern updates,
+// evaluation terms and decisions; norms are double-double (their rounding
+// to double does not depend on the order of the terms, the device row
+// order included, short of a sum within about 2^-100 of a rounding
+// boundary). So each instance's iterates, iteration and restart counts
+// equal what nir_pdhg_solve returns for it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <memory>
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -945,6 +954,7 @@ struct nir_batch {
     int count = 0;
     std::vector<Inst> h_inst;
     std::vector<std::vector<double>> h_cost;   // for the objective of the result
+    std::vector<std::vector<int>> h_order;     // per instance: device row r is the caller's row h_order[r]
     std::vector<double> h_offset;
     DevArray<Inst> inst;
     DevArray<int> arow, acol, trow, tcol, sacol, stcol;
@@ -1179,6 +1189,7 @@ NIR_API int nir_batch_create_warm(nir_ctx* ctx, const nir_lp* const* lps, int32_
         // Host layout: offsets, CSR(A) by counting transpose, CSR(A^T) = CSC.
         B->h_inst.resize(count);
         B->h_cost.resize(count);
+        B->h_order.resize(count);
         B->h_offset.resize(count);
         long long xo = 0, yo = 0, nz = 0, arp = 0, trp = 0;
         for (int i = 0; i < count; ++i) {
@@ -1213,24 +1224,57 @@ NIR_API int nir_batch_create_warm(nir_ctx* ctx, const nir_lp* const* lps, int32_
         std::vector<int> arow(arp), acol(nz), trow(trp), tcol(nz);
         std::vector<double> aval(nz), tval(nz), cost(xo), collo(xo), colup(xo), rowlo(yo), rowup(yo);
         {
-            std::vector<int> ptr, col;
-            std::vector<double> val;
-            for (int i = 0; i < count; ++i) {
-                const nir_lp* lp = lps[i];
-                const Inst& I = B->h_inst[i];
-                csr_from_csc(lp, ptr, col, val);
-                std::copy(ptr.begin(), ptr.end(), arow.begin() + I.arp);
-                std::copy(col.begin(), col.end(), acol.begin() + I.nz);
-                std::copy(val.begin(), val.end(), aval.begin() + I.nz);
-                for (int j = 0; j <= I.n; ++j) trow[I.trp + j] = I.n > 0 ? static_cast<int>(lp->col_start[j]) : 0;
-                std::copy(lp->row_index, lp->row_index + I.nnz, tcol.begin() + I.nz);
-                std::copy(lp->value, lp->value + I.nnz, tval.begin() + I.nz);
-                std::copy(lp->cost, lp->cost + I.n, cost.begin() + I.xo);
-                std::copy(lp->collo, lp->collo + I.n, collo.begin() + I.xo);
-                std::copy(lp->colup, lp->colup + I.n, colup.begin() + I.xo);
-                std::copy(lp->rowlo, lp->rowlo + I.m, rowlo.begin() + I.yo);
-                std::copy(lp->rowup, lp->rowup + I.m, rowup.begin() + I.yo);
-            }
+            // Instances write disjoint ranges: built by host threads, instance
+            // i on thread i mod nt.
+            const int nt = std::max(1, std::min(16, static_cast<int>(std::thread::hardware_concurrency())));
+            std::vector<std::exception_ptr> err(static_cast<size_t>(nt));
+            auto build = [&](int t0) {
+                try {
+                    std::vector<int> ptr, col, pos;
+                    std::vector<double> val;
+                    for (int i = t0; i < count; i += nt) {
+                        const nir_lp* lp = lps[i];
+                        const Inst& I = B->h_inst[i];
+                        csr_from_csc(lp, ptr, col, val);
+                        // Rows by decreasing length (stable), each row's entries
+                        // in their CSR order; A^T relabelled, its entries in place.
+                        std::vector<int>& order = B->h_order[i];
+                        order.resize(static_cast<size_t>(I.m));
+                        std::iota(order.begin(), order.end(), 0);
+                        std::stable_sort(order.begin(), order.end(),
+                                         [&](int a, int b) { return ptr[a + 1] - ptr[a] > ptr[b + 1] - ptr[b]; });
+                        pos.resize(static_cast<size_t>(I.m));
+                        int at = 0;
+                        arow[I.arp] = 0;
+                        for (int r = 0; r < I.m; ++r) {
+                            const int old = order[r];
+                            pos[old] = r;
+                            for (int p = ptr[old]; p < ptr[old + 1]; ++p, ++at) {
+                                acol[I.nz + at] = col[p];
+                                aval[I.nz + at] = val[p];
+                            }
+                            arow[I.arp + r + 1] = at;
+                            rowlo[I.yo + r] = lp->rowlo[old];
+                            rowup[I.yo + r] = lp->rowup[old];
+                        }
+                        for (int j = 0; j <= I.n; ++j)
+                            trow[I.trp + j] = I.n > 0 ? static_cast<int>(lp->col_start[j]) : 0;
+                        for (int p = 0; p < I.nnz; ++p) tcol[I.nz + p] = pos[lp->row_index[p]];
+                        std::copy(lp->value, lp->value + I.nnz, tval.begin() + I.nz);
+                        std::copy(lp->cost, lp->cost + I.n, cost.begin() + I.xo);
+                        std::copy(lp->collo, lp->collo + I.n, collo.begin() + I.xo);
+                        std::copy(lp->colup, lp->colup + I.n, colup.begin() + I.xo);
+                    }
+                } catch (...) {
+                    err[static_cast<size_t>(t0)] = std::current_exception();
+                }
+            };
+            std::vector<std::thread> pool;
+            for (int t = 1; t < nt; ++t) pool.emplace_back(build, t);
+            build(0);
+            for (auto& t : pool) t.join();
+            for (auto& e : err)
+                if (e) std::rethrow_exception(e);
         }
         hipStream_t s = B->s;
         auto up_i = [&](DevArray<int>& a, const std::vector<int>& v) { a.alloc(v.size()); a.upload(v.data(), v.size(), s); };
@@ -1262,7 +1306,8 @@ NIR_API int nir_batch_create_warm(nir_ctx* ctx, const nir_lp* const* lps, int32_
                     fx[i] = 1;
                 }
                 if (y0s && y0s[i]) {
-                    std::copy(y0s[i], y0s[i] + I.m, hy.begin() + I.yo);
+                    const std::vector<int>& order = B->h_order[i];
+                    for (int r = 0; r < I.m; ++r) hy[I.yo + r] = y0s[i][order[r]];
                     fy[i] = 1;
                 }
             }
@@ -1433,9 +1478,11 @@ NIR_API int nir_batch_result(nir_batch* B, int32_t i, double* x, double* y, nir_
         if (I.m) HIP_CHECK(hipMemcpy(R.data(), B->R.p + I.yo, I.m * sizeof(double), hipMemcpyDeviceToHost));
         if (I.n) HIP_CHECK(hipMemcpy(Cc.data(), B->C.p + I.xo, I.n * sizeof(double), hipMemcpyDeviceToHost));
         for (int j = 0; j < I.n; ++j) xs[j] *= Cc[j];
-        for (int r = 0; r < I.m; ++r) ys[r] *= R[r];
         if (x) std::copy(xs.begin(), xs.end(), x);
-        if (y) std::copy(ys.begin(), ys.end(), y);
+        if (y) {
+            const std::vector<int>& order = B->h_order[i];   // device row r is the caller's row order[r]
+            for (int r = 0; r < I.m; ++r) y[order[r]] = ys[r] * R[r];
+        }
         if (info) {
             std::memset(info, 0, sizeof(*info));
             PdlpState fin = h;
