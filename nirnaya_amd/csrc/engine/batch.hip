@@ -214,20 +214,30 @@ __device__ __forceinline__ T slice_dot(const int* __restrict__ rp, const int* __
             p[u] = t + u < len ? b + lane_rank(mk[u]) : base;
             b += __popcll(mk[u]);
         }
+        // The active steps of a round are a prefix (a row active at step
+        // t + 1 is active at t): steps past it are skipped by the whole wave.
+        int nact = 1;
+#pragma unroll
+        for (int u = 1; u < U; ++u) nact += mk[u] != 0ull ? 1 : 0;
         T a[U];
         int c[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            a[u] = at32(sval, static_cast<unsigned>(p[u]));
-            c[u] = at32(scol, static_cast<unsigned>(p[u]));
+            if (u < nact) {
+                a[u] = at32(sval, static_cast<unsigned>(p[u]));
+                c[u] = at32(scol, static_cast<unsigned>(p[u]));
+            }
         }
         T g[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) g[u] = static_cast<T>(at32(v, static_cast<unsigned>(c[u])));
+        for (int u = 0; u < U; ++u)
+            if (u < nact) g[u] = static_cast<T>(at32(v, static_cast<unsigned>(c[u])));
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const T su = s + a[u] * g[u];
-            s = t + u < len ? su : s;
+            if (u < nact) {
+                const T su = s + a[u] * g[u];
+                s = t + u < len ? su : s;
+            }
         }
         base = b;
     }
