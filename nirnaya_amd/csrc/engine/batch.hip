@@ -22,7 +22,9 @@
 // global memory after a barrier.
 //
 // Matrix streams: the row loops read wave-sliced copies of A and A^T (see
-// slice_dot), so a wave's value and index loads are contiguous runs.
+// slice_dot), so a wave's value and index loads are contiguous runs; their
+// column indices take 16 bits when every instance has max(m, n) <= 65,536
+// (the planning family does), halving the index stream.
 //
 // Work queue: a list of unfinished instances, largest first. A launch of the
 // persistent kernel hands them out by an atomic ticket; the workgroup runs up
@@ -104,7 +106,8 @@ struct Vec {
 struct View {
     const Inst* inst;
     const int *arow, *acol, *trow, *tcol;
-    int *sacol, *stcol;   // wave-sliced column indices of A and A^T (written at setup)
+    void *sacol, *stcol;  // wave-sliced column indices of A and A^T (written at setup),
+    int idx16;            // uint16 when every instance has max(m, n) <= 65536, else int32
     Vec<double> d;
     Vec<float> f;
     double *R, *C, *hs;
@@ -133,7 +136,8 @@ struct View {
 template <typename T>
 struct Local {
     const int *arow, *acol, *trow, *tcol;
-    const int *sacol, *stcol;
+    const void *sacol, *stcol;   // uint16 (idx16) or int32
+    int idx16;
     T *aval, *tval, *sa, *st;
     T *x, *xbar, *xa, *xp, *y, *ya, *yp;
     T *c, *lo, *up, *rl, *ru;
@@ -146,8 +150,10 @@ __device__ __forceinline__ Local<T> local(const View& V, const Vec<T>& v, const 
     L.acol = V.acol + I.nz;
     L.trow = V.trow + I.trp;
     L.tcol = V.tcol + I.nz;
-    L.sacol = V.sacol + I.nz;
-    L.stcol = V.stcol + I.nz;
+    const long long ib = I.nz * (V.idx16 ? 2 : 4);
+    L.sacol = static_cast<const char*>(V.sacol) + ib;
+    L.stcol = static_cast<const char*>(V.stcol) + ib;
+    L.idx16 = V.idx16;
     L.aval = v.aval + I.nz;
     L.tval = v.tval + I.nz;
     L.sa = v.sa + I.nz;
@@ -185,8 +191,8 @@ __device__ __forceinline__ T at32(const T* __restrict__ base, unsigned i) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * static_cast<unsigned>(sizeof(T)));
 }
 
-template <typename T, typename V>
-__device__ __forceinline__ T slice_dot(const int* __restrict__ rp, const int* __restrict__ scol,
+template <typename T, typename IT, typename V>
+__device__ __forceinline__ T slice_dot(const int* __restrict__ rp, const IT* __restrict__ scol,
                                        const T* __restrict__ sval, const V* __restrict__ v, int r) {
     const int start = rp[r];
     const int len = rp[r + 1] - start;
@@ -225,7 +231,7 @@ __device__ __forceinline__ T slice_dot(const int* __restrict__ rp, const int* __
         for (int u = 0; u < U; ++u) {
             if (u < nact) {
                 a[u] = at32(sval, static_cast<unsigned>(p[u]));
-                c[u] = at32(scol, static_cast<unsigned>(p[u]));
+                c[u] = static_cast<int>(at32(scol, static_cast<unsigned>(p[u])));
             }
         }
         T g[U];
@@ -244,11 +250,19 @@ __device__ __forceinline__ T slice_dot(const int* __restrict__ rp, const int* __
     return s;
 }
 
+// slice_dot over the batch's index width (uniform: one width per batch).
+template <typename T, typename V>
+__device__ __forceinline__ T slice_row(const int* __restrict__ rp, const void* scol, int idx16,
+                                       const T* __restrict__ sval, const V* __restrict__ v, int r) {
+    return idx16 ? slice_dot<T>(rp, static_cast<const unsigned short*>(scol), sval, v, r)
+                 : slice_dot<T>(rp, static_cast<const int*>(scol), sval, v, r);
+}
+
 // Writes the wave-sliced copy of a CSR matrix (rows x ?, row pointers rp):
 // the positions slice_dot reads, by the same traversal.
 template <int BT, typename T>
 __device__ void wg_slice(const int* __restrict__ rp, const int* __restrict__ col, const T* __restrict__ val,
-                         int rows, int* __restrict__ scol, T* __restrict__ sval) {
+                         int rows, void* scol, int idx16, T* __restrict__ sval) {
     for (int r = threadIdx.x; r < rows; r += BT) {
         const int start = rp[r];
         const int len = rp[r + 1] - start;
@@ -258,7 +272,8 @@ __device__ void wg_slice(const int* __restrict__ rp, const int* __restrict__ col
             if (mask == 0ull) break;
             if (t < len) {
                 const int p = base + lane_rank(mask);
-                scol[p] = col[start + t];
+                if (idx16) static_cast<unsigned short*>(scol)[p] = static_cast<unsigned short>(col[start + t]);
+                else static_cast<int*>(scol)[p] = col[start + t];
                 sval[p] = val[start + t];
             }
             base += __popcll(mask);
@@ -316,8 +331,8 @@ __device__ void wg_eval(const View& V, const Inst& I, const Vec<TI>& it, Shared<
     // double-double accumulators live together.
     double* gs = V.d.xbar + I.xo;
     double* hs = V.hs + I.yo;
-    for (int j = threadIdx.x; j < I.n; j += BT) gs[j] = slice_dot<double>(D.trow, D.stcol, D.st, yp, j);
-    for (int i = threadIdx.x; i < I.m; i += BT) hs[i] = slice_dot<double>(D.arow, D.sacol, D.sa, xp, i);
+    for (int j = threadIdx.x; j < I.n; j += BT) gs[j] = slice_row<double>(D.trow, D.stcol, D.idx16, D.st, yp, j);
+    for (int i = threadIdx.x; i < I.m; i += BT) hs[i] = slice_row<double>(D.arow, D.sacol, D.idx16, D.sa, xp, i);
     // FP32 iterates are deltas from the FP64 anchor: T(z) = anchor + delta,
     // A^T yp = GA + A^T dyp, A xp = HA + A dxp.
     constexpr bool kDelta = std::is_same<TI, float>::value;
@@ -368,7 +383,7 @@ __device__ void wg_iteration(const Local<T>& L, const Inst& I, const PdlpState& 
     constexpr bool kDelta = std::is_same<T, float>::value;   // FP32: re-centred form, zero anchor delta
     for (int j = threadIdx.x; j < I.n; j += BT) {
         const T xj = L.x[j], cj = L.c[j], lj = L.lo[j], uj = L.up[j], aj = kDelta ? T(0) : L.xa[j];
-        const T g = slice_dot<T>(L.trow, L.stcol, L.st, L.y, j);
+        const T g = slice_row<T>(L.trow, L.stcol, L.idx16, L.st, L.y, j);
         const PrimalOut<T> r = primal_point<T>(xj, g, cj, lj, uj, aj, tau, hw);
         L.xbar[j] = r.xb;
         L.x[j] = r.next;
@@ -378,7 +393,7 @@ __device__ void wg_iteration(const Local<T>& L, const Inst& I, const PdlpState& 
     const T sigma = static_cast<T>(s.sigma);
     for (int i = threadIdx.x; i < I.m; i += BT) {
         const T yi = L.y[i], rli = L.rl[i], rui = L.ru[i], ai = L.ya[i];
-        const T h = slice_dot<T>(L.arow, L.sacol, L.sa, L.xbar, i);
+        const T h = slice_row<T>(L.arow, L.sacol, L.idx16, L.sa, L.xbar, i);
         const DualOut<T> r = kDelta ? dual_point_delta<T>(yi, h, rli, rui, ai, sigma, hw)
                                     : dual_point<T>(yi, h, rli, rui, ai, sigma, hw);
         L.y[i] = r.next;
@@ -427,7 +442,7 @@ __device__ __forceinline__ void wg_iteration_staged(const Local<T>& L, const Ins
     T* sx = two ? stage + I.m : stage;
     for (int j = threadIdx.x; j < I.n; j += BT) {
         const T xj = L.x[j], cj = L.c[j], lj = L.lo[j], uj = L.up[j], aj = kDelta ? T(0) : L.xa[j];
-        const T g = slice_dot<T>(L.trow, L.stcol, L.st, stage, j);
+        const T g = slice_row<T>(L.trow, L.stcol, L.idx16, L.st, stage, j);
         const PrimalOut<T> r = primal_point<T>(xj, g, cj, lj, uj, aj, tau, hw);
         if (two) sx[j] = r.xb;
         else L.xbar[j] = r.xb;
@@ -442,7 +457,7 @@ __device__ __forceinline__ void wg_iteration_staged(const Local<T>& L, const Ins
     const T sigma = static_cast<T>(s.sigma);
     for (int i = threadIdx.x; i < I.m; i += BT) {
         const T yi = L.y[i], rli = L.rl[i], rui = L.ru[i], ai = L.ya[i];
-        const T h = slice_dot<T>(L.arow, L.sacol, L.sa, sx, i);
+        const T h = slice_row<T>(L.arow, L.sacol, L.idx16, L.sa, sx, i);
         const DualOut<T> r = kDelta ? dual_point_delta<T>(yi, h, rli, rui, ai, sigma, hw)
                                     : dual_point<T>(yi, h, rli, rui, ai, sigma, hw);
         L.y[i] = r.next;
@@ -486,8 +501,8 @@ __device__ void wg_recenter(const View& V, const Inst& I, const PdlpState& s, in
         for (int i = threadIdx.x; i < I.m; i += BT) D.ya[i] = D.ya[i] + static_cast<double>(F.yp[i]);
         __syncthreads();
     }
-    for (int i = threadIdx.x; i < I.m; i += BT) HA[i] = slice_dot<double>(D.arow, D.sacol, D.sa, D.xa, i);
-    for (int j = threadIdx.x; j < I.n; j += BT) GA[j] = slice_dot<double>(D.trow, D.stcol, D.st, D.ya, j);
+    for (int i = threadIdx.x; i < I.m; i += BT) HA[i] = slice_row<double>(D.arow, D.sacol, D.idx16, D.sa, D.xa, i);
+    for (int j = threadIdx.x; j < I.n; j += BT) GA[j] = slice_row<double>(D.trow, D.stcol, D.idx16, D.st, D.ya, j);
     __syncthreads();
     const double sigma = s.sigma;
     for (int j = threadIdx.x; j < I.n; j += BT) {
@@ -764,8 +779,8 @@ __device__ void wg_setup(const View& V, int i, Shared<BT>& sh) {
         __syncthreads();
     }
     // Wave-sliced copies of the scaled matrices for the row loops.
-    wg_slice<BT>(D.arow, D.acol, aval, m, V.sacol + I.nz, D.sa);
-    wg_slice<BT>(D.trow, D.tcol, tval, n, V.stcol + I.nz, D.st);
+    wg_slice<BT>(D.arow, D.acol, aval, m, const_cast<void*>(D.sacol), V.idx16, D.sa);
+    wg_slice<BT>(D.trow, D.tcol, tval, n, const_cast<void*>(D.stcol), V.idx16, D.st);
     __syncthreads();
 
     // Scaled vectors (absent bounds as +-inf) and the norms.
@@ -816,9 +831,9 @@ __device__ void wg_setup(const View& V, int i, Shared<BT>& sh) {
             __syncthreads();
         }
         for (int it = 0; it < 200; ++it) {
-            for (int r = tid; r < m; r += BT) v[r] = slice_dot<double>(D.arow, D.sacol, D.sa, u, r);
+            for (int r = tid; r < m; r += BT) v[r] = slice_row<double>(D.arow, D.sacol, D.idx16, D.sa, u, r);
             __syncthreads();
-            for (int j = tid; j < n; j += BT) u[j] = slice_dot<double>(D.trow, D.stcol, D.st, v, j);
+            for (int j = tid; j < n; j += BT) u[j] = slice_row<double>(D.trow, D.stcol, D.idx16, D.st, v, j);
             __syncthreads();
             const double nw = wg_norm<BT>(u, n, sh);
             if (!(nw > 0.0) || !isfinite(nw)) break;
@@ -967,7 +982,9 @@ struct nir_batch {
     std::vector<std::vector<int>> h_order;     // per instance: device row r is the caller's row h_order[r]
     std::vector<double> h_offset;
     DevArray<Inst> inst;
-    DevArray<int> arow, acol, trow, tcol, sacol, stcol;
+    DevArray<int> arow, acol, trow, tcol;
+    DevArray<unsigned char> sacol, stcol;   // wave-sliced indices, 2 or 4 bytes each
+    int idx16 = 0;
     HostVec<double> d;
     HostVec<float> f;
     DevArray<double> R, C, hs, HA, GA, cost, collo, colup, rowlo, rowup, offset, x0, y0;
@@ -996,6 +1013,7 @@ struct nir_batch {
         V.tcol = tcol.p;
         V.sacol = sacol.p;
         V.stcol = stcol.p;
+        V.idx16 = idx16;
         V.d = d.view();
         V.f = prm.precision == NIR_PREC_MIXED ? f.view() : Vec<float>{};
         V.R = R.p;
@@ -1330,8 +1348,15 @@ NIR_API int nir_batch_create_warm(nir_ctx* ctx, const nir_lp* const* lps, int32_
         B->d.aval.upload(aval.data(), nz, s);
         B->d.tval.upload(tval.data(), nz, s);
         if (mixed) B->f.alloc(xo, yo, nz, false);
-        B->sacol.alloc(nz);   // written by the setup kernel
-        B->stcol.alloc(nz);
+        // Sliced indices (written by the setup kernel) in 16 bits when every
+        // instance's rows and columns fit.
+        {
+            int most = 0;
+            for (const Inst& I : B->h_inst) most = std::max(most, std::max(I.m, I.n));
+            B->idx16 = most <= 65536 ? 1 : 0;
+            B->sacol.alloc(static_cast<size_t>(nz) * (B->idx16 ? 2 : 4));
+            B->stcol.alloc(static_cast<size_t>(nz) * (B->idx16 ? 2 : 4));
+        }
         B->R.alloc(yo);
         B->C.alloc(xo);
         B->hs.alloc(yo);
