@@ -196,7 +196,8 @@ typedef struct {
 } nir_batch_stats;
 NIR_API void nir_batch_opts_default(nir_batch_opts* o);
 /* Uploads and prepares every instance (scaling, power iteration, start point
- * evaluation) on the device; `lps` holds `count` pointers. */
+ * evaluation) on the device; `lps` holds `count` pointers. Each instance must
+ * have fewer than 2^28 rows, columns and nonzeros (-1 otherwise). */
 NIR_API int nir_batch_create(nir_ctx* ctx, const nir_lp* const* lps, int32_t count, const nir_params* p,
                              const nir_batch_opts* o, nir_batch** out);
 /* The same with per-instance warm starts: x0s / y0s hold `count` pointers
