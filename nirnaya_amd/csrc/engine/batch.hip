@@ -154,8 +154,8 @@ __device__ __forceinline__ Local<T> local(const View& V, const Vec<T>& v, const 
     L.sacol = static_cast<const char*>(V.sacol) + ib;
     L.stcol = static_cast<const char*>(V.stcol) + ib;
     L.idx16 = V.idx16;
-    L.aval = v.aval + I.nz;
-    L.tval = v.tval + I.nz;
+    L.aval = v.aval ? v.aval + I.nz : nullptr;   // CSR values: FP64 only
+    L.tval = v.tval ? v.tval + I.nz : nullptr;
     L.sa = v.sa + I.nz;
     L.st = v.st + I.nz;
     L.x = v.x + I.xo; L.xbar = v.xbar + I.xo; L.xa = v.xa + I.xo; L.xp = v.xp + I.xo;

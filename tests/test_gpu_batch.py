@@ -111,6 +111,17 @@ def test_batch_lds_staging_regions_match_single(engine, prec):
     assert st.threads == 1024 and all(r.status == 0 for r in res)
 
 
+def test_batch_wide_instance_32bit_indices_unstaged(engine):
+    # 66,000 columns: the sliced indices take 32 bits and no instance's
+    # operand fits in LDS (global gathers). An iteration cap keeps it short;
+    # the capped iterates must still equal the single-LP engine's.
+    lps = [gen_equality_lp(41, 300, 66000, 3), gen_staircase_lp(20263103, 5, 7)]
+    for prec in (0, 1):
+        params = default_params(mode=MODE_PDLP, precision=prec, max_iters=2048, tol=1e-9)
+        res, st = check_batch(engine, lps, params)
+        assert res[0].status == 6 and res[0].iterations == 2048
+
+
 def test_batch_warm_starts_and_primal_weight_match_single(engine):
     lps = [gen_staircase_lp(20263000 + k, T, K) for k, (T, K) in enumerate([(4, 5), (6, 8), (9, 12)])]
     lps.append(gen_equality_lp(31, 300, 700, 5))
