@@ -110,7 +110,7 @@ struct Options {
 
     void apply_preset(const std::string& p);
     // Returns false for an unknown key or a value that does not parse
-    // (the reference lets std::stod exceptions escape; this build does not).
+    // (the reference lets std::stod exceptions escape; here they are caught).
     bool set_by_name(const std::string& key, const std::string& val);
 };
 
