@@ -2,11 +2,12 @@
 //
 // Layout in HBM: every instance's CSR(A) and CSR(A^T) (int32 local indices,
 // FP64 values scaled in place, FP32 copies in mixed mode) and its vectors are
-// concatenated; an instance table holds the offsets. An instance's rows are
-// stored by decreasing length (a stable order; each row keeps its entries in
-// CSR order and A^T its entries in place, relabelled), so that the 64-row
-// slices of the row loops hold rows of like length; row-space inputs and
-// results go through the order on the host. One workgroup owns one
+// concatenated; an instance table holds the offsets. An instance's rows and
+// columns are stored by decreasing length (stable orders; every row of A and
+// of A^T keeps its entries in the caller's order under the new labels), so
+// that the 64-row slices of the row loops hold rows of like length; inputs
+// and results go through the orders on the host, and the power iteration's
+// start vector is a function of the caller's column index (colmap). One workgroup owns one
 // instance at a time and runs whole PDHG iterations on it with workgroup
 // barriers between the primal and the dual half (an instance of the
 // planning family fits comfortably in one CU's share of L2).
@@ -130,6 +131,7 @@ struct View {
     const double* y0;  // per-instance flags; null when no instance has one
     const int* has_x0;
     const int* has_y0;
+    const int* colmap;  // device column j of an instance is the caller's column colmap[xo + j]
 };
 
 // Instance-local pointers of one precision.
@@ -822,8 +824,10 @@ __device__ void wg_setup(const View& V, int i, Shared<BT>& sh) {
     if (n > 0 && m > 0 && I.nnz > 0) {
         double* u = D.xbar;
         double* v = D.yp;
-        for (int j = tid; j < n; j += BT)
-            u[j] = 1.0 + static_cast<double>((static_cast<unsigned>(j) * 2654435761u) % 1000u) / 1000.0;
+        for (int j = tid; j < n; j += BT) {
+            const unsigned jo = static_cast<unsigned>(V.colmap[I.xo + j]);   // the caller's column index
+            u[j] = 1.0 + static_cast<double>((jo * 2654435761u) % 1000u) / 1000.0;
+        }
         __syncthreads();
         const double nu = wg_norm<BT>(u, n, sh);
         if (nu > 0) {
@@ -980,6 +984,7 @@ struct nir_batch {
     std::vector<Inst> h_inst;
     std::vector<std::vector<double>> h_cost;   // for the objective of the result
     std::vector<std::vector<int>> h_order;     // per instance: device row r is the caller's row h_order[r]
+    std::vector<std::vector<int>> h_corder;    // per instance: device column j is the caller's column h_corder[j]
     std::vector<double> h_offset;
     DevArray<Inst> inst;
     DevArray<int> arow, acol, trow, tcol;
@@ -988,7 +993,7 @@ struct nir_batch {
     HostVec<double> d;
     HostVec<float> f;
     DevArray<double> R, C, hs, HA, GA, cost, collo, colup, rowlo, rowup, offset, x0, y0;
-    DevArray<int> has_x0, has_y0;
+    DevArray<int> has_x0, has_y0, colmap;
     bool warm = false;
     DevArray<PdlpState> st;
     DevArray<Ctl> ctl;
@@ -1044,6 +1049,7 @@ struct nir_batch {
         V.y0 = warm ? y0.p : nullptr;
         V.has_x0 = warm ? has_x0.p : nullptr;
         V.has_y0 = warm ? has_y0.p : nullptr;
+        V.colmap = colmap.p;
         return V;
     }
     ~nir_batch() {
@@ -1218,6 +1224,7 @@ NIR_API int nir_batch_create_warm(nir_ctx* ctx, const nir_lp* const* lps, int32_
         B->h_inst.resize(count);
         B->h_cost.resize(count);
         B->h_order.resize(count);
+        B->h_corder.resize(count);
         B->h_offset.resize(count);
         long long xo = 0, yo = 0, nz = 0, arp = 0, trp = 0;
         for (int i = 0; i < count; ++i) {
@@ -1251,6 +1258,7 @@ NIR_API int nir_batch_create_warm(nir_ctx* ctx, const nir_lp* const* lps, int32_
         B->total_nz = nz;
         std::vector<int> arow(arp), acol(nz), trow(trp), tcol(nz);
         std::vector<double> aval(nz), tval(nz), cost(xo), collo(xo), colup(xo), rowlo(yo), rowup(yo);
+        std::vector<int> colmap(xo);
         {
             // Instances write disjoint ranges: built by host threads, instance
             // i on thread i mod nt.
@@ -1258,40 +1266,56 @@ NIR_API int nir_batch_create_warm(nir_ctx* ctx, const nir_lp* const* lps, int32_
             std::vector<std::exception_ptr> err(static_cast<size_t>(nt));
             auto build = [&](int t0) {
                 try {
-                    std::vector<int> ptr, col, pos;
+                    std::vector<int> ptr, col, pos, cpos;
                     std::vector<double> val;
                     for (int i = t0; i < count; i += nt) {
                         const nir_lp* lp = lps[i];
                         const Inst& I = B->h_inst[i];
                         csr_from_csc(lp, ptr, col, val);
-                        // Rows by decreasing length (stable), each row's entries
-                        // in their CSR order; A^T relabelled, its entries in place.
+                        // Rows and columns by decreasing length (stable orders),
+                        // every row and column keeping its entries in the
+                        // caller's order under the new labels.
+                        auto by_length = [](std::vector<int>& ord, int k, auto len) {
+                            ord.resize(static_cast<size_t>(k));
+                            std::iota(ord.begin(), ord.end(), 0);
+                            std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return len(a) > len(b); });
+                        };
                         std::vector<int>& order = B->h_order[i];
-                        order.resize(static_cast<size_t>(I.m));
-                        std::iota(order.begin(), order.end(), 0);
-                        std::stable_sort(order.begin(), order.end(),
-                                         [&](int a, int b) { return ptr[a + 1] - ptr[a] > ptr[b + 1] - ptr[b]; });
+                        std::vector<int>& corder = B->h_corder[i];
+                        by_length(order, I.m, [&](int r) { return ptr[r + 1] - ptr[r]; });
+                        by_length(corder, I.n, [&](int j) {
+                            return static_cast<int>(lp->col_start[j + 1] - lp->col_start[j]);
+                        });
                         pos.resize(static_cast<size_t>(I.m));
+                        cpos.resize(static_cast<size_t>(I.n));
+                        for (int r = 0; r < I.m; ++r) pos[order[r]] = r;
+                        for (int j = 0; j < I.n; ++j) cpos[corder[j]] = j;
                         int at = 0;
                         arow[I.arp] = 0;
                         for (int r = 0; r < I.m; ++r) {
                             const int old = order[r];
-                            pos[old] = r;
                             for (int p = ptr[old]; p < ptr[old + 1]; ++p, ++at) {
-                                acol[I.nz + at] = col[p];
+                                acol[I.nz + at] = cpos[col[p]];
                                 aval[I.nz + at] = val[p];
                             }
                             arow[I.arp + r + 1] = at;
                             rowlo[I.yo + r] = lp->rowlo[old];
                             rowup[I.yo + r] = lp->rowup[old];
                         }
-                        for (int j = 0; j <= I.n; ++j)
-                            trow[I.trp + j] = I.n > 0 ? static_cast<int>(lp->col_start[j]) : 0;
-                        for (int p = 0; p < I.nnz; ++p) tcol[I.nz + p] = pos[lp->row_index[p]];
-                        std::copy(lp->value, lp->value + I.nnz, tval.begin() + I.nz);
-                        std::copy(lp->cost, lp->cost + I.n, cost.begin() + I.xo);
-                        std::copy(lp->collo, lp->collo + I.n, collo.begin() + I.xo);
-                        std::copy(lp->colup, lp->colup + I.n, colup.begin() + I.xo);
+                        at = 0;
+                        trow[I.trp] = 0;
+                        for (int j = 0; j < I.n; ++j) {
+                            const int old = corder[j];
+                            for (long long p = lp->col_start[old]; p < lp->col_start[old + 1]; ++p, ++at) {
+                                tcol[I.nz + at] = pos[lp->row_index[p]];
+                                tval[I.nz + at] = lp->value[p];
+                            }
+                            trow[I.trp + j + 1] = at;
+                            cost[I.xo + j] = lp->cost[old];
+                            collo[I.xo + j] = lp->collo[old];
+                            colup[I.xo + j] = lp->colup[old];
+                            colmap[I.xo + j] = old;
+                        }
                     }
                 } catch (...) {
                     err[static_cast<size_t>(t0)] = std::current_exception();
@@ -1322,6 +1346,7 @@ NIR_API int nir_batch_create_warm(nir_ctx* ctx, const nir_lp* const* lps, int32_
         up_d(B->rowlo, rowlo);
         up_d(B->rowup, rowup);
         up_d(B->offset, B->h_offset);
+        up_i(B->colmap, colmap);
         if (x0s || y0s) {
             // Warm starts, concatenated like the column / row vectors.
             B->warm = true;
@@ -1330,7 +1355,8 @@ NIR_API int nir_batch_create_warm(nir_ctx* ctx, const nir_lp* const* lps, int32_
             for (int i = 0; i < count; ++i) {
                 const Inst& I = B->h_inst[i];
                 if (x0s && x0s[i]) {
-                    std::copy(x0s[i], x0s[i] + I.n, hx.begin() + I.xo);
+                    const std::vector<int>& corder = B->h_corder[i];
+                    for (int j = 0; j < I.n; ++j) hx[I.xo + j] = x0s[i][corder[j]];
                     fx[i] = 1;
                 }
                 if (y0s && y0s[i]) {
@@ -1512,7 +1538,13 @@ NIR_API int nir_batch_result(nir_batch* B, int32_t i, double* x, double* y, nir_
         std::vector<double> R(static_cast<size_t>(I.m)), Cc(static_cast<size_t>(I.n));
         if (I.m) HIP_CHECK(hipMemcpy(R.data(), B->R.p + I.yo, I.m * sizeof(double), hipMemcpyDeviceToHost));
         if (I.n) HIP_CHECK(hipMemcpy(Cc.data(), B->C.p + I.xo, I.n * sizeof(double), hipMemcpyDeviceToHost));
-        for (int j = 0; j < I.n; ++j) xs[j] *= Cc[j];
+        {
+            // Device column j is the caller's column corder[j].
+            const std::vector<int>& corder = B->h_corder[i];
+            std::vector<double> xo(static_cast<size_t>(I.n));
+            for (int j = 0; j < I.n; ++j) xo[corder[j]] = xs[j] * Cc[j];
+            xs.swap(xo);
+        }
         if (x) std::copy(xs.begin(), xs.end(), x);
         if (y) {
             const std::vector<int>& order = B->h_order[i];   // device row r is the caller's row order[r]
