@@ -432,7 +432,9 @@ __device__ __forceinline__ void stage_copy(T* dst, const T* __restrict__ src, in
 //   two regions (m + n entries fit): y at stage[0, m), xbar at stage[m, m+n);
 //     each lane writes the xbar and y entries it computes straight into
 //     LDS (nothing reads the region a half writes until the barrier after
-//     it), so an iteration has two barriers and no copies;
+//     it), so an iteration has two barriers and no copies; y is read from
+//     its region and written to global memory only by the block's last
+//     iteration (`save`), where the evaluation and the next block read it;
 //   one region (max(m, n) entries): y, then xbar after the primal half, then
 //     y again, each copied in from global memory after a barrier.
 template <typename T, int BT>
@@ -458,12 +460,12 @@ __device__ __forceinline__ void wg_iteration_staged(const Local<T>& L, const Ins
     }
     const T sigma = static_cast<T>(s.sigma);
     for (int i = threadIdx.x; i < I.m; i += BT) {
-        const T yi = L.y[i], rli = L.rl[i], rui = L.ru[i], ai = L.ya[i];
+        const T yi = two ? stage[i] : L.y[i], rli = L.rl[i], rui = L.ru[i], ai = L.ya[i];
         const T h = slice_row<T>(L.arow, L.sacol, L.idx16, L.sa, sx, i);
         const DualOut<T> r = kDelta ? dual_point_delta<T>(yi, h, rli, rui, ai, sigma, hw)
                                     : dual_point<T>(yi, h, rli, rui, ai, sigma, hw);
-        L.y[i] = r.next;
         if (two) stage[i] = r.next;
+        if (!two || save) L.y[i] = r.next;
         if (save) L.yp[i] = r.yp;
     }
     __syncthreads();
