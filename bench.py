@@ -261,6 +261,20 @@ def small_lp_workload(args):
                      "total_s": round(time.perf_counter() - t0, 5),
                      "rel_kkt": [r.info.rel_primal, r.info.rel_dual, r.info.rel_gap]}
         log(f"[bench] GPU {name} to 1e-6: {gpu[name]}")
+        # The same solve as a batch of one: one persistent workgroup runs the
+        # whole solve (the same iterates, bit for bit), no per-iteration
+        # launches; setup timed apart.
+        t0 = time.perf_counter()
+        b = eng.batch([lp], q)
+        t1 = time.perf_counter()
+        b.solve()
+        t2 = time.perf_counter()
+        rb = b.result(0)
+        b.close()
+        gpu[name + "_batch_of_one"] = {"status": rb.status, "iterations": rb.iterations,
+                                       "setup_s": round(t1 - t0, 5), "solve_s": round(t2 - t1, 5),
+                                       "same_iterates": bool(np.array_equal(rb.x, r.x) and np.array_equal(rb.y, r.y))}
+        log(f"[bench] GPU {name} as a batch of one: {gpu[name + '_batch_of_one']}")
     # reference form on the GPU, KKT on the device between chunks
     s = eng.session(lp, default_params(mode=MODE_REFERENCE, max_iters=10 ** 12, check_termination=0))
     solve, it, k = 0.0, 0, np.ones(5)
