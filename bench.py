@@ -551,8 +551,8 @@ def main():
                     help="planning512: skip the one-at-a-time sample (batch solves only)")
     ap.add_argument("--ws-m", type=int, default=5000)
     ap.add_argument("--ws-n", type=int, default=12000)
-    ap.add_argument("--ws-train", type=int, default=2048)
-    ap.add_argument("--ws-test", type=int, default=128)
+    ap.add_argument("--ws-train", type=int, default=8192)
+    ap.add_argument("--ws-test", type=int, default=1024)
     ap.add_argument("--ws-epochs", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:

@@ -54,18 +54,46 @@ class ParametricFamily:
                   np.ascontiguousarray(b), b.copy())
 
 
+def top_eigh(G: np.ndarray, r: int, oversample: int = 32, sweeps: int = 8, seed: int = 0):
+    """Leading r eigenpairs of the symmetric positive semidefinite G (N x N),
+    descending, by block subspace iteration with a Rayleigh-Ritz step: Q spans
+    G^q Omega (Omega N x (r + oversample) Gaussian), then the eigenpairs of
+    Q^T G Q give Ritz values and vectors V = Q U, for which V^T G V is
+    diagonal, so POD modes X V S^-1 are orthonormal. The leading pairs
+    converge like (lambda_{r+oversample+1} / lambda_i)^sweeps; costs
+    O(N^2 (r + oversample)) per sweep instead of a full O(N^3)
+    eigendecomposition."""
+    n = G.shape[0]
+    k = min(n, r + oversample)
+    Q = np.random.default_rng(seed).standard_normal((n, k))
+    Q, _ = np.linalg.qr(G @ Q)
+    for _ in range(sweeps):
+        Q, _ = np.linalg.qr(G @ Q)
+    w, U = np.linalg.eigh(Q.T @ G @ Q)
+    order = np.argsort(w)[::-1][:r]
+    return w[order], Q @ U[:, order]
+
+
 class Pod:
     """Mean-centred POD of snapshots S [N][len]: modes [len][r] with orthonormal
-    columns, singular values, and the coefficients of the training snapshots."""
+    columns, singular values, and the coefficients of the training snapshots.
+    The eigenpairs of the snapshot Gram matrix are exact (a full symmetric
+    eigendecomposition) up to N = exact_limit snapshots and from block
+    subspace iteration (top_eigh) beyond, where only the leading `rank` of N
+    pairs are used."""
 
-    def __init__(self, engine: Engine, S: np.ndarray, rank: int, rel_floor: float = 1e-12):
+    def __init__(self, engine: Engine, S: np.ndarray, rank: int, rel_floor: float = 1e-12,
+                 exact_limit: int = 2048):
         S = np.asarray(S, dtype=np.float64)
         self.mean = S.mean(axis=0)
         X = np.ascontiguousarray((S - self.mean).T)           # [len][N]
         G = engine.pod_gram(X)                                  # X^T X on the device
-        lam, V = np.linalg.eigh(G)
-        order = np.argsort(lam)[::-1]
-        lam, V = lam[order], V[:, order]
+        if G.shape[0] > exact_limit and 4 * rank < G.shape[0]:
+            lam, V = top_eigh(G, rank)
+        else:
+            lam, V = np.linalg.eigh(G)
+            order = np.argsort(lam)[::-1]
+            lam, V = lam[order], V[:, order]
         keep = int(np.sum(lam > rel_floor * max(lam[0], 1e-300))) if lam.size else 0
         r = max(1, min(rank, keep))
         self.sing = np.sqrt(np.maximum(lam[:r], 0.0))

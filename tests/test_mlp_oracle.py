@@ -111,3 +111,27 @@ def test_pod_host_logic_matches_svd():
     assert np.allclose(pod.train_coef, np.stack([pod.encode(s) for s in S]), atol=1e-8 * sv[0])
     rec = np.stack([pod.decode(pod.encode(s)) for s in S])
     assert np.max(np.abs(rec - S)) < 1e-4
+
+
+def test_pod_subspace_iteration_matches_svd_on_leading_modes():
+    # Above exact_limit snapshots the POD takes the leading eigenpairs of the
+    # Gram matrix by block subspace iteration: against numpy's SVD, the
+    # leading singular values and subspace agree, the modes are orthonormal,
+    # and the training coefficients are the modes' projections.
+    from nirnaya_amd.warmstart import Pod
+    rng = np.random.default_rng(12)
+    N, L, r = 600, 900, 32
+    basis = np.linalg.qr(rng.standard_normal((L, 120)))[0]
+    coef = rng.standard_normal((N, 120)) * (np.arange(1, 121) ** -1.5) * 10.0
+    S = coef @ basis.T + 1e-4 * rng.standard_normal((N, L))
+    pod = Pod(_NumpyPodEngine(), S, rank=r, exact_limit=100)
+    Xc = S - S.mean(axis=0)
+    U, sv, _ = np.linalg.svd(Xc.T, full_matrices=False)
+    assert pod.rank == r
+    assert np.allclose(pod.sing[:16], sv[:16], rtol=1e-9)
+    assert np.allclose(pod.sing, sv[:r], rtol=1e-3)
+    assert np.allclose(pod.modes.T @ pod.modes, np.eye(r), atol=1e-9)
+    P1 = pod.modes[:, :8] @ pod.modes[:, :8].T
+    P2 = U[:, :8] @ U[:, :8].T
+    assert np.linalg.norm(P1 - P2) < 1e-8
+    assert np.allclose(pod.train_coef, np.stack([pod.encode(s) for s in S]), atol=1e-8 * sv[0])
