@@ -191,6 +191,24 @@ def planning512(args):
         if name == "fp64":
             its = np.array([r.iterations for r in res])
         log(f"[bench] batch {name}: {out[name]}")
+    # Mixed precision with the FP64 polish tail to 1e-8 (relative KKT of every
+    # instance under the FP64 evaluation).
+    p = default_params(mode=MODE_PDLP, precision=PREC_MIXED, max_iters=2_000_000, tol=1e-6, polish_tol=1e-8)
+    t0 = time.perf_counter()
+    b = eng.batch(lps, p)
+    t1 = time.perf_counter()
+    b.solve()
+    t2 = time.perf_counter()
+    res = [b.result(i) for i in range(len(lps))]
+    st = b.stats()
+    b.close()
+    worst = max(max(r.info.rel_primal, r.info.rel_dual, r.info.rel_gap) for r in res)
+    out["mixed_polish_1e-8"] = {"setup_s": t1 - t0, "solve_s": t2 - t1, "total_s": t2 - t0,
+                                "optimal": int(sum(r.status == 0 for r in res)),
+                                "iterations": int(st.total_iterations),
+                                "fp64_tail_iterations": int(sum(r.info.polish_iterations for r in res)),
+                                "worst_rel_kkt": worst}
+    log(f"[bench] batch mixed + FP64 polish to 1e-8: {out['mixed_polish_1e-8']}")
     # The critical path: the instance with the most nonzero-iterations, alone
     # in a batch (one workgroup), against the whole batch's solve time.
     work = its * np.array([lp.nnz for lp in lps], dtype=np.float64)
@@ -215,6 +233,19 @@ def planning512(args):
         si += r.iterations
         assert r.iterations == its[i]
     one = ts / si * its.sum() if samp else None
+    # FP64 one at a time to 1e-8 on the same sample, scaled by the instance
+    # count (the batch's iteration counts to 1e-8 differ: mixed phase + FP64
+    # tail), against the mixed + polish batch.
+    one8 = None
+    if samp:
+        p8 = default_params(mode=MODE_PDLP, max_iters=2_000_000, tol=1e-8)
+        t8 = 0.0
+        for i in samp:
+            t0 = time.perf_counter()
+            r = eng.solve(lps[i], p8)
+            t8 += time.perf_counter() - t0
+            assert r.status == 0
+        one8 = t8 / len(samp) * len(lps)
     eng.close()
     hist, edges = np.histogram(ms, bins=[0, 1000, 2000, 5000, 10000, 20000])
     line = {
@@ -223,10 +254,14 @@ def planning512(args):
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "512 multi-period planning LPs, T~U{4..48} x K~U{5..40}, seeds 20263*1000+i",
                    "rows_histogram": {f"{int(a)}-{int(b)}": int(h) for a, b, h in zip(edges[:-1], edges[1:], hist)}},
-        "batch_fp64": out["fp64"], "batch_mixed": out["mixed"], "largest_instance": out["largest"],
+        "batch_fp64": out["fp64"], "batch_mixed": out["mixed"], "batch_mixed_polish_1e-8": out["mixed_polish_1e-8"],
+        "largest_instance": out["largest"],
         "one_at_a_time_fp64_s": round(one, 3) if one else None,
         "one_at_a_time_sample": f"{len(samp)} instances (every 32nd), {ts:.2f}s for {si} iterations",
         "speedup_vs_one_at_a_time": round(one / out["fp64"]["total_s"], 2) if one else None,
+        "one_at_a_time_fp64_1e-8_s": round(one8, 3) if one8 else None,
+        "speedup_mixed_polish_1e-8_vs_one_at_a_time": round(one8 / out["mixed_polish_1e-8"]["total_s"], 2)
+        if one8 else None,
     }
     print(json.dumps(line), flush=True)
 
