@@ -1,16 +1,16 @@
 // Batched restarted PDHG: many LPs of uneven size in persistent launches.
 //
 // Layout in HBM: every instance's CSR(A) and CSR(A^T) (int32 local indices,
-// FP64 values scaled in place, FP32 copies in mixed mode) and its vectors are
-// concatenated; an instance table holds the offsets. An instance's rows and
-// columns are stored by decreasing length (stable orders; every row of A and
-// of A^T keeps its entries in the caller's order under the new labels), so
-// that the 64-row slices of the row loops hold rows of like length; inputs
-// and results go through the orders on the host, and the power iteration's
-// start vector is a function of the caller's column index (colmap). One workgroup owns one
-// instance at a time and runs whole PDHG iterations on it with workgroup
-// barriers between the primal and the dual half (an instance of the
-// planning family fits comfortably in one CU's share of L2).
+// FP64 values scaled in place), their wave-sliced copies (FP64, and FP32 in
+// mixed mode) and its vectors are concatenated; an instance table holds the
+// offsets. An instance's rows and columns are stored by decreasing length
+// (stable orders; every row of A and of A^T keeps its entries in the
+// caller's order under the new labels), so that the 64-row slices of the
+// row loops hold rows of like length; inputs and results go through the
+// orders on the host, and the power iteration's start vector is a function
+// of the caller's column index (colmap). One workgroup owns one instance at
+// a time and runs whole PDHG iterations on it with workgroup barriers
+// between the primal and the dual half.
 //
 // Operand staging: the products gather y (for A^T y) and xbar (for A xbar)
 // at random. When the largest instance's max(m, n) FP64 entries fit in one
@@ -18,9 +18,9 @@
 // operand being gathered in LDS, so the random reads hit LDS instead of the
 // vector memory path: an instance whose m + n entries fit (in the precision
 // of the phase) keeps y and xbar in two regions that the lanes computing
-// them write directly; a larger one keeps one region - y on entry, xbar
-// after the primal half, y again after the dual half, each copied in from
-// global memory after a barrier.
+// them write directly, y staying there across an evaluation block; a larger
+// one keeps one region - y on entry, xbar after the primal half, y again
+// after the dual half, each copied in from global memory after a barrier.
 //
 // Matrix streams: the row loops read wave-sliced copies of A and A^T (see
 // slice_dot), so a wave's value and index loads are contiguous runs; their
