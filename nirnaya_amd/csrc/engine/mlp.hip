@@ -60,6 +60,30 @@ struct Epi {
     int rows_valid;     // rows >= this are written as zero (batch padding)
 };
 
+// Epilogue of one 32 x 32 accumulator block at (r0, c0): bias, activation,
+// padding rows, FP32 and bf16 stores. Output layout of the 32x32 MFMA: lane
+// holds column lane % 32 and rows 8 (r / 4) + 4 (lane / 32) + r % 4 for its
+// 16 accumulators r.
+__device__ __forceinline__ void epi_block(const f32x16& acc, int r0, int c0, int lane, const Epi& e) {
+    const int col = c0 + (lane & 31);
+    const float bias = e.bias ? e.bias[col] : 0.0f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = r0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const size_t o = static_cast<size_t>(row) * e.ldo + col;
+        float v = acc[r] + bias;
+        float a = v;
+        if (e.act == ACT_GELU) a = gelu(v);
+        else if (e.act == ACT_GELU_BWD) a = v * gelu_grad(e.z[o]);
+        if (row >= e.rows_valid) {
+            v = 0.0f;
+            a = 0.0f;
+        }
+        if (e.out) e.out[o] = e.act == ACT_GELU_BWD ? a : v;
+        if (e.out_bf) e.out_bf[o] = static_cast<bf16>(a);
+    }
+}
+
 // C[M x N] = sum_k A(m, k) B(n, k). A is [M][K] (lda) or, with TA, [K][M];
 // B is [N][K] (ldb) or, with TB, [K][N]. M, N multiples of 64, K of 32.
 template <bool TA, bool TB>
@@ -102,25 +126,87 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16* _
         }
         __syncthreads();
     }
-    // Output layout of the 32x32 MFMA: lane holds column lane % 32 and rows
-    // 8 (r / 4) + 4 (lane / 32) + r % 4 for its 16 accumulators r.
-    const int col = n0 + wc * 32 + (lane & 31);
-    const float bias = e.bias ? e.bias[col] : 0.0f;
+    epi_block(acc, m0 + wr * 32, n0 + wc * 32, lane, e);
+}
+
+// The same product for large shapes: 128 x 128 tiles, each of the four waves
+// a 64 x 64 quarter as 2 x 2 blocks of 32x32x16 MFMAs (four MFMAs per 16-wide
+// k step instead of one) over 64-wide k slices, the next slice loaded from
+// global memory into registers while the current one is multiplied out of
+// LDS. M, N multiples of 128, K of 64.
+constexpr int GM = 128, GN = 128, GK = 64, GLP = GK + 8;   // k slice and LDS row pitch (bf16)
+
+template <bool TP>
+__device__ __forceinline__ void big_load(const bf16* __restrict__ P, int ld, int p0, int k0, int tid, bf16x8 (&v)[4]) {
+    if (!TP) {   // P is [rows][K]: thread -> row tid / 2, 32 k values
+        const int r = tid >> 1, c = (tid & 1) * 32;
+        const bf16* q = P + static_cast<size_t>(p0 + r) * ld + k0 + c;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int row = m0 + wr * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const size_t o = static_cast<size_t>(row) * e.ldo + col;
-        float v = acc[r] + bias;
-        float a = v;
-        if (e.act == ACT_GELU) a = gelu(v);
-        else if (e.act == ACT_GELU_BWD) a = v * gelu_grad(e.z[o]);
-        if (row >= e.rows_valid) {
-            v = 0.0f;
-            a = 0.0f;
-        }
-        if (e.out) e.out[o] = e.act == ACT_GELU_BWD ? a : v;
-        if (e.out_bf) e.out_bf[o] = static_cast<bf16>(a);
+        for (int i = 0; i < 4; ++i) v[i] = *reinterpret_cast<const bf16x8*>(q + 8 * i);
+    } else {     // P is [K][rows]: thread -> k tid / 4, 32 row values
+        const int k = tid >> 2, c = (tid & 3) * 32;
+        const bf16* q = P + static_cast<size_t>(k0 + k) * ld + p0 + c;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = *reinterpret_cast<const bf16x8*>(q + 8 * i);
     }
+}
+
+template <bool TP>
+__device__ __forceinline__ void big_store(bf16* S, int tid, const bf16x8 (&v)[4]) {
+    if (!TP) {
+        const int r = tid >> 1, c = (tid & 1) * 32;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) *reinterpret_cast<bf16x8*>(&S[r * GLP + c + 8 * i]) = v[i];
+    } else {
+        const int k = tid >> 2, c = (tid & 3) * 32;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) S[(c + 8 * i + j) * GLP + k] = v[i][j];
+    }
+}
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_gemm_big(int M, int N, int K, const bf16* __restrict__ A, int lda,
+                                                  const bf16* __restrict__ B, int ldb, Epi e) {
+    __shared__ bf16 As[GM * GLP];
+    __shared__ bf16 Bs[GN * GLP];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int m0 = blockIdx.y * GM, n0 = blockIdx.x * GN;
+    const int wr = w >> 1, wc = w & 1;
+    f32x16 acc[2][2] = {};
+    bf16x8 va[4], vb[4];
+    big_load<TA>(A, lda, m0, 0, tid, va);
+    big_load<TB>(B, ldb, n0, 0, tid, vb);
+    for (int k0 = 0; k0 < K; k0 += GK) {
+        big_store<TA>(As, tid, va);
+        big_store<TB>(Bs, tid, vb);
+        __syncthreads();
+        if (k0 + GK < K) {   // next slice in flight during the MFMAs
+            big_load<TA>(A, lda, m0, k0 + GK, tid, va);
+            big_load<TB>(B, ldb, n0, k0 + GK, tid, vb);
+        }
+#pragma unroll
+        for (int kk = 0; kk < GK; kk += 16) {
+            bf16x8 a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                a[i] = *reinterpret_cast<const bf16x8*>(
+                    &As[(wr * 64 + i * 32 + (lane & 31)) * GLP + kk + 8 * (lane >> 5)]);
+                b[i] = *reinterpret_cast<const bf16x8*>(
+                    &Bs[(wc * 64 + i * 32 + (lane & 31)) * GLP + kk + 8 * (lane >> 5)]);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) epi_block(acc[i][j], m0 + wr * 64 + i * 32, n0 + wc * 64 + j * 32, lane, e);
 }
 
 // Mean squared error over rows_valid x n: loss partials per block and the
@@ -268,9 +354,17 @@ namespace {
 
 int pad64(int r) { return (r + 63) / 64 * 64; }
 
+// 128 x 128 tiles when the shape has them and they fill the chip (256 CUs);
+// 64 x 64 tiles otherwise (the network's training shapes: more workgroups
+// for a latency-bound product).
 template <bool TA, bool TB>
 void gemm(hipStream_t s, int M, int N, int K, const bf16* A, int lda, const bf16* B, int ldb, const Epi& e) {
     if (M % 64 || N % 64 || K % 32) throw std::invalid_argument("gemm shape not a multiple of the tile");
+    if (M % GM == 0 && N % GN == 0 && K % GK == 0 && (M / GM) * (N / GN) >= 256) {
+        dim3 grid(N / GN, M / GM);
+        k_gemm_big<TA, TB><<<grid, 256, 0, s>>>(M, N, K, A, lda, B, ldb, e);
+        return;
+    }
     dim3 grid(N / 64, M / 64);
     k_gemm<TA, TB><<<grid, 256, 0, s>>>(M, N, K, A, lda, B, ldb, e);
 }

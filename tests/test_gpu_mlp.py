@@ -37,6 +37,24 @@ def test_gemm_layouts_match_numpy(engine, ta, tb):
     assert err <= 1e-5 * scale
 
 
+@pytest.mark.parametrize("ta,tb", [(False, False), (False, True), (True, False), (True, True)])
+def test_large_tile_gemm_layouts_match_numpy(engine, ta, tb):
+    # 2048 x 2048 outputs = 256 tiles of 128 x 128 and K a multiple of 64:
+    # the large-tile kernel.
+    rng = np.random.default_rng(6)
+    M, N, K = 2048, 2048, 192
+    A = rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
+    B = rng.standard_normal((K, N) if tb else (N, K)).astype(np.float32)
+    C = engine.gemm_check(A, B, ta, tb)
+    Ab = mo.bf16_round(A).astype(np.float64)
+    Bb = mo.bf16_round(B).astype(np.float64)
+    ref = (Ab.T if ta else Ab) @ (Bb if tb else Bb.T)
+    scale = np.sqrt(K) * np.max(np.abs(Ab)) * np.max(np.abs(Bb))
+    err = np.max(np.abs(C - ref))
+    print(f"large tiles ta={ta} tb={tb}: max err {err:.2e} (scale {scale:.1f})")
+    assert err <= 1e-5 * scale
+
+
 def _net(engine, seed=0):
     n_in, hid, n_out = 64, 128, 192
     net = engine.mlp(n_in, hid, n_out, max_rows=256, seed=seed)
