@@ -132,8 +132,9 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16* _
 // The same product for large shapes: 128 x 128 tiles, each of the four waves
 // a 64 x 64 quarter as 2 x 2 blocks of 32x32x16 MFMAs (four MFMAs per 16-wide
 // k step instead of one) over 64-wide k slices, the next slice loaded from
-// global memory into registers while the current one is multiplied out of
-// LDS. M, N multiples of 128, K of 64.
+// global memory into registers and stored into the second LDS buffer while
+// the current one is multiplied out of the first. M, N multiples of 128, K
+// of 64.
 constexpr int GM = 128, GN = 128, GK = 64, GLP = GK + 8;   // k slice and LDS row pitch (bf16)
 
 template <bool TP>
@@ -169,8 +170,10 @@ __device__ __forceinline__ void big_store(bf16* S, int tid, const bf16x8 (&v)[4]
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm_big(int M, int N, int K, const bf16* __restrict__ A, int lda,
                                                   const bf16* __restrict__ B, int ldb, Epi e) {
-    __shared__ bf16 As[GM * GLP];
-    __shared__ bf16 Bs[GN * GLP];
+    // Two LDS buffers: slice s + 1 is stored into one while slice s is read
+    // from the other, one barrier per slice.
+    __shared__ bf16 As[2][GM * GLP];
+    __shared__ bf16 Bs[2][GN * GLP];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int m0 = blockIdx.y * GM, n0 = blockIdx.x * GN;
     const int wr = w >> 1, wc = w & 1;
@@ -178,30 +181,39 @@ __global__ __launch_bounds__(256) void k_gemm_big(int M, int N, int K, const bf1
     bf16x8 va[4], vb[4];
     big_load<TA>(A, lda, m0, 0, tid, va);
     big_load<TB>(B, ldb, n0, 0, tid, vb);
+    big_store<TA>(As[0], tid, va);
+    big_store<TB>(Bs[0], tid, vb);
+    __syncthreads();
+    int cur = 0;
     for (int k0 = 0; k0 < K; k0 += GK) {
-        big_store<TA>(As, tid, va);
-        big_store<TB>(Bs, tid, vb);
-        __syncthreads();
-        if (k0 + GK < K) {   // next slice in flight during the MFMAs
+        const bool more = k0 + GK < K;
+        if (more) {   // next slice in flight during the MFMAs
             big_load<TA>(A, lda, m0, k0 + GK, tid, va);
             big_load<TB>(B, ldb, n0, k0 + GK, tid, vb);
         }
+        const bf16* as = As[cur];
+        const bf16* bs = Bs[cur];
 #pragma unroll
         for (int kk = 0; kk < GK; kk += 16) {
             bf16x8 a[2], b[2];
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
                 a[i] = *reinterpret_cast<const bf16x8*>(
-                    &As[(wr * 64 + i * 32 + (lane & 31)) * GLP + kk + 8 * (lane >> 5)]);
+                    &as[(wr * 64 + i * 32 + (lane & 31)) * GLP + kk + 8 * (lane >> 5)]);
                 b[i] = *reinterpret_cast<const bf16x8*>(
-                    &Bs[(wc * 64 + i * 32 + (lane & 31)) * GLP + kk + 8 * (lane >> 5)]);
+                    &bs[(wc * 64 + i * 32 + (lane & 31)) * GLP + kk + 8 * (lane >> 5)]);
             }
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
+        if (more) {
+            big_store<TA>(As[cur ^ 1], tid, va);
+            big_store<TB>(Bs[cur ^ 1], tid, vb);
+        }
         __syncthreads();
+        cur ^= 1;
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
