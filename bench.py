@@ -561,6 +561,34 @@ def warmstart_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def gemm_workload(args):
+    """The warm start's bf16 MFMA GEMM through nir_mlp_gemm_check (FP32
+    inputs rounded to bf16 on the device, C = A B^T in FP32) at a large shape
+    (4096^3: the 128 x 128-tile kernel) and at the network's training shape
+    (256 x 512 x 512: the 64 x 64-tile kernel), three calls each. The kernel
+    durations come from a kernel trace of this run (the call also uploads,
+    converts and downloads); the line reports the checks' errors."""
+    import numpy as np
+    from nirnaya_amd.engine import Engine
+    eng = Engine(0)
+    rng = np.random.default_rng(0)
+    shapes = {}
+    for (M, N, K) in ((256, 512, 512), (4096, 4096, 4096)):
+        A = rng.standard_normal((M, K)).astype(np.float32)
+        B = rng.standard_normal((N, K)).astype(np.float32)
+        for _ in range(3):
+            C = eng.gemm_check(A, B)
+        rows = slice(0, 8)
+        err = float(np.abs(C[rows] - A[rows].astype(np.float64) @ B.T.astype(np.float64)).max())
+        shapes[f"{M}x{N}x{K}"] = {"flop": 2.0 * M * N * K, "max_abs_diff_first_rows_vs_unrounded_fp64": err}
+        log(f"[bench] gemm {M}x{N}x{K}: {shapes[f'{M}x{N}x{K}']}")
+    eng.close()
+    print(json.dumps({"metric": "bf16 MFMA GEMM (kernel durations from a kernel trace of this run)",
+                      "value": None, "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "bf16",
+                      "data": "synthetic", "config": {"workload": "C = A B^T, N(0,1) inputs", "shapes": shapes}}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -577,7 +605,8 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the 256 MiB copy kernel (reference for rocprofv3 byte counters)")
-    ap.add_argument("--workload", default="large-lp", choices=["large-lp", "small-lp", "mixed", "planning512", "warmstart"],
+    ap.add_argument("--workload", default="large-lp",
+                    choices=["large-lp", "small-lp", "mixed", "planning512", "warmstart", "gemm"],
                     help="large-lp: the headline line (default); small-lp: the reference's CPU-runnable "
                          "2,000 x 5,000 case; mixed: mixed precision on the large instance; "
                          "planning512: the batched workload; warmstart: the learned warm start on a parametric "
@@ -602,6 +631,9 @@ def main():
         return
     if args.workload == "mixed":
         mixed_workload(args)
+        return
+    if args.workload == "gemm":
+        gemm_workload(args)
         return
     if args.workload == "small-lp":
         small_lp_workload(args)
