@@ -278,12 +278,15 @@ TileShape autotune(int nrows, int ncols, const std::vector<int>& rowptr, const i
         if (it != cache.end()) return it->second;
     }
     std::vector<TileShape> cands;
-    // Power-of-two segment widths plus the even split in two (a multiple of
-    // 1024 columns), which keeps an FP32 operand of a few MB per segment
-    // within one XCD's L2 in two passes.
+    // Power-of-two segment widths plus the even splits in two, three and
+    // four (multiples of 1024 columns): equal segments keep every pass's
+    // operand slice the same size (an FP32 operand of a few MB per segment
+    // fits one XCD's L2 in two passes).
     std::vector<int> widths = {0, 1 << 19, 1 << 18, 1 << 17};
-    const int half = static_cast<int>((static_cast<long long>(ncols) / 2 + 1023) / 1024 * 1024);
-    if (std::find(widths.begin(), widths.end(), half) == widths.end()) widths.push_back(half);
+    for (int parts = 2; parts <= 4; ++parts) {
+        const int w = static_cast<int>((static_cast<long long>(ncols) / parts + 1023) / 1024 * 1024);
+        if (w > 0 && std::find(widths.begin(), widths.end(), w) == widths.end()) widths.push_back(w);
+    }
     for (int w : widths)
         if (w == 0 || w < ncols)
             for (int t : {1024, 2048, 4096})
