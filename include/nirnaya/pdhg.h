@@ -152,6 +152,13 @@ NIR_API int nir_session_kkt(nir_session* s, double* out5);
  * structure, the segment passes and the fused epilogue add. No counterpart
  * in the reference. */
 NIR_API int nir_session_gather_floor(nir_session* s, int32_t which, int32_t reps, double* us);
+/* pdlp sessions: the engine's scaling factors, A' = diag(R) A diag(C) (R: nrow,
+ * C: ncol; either may be NULL) - 10 exponent-domain Ruiz passes and one
+ * Pock-Chambolle pass, every factor a power of two. This is the engine's own
+ * equilibration, applied after (and independent of) the caller's
+ * src/model/scaling.hpp:56-154 geometric scaling; no counterpart in the
+ * reference. Reference sessions: error. */
+NIR_API int nir_session_scaling(nir_session* s, double* R, double* C);
 NIR_API void nir_session_destroy(nir_session* s);
 
 /* Copies `bytes` (read) into a second buffer (write) with a plain grid-stride

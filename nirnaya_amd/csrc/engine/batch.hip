@@ -1456,6 +1456,8 @@ NIR_API int nir_batch_create_warm(nir_ctx* ctx, const nir_lp* const* lps, int32_
         return 0;
     } catch (const std::exception& ex) {
         ctx->error = ex.what();
+        // Setup kernels and uploads may still be queued against B's buffers.
+        if (B->s) (void)hipStreamSynchronize(B->s);
         return -1;
     }
 }
@@ -1616,6 +1618,10 @@ NIR_API int nir_batch_get_stats(nir_batch* B, nir_batch_stats* out) {
 NIR_API void nir_batch_destroy(nir_batch* B) {
     if (!B) return;
     (void)hipSetDevice(B->ctx->device);
+    // A solve that returned an error may leave a graph replay and the poll
+    // copy into the pinned word in flight; they use the buffers, graph and
+    // events the destructor frees.
+    (void)hipStreamSynchronize(B->s);
     delete B;
 }
 

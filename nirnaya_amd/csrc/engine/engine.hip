@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cstddef>
+#include <cstdint>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -265,13 +266,31 @@ float time_layout(int nrows, int ncols, const std::vector<int>& rowptr, const in
 // Autotuner: time y = M v for every candidate layout (segment width, tile,
 // fused or per-segment launches) on the real matrix, in the precision the
 // iterations will run in (FP32 for mixed precision), and keep the fastest.
-// Results are cached per matrix signature and precision; the choice never
-// changes results except for rows longer than a tile.
+// Results are cached per matrix (its shape, a hash of its row pointers and
+// of sampled column indices) and precision, so two matrices of one size (A
+// and the A^T of a square A, or two LPs solved in one process) are timed
+// separately. The choice never changes results: every row, of any length,
+// is summed in sequential column order.
+unsigned long long layout_signature(int nrows, const std::vector<int>& rowptr, const int* col) {
+    unsigned long long h = 0x243f6a8885a308d3ull;
+    auto mix = [&](unsigned long long v) {
+        h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+        h *= 0xff51afd7ed558ccdull;
+        h ^= h >> 33;
+    };
+    for (int r = 0; r <= nrows; ++r) mix(static_cast<unsigned long long>(static_cast<unsigned>(rowptr[r])));
+    const long long nnz = rowptr[nrows];
+    const long long step = std::max<long long>(1, nnz / 4096);
+    for (long long p = 0; p < nnz; p += step) mix(static_cast<unsigned long long>(static_cast<unsigned>(col[p])));
+    return h;
+}
+
 TileShape autotune(int nrows, int ncols, const std::vector<int>& rowptr, const int* col, const double* val,
                    hipStream_t s, bool fp32, bool allow_fused) {
     static std::mutex mu;
-    static std::map<std::tuple<int, int, int, int>, TileShape> cache;
-    const auto key = std::make_tuple(nrows, ncols, rowptr.back(), (fp32 ? 1 : 0) + (allow_fused ? 2 : 0));
+    static std::map<std::tuple<int, int, int, int, unsigned long long>, TileShape> cache;
+    const auto key = std::make_tuple(nrows, ncols, rowptr.back(), (fp32 ? 1 : 0) + (allow_fused ? 2 : 0),
+                                     layout_signature(nrows, rowptr, col));
     {
         std::lock_guard<std::mutex> lock(mu);
         auto it = cache.find(key);
@@ -644,6 +663,10 @@ NIR_API int nir_session_create(nir_ctx* ctx, const nir_lp* lp, const nir_params*
         S->nnz = lp->ncol > 0 ? lp->col_start[lp->ncol] : 0;
         S->obj_offset = lp->obj_offset;
         S->max_iters = std::max<long long>(0, S->prm.max_iters);
+        // The reference loop counts iterations in its index type (int32,
+        // src/core/types.hpp:12), and so does the device state here.
+        if (S->prm.mode == NIR_MODE_REFERENCE && S->max_iters > INT32_MAX)
+            throw std::invalid_argument("reference mode: max_iters above 2^31 - 1");
         const TileShape base = shape_from_params(S->prm, false);
         const TileShape base_t = shape_from_params(S->prm, true);
 
@@ -713,6 +736,9 @@ NIR_API int nir_session_create(nir_ctx* ctx, const nir_lp* lp, const nir_params*
     } catch (const std::exception& ex) {
         ctx->error = ex.what();
         g_last_error = ex.what();
+        // Work already queued on the stream (uploads, autotune products, the
+        // power iteration) still reads the buffers S owns: drain it first.
+        if (S->s) (void)hipStreamSynchronize(S->s);
         return -1;
     }
 }
@@ -978,6 +1004,19 @@ NIR_API int nir_calibrate_gather(nir_ctx* ctx, int64_t n, int32_t len, int32_t w
         return 0;
     } catch (const std::exception& ex) {
         ctx->error = ex.what();
+        return -1;
+    }
+}
+
+NIR_API int nir_session_scaling(nir_session* S, double* R, double* C) {
+    if (!S) return -1;
+    try {
+        HIP_CHECK(hipSetDevice(S->ctx->device));
+        if (!S->pdlp) throw std::invalid_argument("scaling factors exist in pdlp mode only");
+        pdlp_scaling(*S->pdlp, S->s, R, C);
+        return 0;
+    } catch (const std::exception& ex) {
+        S->ctx->error = ex.what();
         return -1;
     }
 }

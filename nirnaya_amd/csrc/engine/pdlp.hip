@@ -1056,6 +1056,13 @@ int pdlp_result(Pdlp& P, hipStream_t s, double* x, double* y, nir_info* info) {
     return 0;
 }
 
+// The accumulated Ruiz + Pock-Chambolle factors (R: m, C: n; either may be NULL).
+void pdlp_scaling(Pdlp& P, hipStream_t s, double* R, double* C) {
+    if (R) P.R.down(R, static_cast<size_t>(P.m), s);
+    if (C) P.C.down(C, static_cast<size_t>(P.n), s);
+    PD_CHECK(hipStreamSynchronize(s));
+}
+
 int pdlp_time(Pdlp& P, hipStream_t s, long long iters, nir_timing* out) {
     // Events around every iteration kernel of eval_freq-blocks (all segment
     // passes of a product between one pair of events).

@@ -42,6 +42,7 @@ void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm,
 void pdlp_run(Pdlp& P, hipStream_t s, const nir_params& prm, long long budget);
 int pdlp_result(Pdlp& P, hipStream_t s, double* x, double* y, nir_info* info);
 int pdlp_time(Pdlp& P, hipStream_t s, long long iters, nir_timing* out);
+void pdlp_scaling(Pdlp& P, hipStream_t s, double* R, double* C);
 
 }  // namespace gpu
 }  // namespace nir

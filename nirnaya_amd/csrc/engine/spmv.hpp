@@ -12,7 +12,8 @@
 //      order and the same rounding as a sequential CSR/CSC dot product, so
 //      results are bit-identical to it;
 //   3. the caller's epilogue consumes the row sum (fused vector update).
-// Rows longer than TILE use strided lane partials and a fixed-order LDS tree.
+// A row longer than TILE is a block of its own and is summed as a chain of
+// tiles with the same sequential order.
 //
 // Column segments: a matrix may be split by column ranges into S segments
 // (each a CSR over all rows, stored segment-major in one value/index array).
@@ -92,20 +93,45 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
     const int cnt = p1 - p0;
     if (r1 == r0) return;   // empty matrix: the one block has no rows
     if (cnt > TILE) {
-        // One long row: lane partials then a fixed-order tree.
-        T s = T(0);
-        for (int p = p0 + tid; p < p1; p += kThreads) s += load_stream(A.val + p) * static_cast<T>(v[load_stream(A.col + p)]);
-        prod[tid] = s;
-        __syncthreads();
-        for (int w = kThreads / 2; w > 0; w >>= 1) {
-            if (tid < w) prod[tid] += prod[tid + w];
+        // One row longer than a tile: a chain of tiles. All lanes park one
+        // tile of the row's products in LDS, then lane 0 adds them in column
+        // order to the running sum, so the row keeps the sequential order and
+        // rounding of a CSR dot product at any length. The next tile's index
+        // and value loads are issued before lane 0 sums the current one.
+        constexpr int PER = TILE / kThreads;
+        T s = P.first ? T(0) : P.part[r0];   // lane 0's copy is the one used
+        int ci[PER];
+        T vi[PER];
+        auto load = [&](int c0) {
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int q = c0 + tid + u * kThreads;
+                const bool in = q < p1;
+                const int slot = in ? q : 0;
+                const int cl = load_stream(A.col + slot);
+                const T vl = load_stream(A.val + slot);
+                ci[u] = in ? cl : 0;
+                vi[u] = in ? vl : T(0);
+            }
+        };
+        load(p0);
+        for (int c0 = p0; c0 < p1; c0 += TILE) {
+            T g[PER];
+#pragma unroll
+            for (int u = 0; u < PER; ++u) g[u] = static_cast<T>(v[ci[u]]);
+#pragma unroll
+            for (int u = 0; u < PER; ++u) prod[lds_at(tid + u * kThreads)] = vi[u] * g[u];
+            __syncthreads();
+            if (c0 + TILE < p1) load(c0 + TILE);
+            if (tid == 0) {
+                const int e = p1 - c0 < TILE ? p1 - c0 : TILE;
+                for (int q = 0; q < e; ++q) s += prod[lds_at(q)];
+            }
             __syncthreads();
         }
         if (tid == 0) {
-            const T init = P.first ? T(0) : P.part[r0];
-            const T sum = init + prod[0];
-            if (P.last) on_row(r0, sum, prefetch(r0));
-            else P.part[r0] = sum;
+            if (P.last) on_row(r0, s, prefetch(r0));
+            else P.part[r0] = s;
         }
         return;
     }

@@ -126,6 +126,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.nir_session_shape.argtypes = [vp, _i32p]
     lib.nir_session_kkt.argtypes = [vp, _dp]
     lib.nir_session_gather_floor.argtypes = [vp, C.c_int32, C.c_int32, _dp]
+    lib.nir_session_scaling.argtypes = [vp, _dp, _dp]
     lib.nir_calibrate_copy.argtypes = [vp, C.c_int64, _dp]
     lib.nir_calibrate_gather.argtypes = [vp, C.c_int64, C.c_int32, C.c_int32, C.c_int32, _dp]
     lib.nir_pdhg_solve.argtypes = [vp, C.POINTER(NirLp), C.POINTER(NirParams), _dp, _dp, C.POINTER(NirInfo)]
@@ -489,6 +490,14 @@ class Session:
         if self.lib.nir_session_kkt(self.h, out.ctypes.data_as(C.POINTER(C.c_double))) != 0:
             raise EngineError("nir_session_kkt: " + self.eng.error())
         return out
+
+    def scaling(self):
+        """(R, C): the pdlp engine's own Ruiz + Pock-Chambolle factors."""
+        R = np.zeros(self.lp.m)
+        Cc = np.zeros(self.lp.n)
+        if self.lib.nir_session_scaling(self.h, _ptr(R, C.c_double), _ptr(Cc, C.c_double)) != 0:
+            raise EngineError("nir_session_scaling: " + self.eng.error())
+        return R, Cc
 
     def gather_floor(self, which: int, reps: int = 20) -> float:
         """Microseconds per product (0: A xbar, 1: A^T y) of the access-only

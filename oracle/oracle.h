@@ -20,6 +20,13 @@
  *   or_pdhg_steps    the loop of :59-101 in chunks    chunked run = uninterrupted run
  *   or_dd_verify     src/io/sol_io.hpp:25-106         KAT unit_tests.cpp:29-42,
  *                                                     README.md:84 (2.55e-14)
+ *   or_kkt           build-defined (PDLP relative     zero at the A.5 optimum; closed-form
+ *                    KKT, caller units)               nonzero terms at perturbed points of
+ *                                                     the A.5 and unit_tests.cpp:316-331 LPs
+ *   or_scale_pdlp    build-defined (Ruiz + Pock-      exact powers of two; inf-norm
+ *                    Chambolle)                       equilibrium after the Ruiz passes;
+ *                                                     hand-computed factors; the final
+ *                                                     pass = one PC pass on the Ruiz result
  *   or_pdlp          build-defined restarted PDHG     KKT = 0 at known optima,
  *                                                     scaling invariance, HiGHS optimum;
  *                    + ray certificates at restarts   status 21 / 22 exactly on LPs
@@ -91,6 +98,12 @@ typedef struct {
     int64_t iterations;
     double omega, eta, rel_p, rel_d, rel_g, pobj, dobj;
 } or_pdlp_info;
+
+/* The pdlp mode's scaling factors: `ruiz_passes` exponent-domain Ruiz passes
+ * (rows by max |a|, then columns of the row-scaled matrix), then `pc_passes`
+ * Pock-Chambolle passes (alpha = 1: 1-norms). The engine uses 10 + 1.
+ * R (m) and C (n) are exact powers of two; scaled A' = diag(R) A diag(C). */
+void or_scale_pdlp(const or_lp* lp, int32_t ruiz_passes, int32_t pc_passes, double* R, double* C);
 
 /* The engine's pdlp mode in FP64 (see pdhg_oracle.c). x (n) and y (m) are
  * returned in the LP's own units. */

@@ -593,70 +593,104 @@ static int cert_check(const pmat* M, const double* c, const double* lo, const do
     return 0;
 }
 
-void or_pdlp(const or_lp* lp, const or_pdlp_params* prm, double* x_out, double* y_out, or_pdlp_info* info) {
+/* CSR(A) beside the LP's CSC, both with their own (to be scaled) values. */
+static void pmat_build(const or_lp* lp, pmat* Mt) {
     const int32_t m = lp->m, n = lp->n;
     const int64_t nnz = n > 0 ? lp->start[n] : 0;
-    const size_t M1 = (size_t)(m > 0 ? m : 1), N1 = (size_t)(n > 0 ? n : 1), Z1 = (size_t)(nnz > 0 ? nnz : 1);
-    pmat Mt;
-    Mt.m = m;
-    Mt.n = n;
-    Mt.rp = calloc(M1 + 1, sizeof(int32_t));
-    Mt.ci = malloc(Z1 * sizeof(int32_t));
-    Mt.av = malloc(Z1 * sizeof(double));
-    Mt.cs = lp->start;
-    Mt.ri = lp->index;
-    Mt.tv = malloc(Z1 * sizeof(double));
+    const size_t M1 = (size_t)(m > 0 ? m : 1), Z1 = (size_t)(nnz > 0 ? nnz : 1);
+    Mt->m = m;
+    Mt->n = n;
+    Mt->rp = calloc(M1 + 1, sizeof(int32_t));
+    Mt->ci = malloc(Z1 * sizeof(int32_t));
+    Mt->av = malloc(Z1 * sizeof(double));
+    Mt->cs = lp->start;
+    Mt->ri = lp->index;
+    Mt->tv = malloc(Z1 * sizeof(double));
     for (int64_t p = 0; p < nnz; ++p) {
-        Mt.rp[lp->index[p] + 1]++;
-        Mt.tv[p] = lp->value[p];
+        Mt->rp[lp->index[p] + 1]++;
+        Mt->tv[p] = lp->value[p];
     }
-    for (int32_t i = 0; i < m; ++i) Mt.rp[i + 1] += Mt.rp[i];
-    {
-        int32_t* fill = malloc(M1 * sizeof(int32_t));
-        for (int32_t i = 0; i < m; ++i) fill[i] = Mt.rp[i];
-        for (int32_t j = 0; j < n; ++j)
-            for (int64_t p = lp->start[j]; p < lp->start[j + 1]; ++p) {
-                const int32_t q = fill[lp->index[p]]++;
-                Mt.ci[q] = j;
-                Mt.av[q] = lp->value[p];
-            }
-        free(fill);
-    }
-    /* scaling */
-    double* R = malloc(M1 * sizeof(double));
-    double* C = malloc(N1 * sizeof(double));
+    for (int32_t i = 0; i < m; ++i) Mt->rp[i + 1] += Mt->rp[i];
+    int32_t* fill = malloc(M1 * sizeof(int32_t));
+    for (int32_t i = 0; i < m; ++i) fill[i] = Mt->rp[i];
+    for (int32_t j = 0; j < n; ++j)
+        for (int64_t p = lp->start[j]; p < lp->start[j + 1]; ++p) {
+            const int32_t q = fill[lp->index[p]]++;
+            Mt->ci[q] = j;
+            Mt->av[q] = lp->value[p];
+        }
+    free(fill);
+}
+
+static void pmat_free(pmat* Mt) {
+    free(Mt->rp);
+    free(Mt->ci);
+    free(Mt->av);
+    free(Mt->tv);
+}
+
+/* The pdlp mode's scaling, applied to Mt's values in place: `ruiz` passes of
+ * exponent-domain Ruiz equilibration (factor 2^-floor((e+1)/2) of each row's
+ * max |a|, e = floor(log2), then of each column's max |a| on the row-scaled
+ * matrix), then `pc` Pock-Chambolle passes with alpha = 1 (the same with the
+ * 1-norms sum |a| instead of the maxima). R (m) and C (n) receive the
+ * accumulated factors, every one an exact power of two, so the scaled
+ * values are a_ij * R_i * C_j exactly. */
+static void pdlp_scale(pmat* Mt, const or_lp* lp, int ruiz, int pc, double* R, double* C) {
+    const int32_t m = lp->m, n = lp->n;
+    const int64_t nnz = n > 0 ? lp->start[n] : 0;
+    const size_t M1 = (size_t)(m > 0 ? m : 1), N1 = (size_t)(n > 0 ? n : 1);
     for (int32_t i = 0; i < m; ++i) R[i] = 1.0;
     for (int32_t j = 0; j < n; ++j) C[j] = 1.0;
     double* f = malloc((M1 > N1 ? M1 : N1) * sizeof(double));
-    for (int pass = 0; pass < 11; ++pass) {
-        const int sum = pass >= 10;
+    for (int pass = 0; pass < ruiz + pc; ++pass) {
+        const int sum = pass >= ruiz;
         for (int32_t i = 0; i < m; ++i) {
             double acc = 0.0;
-            for (int32_t p = Mt.rp[i]; p < Mt.rp[i + 1]; ++p) {
-                const double a = fabs(Mt.av[p]);
+            for (int32_t p = Mt->rp[i]; p < Mt->rp[i + 1]; ++p) {
+                const double a = fabs(Mt->av[p]);
                 acc = sum ? acc + a : smax(acc, a);
             }
             f[i] = pow2_inv_sqrt(acc);
             R[i] *= f[i];
         }
         for (int32_t i = 0; i < m; ++i)
-            for (int32_t p = Mt.rp[i]; p < Mt.rp[i + 1]; ++p) Mt.av[p] = Mt.av[p] * (f[i] * 1.0);
-        for (int64_t p = 0; p < nnz; ++p) Mt.tv[p] = Mt.tv[p] * (1.0 * f[lp->index[p]]);
+            for (int32_t p = Mt->rp[i]; p < Mt->rp[i + 1]; ++p) Mt->av[p] = Mt->av[p] * (f[i] * 1.0);
+        for (int64_t p = 0; p < nnz; ++p) Mt->tv[p] = Mt->tv[p] * (1.0 * f[lp->index[p]]);
         for (int32_t j = 0; j < n; ++j) {
             double acc = 0.0;
             for (int64_t p = lp->start[j]; p < lp->start[j + 1]; ++p) {
-                const double a = fabs(Mt.tv[p]);
+                const double a = fabs(Mt->tv[p]);
                 acc = sum ? acc + a : smax(acc, a);
             }
             f[j] = pow2_inv_sqrt(acc);
             C[j] *= f[j];
         }
         for (int32_t j = 0; j < n; ++j)
-            for (int64_t p = lp->start[j]; p < lp->start[j + 1]; ++p) Mt.tv[p] = Mt.tv[p] * (f[j] * 1.0);
+            for (int64_t p = lp->start[j]; p < lp->start[j + 1]; ++p) Mt->tv[p] = Mt->tv[p] * (f[j] * 1.0);
         for (int32_t i = 0; i < m; ++i)
-            for (int32_t p = Mt.rp[i]; p < Mt.rp[i + 1]; ++p) Mt.av[p] = Mt.av[p] * (1.0 * f[Mt.ci[p]]);
+            for (int32_t p = Mt->rp[i]; p < Mt->rp[i + 1]; ++p) Mt->av[p] = Mt->av[p] * (1.0 * f[Mt->ci[p]]);
     }
     free(f);
+}
+
+void or_scale_pdlp(const or_lp* lp, int32_t ruiz_passes, int32_t pc_passes, double* R, double* C) {
+    pmat Mt;
+    pmat_build(lp, &Mt);
+    pdlp_scale(&Mt, lp, ruiz_passes, pc_passes, R, C);
+    pmat_free(&Mt);
+}
+
+void or_pdlp(const or_lp* lp, const or_pdlp_params* prm, double* x_out, double* y_out, or_pdlp_info* info) {
+    const int32_t m = lp->m, n = lp->n;
+    const int64_t nnz = n > 0 ? lp->start[n] : 0;
+    const size_t M1 = (size_t)(m > 0 ? m : 1), N1 = (size_t)(n > 0 ? n : 1);
+    pmat Mt;
+    pmat_build(lp, &Mt);
+    /* scaling: 10 Ruiz passes, then one Pock-Chambolle pass */
+    double* R = malloc(M1 * sizeof(double));
+    double* C = malloc(N1 * sizeof(double));
+    pdlp_scale(&Mt, lp, 10, 1, R, C);
     /* vectors in the scaled space */
     double *c = malloc(N1 * 8), *lo = malloc(N1 * 8), *up = malloc(N1 * 8), *ct = malloc(N1 * 8);
     double *rl = malloc(M1 * 8), *ru = malloc(M1 * 8), *rt = malloc(M1 * 8);
@@ -803,7 +837,7 @@ void or_pdlp(const or_lp* lp, const or_pdlp_params* prm, double* x_out, double* 
         info->pobj = s.pobj;
         info->dobj = s.dobj;
     }
-    free(Mt.rp); free(Mt.ci); free(Mt.av); free(Mt.tv); free(R); free(C);
+    pmat_free(&Mt); free(R); free(C);
     free(c); free(lo); free(up); free(ct); free(rl); free(ru); free(rt);
     free(x); free(xa); free(xp); free(xb); free(y); free(ya); free(yp); free(dxr); free(dyr);
 }

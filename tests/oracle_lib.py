@@ -61,6 +61,7 @@ def oracle():
         L.or_pdhg_steps.argtypes = [C.POINTER(OrLp), C.c_int64, C.c_double, C.c_double, _dp, _dp]
         L.or_dd_verify.argtypes = [C.POINTER(OrLp), _dp, C.c_double, C.c_double, _dp, _dp, _dp]
         L.or_kkt.argtypes = [C.POINTER(OrLp), _dp, _dp, _dp]
+        L.or_scale_pdlp.argtypes = [C.POINTER(OrLp), C.c_int32, C.c_int32, _dp, _dp]
         L.or_pdlp.argtypes = [C.POINTER(OrLp), C.POINTER(OrPdlpParams), _dp, _dp, C.POINTER(OrPdlpInfo)]
         _oracle = L
     return _oracle
@@ -114,6 +115,14 @@ class Oracle:
         Cc = np.zeros(self.lp.n)
         active = self.L.or_scale_geometric(C.byref(self.s), _p(R), _p(Cc))
         return bool(active), R, Cc
+
+    def scale_pdlp(self, ruiz=10, pc=1):
+        """Row and column factors of the pdlp mode's scaling (10 Ruiz passes and
+        one Pock-Chambolle pass unless asked otherwise)."""
+        R = np.zeros(self.lp.m)
+        Cc = np.zeros(self.lp.n)
+        self.L.or_scale_pdlp(C.byref(self.s), int(ruiz), int(pc), _p(R), _p(Cc))
+        return R, Cc
 
     def pdhg(self, K, check=True, x0=None, y0=None):
         x = np.zeros(self.lp.n)

@@ -212,3 +212,37 @@ def test_batch_certifies_infeasible_members(engine):
         for r, kind in zip(res, kinds):
             assert r.status == 1 if kind == "dual" else r.status in (2, 3)
         assert [r.status for r in res[len(bad):]] == [0, 0]
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_cfg3_all_512_planning_lps(engine, prec):
+    """The batched workload as specified: 512 multi-period planning LPs
+    (seeds 20263*1000 + i, T ~ U{4..48} periods x K ~ U{5..40} products) in
+    one persistent batch to relative KKT 1e-6, FP64 and mixed precision.
+    Every instance must end Optimal with the oracle's independent or_kkt
+    (double-double, on the host) at or below the tolerance; in FP64 the two
+    smallest members must equal or_pdlp's iterates bit for bit."""
+    lps = [gen_planning_lp(20263 * 1000 + i) for i in range(512)]
+    params = default_params(mode=MODE_PDLP, precision=prec, max_iters=400000, tol=1e-6)
+    b = engine.batch(lps, params)
+    try:
+        b.solve()
+        res = [b.result(i) for i in range(len(lps))]
+        st = b.stats()
+    finally:
+        b.close()
+    worst = 0.0
+    for i, (lp, r) in enumerate(zip(lps, res)):
+        assert r.status == 0, f"instance {i} status {r.status}"
+        k = Oracle(lp).kkt(r.x, r.y)
+        worst = max(worst, k[0], k[1], k[2])
+        assert max(k[0], k[1], k[2]) <= 1e-6, (i, k[:3])
+    print(f"prec {prec}: 512 optimal, worst or_kkt {worst:.3e}, iterations {st.total_iterations}, "
+          f"threads {st.threads}, solve {st.solve_seconds:.2f}s")
+    if prec == 0:
+        small = sorted(range(len(lps)), key=lambda i: lps[i].nnz)[:2]
+        for i in small:
+            xo, yo, info = Oracle(lps[i]).pdlp(max_iters=400000, tol=1e-6)
+            print(f"instance {i} (m={lps[i].m}, n={lps[i].n}): it {res[i].iterations}/{info.iterations}")
+            assert res[i].iterations == info.iterations and res[i].info.restarts == info.restarts
+            assert np.array_equal(res[i].x, xo) and np.array_equal(res[i].y, yo)

@@ -52,6 +52,35 @@ def test_fp64_pdlp_matches_oracle(engine, name):
         assert abs(k[3] - HIGHS[name]) <= 1e-4 * (1 + abs(HIGHS[name]))
 
 
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_device_scaling_factors_equal_oracle(engine, name):
+    """The device's Ruiz + Pock-Chambolle factors (k_row_measure, k_factor,
+    k_scale_csr) equal or_scale_pdlp's bit for bit."""
+    lp = CASES[name]()
+    s = engine.session(lp, default_params(mode=MODE_PDLP, max_iters=1, tol=1e-6))
+    try:
+        R, Cs = s.scaling()
+    finally:
+        s.close()
+    Ro, Co = Oracle(lp).scale_pdlp()
+    assert np.array_equal(R, Ro) and np.array_equal(Cs, Co)
+
+
+def test_device_scaling_factors_full_size_segmented(engine):
+    """The same on the 400k x 1.2M instance, with A in three column segments
+    and A^T in two (the measures and the scaling walk every segment)."""
+    lp = gen_mixed_lp(20261, 400_000, 1_200_000, 6.0, 0.01, 1e3)
+    s = engine.session(lp, default_params(mode=MODE_PDLP, max_iters=1, tol=1e-6, tile_nnz=2048,
+                                          seg_width=400_384, seg_width_t=200_704))
+    try:
+        R, Cs = s.scaling()
+        assert s.shape()[2] == 3 and s.shape()[6] == 2
+    finally:
+        s.close()
+    Ro, Co = Oracle(lp).scale_pdlp()
+    assert np.array_equal(R, Ro) and np.array_equal(Cs, Co)
+
+
 def test_segmented_pdlp_matches_oracle(engine):
     lp = gen_staircase_lp(3, 6, 8)
     r = engine.solve(lp, default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-6, seg_width=37, tile_rows=64))
@@ -194,3 +223,36 @@ def test_python_wrapper_reports_infeasible():
         m.set_option(k, v)
     m.optimize()
     assert m.Status == 1
+
+
+def test_cfg2_mixed_precision_full_instance(engine):
+    """Mixed precision on the 400k x 1.2M instance (m = 400,000, n = 1,200,000,
+    ~9.6M nonzeros, seed 20261) to relative KKT 1e-6: the returned (x, y)
+    is certified by the oracle's or_kkt (an independent double-double
+    evaluation on the host), and its error against the FP64 engine's
+    solution to the same tolerance is reported and bounded. The LP's
+    optimal face is not a single point, so x may differ along it by more
+    than the tolerance; the objective may differ by at most the two gaps."""
+    lp = gen_mixed_lp(20261, 400_000, 1_200_000, 6.0, 0.01, 1e3)
+    o = Oracle(lp)
+    mx = engine.solve(lp, default_params(mode=MODE_PDLP, precision=PREC_MIXED, max_iters=2_000_000, tol=1e-6))
+    km = o.kkt(mx.x, mx.y)
+    print(f"mixed: status {mx.status} it {mx.iterations} (fp64 {mx.info.polish_iterations}) restarts "
+          f"{mx.info.restarts} engine kkt {mx.info.rel_primal:.3e} {mx.info.rel_dual:.3e} {mx.info.rel_gap:.3e} "
+          f"or_kkt {km[0]:.3e} {km[1]:.3e} {km[2]:.3e}")
+    assert mx.status == 0
+    assert max(km[0], km[1], km[2]) <= 1e-6
+    # The engine's own evaluation agrees (the gap's numerator P - D cancels, so
+    # its rounding shows at ~1e-15 of the objectives: compared absolutely).
+    for a, b in ((km[0], mx.info.rel_primal), (km[1], mx.info.rel_dual), (km[2], mx.info.rel_gap)):
+        assert abs(a - b) <= 1e-12
+    fp = engine.solve(lp, default_params(mode=MODE_PDLP, max_iters=2_000_000, tol=1e-6))
+    kf = o.kkt(fp.x, fp.y)
+    assert fp.status == 0 and max(kf[0], kf[1], kf[2]) <= 1e-6
+    ex = np.linalg.norm(mx.x - fp.x) / np.linalg.norm(fp.x)
+    ey = np.linalg.norm(mx.y - fp.y) / np.linalg.norm(fp.y)
+    eo = abs(km[3] - kf[3]) / (1 + abs(kf[3]))
+    print(f"fp64: it {fp.iterations}; mixed vs fp64: |x-x64|/|x64| {ex:.2e} |y-y64|/|y64| {ey:.2e} "
+          f"objective {eo:.2e}")
+    assert ex <= 1e-2 and ey <= 1e-2
+    assert eo <= 4e-6

@@ -3,8 +3,8 @@
 The engine's CSR-stream kernels sum every row in ascending column order from
 0.0 with products rounded before the add (no FMA contraction), exactly as the
 reference's sequential loops do, so iterates are expected to agree bit for bit;
-the acceptance bar is 1e-10 relative (rows longer than one LP tile use a tree
-reduction and are held to that bar only).
+rows longer than one LP tile are summed as a chain of tiles in the same order,
+so every row length is held to bit equality.
 """
 import ctypes as C
 
@@ -129,13 +129,52 @@ def test_edge_shapes(engine):
     lp = LP.from_dense(A, rng.standard_normal(n), l, u, rl, ru)
     r = run_ref(engine, lp, 300)
     xo, yo, _, _, _ = Oracle(lp).pdhg(300, check=False)
-    assert rel(r.x, xo) <= 1e-10 and rel(r.y, yo) <= 1e-10
-    # Without the long row every sum keeps the sequential order: bit exact.
+    assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+    # Without the long row as well.
     A[m - 1, :] = 0.0
     lp2 = LP.from_dense(A, lp.c, l, u, rl, ru)
     r2 = run_ref(engine, lp2, 300)
     xo2, yo2, _, _, _ = Oracle(lp2).pdhg(300, check=False)
     assert np.array_equal(r2.x, xo2) and np.array_equal(r2.y, yo2)
+
+
+def _long_row_lp():
+    """12,000 x 10,500 LP with one 10,500-nonzero row (a long row of A) and
+    one 12,000-nonzero column (a long row of A^T); every other column has 3
+    random rows."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(11)
+    m, n = 12_000, 10_500
+    rows = [rng.choice(m, size=3, replace=False) for _ in range(n)]
+    ri = np.concatenate(rows)
+    ci = np.repeat(np.arange(n), 3)
+    dense_row, dense_col = 3, 5
+    ri = np.concatenate([ri, np.full(n, dense_row), np.arange(m)])
+    ci = np.concatenate([ci, np.arange(n), np.full(m, dense_col)])
+    A = sp.csc_matrix((rng.standard_normal(ri.size), (ri, ci)), shape=(m, n))
+    A.sum_duplicates()
+    A.sort_indices()
+    x0 = rng.uniform(0, 1, n)
+    b = A @ x0
+    c = A.T @ rng.standard_normal(m) + rng.uniform(0, 1, n)
+    assert np.diff(A.indptr).max() == m and np.diff(A.tocsr().indptr).max() >= n
+    return LP(m, n, A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data.astype(np.float64), c,
+              np.zeros(n), np.full(n, 10.0), b.copy(), b.copy())
+
+
+@pytest.mark.parametrize("tile,seg,fused", [(1024, 0, 0), (2048, 0, 0), (4096, 0, 0), (2048, 3000, 0),
+                                            (1024, 4096, 1)])
+def test_rows_longer_than_a_tile_bit_identical(engine, tile, seg, fused):
+    """Rows of 10,500 and 12,000 nonzeros (up to 12 tiles each): the chain
+    of tiles keeps the sequential sum, so iterates equal the oracle bit for
+    bit in every tile size and segment layout (a fused layout falls back to
+    per-segment launches where a row does not fit one tile per segment)."""
+    lp = _long_row_lp()
+    r = run_ref(engine, lp, 40, tile_nnz=tile, seg_width=seg, seg_fused=fused)
+    xo, yo, tau, _, V = Oracle(lp).pdhg(40, check=False)
+    assert r.tau == tau
+    assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+    assert r.info.max_primal_viol == V
 
 
 def test_warm_start_matches_oracle(engine):
@@ -257,3 +296,20 @@ def test_full_size_cfg1_parity(engine):
     xo, yo, tau, _, _ = Oracle(lp).pdhg(12, check=False)
     assert r.tau == tau
     assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+    # The layouts the autotuner picks on MI355X, set explicitly so they are
+    # covered whatever the timing picks: A in three equal segments of
+    # 400,384 columns and A^T in two of 200,704, with one launch per segment
+    # and with all segments walked in one launch.
+    for fused in (0, 1):
+        p = default_params(mode=MODE_REFERENCE, max_iters=12, check_termination=0, tile_nnz=2048,
+                           seg_width=400_384, seg_width_t=200_704, seg_fused=fused)
+        s = engine.session(lp, p)
+        try:
+            s.solve()
+            r = s.result()
+            shape = tuple(s.shape())
+        finally:
+            s.close()
+        assert shape == (2048, 256, 3, fused, 2048, 256, 2, fused), shape
+        assert r.tau == tau
+        assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo), fused

@@ -47,7 +47,7 @@ def _pipeline(engine, b_amp, c_amp, n_train=512, n_test=24):
         c = engine.solve(lp, params)
         x0, y0 = ws.predict(t)
         w, rec = solve_warm(engine, lp, params, x0, y0, ws.primal_weight)
-        assert c.status == 0 and w.status == 0
+        assert c.status == 0 and w.status == 0 and rec.status == 0
         fallbacks += rec.fallback
         cold.append(c.iterations)
         warm.append(rec.iterations)
@@ -58,11 +58,18 @@ def _pipeline(engine, b_amp, c_amp, n_train=512, n_test=24):
     return np.array(cold), np.array(warm), fallbacks
 
 
-def test_warm_start_saves_iterations_on_a_predictable_family(engine):
-    cold, warm, _ = _pipeline(engine, 0.001, 0.0005)
+def test_extra_small_amplitude_family_saves_iterations(engine):
+    """Extra family (b +-0.1 %, c +-0.05 %), not the specified amplitudes:
+    there the prediction is close and the median gain is asserted."""
+    cold, warm, fallbacks = _pipeline(engine, 0.001, 0.0005)
+    assert fallbacks == 0
     assert np.median(warm) <= 0.9 * np.median(cold)
 
 
 def test_warm_start_reaches_tolerance_at_default_amplitudes(engine):
+    """The specified amplitudes (b +-10 %, c +-5 %): every warm-started solve
+    must reach the tolerance on its own (no cold fallback); the iteration
+    change is reported."""
     cold, warm, fallbacks = _pipeline(engine, 0.1, 0.05, n_train=256, n_test=12)
-    print(f"iterations saved at default amplitudes: {1 - warm.sum() / cold.sum():+.1%} (reported, not asserted)")
+    assert fallbacks == 0
+    print(f"iterations saved at default amplitudes: {1 - warm.sum() / cold.sum():+.1%}")

@@ -112,6 +112,117 @@ def test_kkt_zero_at_known_optimum():
     assert Oracle(lp).kkt(np.array([3.0, 1.0]), np.array([-1.5, 0.5]))[1] > 0   # sign of a >= row dual
 
 
+def test_kkt_closed_form_terms():
+    """Nonzero relative-KKT terms at perturbed points, by hand.
+
+    A.5 LP (unit_tests.cpp:116-142): min 2x1 + 3x2, x1 + x2 >= 4,
+    x1 + 3x2 >= 6, x >= 0. At x = (2, 1), y = (1, 1):
+      Ax = (3, 5), proj onto [rl, inf) = (4, 6), residual (-1, -1): ||.|| = sqrt(2);
+      b = (|4|, |6|), ||b||_2 = sqrt(52)            -> primal sqrt(2) / (1 + sqrt(52));
+      lam = c - A^T y = (2, 3) - (2, 4) = (0, -1); x >= 0 allows lam >= 0 only,
+      violation (0, -1); ||c||_2 = sqrt(13)          -> dual 1 / (1 + sqrt(13));
+      P = 2*2 + 3*1 = 7, D = rl^T y^+ + l^T lam^+ = 4 + 6 + 0 = 10
+                                                     -> gap 3 / (1 + 7 + 10).
+    With offset 5 both objectives move by 5: gap 3 / (1 + 12 + 15).
+    Ranged LP (unit_tests.cpp:316-331): x in [0, 20]^2, rows [4, 20] and
+    [6, 30]. At x = (2, 1), y = (1, -1): residual (-1, -1), b_i = max(|rl|, |ru|)
+    = (20, 30) -> primal sqrt(2) / (1 + sqrt(1300)); lam = (2, 3) - (0, -2) = (2, 5)
+    with both column bounds finite -> dual 0; D = 4*1 + 30*(-1) + 0 = -26, P = 7
+    -> gap 33 / (1 + 7 + 26)."""
+    A = [[1.0, 1.0], [1.0, 3.0]]
+    inf = math.inf
+    lp = LP.from_dense(A, [2.0, 3.0], [0, 0], [inf, inf], [4.0, 6.0], [inf, inf])
+    k = Oracle(lp).kkt(np.array([2.0, 1.0]), np.array([1.0, 1.0]))
+    assert k[0] == pytest.approx(math.sqrt(2) / (1 + math.sqrt(52)), rel=1e-15)
+    assert k[1] == pytest.approx(1 / (1 + math.sqrt(13)), rel=1e-15)
+    assert k[2] == pytest.approx(3 / 18, rel=1e-15)
+    assert (k[3], k[4]) == (7.0, 10.0)
+    lp5 = LP.from_dense(A, [2.0, 3.0], [0, 0], [inf, inf], [4.0, 6.0], [inf, inf], offset=5.0)
+    k5 = Oracle(lp5).kkt(np.array([2.0, 1.0]), np.array([1.0, 1.0]))
+    assert k5[2] == pytest.approx(3 / 28, rel=1e-15) and (k5[3], k5[4]) == (12.0, 15.0)
+    r = Oracle(lpkit.pdhg_unit_lp()).kkt(np.array([2.0, 1.0]), np.array([1.0, -1.0]))
+    assert r[0] == pytest.approx(math.sqrt(2) / (1 + math.sqrt(1300)), rel=1e-15)
+    assert r[1] == 0.0
+    assert r[2] == pytest.approx(33 / 34, rel=1e-15)
+    assert (r[3], r[4]) == (7.0, -26.0)
+
+
+def test_pdlp_scaling_hand_computed():
+    """Factors of the pdlp scaling, derived by hand (f(v) = 2^-floor((e+1)/2),
+    e = floor(log2 v), rows first, then columns of the row-scaled matrix).
+
+    A = [[64, 1], [2, 1/4]]. Ruiz pass 1: row maxima 64, 2 -> f = 1/8, 1/2;
+    row-scaled [[8, 1/8], [1, 1/8]]; column maxima 8, 1/8 -> 1/4, 2; scaled
+    [[2, 1/4], [1/4, 1/4]]. Pass 2: rows 2, 1/4 -> 1/2, 2; [[1, 1/8], [1/2, 1/2]];
+    columns 1, 1/2 -> 1, 1. Pass 3 changes nothing: R = (1/16, 1), C = (1/4, 2).
+    Pock-Chambolle on that: row sums 9/8, 1 -> 1, 1; column sums 3/2, 5/8 -> 1, 1.
+    A = ones(1 x 16): Ruiz leaves it (maxima 1); Pock-Chambolle: row sum 16 ->
+    1/4, then column sums 1/4 -> 2. Columns-before-rows would give
+    R = (1/4, 2), C = (1/8, 1) after the first pass; 2-norms instead of maxima
+    would give R = 1/4, C = 2 on the ones row after two Ruiz passes."""
+    lp = LP.from_dense([[64.0, 1.0], [2.0, 0.25]], [0, 0], [0, 0], [1, 1], [0, 0], [1, 1])
+    o = Oracle(lp)
+    R, Cc = o.scale_pdlp(1, 0)
+    assert R.tolist() == [1 / 8, 1 / 2] and Cc.tolist() == [1 / 4, 2.0]
+    for ruiz, pc in ((2, 0), (3, 0), (10, 0), (10, 1)):
+        R, Cc = o.scale_pdlp(ruiz, pc)
+        assert R.tolist() == [1 / 16, 1.0] and Cc.tolist() == [1 / 4, 2.0], (ruiz, pc)
+    ones = LP.from_dense(np.ones((1, 16)), np.zeros(16), np.zeros(16), np.ones(16), [0.0], [1.0])
+    o = Oracle(ones)
+    R, Cc = o.scale_pdlp(10, 0)
+    assert R.tolist() == [1.0] and Cc.tolist() == [1.0] * 16
+    R, Cc = o.scale_pdlp(0, 1)
+    assert R.tolist() == [0.25] and Cc.tolist() == [2.0] * 16
+    R, Cc = o.scale_pdlp(10, 1)
+    assert R.tolist() == [0.25] and Cc.tolist() == [2.0] * 16
+
+
+def _pow2_round_inv_sqrt(v):
+    """2^-floor((e+1)/2) with e = floor(log2 v): the power of two within a
+    factor sqrt(2) of 1/sqrt(v) (1 for v = 0)."""
+    out = np.ones_like(v)
+    pos = v > 0
+    e = np.frexp(v[pos])[1] - 1
+    out[pos] = np.ldexp(1.0, -np.floor_divide(e + 1, 2))
+    return out
+
+
+@pytest.mark.parametrize("make", [
+    lambda: gen_mixed_lp(5, 300, 900, 6.0, 0.01, 1e3),
+    lambda: gen_equality_lp(3, 200, 500, 5),
+    lambda: gen_staircase_lp(3, 6, 8),
+])
+def test_pdlp_scaling_invariants(make):
+    """Exact powers of two; after the 10 Ruiz passes the matrix is at the
+    rounded Ruiz fixed point (another pass changes nothing, and every row's
+    and column's max |a| lies in [1/2, 2)); the Pock-Chambolle pass is one
+    row-then-column 1-norm step on the Ruiz-scaled matrix; and the scaled
+    LP has the same objective values (c' = c C, x' = x / C)."""
+    import scipy.sparse as sp
+    lp = make()
+    o = Oracle(lp)
+    A = sp.csc_matrix((lp.value, lp.index, lp.start), shape=(lp.m, lp.n))
+    R10, C10 = o.scale_pdlp(10, 0)
+    R11, C11 = o.scale_pdlp(10, 1)
+    for v in np.concatenate([R10, C10, R11, C11]):
+        mant, _ = math.frexp(v)
+        assert mant == 0.5
+    S = abs(sp.diags(R10) @ A @ sp.diags(C10)).tocsr()
+    rmax = S.max(axis=1).toarray().ravel()
+    cmax = S.max(axis=0).toarray().ravel()
+    nzr, nzc = np.diff(S.indptr) > 0, np.diff(S.tocsc().indptr) > 0
+    assert np.all((rmax[nzr] >= 0.5) & (rmax[nzr] < 2.0))
+    assert np.all((cmax[nzc] >= 0.5) & (cmax[nzc] < 2.0))
+    R11r, C11r = o.scale_pdlp(11, 0)
+    assert np.array_equal(R11r, R10) and np.array_equal(C11r, C10)
+    fr = _pow2_round_inv_sqrt(np.asarray(S.sum(axis=1)).ravel())
+    S2 = sp.diags(fr) @ S
+    fc = _pow2_round_inv_sqrt(np.asarray(S2.sum(axis=0)).ravel())
+    assert np.array_equal(R11, R10 * fr) and np.array_equal(C11, C10 * fc)
+    x = np.random.default_rng(0).uniform(0, 1, lp.n)
+    assert np.dot(lp.c * C11, x / C11) == pytest.approx(np.dot(lp.c, x), rel=1e-14)
+
+
 def test_scaling_invariants():
     lp = gen_mixed_lp(5, 300, 900, 6.0, 0.01, 1e3)
     active, R, Cs = Oracle(lp).scale()
