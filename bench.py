@@ -92,7 +92,7 @@ def plain_to_tolerance(eng, lp, cap_s, tol=1e-6, chunk=8192):
     chunks only (the evaluations are not part of the plain loop)."""
     import numpy as np
     from nirnaya_amd.engine import MODE_REFERENCE, default_params
-    p = default_params(mode=MODE_REFERENCE, max_iters=10 ** 12, check_termination=1, tile_nnz=-1)
+    p = default_params(mode=MODE_REFERENCE, max_iters=2 ** 31 - 1, check_termination=1, tile_nnz=-1)
     s = eng.session(lp, p)
     solve = 0.0
     trace = []
@@ -311,7 +311,7 @@ def small_lp_workload(args):
                                        "same_iterates": bool(np.array_equal(rb.x, r.x) and np.array_equal(rb.y, r.y))}
         log(f"[bench] GPU {name} as a batch of one: {gpu[name + '_batch_of_one']}")
     # reference form on the GPU, KKT on the device between chunks
-    s = eng.session(lp, default_params(mode=MODE_REFERENCE, max_iters=10 ** 12, check_termination=0))
+    s = eng.session(lp, default_params(mode=MODE_REFERENCE, max_iters=2 ** 31 - 1, check_termination=0))
     solve, it, k = 0.0, 0, np.ones(5)
     while it < cap and max(k[:3]) > tol:
         step = min(chunk, cap - it)

@@ -46,6 +46,8 @@ EngineRun run_engine(const Model& lp, const Options& opt, const Deadline& deadli
     // Reference cap: max(5000, simplex_iter_limit / 10) (pdhg_solver.hpp:60).
     long long cap = std::max<long long>(5000, opt.simplex_iter_limit / 10);
     if (opt.pdhg.max_iters > 0) cap = opt.pdhg.max_iters;
+    // The reference loop's counter is an Idx (int32, src/core/types.hpp:12).
+    if (!pdlp) cap = std::min<long long>(cap, 2147483647LL);
     p.max_iters = cap;
     p.tol = opt.pdhg.tol;
     p.polish_tol = opt.pdhg.polish_tol;
