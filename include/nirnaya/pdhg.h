@@ -275,6 +275,32 @@ typedef struct {
 NIR_API int nir_verify_solution(nir_ctx* ctx, const nir_lp* lp, const double* x, double claimed_obj, double tol,
                                 nir_verify_report* out);
 
+/* Allocation-free variant of nir_verify_solution, for callers that cannot
+ * call hipMalloc (tight serving loops, code that must not allocate). It has
+ * no counterpart in the reference. The LP and x are already on the device, as
+ * CSR (row_start int32[nrow + 1], col_index int32[nnz] ascending within each
+ * row, value double[nnz]) plus the bound and cost vectors; the same kernels
+ * as nir_verify_solution compute the same report bit for bit. Block partials
+ * live in a static device array of 3 MiB per GPU (held until the process
+ * ends), so nrow and ncol must be at most 16,777,216 (-1 otherwise) and the
+ * entry point calls no hipMalloc / hipFree. Every call on one GPU shares that
+ * array: calls that overlap on one GPU (two contexts, two threads) race. */
+typedef struct {
+    int32_t nrow;
+    int32_t ncol;
+    const int32_t* row_start; /* device, nrow + 1 */
+    const int32_t* col_index; /* device, nnz */
+    const double* value;      /* device, nnz */
+    const double* cost;       /* device, ncol */
+    const double* collo;      /* device, ncol */
+    const double* colup;      /* device, ncol */
+    const double* rowlo;      /* device, nrow */
+    const double* rowup;      /* device, nrow */
+    double obj_offset;
+} nir_dev_lp;
+NIR_API int nir_verify_solution_noalloc(nir_ctx* ctx, const nir_dev_lp* lp, const double* x_dev,
+                                        double claimed_obj, double tol, nir_verify_report* out);
+
 /* ---- synthetic instances (seeded; generated on the host) ---------------- */
 typedef struct nir_owned_lp nir_owned_lp;
 

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -193,6 +194,89 @@ __global__ __launch_bounds__(256) void k_kkt_cols(int n, int nseg, const int* __
     block_sum4(t, sh, part);
 }
 
+// ------------------------------------------------------------------ allocation-free certifier
+
+// Largest nrow / ncol of nir_verify_solution_noalloc: at most kNoAllocBlocks
+// 256-lane blocks per pass, so its block partials fit the static scratch.
+constexpr int kNoAllocBlocks = 65536;
+constexpr long long kNoAllocMaxDim = static_cast<long long>(kNoAllocBlocks) * 256;
+
+// Result of the final reduction, read back by the host.
+struct VerifyOut {
+    double max_row, max_bound, obj;
+    int row_at, col_at, nonfinite, pad;
+};
+
+// Scratch of the allocation-free entry point: row maxima, column maxima,
+// objective dd partials (one slot per block each), the non-finite column
+// index and the reduced result. One copy per GPU, shared by every call on it.
+constexpr size_t kScratchRowOff = 0;
+constexpr size_t kScratchColOff = kScratchRowOff + sizeof(MaxAt) * kNoAllocBlocks;
+constexpr size_t kScratchObjOff = kScratchColOff + sizeof(MaxAt) * kNoAllocBlocks;
+constexpr size_t kScratchNfOff = kScratchObjOff + sizeof(dd) * kNoAllocBlocks;
+constexpr size_t kScratchOutOff = kScratchNfOff + 256;
+constexpr size_t kScratchBytes = kScratchOutOff + 256;
+static_assert(sizeof(VerifyOut) <= 256, "result slot");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+__global__ void k_set_int(int* p, int v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// Reduces the block partials in the order the host loop of
+// nir_verify_solution does: the largest positive violation, the first block
+// attaining it; the objective as offset + partials added in block order with
+// the same double-double sum.
+__global__ __launch_bounds__(256) void k_verify_final(int gr, int gc, const MaxAt* __restrict__ prow,
+                                                      const MaxAt* __restrict__ pcol, const dd* __restrict__ pobj,
+                                                      const int* __restrict__ nonfinite, double offset,
+                                                      VerifyOut* out) {
+    __shared__ MaxAt sh[256];
+    __shared__ int shb[256];
+    auto reduce = [&](const MaxAt* part, int count, double& v, int& at) {
+        MaxAt best{0.0, -1};
+        int bb = -1;
+        for (int b = threadIdx.x; b < count; b += 256)
+            if (part[b].at >= 0 && part[b].v > best.v) {
+                best = part[b];
+                bb = b;
+            }
+        sh[threadIdx.x] = best;
+        shb[threadIdx.x] = bb;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) {
+                const MaxAt o = sh[threadIdx.x + w];
+                const int ob = shb[threadIdx.x + w];
+                const MaxAt c = sh[threadIdx.x];
+                const int cb = shb[threadIdx.x];
+                if (o.v > c.v || (o.v == c.v && ob >= 0 && (cb < 0 || ob < cb))) {
+                    sh[threadIdx.x] = o;
+                    shb[threadIdx.x] = ob;
+                }
+            }
+            __syncthreads();
+        }
+        v = sh[0].v;
+        at = shb[0] >= 0 ? sh[0].at : -1;
+        __syncthreads();
+    };
+    double vr, vc;
+    int ar, ac;
+    reduce(prow, gr, vr, ar);
+    reduce(pcol, gc, vc, ac);
+    if (threadIdx.x == 0) {
+        dd acc{offset, 0.0};
+        for (int b = 0; b < gc; ++b) acc = dd_add(acc, pobj[b]);
+        out->max_row = vr;
+        out->row_at = ar;
+        out->max_bound = vc;
+        out->col_at = ac;
+        out->obj = acc.hi + acc.lo;
+        out->nonfinite = *nonfinite;
+    }
+}
+
 }  // namespace verify
 
 void kkt_device(hipStream_t s, const CsrShape& A, const double* aval, const CsrShape& AT, const double* atval,
@@ -236,6 +320,29 @@ void kkt_device(hipStream_t s, const CsrShape& A, const double* aval, const CsrS
 
 using namespace nir::gpu;
 using namespace nir::gpu::verify;
+
+namespace {
+
+// Device address of `scratch` on each GPU, looked up once per device with
+// that device current. Host code uses only these addresses: the array's name
+// in host code is a host-side stand-in, not the device address.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kMaxDevices] = {};
+
+unsigned char* noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) throw std::invalid_argument("device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* ptr = nullptr;
+        HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw std::runtime_error("hipGetSymbolAddress(scratch) returned null");
+        g_scratch[device] = static_cast<unsigned char*>(ptr);
+    }
+    return g_scratch[device];
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -314,6 +421,63 @@ NIR_API int nir_verify_solution(nir_ctx* ctx, const nir_lp* lp, const double* x,
         DDHost acc(lp->obj_offset);
         for (int b = 0; b < gc && n > 0; ++b) acc.add(ho[b].hi, ho[b].lo);
         out->computed_obj = acc.value();
+        out->obj_rel_err = std::isfinite(claimed_obj)
+                               ? std::fabs(out->computed_obj - claimed_obj) / (1.0 + std::fabs(out->computed_obj))
+                               : 0.0;
+        out->passed = out->nonfinite_col < 0 && out->max_bound_viol <= tol && out->max_row_viol <= tol &&
+                              out->obj_rel_err <= 1e-5
+                          ? 1
+                          : 0;
+        return 0;
+    } catch (const std::exception& ex) {
+        ctx->error = ex.what();
+        return -1;
+    }
+}
+
+NIR_API int nir_verify_solution_noalloc(nir_ctx* ctx, const nir_dev_lp* lp, const double* x, double claimed_obj,
+                                        double tol, nir_verify_report* out) {
+    if (!ctx || !out) return -1;
+    try {
+        if (!lp) throw std::invalid_argument("null LP");
+        if (lp->nrow < 0 || lp->ncol < 0) throw std::invalid_argument("negative dimensions");
+        if (lp->nrow > kNoAllocMaxDim || lp->ncol > kNoAllocMaxDim)
+            throw std::invalid_argument("nir_verify_solution_noalloc: nrow and ncol must be at most 16,777,216 "
+                                        "(the static scratch holds 65,536 block partials per pass)");
+        if (lp->nrow > 0 && (!lp->row_start || !lp->rowlo || !lp->rowup))
+            throw std::invalid_argument("missing device row arrays");
+        if (lp->ncol > 0 && (!x || !lp->cost || !lp->collo || !lp->colup))
+            throw std::invalid_argument("missing device column arrays or solution vector");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        unsigned char* base = noalloc_scratch(ctx->device);
+        hipStream_t s = ctx->stream;
+        const int m = lp->nrow, n = lp->ncol;
+        const int gr = std::max(1, (m + 255) / 256), gc = std::max(1, (n + 255) / 256);
+        MaxAt* prow = reinterpret_cast<MaxAt*>(base + kScratchRowOff);
+        MaxAt* pcol = reinterpret_cast<MaxAt*>(base + kScratchColOff);
+        dd* pobj = reinterpret_cast<dd*>(base + kScratchObjOff);
+        int* nf = reinterpret_cast<int*>(base + kScratchNfOff);
+        VerifyOut* res = reinterpret_cast<VerifyOut*>(base + kScratchOutOff);
+        const int big = 0x7fffffff;
+        // The partial slots are written in full by the two passes below; only
+        // the non-finite index must start fresh (a kernel, not a memset node).
+        k_set_int<<<1, 64, 0, s>>>(nf, big);
+        k_verify_rows<<<gr, 256, 0, s>>>(m, lp->row_start, lp->col_index, lp->value, x, lp->rowlo, lp->rowup, prow);
+        k_verify_cols<<<gc, 256, 0, s>>>(n, x, lp->collo, lp->colup, lp->cost, pcol, pobj, nf);
+        k_verify_final<<<1, 256, 0, s>>>(m > 0 ? gr : 0, n > 0 ? gc : 0, prow, pcol, pobj, nf, lp->obj_offset, res);
+        HIP_CHECK(hipGetLastError());
+        VerifyOut h{};
+        HIP_CHECK(hipMemcpyAsync(&h, res, sizeof(h), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        std::memset(out, 0, sizeof(*out));
+        out->max_row_viol = h.max_row;
+        out->worst_row = h.row_at >= 0 ? h.row_at / 2 : -1;
+        out->worst_row_upper = h.row_at >= 0 ? (h.row_at & 1) : 0;
+        out->max_bound_viol = h.max_bound;
+        out->worst_col = h.col_at >= 0 ? h.col_at / 2 : -1;
+        out->worst_col_upper = h.col_at >= 0 ? (h.col_at & 1) : 0;
+        out->nonfinite_col = h.nonfinite == big ? -1 : h.nonfinite;
+        out->computed_obj = h.obj;
         out->obj_rel_err = std::isfinite(claimed_obj)
                                ? std::fabs(out->computed_obj - claimed_obj) / (1.0 + std::fabs(out->computed_obj))
                                : 0.0;

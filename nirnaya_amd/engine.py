@@ -99,6 +99,13 @@ class NirVerifyReport(C.Structure):
                 ("passed", C.c_int32)]
 
 
+class NirDevLp(C.Structure):
+    """Device-resident CSR LP of nir_verify_solution_noalloc (device addresses)."""
+    _fields_ = [("nrow", C.c_int32), ("ncol", C.c_int32), ("row_start", C.c_void_p), ("col_index", C.c_void_p),
+                ("value", C.c_void_p), ("cost", C.c_void_p), ("collo", C.c_void_p), ("colup", C.c_void_p),
+                ("rowlo", C.c_void_p), ("rowup", C.c_void_p), ("obj_offset", C.c_double)]
+
+
 _lib = None
 
 
@@ -158,6 +165,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.nir_mlp_gemm_check.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, _fp, _fp, C.c_int32, C.c_int32, _fp]
     lib.nir_verify_solution.argtypes = [vp, C.POINTER(NirLp), _dp, C.c_double, C.c_double,
                                         C.POINTER(NirVerifyReport)]
+    lib.nir_verify_solution_noalloc.argtypes = [vp, C.POINTER(NirDevLp), vp, C.c_double, C.c_double,
+                                                C.POINTER(NirVerifyReport)]
     lib.nir_pod_gram.argtypes = [vp, _dp, C.c_int32, C.c_int32, _dp]
     lib.nir_pod_project.argtypes = [vp, _dp, C.c_int32, C.c_int32, _dp, C.c_int32, _dp]
     lib.nir_gen_equality_lp.argtypes = [C.c_uint64, C.c_int32, C.c_int32, C.c_int32]
@@ -393,6 +402,15 @@ class Engine:
         if self.lib.nir_verify_solution(self.h, C.byref(s), _ptr(x, C.c_double), float(claimed_obj), float(tol),
                                         C.byref(rep)) != 0:
             raise EngineError("nir_verify_solution: " + self.error())
+        return rep
+
+    def verify_noalloc(self, dev_lp: "NirDevLp", x_dev: int, claimed_obj: float, tol: float = 1e-6) -> NirVerifyReport:
+        """nir_verify_solution_noalloc on a device-resident LP and x (device
+        addresses owned by the caller)."""
+        rep = NirVerifyReport()
+        if self.lib.nir_verify_solution_noalloc(self.h, C.byref(dev_lp), C.c_void_p(x_dev), float(claimed_obj),
+                                                float(tol), C.byref(rep)) != 0:
+            raise EngineError("nir_verify_solution_noalloc: " + self.error())
         return rep
 
     def pod_gram(self, X: np.ndarray) -> np.ndarray:
