@@ -220,6 +220,53 @@ NIR_API int nir_batch_result(nir_batch* b, int32_t i, double* x, double* y, nir_
 NIR_API int nir_batch_get_stats(nir_batch* b, nir_batch_stats* out);
 NIR_API void nir_batch_destroy(nir_batch* b);
 
+/* ---- parametric families: one matrix, many right-hand sides ------------
+ * No counterpart in the reference (src/gpu/pdhg_solver.hpp:25-107 solves one
+ * LP; its products are the single-vector src/core/csc_matrix.hpp:99-120).
+ * Members of a family share the constraint matrix and column bounds of
+ * `base` and have their own costs and row bounds. The engine keeps 64
+ * members per wave64 in interleaved vectors, so the matrix is streamed once
+ * per 64 members and every operand gather is a coalesced 512-byte read;
+ * `count` members are streamed through a pool of `lanes` lanes (finished
+ * members are handed out and their lanes refilled between evaluation
+ * blocks). Every member gets exactly what nir_pdhg_solve returns for it
+ * alone (pdlp mode, FP64): the same x, y, iteration and restart counts,
+ * status and objective. */
+typedef struct nir_family nir_family;
+typedef struct {
+    int32_t lanes; /* members iterated together, a multiple of 64 (default 1024) */
+    int32_t reserved;
+} nir_family_opts;
+typedef struct {
+    int32_t lanes;
+    int32_t stopped;          /* should_stop ended the last solve */
+    int64_t instances;        /* members finished */
+    int64_t blocks;           /* evaluation blocks run (eval_freq iterations of every lane each) */
+    int64_t refills;          /* start-point evaluations of refilled lanes */
+    int64_t total_iterations; /* summed over members */
+    double setup_seconds, solve_seconds;
+} nir_family_stats;
+/* Supplies member `index`: cost (ncol), rowlo and rowup (nrow), and an
+ * optional warm start x0 (ncol) / y0 (nrow) flagged in *warm (bit 0: x0,
+ * bit 1: y0; the buffers arrive zeroed). Returns 0, or nonzero to abort. */
+typedef int (*nir_family_fill_fn)(void* user, int64_t index, double* cost, double* rowlo, double* rowup,
+                                  double* x0, double* y0, int32_t* warm);
+/* Receives member `index`'s solution (x: ncol, y: nrow, valid during the
+ * call) and its nir_info; members arrive in the order they finish. */
+typedef int (*nir_family_done_fn)(void* user, int64_t index, const double* x, const double* y,
+                                  const nir_info* info);
+NIR_API void nir_family_opts_default(nir_family_opts* o);
+/* Uploads and scales the shared matrix (pdlp mode FP64 parameters; warm
+ * starts come per member through the fill callback). */
+NIR_API int nir_family_create(nir_ctx* ctx, const nir_lp* base, const nir_params* p, const nir_family_opts* o,
+                              nir_family** out);
+/* Solves members 0 .. count-1; should_stop (polled between blocks) ends the
+ * running members with status 4 and leaves the rest unsolved. */
+NIR_API int nir_family_solve(nir_family* f, int64_t count, nir_family_fill_fn fill, nir_family_done_fn done,
+                             void* user);
+NIR_API int nir_family_get_stats(nir_family* f, nir_family_stats* out);
+NIR_API void nir_family_destroy(nir_family* f);
+
 /* ---- learned warm start -------------------------------------------------
  * No counterpart in the reference. A small MLP (n_in -> hidden -> hidden ->
  * n_out, GELU, bf16 MFMA GEMMs with FP32 accumulation, Adam) trained on the

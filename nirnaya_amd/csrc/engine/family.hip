@@ -1,0 +1,909 @@
+// Parametric-family engine: many LPs that share one constraint matrix A and
+// its column bounds and differ in costs and row bounds (the family
+// b(theta), c(theta) of a learned warm start), solved by the restarted PDHG
+// of pdlp mode in FP64, every instance on exactly the single-LP trajectory.
+//
+// Layout: instances occupy lanes. A group of 64 lanes (one wave64) keeps its
+// 64 instances' vectors interleaved, element j of lane q at
+// [group][j][64] + q, so a nonzero a_ij of a row product is loaded once
+// (a wave-uniform scalar load) and its gathered operand entries for all 64
+// instances are one contiguous 512-byte read. Each lane sums its row in
+// ascending column order from 0.0, the single-LP engine's sequential order,
+// so every iterate is the same bit for bit; the evaluation terms use the
+// same functions (pdlp_core.hpp) and double-double sums.
+//
+// The matrix is streamed once per group and product instead of once per
+// instance, and the gathers fetch whole cache lines of useful data: the
+// random 8-byte gathers that bound the single-LP products become coalesced
+// 512-byte ones.
+//
+// Work queue: the host streams any number of instances through a fixed pool
+// of lanes. Between evaluation blocks (eval_freq iterations, one hipGraph
+// replay) it harvests finished lanes (their T(z), unscaled on the host) and
+// refills free lanes with the next instances: their scaled vectors and state
+// are prepared on the host by the single-LP engine's own set-up code
+// (pdlp_prepare_instance), scattered into the lanes, and evaluated at their
+// start point (mode 1) before they iterate. Finished and empty lanes are
+// skipped by every kernel (whole waves return when all 64 lanes are idle).
+//
+// The reference solves one LP per call with a CPU loop
+// (src/gpu/pdhg_solver.hpp:25-107, single-vector products
+// src/core/csc_matrix.hpp:99-120); this engine has no counterpart there.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/nirnaya/pdhg.h"
+#include "ctx.hpp"
+#include "devmat.hpp"
+#include "lpcheck.hpp"
+#include "pdlp.hpp"
+#include "pdlp_core.hpp"
+
+namespace nir {
+namespace gpu {
+namespace fam {
+
+constexpr int kL = 64;              // lanes per group (one wave64)
+constexpr int kRowsPerWave = 4;     // rows per wave in the iteration kernels
+constexpr int kEvalBlocks = 256;    // evaluation workgroups per group (grid-stride over rows)
+constexpr int kStageMax = 256;      // lanes refilled or harvested per transfer
+
+struct View {
+    int m, n, G;
+    int ebc, ebr;   // evaluation workgroups per group over columns / rows
+    const int* arp;
+    const int* acol;
+    const double* aval;   // CSR of the scaled A (m rows)
+    const int* trp;
+    const int* tcol;
+    const double* tval;   // CSR of the scaled A^T (n rows)
+    const double* lo;
+    const double* up;        // scaled column bounds, shared
+    const double* rscale;    // caller-unit scales R and C
+    const double* cscale;
+    double *x, *xbar, *xa, *xp, *c;    // [G][n][kL]
+    double *y, *ya, *yp, *rl, *ru;     // [G][m][kL]
+    double *qx, *rx;                   // certificate: old anchor, anchor step (n)
+    double *qy, *ry;                   // (m)
+    PdlpState* st;                     // [G * kL]
+    double* ltau;
+    double* lsigma;
+    int* lkb;
+    int* ldone;
+    int* lrestart;
+    int* lfresh;
+    dd* part_c;   // [G][ebc][5][kL]
+    dd* part_r;   // [G][ebr][6][kL]
+    dd* cert_c;   // [G][ebc][6][kL]
+    dd* cert_r;   // [G][ebr][6][kL]
+};
+
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+// Row r of a CSR matrix times the lane's interleaved operand v (already
+// offset to the lane): sum in ascending column order from 0.0, products
+// rounded before the add (no contraction), four operand reads in flight.
+__device__ __forceinline__ double row_sum(const int* __restrict__ rp, const int* __restrict__ col,
+                                          const double* __restrict__ val, const double* __restrict__ v, int r) {
+    const int p0 = rp[r], p1 = rp[r + 1];
+    double s = 0.0;
+    int p = p0;
+    for (; p + 4 <= p1; p += 4) {
+        const int c0 = col[p], c1 = col[p + 1], c2 = col[p + 2], c3 = col[p + 3];
+        const double a0 = val[p], a1 = val[p + 1], a2 = val[p + 2], a3 = val[p + 3];
+        const double g0 = v[static_cast<size_t>(c0) * kL], g1 = v[static_cast<size_t>(c1) * kL];
+        const double g2 = v[static_cast<size_t>(c2) * kL], g3 = v[static_cast<size_t>(c3) * kL];
+        s += a0 * g0;
+        s += a1 * g1;
+        s += a2 * g2;
+        s += a3 * g3;
+    }
+    for (; p < p1; ++p) s += val[p] * v[static_cast<size_t>(col[p]) * kL];
+    return s;
+}
+
+// The same product in double-double (certificates).
+__device__ __forceinline__ dd row_dot_lane(const int* __restrict__ rp, const int* __restrict__ col,
+                                           const double* __restrict__ val, const double* __restrict__ v, int r) {
+    dd acc = dd_zero();
+    for (int p = rp[r]; p < rp[r + 1]; ++p) acc = dd_add(acc, dd_prod(val[p], v[static_cast<size_t>(col[p]) * kL]));
+    return acc;
+}
+
+// Primal half of an iteration over rows of A^T: T(z)_x, the reflected point
+// and the Halpern step, per lane (pdlp.hip k_primal_pdlp).
+__global__ __launch_bounds__(256) void k_fam_primal(View V, int k_lo, int save) {
+    const int g = blockIdx.y;
+    const int lane = threadIdx.x & (kL - 1);
+    const int li = g * kL + lane;
+    const bool done = V.ldone[li] != 0;
+    if (__all(done)) return;
+    const double tau = V.ltau[li];
+    const Halpern<double> hw = halpern<double>(V.lkb[li] + k_lo);
+    const int r0 = (blockIdx.x * 4 + wave_id()) * kRowsPerWave;
+    const double* yv = V.y + static_cast<size_t>(g) * V.m * kL + lane;
+    const size_t xb = static_cast<size_t>(g) * V.n * kL + lane;
+    const int r1 = min(r0 + kRowsPerWave, V.n);
+    for (int r = r0; r < r1; ++r) {
+        const double gs = row_sum(V.trp, V.tcol, V.tval, yv, r);
+        if (!done) {
+            const size_t e = xb + static_cast<size_t>(r) * kL;
+            const PrimalOut<double> o = primal_point<double>(V.x[e], gs, V.c[e], V.lo[r], V.up[r], V.xa[e], tau, hw);
+            V.xbar[e] = o.xb;
+            V.x[e] = o.next;
+            if (save) V.xp[e] = o.cand;
+        }
+    }
+}
+
+// Dual half over rows of A (pdlp.hip k_dual_pdlp).
+__global__ __launch_bounds__(256) void k_fam_dual(View V, int k_lo, int save) {
+    const int g = blockIdx.y;
+    const int lane = threadIdx.x & (kL - 1);
+    const int li = g * kL + lane;
+    const bool done = V.ldone[li] != 0;
+    if (__all(done)) return;
+    const double sigma = V.lsigma[li];
+    const Halpern<double> hw = halpern<double>(V.lkb[li] + k_lo);
+    const int r0 = (blockIdx.x * 4 + wave_id()) * kRowsPerWave;
+    const double* xv = V.xbar + static_cast<size_t>(g) * V.n * kL + lane;
+    const size_t yb = static_cast<size_t>(g) * V.m * kL + lane;
+    const int r1 = min(r0 + kRowsPerWave, V.m);
+    for (int r = r0; r < r1; ++r) {
+        const double h = row_sum(V.arp, V.acol, V.aval, xv, r);
+        if (!done) {
+            const size_t e = yb + static_cast<size_t>(r) * kL;
+            const DualOut<double> o = dual_point<double>(V.y[e], h, V.rl[e], V.ru[e], V.ya[e], sigma, hw);
+            V.y[e] = o.next;
+            if (save) V.yp[e] = o.yp;
+        }
+    }
+}
+
+// Lanes an evaluation pass serves: fresh lanes (start point, mode 1) or the
+// running ones (after a block, mode 0).
+template <int FRESH>
+__device__ __forceinline__ bool eval_lane(const View& V, int li) {
+    if constexpr (FRESH) return V.lfresh[li] != 0;
+    else return V.ldone[li] == 0;
+}
+
+// Sum NQ dd terms per lane over the workgroup's four waves; wave 0 writes
+// them to part[((g * blocks + b) * NQ + q) * kL + lane].
+template <int NQ>
+__device__ __forceinline__ void waves_to_part(dd (&acc)[NQ], dd* sh, dd* part, int g, int blocks) {
+    const int lane = threadIdx.x & (kL - 1), w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) sh[(w * NQ + q) * kL + lane] = acc[q];
+    __syncthreads();
+    if (w == 0) {
+        for (int ww = 1; ww < 4; ++ww)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) acc[q] = dd_add(acc[q], sh[(ww * NQ + q) * kL + lane]);
+        dd* out = part + (static_cast<size_t>(g) * blocks + blockIdx.x) * NQ * kL + lane;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) out[static_cast<size_t>(q) * kL] = acc[q];
+    }
+}
+
+// Column evaluation terms (pdlp.hip k_eval_cols): A^T yp per lane. FRESH
+// (a template parameter, so each pass is straight-line code) selects the
+// lanes: refilled ones at their start point, or the running ones.
+template <int FRESH>
+__global__ __launch_bounds__(256) void k_fam_eval_cols(View V) {
+    __shared__ dd sh[4 * 5 * kL];
+    const int g = blockIdx.y;
+    const int lane = threadIdx.x & (kL - 1);
+    const int li = g * kL + lane;
+    if (__all(!eval_lane<FRESH>(V, li))) return;
+    dd acc[5] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
+    const double* yv = V.yp + static_cast<size_t>(g) * V.m * kL + lane;
+    const size_t xb = static_cast<size_t>(g) * V.n * kL + lane;
+    for (int r = blockIdx.x * 4 + wave_id(); r < V.n; r += gridDim.x * 4) {
+        const double gs = row_sum(V.trp, V.tcol, V.tval, yv, r);
+        const size_t e = xb + static_cast<size_t>(r) * kL;
+        dd t[5];
+        col_terms(gs, V.c[e], V.lo[r], V.up[r], V.cscale[r], V.xp[e], V.xa[e], t);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) acc[q] = dd_add(acc[q], t[q]);
+    }
+    waves_to_part<5>(acc, sh, V.part_c, g, V.ebc);
+}
+
+// Row evaluation terms (pdlp.hip k_eval_rows): A xp per lane.
+template <int FRESH>
+__global__ __launch_bounds__(256) void k_fam_eval_rows(View V) {
+    __shared__ dd sh[4 * 6 * kL];
+    const int g = blockIdx.y;
+    const int lane = threadIdx.x & (kL - 1);
+    const int li = g * kL + lane;
+    if (__all(!eval_lane<FRESH>(V, li))) return;
+    dd acc[6] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
+    const double* xv = V.xp + static_cast<size_t>(g) * V.n * kL + lane;
+    const size_t yb = static_cast<size_t>(g) * V.m * kL + lane;
+    for (int r = blockIdx.x * 4 + wave_id(); r < V.m; r += gridDim.x * 4) {
+        const double h = row_sum(V.arp, V.acol, V.aval, xv, r);
+        const size_t e = yb + static_cast<size_t>(r) * kL;
+        dd t[6];
+        row_terms(h, V.rl[e], V.ru[e], V.rscale[r], V.yp[e], V.ya[e], t);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) acc[q] = dd_add(acc[q], t[q]);
+    }
+    waves_to_part<6>(acc, sh, V.part_r, g, V.ebr);
+}
+
+// Per-lane reduction of the evaluation partials and the decision
+// (pdlp_decide, as k_decide). One workgroup of 64 lanes per group.
+template <int FRESH>
+__global__ __launch_bounds__(kL) void k_fam_decide(View V) {
+    const int g = blockIdx.x, lane = threadIdx.x, li = g * kL + lane;
+    if (!eval_lane<FRESH>(V, li)) return;
+    dd tot[11];
+    for (int q = 0; q < 11; ++q) tot[q] = dd_zero();
+    for (int b = 0; b < V.ebc; ++b) {
+        const dd* p = V.part_c + (static_cast<size_t>(g) * V.ebc + b) * 5 * kL + lane;
+        for (int q = 0; q < 5; ++q) tot[q] = dd_add(tot[q], p[static_cast<size_t>(q) * kL]);
+    }
+    for (int b = 0; b < V.ebr; ++b) {
+        const dd* p = V.part_r + (static_cast<size_t>(g) * V.ebr + b) * 6 * kL + lane;
+        for (int q = 0; q < 6; ++q) tot[5 + q] = dd_add(tot[5 + q], p[static_cast<size_t>(q) * kL]);
+    }
+    PdlpState s = V.st[li];
+    pdlp_decide(s, tot, FRESH);
+    V.st[li] = s;
+    V.ltau[li] = s.tau;
+    V.lsigma[li] = s.sigma;
+    V.lkb[li] = s.k_base;
+    V.ldone[li] = s.done;
+    V.lrestart[li] = s.restart;
+    if constexpr (FRESH) V.lfresh[li] = 0;
+}
+
+__device__ __forceinline__ bool cert_lane(const View& V, int li) { return !V.ldone[li] && V.lrestart[li]; }
+
+// Restart bookkeeping over the interleaved vectors; blockIdx.y = group.
+// STAGE 0: keep the old anchor (k_cert_save); 1: iterate and anchor := T(z)
+// (k_restart_copy); 2: the anchor step (k_cert_diff). The stage is a
+// template parameter, not a kernel argument: one straight-line kernel per
+// stage.
+template <int STAGE>
+__global__ __launch_bounds__(256) void k_fam_restart(View V) {
+    const int g = blockIdx.y;
+    const int lane = threadIdx.x & (kL - 1);
+    const int li = g * kL + lane;
+    if (!cert_lane(V, li)) return;
+    const size_t nx = static_cast<size_t>(V.n) * kL, ny = static_cast<size_t>(V.m) * kL;
+    const size_t bx = static_cast<size_t>(g) * nx, by = static_cast<size_t>(g) * ny;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    const size_t t0 = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (size_t t = t0; t < nx; t += stride) {
+        const size_t e = bx + t;
+        if constexpr (STAGE == 0) {
+            V.qx[e] = V.xa[e];
+        } else if constexpr (STAGE == 1) {
+            const double v = V.xp[e];
+            V.x[e] = v;
+            V.xa[e] = v;
+        } else {
+            V.rx[e] = V.xa[e] - V.qx[e];
+        }
+    }
+    for (size_t t = t0; t < ny; t += stride) {
+        const size_t e = by + t;
+        if constexpr (STAGE == 0) {
+            V.qy[e] = V.ya[e];
+        } else if constexpr (STAGE == 1) {
+            const double v = V.yp[e];
+            V.y[e] = v;
+            V.ya[e] = v;
+        } else {
+            V.ry[e] = V.ya[e] - V.qy[e];
+        }
+    }
+}
+
+// Certificate terms (pdlp.hip k_cert_cols / k_cert_rows) per lane.
+__global__ __launch_bounds__(256) void k_fam_cert_cols(View V) {
+    __shared__ dd sh[4 * 6 * kL];
+    const int g = blockIdx.y;
+    const int lane = threadIdx.x & (kL - 1);
+    const int li = g * kL + lane;
+    if (__all(!cert_lane(V, li))) return;
+    dd acc[6] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
+    const double* dyv = V.ry + static_cast<size_t>(g) * V.m * kL + lane;
+    const size_t xb = static_cast<size_t>(g) * V.n * kL + lane;
+    for (int r = blockIdx.x * 4 + wave_id(); r < V.n; r += gridDim.x * 4) {
+        const double lam = -dd_val(row_dot_lane(V.trp, V.tcol, V.tval, dyv, r));
+        const size_t e = xb + static_cast<size_t>(r) * kL;
+        cert_col_terms(lam, V.c[e], V.lo[r], V.up[r], V.rx[e], acc);
+    }
+    waves_to_part<6>(acc, sh, V.cert_c, g, V.ebc);
+}
+
+__global__ __launch_bounds__(256) void k_fam_cert_rows(View V) {
+    __shared__ dd sh[4 * 6 * kL];
+    const int g = blockIdx.y;
+    const int lane = threadIdx.x & (kL - 1);
+    const int li = g * kL + lane;
+    if (__all(!cert_lane(V, li))) return;
+    dd acc[6] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
+    const double* dxv = V.rx + static_cast<size_t>(g) * V.n * kL + lane;
+    const size_t yb = static_cast<size_t>(g) * V.m * kL + lane;
+    for (int r = blockIdx.x * 4 + wave_id(); r < V.m; r += gridDim.x * 4) {
+        const double h = dd_val(row_dot_lane(V.arp, V.acol, V.aval, dxv, r));
+        const size_t e = yb + static_cast<size_t>(r) * kL;
+        cert_row_terms(h, V.rl[e], V.ru[e], V.ry[e], acc);
+    }
+    waves_to_part<6>(acc, sh, V.cert_r, g, V.ebr);
+}
+
+__global__ __launch_bounds__(kL) void k_fam_cert_decide(View V) {
+    const int g = blockIdx.x, lane = threadIdx.x, li = g * kL + lane;
+    if (!cert_lane(V, li)) return;
+    dd tot[6];
+    for (int q = 0; q < 6; ++q) tot[q] = dd_zero();
+    for (int b = 0; b < V.ebc; ++b) {
+        const dd* p = V.cert_c + (static_cast<size_t>(g) * V.ebc + b) * 6 * kL + lane;
+        for (int q = 0; q < 6; ++q) tot[q] = dd_add(tot[q], p[static_cast<size_t>(q) * kL]);
+    }
+    for (int b = 0; b < V.ebr; ++b) {
+        const dd* p = V.cert_r + (static_cast<size_t>(g) * V.ebr + b) * 6 * kL + lane;
+        for (int q = 0; q < 6; ++q) tot[q] = dd_add(tot[q], p[static_cast<size_t>(q) * kL]);
+    }
+    const int code = cert_decide(tot);
+    if (code) {
+        V.st[li].done = 1;
+        V.st[li].status = code;
+        V.st[li].restart = 0;
+        V.ldone[li] = 1;
+        V.lrestart[li] = 0;
+    }
+}
+
+// Refill: slot f (blockIdx.y) of the staging arrays goes into lane lanes[f].
+__global__ __launch_bounds__(256) void k_fam_scatter(View V, int count, const int* __restrict__ lanes,
+                                                     const double* __restrict__ sc, const double* __restrict__ sx,
+                                                     const double* __restrict__ srl, const double* __restrict__ sru,
+                                                     const double* __restrict__ sy, const PdlpState* __restrict__ sst) {
+    const int f = blockIdx.y;
+    if (f >= count) return;
+    const int li = lanes[f];
+    const int g = li / kL, q = li % kL;
+    const size_t bx = static_cast<size_t>(g) * V.n * kL + q, by = static_cast<size_t>(g) * V.m * kL + q;
+    const int stride = gridDim.x * blockDim.x;
+    const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int j = t0; j < V.n; j += stride) {
+        const size_t e = bx + static_cast<size_t>(j) * kL;
+        const double v = sx[static_cast<size_t>(f) * V.n + j];
+        V.x[e] = v;
+        V.xbar[e] = v;
+        V.xa[e] = v;
+        V.xp[e] = v;
+        V.c[e] = sc[static_cast<size_t>(f) * V.n + j];
+    }
+    for (int i = t0; i < V.m; i += stride) {
+        const size_t e = by + static_cast<size_t>(i) * kL;
+        const double v = sy[static_cast<size_t>(f) * V.m + i];
+        V.y[e] = v;
+        V.ya[e] = v;
+        V.yp[e] = v;
+        V.rl[e] = srl[static_cast<size_t>(f) * V.m + i];
+        V.ru[e] = sru[static_cast<size_t>(f) * V.m + i];
+    }
+    if (t0 == 0) {
+        const PdlpState s = sst[f];
+        V.st[li] = s;
+        V.ltau[li] = s.tau;
+        V.lsigma[li] = s.sigma;
+        V.lkb[li] = 0;
+        V.ldone[li] = 0;
+        V.lrestart[li] = 0;
+        V.lfresh[li] = 1;
+    }
+}
+
+// Harvest: T(z) and the scaled cost of lane lanes[f] into slot f.
+__global__ __launch_bounds__(256) void k_fam_gather(View V, int count, const int* __restrict__ lanes,
+                                                    double* __restrict__ ox, double* __restrict__ oc,
+                                                    double* __restrict__ oy, PdlpState* __restrict__ ost) {
+    const int f = blockIdx.y;
+    if (f >= count) return;
+    const int li = lanes[f];
+    const int g = li / kL, q = li % kL;
+    const size_t bx = static_cast<size_t>(g) * V.n * kL + q, by = static_cast<size_t>(g) * V.m * kL + q;
+    const int stride = gridDim.x * blockDim.x;
+    const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int j = t0; j < V.n; j += stride) {
+        const size_t e = bx + static_cast<size_t>(j) * kL;
+        ox[static_cast<size_t>(f) * V.n + j] = V.xp[e];
+        oc[static_cast<size_t>(f) * V.n + j] = V.c[e];
+    }
+    for (int i = t0; i < V.m; i += stride) oy[static_cast<size_t>(f) * V.m + i] = V.yp[by + static_cast<size_t>(i) * kL];
+    if (t0 == 0) ost[f] = V.st[li];
+}
+
+// Stop request: every running lane ends with status 4 (time limit).
+__global__ void k_fam_stop(View V, int lanes) {
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= lanes || V.ldone[li]) return;
+    V.st[li].done = 1;
+    V.st[li].status = 4;
+    V.ldone[li] = 1;
+}
+
+}  // namespace fam
+}  // namespace gpu
+}  // namespace nir
+
+using namespace nir::gpu;
+using namespace nir::gpu::fam;
+
+struct nir_family {
+    nir_ctx* ctx = nullptr;
+    hipStream_t s = nullptr;
+    nir_params prm{};
+    nir_family_opts opts{};
+    int m = 0, n = 0;
+    long long nnz = 0;
+    int G = 0, lanes = 0;
+    // shared data
+    std::vector<int64_t> col_start;
+    std::vector<int32_t> row_index;
+    std::vector<double> value, collo, colup;
+    double offset = 0.0;
+    std::vector<double> R, C;
+    double eta = 0.0;
+    DevMatrix A, AT;
+    DevArray<double> dR, dC, lo, up;
+    // lanes
+    DevArray<double> x, xbar, xa, xp, c, y, ya, yp, rl, ru, qx, rx, qy, ry;
+    DevArray<PdlpState> st;
+    DevArray<double> ltau, lsigma;
+    DevArray<int> lkb, ldone, lrestart, lfresh;
+    DevArray<dd> part_c, part_r, cert_c, cert_r;
+    // staging
+    DevArray<int> s_lanes;
+    DevArray<double> s_c, s_x, s_rl, s_ru, s_y;
+    DevArray<PdlpState> s_st;
+    std::vector<int> h_done;
+    int ebc = 1, ebr = 1;
+    // graph of one evaluation block
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    nir_family_stats stats{};
+
+    ~nir_family() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+
+    View view() {
+        View V{};
+        V.m = m;
+        V.n = n;
+        V.G = G;
+        V.ebc = ebc;
+        V.ebr = ebr;
+        V.arp = A.shape.rowptr.p;
+        V.acol = A.shape.col.p;
+        V.aval = A.val.p;
+        V.trp = AT.shape.rowptr.p;
+        V.tcol = AT.shape.col.p;
+        V.tval = AT.val.p;
+        V.lo = lo.p;
+        V.up = up.p;
+        V.rscale = dR.p;
+        V.cscale = dC.p;
+        V.x = x.p; V.xbar = xbar.p; V.xa = xa.p; V.xp = xp.p; V.c = c.p;
+        V.y = y.p; V.ya = ya.p; V.yp = yp.p; V.rl = rl.p; V.ru = ru.p;
+        V.qx = qx.p; V.rx = rx.p; V.qy = qy.p; V.ry = ry.p;
+        V.st = st.p;
+        V.ltau = ltau.p;
+        V.lsigma = lsigma.p;
+        V.lkb = lkb.p;
+        V.ldone = ldone.p;
+        V.lrestart = lrestart.p;
+        V.lfresh = lfresh.p;
+        V.part_c = part_c.p;
+        V.part_r = part_r.p;
+        V.cert_c = cert_c.p;
+        V.cert_r = cert_r.p;
+        return V;
+    }
+};
+
+namespace {
+
+int iter_grid(int rows) { return std::max(1, (rows + 4 * kRowsPerWave - 1) / (4 * kRowsPerWave)); }
+
+template <int FRESH>
+void launch_eval(nir_family* F, const View& V) {
+    k_fam_eval_cols<FRESH><<<dim3(F->ebc, F->G), 256, 0, F->s>>>(V);
+    k_fam_eval_rows<FRESH><<<dim3(F->ebr, F->G), 256, 0, F->s>>>(V);
+    k_fam_decide<FRESH><<<F->G, kL, 0, F->s>>>(V);
+}
+
+// One evaluation block: eval_freq iterations, the evaluation and decision,
+// the restart copy and the certificate test of a restart (the single-LP
+// order of pdlp.hip launch_eval in FP64).
+void enqueue_block(nir_family* F) {
+    const View V = F->view();
+    const int ef = std::max(1, F->prm.eval_freq);
+    const dim3 gp(iter_grid(F->n), F->G), gd(iter_grid(F->m), F->G);
+    for (int kl = 0; kl < ef; ++kl) {
+        const int save = kl == ef - 1 ? 1 : 0;
+        k_fam_primal<<<gp, 256, 0, F->s>>>(V, kl, save);
+        k_fam_dual<<<gd, 256, 0, F->s>>>(V, kl, save);
+    }
+    launch_eval<0>(F, V);
+    const int gv = std::max(1, std::min(1024, (std::max(F->n, F->m) * kL + 255) / 256));
+    k_fam_restart<0><<<dim3(gv, F->G), 256, 0, F->s>>>(V);
+    k_fam_restart<1><<<dim3(gv, F->G), 256, 0, F->s>>>(V);
+    k_fam_restart<2><<<dim3(gv, F->G), 256, 0, F->s>>>(V);
+    k_fam_cert_cols<<<dim3(F->ebc, F->G), 256, 0, F->s>>>(V);
+    k_fam_cert_rows<<<dim3(F->ebr, F->G), 256, 0, F->s>>>(V);
+    k_fam_cert_decide<<<F->G, kL, 0, F->s>>>(V);
+    HIP_CHECK(hipGetLastError());
+}
+
+void run_block(nir_family* F) {
+    if (F->prm.use_graph) {
+        if (!F->gexec) {
+            HIP_CHECK(hipStreamBeginCapture(F->s, hipStreamCaptureModeThreadLocal));
+            enqueue_block(F);
+            HIP_CHECK(hipStreamEndCapture(F->s, &F->graph));
+            HIP_CHECK(hipGraphInstantiate(&F->gexec, F->graph, nullptr, nullptr, 0));
+        }
+        HIP_CHECK(hipGraphLaunch(F->gexec, F->s));
+    } else {
+        enqueue_block(F);
+    }
+}
+
+// A prepared instance waiting for a lane.
+struct Prepared {
+    long long index;
+    PdlpInstance inst;
+};
+
+}  // namespace
+
+extern "C" {
+
+NIR_API void nir_family_opts_default(nir_family_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->lanes = 1024;
+}
+
+NIR_API int nir_family_create(nir_ctx* ctx, const nir_lp* base, const nir_params* p, const nir_family_opts* o,
+                              nir_family** out) {
+    if (!ctx || !out) return -1;
+    *out = nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    auto F = std::make_unique<nir_family>();
+    try {
+        check_lp(base);
+        if (!p) throw std::invalid_argument("family needs parameters");
+        F->prm = *p;
+        if (o) F->opts = *o;
+        else nir_family_opts_default(&F->opts);
+        if (F->prm.mode != NIR_MODE_PDLP) throw std::invalid_argument("the family engine runs pdlp mode only");
+        if (F->prm.precision != NIR_PREC_FP64) throw std::invalid_argument("the family engine runs FP64 only");
+        if (F->prm.x0 || F->prm.y0 || F->prm.unscale_row || F->prm.unscale_col)
+            throw std::invalid_argument("family solves take warm starts per instance and no unscaling vectors");
+        if (F->opts.lanes <= 0 || F->opts.lanes % kL != 0)
+            throw std::invalid_argument("family lanes must be a positive multiple of 64");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        F->ctx = ctx;
+        F->s = ctx->stream;
+        const int m = base->nrow, n = base->ncol;
+        F->m = m;
+        F->n = n;
+        F->nnz = n > 0 ? base->col_start[n] : 0;
+        if (F->nnz >= (1LL << 31)) throw std::invalid_argument("family matrix has 2^31 or more nonzeros");
+        F->col_start.assign(base->col_start, base->col_start + n + 1);
+        F->row_index.assign(base->row_index, base->row_index + F->nnz);
+        F->value.assign(base->value, base->value + F->nnz);
+        F->collo.assign(base->collo, base->collo + n);
+        F->colup.assign(base->colup, base->colup + n);
+        F->offset = base->obj_offset;
+        F->G = F->opts.lanes / kL;
+        F->lanes = F->opts.lanes;
+
+        // The matrix in one segment (CSR of A, CSR of A^T = CSC of A), scaled
+        // on the device by the single-LP engine's own set-up.
+        std::vector<int> a_ptr, a_col, at_ptr(static_cast<size_t>(n) + 1);
+        std::vector<double> a_val;
+        csr_from_csc(base, a_ptr, a_col, a_val);
+        for (int j = 0; j <= n; ++j) at_ptr[j] = static_cast<int>(base->col_start[j]);
+        TileShape plain;
+        upload_matrix(F->A, m, n, a_ptr, a_col.data(), a_val.data(), plain, F->s);
+        upload_matrix(F->AT, n, m, at_ptr, base->row_index, base->value, plain, F->s);
+        F->dR.alloc(m);
+        F->dC.alloc(n);
+        F->eta = pdlp_prepare_matrix(F->s, m, n, F->nnz, &F->A, &F->AT, F->dR.p, F->dC.p, F->R, F->C);
+        {
+            // The scaled column bounds are the same for every member.
+            PdlpInstance I;
+            pdlp_prepare_instance(base, F->prm, F->R, F->C, F->eta, false, I);
+            F->lo.alloc(n);
+            F->lo.upload(I.lo.data(), n, F->s);
+            F->up.alloc(n);
+            F->up.upload(I.up.data(), n, F->s);
+        }
+
+        const size_t L = static_cast<size_t>(F->lanes);
+        const size_t NX = L * n, NY = L * m;
+        for (auto* b : {&F->x, &F->xbar, &F->xa, &F->xp, &F->c, &F->qx, &F->rx}) b->alloc(NX);
+        for (auto* b : {&F->y, &F->ya, &F->yp, &F->rl, &F->ru, &F->qy, &F->ry}) b->alloc(NY);
+        F->st.alloc(L);
+        F->ltau.alloc(L);
+        F->lsigma.alloc(L);
+        F->lkb.alloc(L);
+        F->ldone.alloc(L);
+        F->lrestart.alloc(L);
+        F->lfresh.alloc(L);
+        {
+            // Every lane starts empty (done, not fresh).
+            std::vector<int> ones(L, 1), zeros(L, 0);
+            std::vector<double> dz(L, 0.0);
+            std::vector<PdlpState> sz(L);
+            for (auto& z : sz) {
+                z = PdlpState{};
+                z.done = 1;
+            }
+            F->ldone.upload(ones.data(), L, F->s);
+            F->lkb.upload(zeros.data(), L, F->s);
+            F->lrestart.upload(zeros.data(), L, F->s);
+            F->lfresh.upload(zeros.data(), L, F->s);
+            F->ltau.upload(dz.data(), L, F->s);
+            F->lsigma.upload(dz.data(), L, F->s);
+            F->st.upload(sz.data(), L, F->s);
+            HIP_CHECK(hipStreamSynchronize(F->s));
+        }
+        F->ebc = std::max(1, std::min(kEvalBlocks, (n + 255) / 256));
+        F->ebr = std::max(1, std::min(kEvalBlocks, (m + 255) / 256));
+        F->part_c.alloc(static_cast<size_t>(F->G) * F->ebc * 5 * kL);
+        F->part_r.alloc(static_cast<size_t>(F->G) * F->ebr * 6 * kL);
+        F->cert_c.alloc(static_cast<size_t>(F->G) * F->ebc * 6 * kL);
+        F->cert_r.alloc(static_cast<size_t>(F->G) * F->ebr * 6 * kL);
+        const size_t S = static_cast<size_t>(std::min(F->lanes, kStageMax));
+        F->s_lanes.alloc(S);
+        F->s_c.alloc(S * n);
+        F->s_x.alloc(S * n);
+        F->s_rl.alloc(S * m);
+        F->s_ru.alloc(S * m);
+        F->s_y.alloc(S * m);
+        F->s_st.alloc(S);
+        F->h_done.assign(L, 1);
+        std::memset(&F->stats, 0, sizeof(F->stats));
+        F->stats.lanes = F->lanes;
+        F->stats.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        *out = F.release();
+        return 0;
+    } catch (const std::exception& ex) {
+        ctx->error = ex.what();
+        if (F->s) (void)hipStreamSynchronize(F->s);
+        return -1;
+    }
+}
+
+NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fill, nir_family_done_fn done,
+                             void* user) {
+    if (!F || !fill || !done || count < 0) return -1;
+    try {
+        HIP_CHECK(hipSetDevice(F->ctx->device));
+        auto t0 = std::chrono::steady_clock::now();
+        const int m = F->m, n = F->n;
+        const size_t S = static_cast<size_t>(std::min(F->lanes, kStageMax));
+        std::vector<long long> lane_inst(static_cast<size_t>(F->lanes), -1);
+        std::vector<int> free_lanes;
+        for (int l = F->lanes - 1; l >= 0; --l) free_lanes.push_back(l);
+        long long next = 0, finished = 0;
+        int active = 0;
+        std::vector<double> cost(n), rlo(m), rup(m), x0(n), y0(m);
+        std::vector<Prepared> ready;
+        // Prepare one instance on the host with the single-LP set-up code.
+        auto prepare = [&](long long idx) {
+            int32_t warm = 0;
+            std::fill(x0.begin(), x0.end(), 0.0);
+            std::fill(y0.begin(), y0.end(), 0.0);
+            if (fill(user, idx, cost.data(), rlo.data(), rup.data(), x0.data(), y0.data(), &warm) != 0)
+                throw std::runtime_error("family fill callback failed for instance " + std::to_string(idx));
+            nir_lp lp{};
+            lp.nrow = m;
+            lp.ncol = n;
+            lp.col_start = F->col_start.data();
+            lp.row_index = F->row_index.data();
+            lp.value = F->value.data();
+            lp.cost = cost.data();
+            lp.collo = F->collo.data();
+            lp.colup = F->colup.data();
+            lp.rowlo = rlo.data();
+            lp.rowup = rup.data();
+            lp.obj_offset = F->offset;
+            nir_params p = F->prm;
+            p.x0 = (warm & 1) ? x0.data() : nullptr;
+            p.y0 = (warm & 2) ? y0.data() : nullptr;
+            Prepared P;
+            P.index = idx;
+            pdlp_prepare_instance(&lp, p, F->R, F->C, F->eta, false, P.inst);
+            ready.push_back(std::move(P));
+        };
+        std::vector<double> hc(S * n), hx(S * n), hrl(S * m), hru(S * m), hy(S * m);
+        std::vector<PdlpState> hst(S);
+        std::vector<int> hl(S);
+        // Move prepared instances into free lanes (chunks of S), then
+        // evaluate them at their start point.
+        auto refill = [&]() {
+            bool any = false;
+            while (!ready.empty() && !free_lanes.empty()) {
+                const size_t k = std::min({S, ready.size(), free_lanes.size()});
+                for (size_t f = 0; f < k; ++f) {
+                    Prepared& P = ready[ready.size() - k + f];
+                    const int l = free_lanes.back();
+                    free_lanes.pop_back();
+                    hl[f] = l;
+                    lane_inst[l] = P.index;
+                    std::copy(P.inst.c.begin(), P.inst.c.end(), hc.begin() + f * n);
+                    std::copy(P.inst.x0.begin(), P.inst.x0.end(), hx.begin() + f * n);
+                    std::copy(P.inst.rl.begin(), P.inst.rl.end(), hrl.begin() + f * m);
+                    std::copy(P.inst.ru.begin(), P.inst.ru.end(), hru.begin() + f * m);
+                    std::copy(P.inst.y0.begin(), P.inst.y0.end(), hy.begin() + f * m);
+                    hst[f] = P.inst.st;
+                }
+                ready.resize(ready.size() - k);
+                F->s_lanes.upload(hl.data(), k, F->s);
+                F->s_c.upload(hc.data(), k * n, F->s);
+                F->s_x.upload(hx.data(), k * n, F->s);
+                F->s_rl.upload(hrl.data(), k * m, F->s);
+                F->s_ru.upload(hru.data(), k * m, F->s);
+                F->s_y.upload(hy.data(), k * m, F->s);
+                F->s_st.upload(hst.data(), k, F->s);
+                const int gx = std::max(1, std::min(64, (std::max(n, m) + 255) / 256));
+                k_fam_scatter<<<dim3(gx, static_cast<unsigned>(k)), 256, 0, F->s>>>(
+                    F->view(), static_cast<int>(k), F->s_lanes.p, F->s_c.p, F->s_x.p, F->s_rl.p, F->s_ru.p, F->s_y.p,
+                    F->s_st.p);
+                HIP_CHECK(hipGetLastError());
+                // The staging buffers are reused by the next chunk.
+                HIP_CHECK(hipStreamSynchronize(F->s));
+                active += static_cast<int>(k);
+                any = true;
+            }
+            if (any) {
+                launch_eval<1>(F, F->view());
+                HIP_CHECK(hipGetLastError());
+                ++F->stats.refills;
+            }
+        };
+        std::vector<double> gx_(S * n), gc_(S * n), gy_(S * m);
+        std::vector<PdlpState> gst(S);
+        // Harvest every finished lane: T(z) unscaled, the objective in
+        // double-double from the caller's costs (scaled cost / C, exact), the
+        // status and counters as nir_session_result reports them.
+        auto harvest = [&]() {
+            F->ldone.download(F->h_done.data(), F->h_done.size(), F->s);
+            HIP_CHECK(hipStreamSynchronize(F->s));
+            std::vector<int> fin;
+            for (int l = 0; l < F->lanes; ++l)
+                if (F->h_done[l] && lane_inst[l] >= 0) fin.push_back(l);
+            for (size_t b = 0; b < fin.size(); b += S) {
+                const size_t k = std::min(S, fin.size() - b);
+                std::copy(fin.begin() + b, fin.begin() + b + k, hl.begin());
+                F->s_lanes.upload(hl.data(), k, F->s);
+                const int gx = std::max(1, std::min(64, (std::max(n, m) + 255) / 256));
+                k_fam_gather<<<dim3(gx, static_cast<unsigned>(k)), 256, 0, F->s>>>(
+                    F->view(), static_cast<int>(k), F->s_lanes.p, F->s_x.p, F->s_c.p, F->s_y.p, F->s_st.p);
+                HIP_CHECK(hipGetLastError());
+                F->s_x.download(gx_.data(), k * n, F->s);
+                F->s_c.download(gc_.data(), k * n, F->s);
+                F->s_y.download(gy_.data(), k * m, F->s);
+                F->s_st.download(gst.data(), k, F->s);
+                HIP_CHECK(hipStreamSynchronize(F->s));
+                for (size_t f = 0; f < k; ++f) {
+                    const int l = hl[f];
+                    double* xs = gx_.data() + f * n;
+                    double* ys = gy_.data() + f * m;
+                    const double* cs = gc_.data() + f * n;
+                    for (int j = 0; j < n; ++j) xs[j] *= F->C[j];
+                    for (int i = 0; i < m; ++i) ys[i] *= F->R[i];
+                    const PdlpState& h = gst[f];
+                    nir_info info;
+                    std::memset(&info, 0, sizeof(info));
+                    info.status = sov_status(h);
+                    info.iterations = h.total;
+                    info.restarts = h.restarts;
+                    DDHost acc(F->offset);
+                    for (int j = 0; j < n; ++j) {
+                        const double cj = cs[j] / F->C[j];
+                        if (cj != 0.0 && xs[j] != 0.0) acc.add_prod(cj, xs[j]);
+                    }
+                    info.objective = acc.value();
+                    info.tau = h.tau;
+                    info.sigma = h.sigma;
+                    info.primal_weight = h.omega;
+                    info.rel_primal = h.rel_p;
+                    info.rel_dual = h.rel_d;
+                    info.rel_gap = h.rel_g;
+                    info.setup_seconds = F->stats.setup_seconds;
+                    F->stats.total_iterations += h.total;
+                    const long long idx = lane_inst[l];
+                    lane_inst[l] = -1;
+                    free_lanes.push_back(l);
+                    --active;
+                    ++finished;
+                    if (done(user, idx, xs, ys, &info) != 0)
+                        throw std::runtime_error("family done callback failed for instance " + std::to_string(idx));
+                }
+            }
+        };
+        bool stopped = false;
+        // Every member ends by its iteration cap, so some lane must finish
+        // within cap / eval_freq blocks of the last one that did.
+        const long long drain_bound = std::max<long long>(1, F->prm.max_iters) / std::max(1, F->prm.eval_freq) + 8;
+        long long blocks_idle = 0, finished_seen = 0;
+        while (finished < count) {
+            if (finished != finished_seen) {
+                finished_seen = finished;
+                blocks_idle = 0;
+            } else if (blocks_idle > drain_bound) {
+                throw std::runtime_error("family did not drain within its iteration bound");
+            }
+            // Keep enough prepared instances for every free lane.
+            while (next < count && static_cast<long long>(ready.size()) < static_cast<long long>(free_lanes.size()))
+                prepare(next++);
+            refill();
+            if (active == 0) {
+                harvest();   // instances that finished at their start point
+                continue;
+            }
+            if (F->prm.should_stop && F->prm.should_stop(F->prm.stop_user)) {
+                k_fam_stop<<<(F->lanes + 255) / 256, 256, 0, F->s>>>(F->view(), F->lanes);
+                HIP_CHECK(hipGetLastError());
+                harvest();
+                stopped = true;
+                break;
+            }
+            run_block(F);
+            ++F->stats.blocks;
+            ++blocks_idle;
+            // Prepare the next instances while the block runs.
+            while (next < count && static_cast<long long>(ready.size()) < std::max<long long>(1, F->lanes / 16))
+                prepare(next++);
+            harvest();
+        }
+        HIP_CHECK(hipStreamSynchronize(F->s));
+        F->stats.instances += finished;
+        F->stats.stopped = stopped ? 1 : 0;
+        F->stats.solve_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return 0;
+    } catch (const std::exception& ex) {
+        F->ctx->error = ex.what();
+        (void)hipStreamSynchronize(F->s);
+        return -1;
+    }
+}
+
+NIR_API int nir_family_get_stats(nir_family* F, nir_family_stats* out) {
+    if (!F || !out) return -1;
+    *out = F->stats;
+    return 0;
+}
+
+NIR_API void nir_family_destroy(nir_family* F) {
+    if (!F) return;
+    (void)hipSetDevice(F->ctx->device);
+    (void)hipStreamSynchronize(F->s);
+    delete F;
+}
+
+}  // extern "C"

@@ -741,36 +741,16 @@ void spmv_plain(const DevMatrix& M, const double* in, double* out, hipStream_t s
 
 }  // namespace
 
-void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm, DevMatrix* Am, DevMatrix* ATm) {
-    auto t0 = std::chrono::steady_clock::now();
-    P.n = lp->ncol;
-    P.m = lp->nrow;
-    P.nnz = P.n > 0 ? lp->col_start[P.n] : 0;
-    P.A = Am;
-    P.AT = ATm;
-    P.eval_freq = std::max(1, prm.eval_freq);
-    P.use_graph = prm.use_graph != 0;
-    P.tol = prm.tol > 0 ? prm.tol : 1e-6;
-    P.mixed = prm.precision == NIR_PREC_MIXED;
-    P.polish_tol = prm.polish_tol;
-    P.offset = lp->obj_offset;
-    P.should_stop = prm.should_stop;
-    P.stop_user = prm.stop_user;
-    P.h_c.assign(lp->cost, lp->cost + P.n);
-    const int n = P.n, m = P.m;
-
+double pdlp_prepare_matrix(hipStream_t s, int m, int n, long long nnz, DevMatrix* Am, DevMatrix* ATm, double* dR,
+                           double* dC, std::vector<double>& R, std::vector<double>& Cc) {
     // ---- exponent-domain Ruiz (10 passes) + Pock-Chambolle, on the device
-    P.R.alloc(m);
-    P.C.alloc(n);
     Buf<double> mr, mc, fr, fc, onesr, onesc;
     mr.alloc(m); mc.alloc(n); fr.alloc(m); fc.alloc(n); onesr.alloc(m); onesc.alloc(n);
-    {
-        std::vector<double> ones_m(static_cast<size_t>(m), 1.0), ones_n(static_cast<size_t>(n), 1.0);
-        P.R.up(ones_m.data(), m, s);
-        P.C.up(ones_n.data(), n, s);
-        onesr.up(ones_m.data(), m, s);
-        onesc.up(ones_n.data(), n, s);
-    }
+    std::vector<double> ones_m(static_cast<size_t>(m), 1.0), ones_n(static_cast<size_t>(n), 1.0);
+    if (m) PD_CHECK(hipMemcpyAsync(dR, ones_m.data(), m * sizeof(double), hipMemcpyHostToDevice, s));
+    if (n) PD_CHECK(hipMemcpyAsync(dC, ones_n.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+    onesr.up(ones_m.data(), m, s);
+    onesc.up(ones_n.data(), n, s);
     const CsrShape& SA = Am->shape;
     const CsrShape& ST = ATm->shape;
     double* a_val = Am->val.p;
@@ -780,70 +760,23 @@ void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm,
         const int what = pass < 10 ? 0 : 1;   // 10 Ruiz (max |a|) passes, then one Pock-Chambolle (sum |a|)
         if (m > 0) {
             k_row_measure<<<gm, 256, 0, s>>>(SA.rowptr.p, SA.nseg, m, a_val, mr.p, what);
-            k_factor<<<gm, 256, 0, s>>>(mr.p, fr.p, P.R.p, m);
+            k_factor<<<gm, 256, 0, s>>>(mr.p, fr.p, dR, m);
             k_scale_csr<<<gm, 256, 0, s>>>(SA.rowptr.p, SA.nseg, m, SA.col.p, a_val, fr.p, onesc.p);
         }
         if (n > 0) {
             k_scale_csr<<<gn, 256, 0, s>>>(ST.rowptr.p, ST.nseg, n, ST.col.p, at_val, onesc.p, fr.p);
             k_row_measure<<<gn, 256, 0, s>>>(ST.rowptr.p, ST.nseg, n, at_val, mc.p, what);
-            k_factor<<<gn, 256, 0, s>>>(mc.p, fc.p, P.C.p, n);
+            k_factor<<<gn, 256, 0, s>>>(mc.p, fc.p, dC, n);
             k_scale_csr<<<gn, 256, 0, s>>>(ST.rowptr.p, ST.nseg, n, ST.col.p, at_val, fc.p, onesr.p);
         }
         if (m > 0) k_scale_csr<<<gm, 256, 0, s>>>(SA.rowptr.p, SA.nseg, m, SA.col.p, a_val, onesr.p, fc.p);
         PD_CHECK(hipGetLastError());
     }
-    std::vector<double> R(static_cast<size_t>(m)), Cc(static_cast<size_t>(n));
-    P.R.down(R.data(), m, s);
-    P.C.down(Cc.data(), n, s);
+    R.assign(static_cast<size_t>(m), 0.0);
+    Cc.assign(static_cast<size_t>(n), 0.0);
+    if (m) PD_CHECK(hipMemcpyAsync(R.data(), dR, m * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (n) PD_CHECK(hipMemcpyAsync(Cc.data(), dC, n * sizeof(double), hipMemcpyDeviceToHost, s));
     PD_CHECK(hipStreamSynchronize(s));
-
-    // ---- scaled vectors (absent bounds as +-inf) and caller-unit scales
-    auto finite_b = [](double b) { return std::isfinite(b) && std::fabs(b) < 1e20; };
-    std::vector<double> c(n), lo(n), up(n), rl(m), ru(m), rt(m), ct(n);
-    DDHost bn, cn;
-    for (int j = 0; j < n; ++j) {
-        c[j] = lp->cost[j] * Cc[j];
-        lo[j] = finite_b(lp->collo[j]) ? lp->collo[j] / Cc[j] : -INFINITY;
-        up[j] = finite_b(lp->colup[j]) ? lp->colup[j] / Cc[j] : INFINITY;
-        const double us = prm.unscale_col ? prm.unscale_col[j] : 1.0;
-        ct[j] = Cc[j] / us;   // lambda_caller = lambda' / C  ->  lambda_user = lambda_caller / unscale
-        const double cu = lp->cost[j] / us;
-        cn.add_prod(cu, cu);
-    }
-    for (int i = 0; i < m; ++i) {
-        rl[i] = finite_b(lp->rowlo[i]) ? lp->rowlo[i] * R[i] : -INFINITY;
-        ru[i] = finite_b(lp->rowup[i]) ? lp->rowup[i] * R[i] : INFINITY;
-        const double us = prm.unscale_row ? prm.unscale_row[i] : 1.0;
-        rt[i] = R[i] * us;
-        double b = 0.0;
-        if (finite_b(lp->rowlo[i])) b = std::fabs(lp->rowlo[i]);
-        if (finite_b(lp->rowup[i])) b = std::max(b, std::fabs(lp->rowup[i]));
-        b = b / us;
-        bn.add_prod(b, b);
-    }
-    P.d.alloc(n, m);
-    P.d.c.up(c.data(), n, s);
-    P.d.lo.up(lo.data(), n, s);
-    P.d.up.up(up.data(), n, s);
-    P.d.rl.up(rl.data(), m, s);
-    P.d.ru.up(ru.data(), m, s);
-    P.Rtot.alloc(m);
-    P.Ctot.alloc(n);
-    P.Rtot.up(rt.data(), m, s);
-    P.Ctot.up(ct.data(), n, s);
-
-    // ---- start point: clamp(0) or the warm start, in the scaled space
-    std::vector<double> x0(n), y0(m, 0.0);
-    for (int j = 0; j < n; ++j) {
-        double v = prm.x0 ? prm.x0[j] / Cc[j] : 0.0;
-        v = (lo[j] < v) ? v : lo[j];
-        v = (v < up[j]) ? v : up[j];
-        x0[j] = v;
-    }
-    if (prm.y0)
-        for (int i = 0; i < m; ++i) y0[i] = prm.y0[i] / R[i];
-    for (auto* b : {&P.d.x, &P.d.xa, &P.d.xp, &P.d.xbar}) b->up(x0.data(), n, s);
-    for (auto* b : {&P.d.y, &P.d.ya, &P.d.yp}) b->up(y0.data(), m, s);
 
     // ---- power iteration on A'^T A'
     Buf<double> u, v;
@@ -867,7 +800,7 @@ void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm,
         return std::sqrt(dd_val(t));
     };
     double lam = 0.0;
-    if (n > 0 && m > 0 && P.nnz > 0) {
+    if (n > 0 && m > 0 && nnz > 0) {
         double nu = norm_of_u();
         if (nu > 0) k_scale_vec<<<gn, 256, 0, s>>>(u.p, n, nu);
         for (int it = 0; it < 200; ++it) {
@@ -881,9 +814,52 @@ void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm,
             if (it > 0 && std::fabs(lam - prev) <= 1e-7 * lam) break;
         }
     }
+    PD_CHECK(hipStreamSynchronize(s));
     const double smax = lam > 0.0 ? std::sqrt(lam) : 1.0;
-    const double eta = 0.998 / smax;
+    return 0.998 / smax;
+}
 
+void pdlp_prepare_instance(const nir_lp* lp, const nir_params& prm, const std::vector<double>& R,
+                           const std::vector<double>& Cc, double eta, bool mixed, PdlpInstance& out) {
+    const int n = lp->ncol, m = lp->nrow;
+    const double tol = prm.tol > 0 ? prm.tol : 1e-6;
+    // ---- scaled vectors (absent bounds as +-inf) and caller-unit scales
+    auto finite_b = [](double b) { return std::isfinite(b) && std::fabs(b) < 1e20; };
+    std::vector<double>&c = out.c, &lo = out.lo, &up = out.up, &rl = out.rl, &ru = out.ru, &rt = out.rt,
+                       &ct = out.ct;
+    c.resize(n); lo.resize(n); up.resize(n); rl.resize(m); ru.resize(m); rt.resize(m); ct.resize(n);
+    DDHost bn, cn;
+    for (int j = 0; j < n; ++j) {
+        c[j] = lp->cost[j] * Cc[j];
+        lo[j] = finite_b(lp->collo[j]) ? lp->collo[j] / Cc[j] : -INFINITY;
+        up[j] = finite_b(lp->colup[j]) ? lp->colup[j] / Cc[j] : INFINITY;
+        const double us = prm.unscale_col ? prm.unscale_col[j] : 1.0;
+        ct[j] = Cc[j] / us;   // lambda_caller = lambda' / C  ->  lambda_user = lambda_caller / unscale
+        const double cu = lp->cost[j] / us;
+        cn.add_prod(cu, cu);
+    }
+    for (int i = 0; i < m; ++i) {
+        rl[i] = finite_b(lp->rowlo[i]) ? lp->rowlo[i] * R[i] : -INFINITY;
+        ru[i] = finite_b(lp->rowup[i]) ? lp->rowup[i] * R[i] : INFINITY;
+        const double us = prm.unscale_row ? prm.unscale_row[i] : 1.0;
+        rt[i] = R[i] * us;
+        double b = 0.0;
+        if (finite_b(lp->rowlo[i])) b = std::fabs(lp->rowlo[i]);
+        if (finite_b(lp->rowup[i])) b = std::max(b, std::fabs(lp->rowup[i]));
+        b = b / us;
+        bn.add_prod(b, b);
+    }
+    // ---- start point: clamp(0) or the warm start, in the scaled space
+    out.x0.assign(static_cast<size_t>(n), 0.0);
+    out.y0.assign(static_cast<size_t>(m), 0.0);
+    for (int j = 0; j < n; ++j) {
+        double v = prm.x0 ? prm.x0[j] / Cc[j] : 0.0;
+        v = (lo[j] < v) ? v : lo[j];
+        v = (v < up[j]) ? v : up[j];
+        out.x0[j] = v;
+    }
+    if (prm.y0)
+        for (int i = 0; i < m; ++i) out.y0[i] = prm.y0[i] / R[i];
     // ---- primal weight and state
     DDHost cs, bs;
     for (int j = 0; j < n; ++j) cs.add_prod(c[j], c[j]);
@@ -896,18 +872,62 @@ void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm,
     const double cnorm = std::sqrt(cs.value()), bnorm = std::sqrt(bs.value());
     const double omega = prm.primal_weight > 0 ? prm.primal_weight
                                                : ((cnorm > 1e-10 && bnorm > 1e-10) ? cnorm / bnorm : 1.0);
-    PdlpState h{};
+    PdlpState& h = out.st;
+    h = PdlpState{};
     h.omega = omega;
     h.eta = eta;
     h.tau = eta / omega;
     h.sigma = eta * omega;
-    h.tol = P.mixed ? P.tol : (P.polish_tol > 0 ? std::min(P.tol, P.polish_tol) : P.tol);
+    h.tol = mixed ? tol : (prm.polish_tol > 0 ? std::min(tol, prm.polish_tol) : tol);
     h.bnorm_o = std::sqrt(bn.value());
     h.cnorm_o = std::sqrt(cn.value());
-    h.offset = P.offset;
+    h.offset = lp->obj_offset;
     h.max_iters = std::max<long long>(1, prm.max_iters);
-    h.eval_freq = P.eval_freq;
-    h.watch_stall = P.mixed ? 1 : 0;
+    h.eval_freq = std::max(1, prm.eval_freq);
+    h.watch_stall = mixed ? 1 : 0;
+}
+
+void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm, DevMatrix* Am, DevMatrix* ATm) {
+    auto t0 = std::chrono::steady_clock::now();
+    P.n = lp->ncol;
+    P.m = lp->nrow;
+    P.nnz = P.n > 0 ? lp->col_start[P.n] : 0;
+    P.A = Am;
+    P.AT = ATm;
+    P.eval_freq = std::max(1, prm.eval_freq);
+    P.use_graph = prm.use_graph != 0;
+    P.tol = prm.tol > 0 ? prm.tol : 1e-6;
+    P.mixed = prm.precision == NIR_PREC_MIXED;
+    P.polish_tol = prm.polish_tol;
+    P.offset = lp->obj_offset;
+    P.should_stop = prm.should_stop;
+    P.stop_user = prm.stop_user;
+    P.h_c.assign(lp->cost, lp->cost + P.n);
+    const int n = P.n, m = P.m;
+    const CsrShape& SA = Am->shape;
+    const CsrShape& ST = ATm->shape;
+    double* a_val = Am->val.p;
+    double* at_val = ATm->val.p;
+
+    P.R.alloc(m);
+    P.C.alloc(n);
+    std::vector<double> R, Cc;
+    const double eta = pdlp_prepare_matrix(s, m, n, P.nnz, Am, ATm, P.R.p, P.C.p, R, Cc);
+    PdlpInstance I;
+    pdlp_prepare_instance(lp, prm, R, Cc, eta, P.mixed, I);
+    P.d.alloc(n, m);
+    P.d.c.up(I.c.data(), n, s);
+    P.d.lo.up(I.lo.data(), n, s);
+    P.d.up.up(I.up.data(), n, s);
+    P.d.rl.up(I.rl.data(), m, s);
+    P.d.ru.up(I.ru.data(), m, s);
+    P.Rtot.alloc(m);
+    P.Ctot.alloc(n);
+    P.Rtot.up(I.rt.data(), m, s);
+    P.Ctot.up(I.ct.data(), n, s);
+    for (auto* b : {&P.d.x, &P.d.xa, &P.d.xp, &P.d.xbar}) b->up(I.x0.data(), n, s);
+    for (auto* b : {&P.d.y, &P.d.ya, &P.d.yp}) b->up(I.y0.data(), m, s);
+    const PdlpState h = I.st;
     P.st.alloc(1);
     P.st.up(&h, 1, s);
     PD_CHECK(hipHostMalloc(reinterpret_cast<void**>(&P.h_st), 2 * sizeof(PdlpState), hipHostMallocDefault));

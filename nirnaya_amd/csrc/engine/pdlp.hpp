@@ -4,7 +4,10 @@
 
 #include <cmath>
 
+#include <vector>
+
 #include "../../../include/nirnaya/pdhg.h"
+#include "pdlp_core.hpp"
 #include "spmv.hpp"
 
 namespace nir {
@@ -43,6 +46,23 @@ void pdlp_run(Pdlp& P, hipStream_t s, const nir_params& prm, long long budget);
 int pdlp_result(Pdlp& P, hipStream_t s, double* x, double* y, nir_info* info);
 int pdlp_time(Pdlp& P, hipStream_t s, long long iters, nir_timing* out);
 void pdlp_scaling(Pdlp& P, hipStream_t s, double* R, double* C);
+
+// The matrix half of pdlp mode's set-up, shared by the single-LP engine and
+// the family engine: scales the device matrices A and A^T in place (10 Ruiz
+// passes, one Pock-Chambolle pass; factors accumulated into the device
+// arrays dR, dC and returned in R, C) and returns the step scale
+// eta = 0.998 / sigma_max of the scaled matrix (power iteration).
+double pdlp_prepare_matrix(hipStream_t s, int m, int n, long long nnz, DevMatrix* A, DevMatrix* AT, double* dR,
+                           double* dC, std::vector<double>& R, std::vector<double>& C);
+
+// The instance half, on the host: an LP's costs and bounds in the scaled
+// space, its caller-unit scales, its start point and its initial state.
+struct PdlpInstance {
+    std::vector<double> c, lo, up, rl, ru, rt, ct, x0, y0;
+    PdlpState st;
+};
+void pdlp_prepare_instance(const nir_lp* lp, const nir_params& prm, const std::vector<double>& R,
+                           const std::vector<double>& C, double eta, bool mixed, PdlpInstance& out);
 
 }  // namespace gpu
 }  // namespace nir

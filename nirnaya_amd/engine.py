@@ -37,6 +37,7 @@ _dp = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)
 _i32p = C.POINTER(C.c_int32)
 _STOP_CB = C.CFUNCTYPE(C.c_int, C.c_void_p)
+_FAMILY_FILL_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, _dp, _dp, _dp, _dp, _dp, _i32p)
 
 
 class NirLp(C.Structure):
@@ -97,6 +98,19 @@ class NirVerifyReport(C.Structure):
                 ("obj_rel_err", C.c_double), ("worst_col", C.c_int32), ("worst_col_upper", C.c_int32),
                 ("worst_row", C.c_int32), ("worst_row_upper", C.c_int32), ("nonfinite_col", C.c_int32),
                 ("passed", C.c_int32)]
+
+
+_FAMILY_DONE_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, _dp, _dp, C.POINTER(NirInfo))
+
+
+class NirFamilyOpts(C.Structure):
+    _fields_ = [("lanes", C.c_int32), ("reserved", C.c_int32)]
+
+
+class NirFamilyStats(C.Structure):
+    _fields_ = [("lanes", C.c_int32), ("stopped", C.c_int32), ("instances", C.c_int64), ("blocks", C.c_int64),
+                ("refills", C.c_int64), ("total_iterations", C.c_int64), ("setup_seconds", C.c_double),
+                ("solve_seconds", C.c_double)]
 
 
 class NirDevLp(C.Structure):
@@ -165,6 +179,12 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.nir_mlp_gemm_check.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, _fp, _fp, C.c_int32, C.c_int32, _fp]
     lib.nir_verify_solution.argtypes = [vp, C.POINTER(NirLp), _dp, C.c_double, C.c_double,
                                         C.POINTER(NirVerifyReport)]
+    lib.nir_family_opts_default.argtypes = [C.POINTER(NirFamilyOpts)]
+    lib.nir_family_create.argtypes = [vp, C.POINTER(NirLp), C.POINTER(NirParams), C.POINTER(NirFamilyOpts),
+                                      C.POINTER(vp)]
+    lib.nir_family_solve.argtypes = [vp, C.c_int64, _FAMILY_FILL_CB, _FAMILY_DONE_CB, vp]
+    lib.nir_family_get_stats.argtypes = [vp, C.POINTER(NirFamilyStats)]
+    lib.nir_family_destroy.argtypes = [vp]
     lib.nir_verify_solution_noalloc.argtypes = [vp, C.POINTER(NirDevLp), vp, C.c_double, C.c_double,
                                                 C.POINTER(NirVerifyReport)]
     lib.nir_pod_gram.argtypes = [vp, _dp, C.c_int32, C.c_int32, _dp]
@@ -438,6 +458,9 @@ class Engine:
               workgroups: int = 0, x0s=None, y0s=None) -> "Batch":
         return Batch(self, lps, params, threads, quantum, rounds_per_graph, workgroups, x0s, y0s)
 
+    def family(self, base: LP, params: NirParams, lanes: int = 1024) -> "Family":
+        return Family(self, base, params, lanes)
+
     def session(self, lp: LP, params: Optional[NirParams] = None, x0=None, y0=None) -> "Session":
         return Session(self, lp, params, x0, y0)
 
@@ -528,6 +551,87 @@ class Session:
     def close(self):
         if getattr(self, "h", None):
             self.lib.nir_session_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Family:
+    """Members of a parametric family (shared A and column bounds, own costs
+    and row bounds) streamed through the family engine (nir_family_*)."""
+
+    def __init__(self, eng: Engine, base: LP, params: NirParams, lanes: int = 1024):
+        self.eng, self.lib, self.base = eng, eng.lib, base
+        self.params = params
+        self._s = base.struct()
+        opts = NirFamilyOpts()
+        self.lib.nir_family_opts_default(C.byref(opts))
+        opts.lanes = lanes
+        h = C.c_void_p()
+        if self.lib.nir_family_create(eng.h, C.byref(self._s), C.byref(params), C.byref(opts), C.byref(h)) != 0:
+            raise EngineError("nir_family_create: " + eng.error())
+        self.h = h
+
+    def solve(self, count: int, member, on_result=None):
+        """Solves members 0 .. count-1. member(i) returns (cost, rowlo, rowup)
+        or (cost, rowlo, rowup, x0, y0) with x0 / y0 possibly None;
+        on_result(i, x, y, info) receives each solution as it finishes
+        (copies); without it the results are returned as a list of Result."""
+        m, n = self.base.m, self.base.n
+        out = [None] * count if on_result is None else None
+        err = []
+
+        def fill(_u, idx, cost, rl, ru, x0, y0, warm):
+            try:
+                v = member(int(idx))
+                C.memmove(cost, np.ascontiguousarray(v[0], np.float64).ctypes.data, 8 * n)
+                C.memmove(rl, np.ascontiguousarray(v[1], np.float64).ctypes.data, 8 * m)
+                C.memmove(ru, np.ascontiguousarray(v[2], np.float64).ctypes.data, 8 * m)
+                w = 0
+                if len(v) > 3 and v[3] is not None:
+                    C.memmove(x0, np.ascontiguousarray(v[3], np.float64).ctypes.data, 8 * n)
+                    w |= 1
+                if len(v) > 4 and v[4] is not None:
+                    C.memmove(y0, np.ascontiguousarray(v[4], np.float64).ctypes.data, 8 * m)
+                    w |= 2
+                warm[0] = w
+                return 0
+            except Exception as e:   # reported after the call returns
+                err.append(e)
+                return 1
+
+        def done(_u, idx, x, y, info):
+            try:
+                xs = np.ctypeslib.as_array(x, shape=(n,)).copy() if n else np.zeros(0)
+                ys = np.ctypeslib.as_array(y, shape=(m,)).copy() if m else np.zeros(0)
+                inf = NirInfo.from_buffer_copy(info.contents)
+                if on_result is None:
+                    out[int(idx)] = Result(xs, ys, inf.status, inf.iterations, inf.objective, inf.tau, inf.sigma, inf)
+                else:
+                    on_result(int(idx), xs, ys, inf)
+                return 0
+            except Exception as e:
+                err.append(e)
+                return 1
+
+        fcb, dcb = _FAMILY_FILL_CB(fill), _FAMILY_DONE_CB(done)
+        rc = self.lib.nir_family_solve(self.h, int(count), fcb, dcb, None)
+        if err:
+            raise err[0]
+        if rc != 0:
+            raise EngineError("nir_family_solve: " + self.eng.error())
+        return out
+
+    def stats(self) -> NirFamilyStats:
+        st = NirFamilyStats()
+        if self.lib.nir_family_get_stats(self.h, C.byref(st)) != 0:
+            raise EngineError("nir_family_get_stats: " + self.eng.error())
+        return st
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nir_family_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -1,0 +1,141 @@
+"""GPU parity of the parametric-family engine (nir_family_*) against the
+single-LP engine and the oracle.
+
+Members share A and the column bounds and have their own costs and row
+bounds; each runs on a lane of a 64-member group with interleaved vectors.
+Every member must return exactly what nir_pdhg_solve returns for it alone:
+x and y bit for bit, the same status, iteration and restart counts and
+objective, whatever lane it ran on and whenever its lane was refilled. The
+single-LP FP64 trajectory is itself or_pdlp's, which one member checks
+directly."""
+import numpy as np
+import pytest
+
+from nirnaya_amd.engine import LP, MODE_PDLP, _STOP_CB, default_params
+from nirnaya_amd.warmstart import ParametricFamily
+from oracle_lib import Oracle
+import lpkit
+
+pytestmark = pytest.mark.gpu
+
+
+def same(a, b):
+    return (np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y) and a.status == b.status
+            and a.iterations == b.iterations and a.info.restarts == b.info.restarts and a.objective == b.objective
+            and a.info.primal_weight == b.info.primal_weight)
+
+
+def members(fam, thetas):
+    def member(i):
+        lp = fam.instance(thetas[i])
+        return lp.c, lp.rl, lp.ru
+    return member
+
+
+@pytest.mark.parametrize("lanes", [64, 128])
+def test_family_members_match_single_lp(engine, lanes):
+    """100 members of a 400 x 1,000 family through 64 or 128 lanes (lanes
+    refilled many times over)."""
+    fam = ParametricFamily(20264, 400, 1000, 5)
+    th = fam.thetas(100, 11)
+    params = default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-6)
+    F = engine.family(fam.base, params, lanes=lanes)
+    try:
+        res = F.solve(len(th), members(fam, th))
+        st = F.stats()
+    finally:
+        F.close()
+    assert st.instances == len(th) and (st.refills >= 2 or lanes >= len(th))
+    its = []
+    for i, t in enumerate(th):
+        s = engine.solve(fam.instance(t), params)
+        its.append(s.iterations)
+        assert same(res[i], s), f"member {i}: it {res[i].iterations}/{s.iterations}"
+    print(f"lanes {lanes}: 100 members equal the single-LP engine; iterations {min(its)}..{max(its)}, "
+          f"blocks {st.blocks}, refills {st.refills}, solve {st.solve_seconds:.2f}s")
+    xo, yo, info = Oracle(fam.instance(th[0])).pdlp(max_iters=400000, tol=1e-6)
+    assert res[0].iterations == info.iterations and np.array_equal(res[0].x, xo) and np.array_equal(res[0].y, yo)
+
+
+def test_family_warm_starts_cap_and_start_point_optimum(engine):
+    """Per-member warm starts (x only, y only, both), a member warm-started
+    at its optimum (done at iteration 0), and an iteration cap (status 6)."""
+    fam = ParametricFamily(20264, 300, 700, 5)
+    th = fam.thetas(12, 5)
+    params = default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-6, primal_weight=0.5)
+    cold = [engine.solve(fam.instance(t), params) for t in th]
+    rng = np.random.default_rng(2)
+    starts = []
+    for i, r in enumerate(cold):
+        x0 = r.x * (1 + 0.01 * rng.standard_normal(r.x.size)) if i % 3 != 1 else None
+        y0 = r.y * 0.9 if i % 3 != 0 else None
+        starts.append((x0, y0))
+    F = engine.family(fam.base, params, lanes=64)
+    try:
+        res = F.solve(len(th), lambda i: (*members(fam, th)(i), *starts[i]))
+    finally:
+        F.close()
+    for i, t in enumerate(th):
+        s = engine.solve(fam.instance(t), params, x0=starts[i][0], y0=starts[i][1])
+        assert same(res[i], s), f"member {i}"
+    unit = lpkit.pdhg_unit_lp()
+    p9 = default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-9)
+    F = engine.family(unit, p9, lanes=64)
+    try:
+        r = F.solve(2, lambda i: (unit.c, unit.rl, unit.ru, np.array([3.0, 1.0]) if i == 0 else None,
+                                  np.array([1.5, 0.5]) if i == 0 else None))
+    finally:
+        F.close()
+    assert r[0].status == 0 and r[0].iterations == 0
+    assert same(r[1], engine.solve(unit, p9))
+    capped = default_params(mode=MODE_PDLP, max_iters=640, tol=1e-12)
+    F = engine.family(fam.base, capped, lanes=64)
+    try:
+        rc = F.solve(3, members(fam, th))
+    finally:
+        F.close()
+    for i in range(3):
+        s = engine.solve(fam.instance(th[i]), capped)
+        assert rc[i].status == 6 and rc[i].iterations == 640 and same(rc[i], s)
+
+
+def test_family_certifies_infeasible_members(engine):
+    """Members of an infeasible base (costs perturbed) end with the single-LP
+    engine's certificate status and iterate."""
+    for name, (kind, lp) in sorted(lpkit.infeasible_lps().items()):
+        params = default_params(mode=MODE_PDLP, max_iters=50000, tol=1e-6)
+        rng = np.random.default_rng(7)
+        cs = [lp.c * (1 + 0.05 * rng.standard_normal(lp.n)) for _ in range(5)]
+        F = engine.family(lp, params, lanes=64)
+        try:
+            res = F.solve(len(cs), lambda i: (cs[i], lp.rl, lp.ru))
+        finally:
+            F.close()
+        for i, c in enumerate(cs):
+            one = LP(lp.m, lp.n, lp.start, lp.index, lp.value, c, lp.l, lp.u, lp.rl, lp.ru)
+            s = engine.solve(one, params)
+            print(f"{name} member {i}: status {res[i].status}/{s.status} it {res[i].iterations}/{s.iterations}")
+            assert same(res[i], s)
+            assert res[i].status == 1 if kind == "dual" else res[i].status in (2, 3)
+
+
+def test_family_should_stop(engine):
+    fam = ParametricFamily(20264, 400, 1000, 5)
+    th = fam.thetas(130, 3)
+    calls = []
+
+    def stop(_u):
+        calls.append(1)
+        return 1 if len(calls) >= 4 else 0
+
+    params = default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-9)
+    cb = _STOP_CB(stop)
+    params.should_stop = cb
+    F = engine.family(fam.base, params, lanes=64)
+    try:
+        got = {}
+        F.solve(len(th), members(fam, th), on_result=lambda i, x, y, info: got.__setitem__(i, info.status))
+        st = F.stats()
+    finally:
+        F.close()
+    assert st.stopped == 1 and len(got) == 64 and set(got.values()) == {4}
