@@ -248,9 +248,11 @@ typedef struct {
 } nir_family_stats;
 /* Supplies member `index`: cost (ncol), rowlo and rowup (nrow), and an
  * optional warm start x0 (ncol) / y0 (nrow) flagged in *warm (bit 0: x0,
- * bit 1: y0; the buffers arrive zeroed). Returns 0, or nonzero to abort. */
+ * bit 1: y0; the buffers arrive zeroed) with an optional initial primal
+ * weight (*primal_weight > 0 replaces nir_params.primal_weight for this
+ * member). Returns 0, or nonzero to abort. */
 typedef int (*nir_family_fill_fn)(void* user, int64_t index, double* cost, double* rowlo, double* rowup,
-                                  double* x0, double* y0, int32_t* warm);
+                                  double* x0, double* y0, int32_t* warm, double* primal_weight);
 /* Receives member `index`'s solution (x: ncol, y: nrow, valid during the
  * call) and its nir_info; members arrive in the order they finish. */
 typedef int (*nir_family_done_fn)(void* user, int64_t index, const double* x, const double* y,

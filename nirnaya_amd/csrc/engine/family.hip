@@ -132,15 +132,14 @@ __global__ __launch_bounds__(256) void k_fam_primal(View V, int k_lo, int save) 
     const double* yv = V.y + static_cast<size_t>(g) * V.m * kL + lane;
     const size_t xb = static_cast<size_t>(g) * V.n * kL + lane;
     const int r1 = min(r0 + kRowsPerWave, V.n);
+    if (done) return;   // finished lanes issue no loads; their waves carry on for the others
     for (int r = r0; r < r1; ++r) {
         const double gs = row_sum(V.trp, V.tcol, V.tval, yv, r);
-        if (!done) {
-            const size_t e = xb + static_cast<size_t>(r) * kL;
-            const PrimalOut<double> o = primal_point<double>(V.x[e], gs, V.c[e], V.lo[r], V.up[r], V.xa[e], tau, hw);
-            V.xbar[e] = o.xb;
-            V.x[e] = o.next;
-            if (save) V.xp[e] = o.cand;
-        }
+        const size_t e = xb + static_cast<size_t>(r) * kL;
+        const PrimalOut<double> o = primal_point<double>(V.x[e], gs, V.c[e], V.lo[r], V.up[r], V.xa[e], tau, hw);
+        V.xbar[e] = o.xb;
+        V.x[e] = o.next;
+        if (save) V.xp[e] = o.cand;
     }
 }
 
@@ -157,14 +156,13 @@ __global__ __launch_bounds__(256) void k_fam_dual(View V, int k_lo, int save) {
     const double* xv = V.xbar + static_cast<size_t>(g) * V.n * kL + lane;
     const size_t yb = static_cast<size_t>(g) * V.m * kL + lane;
     const int r1 = min(r0 + kRowsPerWave, V.m);
+    if (done) return;
     for (int r = r0; r < r1; ++r) {
         const double h = row_sum(V.arp, V.acol, V.aval, xv, r);
-        if (!done) {
-            const size_t e = yb + static_cast<size_t>(r) * kL;
-            const DualOut<double> o = dual_point<double>(V.y[e], h, V.rl[e], V.ru[e], V.ya[e], sigma, hw);
-            V.y[e] = o.next;
-            if (save) V.yp[e] = o.yp;
-        }
+        const size_t e = yb + static_cast<size_t>(r) * kL;
+        const DualOut<double> o = dual_point<double>(V.y[e], h, V.rl[e], V.ru[e], V.ya[e], sigma, hw);
+        V.y[e] = o.next;
+        if (save) V.yp[e] = o.yp;
     }
 }
 
@@ -203,11 +201,12 @@ __global__ __launch_bounds__(256) void k_fam_eval_cols(View V) {
     const int g = blockIdx.y;
     const int lane = threadIdx.x & (kL - 1);
     const int li = g * kL + lane;
-    if (__all(!eval_lane<FRESH>(V, li))) return;
+    const bool serve = eval_lane<FRESH>(V, li);
+    if (__all(!serve)) return;
     dd acc[5] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
     const double* yv = V.yp + static_cast<size_t>(g) * V.m * kL + lane;
     const size_t xb = static_cast<size_t>(g) * V.n * kL + lane;
-    for (int r = blockIdx.x * 4 + wave_id(); r < V.n; r += gridDim.x * 4) {
+    for (int r = blockIdx.x * 4 + wave_id(); serve && r < V.n; r += gridDim.x * 4) {
         const double gs = row_sum(V.trp, V.tcol, V.tval, yv, r);
         const size_t e = xb + static_cast<size_t>(r) * kL;
         dd t[5];
@@ -225,11 +224,12 @@ __global__ __launch_bounds__(256) void k_fam_eval_rows(View V) {
     const int g = blockIdx.y;
     const int lane = threadIdx.x & (kL - 1);
     const int li = g * kL + lane;
-    if (__all(!eval_lane<FRESH>(V, li))) return;
+    const bool serve = eval_lane<FRESH>(V, li);
+    if (__all(!serve)) return;
     dd acc[6] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
     const double* xv = V.xp + static_cast<size_t>(g) * V.n * kL + lane;
     const size_t yb = static_cast<size_t>(g) * V.m * kL + lane;
-    for (int r = blockIdx.x * 4 + wave_id(); r < V.m; r += gridDim.x * 4) {
+    for (int r = blockIdx.x * 4 + wave_id(); serve && r < V.m; r += gridDim.x * 4) {
         const double h = row_sum(V.arp, V.acol, V.aval, xv, r);
         const size_t e = yb + static_cast<size_t>(r) * kL;
         dd t[6];
@@ -316,11 +316,12 @@ __global__ __launch_bounds__(256) void k_fam_cert_cols(View V) {
     const int g = blockIdx.y;
     const int lane = threadIdx.x & (kL - 1);
     const int li = g * kL + lane;
-    if (__all(!cert_lane(V, li))) return;
+    const bool serve = cert_lane(V, li);
+    if (__all(!serve)) return;
     dd acc[6] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
     const double* dyv = V.ry + static_cast<size_t>(g) * V.m * kL + lane;
     const size_t xb = static_cast<size_t>(g) * V.n * kL + lane;
-    for (int r = blockIdx.x * 4 + wave_id(); r < V.n; r += gridDim.x * 4) {
+    for (int r = blockIdx.x * 4 + wave_id(); serve && r < V.n; r += gridDim.x * 4) {
         const double lam = -dd_val(row_dot_lane(V.trp, V.tcol, V.tval, dyv, r));
         const size_t e = xb + static_cast<size_t>(r) * kL;
         cert_col_terms(lam, V.c[e], V.lo[r], V.up[r], V.rx[e], acc);
@@ -333,11 +334,12 @@ __global__ __launch_bounds__(256) void k_fam_cert_rows(View V) {
     const int g = blockIdx.y;
     const int lane = threadIdx.x & (kL - 1);
     const int li = g * kL + lane;
-    if (__all(!cert_lane(V, li))) return;
+    const bool serve = cert_lane(V, li);
+    if (__all(!serve)) return;
     dd acc[6] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
     const double* dxv = V.rx + static_cast<size_t>(g) * V.n * kL + lane;
     const size_t yb = static_cast<size_t>(g) * V.m * kL + lane;
-    for (int r = blockIdx.x * 4 + wave_id(); r < V.m; r += gridDim.x * 4) {
+    for (int r = blockIdx.x * 4 + wave_id(); serve && r < V.m; r += gridDim.x * 4) {
         const double h = dd_val(row_dot_lane(V.arp, V.acol, V.aval, dxv, r));
         const size_t e = yb + static_cast<size_t>(r) * kL;
         cert_row_terms(h, V.rl[e], V.ru[e], V.ry[e], acc);
@@ -715,9 +717,10 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
         // Prepare one instance on the host with the single-LP set-up code.
         auto prepare = [&](long long idx) {
             int32_t warm = 0;
+            double weight = 0.0;
             std::fill(x0.begin(), x0.end(), 0.0);
             std::fill(y0.begin(), y0.end(), 0.0);
-            if (fill(user, idx, cost.data(), rlo.data(), rup.data(), x0.data(), y0.data(), &warm) != 0)
+            if (fill(user, idx, cost.data(), rlo.data(), rup.data(), x0.data(), y0.data(), &warm, &weight) != 0)
                 throw std::runtime_error("family fill callback failed for instance " + std::to_string(idx));
             nir_lp lp{};
             lp.nrow = m;
@@ -734,6 +737,7 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
             nir_params p = F->prm;
             p.x0 = (warm & 1) ? x0.data() : nullptr;
             p.y0 = (warm & 2) ? y0.data() : nullptr;
+            if (weight > 0.0) p.primal_weight = weight;
             Prepared P;
             P.index = idx;
             pdlp_prepare_instance(&lp, p, F->R, F->C, F->eta, false, P.inst);

@@ -37,7 +37,7 @@ _dp = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)
 _i32p = C.POINTER(C.c_int32)
 _STOP_CB = C.CFUNCTYPE(C.c_int, C.c_void_p)
-_FAMILY_FILL_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, _dp, _dp, _dp, _dp, _dp, _i32p)
+_FAMILY_FILL_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, _dp, _dp, _dp, _dp, _dp, _i32p, _dp)
 
 
 class NirLp(C.Structure):
@@ -574,15 +574,16 @@ class Family:
         self.h = h
 
     def solve(self, count: int, member, on_result=None):
-        """Solves members 0 .. count-1. member(i) returns (cost, rowlo, rowup)
-        or (cost, rowlo, rowup, x0, y0) with x0 / y0 possibly None;
+        """Solves members 0 .. count-1. member(i) returns (cost, rowlo, rowup),
+        optionally followed by x0, y0 (each possibly None) and an initial
+        primal weight (0 or None = the parameters' own);
         on_result(i, x, y, info) receives each solution as it finishes
         (copies); without it the results are returned as a list of Result."""
         m, n = self.base.m, self.base.n
         out = [None] * count if on_result is None else None
         err = []
 
-        def fill(_u, idx, cost, rl, ru, x0, y0, warm):
+        def fill(_u, idx, cost, rl, ru, x0, y0, warm, weight):
             try:
                 v = member(int(idx))
                 C.memmove(cost, np.ascontiguousarray(v[0], np.float64).ctypes.data, 8 * n)
@@ -596,6 +597,8 @@ class Family:
                     C.memmove(y0, np.ascontiguousarray(v[4], np.float64).ctypes.data, 8 * m)
                     w |= 2
                 warm[0] = w
+                if len(v) > 5 and v[5]:
+                    weight[0] = float(v[5])
                 return 0
             except Exception as e:   # reported after the call returns
                 err.append(e)

@@ -46,9 +46,14 @@ class ParametricFamily:
     def thetas(self, count: int, seed: int) -> np.ndarray:
         return np.random.default_rng(seed).standard_normal((count, self.dim))
 
-    def instance(self, theta: np.ndarray) -> LP:
+    def rhs_cost(self, theta: np.ndarray):
+        """(b(theta), c(theta)) of the member at theta."""
         b = self.b0 * (1.0 + self.b_amp * np.tanh(self.B @ theta))
         c = self.c0 + self.c_amp * (self.C @ theta)
+        return b, c
+
+    def instance(self, theta: np.ndarray) -> LP:
+        b, c = self.rhs_cost(theta)
         base = self.base
         return LP(self.m, self.n, base.start, base.index, base.value, np.ascontiguousarray(c), base.l, base.u,
                   np.ascontiguousarray(b), b.copy())
@@ -116,13 +121,15 @@ class WarmStart:
     """theta -> (x0, y0) through the MLP and the two POD bases."""
 
     def __init__(self, engine: Engine, pod_x: Pod, pod_y: Pod, hidden: int = 512, seed: int = 0,
-                 max_rows: int = 512, dim: int = THETA_DIM, primal_weight: float = 0.0):
+                 max_rows: int = 512, dim: int = THETA_DIM, primal_weight: float = 0.0, out_width: int = 0):
         self.engine, self.pod_x, self.pod_y, self.dim = engine, pod_x, pod_y, dim
         # The primal weight the family's training solves ended with (median),
         # carried into warm solves with the predicted point; 0 = engine default.
         self.primal_weight = primal_weight
         self.n_coef = pod_x.rank + pod_y.rank
-        self.n_out = _pad64(self.n_coef)
+        # Output width: the coefficients padded to a multiple of 64, or a
+        # fixed width (512 for ranks 256 + 256) whose unused outputs train to 0.
+        self.n_out = max(_pad64(self.n_coef), _pad64(out_width))
         self.net = engine.mlp(dim, hidden, self.n_out, max_rows, seed)
         self.mu = np.zeros(self.n_out)
         self.sd = np.ones(self.n_out)
@@ -151,6 +158,31 @@ class WarmStart:
         return self.pod_x.decode(alpha[:rx]), self.pod_y.decode(alpha[rx:self.n_coef])
 
 
+def family_solve(engine: Engine, family: ParametricFamily, thetas: np.ndarray, params, starts=None,
+                 primal_weight: float = 0.0, lanes: int = 0, on_result=None):
+    """Members theta_i solved by the family engine (nir_family_*: one copy of
+    A, 64 members per wave, the single-LP engine's results bit for bit).
+    starts[i] = (x0, y0) warm-starts member i with `primal_weight`. Returns
+    (results in theta order, or None with on_result, and seconds)."""
+    import time
+    count = len(thetas)
+    lanes = lanes or min(1024, max(64, (count + 63) // 64 * 64))
+    F = engine.family(family.base, params, lanes=lanes)
+
+    def member(i):
+        b, c = family.rhs_cost(thetas[i])
+        if starts is None:
+            return c, b, b
+        x0, y0 = starts[i]
+        return c, b, b, x0, y0, primal_weight
+    try:
+        t0 = time.perf_counter()
+        res = F.solve(count, member, on_result)
+        return res, time.perf_counter() - t0
+    finally:
+        F.close()
+
+
 @dataclass
 class SolveRecord:
     status: int
@@ -171,6 +203,9 @@ class PipelineReport:
     train_solve_seconds: float = 0.0
     pod_seconds: float = 0.0
     train_seconds: float = 0.0
+    cold_seconds: float = 0.0
+    warm_seconds: float = 0.0
+    predict_seconds: float = 0.0
 
 
 def warm_params(params, primal_weight: float):
@@ -198,31 +233,30 @@ def solve_warm(engine: Engine, lp: LP, params, x0, y0, primal_weight: float = 0.
 
 def run_pipeline(engine: Engine, family: ParametricFamily, n_train: int, n_test: int, rank: int = 256,
                  epochs: int = 200, batch: int = 256, lr: float = 1e-3, tol: float = 1e-6, precision: int = 0,
-                 max_iters: int = 400000, seed: int = 7, log=None) -> PipelineReport:
-    """Generate training solves with the batched engine, fit the POD bases and
-    the MLP on the device, then solve held-out thetas cold and warm."""
+                 max_iters: int = 400000, seed: int = 7, log=None, train_tol: float = 0.0,
+                 out_width: int = 512) -> PipelineReport:
+    """Training solves by the family engine (to `train_tol`, default `tol`),
+    POD bases and the MLP fitted on the device, then the held-out thetas
+    solved cold and warm-started (the family engine again, to `tol`). A warm
+    solve that misses the tolerance falls back to its cold solve."""
     import time
     say = log or (lambda *_: None)
-    params = default_params(mode=MODE_PDLP, precision=precision, max_iters=max_iters, tol=tol)
+    if precision != 0:
+        raise ValueError("the family engine runs FP64")
+    params = default_params(mode=MODE_PDLP, max_iters=max_iters, tol=tol)
+    tparams = default_params(mode=MODE_PDLP, max_iters=max_iters, tol=train_tol or tol)
     th_train = family.thetas(n_train, seed)
     th_test = family.thetas(n_test, seed + 1)
-    t0 = time.perf_counter()
-    lps = [family.instance(t) for t in th_train]
-    b = engine.batch(lps, params)
-    try:
-        b.solve()
-        res = [b.result(i) for i in range(n_train)]
-    finally:
-        b.close()
-    t_solve = time.perf_counter() - t0
+    res, t_solve = family_solve(engine, family, th_train, tparams)
     ok = [i for i, r in enumerate(res) if r.status == 0]   # screen out failed / infeasible thetas
     say(f"training solves: {len(ok)}/{n_train} reached tolerance in {t_solve:.1f}s")
     t0 = time.perf_counter()
     pod_x = Pod(engine, np.stack([res[i].x for i in ok]), rank)
     pod_y = Pod(engine, np.stack([res[i].y for i in ok]), rank)
     t_pod = time.perf_counter() - t0
-    ws = WarmStart(engine, pod_x, pod_y, max_rows=max(64, batch),
-                   primal_weight=float(np.median([res[i].info.primal_weight for i in ok])))
+    omega = float(np.median([res[i].info.primal_weight for i in ok]))
+    del res
+    ws = WarmStart(engine, pod_x, pod_y, max_rows=max(64, batch), primal_weight=omega, out_width=out_width)
     t0 = time.perf_counter()
     losses = ws.fit(th_train[ok], epochs=epochs, batch=batch, lr=lr, seed=seed)
     t_fit = time.perf_counter() - t0
@@ -230,15 +264,18 @@ def run_pipeline(engine: Engine, family: ParametricFamily, n_train: int, n_test:
         f"({t_fit:.1f}s)")
     rep = PipelineReport(n_train, len(ok), pod_x.rank, pod_y.rank, [float(v) for v in losses],
                          train_solve_seconds=t_solve, pod_seconds=t_pod, train_seconds=t_fit)
-    for t in th_test:
-        lp = family.instance(t)
-        t0 = time.perf_counter()
-        c = engine.solve(lp, params)
-        rep.cold.append(SolveRecord(c.status, c.iterations, time.perf_counter() - t0))
-        x0, y0 = ws.predict(t)
-        _, w = solve_warm(engine, lp, params, x0, y0, ws.primal_weight)
-        rep.warm.append(w)
-        say(f"held-out: cold {c.iterations} it ({rep.cold[-1].seconds:.3f}s) warm {w.iterations} it "
-            f"({w.seconds:.3f}s){' fallback' if w.fallback else ''}")
+    cold, t_cold = family_solve(engine, family, th_test, params)
+    t0 = time.perf_counter()
+    preds = [ws.predict(t) for t in th_test]
+    t_pred = time.perf_counter() - t0
+    warm, t_warm = family_solve(engine, family, th_test, params, starts=preds, primal_weight=ws.primal_weight)
+    per_c, per_w = t_cold / max(1, n_test), (t_warm + t_pred) / max(1, n_test)
+    for c, w in zip(cold, warm):
+        rep.cold.append(SolveRecord(c.status, c.iterations, per_c))
+        if w.status == 0:
+            rep.warm.append(SolveRecord(w.status, w.iterations, per_w))
+        else:
+            rep.warm.append(SolveRecord(c.status, w.iterations + c.iterations, per_w + per_c, fallback=True))
+    rep.cold_seconds, rep.warm_seconds, rep.predict_seconds = t_cold, t_warm, t_pred
     ws.net.close()
     return rep

@@ -73,3 +73,49 @@ def test_warm_start_reaches_tolerance_at_default_amplitudes(engine):
     cold, warm, fallbacks = _pipeline(engine, 0.1, 0.05, n_train=256, n_test=12)
     assert fallbacks == 0
     print(f"iterations saved at default amplitudes: {1 - warm.sum() / cold.sum():+.1%}")
+
+
+def test_cfg4_full_size_family_warm_start(engine):
+    """The specified family at full size: m = 50,000, n = 120,000 (600,000
+    nonzeros), theta in R^64, b +-10 %, c +-5 %, with a test-sized training
+    set (64 solves) and 16 held-out thetas. Training and held-out solves run
+    on the family engine (one copy of A, 64 members per wave); POD ranks are
+    min(256, 63) each and the MLP is 64 -> 512 -> 512 -> 512 (bf16 MFMA,
+    GELU). Every warm-started solve must reach the tolerance on its own, and
+    the oracle's or_kkt (double-double on the host) certifies every warm and
+    cold solution; the iterations saved are reported. One member is checked
+    against the single-LP engine bit for bit."""
+    from nirnaya_amd.warmstart import family_solve
+    from oracle_lib import Oracle
+    fam = ParametricFamily(20264, 50_000, 120_000, 5)
+    params = default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-6)
+    th = fam.thetas(64, 1)
+    res, t_train = family_solve(engine, fam, th, params)
+    ok = [i for i, r in enumerate(res) if r.status == 0]
+    assert len(ok) >= 60
+    one = engine.solve(fam.instance(th[0]), params)
+    assert np.array_equal(one.x, res[0].x) and np.array_equal(one.y, res[0].y) and one.iterations == res[0].iterations
+    pod_x = Pod(engine, np.stack([res[i].x for i in ok]), 256)
+    pod_y = Pod(engine, np.stack([res[i].y for i in ok]), 256)
+    omega = float(np.median([res[i].info.primal_weight for i in ok]))
+    ws = WarmStart(engine, pod_x, pod_y, hidden=512, max_rows=64, primal_weight=omega, out_width=512)
+    assert ws.n_out == 512
+    losses = ws.fit(th[ok], epochs=200, batch=64, lr=1e-3, seed=3)
+    assert losses[-1] < 0.5 * losses[0]
+    test = fam.thetas(16, 2)
+    cold, t_cold = family_solve(engine, fam, test, params)
+    preds = [ws.predict(t) for t in test]
+    warm, t_warm = family_solve(engine, fam, test, params, starts=preds, primal_weight=omega)
+    ws.net.close()
+    for i, t in enumerate(test):
+        lp = fam.instance(t)
+        kc = Oracle(lp).kkt(cold[i].x, cold[i].y)
+        kw = Oracle(lp).kkt(warm[i].x, warm[i].y)
+        assert cold[i].status == 0 and warm[i].status == 0, i
+        assert max(kc[:3]) <= 1e-6 and max(kw[:3]) <= 1e-6, (i, kc[:3], kw[:3])
+    ci = np.array([r.iterations for r in cold])
+    wi = np.array([r.iterations for r in warm])
+    print(f"full size: 64 training solves in {t_train:.1f}s (iterations median {np.median([r.iterations for r in res]):.0f}); "
+          f"POD ranks {pod_x.rank}/{pod_y.rank}; loss {losses[0]:.3e} -> {losses[-1]:.3e}; held-out cold median "
+          f"{np.median(ci):.0f} ({t_cold:.1f}s), warm median {np.median(wi):.0f} ({t_warm:.1f}s); saved "
+          f"{1 - wi.sum() / ci.sum():+.1%}")
