@@ -459,103 +459,151 @@ def mixed_workload(args):
 
 
 def warmstart_workload(args):
-    """Learned warm start on a parametric family (fixed A; b, c move with
-    theta in R^64): training solves by the batched engine, POD bases and the
-    64->512->512->512 bf16 MLP fitted on the device, then the held-out thetas
-    solved cold and warm-started. Reports iterations and time to relative KKT
-    1e-6 for (i) the FP64 single-LP engine one LP at a time, cold (the plain
-    baseline; timed on a sample, scaled by exact iteration counts), (ii) the
-    batched mixed-precision engine, cold, (iii) the same, warm-started."""
+    """Learned warm start on the parametric family (fixed A as the 2,000 x
+    5,000 equality LP scaled to m x n, 5 nonzeros per column; b = b0 (1 + 0.1
+    tanh(B theta)), c = c0 + 0.05 C theta, theta in R^64): `ws_train`
+    training solves on the family engine (64 members per wave, one copy of
+    A) to `ws_train_tol`, POD bases of rank 256 (x) + 256 (y) from the
+    snapshot Gram matrices on the device, the 64 -> 512 -> 512 -> 512 bf16 MLP
+    trained on the device, then `ws_test` held-out thetas to relative KKT 1e-6
+    cold and warm-started (family engine). Time to 1e-6 is reported for the
+    warm-started family solve, the cold family solve, and plain FP64 GPU PDHG
+    one LP at a time (the single-LP engine; a sample of held-out LPs timed and
+    scaled by the exact iteration counts of all of them, which the family
+    engine reproduces bit for bit)."""
     import numpy as np
-    from nirnaya_amd.engine import Engine, MODE_PDLP, PREC_MIXED, default_params
-    from nirnaya_amd.warmstart import ParametricFamily, Pod, WarmStart
+    from nirnaya_amd.engine import Engine, MODE_PDLP, default_params
+    from nirnaya_amd.warmstart import ParametricFamily, Pod, WarmStart, family_solve
     m, n = args.ws_m, args.ws_n
+    t_all = time.perf_counter()
     fam = ParametricFamily(20264, m, n, 5)
     eng = Engine(0)
-    mixed = default_params(mode=MODE_PDLP, precision=PREC_MIXED, max_iters=400000, tol=1e-6)
+    params = default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-6)
+    tparams = default_params(mode=MODE_PDLP, max_iters=400000, tol=args.ws_train_tol)
     th_train = fam.thetas(args.ws_train, 1)
     th_test = fam.thetas(args.ws_test, 2)
-
-    def run_batch(thetas, params, x0s=None, y0s=None):
-        t0 = time.perf_counter()
-        b = eng.batch([fam.instance(t) for t in thetas], params, x0s=x0s, y0s=y0s)
-        b.solve()
-        dt = time.perf_counter() - t0
-        res = [b.result(i) for i in range(len(thetas))]
-        b.close()
-        return res, dt
-
-    res, t_train = run_batch(th_train, mixed)
-    ok = [i for i, r in enumerate(res) if r.status == 0]
-    log(f"[bench] {len(ok)}/{len(res)} training solves optimal in {t_train:.1f}s")
+    N = len(th_train)
+    Sx, Sy = np.empty((N, n)), np.empty((N, m))
+    st, om, its = np.zeros(N, np.int64), np.zeros(N), np.zeros(N, np.int64)
     t0 = time.perf_counter()
-    pod_x = Pod(eng, np.stack([res[i].x for i in ok]), 256)
-    pod_y = Pod(eng, np.stack([res[i].y for i in ok]), 256)
+    count = [0]
+
+    def keep(i, x, y, info):
+        Sx[i], Sy[i] = x, y
+        st[i], om[i], its[i] = info.status, info.primal_weight, info.iterations
+        count[0] += 1
+        if count[0] % 1000 == 0:
+            log(f"[bench] training solves: {count[0]}/{N} in {time.perf_counter() - t0:.0f}s")
+    _, t_train = family_solve(eng, fam, th_train, tparams, on_result=keep, lanes=1024)
+    ok = st == 0
+    train_iters = int(its.sum())
+    log(f"[bench] {int(ok.sum())}/{N} training solves reached {args.ws_train_tol:g} in {t_train:.1f}s "
+        f"({train_iters} member-iterations, median {np.median(its):.0f}); "
+        f"{t_train / max(1, train_iters) * 1e6:.3f} us per member-iteration")
+    t0 = time.perf_counter()
+    pod_x = Pod(eng, Sx[ok], 256, log=lambda msg: log("[bench] " + msg))
+    del Sx
+    log(f"[bench] POD x rank {pod_x.rank} in {time.perf_counter() - t0:.1f}s")
+    pod_y = Pod(eng, Sy[ok], 256, log=lambda msg: log("[bench] " + msg))
+    del Sy
     t_pod = time.perf_counter() - t0
-    ws = WarmStart(eng, pod_x, pod_y, hidden=512, max_rows=256,
-                   primal_weight=float(np.median([res[i].info.primal_weight for i in ok])))
+    omega = float(np.median(om[ok]))
+    log(f"[bench] POD y rank {pod_y.rank}; POD {t_pod:.1f}s; median primal weight {omega:.4g}")
+    ws = WarmStart(eng, pod_x, pod_y, hidden=512, max_rows=256, primal_weight=omega, out_width=512)
     t0 = time.perf_counter()
     losses = ws.fit(th_train[ok], epochs=args.ws_epochs, batch=256, lr=1e-3, seed=3)
     t_fit = time.perf_counter() - t0
     rows, ms, flops = ws.net.train_stats()
-    log(f"[bench] POD ranks {pod_x.rank}/{pod_y.rank} ({t_pod:.1f}s); MLP loss {losses[0]:.3e} -> {losses[-1]:.3e} "
-        f"in {t_fit:.1f}s ({flops / (ms * 1e-3) / 1e12:.2f} TFLOP/s in the training GEMMs)")
+    log(f"[bench] MLP loss {losses[0]:.3e} -> {losses[-1]:.3e} in {t_fit:.1f}s "
+        f"({flops / (ms * 1e-3) / 1e12:.2f} TFLOP/s in the training GEMMs)")
+    T = len(th_test)
+    cold_x, cold_y = np.empty((T, n)), np.empty((T, m))
+    ci, cs = np.zeros(T, np.int64), np.zeros(T, np.int64)
+    wi, wsts = np.zeros(T, np.int64), np.zeros(T, np.int64)
+
+    t0 = time.perf_counter()
+    cnt = [0]
+
+    def keep_cold(i, x, y, info):
+        cold_x[i], cold_y[i], ci[i], cs[i] = x, y, info.iterations, info.status
+        cnt[0] += 1
+        if cnt[0] % 100 == 0:
+            log(f"[bench] held-out cold: {cnt[0]}/{T} in {time.perf_counter() - t0:.0f}s")
+    _, t_cold = family_solve(eng, fam, th_test, params, on_result=keep_cold)
+    log(f"[bench] held-out cold: {T} LPs to 1e-6 in {t_cold:.1f}s, iterations median {np.median(ci):.0f}")
     t0 = time.perf_counter()
     preds = [ws.predict(t) for t in th_test]
     t_pred = time.perf_counter() - t0
-    cold, t_cold = run_batch(th_test, mixed)
-    # Prediction quality on the held-out solutions: error of the prediction
-    # and of the POD mean alone, relative to the solution's distance from the
-    # mean.
-    ex = [np.linalg.norm(p[0] - r.x) / max(np.linalg.norm(r.x - pod_x.mean), 1e-30) for p, r in zip(preds, cold)]
-    ey = [np.linalg.norm(p[1] - r.y) / max(np.linalg.norm(r.y - pod_y.mean), 1e-30) for p, r in zip(preds, cold)]
+    ex = [np.linalg.norm(p[0] - cold_x[i]) / max(np.linalg.norm(cold_x[i] - pod_x.mean), 1e-30)
+          for i, p in enumerate(preds)]
+    ey = [np.linalg.norm(p[1] - cold_y[i]) / max(np.linalg.norm(cold_y[i] - pod_y.mean), 1e-30)
+          for i, p in enumerate(preds)]
     log(f"[bench] held-out prediction error / distance from the mean: x {np.median(ex):.3f}, y {np.median(ey):.3f}")
-    from nirnaya_amd.warmstart import warm_params
-    warm, t_warm = run_batch(th_test, warm_params(mixed, ws.primal_weight), [p[0] for p in preds],
-                             [p[1] for p in preds])
-    # Cold start is the fallback of a warm solve that misses the tolerance.
-    fallback = [i for i, r in enumerate(warm) if r.status != 0]
-    if fallback:
-        redo, t_redo = run_batch(th_test[fallback], mixed)
-        t_warm += t_redo
-        for k, i in enumerate(fallback):
-            warm[i] = redo[k]
+
+    t0 = time.perf_counter()
+    cnt[0] = 0
+
+    def keep_warm(i, x, y, info):
+        wi[i], wsts[i] = info.iterations, info.status
+        cnt[0] += 1
+        if cnt[0] % 100 == 0:
+            log(f"[bench] held-out warm: {cnt[0]}/{T} in {time.perf_counter() - t0:.0f}s")
+    _, t_warm = family_solve(eng, fam, th_test, params, starts=preds, primal_weight=omega, on_result=keep_warm)
+    log(f"[bench] held-out warm: {T} LPs to 1e-6 in {t_warm:.1f}s, iterations median {np.median(wi):.0f}")
+    # A warm solve that misses the tolerance falls back to its cold solve.
+    fallback = [i for i in range(T) if wsts[i] != 0]
+    t_fallback = t_cold / T * len(fallback)
+    eff = wi.copy()
+    for i in fallback:
+        eff[i] += ci[i]
     plain = default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-6)
-    samp = list(range(0, len(th_test), max(1, len(th_test) // 16)))
-    ts, si, its64 = 0.0, 0, []
+    samp = list(range(0, T, max(1, T // args.ws_plain_sample)))
+    ts, si = 0.0, 0
     for i in samp:
         t0 = time.perf_counter()
         r = eng.solve(fam.instance(th_test[i]), plain)
         ts += time.perf_counter() - t0
         si += r.iterations
-        its64.append(r.iterations)
-    plain_per_lp = ts / len(samp)
-    ci = np.array([r.iterations for r in cold])
-    wi = np.array([r.iterations for r in warm])
+        assert r.iterations == ci[i], (i, r.iterations, ci[i])
+    plain_total = ts / si * ci.sum()
     ws.net.close()
     eng.close()
+    nnz = fam.base.nnz
+    # Algorithmic bytes of one member-iteration on the family engine (FP64):
+    # primal x, anchor, cost read + x, xbar written, y gathered once; dual y,
+    # anchor, row bounds read + y written, xbar gathered once; the matrix and
+    # the column bounds shared by the 64 members of a wave.
+    b_member = 8 * (5 * n + m) + 8 * (5 * m + n) + (2 * nnz * 12 + 4 * (m + n + 2) + 16 * n) / 64
+    us_member = t_train / max(1, train_iters) * 1e6
     line = {
         "metric": "time-to-1e-6 relative KKT on held-out parametric LPs, learned warm start",
-        "value": round(t_warm + t_pred, 3), "unit": "s", "n_gpus": 1, "higher_is_better": False,
-        "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": f"parametric equality LP family m={m} n={n} (5 nnz per column), theta in R^64, "
-                               f"{args.ws_train} training solves, {args.ws_test} held-out",
+        "value": round(t_warm + t_pred + t_fallback, 3), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"parametric equality LP family m={m} n={n} ({nnz} nonzeros), theta in R^64, "
+                               f"b +-10%, c +-5%, {N} training solves to {args.ws_train_tol:g}, {T} held-out to 1e-6",
                    "pod_rank_x": pod_x.rank, "pod_rank_y": pod_y.rank, "mlp": "64-512-512-512 bf16 GELU"},
+        "training": {"solves": N, "optimal": int(ok.sum()), "seconds": round(t_train, 3),
+                     "member_iterations": train_iters, "iterations_median": float(np.median(its)),
+                     "us_per_member_iteration": round(us_member, 4),
+                     "algorithmic_bytes_per_member_iteration": b_member,
+                     "achieved_GBs": round(b_member / us_member / 1e3, 1)},
         "held_out_iterations": {"cold_median": float(np.median(ci)), "warm_median": float(np.median(wi)),
-                                "cold_mean": float(ci.mean()), "warm_mean": float(wi.mean()),
-                                "saved_fraction_mean": float(1 - wi.sum() / ci.sum()),
+                                "cold_mean": float(ci.mean()), "warm_mean": float(eff.mean()),
+                                "saved_fraction_mean": float(1 - eff.sum() / ci.sum()),
+                                "saved_fraction_median": float(1 - np.median(wi) / np.median(ci)),
                                 "fallbacks": len(fallback)},
-        "warm_primal_weight": ws.primal_weight,
+        "warm_primal_weight": omega,
         "held_out_prediction_error": {"x_median": float(np.median(ex)), "y_median": float(np.median(ey)),
                                       "note": "||prediction - solution|| / ||solution - POD mean||"},
         "mlp_loss": {"first_epoch": float(losses[0]), "last_epoch": float(losses[-1])},
-        "time_s": {"plain_fp64_one_at_a_time": round(plain_per_lp * len(th_test), 3),
-                   "batched_mixed_cold": round(t_cold, 3), "batched_mixed_warm": round(t_warm, 3),
-                   "prediction": round(t_pred, 3), "training_solves": round(t_train, 3),
-                   "pod": round(t_pod, 3), "mlp_training": round(t_fit, 3)},
-        "speedup_vs_plain_fp64": round(plain_per_lp * len(th_test) / (t_warm + t_pred), 2),
-        "plain_sample": f"{len(samp)} held-out LPs solved one at a time, {ts:.2f}s, iterations median "
-                        f"{float(np.median(its64))}",
+        "time_s": {"plain_fp64_one_at_a_time": round(plain_total, 3), "family_cold": round(t_cold, 3),
+                   "family_warm": round(t_warm, 3), "prediction": round(t_pred, 3),
+                   "training_solves": round(t_train, 3), "pod": round(t_pod, 3), "mlp_training": round(t_fit, 3),
+                   "whole_workload": round(time.perf_counter() - t_all, 1)},
+        "speedup_warm_family_vs_plain_fp64": round(plain_total / (t_warm + t_pred + t_fallback), 2),
+        "speedup_cold_family_vs_plain_fp64": round(plain_total / t_cold, 2),
+        "plain_sample": f"{len(samp)} held-out LPs solved one at a time by the single-LP engine in {ts:.2f}s "
+                        f"({si} iterations, each equal to its family count); scaled by all {T} held-out counts",
         "mlp_train_tflops": round(flops / (ms * 1e-3) / 1e12, 3),
     }
     print(json.dumps(line), flush=True)
@@ -613,11 +661,15 @@ def main():
                          "family")
     ap.add_argument("--batch-only", action="store_true",
                     help="planning512: skip the one-at-a-time sample (batch solves only)")
-    ap.add_argument("--ws-m", type=int, default=5000)
-    ap.add_argument("--ws-n", type=int, default=12000)
-    ap.add_argument("--ws-train", type=int, default=8192)
-    ap.add_argument("--ws-test", type=int, default=1024)
-    ap.add_argument("--ws-epochs", type=int, default=1000)
+    ap.add_argument("--ws-m", type=int, default=50000)
+    ap.add_argument("--ws-n", type=int, default=120000)
+    ap.add_argument("--ws-train", type=int, default=20000)
+    ap.add_argument("--ws-train-tol", type=float, default=1e-4,
+                    help="relative KKT of the training solves (the held-out solves run to 1e-6)")
+    ap.add_argument("--ws-test", type=int, default=1000)
+    ap.add_argument("--ws-epochs", type=int, default=300)
+    ap.add_argument("--ws-plain-sample", type=int, default=32,
+                    help="held-out LPs solved one at a time by the single-LP engine for the plain baseline")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is design-only here.",

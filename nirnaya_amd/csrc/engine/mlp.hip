@@ -14,6 +14,7 @@
 // activations and back-propagated errors enter the GEMMs as bf16
 // (round-to-nearest-even); every sum, bias, GELU and loss term is FP32.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <chrono>
@@ -292,40 +293,15 @@ __global__ void k_gather(const float* x, const float* target, const int* idx, in
 
 // ------------------------------------------------------------------ POD (FP64)
 
-// G = X^T X for X [len][N] (column a = snapshot a), 16x16 tiles per block.
-__global__ void k_gram(const double* X, int len, int N, double* G) {
-    __shared__ double ta[64][17], tb[64][17];
-    const int a0 = blockIdx.y * 16, b0 = blockIdx.x * 16;
-    if (b0 < a0) return;   // upper triangle, mirrored below
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    double s = 0.0;
-    for (int r0 = 0; r0 < len; r0 += 64) {
-        for (int q = threadIdx.x; q < 64 * 16; q += 256) {
-            const int r = q >> 4, c = q & 15;
-            const int row = r0 + r;
-            ta[r][c] = (row < len && a0 + c < N) ? X[static_cast<size_t>(row) * N + a0 + c] : 0.0;
-            tb[r][c] = (row < len && b0 + c < N) ? X[static_cast<size_t>(row) * N + b0 + c] : 0.0;
-        }
-        __syncthreads();
-        for (int r = 0; r < 64; ++r) s += ta[r][ty] * tb[r][tx];
-        __syncthreads();
+// Mirror the upper triangle of a column-major N x N matrix into its lower one.
+__global__ void k_mirror_upper(double* G, int N) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    const long long total = static_cast<long long>(N) * N;
+    for (long long e = t; e < total; e += stride) {
+        const int i = static_cast<int>(e % N), j = static_cast<int>(e / N);
+        if (i > j) G[e] = G[static_cast<size_t>(j) + static_cast<size_t>(i) * N];
     }
-    const int a = a0 + ty, b = b0 + tx;
-    if (a < N && b < N) {
-        G[static_cast<size_t>(a) * N + b] = s;
-        G[static_cast<size_t>(b) * N + a] = s;
-    }
-}
-
-// Y[len][R] = X[len][N] V[N][R] (row-major; one thread per output).
-__global__ void k_matmul_rows(const double* X, int len, int N, const double* V, int R, double* Y) {
-    const long long i = blockIdx.x * 256ll + threadIdx.x;
-    if (i >= static_cast<long long>(len) * R) return;
-    const int row = static_cast<int>(i / R), c = static_cast<int>(i % R);
-    const double* x = X + static_cast<size_t>(row) * N;
-    double s = 0.0;
-    for (int k = 0; k < N; ++k) s += x[k] * V[static_cast<size_t>(k) * R + c];
-    Y[i] = s;
 }
 
 }  // namespace mlp
@@ -333,6 +309,13 @@ __global__ void k_matmul_rows(const double* X, int len, int N, const double* V, 
 }  // namespace nir
 
 using namespace nir::gpu;
+
+#define ROCBLAS_CHECK(expr)                                                                      \
+    do {                                                                                         \
+        rocblas_status st_ = (expr);                                                             \
+        if (st_ != rocblas_status_success)                                                       \
+            throw std::runtime_error(std::string(#expr) + ": " + rocblas_status_to_string(st_)); \
+    } while (0)
 using namespace nir::gpu::mlp;
 
 struct nir_mlp {
@@ -712,6 +695,7 @@ NIR_API int nir_mlp_gemm_check(nir_ctx* ctx, int32_t M, int32_t N, int32_t K, co
 // nir_pod_project: Y = X V for V [N][R] (Y [len][R]).
 NIR_API int nir_pod_gram(nir_ctx* ctx, const double* X, int32_t len, int32_t N, double* G) {
     if (!ctx || !X || !G || len < 0 || N <= 0) return -1;
+    rocblas_handle h = nullptr;
     try {
         HIP_CHECK(hipSetDevice(ctx->device));
         hipStream_t s = ctx->stream;
@@ -719,13 +703,24 @@ NIR_API int nir_pod_gram(nir_ctx* ctx, const double* X, int32_t len, int32_t N, 
         dx.alloc(static_cast<size_t>(len) * N);
         dg.alloc(static_cast<size_t>(N) * N);
         dx.upload(X, static_cast<size_t>(len) * N, s);
-        dim3 grid((N + 15) / 16, (N + 15) / 16);
-        k_gram<<<grid, 256, 0, s>>>(dx.p, len, N, dg.p);
+        // Row-major X [len][N] is the column-major N x len matrix A, so
+        // G = X^T X = A A^T: one rocBLAS DSYRK (upper triangle), mirrored.
+        ROCBLAS_CHECK(rocblas_create_handle(&h));
+        ROCBLAS_CHECK(rocblas_set_stream(h, s));
+        const double one = 1.0, zero = 0.0;
+        ROCBLAS_CHECK(rocblas_dsyrk(h, rocblas_fill_upper, rocblas_operation_none, N, len, &one, dx.p, N, &zero,
+                                    dg.p, N));
+        k_mirror_upper<<<2048, 256, 0, s>>>(dg.p, N);
         HIP_CHECK(hipGetLastError());
         dg.download(G, static_cast<size_t>(N) * N, s);
         HIP_CHECK(hipStreamSynchronize(s));
+        (void)rocblas_destroy_handle(h);
         return 0;
     } catch (const std::exception& ex) {
+        if (h) {
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)rocblas_destroy_handle(h);
+        }
         ctx->error = ex.what();
         return -1;
     }
@@ -734,6 +729,7 @@ NIR_API int nir_pod_gram(nir_ctx* ctx, const double* X, int32_t len, int32_t N, 
 NIR_API int nir_pod_project(nir_ctx* ctx, const double* X, int32_t len, int32_t N, const double* V, int32_t R,
                             double* Y) {
     if (!ctx || !X || !V || !Y || len < 0 || N <= 0 || R <= 0) return -1;
+    rocblas_handle h = nullptr;
     try {
         HIP_CHECK(hipSetDevice(ctx->device));
         hipStream_t s = ctx->stream;
@@ -743,13 +739,22 @@ NIR_API int nir_pod_project(nir_ctx* ctx, const double* X, int32_t len, int32_t 
         dy.alloc(static_cast<size_t>(len) * R);
         dx.upload(X, static_cast<size_t>(len) * N, s);
         dv.upload(V, static_cast<size_t>(N) * R, s);
-        const long long total = static_cast<long long>(len) * R;
-        if (total > 0) k_matmul_rows<<<static_cast<unsigned>((total + 255) / 256), 256, 0, s>>>(dx.p, len, N, dv.p, R, dy.p);
-        HIP_CHECK(hipGetLastError());
+        // Column-major: Y^T (R x len) = V^T (R x N) X^T (N x len), one DGEMM.
+        ROCBLAS_CHECK(rocblas_create_handle(&h));
+        ROCBLAS_CHECK(rocblas_set_stream(h, s));
+        const double one = 1.0, zero = 0.0;
+        if (len > 0)
+            ROCBLAS_CHECK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, R, len, N, &one, dv.p, R,
+                                        dx.p, N, &zero, dy.p, R));
         dy.download(Y, static_cast<size_t>(len) * R, s);
         HIP_CHECK(hipStreamSynchronize(s));
+        (void)rocblas_destroy_handle(h);
         return 0;
     } catch (const std::exception& ex) {
+        if (h) {
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)rocblas_destroy_handle(h);
+        }
         ctx->error = ex.what();
         return -1;
     }

@@ -88,11 +88,17 @@ class Pod:
     pairs are used."""
 
     def __init__(self, engine: Engine, S: np.ndarray, rank: int, rel_floor: float = 1e-12,
-                 exact_limit: int = 2048):
+                 exact_limit: int = 2048, log=None):
         S = np.asarray(S, dtype=np.float64)
         self.mean = S.mean(axis=0)
-        X = np.ascontiguousarray((S - self.mean).T)           # [len][N]
+        # Centred snapshots transposed to [len][N], a block of snapshots at a
+        # time (the only extra copy of S).
+        X = np.empty((S.shape[1], S.shape[0]))
+        for i0 in range(0, S.shape[0], 256):
+            X[:, i0:i0 + 256] = (S[i0:i0 + 256] - self.mean).T
         G = engine.pod_gram(X)                                  # X^T X on the device
+        if log:
+            log(f"POD: Gram matrix {G.shape[0]} x {G.shape[0]} of snapshots of length {X.shape[0]}")
         if G.shape[0] > exact_limit and 4 * rank < G.shape[0]:
             lam, V = top_eigh(G, rank)
         else:
