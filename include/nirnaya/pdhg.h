@@ -350,6 +350,28 @@ typedef struct {
 NIR_API int nir_verify_solution_noalloc(nir_ctx* ctx, const nir_dev_lp* lp, const double* x_dev,
                                         double claimed_obj, double tol, nir_verify_report* out);
 
+/* ---- presolve (host) ------------------------------------------------------
+ * The facade's presolve (src/presolve/presolve.hpp:32-345, rules R1-R7, R14
+ * and R2 in the reference's order and tolerances) on one LP, for callers that
+ * run the engine without the facade and for parity checks. `integer` (ncol
+ * flags, may be NULL) marks integer columns (rule R7 tightens their bounds).
+ * Outputs in the caller's buffers, sized by the original model: fixed_x
+ * (ncol, values of removed columns), col_map (reduced ncol entries: original
+ * column of each kept column), row_map (reduced nrow), and the reduced
+ * model's bounds collo/colup (reduced ncol) and rowlo/rowup (reduced nrow);
+ * any buffer may be NULL. Returns 0, or -1 with nir_last_error() set. */
+typedef struct {
+    int32_t status;        /* sov Status: 0 (solve the reduced model, or solved), 1 infeasible, 2 unbounded */
+    int32_t solved;        /* presolve settled the model */
+    int32_t nrow, ncol;    /* reduced model */
+    int32_t rows_removed, cols_removed, tightenings;
+    int32_t reserved;
+    double obj_offset;     /* reduced model's objective offset */
+} nir_presolve_info;
+NIR_API int nir_lp_presolve(const nir_lp* lp, const int8_t* integer, int32_t level, nir_presolve_info* info,
+                            double* fixed_x, int32_t* col_map, int32_t* row_map, double* collo, double* colup,
+                            double* rowlo, double* rowup);
+
 /* ---- synthetic instances (seeded; generated on the host) ---------------- */
 typedef struct nir_owned_lp nir_owned_lp;
 

@@ -6,6 +6,7 @@
 // (plus nonnegative slack on inequality rows) and costs are A^T y0 + s with
 // s >= 0 and y0 sign-feasible, so every instance is primal and dual feasible.
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <cstdint>
 #include <memory>
@@ -399,5 +400,71 @@ NIR_API nir_owned_lp* nir_lp_read_lpb(const char* path) {
     }
 }
 NIR_API void nir_owned_lp_free(nir_owned_lp* g) { delete g; }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ presolve C ABI
+
+extern "C" {
+
+NIR_API int nir_lp_presolve(const nir_lp* lp, const int8_t* integer, int32_t level, nir_presolve_info* info,
+                            double* fixed_x, int32_t* col_map, int32_t* row_map, double* collo, double* colup,
+                            double* rowlo, double* rowup) {
+    try {
+        if (!lp || lp->nrow < 0 || lp->ncol < 0) throw std::invalid_argument("invalid LP");
+        if (lp->ncol > 0 && (!lp->col_start || !lp->cost || !lp->collo || !lp->colup))
+            throw std::invalid_argument("missing column arrays");
+        if (lp->nrow > 0 && (!lp->rowlo || !lp->rowup)) throw std::invalid_argument("missing row arrays");
+        nir::Model M;
+        M.name = "lp";
+        M.ncol = lp->ncol;
+        M.nrow = lp->nrow;
+        const long long nnz = lp->ncol > 0 ? lp->col_start[lp->ncol] : 0;
+        M.cost.assign(lp->cost, lp->cost + lp->ncol);
+        M.collo.assign(lp->collo, lp->collo + lp->ncol);
+        M.colup.assign(lp->colup, lp->colup + lp->ncol);
+        M.vartype.assign(static_cast<size_t>(lp->ncol), nir::VarType::Continuous);
+        if (integer)
+            for (int j = 0; j < lp->ncol; ++j)
+                if (integer[j]) M.vartype[j] = nir::VarType::Integer;
+        M.rowlo.assign(lp->rowlo, lp->rowlo + lp->nrow);
+        M.rowup.assign(lp->rowup, lp->rowup + lp->nrow);
+        M.obj_offset = lp->obj_offset;
+        M.A = nir::CscMatrix(lp->nrow, lp->ncol);
+        if (lp->ncol > 0) {
+            M.A.start.assign(lp->col_start, lp->col_start + lp->ncol + 1);
+            M.A.index.assign(lp->row_index, lp->row_index + nnz);
+            M.A.value.assign(lp->value, lp->value + nnz);
+        }
+        M.Q = nir::CscMatrix(lp->ncol, lp->ncol);
+        for (int j = 0; j < lp->ncol; ++j) M.col_names.push_back("C" + std::to_string(j));
+        for (int i = 0; i < lp->nrow; ++i) M.row_names.push_back("R" + std::to_string(i));
+        const std::string defect = M.A.check();
+        if (!defect.empty()) throw std::invalid_argument("matrix: " + defect);
+        const nir::Presolved P = nir::presolve(M, level, 20);
+        if (info) {
+            std::memset(info, 0, sizeof(*info));
+            info->status = static_cast<int32_t>(P.status);
+            info->solved = P.solved ? 1 : 0;
+            info->nrow = P.reduced.nrow;
+            info->ncol = P.reduced.ncol;
+            info->rows_removed = P.rows_removed;
+            info->cols_removed = P.cols_removed;
+            info->tightenings = P.tightenings;
+            info->obj_offset = P.reduced.obj_offset;
+        }
+        if (fixed_x) std::copy(P.fixed_x.begin(), P.fixed_x.end(), fixed_x);
+        if (col_map) std::copy(P.col_map.begin(), P.col_map.end(), col_map);
+        if (row_map) std::copy(P.row_map.begin(), P.row_map.end(), row_map);
+        if (collo) std::copy(P.reduced.collo.begin(), P.reduced.collo.end(), collo);
+        if (colup) std::copy(P.reduced.colup.begin(), P.reduced.colup.end(), colup);
+        if (rowlo) std::copy(P.reduced.rowlo.begin(), P.reduced.rowlo.end(), rowlo);
+        if (rowup) std::copy(P.reduced.rowup.begin(), P.reduced.rowup.end(), rowup);
+        return 0;
+    } catch (const std::exception& ex) {
+        nir::record_last_error(ex.what());
+        return -1;
+    }
+}
 
 }  // extern "C"

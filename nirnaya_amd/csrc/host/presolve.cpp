@@ -291,3 +291,4 @@ Presolved presolve(const Model& orig, int level, int max_rounds) {
 }
 
 }  // namespace nir
+

@@ -113,6 +113,12 @@ class NirFamilyStats(C.Structure):
                 ("solve_seconds", C.c_double)]
 
 
+class NirPresolveInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("solved", C.c_int32), ("nrow", C.c_int32), ("ncol", C.c_int32),
+                ("rows_removed", C.c_int32), ("cols_removed", C.c_int32), ("tightenings", C.c_int32),
+                ("reserved", C.c_int32), ("obj_offset", C.c_double)]
+
+
 class NirDevLp(C.Structure):
     """Device-resident CSR LP of nir_verify_solution_noalloc (device addresses)."""
     _fields_ = [("nrow", C.c_int32), ("ncol", C.c_int32), ("row_start", C.c_void_p), ("col_index", C.c_void_p),
@@ -185,6 +191,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.nir_family_solve.argtypes = [vp, C.c_int64, _FAMILY_FILL_CB, _FAMILY_DONE_CB, vp]
     lib.nir_family_get_stats.argtypes = [vp, C.POINTER(NirFamilyStats)]
     lib.nir_family_destroy.argtypes = [vp]
+    lib.nir_lp_presolve.argtypes = [C.POINTER(NirLp), C.POINTER(C.c_int8), C.c_int32, C.POINTER(NirPresolveInfo),
+                                    _dp, _i32p, _i32p, _dp, _dp, _dp, _dp]
     lib.nir_verify_solution_noalloc.argtypes = [vp, C.POINTER(NirDevLp), vp, C.c_double, C.c_double,
                                                 C.POINTER(NirVerifyReport)]
     lib.nir_pod_gram.argtypes = [vp, _dp, C.c_int32, C.c_int32, _dp]
@@ -768,3 +776,28 @@ class Mlp:
 
     def __del__(self):
         self.close()
+
+
+def presolve(lp: LP, integer=None, level: int = 2):
+    """The facade's presolve on one LP (nir_lp_presolve): a dict with the
+    info fields, fixed_x, col_map, row_map and the reduced bounds."""
+    lib = load_library()
+    integer = None if integer is None else np.ascontiguousarray(integer, dtype=np.int8)
+    info = NirPresolveInfo()
+    fixed = np.zeros(lp.n)
+    cmap = np.zeros(max(lp.n, 1), dtype=np.int32)
+    rmap = np.zeros(max(lp.m, 1), dtype=np.int32)
+    cl, cu = np.zeros(max(lp.n, 1)), np.zeros(max(lp.n, 1))
+    rl, ru = np.zeros(max(lp.m, 1)), np.zeros(max(lp.m, 1))
+    s = lp.struct()
+    ip = integer.ctypes.data_as(C.POINTER(C.c_int8)) if integer is not None else None
+    if lib.nir_lp_presolve(C.byref(s), ip, int(level), C.byref(info), _ptr(fixed, C.c_double),
+                           cmap.ctypes.data_as(_i32p), rmap.ctypes.data_as(_i32p), _ptr(cl, C.c_double),
+                           _ptr(cu, C.c_double), _ptr(rl, C.c_double), _ptr(ru, C.c_double)) != 0:
+        raise EngineError("nir_lp_presolve: " + lib.nir_last_error().decode())
+    nc, nr = info.ncol, info.nrow
+    return {"status": info.status, "solved": info.solved, "nrow": nr, "ncol": nc,
+            "rows_removed": info.rows_removed, "cols_removed": info.cols_removed,
+            "tightenings": info.tightenings, "obj_offset": info.obj_offset, "fixed_x": fixed,
+            "col_map": cmap[:nc].copy(), "row_map": rmap[:nr].copy(), "collo": cl[:nc].copy(),
+            "colup": cu[:nc].copy(), "rowlo": rl[:nr].copy(), "rowup": ru[:nr].copy()}

@@ -122,4 +122,34 @@ int ref_verify(int32_t m, int32_t n, const int64_t* start, const int32_t* index,
     return rep.passed ? 1 : 0;
 }
 
+// Presolver::presolve (src/presolve/presolve.hpp:32-345) at `level` on the LP
+// with the flagged columns integer. ints = {status, solved, reduced nrow,
+// reduced ncol, rows removed, cols removed, tightenings}; offset = reduced
+// objective offset; the arrays as nir_lp_presolve returns them.
+void ref_presolve(int32_t m, int32_t n, const int64_t* start, const int32_t* index, const double* value,
+                  const double* c, const double* l, const double* u, const double* rl, const double* ru,
+                  const int8_t* integer, int32_t level, int32_t* ints, double* offset, double* fixed_x,
+                  int32_t* col_map, int32_t* row_map, double* collo, double* colup, double* rowlo, double* rowup) {
+    sov::Model M = make_model(m, n, start, index, value, c, l, u, rl, ru);
+    if (integer)
+        for (int32_t j = 0; j < n; ++j)
+            if (integer[j]) M.vartype[j] = sov::VarType::Integer;
+    const sov::PresolveResult r = sov::Presolver::presolve(M, level);
+    ints[0] = static_cast<int32_t>(r.status);
+    ints[1] = r.solved_in_presolve ? 1 : 0;
+    ints[2] = r.reduced.nrow;
+    ints[3] = r.reduced.ncol;
+    ints[4] = r.rows_removed;
+    ints[5] = r.cols_removed;
+    ints[6] = r.bound_tightenings;
+    *offset = r.reduced.obj_offset;
+    std::memcpy(fixed_x, r.fixed_x.data(), sizeof(double) * r.fixed_x.size());
+    for (size_t j = 0; j < r.orig_col_map.size(); ++j) col_map[j] = r.orig_col_map[j];
+    for (size_t i = 0; i < r.orig_row_map.size(); ++i) row_map[i] = r.orig_row_map[i];
+    std::memcpy(collo, r.reduced.collo.data(), sizeof(double) * r.reduced.collo.size());
+    std::memcpy(colup, r.reduced.colup.data(), sizeof(double) * r.reduced.colup.size());
+    std::memcpy(rowlo, r.reduced.rowlo.data(), sizeof(double) * r.reduced.rowlo.size());
+    std::memcpy(rowup, r.reduced.rowup.data(), sizeof(double) * r.reduced.rowup.size());
+}
+
 }  // extern "C"

@@ -85,6 +85,8 @@ def ref():
         L.ref_spmv.argtypes = csc + [_dp, _dp]
         L.ref_spmv_t.argtypes = csc + [_dp, _dp]
         L.ref_verify.argtypes = lpargs + [_dp, C.c_double, C.c_double, _dp]
+        L.ref_presolve.argtypes = lpargs + [C.POINTER(C.c_int8), C.c_int32, _i32p, _dp, _dp, _i32p, _i32p, _dp, _dp,
+                                            _dp, _dp]
         _ref = L
     return _ref
 
@@ -211,3 +213,26 @@ class Reference:
         out = np.zeros(3)
         ok = self.L.ref_verify(*self.csc, *self.vecs, _p(x), float(claimed), float(tol), _p(out))
         return bool(ok), out[0], out[1], out[2]
+
+
+def reference_presolve(lp, integer=None, level=2):
+    """The reference's own Presolver::presolve (oracle/_ref) on one LP, in the
+    form nirnaya_amd.engine.presolve returns."""
+    L = ref()
+    r = Reference(lp)
+    integer = None if integer is None else np.ascontiguousarray(integer, dtype=np.int8)
+    ints = np.zeros(7, dtype=np.int32)
+    off = C.c_double()
+    fixed = np.zeros(lp.n)
+    cmap = np.zeros(max(lp.n, 1), dtype=np.int32)
+    rmap = np.zeros(max(lp.m, 1), dtype=np.int32)
+    cl, cu = np.zeros(max(lp.n, 1)), np.zeros(max(lp.n, 1))
+    rl, ru = np.zeros(max(lp.m, 1)), np.zeros(max(lp.m, 1))
+    ip = integer.ctypes.data_as(C.POINTER(C.c_int8)) if integer is not None else None
+    L.ref_presolve(*r.csc, *r.vecs, ip, int(level), _p(ints, C.c_int32), C.byref(off), _p(fixed),
+                   _p(cmap, C.c_int32), _p(rmap, C.c_int32), _p(cl), _p(cu), _p(rl), _p(ru))
+    nr, nc = int(ints[2]), int(ints[3])
+    return {"status": int(ints[0]), "solved": int(ints[1]), "nrow": nr, "ncol": nc, "rows_removed": int(ints[4]),
+            "cols_removed": int(ints[5]), "tightenings": int(ints[6]), "obj_offset": off.value, "fixed_x": fixed,
+            "col_map": cmap[:nc].copy(), "row_map": rmap[:nr].copy(), "collo": cl[:nc].copy(),
+            "colup": cu[:nc].copy(), "rowlo": rl[:nr].copy(), "rowup": ru[:nr].copy()}

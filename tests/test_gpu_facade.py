@@ -123,3 +123,39 @@ def test_facade_feasibility_polish_makes_the_certificate_pass(tmp_path):
     assert re.search(r"PASS=YES", r.stdout)
     m = re.search(r"max row violation ([0-9.e+-]+) -> ([0-9.e+-]+)", r.stdout)
     assert m and float(m.group(1)) > 1e-6 >= float(m.group(2))
+
+
+def _rule_lps():
+    """One LP per presolve rule with an LP left for the engine afterwards
+    (the models of tests/test_presolve_parity.py, continuous)."""
+    from nirnaya_amd.engine import LP
+    inf = np.inf
+    return {
+        "R4_fixed": LP.from_dense([[1.0, 1.0, 0.0], [2.0, 0.0, 1.0]], [1.0, 2.0, 3.0], [2.0, 0.0, 0.0],
+                                  [2.0, 5.0, 5.0], [3.0, 4.0], [8.0, 9.0]),
+        "R2_empty_cols": LP.from_dense([[1.0, 0.0, 0.0, 0.0, 1.0], [1.0, 0.0, 0.0, 0.0, -1.0]],
+                                       [1.0, 3.0, -2.0, 0.0, 1.0], [0.0, 1.5, -1.0, 0.5, 0.0],
+                                       [4.0, 6.0, 7.0, 3.0, 4.0], [1.0, -2.0], [5.0, 2.0]),
+        "R5_redundant": LP.from_dense([[1.0, 1.0], [1.0, -1.0], [1.0, 2.0]], [1.0, 1.0], [0.0, 0.0], [10.0, 10.0],
+                                      [-inf, -50.0, 3.0], [100.0, inf, 25.0]),
+        "R14_forcing": LP.from_dense([[1.0, 1.0, 0.0, 0.0, 1.0], [0.0, 0.0, -1.0, 1.0, 0.0],
+                                      [1.0, 0.0, 1.0, 1.0, 1.0], [0.0, 1.0, 0.0, 2.0, 1.0]],
+                                     [1.0, -1.0, 2.0, 1.0, 1.0], [0.0, 0.0, 0.0, 0.0, 0.0],
+                                     [5.0, 5.0, 2.0, 8.0, 3.0], [-inf, 8.0, 0.0, 1.0], [0.0, inf, 30.0, 40.0]),
+    }
+
+
+@needs_ref
+@pytest.mark.parametrize("rule", ["R4_fixed", "R2_empty_cols", "R5_redundant", "R14_forcing"])
+def test_presolve_rules_through_both_facades(rule):
+    """Each rule's LP through the whole facade (presolve, engine, postsolve,
+    certificate) here and in the reference's facade: same status, x and
+    objective."""
+    lp = _rule_lps()[rule]
+    ours, theirs = _pair(lp)
+    print(f"{rule}: status {ours.Status}/{theirs.Status} obj {ours.ObjVal}/{theirs.ObjVal}")
+    assert ours.Status == theirs.Status
+    if ours.Status == 0:
+        xo, xt = np.array(ours.X), np.array(theirs.X)
+        assert np.max(np.abs(xo - xt)) <= 1e-10 * max(1.0, np.max(np.abs(xt)))
+        assert ours.ObjVal == pytest.approx(theirs.ObjVal, rel=1e-10, abs=1e-10)
