@@ -372,6 +372,20 @@ NIR_API int nir_lp_presolve(const nir_lp* lp, const int8_t* integer, int32_t lev
                             double* fixed_x, int32_t* col_map, int32_t* row_map, double* collo, double* colup,
                             double* rowlo, double* rowup);
 
+/* ---- crossover hook (host) ------------------------------------------------
+ * A simplex starting basis guessed from a PDHG point x (ncol), in the layout
+ * of the reference's BasisSnapshot (src/lp/lp_solver.hpp:18-21) as its
+ * LpSolver::set_basis takes it (:896-907): status[ncol + nrow] (the columns,
+ * then one logical per row, valued at the row activity and bounded by the
+ * row bounds; codes as src/core/types.hpp:40-47) and basic_index[nrow]
+ * (ascending variable numbers). Variables strictly inside their bounds
+ * (farther than tol * (1 + |bound|) from each finite bound) are basic, the
+ * most interior m of them when there are more; logicals of the rows with the
+ * most slack fill the basis up to nrow. No counterpart in the reference's
+ * PDHG, which has no crossover. */
+enum { NIR_VS_BASIC = 0, NIR_VS_AT_LOWER = 1, NIR_VS_AT_UPPER = 2, NIR_VS_FIXED = 3, NIR_VS_FREE_ZERO = 4 };
+NIR_API int nir_basis_guess(const nir_lp* lp, const double* x, double tol, int8_t* status, int32_t* basic_index);
+
 /* ---- synthetic instances (seeded; generated on the host) ---------------- */
 typedef struct nir_owned_lp nir_owned_lp;
 

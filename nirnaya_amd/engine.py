@@ -193,6 +193,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.nir_family_destroy.argtypes = [vp]
     lib.nir_lp_presolve.argtypes = [C.POINTER(NirLp), C.POINTER(C.c_int8), C.c_int32, C.POINTER(NirPresolveInfo),
                                     _dp, _i32p, _i32p, _dp, _dp, _dp, _dp]
+    lib.nir_basis_guess.argtypes = [C.POINTER(NirLp), _dp, C.c_double, C.POINTER(C.c_int8), _i32p]
     lib.nir_verify_solution_noalloc.argtypes = [vp, C.POINTER(NirDevLp), vp, C.c_double, C.c_double,
                                                 C.POINTER(NirVerifyReport)]
     lib.nir_pod_gram.argtypes = [vp, _dp, C.c_int32, C.c_int32, _dp]
@@ -801,3 +802,17 @@ def presolve(lp: LP, integer=None, level: int = 2):
             "tightenings": info.tightenings, "obj_offset": info.obj_offset, "fixed_x": fixed,
             "col_map": cmap[:nc].copy(), "row_map": rmap[:nr].copy(), "collo": cl[:nc].copy(),
             "colup": cu[:nc].copy(), "rowlo": rl[:nr].copy(), "rowup": ru[:nr].copy()}
+
+
+def basis_guess(lp: LP, x, tol: float = 1e-6):
+    """Crossover hook (nir_basis_guess): (status[n + m], basic_index[m]) of a
+    simplex starting basis guessed from the point x."""
+    lib = load_library()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    status = np.zeros(lp.n + lp.m, dtype=np.int8)
+    basic = np.zeros(max(lp.m, 1), dtype=np.int32)
+    s = lp.struct()
+    if lib.nir_basis_guess(C.byref(s), _ptr(x, C.c_double), float(tol), status.ctypes.data_as(C.POINTER(C.c_int8)),
+                           basic.ctypes.data_as(_i32p)) != 0:
+        raise EngineError("nir_basis_guess: " + lib.nir_last_error().decode())
+    return status, basic[:lp.m].copy()

@@ -152,4 +152,31 @@ void ref_presolve(int32_t m, int32_t n, const int64_t* start, const int32_t* ind
     std::memcpy(rowup, r.reduced.rowup.data(), sizeof(double) * r.reduced.rowup.size());
 }
 
+// The reference's simplex (LpSolver, src/lp/lp_solver.hpp) on the LP, from
+// its slack basis (status == NULL) or from the given BasisSnapshot
+// (status: n + m codes, basic: m variable numbers) through set_basis.
+// Returns the status; iterations and objective through the pointers, x (n).
+int ref_simplex(int32_t m, int32_t n, const int64_t* start, const int32_t* index, const double* value,
+                const double* c, const double* l, const double* u, const double* rl, const double* ru,
+                const int8_t* status, const int32_t* basic, int64_t* iters, double* objective, double* x) {
+    sov::Model M = make_model(m, n, start, index, value, c, l, u, rl, ru);
+    sov::Options opt;
+    opt.log_level = 0;
+    sov::LpSolver s(opt);
+    s.load(M);
+    if (status) {
+        sov::BasisSnapshot snap;
+        snap.status.resize(static_cast<size_t>(n) + m);
+        for (int32_t v = 0; v < n + m; ++v) snap.status[v] = static_cast<sov::VarStatus>(status[v]);
+        snap.basic_index.assign(basic, basic + m);
+        s.set_basis(snap);
+    }
+    sov::Deadline dl;
+    const sov::Status st = s.solve(dl);
+    *iters = s.stats.iterations;
+    *objective = s.objective;
+    for (int32_t j = 0; j < n; ++j) x[j] = s.value[j];
+    return static_cast<int>(st);
+}
+
 }  // extern "C"

@@ -85,6 +85,7 @@ def ref():
         L.ref_spmv.argtypes = csc + [_dp, _dp]
         L.ref_spmv_t.argtypes = csc + [_dp, _dp]
         L.ref_verify.argtypes = lpargs + [_dp, C.c_double, C.c_double, _dp]
+        L.ref_simplex.argtypes = lpargs + [C.POINTER(C.c_int8), _i32p, _i64p, _dp, _dp]
         L.ref_presolve.argtypes = lpargs + [C.POINTER(C.c_int8), C.c_int32, _i32p, _dp, _dp, _i32p, _i32p, _dp, _dp,
                                             _dp, _dp]
         _ref = L
@@ -236,3 +237,19 @@ def reference_presolve(lp, integer=None, level=2):
             "cols_removed": int(ints[5]), "tightenings": int(ints[6]), "obj_offset": off.value, "fixed_x": fixed,
             "col_map": cmap[:nc].copy(), "row_map": rmap[:nr].copy(), "collo": cl[:nc].copy(),
             "colup": cu[:nc].copy(), "rowlo": rl[:nr].copy(), "rowup": ru[:nr].copy()}
+
+
+def reference_simplex(lp, status=None, basic=None):
+    """The reference's simplex (oracle/_ref LpSolver) on one LP, cold or from a
+    basis: (status, iterations, objective, x)."""
+    L = ref()
+    r = Reference(lp)
+    it = C.c_int64()
+    obj = C.c_double()
+    x = np.zeros(lp.n)
+    sp = None if status is None else np.ascontiguousarray(status, np.int8).ctypes.data_as(C.POINTER(C.c_int8))
+    bp = None if basic is None else np.ascontiguousarray(basic, np.int32).ctypes.data_as(C.POINTER(C.c_int32))
+    keep = (status, basic)
+    st = L.ref_simplex(*r.csc, *r.vecs, sp, bp, C.byref(it), C.byref(obj), _p(x))
+    del keep
+    return st, it.value, obj.value, x
