@@ -157,6 +157,44 @@ def time_to_tolerance(eng_factory, lp, plain_cap_s):
     return out
 
 
+def batched_leg():
+    """All 512 planning LPs (seeds 20263*1000 + i) to relative KKT 1e-6, all
+    measured: the batched persistent engine in FP64 and in mixed precision
+    (setup + solve), and plain FP64 GPU PDHG one LP at a time (the single-LP
+    engine, every one of the 512 solved and timed)."""
+    import numpy as np
+    from nirnaya_amd.engine import Engine, MODE_PDLP, PREC_MIXED, default_params, gen_planning_lp
+    lps = [gen_planning_lp(20263 * 1000 + i) for i in range(512)]
+    eng = Engine(0)
+    out = {}
+    for name, prec in (("batch_fp64", 0), ("batch_mixed", PREC_MIXED)):
+        p = default_params(mode=MODE_PDLP, precision=prec, max_iters=400000, tol=1e-6)
+        t0 = time.perf_counter()
+        b = eng.batch(lps, p)
+        b.solve()
+        dt = time.perf_counter() - t0
+        res = [b.result(i) for i in range(len(lps))]
+        b.close()
+        out[name] = {"seconds": round(dt, 3), "optimal": int(sum(r.status == 0 for r in res)),
+                     "iterations": int(sum(r.iterations for r in res))}
+        log(f"[bench] 512 planning LPs, {name}: {out[name]}")
+    p = default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-6)
+    t0 = time.perf_counter()
+    opt = its = 0
+    for lp in lps:
+        r = eng.solve(lp, p)
+        opt += r.status == 0
+        its += r.iterations
+    one = time.perf_counter() - t0
+    eng.close()
+    out["one_at_a_time_fp64"] = {"seconds": round(one, 3), "optimal": int(opt), "iterations": int(its)}
+    out["speedup_batch_fp64"] = round(one / out["batch_fp64"]["seconds"], 2)
+    out["speedup_batch_mixed"] = round(one / out["batch_mixed"]["seconds"], 2)
+    log(f"[bench] 512 planning LPs one at a time (FP64 single-LP engine): {one:.1f}s; batch speedups "
+        f"{out['speedup_batch_fp64']}x (FP64), {out['speedup_batch_mixed']}x (mixed)")
+    return out
+
+
 def planning512(args):
     """Batched workload: 512 planning LPs of uneven size (T ~ U{4..48}
     periods x K ~ U{5..40} products, seeds 20263*1000 + i), restarted PDHG to
@@ -644,6 +682,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ttt", action="store_true", help="skip the time-to-1e-6 solves (pdlp mode, FP64 and mixed)")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the 512 planning LPs (batched engine vs one LP at a time, all measured)")
     ap.add_argument("--plain-cap", type=float, default=100.0,
                     help="seconds of iteration time for the unrestarted reference form toward 1e-6 (0 = skip)")
     ap.add_argument("--tile", type=int, default=-1, help="LDS tile (-1: autotune tile, rows and segments)")
@@ -756,6 +796,10 @@ def main():
     if not args.no_ttt:
         ttt = time_to_tolerance(eng_factory=lambda: Engine(0), lp=lp, plain_cap_s=args.plain_cap)
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_leg()
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] timing the reference CPU loop on a bounded sample ...")
@@ -794,6 +838,7 @@ def main():
                     "iteration_bytes": t.iter_bytes,
                     "iteration_gbs": t.iter_bytes * args.steps / (t.total_ms / 1e3) / 1e9},
         "time_to_1e-6": ttt,
+        "batched_512_planning_lps": batched,
         "calibration": calibration,
         "cpu_baseline": cpu,
     }

@@ -22,7 +22,7 @@ def test_more_than_one_gpu_is_refused():
 def test_default_run_prints_the_contract_line():
     # A short run of the default workload (the 400,000 x 1,200,000 LP) with
     # the CPU baseline's bounded sample, without the time-to-tolerance solves.
-    r = subprocess.run([sys.executable, "-u", BENCH, "--steps", "50", "--warmup", "10", "--no-ttt"],
+    r = subprocess.run([sys.executable, "-u", BENCH, "--steps", "50", "--warmup", "10", "--no-ttt", "--no-batched"],
                        capture_output=True, text=True, timeout=600, cwd=ROOT)
     assert r.returncode == 0, r.stderr[-2000:]
     lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
