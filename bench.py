@@ -647,6 +647,38 @@ def warmstart_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def family_workload(args):
+    """The parametric-family engine alone on the 50,000 x 120,000 family:
+    `fam_lanes` members (one per lane) iterated for `steps` iterations each
+    (an iteration cap, tolerance 1e-12 so that none stops earlier): time per
+    member-iteration and algorithmic bandwidth, the workload the family
+    kernels' rocprofv3 summaries are taken on."""
+    import numpy as np
+    from nirnaya_amd.engine import Engine, MODE_PDLP, default_params
+    from nirnaya_amd.warmstart import ParametricFamily, family_solve
+    m, n = args.ws_m, args.ws_n
+    fam = ParametricFamily(20264, m, n, 5)
+    eng = Engine(0)
+    th = fam.thetas(args.fam_lanes, 1)
+    p = default_params(mode=MODE_PDLP, max_iters=args.steps, tol=1e-12)
+    its = []
+    _, dt = family_solve(eng, fam, th, p, lanes=args.fam_lanes, on_result=lambda i, x, y, info: its.append(info.iterations))
+    eng.close()
+    total = int(sum(its))
+    nnz = fam.base.nnz
+    b_member = 8 * (5 * n + m) + 8 * (5 * m + n) + (2 * nnz * 12 + 4 * (m + n + 2) + 16 * n) / 64
+    us = dt / max(1, total) * 1e6
+    line = {"metric": "parametric-family PDHG member-iterations/s (family engine, FP64)",
+            "value": round(total / dt, 1), "unit": "member-iterations/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"parametric equality LP family m={m} n={n} ({nnz} nonzeros), "
+                                   f"{args.fam_lanes} members x {args.steps} iterations"},
+            "us_per_member_iteration": round(us, 4), "algorithmic_bytes_per_member_iteration": b_member,
+            "achieved_GBs": round(b_member / us / 1e3, 1),
+            "note": "includes member set-up on the host and the start-point evaluations"}
+    print(json.dumps(line), flush=True)
+
+
 def gemm_workload(args):
     """The warm start's bf16 MFMA GEMM through nir_mlp_gemm_check (FP32
     inputs rounded to bf16 on the device, C = A B^T in FP32) at a large shape
@@ -694,7 +726,7 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the 256 MiB copy kernel (reference for rocprofv3 byte counters)")
     ap.add_argument("--workload", default="large-lp",
-                    choices=["large-lp", "small-lp", "mixed", "planning512", "warmstart", "gemm"],
+                    choices=["large-lp", "small-lp", "mixed", "planning512", "warmstart", "family", "gemm"],
                     help="large-lp: the headline line (default); small-lp: the reference's CPU-runnable "
                          "2,000 x 5,000 case; mixed: mixed precision on the large instance; "
                          "planning512: the batched workload; warmstart: the learned warm start on a parametric "
@@ -708,6 +740,7 @@ def main():
                     help="relative KKT of the training solves (the held-out solves run to 1e-6)")
     ap.add_argument("--ws-test", type=int, default=1000)
     ap.add_argument("--ws-epochs", type=int, default=300)
+    ap.add_argument("--fam-lanes", type=int, default=1024, help="family: members iterated together")
     ap.add_argument("--ws-plain-sample", type=int, default=32,
                     help="held-out LPs solved one at a time by the single-LP engine for the plain baseline")
     args = ap.parse_args()
@@ -723,6 +756,9 @@ def main():
         return
     if args.workload == "mixed":
         mixed_workload(args)
+        return
+    if args.workload == "family":
+        family_workload(args)
         return
     if args.workload == "gemm":
         gemm_workload(args)
