@@ -486,6 +486,7 @@ def mixed_workload(args):
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_dual_pdlp<float> (FP32 A xbar + dual step); one launch = its segment passes",
                      "bytes_per_launch": t.spmv_bytes, "avg_launch_us": spmv_avg_s * 1e6,
+                     "launches_averaged": k_iters,
                      "dispatches_per_launch": t.spmv_passes},
         "kernels": {"spmtv_primal_avg_us": spmtv_avg_s * 1e6, "spmtv_bytes_per_launch": t.spmtv_bytes,
                     "iteration_bytes": t.iter_bytes},
@@ -532,7 +533,9 @@ def warmstart_workload(args):
         count[0] += 1
         if count[0] % 1000 == 0:
             log(f"[bench] training solves: {count[0]}/{N} in {time.perf_counter() - t0:.0f}s")
-    _, t_train = family_solve(eng, fam, th_train, tparams, on_result=keep, lanes=1024)
+    _, t_train, fst = family_solve(eng, fam, th_train, tparams, on_result=keep, lanes=1024, with_stats=True)
+    log(f"[bench] training family engine: {fst.blocks} blocks, {fst.refills} refills, {fst.compactions} compactions, "
+        f"host set-up {fst.prepare_seconds:.1f}s, host waiting for blocks {fst.wait_seconds:.1f}s")
     ok = st == 0
     train_iters = int(its.sum())
     log(f"[bench] {int(ok.sum())}/{N} training solves reached {args.ws_train_tol:g} in {t_train:.1f}s "
@@ -621,6 +624,10 @@ def warmstart_workload(args):
                                f"b +-10%, c +-5%, {N} training solves to {args.ws_train_tol:g}, {T} held-out to 1e-6",
                    "pod_rank_x": pod_x.rank, "pod_rank_y": pod_y.rank, "mlp": "64-512-512-512 bf16 GELU"},
         "training": {"solves": N, "optimal": int(ok.sum()), "seconds": round(t_train, 3),
+                     "family_engine": {"blocks": int(fst.blocks), "refills": int(fst.refills),
+                                       "compactions": int(fst.compactions),
+                                       "host_setup_s": round(fst.prepare_seconds, 2),
+                                       "host_wait_s": round(fst.wait_seconds, 2)},
                      "member_iterations": train_iters, "iterations_median": float(np.median(its)),
                      "us_per_member_iteration": round(us_member, 4),
                      "algorithmic_bytes_per_member_iteration": b_member,
@@ -789,7 +796,7 @@ def main():
             calibration["gather_rate_G_per_s"][str(length * 8)] = {"gather": round(g, 1), "value_x_gather": round(gv, 1)}
             log(f"[bench] random FP64 gathers, operand {length * 8 / 1e6:.2f} MB: {g:.0f} G/s alone, "
                 f"{gv:.0f} G/s with the value stream")
-    params = default_params(mode=MODE_REFERENCE, max_iters=args.warmup + 2 * args.steps + 64, check_termination=0,
+    params = default_params(mode=MODE_REFERENCE, max_iters=args.warmup + 2 * args.steps + 2064, check_termination=0,
                             tile_nnz=args.tile, tile_rows=args.tile_rows, seg_width=args.seg,
                             seg_width_t=args.seg_t, seg_fused=args.fused)
     sess = eng.session(lp, params)
@@ -803,9 +810,14 @@ def main():
     # for the throughput, then the same kernels with events around every
     # dispatch for the per-kernel durations.
     t = sess.time(args.steps)
-    log(f"[bench] timed {args.steps} iterations: {t.total_ms:.1f} ms; per-dispatch sweep: A xbar "
-        f"{t.spmv_ms / args.steps * 1e3:.1f} us in {t.spmv_passes} passes, A^T y {t.spmtv_ms / args.steps * 1e3:.1f} us "
-        f"in {t.spmtv_passes} passes")
+    # Per-kernel durations (the roofline) from at least 1,000 iterations of
+    # the per-dispatch sweep, so that a short run still averages over many
+    # launches; `value` is the production sweep of exactly `steps`.
+    kt = t if args.steps >= 1000 else sess.time(1000)
+    k_iters = max(args.steps, 1000)
+    log(f"[bench] timed {args.steps} iterations: {t.total_ms:.1f} ms; per-dispatch sweep over {k_iters}: A xbar "
+        f"{kt.spmv_ms / k_iters * 1e3:.1f} us in {kt.spmv_passes} passes, A^T y {kt.spmtv_ms / k_iters * 1e3:.1f} us "
+        f"in {kt.spmtv_passes} passes")
     # Access floors: the same layouts' value/index streams and operand gathers
     # with nothing else (no row sums, no epilogue).
     floor_a, floor_t = sess.gather_floor(0), sess.gather_floor(1)
@@ -814,8 +826,8 @@ def main():
     eng.close()
 
     it_s = args.steps / (t.total_ms / 1e3)
-    spmv_avg_s = t.spmv_ms / args.steps / 1e3
-    spmtv_avg_s = t.spmtv_ms / args.steps / 1e3
+    spmv_avg_s = kt.spmv_ms / k_iters / 1e3
+    spmtv_avg_s = kt.spmtv_ms / k_iters / 1e3
     achieved = t.spmv_bytes / spmv_avg_s / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "spmv_cfg1_traffic.json")
@@ -862,6 +874,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_dual_ref (A xbar + fused dual step); one launch = its column-segment passes",
                      "bytes_per_launch": t.spmv_bytes, "avg_launch_us": spmv_avg_s * 1e6,
+                     "launches_averaged": k_iters,
                      "dispatches_per_launch": t.spmv_passes,
                      "avg_dispatch_us": spmv_avg_s * 1e6 / t.spmv_passes,
                      "shape": {"tile": shape[0], "rows": shape[1], "segments": shape[2], "fused": shape[3]},

@@ -245,6 +245,10 @@ typedef struct {
     int64_t refills;          /* start-point evaluations of refilled lanes */
     int64_t total_iterations; /* summed over members */
     double setup_seconds, solve_seconds;
+    int64_t compactions;      /* moves of running members into the lowest groups */
+    int64_t group_blocks;     /* evaluation blocks x groups launched */
+    double prepare_seconds;   /* host: member set-up (fill callback + scaling) */
+    double wait_seconds;      /* host waiting for blocks to finish */
 } nir_family_stats;
 /* Supplies member `index`: cost (ncol), rowlo and rowup (nrow), and an
  * optional warm start x0 (ncol) / y0 (nrow) flagged in *warm (bit 0: x0,

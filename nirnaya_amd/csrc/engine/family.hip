@@ -35,6 +35,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -412,6 +413,48 @@ __global__ __launch_bounds__(256) void k_fam_scatter(View V, int count, const in
     }
 }
 
+// Compaction: lane src[f] moves to the free lane dst[f] (every vector an
+// iteration or evaluation block reads, and the state); src is left empty.
+__global__ __launch_bounds__(256) void k_fam_move(View V, int count, const int* __restrict__ src,
+                                                  const int* __restrict__ dst) {
+    const int f = blockIdx.y;
+    if (f >= count) return;
+    const int ls = src[f], ld = dst[f];
+    const int gs = ls / kL, qs = ls % kL, gd = ld / kL, qd = ld % kL;
+    const size_t sx = static_cast<size_t>(gs) * V.n * kL + qs, dx = static_cast<size_t>(gd) * V.n * kL + qd;
+    const size_t sy = static_cast<size_t>(gs) * V.m * kL + qs, dy = static_cast<size_t>(gd) * V.m * kL + qd;
+    const int stride = gridDim.x * blockDim.x;
+    const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int j = t0; j < V.n; j += stride) {
+        const size_t a = sx + static_cast<size_t>(j) * kL, b = dx + static_cast<size_t>(j) * kL;
+        V.x[b] = V.x[a];
+        V.xbar[b] = V.xbar[a];
+        V.xa[b] = V.xa[a];
+        V.xp[b] = V.xp[a];
+        V.c[b] = V.c[a];
+    }
+    for (int i = t0; i < V.m; i += stride) {
+        const size_t a = sy + static_cast<size_t>(i) * kL, b = dy + static_cast<size_t>(i) * kL;
+        V.y[b] = V.y[a];
+        V.ya[b] = V.ya[a];
+        V.yp[b] = V.yp[a];
+        V.rl[b] = V.rl[a];
+        V.ru[b] = V.ru[a];
+    }
+    if (t0 == 0) {
+        V.st[ld] = V.st[ls];
+        V.ltau[ld] = V.ltau[ls];
+        V.lsigma[ld] = V.lsigma[ls];
+        V.lkb[ld] = V.lkb[ls];
+        V.ldone[ld] = V.ldone[ls];
+        V.lrestart[ld] = V.lrestart[ls];
+        V.lfresh[ld] = V.lfresh[ls];
+        V.ldone[ls] = 1;
+        V.lrestart[ls] = 0;
+        V.lfresh[ls] = 0;
+    }
+}
+
 // Harvest: T(z) and the scaled cost of lane lanes[f] into slot f.
 __global__ __launch_bounds__(256) void k_fam_gather(View V, int count, const int* __restrict__ lanes,
                                                     double* __restrict__ ox, double* __restrict__ oc,
@@ -456,6 +499,7 @@ struct nir_family {
     int m = 0, n = 0;
     long long nnz = 0;
     int G = 0, lanes = 0;
+    int Gl = 0;   // groups launched: those up to the highest occupied lane
     // shared data
     std::vector<int64_t> col_start;
     std::vector<int32_t> row_index;
@@ -472,7 +516,7 @@ struct nir_family {
     DevArray<int> lkb, ldone, lrestart, lfresh;
     DevArray<dd> part_c, part_r, cert_c, cert_r;
     // staging
-    DevArray<int> s_lanes;
+    DevArray<int> s_lanes, s_dst;
     DevArray<double> s_c, s_x, s_rl, s_ru, s_y;
     DevArray<PdlpState> s_st;
     std::vector<int> h_done;
@@ -480,6 +524,7 @@ struct nir_family {
     // graph of one evaluation block
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    int graph_groups = 0;   // Gl the graph was captured with
     nir_family_stats stats{};
 
     ~nir_family() {
@@ -528,9 +573,9 @@ int iter_grid(int rows) { return std::max(1, (rows + 4 * kRowsPerWave - 1) / (4 
 
 template <int FRESH>
 void launch_eval(nir_family* F, const View& V) {
-    k_fam_eval_cols<FRESH><<<dim3(F->ebc, F->G), 256, 0, F->s>>>(V);
-    k_fam_eval_rows<FRESH><<<dim3(F->ebr, F->G), 256, 0, F->s>>>(V);
-    k_fam_decide<FRESH><<<F->G, kL, 0, F->s>>>(V);
+    k_fam_eval_cols<FRESH><<<dim3(F->ebc, F->Gl), 256, 0, F->s>>>(V);
+    k_fam_eval_rows<FRESH><<<dim3(F->ebr, F->Gl), 256, 0, F->s>>>(V);
+    k_fam_decide<FRESH><<<F->Gl, kL, 0, F->s>>>(V);
 }
 
 // One evaluation block: eval_freq iterations, the evaluation and decision,
@@ -539,7 +584,7 @@ void launch_eval(nir_family* F, const View& V) {
 void enqueue_block(nir_family* F) {
     const View V = F->view();
     const int ef = std::max(1, F->prm.eval_freq);
-    const dim3 gp(iter_grid(F->n), F->G), gd(iter_grid(F->m), F->G);
+    const dim3 gp(iter_grid(F->n), F->Gl), gd(iter_grid(F->m), F->Gl);
     for (int kl = 0; kl < ef; ++kl) {
         const int save = kl == ef - 1 ? 1 : 0;
         k_fam_primal<<<gp, 256, 0, F->s>>>(V, kl, save);
@@ -547,22 +592,30 @@ void enqueue_block(nir_family* F) {
     }
     launch_eval<0>(F, V);
     const int gv = std::max(1, std::min(1024, (std::max(F->n, F->m) * kL + 255) / 256));
-    k_fam_restart<0><<<dim3(gv, F->G), 256, 0, F->s>>>(V);
-    k_fam_restart<1><<<dim3(gv, F->G), 256, 0, F->s>>>(V);
-    k_fam_restart<2><<<dim3(gv, F->G), 256, 0, F->s>>>(V);
-    k_fam_cert_cols<<<dim3(F->ebc, F->G), 256, 0, F->s>>>(V);
-    k_fam_cert_rows<<<dim3(F->ebr, F->G), 256, 0, F->s>>>(V);
-    k_fam_cert_decide<<<F->G, kL, 0, F->s>>>(V);
+    k_fam_restart<0><<<dim3(gv, F->Gl), 256, 0, F->s>>>(V);
+    k_fam_restart<1><<<dim3(gv, F->Gl), 256, 0, F->s>>>(V);
+    k_fam_restart<2><<<dim3(gv, F->Gl), 256, 0, F->s>>>(V);
+    k_fam_cert_cols<<<dim3(F->ebc, F->Gl), 256, 0, F->s>>>(V);
+    k_fam_cert_rows<<<dim3(F->ebr, F->Gl), 256, 0, F->s>>>(V);
+    k_fam_cert_decide<<<F->Gl, kL, 0, F->s>>>(V);
     HIP_CHECK(hipGetLastError());
 }
 
 void run_block(nir_family* F) {
     if (F->prm.use_graph) {
+        if (F->gexec && F->graph_groups != F->Gl) {
+            // Compaction changed the launched groups: capture again.
+            HIP_CHECK(hipGraphExecDestroy(F->gexec));
+            HIP_CHECK(hipGraphDestroy(F->graph));
+            F->gexec = nullptr;
+            F->graph = nullptr;
+        }
         if (!F->gexec) {
             HIP_CHECK(hipStreamBeginCapture(F->s, hipStreamCaptureModeThreadLocal));
             enqueue_block(F);
             HIP_CHECK(hipStreamEndCapture(F->s, &F->graph));
             HIP_CHECK(hipGraphInstantiate(&F->gexec, F->graph, nullptr, nullptr, 0));
+            F->graph_groups = F->Gl;
         }
         HIP_CHECK(hipGraphLaunch(F->gexec, F->s));
     } else {
@@ -680,6 +733,7 @@ NIR_API int nir_family_create(nir_ctx* ctx, const nir_lp* base, const nir_params
         F->cert_r.alloc(static_cast<size_t>(F->G) * F->ebr * 6 * kL);
         const size_t S = static_cast<size_t>(std::min(F->lanes, kStageMax));
         F->s_lanes.alloc(S);
+        F->s_dst.alloc(S);
         F->s_c.alloc(S * n);
         F->s_x.alloc(S * n);
         F->s_rl.alloc(S * m);
@@ -708,14 +762,30 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
         const int m = F->m, n = F->n;
         const size_t S = static_cast<size_t>(std::min(F->lanes, kStageMax));
         std::vector<long long> lane_inst(static_cast<size_t>(F->lanes), -1);
+        // Free lanes, kept in descending order so that back() is the lowest:
+        // refills fill the lowest groups first, and only the groups up to the
+        // highest occupied lane are launched.
         std::vector<int> free_lanes;
         for (int l = F->lanes - 1; l >= 0; --l) free_lanes.push_back(l);
+        auto update_groups = [&]() {
+            int hi = -1;
+            for (int l = F->lanes - 1; l >= 0; --l)
+                if (lane_inst[l] >= 0) {
+                    hi = l;
+                    break;
+                }
+            F->Gl = hi < 0 ? 1 : hi / kL + 1;
+        };
+        update_groups();
+        using clock = std::chrono::steady_clock;
+        auto secs = [](clock::time_point a) { return std::chrono::duration<double>(clock::now() - a).count(); };
         long long next = 0, finished = 0;
         int active = 0;
         std::vector<double> cost(n), rlo(m), rup(m), x0(n), y0(m);
         std::vector<Prepared> ready;
         // Prepare one instance on the host with the single-LP set-up code.
         auto prepare = [&](long long idx) {
+            const auto tp = clock::now();
             int32_t warm = 0;
             double weight = 0.0;
             std::fill(x0.begin(), x0.end(), 0.0);
@@ -742,6 +812,7 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
             P.index = idx;
             pdlp_prepare_instance(&lp, p, F->R, F->C, F->eta, false, P.inst);
             ready.push_back(std::move(P));
+            F->stats.prepare_seconds += secs(tp);
         };
         std::vector<double> hc(S * n), hx(S * n), hrl(S * m), hru(S * m), hy(S * m);
         std::vector<PdlpState> hst(S);
@@ -784,22 +855,40 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
                 any = true;
             }
             if (any) {
+                update_groups();
                 launch_eval<1>(F, F->view());
                 HIP_CHECK(hipGetLastError());
                 ++F->stats.refills;
             }
         };
-        std::vector<double> gx_(S * n), gc_(S * n), gy_(S * m);
-        std::vector<PdlpState> gst(S);
         // Harvest every finished lane: T(z) unscaled, the objective in
         // double-double from the caller's costs (scaled cost / C, exact), the
         // status and counters as nir_session_result reports them.
-        auto harvest = [&]() {
+        // Finished members fetched from the device and waiting for the host
+        // to unscale them and hand them out (done while the next block runs).
+        struct Fetched {
+            std::vector<long long> index;
+            std::vector<double> x, c, y;
+            std::vector<PdlpState> st;
+        };
+        Fetched pending;
+        long long fetched = 0;
+        // Device half of a harvest: which lanes finished, and their T(z),
+        // scaled costs and states copied to the host; the lanes are free
+        // for refills at once.
+        auto fetch = [&]() {
+            const auto tw = clock::now();
             F->ldone.download(F->h_done.data(), F->h_done.size(), F->s);
             HIP_CHECK(hipStreamSynchronize(F->s));
+            F->stats.wait_seconds += secs(tw);
             std::vector<int> fin;
             for (int l = 0; l < F->lanes; ++l)
                 if (F->h_done[l] && lane_inst[l] >= 0) fin.push_back(l);
+            Fetched out;
+            out.x.resize(fin.size() * n);
+            out.c.resize(fin.size() * n);
+            out.y.resize(fin.size() * m);
+            out.st.resize(fin.size());
             for (size_t b = 0; b < fin.size(); b += S) {
                 const size_t k = std::min(S, fin.size() - b);
                 std::copy(fin.begin() + b, fin.begin() + b + k, hl.begin());
@@ -808,56 +897,105 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
                 k_fam_gather<<<dim3(gx, static_cast<unsigned>(k)), 256, 0, F->s>>>(
                     F->view(), static_cast<int>(k), F->s_lanes.p, F->s_x.p, F->s_c.p, F->s_y.p, F->s_st.p);
                 HIP_CHECK(hipGetLastError());
-                F->s_x.download(gx_.data(), k * n, F->s);
-                F->s_c.download(gc_.data(), k * n, F->s);
-                F->s_y.download(gy_.data(), k * m, F->s);
-                F->s_st.download(gst.data(), k, F->s);
+                F->s_x.download(out.x.data() + b * n, k * n, F->s);
+                F->s_c.download(out.c.data() + b * n, k * n, F->s);
+                F->s_y.download(out.y.data() + b * m, k * m, F->s);
+                F->s_st.download(out.st.data() + b, k, F->s);
                 HIP_CHECK(hipStreamSynchronize(F->s));
-                for (size_t f = 0; f < k; ++f) {
-                    const int l = hl[f];
-                    double* xs = gx_.data() + f * n;
-                    double* ys = gy_.data() + f * m;
-                    const double* cs = gc_.data() + f * n;
-                    for (int j = 0; j < n; ++j) xs[j] *= F->C[j];
-                    for (int i = 0; i < m; ++i) ys[i] *= F->R[i];
-                    const PdlpState& h = gst[f];
-                    nir_info info;
-                    std::memset(&info, 0, sizeof(info));
-                    info.status = sov_status(h);
-                    info.iterations = h.total;
-                    info.restarts = h.restarts;
-                    DDHost acc(F->offset);
-                    for (int j = 0; j < n; ++j) {
-                        const double cj = cs[j] / F->C[j];
-                        if (cj != 0.0 && xs[j] != 0.0) acc.add_prod(cj, xs[j]);
-                    }
-                    info.objective = acc.value();
-                    info.tau = h.tau;
-                    info.sigma = h.sigma;
-                    info.primal_weight = h.omega;
-                    info.rel_primal = h.rel_p;
-                    info.rel_dual = h.rel_d;
-                    info.rel_gap = h.rel_g;
-                    info.setup_seconds = F->stats.setup_seconds;
-                    F->stats.total_iterations += h.total;
-                    const long long idx = lane_inst[l];
-                    lane_inst[l] = -1;
-                    free_lanes.push_back(l);
-                    --active;
-                    ++finished;
-                    if (done(user, idx, xs, ys, &info) != 0)
-                        throw std::runtime_error("family done callback failed for instance " + std::to_string(idx));
-                }
             }
+            for (int l : fin) {
+                out.index.push_back(lane_inst[l]);
+                lane_inst[l] = -1;
+                free_lanes.push_back(l);
+                --active;
+            }
+            fetched += static_cast<long long>(fin.size());
+            if (!fin.empty()) {
+                std::sort(free_lanes.begin(), free_lanes.end(), std::greater<int>());
+                update_groups();
+            }
+            return out;
+        };
+        // Host half: T(z) unscaled, the objective in double-double from the
+        // caller's costs (scaled cost / C, exact), the status and counters as
+        // nir_session_result reports them, handed to the callback.
+        auto process = [&](Fetched& B) {
+            for (size_t f = 0; f < B.index.size(); ++f) {
+                double* xs = B.x.data() + f * n;
+                double* ys = B.y.data() + f * m;
+                const double* cs = B.c.data() + f * n;
+                for (int j = 0; j < n; ++j) xs[j] *= F->C[j];
+                for (int i = 0; i < m; ++i) ys[i] *= F->R[i];
+                const PdlpState& h = B.st[f];
+                nir_info info;
+                std::memset(&info, 0, sizeof(info));
+                info.status = sov_status(h);
+                info.iterations = h.total;
+                info.restarts = h.restarts;
+                DDHost acc(F->offset);
+                for (int j = 0; j < n; ++j) {
+                    const double cj = cs[j] / F->C[j];
+                    if (cj != 0.0 && xs[j] != 0.0) acc.add_prod(cj, xs[j]);
+                }
+                info.objective = acc.value();
+                info.tau = h.tau;
+                info.sigma = h.sigma;
+                info.primal_weight = h.omega;
+                info.rel_primal = h.rel_p;
+                info.rel_dual = h.rel_d;
+                info.rel_gap = h.rel_g;
+                info.setup_seconds = F->stats.setup_seconds;
+                F->stats.total_iterations += h.total;
+                ++finished;
+                if (done(user, B.index[f], xs, ys, &info) != 0)
+                    throw std::runtime_error("family done callback failed for instance " +
+                                             std::to_string(B.index[f]));
+            }
+            B = Fetched();
+        };
+        // Once no member waits for a lane, move the running members of the
+        // highest groups into free lanes of the lowest ones, so that the
+        // groups launched shrink with the members left.
+        auto compact = [&]() {
+            if (next < count || !ready.empty() || active == 0) return;
+            const int need = (active + kL - 1) / kL;
+            if (need >= F->Gl) return;
+            std::vector<int> src, dst;
+            for (int l = need * kL; l < F->lanes; ++l)
+                if (lane_inst[l] >= 0) src.push_back(l);
+            for (int l = 0; l < need * kL && dst.size() < src.size(); ++l)
+                if (lane_inst[l] < 0) dst.push_back(l);
+            if (src.empty() || dst.size() != src.size()) return;
+            for (size_t b = 0; b < src.size(); b += S) {
+                const size_t k = std::min(S, src.size() - b);
+                std::copy(src.begin() + b, src.begin() + b + k, hl.begin());
+                F->s_lanes.upload(hl.data(), k, F->s);
+                std::vector<int> hd(dst.begin() + b, dst.begin() + b + k);
+                F->s_dst.upload(hd.data(), k, F->s);
+                const int gx = std::max(1, std::min(64, (std::max(n, m) + 255) / 256));
+                k_fam_move<<<dim3(gx, static_cast<unsigned>(k)), 256, 0, F->s>>>(F->view(), static_cast<int>(k),
+                                                                                   F->s_lanes.p, F->s_dst.p);
+                HIP_CHECK(hipGetLastError());
+                HIP_CHECK(hipStreamSynchronize(F->s));
+            }
+            for (size_t f = 0; f < src.size(); ++f) {
+                lane_inst[dst[f]] = lane_inst[src[f]];
+                lane_inst[src[f]] = -1;
+            }
+            free_lanes.clear();
+            for (int l = F->lanes - 1; l >= 0; --l)
+                if (lane_inst[l] < 0) free_lanes.push_back(l);
+            update_groups();
+            ++F->stats.compactions;
         };
         bool stopped = false;
         // Every member ends by its iteration cap, so some lane must finish
         // within cap / eval_freq blocks of the last one that did.
         const long long drain_bound = std::max<long long>(1, F->prm.max_iters) / std::max(1, F->prm.eval_freq) + 8;
-        long long blocks_idle = 0, finished_seen = 0;
-        while (finished < count) {
-            if (finished != finished_seen) {
-                finished_seen = finished;
+        long long blocks_idle = 0, fetched_seen = 0;
+        while (fetched < count) {
+            if (fetched != fetched_seen) {
+                fetched_seen = fetched;
                 blocks_idle = 0;
             } else if (blocks_idle > drain_bound) {
                 throw std::runtime_error("family did not drain within its iteration bound");
@@ -867,24 +1005,32 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
                 prepare(next++);
             refill();
             if (active == 0) {
-                harvest();   // instances that finished at their start point
+                Fetched B = fetch();   // instances that finished at their start point
+                process(B);
                 continue;
             }
             if (F->prm.should_stop && F->prm.should_stop(F->prm.stop_user)) {
                 k_fam_stop<<<(F->lanes + 255) / 256, 256, 0, F->s>>>(F->view(), F->lanes);
                 HIP_CHECK(hipGetLastError());
-                harvest();
+                process(pending);
+                Fetched B = fetch();
+                process(B);
                 stopped = true;
                 break;
             }
+            compact();
             run_block(F);
             ++F->stats.blocks;
+            F->stats.group_blocks += F->Gl;
             ++blocks_idle;
-            // Prepare the next instances while the block runs.
+            // While the block runs: hand out the previous block's members and
+            // prepare the next ones.
+            process(pending);
             while (next < count && static_cast<long long>(ready.size()) < std::max<long long>(1, F->lanes / 16))
                 prepare(next++);
-            harvest();
+            pending = fetch();
         }
+        process(pending);
         HIP_CHECK(hipStreamSynchronize(F->s));
         F->stats.instances += finished;
         F->stats.stopped = stopped ? 1 : 0;

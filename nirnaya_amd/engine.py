@@ -110,7 +110,8 @@ class NirFamilyOpts(C.Structure):
 class NirFamilyStats(C.Structure):
     _fields_ = [("lanes", C.c_int32), ("stopped", C.c_int32), ("instances", C.c_int64), ("blocks", C.c_int64),
                 ("refills", C.c_int64), ("total_iterations", C.c_int64), ("setup_seconds", C.c_double),
-                ("solve_seconds", C.c_double)]
+                ("solve_seconds", C.c_double), ("compactions", C.c_int64), ("group_blocks", C.c_int64),
+                ("prepare_seconds", C.c_double), ("wait_seconds", C.c_double)]
 
 
 class NirPresolveInfo(C.Structure):

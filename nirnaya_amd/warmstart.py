@@ -165,11 +165,12 @@ class WarmStart:
 
 
 def family_solve(engine: Engine, family: ParametricFamily, thetas: np.ndarray, params, starts=None,
-                 primal_weight: float = 0.0, lanes: int = 0, on_result=None):
+                 primal_weight: float = 0.0, lanes: int = 0, on_result=None, with_stats: bool = False):
     """Members theta_i solved by the family engine (nir_family_*: one copy of
     A, 64 members per wave, the single-LP engine's results bit for bit).
     starts[i] = (x0, y0) warm-starts member i with `primal_weight`. Returns
-    (results in theta order, or None with on_result, and seconds)."""
+    (results in theta order, or None with on_result, and seconds; plus the
+    engine's nir_family_stats with `with_stats`)."""
     import time
     count = len(thetas)
     lanes = lanes or min(1024, max(64, (count + 63) // 64 * 64))
@@ -184,7 +185,8 @@ def family_solve(engine: Engine, family: ParametricFamily, thetas: np.ndarray, p
     try:
         t0 = time.perf_counter()
         res = F.solve(count, member, on_result)
-        return res, time.perf_counter() - t0
+        dt = time.perf_counter() - t0
+        return (res, dt, F.stats()) if with_stats else (res, dt)
     finally:
         F.close()
 
