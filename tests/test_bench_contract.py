@@ -44,3 +44,19 @@ def test_default_run_prints_the_contract_line():
     for key in ("value", "unit", "cores", "kind", "sample"):
         assert key in cb, key
     assert cb["kind"] in ("reference", "port") and cb["value"] > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("argv", [
+    ["--workload", "mixed", "--steps", "16", "--warmup", "4", "--no-ttt", "--no-cpu-baseline"],
+    ["--workload", "family", "--steps", "64", "--fam-lanes", "128", "--ws-m", "5000", "--ws-n", "12000"],
+], ids=["mixed", "family"])
+def test_other_workloads_print_one_line(argv):
+    # The secondary workloads' lines (mixed precision on the large LP, the
+    # family engine on a small family) parse and carry a positive value.
+    r = subprocess.run([sys.executable, "-u", BENCH] + argv, capture_output=True, text=True, timeout=300, cwd=ROOT)
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    assert len(lines) == 1, r.stdout[-2000:]
+    d = json.loads(lines[0])
+    assert d["value"] > 0 and d["n_gpus"] == 1 and "workload" in d["config"]

@@ -89,6 +89,36 @@ def test_time_limit_zero_and_interrupt_paths():
     assert m.Status == 4 and m.Iterations == 0   # TimeLimit before iteration 0
 
 
+def test_interrupt_from_another_thread():
+    # sov_interrupt (c_api.cpp) sets the model's stop flag from a second thread
+    # while sov_optimize runs on the first; the engine polls the deadline every
+    # 256 iterations and, like the reference's PdhgSolver on an expired
+    # deadline (src/gpu/pdhg_solver.hpp:63-65), ends with TimeLimit.
+    import threading
+    import time
+    from nirnaya_amd.engine import gen_equality_lp
+    m = build(NirnayaModel("intr"), gen_equality_lp(12, 60, 150, 4))
+    m.set_option("pdhg_tol", "1e-300")            # never met: only the interrupt ends the run
+    m.set_option("pdhg_max_iters", "2000000000")
+    out = {}
+
+    def run():
+        t0 = time.perf_counter()
+        out["rc"] = m.optimize()
+        out["dt"] = time.perf_counter() - t0
+
+    th = threading.Thread(target=run)
+    th.start()
+    time.sleep(2.0)
+    assert th.is_alive(), "the solve ended before the interrupt"
+    assert m.interrupt() == 0
+    th.join(timeout=60)
+    assert not th.is_alive(), "sov_interrupt did not stop the solve"
+    assert out["rc"] == 0
+    assert m.Status == 4 and m.Iterations > 0
+    assert out["dt"] < 30.0
+
+
 def test_checker_certifies_returned_solution(tmp_path):
     import subprocess
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
