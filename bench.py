@@ -486,7 +486,7 @@ def mixed_workload(args):
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_dual_pdlp<float> (FP32 A xbar + dual step); one launch = its segment passes",
                      "bytes_per_launch": t.spmv_bytes, "avg_launch_us": spmv_avg_s * 1e6,
-                     "launches_averaged": k_iters,
+                     "launches_averaged": t.iterations,
                      "dispatches_per_launch": t.spmv_passes},
         "kernels": {"spmtv_primal_avg_us": spmtv_avg_s * 1e6, "spmtv_bytes_per_launch": t.spmtv_bytes,
                     "iteration_bytes": t.iter_bytes},
@@ -572,13 +572,22 @@ def warmstart_workload(args):
             log(f"[bench] held-out cold: {cnt[0]}/{T} in {time.perf_counter() - t0:.0f}s")
     _, t_cold = family_solve(eng, fam, th_test, params, on_result=keep_cold)
     log(f"[bench] held-out cold: {T} LPs to 1e-6 in {t_cold:.1f}s, iterations median {np.median(ci):.0f}")
+    # How far the rank-256 bases can represent the held-out solutions at all:
+    # the projection error, relative to the distance from the mean.
+    dx = np.linalg.norm(cold_x - pod_x.mean, axis=1)
+    dy = np.linalg.norm(cold_y - pod_y.mean, axis=1)
+    px_err = np.linalg.norm(cold_x - pod_x.decode(pod_x.encode(cold_x)), axis=1) / np.maximum(dx, 1e-30)
+    py_err = np.linalg.norm(cold_y - pod_y.decode(pod_y.encode(cold_y)), axis=1) / np.maximum(dy, 1e-30)
+    log(f"[bench] held-out POD projection error / distance from the mean: x {np.median(px_err):.3f}, "
+        f"y {np.median(py_err):.3f}")
+    # Start points: the network's coefficients decoded on the device, where the
+    # family engine reads them (host copies only for the error figures).
     t0 = time.perf_counter()
-    preds = [ws.predict(t) for t in th_test]
+    X0, Y0 = ws.predict_batch(th_test, to_host=True)
     t_pred = time.perf_counter() - t0
-    ex = [np.linalg.norm(p[0] - cold_x[i]) / max(np.linalg.norm(cold_x[i] - pod_x.mean), 1e-30)
-          for i, p in enumerate(preds)]
-    ey = [np.linalg.norm(p[1] - cold_y[i]) / max(np.linalg.norm(cold_y[i] - pod_y.mean), 1e-30)
-          for i, p in enumerate(preds)]
+    ex = np.linalg.norm(X0 - cold_x, axis=1) / np.maximum(dx, 1e-30)
+    ey = np.linalg.norm(Y0 - cold_y, axis=1) / np.maximum(dy, 1e-30)
+    del X0, Y0
     log(f"[bench] held-out prediction error / distance from the mean: x {np.median(ex):.3f}, y {np.median(ey):.3f}")
 
     t0 = time.perf_counter()
@@ -589,7 +598,7 @@ def warmstart_workload(args):
         cnt[0] += 1
         if cnt[0] % 100 == 0:
             log(f"[bench] held-out warm: {cnt[0]}/{T} in {time.perf_counter() - t0:.0f}s")
-    _, t_warm = family_solve(eng, fam, th_test, params, starts=preds, primal_weight=omega, on_result=keep_warm)
+    _, t_warm = family_solve(eng, fam, th_test, params, primal_weight=omega, on_result=keep_warm, device_starts=ws)
     log(f"[bench] held-out warm: {T} LPs to 1e-6 in {t_warm:.1f}s, iterations median {np.median(wi):.0f}")
     # A warm solve that misses the tolerance falls back to its cold solve.
     fallback = [i for i in range(T) if wsts[i] != 0]
@@ -638,6 +647,7 @@ def warmstart_workload(args):
                                 "saved_fraction_median": float(1 - np.median(wi) / np.median(ci)),
                                 "fallbacks": len(fallback)},
         "warm_primal_weight": omega,
+        "held_out_pod_projection_error": {"x_median": float(np.median(px_err)), "y_median": float(np.median(py_err))},
         "held_out_prediction_error": {"x_median": float(np.median(ex)), "y_median": float(np.median(ey)),
                                       "note": "||prediction - solution|| / ||solution - POD mean||"},
         "mlp_loss": {"first_epoch": float(losses[0]), "last_epoch": float(losses[-1])},
@@ -666,6 +676,9 @@ def family_workload(args):
     m, n = args.ws_m, args.ws_n
     fam = ParametricFamily(20264, m, n, 5)
     eng = Engine(0)
+    if args.calibrate:
+        ms = eng.calibrate_copy(256 << 20)
+        log(f"[bench] calibration copy of 256 MiB: {ms:.3f} ms")
     th = fam.thetas(args.fam_lanes, 1)
     p = default_params(mode=MODE_PDLP, max_iters=args.steps, tol=1e-12)
     its = []

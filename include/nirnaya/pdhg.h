@@ -304,6 +304,29 @@ NIR_API int nir_mlp_gemm_check(nir_ctx* ctx, int32_t M, int32_t N, int32_t K, co
 NIR_API int nir_pod_gram(nir_ctx* ctx, const double* X, int32_t len, int32_t N, double* G);
 NIR_API int nir_pod_project(nir_ctx* ctx, const double* X, int32_t len, int32_t N, const double* V, int32_t R,
                             double* Y);
+/* A POD basis kept on the device, the source of learned start points; it
+ * replaces the reference's fixed start (x from the column bounds, y = 0,
+ * src/gpu/pdhg_solver.hpp:47-57) for family members. Modes [len][R] = X V
+ * (X [len][N] centred snapshots, V [N][R]) are computed and kept on the
+ * device with the mean [len]. */
+typedef struct nir_pod nir_pod;
+NIR_API int nir_pod_create(nir_ctx* ctx, const double* X, int32_t len, int32_t N, const double* V, int32_t R,
+                           const double* mean, nir_pod** out);
+/* Copies the modes [len][R] to the host. */
+NIR_API int nir_pod_modes(nir_pod* p, double* modes);
+/* rows [count][len] = mean + alpha [count][R] modes^T on the device. The rows
+ * stay on the device until the next decode or destroy (the family engine's
+ * device starts read them); out, when not null, receives a copy. */
+NIR_API int nir_pod_decode(nir_pod* p, const double* alpha, int32_t count, double* out);
+/* coef [count][R] = modes^T (s - mean) for snapshots s [count][len]. */
+NIR_API int nir_pod_encode(nir_pod* p, const double* s, int32_t count, double* coef);
+NIR_API void nir_pod_destroy(nir_pod* p);
+/* Warm starts read on the device: in the next nir_family_solve calls member
+ * i < rows starts from row i of px's last decode (x0, ncol entries) and of
+ * py's (y0, nrow entries) - scaled and projected exactly as a host x0/y0 -
+ * in place of the fill callback's x0/y0 (its primal weight still applies).
+ * Either pod may be null; both null clears. */
+NIR_API int nir_family_set_device_starts(nir_family* f, const nir_pod* px, const nir_pod* py);
 
 /* ---- double-double solution certifier on the device -----------------------
  * Replaces the host pass of sov::verify_solution (src/io/sol_io.hpp:25-106)

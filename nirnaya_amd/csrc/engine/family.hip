@@ -47,6 +47,7 @@
 #include "lpcheck.hpp"
 #include "pdlp.hpp"
 #include "pdlp_core.hpp"
+#include "pod.hpp"
 
 namespace nir {
 namespace gpu {
@@ -372,10 +373,15 @@ __global__ __launch_bounds__(kL) void k_fam_cert_decide(View V) {
 }
 
 // Refill: slot f (blockIdx.y) of the staging arrays goes into lane lanes[f].
+// A start taken on the device (src[f] >= 0 for x, src[count + f] for y) is
+// row src of dx0 / dy0 in caller units, scaled and projected as the host
+// set-up does it: x = clamp(x0 / C, lo, up), y = y0 / R.
 __global__ __launch_bounds__(256) void k_fam_scatter(View V, int count, const int* __restrict__ lanes,
                                                      const double* __restrict__ sc, const double* __restrict__ sx,
                                                      const double* __restrict__ srl, const double* __restrict__ sru,
-                                                     const double* __restrict__ sy, const PdlpState* __restrict__ sst) {
+                                                     const double* __restrict__ sy, const PdlpState* __restrict__ sst,
+                                                     const long long* __restrict__ src,
+                                                     const double* __restrict__ dx0, const double* __restrict__ dy0) {
     const int f = blockIdx.y;
     if (f >= count) return;
     const int li = lanes[f];
@@ -383,9 +389,17 @@ __global__ __launch_bounds__(256) void k_fam_scatter(View V, int count, const in
     const size_t bx = static_cast<size_t>(g) * V.n * kL + q, by = static_cast<size_t>(g) * V.m * kL + q;
     const int stride = gridDim.x * blockDim.x;
     const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long srx = src[f], sry = src[count + f];
     for (int j = t0; j < V.n; j += stride) {
         const size_t e = bx + static_cast<size_t>(j) * kL;
-        const double v = sx[static_cast<size_t>(f) * V.n + j];
+        double v;
+        if (srx >= 0) {
+            v = dx0[static_cast<size_t>(srx) * V.n + j] / V.cscale[j];
+            v = (V.lo[j] < v) ? v : V.lo[j];
+            v = (v < V.up[j]) ? v : V.up[j];
+        } else {
+            v = sx[static_cast<size_t>(f) * V.n + j];
+        }
         V.x[e] = v;
         V.xbar[e] = v;
         V.xa[e] = v;
@@ -394,7 +408,8 @@ __global__ __launch_bounds__(256) void k_fam_scatter(View V, int count, const in
     }
     for (int i = t0; i < V.m; i += stride) {
         const size_t e = by + static_cast<size_t>(i) * kL;
-        const double v = sy[static_cast<size_t>(f) * V.m + i];
+        const double v = sry >= 0 ? dy0[static_cast<size_t>(sry) * V.m + i] / V.rscale[i]
+                                  : sy[static_cast<size_t>(f) * V.m + i];
         V.y[e] = v;
         V.ya[e] = v;
         V.yp[e] = v;
@@ -519,7 +534,11 @@ struct nir_family {
     DevArray<int> s_lanes, s_dst;
     DevArray<double> s_c, s_x, s_rl, s_ru, s_y;
     DevArray<PdlpState> s_st;
+    DevArray<long long> s_src;   // [2][S]: device start row of x / y per slot, -1 = staged
     std::vector<int> h_done;
+    // device warm starts (nir_family_set_device_starts)
+    const nir_pod* start_x = nullptr;
+    const nir_pod* start_y = nullptr;
     int ebc = 1, ebr = 1;
     // graph of one evaluation block
     hipGraph_t graph = nullptr;
@@ -627,6 +646,7 @@ void run_block(nir_family* F) {
 struct Prepared {
     long long index;
     PdlpInstance inst;
+    bool dev_x = false, dev_y = false;   // start rows read on the device
 };
 
 }  // namespace
@@ -740,6 +760,7 @@ NIR_API int nir_family_create(nir_ctx* ctx, const nir_lp* base, const nir_params
         F->s_ru.alloc(S * m);
         F->s_y.alloc(S * m);
         F->s_st.alloc(S);
+        F->s_src.alloc(2 * S);
         F->h_done.assign(L, 1);
         std::memset(&F->stats, 0, sizeof(F->stats));
         F->stats.lanes = F->lanes;
@@ -781,6 +802,25 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
         auto secs = [](clock::time_point a) { return std::chrono::duration<double>(clock::now() - a).count(); };
         long long next = 0, finished = 0;
         int active = 0;
+        // Device start rows (nir_family_set_device_starts): member idx < rows
+        // reads its start from row idx on the device.
+        int dxn = 0, dyn = 0;
+        const double *dx0 = nullptr, *dy0 = nullptr;
+        {
+            int len = 0, dev = 0;
+            if (F->start_x) {
+                dx0 = pod_rows(F->start_x, &dxn, &len, &dev);
+                if (dx0 && (len != n || dev != F->ctx->device))
+                    throw std::invalid_argument("device x starts: rows of the wrong length or device");
+            }
+            if (F->start_y) {
+                dy0 = pod_rows(F->start_y, &dyn, &len, &dev);
+                if (dy0 && (len != m || dev != F->ctx->device))
+                    throw std::invalid_argument("device y starts: rows of the wrong length or device");
+            }
+            if (!dx0) dxn = 0;
+            if (!dy0) dyn = 0;
+        }
         std::vector<double> cost(n), rlo(m), rup(m), x0(n), y0(m);
         std::vector<Prepared> ready;
         // Prepare one instance on the host with the single-LP set-up code.
@@ -805,11 +845,13 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
             lp.rowup = rup.data();
             lp.obj_offset = F->offset;
             nir_params p = F->prm;
-            p.x0 = (warm & 1) ? x0.data() : nullptr;
-            p.y0 = (warm & 2) ? y0.data() : nullptr;
-            if (weight > 0.0) p.primal_weight = weight;
             Prepared P;
             P.index = idx;
+            P.dev_x = idx < dxn;
+            P.dev_y = idx < dyn;
+            p.x0 = ((warm & 1) && !P.dev_x) ? x0.data() : nullptr;
+            p.y0 = ((warm & 2) && !P.dev_y) ? y0.data() : nullptr;
+            if (weight > 0.0) p.primal_weight = weight;
             pdlp_prepare_instance(&lp, p, F->R, F->C, F->eta, false, P.inst);
             ready.push_back(std::move(P));
             F->stats.prepare_seconds += secs(tp);
@@ -817,6 +859,7 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
         std::vector<double> hc(S * n), hx(S * n), hrl(S * m), hru(S * m), hy(S * m);
         std::vector<PdlpState> hst(S);
         std::vector<int> hl(S);
+        std::vector<long long> hsrc(2 * S);
         // Move prepared instances into free lanes (chunks of S), then
         // evaluate them at their start point.
         auto refill = [&]() {
@@ -835,6 +878,8 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
                     std::copy(P.inst.ru.begin(), P.inst.ru.end(), hru.begin() + f * m);
                     std::copy(P.inst.y0.begin(), P.inst.y0.end(), hy.begin() + f * m);
                     hst[f] = P.inst.st;
+                    hsrc[f] = P.dev_x ? P.index : -1;
+                    hsrc[k + f] = P.dev_y ? P.index : -1;
                 }
                 ready.resize(ready.size() - k);
                 F->s_lanes.upload(hl.data(), k, F->s);
@@ -844,10 +889,11 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
                 F->s_ru.upload(hru.data(), k * m, F->s);
                 F->s_y.upload(hy.data(), k * m, F->s);
                 F->s_st.upload(hst.data(), k, F->s);
+                F->s_src.upload(hsrc.data(), 2 * k, F->s);
                 const int gx = std::max(1, std::min(64, (std::max(n, m) + 255) / 256));
                 k_fam_scatter<<<dim3(gx, static_cast<unsigned>(k)), 256, 0, F->s>>>(
                     F->view(), static_cast<int>(k), F->s_lanes.p, F->s_c.p, F->s_x.p, F->s_rl.p, F->s_ru.p, F->s_y.p,
-                    F->s_st.p);
+                    F->s_st.p, F->s_src.p, dx0, dy0);
                 HIP_CHECK(hipGetLastError());
                 // The staging buffers are reused by the next chunk.
                 HIP_CHECK(hipStreamSynchronize(F->s));
@@ -1041,6 +1087,13 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
         (void)hipStreamSynchronize(F->s);
         return -1;
     }
+}
+
+NIR_API int nir_family_set_device_starts(nir_family* F, const nir_pod* px, const nir_pod* py) {
+    if (!F) return -1;
+    F->start_x = px;
+    F->start_y = py;
+    return 0;
 }
 
 NIR_API int nir_family_get_stats(nir_family* F, nir_family_stats* out) {

@@ -304,6 +304,19 @@ __global__ void k_mirror_upper(double* G, int N) {
     }
 }
 
+// rows[r][j] = v[j] (r < count), or rows[r][j] -= v[j] with SUB.
+template <bool SUB>
+__global__ void k_rows_mean(double* rows, const double* __restrict__ v, int count, int len) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    const long long total = static_cast<long long>(count) * len;
+    for (long long e = t; e < total; e += stride) {
+        const double m = v[e % len];
+        if (SUB) rows[e] -= m;
+        else rows[e] = m;
+    }
+}
+
 }  // namespace mlp
 }  // namespace gpu
 }  // namespace nir
@@ -422,6 +435,33 @@ void check_rows(nir_mlp* P, int rows) {
 }
 
 }  // namespace
+
+// A POD basis resident on the device: modes [len][R] row-major (the
+// column-major R x len matrix M), the mean, and the rows of the last decode
+// ([count][len] row-major), which the family engine can take as warm starts
+// without a trip through the host.
+struct nir_pod {
+    nir_ctx* ctx = nullptr;
+    int len = 0, R = 0;
+    DevArray<double> modes, mean, rows;
+    int rows_count = 0;
+    rocblas_handle h = nullptr;
+    ~nir_pod() {
+        if (h) (void)rocblas_destroy_handle(h);
+    }
+};
+
+namespace nir {
+namespace gpu {
+const double* pod_rows(const nir_pod* P, int* count, int* len, int* device) {
+    if (!P) return nullptr;
+    *count = P->rows_count;
+    *len = P->len;
+    *device = P->ctx->device;
+    return P->rows_count > 0 ? P->rows.p : nullptr;
+}
+}  // namespace gpu
+}  // namespace nir
 
 extern "C" {
 
@@ -758,6 +798,117 @@ NIR_API int nir_pod_project(nir_ctx* ctx, const double* X, int32_t len, int32_t 
         ctx->error = ex.what();
         return -1;
     }
+}
+
+NIR_API int nir_pod_create(nir_ctx* ctx, const double* X, int32_t len, int32_t N, const double* V, int32_t R,
+                           const double* mean, nir_pod** out) {
+    if (!ctx || !X || !V || !mean || !out || len <= 0 || N <= 0 || R <= 0) return -1;
+    *out = nullptr;
+    auto P = std::make_unique<nir_pod>();
+    P->ctx = ctx;
+    P->len = len;
+    P->R = R;
+    try {
+        HIP_CHECK(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        ROCBLAS_CHECK(rocblas_create_handle(&P->h));
+        ROCBLAS_CHECK(rocblas_set_stream(P->h, s));
+        DevArray<double> dx, dv;
+        dx.alloc(static_cast<size_t>(len) * N);
+        dv.alloc(static_cast<size_t>(N) * R);
+        P->modes.alloc(static_cast<size_t>(len) * R);
+        P->mean.alloc(static_cast<size_t>(len));
+        dx.upload(X, static_cast<size_t>(len) * N, s);
+        dv.upload(V, static_cast<size_t>(N) * R, s);
+        P->mean.upload(mean, static_cast<size_t>(len), s);
+        // The same DGEMM as nir_pod_project: M (R x len) = V^T X^T.
+        const double one = 1.0, zero = 0.0;
+        ROCBLAS_CHECK(rocblas_dgemm(P->h, rocblas_operation_none, rocblas_operation_none, R, len, N, &one, dv.p, R,
+                                    dx.p, N, &zero, P->modes.p, R));
+        HIP_CHECK(hipStreamSynchronize(s));
+        *out = P.release();
+        return 0;
+    } catch (const std::exception& ex) {
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->error = ex.what();
+        return -1;
+    }
+}
+
+NIR_API int nir_pod_modes(nir_pod* P, double* modes) {
+    if (!P || !modes) return -1;
+    try {
+        HIP_CHECK(hipSetDevice(P->ctx->device));
+        P->modes.download(modes, static_cast<size_t>(P->len) * P->R, P->ctx->stream);
+        HIP_CHECK(hipStreamSynchronize(P->ctx->stream));
+        return 0;
+    } catch (const std::exception& ex) {
+        P->ctx->error = ex.what();
+        return -1;
+    }
+}
+
+// rows [count][len] = mean + alpha [count][R] M: column-major, rows^T
+// (len x count) = M^T (len x R) alpha^T (R x count), one DGEMM onto the
+// broadcast mean.
+NIR_API int nir_pod_decode(nir_pod* P, const double* alpha, int32_t count, double* out) {
+    if (!P || !alpha || count <= 0) return -1;
+    try {
+        HIP_CHECK(hipSetDevice(P->ctx->device));
+        hipStream_t s = P->ctx->stream;
+        const size_t total = static_cast<size_t>(count) * P->len;
+        if (P->rows.n < total) P->rows.alloc(total);
+        P->rows_count = 0;
+        DevArray<double> da;
+        da.alloc(static_cast<size_t>(count) * P->R);
+        da.upload(alpha, static_cast<size_t>(count) * P->R, s);
+        k_rows_mean<false><<<1024, 256, 0, s>>>(P->rows.p, P->mean.p, count, P->len);
+        HIP_CHECK(hipGetLastError());
+        const double one = 1.0;
+        ROCBLAS_CHECK(rocblas_dgemm(P->h, rocblas_operation_transpose, rocblas_operation_none, P->len, count, P->R,
+                                    &one, P->modes.p, P->R, da.p, P->R, &one, P->rows.p, P->len));
+        if (out) P->rows.download(out, total, s);
+        HIP_CHECK(hipStreamSynchronize(s));
+        P->rows_count = count;
+        return 0;
+    } catch (const std::exception& ex) {
+        (void)hipStreamSynchronize(P->ctx->stream);
+        P->ctx->error = ex.what();
+        return -1;
+    }
+}
+
+// coef [count][R] = (s - mean) M^T: column-major, coef^T (R x count) =
+// M (R x len) (s - mean)^T (len x count).
+NIR_API int nir_pod_encode(nir_pod* P, const double* snap, int32_t count, double* coef) {
+    if (!P || !snap || !coef || count <= 0) return -1;
+    try {
+        HIP_CHECK(hipSetDevice(P->ctx->device));
+        hipStream_t s = P->ctx->stream;
+        DevArray<double> ds, dc;
+        ds.alloc(static_cast<size_t>(count) * P->len);
+        dc.alloc(static_cast<size_t>(count) * P->R);
+        ds.upload(snap, static_cast<size_t>(count) * P->len, s);
+        k_rows_mean<true><<<1024, 256, 0, s>>>(ds.p, P->mean.p, count, P->len);
+        HIP_CHECK(hipGetLastError());
+        const double one = 1.0, zero = 0.0;
+        ROCBLAS_CHECK(rocblas_dgemm(P->h, rocblas_operation_none, rocblas_operation_none, P->R, count, P->len, &one,
+                                    P->modes.p, P->R, ds.p, P->len, &zero, dc.p, P->R));
+        dc.download(coef, static_cast<size_t>(count) * P->R, s);
+        HIP_CHECK(hipStreamSynchronize(s));
+        return 0;
+    } catch (const std::exception& ex) {
+        (void)hipStreamSynchronize(P->ctx->stream);
+        P->ctx->error = ex.what();
+        return -1;
+    }
+}
+
+NIR_API void nir_pod_destroy(nir_pod* P) {
+    if (!P) return;
+    (void)hipSetDevice(P->ctx->device);
+    (void)hipStreamSynchronize(P->ctx->stream);
+    delete P;
 }
 
 }  // extern "C"

@@ -12,6 +12,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 from dataclasses import dataclass
+import weakref
 from typing import Optional
 
 import numpy as np
@@ -199,6 +200,12 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                                                 C.POINTER(NirVerifyReport)]
     lib.nir_pod_gram.argtypes = [vp, _dp, C.c_int32, C.c_int32, _dp]
     lib.nir_pod_project.argtypes = [vp, _dp, C.c_int32, C.c_int32, _dp, C.c_int32, _dp]
+    lib.nir_pod_create.argtypes = [vp, _dp, C.c_int32, C.c_int32, _dp, C.c_int32, _dp, C.POINTER(vp)]
+    lib.nir_pod_modes.argtypes = [vp, _dp]
+    lib.nir_pod_decode.argtypes = [vp, _dp, C.c_int32, _dp]
+    lib.nir_pod_encode.argtypes = [vp, _dp, C.c_int32, _dp]
+    lib.nir_pod_destroy.argtypes = [vp]
+    lib.nir_family_set_device_starts.argtypes = [vp, vp, vp]
     lib.nir_gen_equality_lp.argtypes = [C.c_uint64, C.c_int32, C.c_int32, C.c_int32]
     lib.nir_gen_mixed_lp.argtypes = [C.c_uint64, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double]
     lib.nir_gen_staircase_lp.argtypes = [C.c_uint64, C.c_int32, C.c_int32]
@@ -381,9 +388,17 @@ class Engine:
         if self.lib.nir_ctx_create(device, C.byref(h)) != 0:
             raise EngineUnavailable("nir_ctx_create failed: " + self.lib.nir_last_error().decode())
         self.h = h
+        # Objects made on this context (sessions, batches, families, pods,
+        # networks): closed before the context itself.
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, child):
+        self._children.add(child)
 
     def close(self):
         if getattr(self, "h", None):
+            for child in list(getattr(self, "_children", ())):
+                child.close()
             self.lib.nir_ctx_destroy(self.h)
             self.h = None
 
@@ -461,6 +476,11 @@ class Engine:
             raise EngineError("nir_pod_project: " + self.error())
         return Y
 
+    def pod_basis(self, X: np.ndarray, V: np.ndarray, mean: np.ndarray) -> "DevicePod":
+        """A POD basis kept on the device: modes X V for centred snapshots
+        X [len][N] and V [N][R], and the mean (nir_pod_*)."""
+        return DevicePod(self, X, V, mean)
+
     def mlp(self, n_in: int, hidden: int, n_out: int, max_rows: int, seed: int = 0) -> "Mlp":
         return Mlp(self, n_in, hidden, n_out, max_rows, seed)
 
@@ -488,6 +508,7 @@ class Session:
 
     def __init__(self, eng: Engine, lp: LP, params: Optional[NirParams], x0=None, y0=None):
         self.eng, self.lib, self.lp = eng, eng.lib, lp
+        eng._adopt(self)
         # A private copy: the warm-start pointers set below must not leak into
         # the caller's parameters (and from there into later sessions).
         self.params = NirParams.from_buffer_copy(params) if params is not None else default_params()
@@ -567,12 +588,68 @@ class Session:
         self.close()
 
 
+class DevicePod:
+    """A POD basis resident on the device (nir_pod_*): decode writes start
+    points on the device, where the family engine reads them."""
+
+    def __init__(self, eng: Engine, X: np.ndarray, V: np.ndarray, mean: np.ndarray):
+        self.eng, self.lib = eng, eng.lib
+        eng._adopt(self)
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        mean = np.ascontiguousarray(mean, dtype=np.float64)
+        self.len, self.rank = X.shape[0], V.shape[1]
+        if V.shape[0] != X.shape[1] or mean.shape != (self.len,):
+            raise ValueError("pod_basis: X [len][N], V [N][R] and mean [len] do not match")
+        h = C.c_void_p()
+        if self.lib.nir_pod_create(eng.h, _ptr(X, C.c_double), X.shape[0], X.shape[1], _ptr(V, C.c_double),
+                                   self.rank, _ptr(mean, C.c_double), C.byref(h)) != 0:
+            raise EngineError("nir_pod_create: " + eng.error())
+        self.h = h
+        self.rows = 0   # rows of the last decode held on the device
+
+    def modes(self) -> np.ndarray:
+        out = np.zeros((self.len, self.rank))
+        if self.lib.nir_pod_modes(self.h, _ptr(out, C.c_double)) != 0:
+            raise EngineError("nir_pod_modes: " + self.eng.error())
+        return out
+
+    def decode(self, alpha: np.ndarray, to_host: bool = True):
+        """Rows mean + alpha_i modes^T for alpha [count][R], computed and kept
+        on the device; returned as [count][len] when to_host."""
+        alpha = np.ascontiguousarray(np.atleast_2d(alpha), dtype=np.float64)
+        if alpha.shape[1] != self.rank:
+            raise ValueError("decode: alpha must have rank columns")
+        out = np.zeros((alpha.shape[0], self.len)) if to_host else None
+        if self.lib.nir_pod_decode(self.h, _ptr(alpha, C.c_double), alpha.shape[0],
+                                   _ptr(out, C.c_double) if to_host else None) != 0:
+            raise EngineError("nir_pod_decode: " + self.eng.error())
+        self.rows = alpha.shape[0]
+        return out
+
+    def encode(self, S: np.ndarray) -> np.ndarray:
+        S = np.ascontiguousarray(np.atleast_2d(S), dtype=np.float64)
+        coef = np.zeros((S.shape[0], self.rank))
+        if self.lib.nir_pod_encode(self.h, _ptr(S, C.c_double), S.shape[0], _ptr(coef, C.c_double)) != 0:
+            raise EngineError("nir_pod_encode: " + self.eng.error())
+        return coef
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nir_pod_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
 class Family:
     """Members of a parametric family (shared A and column bounds, own costs
     and row bounds) streamed through the family engine (nir_family_*)."""
 
     def __init__(self, eng: Engine, base: LP, params: NirParams, lanes: int = 1024):
         self.eng, self.lib, self.base = eng, eng.lib, base
+        eng._adopt(self)
         self.params = params
         self._s = base.struct()
         opts = NirFamilyOpts()
@@ -582,6 +659,13 @@ class Family:
         if self.lib.nir_family_create(eng.h, C.byref(self._s), C.byref(params), C.byref(opts), C.byref(h)) != 0:
             raise EngineError("nir_family_create: " + eng.error())
         self.h = h
+
+    def set_device_starts(self, pod_x: Optional["DevicePod"], pod_y: Optional["DevicePod"]):
+        """Members i below the rows of the pods' last decode start from those
+        rows, read on the device (nir_family_set_device_starts)."""
+        if self.lib.nir_family_set_device_starts(self.h, pod_x.h if pod_x is not None else None,
+                                                 pod_y.h if pod_y is not None else None) != 0:
+            raise EngineError("nir_family_set_device_starts: " + self.eng.error())
 
     def solve(self, count: int, member, on_result=None):
         """Solves members 0 .. count-1. member(i) returns (cost, rowlo, rowup),
@@ -657,6 +741,7 @@ class Batch:
     def __init__(self, eng: Engine, lps, params: NirParams, threads=0, quantum=0, rounds_per_graph=0, workgroups=0,
                  x0s=None, y0s=None):
         self.eng, self.lib, self.lps = eng, eng.lib, list(lps)
+        eng._adopt(self)
         self.params = params
         self._structs = [lp.struct() for lp in self.lps]
         arr = (C.POINTER(NirLp) * len(self._structs))(*[C.pointer(s) for s in self._structs])
@@ -717,6 +802,7 @@ class Mlp:
 
     def __init__(self, eng: Engine, n_in: int, hidden: int, n_out: int, max_rows: int, seed: int = 0):
         self.eng, self.lib = eng, eng.lib
+        eng._adopt(self)
         self.n_in, self.hidden, self.n_out = n_in, hidden, n_out
         h = C.c_void_p()
         if self.lib.nir_mlp_create(eng.h, n_in, hidden, n_out, max_rows, seed, C.byref(h)) != 0:

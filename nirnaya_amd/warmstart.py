@@ -85,7 +85,9 @@ class Pod:
     The eigenpairs of the snapshot Gram matrix are exact (a full symmetric
     eigendecomposition) up to N = exact_limit snapshots and from block
     subspace iteration (top_eigh) beyond, where only the leading `rank` of N
-    pairs are used."""
+    pairs are used. The modes and the mean live on the device
+    (`engine.pod_basis`); decoding writes start points there, where the
+    family engine reads them, and `modes` is a host copy made on demand."""
 
     def __init__(self, engine: Engine, S: np.ndarray, rank: int, rel_floor: float = 1e-12,
                  exact_limit: int = 2048, log=None):
@@ -108,15 +110,29 @@ class Pod:
         keep = int(np.sum(lam > rel_floor * max(lam[0], 1e-300))) if lam.size else 0
         r = max(1, min(rank, keep))
         self.sing = np.sqrt(np.maximum(lam[:r], 0.0))
-        self.modes = engine.pod_project(X, V[:, :r] / self.sing)   # X V S^-1 on the device
+        self.basis = engine.pod_basis(X, V[:, :r] / self.sing, self.mean)   # modes X V S^-1, kept on the device
         self.train_coef = V[:, :r] * self.sing                   # [N][r] = modes^T (s - mean)
         self.rank = r
+        self._modes = None
+
+    @property
+    def modes(self) -> np.ndarray:
+        if self._modes is None:
+            self._modes = self.basis.modes()
+        return self._modes
 
     def encode(self, s: np.ndarray) -> np.ndarray:
-        return self.modes.T @ (np.asarray(s, dtype=np.float64) - self.mean)
+        """Coefficients modes^T (s - mean) of one snapshot or of rows [count][len]."""
+        s = np.asarray(s, dtype=np.float64)
+        coef = self.basis.encode(s)
+        return coef[0] if s.ndim == 1 else coef
 
-    def decode(self, alpha: np.ndarray) -> np.ndarray:
-        return self.mean + self.modes @ np.asarray(alpha, dtype=np.float64)
+    def decode(self, alpha: np.ndarray, to_host: bool = True):
+        """mean + modes alpha for one coefficient vector or rows [count][r],
+        computed on the device, where the rows stay for the family engine."""
+        alpha = np.asarray(alpha, dtype=np.float64)
+        rows = self.basis.decode(alpha, to_host)
+        return rows[0] if (to_host and alpha.ndim == 1) else rows
 
 
 def _pad64(k: int) -> int:
@@ -129,6 +145,7 @@ class WarmStart:
     def __init__(self, engine: Engine, pod_x: Pod, pod_y: Pod, hidden: int = 512, seed: int = 0,
                  max_rows: int = 512, dim: int = THETA_DIM, primal_weight: float = 0.0, out_width: int = 0):
         self.engine, self.pod_x, self.pod_y, self.dim = engine, pod_x, pod_y, dim
+        self.max_rows = max_rows
         # The primal weight the family's training solves ended with (median),
         # carried into warm solves with the predicted point; 0 = engine default.
         self.primal_weight = primal_weight
@@ -157,27 +174,52 @@ class WarmStart:
         target[:, :self.n_coef] = (alpha - self.mu[:self.n_coef]) / self.sd[:self.n_coef]
         return self.net.train(thetas.astype(np.float32), target, epochs, batch, lr, seed)
 
+    def coefficients(self, thetas: np.ndarray) -> np.ndarray:
+        """POD coefficients [count][rank_x + rank_y] the network predicts for
+        thetas [count][dim] (forward passes of at most max_rows rows)."""
+        T = np.asarray(thetas, dtype=np.float32).reshape(-1, self.dim)
+        out = np.concatenate([self.net.forward(T[i:i + self.max_rows]) for i in range(0, len(T), self.max_rows)])
+        return (out.astype(np.float64) * self.sd + self.mu)[:, :self.n_coef]
+
     def predict(self, theta: np.ndarray):
-        out = self.net.forward(np.asarray(theta, dtype=np.float32).reshape(1, -1))[0].astype(np.float64)
-        alpha = out * self.sd + self.mu
+        alpha = self.coefficients(theta)[0]
         rx = self.pod_x.rank
-        return self.pod_x.decode(alpha[:rx]), self.pod_y.decode(alpha[rx:self.n_coef])
+        return self.pod_x.decode(alpha[:rx]), self.pod_y.decode(alpha[rx:])
+
+    def predict_batch(self, thetas: np.ndarray, to_host: bool = False):
+        """Start points for thetas [count][dim], decoded on the device, where
+        they stay as the pods' decode rows (the family engine's device starts,
+        `family_solve(..., device_starts=ws)`); host copies (X0 [count][n],
+        Y0 [count][m]) when to_host, else (None, None)."""
+        alpha = self.coefficients(thetas)
+        rx = self.pod_x.rank
+        return self.pod_x.decode(alpha[:, :rx], to_host), self.pod_y.decode(alpha[:, rx:], to_host)
 
 
 def family_solve(engine: Engine, family: ParametricFamily, thetas: np.ndarray, params, starts=None,
-                 primal_weight: float = 0.0, lanes: int = 0, on_result=None, with_stats: bool = False):
+                 primal_weight: float = 0.0, lanes: int = 0, on_result=None, with_stats: bool = False,
+                 device_starts: "WarmStart" = None):
     """Members theta_i solved by the family engine (nir_family_*: one copy of
     A, 64 members per wave, the single-LP engine's results bit for bit).
-    starts[i] = (x0, y0) warm-starts member i with `primal_weight`. Returns
+    starts[i] = (x0, y0) warm-starts member i with `primal_weight`; with
+    device_starts (a WarmStart whose predict_batch ran on these thetas) member
+    i starts from row i of its pods' decode rows, read on the device. Returns
     (results in theta order, or None with on_result, and seconds; plus the
     engine's nir_family_stats with `with_stats`)."""
     import time
     count = len(thetas)
     lanes = lanes or min(1024, max(64, (count + 63) // 64 * 64))
     F = engine.family(family.base, params, lanes=lanes)
+    if device_starts is not None:
+        px, py = device_starts.pod_x.basis, device_starts.pod_y.basis
+        if px.rows < count or py.rows < count:
+            raise ValueError("device starts: predict_batch did not decode a row for every member")
+        F.set_device_starts(px, py)
 
     def member(i):
         b, c = family.rhs_cost(thetas[i])
+        if device_starts is not None:
+            return c, b, b, None, None, primal_weight
         if starts is None:
             return c, b, b
         x0, y0 = starts[i]
@@ -274,9 +316,9 @@ def run_pipeline(engine: Engine, family: ParametricFamily, n_train: int, n_test:
                          train_solve_seconds=t_solve, pod_seconds=t_pod, train_seconds=t_fit)
     cold, t_cold = family_solve(engine, family, th_test, params)
     t0 = time.perf_counter()
-    preds = [ws.predict(t) for t in th_test]
+    ws.predict_batch(th_test)   # start points decoded on the device, read there by the family engine
     t_pred = time.perf_counter() - t0
-    warm, t_warm = family_solve(engine, family, th_test, params, starts=preds, primal_weight=ws.primal_weight)
+    warm, t_warm = family_solve(engine, family, th_test, params, primal_weight=ws.primal_weight, device_starts=ws)
     per_c, per_w = t_cold / max(1, n_test), (t_warm + t_pred) / max(1, n_test)
     for c, w in zip(cold, warm):
         rep.cold.append(SolveRecord(c.status, c.iterations, per_c))

@@ -99,6 +99,43 @@ def test_family_warm_starts_cap_and_start_point_optimum(engine):
         assert rc[i].status == 6 and rc[i].iterations == 640 and same(rc[i], s)
 
 
+def test_family_device_starts_equal_host_starts(engine):
+    """Warm starts decoded on the device (nir_pod_decode) and read there by
+    the family engine (nir_family_set_device_starts) give every member
+    exactly the result of the same start handed over from the host, which is
+    the single-LP engine's with that x0 / y0."""
+    fam = ParametricFamily(20264, 300, 700, 5)
+    th = fam.thetas(20, 9)
+    params = default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-6)
+    cold = [engine.solve(fam.instance(t), params) for t in th[:6]]
+    rng = np.random.default_rng(4)
+    # Bases from the first cold solutions plus noise; starts decoded from
+    # random coefficients around them.
+    Sx = np.stack([r.x for r in cold]) + 0.01 * rng.standard_normal((6, fam.base.n))
+    Sy = np.stack([r.y for r in cold]) + 0.01 * rng.standard_normal((6, fam.base.m))
+    px = engine.pod_basis((Sx - Sx.mean(0)).T.copy(), np.eye(6)[:, :4], Sx.mean(0))
+    py = engine.pod_basis((Sy - Sy.mean(0)).T.copy(), np.eye(6)[:, :4], Sy.mean(0))
+    try:
+        ax, ay = 0.3 * rng.standard_normal((len(th), 4)), 0.3 * rng.standard_normal((len(th), 4))
+        X0, Y0 = px.decode(ax), py.decode(ay)
+        F = engine.family(fam.base, params, lanes=64)
+        try:
+            F.set_device_starts(px, py)
+            dev = F.solve(len(th), lambda i: (*members(fam, th)(i), None, None, 0.7))
+            F.set_device_starts(None, None)
+            host = F.solve(len(th), lambda i: (*members(fam, th)(i), X0[i], Y0[i], 0.7))
+        finally:
+            F.close()
+    finally:
+        px.close()
+        py.close()
+    for i in range(len(th)):
+        assert same(dev[i], host[i]), f"member {i}"
+    p7 = default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-6, primal_weight=0.7)
+    for i in (0, 13):
+        assert same(dev[i], engine.solve(fam.instance(th[i]), p7, x0=X0[i], y0=Y0[i])), f"member {i}"
+
+
 def test_family_certifies_infeasible_members(engine):
     """Members of an infeasible base (costs perturbed) end with the single-LP
     engine's certificate status and iterate."""

@@ -108,3 +108,26 @@ def test_pod_products_match_numpy(engine):
     assert np.allclose(G, X.T @ X, rtol=1e-12, atol=1e-9)
     V = rng.standard_normal((37, 5))
     assert np.allclose(engine.pod_project(X, V), X @ V, rtol=1e-12, atol=1e-10)
+
+
+def test_device_pod_basis_matches_numpy(engine):
+    # The basis kept on the device: modes = X V, decode = mean + alpha modes^T
+    # (one DGEMM onto the broadcast mean), encode = (s - mean) modes; the
+    # decode rows stay on the device (rows) for the family engine.
+    rng = np.random.default_rng(12)
+    X = rng.standard_normal((1500, 40))
+    V = rng.standard_normal((40, 7))
+    mean = rng.standard_normal(1500)
+    pod = engine.pod_basis(X, V, mean)
+    try:
+        M = X @ V
+        assert np.allclose(pod.modes(), M, rtol=1e-12, atol=1e-10)
+        alpha = rng.standard_normal((33, 7))
+        rows = pod.decode(alpha)
+        assert pod.rows == 33
+        assert np.allclose(rows, mean + alpha @ M.T, rtol=1e-12, atol=1e-9)
+        S = rng.standard_normal((9, 1500))
+        assert np.allclose(pod.encode(S), (S - mean) @ M, rtol=1e-12, atol=1e-9)
+        assert pod.decode(alpha[:5], to_host=False) is None and pod.rows == 5
+    finally:
+        pod.close()

@@ -81,7 +81,8 @@ def test_cfg4_full_size_family_warm_start(engine):
     set (64 solves) and 16 held-out thetas. Training and held-out solves run
     on the family engine (one copy of A, 64 members per wave); POD ranks are
     min(256, 63) each and the MLP is 64 -> 512 -> 512 -> 512 (bf16 MFMA,
-    GELU). Every warm-started solve must reach the tolerance on its own, and
+    GELU); the warm starts are decoded on the device and read there by the
+    family engine. Every warm-started solve must reach the tolerance on its own, and
     the oracle's or_kkt (double-double on the host) certifies every warm and
     cold solution; the iterations saved are reported. One member is checked
     against the single-LP engine bit for bit."""
@@ -104,8 +105,8 @@ def test_cfg4_full_size_family_warm_start(engine):
     assert losses[-1] < 0.5 * losses[0]
     test = fam.thetas(16, 2)
     cold, t_cold = family_solve(engine, fam, test, params)
-    preds = [ws.predict(t) for t in test]
-    warm, t_warm = family_solve(engine, fam, test, params, starts=preds, primal_weight=omega)
+    ws.predict_batch(test)   # start points decoded and kept on the device
+    warm, t_warm = family_solve(engine, fam, test, params, primal_weight=omega, device_starts=ws)
     ws.net.close()
     for i, t in enumerate(test):
         lp = fam.instance(t)

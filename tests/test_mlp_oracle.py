@@ -79,16 +79,34 @@ def test_emulated_rounding_changes_little():
     assert 1e-5 < rel < 5e-2   # bf16 rounding is visible but small
 
 
+class _NumpyBasis:
+    """numpy stand-in for the device-resident basis (modes X V, the mean)."""
+
+    def __init__(self, X, V, mean):
+        self._m, self._mean, self.rank, self.rows = X @ V, mean, V.shape[1], 0
+
+    def modes(self):
+        return self._m.copy()
+
+    def decode(self, alpha, to_host=True):
+        alpha = np.atleast_2d(alpha)
+        self.rows = alpha.shape[0]
+        return self._mean + alpha @ self._m.T if to_host else None
+
+    def encode(self, S):
+        return (np.atleast_2d(S) - self._mean) @ self._m
+
+
 class _NumpyPodEngine:
-    """numpy stand-ins for the two device products of the POD."""
+    """numpy stand-ins for the device products of the POD."""
 
     @staticmethod
     def pod_gram(X):
         return X.T @ X
 
     @staticmethod
-    def pod_project(X, V):
-        return X @ V
+    def pod_basis(X, V, mean):
+        return _NumpyBasis(X, V, mean)
 
 
 def test_pod_host_logic_matches_svd():
