@@ -535,6 +535,7 @@ def warmstart_workload(args):
             log(f"[bench] training solves: {count[0]}/{N} in {time.perf_counter() - t0:.0f}s")
     _, t_train, fst = family_solve(eng, fam, th_train, tparams, on_result=keep, lanes=1024, with_stats=True)
     log(f"[bench] training family engine: {fst.blocks} blocks, {fst.refills} refills, {fst.compactions} compactions, "
+        f"{fst.handoffs} members handed to the single-LP engine ({fst.handoff_seconds:.1f}s), "
         f"host set-up {fst.prepare_seconds:.1f}s, host waiting for blocks {fst.wait_seconds:.1f}s")
     ok = st == 0
     train_iters = int(its.sum())
@@ -635,6 +636,8 @@ def warmstart_workload(args):
         "training": {"solves": N, "optimal": int(ok.sum()), "seconds": round(t_train, 3),
                      "family_engine": {"blocks": int(fst.blocks), "refills": int(fst.refills),
                                        "compactions": int(fst.compactions),
+                                       "handoffs": int(fst.handoffs),
+                                       "handoff_s": round(fst.handoff_seconds, 2),
                                        "host_setup_s": round(fst.prepare_seconds, 2),
                                        "host_wait_s": round(fst.wait_seconds, 2)},
                      "member_iterations": train_iters, "iterations_median": float(np.median(its)),

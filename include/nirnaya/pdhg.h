@@ -234,8 +234,11 @@ NIR_API void nir_batch_destroy(nir_batch* b);
  * status and objective. */
 typedef struct nir_family nir_family;
 typedef struct {
-    int32_t lanes; /* members iterated together, a multiple of 64 (default 1024) */
-    int32_t reserved;
+    int32_t lanes;   /* members iterated together, a multiple of 64 (default 1024) */
+    int32_t handoff; /* once no member waits for a lane and at most this many still run
+                        (default 8; 0 = never), each of them finishes alone on the
+                        single-LP engine, continued from its lane's state bit for bit:
+                        a nearly empty 64-lane group iterates far slower per member */
 } nir_family_opts;
 typedef struct {
     int32_t lanes;
@@ -249,6 +252,8 @@ typedef struct {
     int64_t group_blocks;     /* evaluation blocks x groups launched */
     double prepare_seconds;   /* host: member set-up (fill callback + scaling) */
     double wait_seconds;      /* host waiting for blocks to finish */
+    int64_t handoffs;         /* members finished on the single-LP engine */
+    double handoff_seconds;   /* time spent in those single-LP runs */
 } nir_family_stats;
 /* Supplies member `index`: cost (ncol), rowlo and rowup (nrow), and an
  * optional warm start x0 (ncol) / y0 (nrow) flagged in *warm (bit 0: x0,

@@ -36,6 +36,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -490,6 +491,42 @@ __global__ __launch_bounds__(256) void k_fam_gather(View V, int count, const int
     if (t0 == 0) ost[f] = V.st[li];
 }
 
+// Hand-off: the whole state of lane li into contiguous vectors, x, xbar, xa,
+// xp, c (n each) then y, ya, yp, rl, ru (m each); the lane is left empty.
+__global__ __launch_bounds__(256) void k_fam_extract(View V, int li, double* __restrict__ out) {
+    const int g = li / kL, q = li % kL;
+    const size_t bx = static_cast<size_t>(g) * V.n * kL + q, by = static_cast<size_t>(g) * V.m * kL + q;
+    const size_t n = V.n, m = V.m;
+    const int stride = gridDim.x * blockDim.x;
+    const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int j = t0; j < V.n; j += stride) {
+        const size_t e = bx + static_cast<size_t>(j) * kL;
+        out[j] = V.x[e];
+        out[n + j] = V.xbar[e];
+        out[2 * n + j] = V.xa[e];
+        out[3 * n + j] = V.xp[e];
+        out[4 * n + j] = V.c[e];
+    }
+    double* oy = out + 5 * n;
+    for (int i = t0; i < V.m; i += stride) {
+        const size_t e = by + static_cast<size_t>(i) * kL;
+        oy[i] = V.y[e];
+        oy[m + i] = V.ya[e];
+        oy[2 * m + i] = V.yp[e];
+        oy[3 * m + i] = V.rl[e];
+        oy[4 * m + i] = V.ru[e];
+    }
+}
+
+// Marks lane li empty (after a hand-off).
+__global__ void k_fam_release(View V, int li) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        V.ldone[li] = 1;
+        V.lrestart[li] = 0;
+        V.lfresh[li] = 0;
+    }
+}
+
 // Stop request: every running lane ends with status 4 (time limit).
 __global__ void k_fam_stop(View V, int lanes) {
     const int li = blockIdx.x * blockDim.x + threadIdx.x;
@@ -657,6 +694,7 @@ NIR_API void nir_family_opts_default(nir_family_opts* o) {
     if (!o) return;
     std::memset(o, 0, sizeof(*o));
     o->lanes = 1024;
+    o->handoff = 8;
 }
 
 NIR_API int nir_family_create(nir_ctx* ctx, const nir_lp* base, const nir_params* p, const nir_family_opts* o,
@@ -1035,6 +1073,86 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
             ++F->stats.compactions;
         };
         bool stopped = false;
+        // Once no member waits for a lane and at most opts.handoff still run,
+        // each continues alone on the single-LP engine from its lane's state
+        // (vectors and PdlpState copied on the device between blocks, where
+        // both engines stand at the same point of the same iteration), so its
+        // result is the one it would have reached in the family. A lone
+        // member in a 64-lane group walks its rows one lane at a time; the
+        // single-LP engine spreads each row's nonzeros over a workgroup.
+        auto handoff = [&]() -> bool {
+            process(pending);
+            const long long before = fetched;
+            const auto th = clock::now();
+            DevArray<double> ext;
+            ext.alloc(5 * static_cast<size_t>(n) + 5 * static_cast<size_t>(m));
+            std::vector<double> cs(n), xs(n), ys(m);
+            const int gx = std::max(1, std::min(256, (std::max(n, m) + 255) / 256));
+            for (int l = 0; l < F->lanes; ++l) {
+                if (lane_inst[l] < 0) continue;
+                // A stop request ends the remaining lanes in the family (the
+                // main loop's stop path).
+                if (F->prm.should_stop && F->prm.should_stop(F->prm.stop_user)) break;
+                k_fam_extract<<<gx, 256, 0, F->s>>>(F->view(), l, ext.p);
+                HIP_CHECK(hipGetLastError());
+                HIP_CHECK(hipMemcpyAsync(cs.data(), ext.p + 4 * static_cast<size_t>(n), sizeof(double) * n,
+                                         hipMemcpyDeviceToHost, F->s));
+                HIP_CHECK(hipStreamSynchronize(F->s));
+                PdlpAdopt a;
+                a.m = m;
+                a.n = n;
+                a.nnz = F->nnz;
+                a.A = &F->A;
+                a.AT = &F->AT;
+                a.R = F->dR.p;
+                a.C = F->dC.p;
+                a.lo = F->lo.p;
+                a.up = F->up.p;
+                const double* ex = ext.p;
+                const double* ey = ext.p + 5 * static_cast<size_t>(n);
+                a.x = ex;
+                a.xbar = ex + n;
+                a.xa = ex + 2 * static_cast<size_t>(n);
+                a.xp = ex + 3 * static_cast<size_t>(n);
+                a.c = ex + 4 * static_cast<size_t>(n);
+                a.y = ey;
+                a.ya = ey + m;
+                a.yp = ey + 2 * static_cast<size_t>(m);
+                a.rl = ey + 3 * static_cast<size_t>(m);
+                a.ru = ey + 4 * static_cast<size_t>(m);
+                a.st = F->st.p + l;
+                a.h_c.resize(n);
+                for (int j = 0; j < n; ++j) a.h_c[j] = cs[j] / F->C[j];   // caller units (C: powers of two)
+                a.offset = F->offset;
+                nir_info info;
+                {
+                    std::unique_ptr<Pdlp, PdlpDeleter> P(pdlp_create());
+                    pdlp_adopt(*P, F->s, F->prm, a);
+                    pdlp_run(*P, F->s, F->prm, std::numeric_limits<long long>::max() / 4);
+                    pdlp_result(*P, F->s, xs.data(), ys.data(), &info);
+                }
+                info.setup_seconds = F->stats.setup_seconds;
+                info.solve_seconds = 0.0;
+                if (info.status == 4) stopped = true;
+                k_fam_release<<<1, 64, 0, F->s>>>(F->view(), l);
+                HIP_CHECK(hipGetLastError());
+                HIP_CHECK(hipStreamSynchronize(F->s));
+                const long long idx = lane_inst[l];
+                lane_inst[l] = -1;
+                free_lanes.push_back(l);
+                --active;
+                ++fetched;
+                ++finished;
+                ++F->stats.handoffs;
+                F->stats.total_iterations += info.iterations;
+                if (done(user, idx, xs.data(), ys.data(), &info) != 0)
+                    throw std::runtime_error("family done callback failed for instance " + std::to_string(idx));
+            }
+            std::sort(free_lanes.begin(), free_lanes.end(), std::greater<int>());
+            update_groups();
+            F->stats.handoff_seconds += secs(th);
+            return fetched > before;
+        };
         // Every member ends by its iteration cap, so some lane must finish
         // within cap / eval_freq blocks of the last one that did.
         const long long drain_bound = std::max<long long>(1, F->prm.max_iters) / std::max(1, F->prm.eval_freq) + 8;
@@ -1064,6 +1182,8 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
                 stopped = true;
                 break;
             }
+            if (F->opts.handoff > 0 && next >= count && ready.empty() && active <= F->opts.handoff && handoff())
+                continue;
             compact();
             run_block(F);
             ++F->stats.blocks;

@@ -458,6 +458,9 @@ struct Buf {
     void down(T* dst, size_t count, hipStream_t s) const {
         if (count) PD_CHECK(hipMemcpyAsync(dst, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
     }
+    void from_device(const T* src, size_t count, hipStream_t s) {
+        if (count) PD_CHECK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyDeviceToDevice, s));
+    }
 };
 
 template <typename T>
@@ -887,6 +890,24 @@ void pdlp_prepare_instance(const nir_lp* lp, const nir_params& prm, const std::v
     h.watch_stall = mixed ? 1 : 0;
 }
 
+// Poll slots, events, evaluation partials and certificate buffers.
+static void alloc_work(Pdlp& P) {
+    const int n = P.n, m = P.m;
+    const CsrShape& SA = P.A->shape;
+    const CsrShape& ST = P.AT->shape;
+    PD_CHECK(hipHostMalloc(reinterpret_cast<void**>(&P.h_st), 2 * sizeof(PdlpState), hipHostMallocDefault));
+    std::memset(P.h_st, 0, 2 * sizeof(PdlpState));
+    for (auto& e : P.ev) PD_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    P.part_c.alloc(static_cast<size_t>(5) * ST.nblk[ST.nseg - 1]);
+    P.part_r.alloc(static_cast<size_t>(6) * SA.nblk[SA.nseg - 1]);
+    P.qx.alloc(n);
+    P.qy.alloc(m);
+    P.rx.alloc(n);
+    P.ry.alloc(m);
+    P.cert_c.alloc(static_cast<size_t>(6) * std::max(1, (n + kThreads - 1) / kThreads));
+    P.cert_r.alloc(static_cast<size_t>(6) * std::max(1, (m + kThreads - 1) / kThreads));
+}
+
 void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm, DevMatrix* Am, DevMatrix* ATm) {
     auto t0 = std::chrono::steady_clock::now();
     P.n = lp->ncol;
@@ -904,8 +925,6 @@ void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm,
     P.stop_user = prm.stop_user;
     P.h_c.assign(lp->cost, lp->cost + P.n);
     const int n = P.n, m = P.m;
-    const CsrShape& SA = Am->shape;
-    const CsrShape& ST = ATm->shape;
     double* a_val = Am->val.p;
     double* at_val = ATm->val.p;
 
@@ -930,17 +949,7 @@ void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm,
     const PdlpState h = I.st;
     P.st.alloc(1);
     P.st.up(&h, 1, s);
-    PD_CHECK(hipHostMalloc(reinterpret_cast<void**>(&P.h_st), 2 * sizeof(PdlpState), hipHostMallocDefault));
-    std::memset(P.h_st, 0, 2 * sizeof(PdlpState));
-    for (auto& e : P.ev) PD_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    P.part_c.alloc(static_cast<size_t>(5) * ST.nblk[ST.nseg - 1]);
-    P.part_r.alloc(static_cast<size_t>(6) * SA.nblk[SA.nseg - 1]);
-    P.qx.alloc(n);
-    P.qy.alloc(m);
-    P.rx.alloc(n);
-    P.ry.alloc(m);
-    P.cert_c.alloc(static_cast<size_t>(6) * std::max(1, (n + kThreads - 1) / kThreads));
-    P.cert_r.alloc(static_cast<size_t>(6) * std::max(1, (m + kThreads - 1) / kThreads));
+    alloc_work(P);
 
     if (P.mixed) {
         P.a32.val.alloc(P.nnz);
@@ -972,6 +981,56 @@ void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm,
                 P.status = h2.status;
             }
         }
+    }
+    P.setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void pdlp_adopt(Pdlp& P, hipStream_t s, const nir_params& prm, const PdlpAdopt& a) {
+    auto t0 = std::chrono::steady_clock::now();
+    P.n = a.n;
+    P.m = a.m;
+    P.nnz = a.nnz;
+    P.A = a.A;
+    P.AT = a.AT;
+    P.eval_freq = std::max(1, prm.eval_freq);
+    P.use_graph = prm.use_graph != 0;
+    P.tol = prm.tol > 0 ? prm.tol : 1e-6;
+    P.mixed = false;
+    P.polish_tol = prm.polish_tol;
+    P.offset = a.offset;
+    P.should_stop = prm.should_stop;
+    P.stop_user = prm.stop_user;
+    P.h_c = a.h_c;
+    const int n = P.n, m = P.m;
+    P.R.alloc(m);
+    P.C.alloc(n);
+    P.R.from_device(a.R, m, s);
+    P.C.from_device(a.C, n, s);
+    // No unscaling vectors on this path: the KKT units are R and C.
+    P.Rtot.alloc(m);
+    P.Ctot.alloc(n);
+    P.Rtot.from_device(a.R, m, s);
+    P.Ctot.from_device(a.C, n, s);
+    P.d.alloc(n, m);
+    P.d.c.from_device(a.c, n, s);
+    P.d.lo.from_device(a.lo, n, s);
+    P.d.up.from_device(a.up, n, s);
+    P.d.rl.from_device(a.rl, m, s);
+    P.d.ru.from_device(a.ru, m, s);
+    P.d.x.from_device(a.x, n, s);
+    P.d.xbar.from_device(a.xbar, n, s);
+    P.d.xa.from_device(a.xa, n, s);
+    P.d.xp.from_device(a.xp, n, s);
+    P.d.y.from_device(a.y, m, s);
+    P.d.ya.from_device(a.ya, m, s);
+    P.d.yp.from_device(a.yp, m, s);
+    P.st.alloc(1);
+    P.st.from_device(a.st, 1, s);
+    alloc_work(P);
+    const PdlpState h = read_state(P, s);
+    if (h.done) {
+        P.finished = true;
+        P.status = sov_status(h);
     }
     P.setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }

@@ -64,5 +64,24 @@ struct PdlpInstance {
 void pdlp_prepare_instance(const nir_lp* lp, const nir_params& prm, const std::vector<double>& R,
                            const std::vector<double>& C, double eta, bool mixed, PdlpInstance& out);
 
+// A run continued on this engine from a state made elsewhere (a lane of the
+// family engine, between evaluation blocks): the scaled matrices, scaling and
+// column bounds are shared, the member's vectors and state are copied on the
+// device. FP64 only; the next pdlp_run starts with the next block.
+struct PdlpAdopt {
+    int m = 0, n = 0;
+    long long nnz = 0;
+    DevMatrix* A = nullptr;
+    DevMatrix* AT = nullptr;
+    const double *R = nullptr, *C = nullptr, *lo = nullptr, *up = nullptr;   // shared, device
+    const double *c = nullptr, *rl = nullptr, *ru = nullptr;                  // the member's, device
+    const double *x = nullptr, *xbar = nullptr, *xa = nullptr, *xp = nullptr;
+    const double *y = nullptr, *ya = nullptr, *yp = nullptr;
+    const PdlpState* st = nullptr;   // device
+    std::vector<double> h_c;         // caller-unit costs (objective of the result)
+    double offset = 0.0;
+};
+void pdlp_adopt(Pdlp& P, hipStream_t s, const nir_params& prm, const PdlpAdopt& a);
+
 }  // namespace gpu
 }  // namespace nir

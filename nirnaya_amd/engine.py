@@ -105,14 +105,15 @@ _FAMILY_DONE_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, _dp, _dp, C.POINTE
 
 
 class NirFamilyOpts(C.Structure):
-    _fields_ = [("lanes", C.c_int32), ("reserved", C.c_int32)]
+    _fields_ = [("lanes", C.c_int32), ("handoff", C.c_int32)]
 
 
 class NirFamilyStats(C.Structure):
     _fields_ = [("lanes", C.c_int32), ("stopped", C.c_int32), ("instances", C.c_int64), ("blocks", C.c_int64),
                 ("refills", C.c_int64), ("total_iterations", C.c_int64), ("setup_seconds", C.c_double),
                 ("solve_seconds", C.c_double), ("compactions", C.c_int64), ("group_blocks", C.c_int64),
-                ("prepare_seconds", C.c_double), ("wait_seconds", C.c_double)]
+                ("prepare_seconds", C.c_double), ("wait_seconds", C.c_double), ("handoffs", C.c_int64),
+                ("handoff_seconds", C.c_double)]
 
 
 class NirPresolveInfo(C.Structure):
@@ -488,8 +489,10 @@ class Engine:
               workgroups: int = 0, x0s=None, y0s=None) -> "Batch":
         return Batch(self, lps, params, threads, quantum, rounds_per_graph, workgroups, x0s, y0s)
 
-    def family(self, base: LP, params: NirParams, lanes: int = 1024) -> "Family":
-        return Family(self, base, params, lanes)
+    def family(self, base: LP, params: NirParams, lanes: int = 1024, handoff: Optional[int] = None) -> "Family":
+        """handoff: at most this many members still running, with none waiting,
+        finish on the single-LP engine (None = the engine's default, 0 = never)."""
+        return Family(self, base, params, lanes, handoff)
 
     def session(self, lp: LP, params: Optional[NirParams] = None, x0=None, y0=None) -> "Session":
         return Session(self, lp, params, x0, y0)
@@ -647,7 +650,7 @@ class Family:
     """Members of a parametric family (shared A and column bounds, own costs
     and row bounds) streamed through the family engine (nir_family_*)."""
 
-    def __init__(self, eng: Engine, base: LP, params: NirParams, lanes: int = 1024):
+    def __init__(self, eng: Engine, base: LP, params: NirParams, lanes: int = 1024, handoff: Optional[int] = None):
         self.eng, self.lib, self.base = eng, eng.lib, base
         eng._adopt(self)
         self.params = params
@@ -655,6 +658,8 @@ class Family:
         opts = NirFamilyOpts()
         self.lib.nir_family_opts_default(C.byref(opts))
         opts.lanes = lanes
+        if handoff is not None:
+            opts.handoff = handoff
         h = C.c_void_p()
         if self.lib.nir_family_create(eng.h, C.byref(self._s), C.byref(params), C.byref(opts), C.byref(h)) != 0:
             raise EngineError("nir_family_create: " + eng.error())

@@ -136,6 +136,30 @@ def test_family_device_starts_equal_host_starts(engine):
         assert same(dev[i], engine.solve(fam.instance(th[i]), p7, x0=X0[i], y0=Y0[i])), f"member {i}"
 
 
+def test_family_handoff_is_bit_identical(engine):
+    """The last members (at most `handoff` running, none waiting) finish on
+    the single-LP engine from their lanes' state: the results are the same
+    bits as with the hand-off switched off, and as the single-LP engine's."""
+    fam = ParametricFamily(20264, 400, 1000, 5)
+    th = fam.thetas(40, 17)
+    params = default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-6)
+    out = {}
+    for h in (0, 8, 64):
+        F = engine.family(fam.base, params, lanes=64, handoff=h)
+        try:
+            out[h] = (F.solve(len(th), members(fam, th)), F.stats())
+        finally:
+            F.close()
+    # handoff 64 >= 40 members: every member is handed over after its start-point evaluation
+    assert out[0][1].handoffs == 0 and 1 <= out[8][1].handoffs <= 8 and out[64][1].handoffs == len(th)
+    for i in range(len(th)):
+        assert same(out[8][0][i], out[0][0][i]) and same(out[64][0][i], out[0][0][i]), f"member {i}"
+    for i in (0, 39):
+        assert same(out[64][0][i], engine.solve(fam.instance(th[i]), params)), f"member {i}"
+    print(f"hand-off: {out[8][1].handoffs} members in {out[8][1].handoff_seconds:.2f}s; all {len(th)} in "
+          f"{out[64][1].handoff_seconds:.2f}s; family alone {out[0][1].solve_seconds:.2f}s")
+
+
 def test_family_certifies_infeasible_members(engine):
     """Members of an infeasible base (costs perturbed) end with the single-LP
     engine's certificate status and iterate."""
