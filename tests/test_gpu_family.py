@@ -160,6 +160,34 @@ def test_family_handoff_is_bit_identical(engine):
           f"{out[64][1].handoff_seconds:.2f}s; family alone {out[0][1].solve_seconds:.2f}s")
 
 
+def test_family_error_return_then_destroy(engine):
+    """A result callback that fails mid-solve (while the next block runs on
+    the device) makes nir_family_solve return an error; destroying the family
+    right after must wait for the work in flight, and the next family on the
+    same context solves normally."""
+    fam = ParametricFamily(20264, 300, 700, 5)
+    th = fam.thetas(150, 21)
+    params = default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-6)
+    seen = []
+
+    def fail_on_third(i, x, y, info):
+        seen.append(i)
+        if len(seen) == 3:
+            raise RuntimeError("callback failure forced by the test")
+
+    F = engine.family(fam.base, params, lanes=64)
+    with pytest.raises(RuntimeError, match="forced by the test"):
+        F.solve(len(th), members(fam, th), on_result=fail_on_third)
+    F.close()
+    F = engine.family(fam.base, params, lanes=64)
+    try:
+        res = F.solve(4, members(fam, th))
+    finally:
+        F.close()
+    for i in range(4):
+        assert same(res[i], engine.solve(fam.instance(th[i]), params)), f"member {i}"
+
+
 def test_family_certifies_infeasible_members(engine):
     """Members of an infeasible base (costs perturbed) end with the single-LP
     engine's certificate status and iterate."""
