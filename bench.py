@@ -517,8 +517,9 @@ def warmstart_workload(args):
     t_all = time.perf_counter()
     fam = ParametricFamily(20264, m, n, 5)
     eng = Engine(0)
-    params = default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-6)
-    tparams = default_params(mode=MODE_PDLP, max_iters=400000, tol=args.ws_train_tol)
+    prec = 1 if args.precision == "mixed" else 0
+    params = default_params(mode=MODE_PDLP, precision=prec, max_iters=400000, tol=1e-6)
+    tparams = default_params(mode=MODE_PDLP, precision=prec, max_iters=400000, tol=args.ws_train_tol)
     th_train = fam.thetas(args.ws_train, 1)
     th_test = fam.thetas(args.ws_test, 2)
     N = len(th_train)
@@ -535,7 +536,7 @@ def warmstart_workload(args):
             log(f"[bench] training solves: {count[0]}/{N} in {time.perf_counter() - t0:.0f}s")
     _, t_train, fst = family_solve(eng, fam, th_train, tparams, on_result=keep, lanes=1024, with_stats=True)
     log(f"[bench] training family engine: {fst.blocks} blocks, {fst.refills} refills, {fst.compactions} compactions, "
-        f"{fst.handoffs} members handed to the single-LP engine ({fst.handoff_seconds:.1f}s), "
+        f"{fst.handoffs} members handed to the single-LP engine ({fst.handoff_seconds:.1f}s), {fst.tails} FP64 tails, "
         f"host set-up {fst.prepare_seconds:.1f}s, host waiting for blocks {fst.wait_seconds:.1f}s")
     ok = st == 0
     train_iters = int(its.sum())
@@ -615,8 +616,12 @@ def warmstart_workload(args):
         r = eng.solve(fam.instance(th_test[i]), plain)
         ts += time.perf_counter() - t0
         si += r.iterations
-        assert r.iterations == ci[i], (i, r.iterations, ci[i])
-    plain_total = ts / si * ci.sum()
+        if not prec:
+            assert r.iterations == ci[i], (i, r.iterations, ci[i])
+    # FP64 family: the sample's time per iteration times the exact counts of
+    # all held-out LPs (the same in both engines); mixed family: the sample's
+    # mean time per LP times the number of LPs.
+    plain_total = ts / si * ci.sum() if not prec else ts / len(samp) * T
     ws.net.close()
     eng.close()
     nnz = fam.base.nnz
@@ -624,19 +629,22 @@ def warmstart_workload(args):
     # primal x, anchor, cost read + x, xbar written, y gathered once; dual y,
     # anchor, row bounds read + y written, xbar gathered once; the matrix and
     # the column bounds shared by the 64 members of a wave.
-    b_member = 8 * (5 * n + m) + 8 * (5 * m + n) + (2 * nnz * 12 + 4 * (m + n + 2) + 16 * n) / 64
+    if prec:
+        b_member = 4 * (6 * n + m) + 4 * (5 * m + n) + (2 * nnz * 8 + 4 * (m + n + 2)) / 64
+    else:
+        b_member = 8 * (5 * n + m) + 8 * (5 * m + n) + (2 * nnz * 12 + 4 * (m + n + 2) + 16 * n) / 64
     us_member = t_train / max(1, train_iters) * 1e6
     line = {
         "metric": "time-to-1e-6 relative KKT on held-out parametric LPs, learned warm start",
         "value": round(t_warm + t_pred + t_fallback, 3), "unit": "s", "n_gpus": 1, "higher_is_better": False,
-        "dtype": "f64", "data": "synthetic",
+        "dtype": "f32 (iterations) + f64 (evaluations)" if prec else "f64", "data": "synthetic",
         "config": {"workload": f"parametric equality LP family m={m} n={n} ({nnz} nonzeros), theta in R^64, "
                                f"b +-10%, c +-5%, {N} training solves to {args.ws_train_tol:g}, {T} held-out to 1e-6",
                    "pod_rank_x": pod_x.rank, "pod_rank_y": pod_y.rank, "mlp": "64-512-512-512 bf16 GELU"},
         "training": {"solves": N, "optimal": int(ok.sum()), "seconds": round(t_train, 3),
                      "family_engine": {"blocks": int(fst.blocks), "refills": int(fst.refills),
                                        "compactions": int(fst.compactions),
-                                       "handoffs": int(fst.handoffs),
+                                       "handoffs": int(fst.handoffs), "fp64_tails": int(fst.tails),
                                        "handoff_s": round(fst.handoff_seconds, 2),
                                        "host_setup_s": round(fst.prepare_seconds, 2),
                                        "host_wait_s": round(fst.wait_seconds, 2)},
@@ -660,8 +668,11 @@ def warmstart_workload(args):
                    "whole_workload": round(time.perf_counter() - t_all, 1)},
         "speedup_warm_family_vs_plain_fp64": round(plain_total / (t_warm + t_pred + t_fallback), 2),
         "speedup_cold_family_vs_plain_fp64": round(plain_total / t_cold, 2),
-        "plain_sample": f"{len(samp)} held-out LPs solved one at a time by the single-LP engine in {ts:.2f}s "
-                        f"({si} iterations, each equal to its family count); scaled by all {T} held-out counts",
+        "plain_sample": (f"{len(samp)} held-out LPs solved one at a time by the single-LP engine in {ts:.2f}s "
+                         f"({si} iterations, each equal to its family count); scaled by all {T} held-out counts")
+                        if not prec else
+                        (f"{len(samp)} held-out LPs solved one at a time by the FP64 single-LP engine in {ts:.2f}s; "
+                         f"mean time per LP x {T}"),
         "mlp_train_tflops": round(flops / (ms * 1e-3) / 1e12, 3),
     }
     print(json.dumps(line), flush=True)
@@ -683,17 +694,27 @@ def family_workload(args):
         ms = eng.calibrate_copy(256 << 20)
         log(f"[bench] calibration copy of 256 MiB: {ms:.3f} ms")
     th = fam.thetas(args.fam_lanes, 1)
-    p = default_params(mode=MODE_PDLP, max_iters=args.steps, tol=1e-12)
+    p = default_params(mode=MODE_PDLP, precision=1 if args.precision == "mixed" else 0, max_iters=args.steps,
+                       tol=1e-12)
     its = []
     _, dt = family_solve(eng, fam, th, p, lanes=args.fam_lanes, on_result=lambda i, x, y, info: its.append(info.iterations))
     eng.close()
     total = int(sum(its))
     nnz = fam.base.nnz
-    b_member = 8 * (5 * n + m) + 8 * (5 * m + n) + (2 * nnz * 12 + 4 * (m + n + 2) + 16 * n) / 64
+    if args.precision == "mixed":
+        # FP32 phase: primal reads the delta, re-based cost and bounds, writes
+        # the delta and the reflected point, gathers the dual delta; dual
+        # reads the delta, re-based row bounds and -Y / sigma, writes the
+        # delta, gathers the reflected point; FP32 matrix values shared.
+        b_member = 4 * (6 * n + m) + 4 * (5 * m + n) + (2 * nnz * 8 + 4 * (m + n + 2)) / 64
+    else:
+        b_member = 8 * (5 * n + m) + 8 * (5 * m + n) + (2 * nnz * 12 + 4 * (m + n + 2) + 16 * n) / 64
     us = dt / max(1, total) * 1e6
-    line = {"metric": "parametric-family PDHG member-iterations/s (family engine, FP64)",
+    mixed = args.precision == "mixed"
+    line = {"metric": "parametric-family PDHG member-iterations/s (family engine, "
+                      + ("mixed precision, FP32 phase)" if mixed else "FP64)"),
             "value": round(total / dt, 1), "unit": "member-iterations/s", "n_gpus": 1, "higher_is_better": True,
-            "dtype": "f64", "data": "synthetic",
+            "dtype": "f32 (iterations) + f64 (evaluations)" if mixed else "f64", "data": "synthetic",
             "config": {"workload": f"parametric equality LP family m={m} n={n} ({nnz} nonzeros), "
                                    f"{args.fam_lanes} members x {args.steps} iterations"},
             "us_per_member_iteration": round(us, 4), "algorithmic_bytes_per_member_iteration": b_member,
@@ -764,6 +785,8 @@ def main():
     ap.add_argument("--ws-test", type=int, default=1000)
     ap.add_argument("--ws-epochs", type=int, default=300)
     ap.add_argument("--fam-lanes", type=int, default=1024, help="family: members iterated together")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
+                    help="family and warmstart workloads: the family engine's precision")
     ap.add_argument("--ws-plain-sample", type=int, default=32,
                     help="held-out LPs solved one at a time by the single-LP engine for the plain baseline")
     args = ap.parse_args()

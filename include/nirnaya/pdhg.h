@@ -230,8 +230,8 @@ NIR_API void nir_batch_destroy(nir_batch* b);
  * `count` members are streamed through a pool of `lanes` lanes (finished
  * members are handed out and their lanes refilled between evaluation
  * blocks). Every member gets exactly what nir_pdhg_solve returns for it
- * alone (pdlp mode, FP64): the same x, y, iteration and restart counts,
- * status and objective. */
+ * alone (pdlp mode, FP64 or mixed precision): the same x, y, iteration and
+ * restart counts, status and objective. */
 typedef struct nir_family nir_family;
 typedef struct {
     int32_t lanes;   /* members iterated together, a multiple of 64 (default 1024) */
@@ -254,6 +254,9 @@ typedef struct {
     double wait_seconds;      /* host waiting for blocks to finish */
     int64_t handoffs;         /* members finished on the single-LP engine */
     double handoff_seconds;   /* time spent in those single-LP runs */
+    int64_t tails;            /* mixed precision: members whose FP32 phase ended needing
+                                 the FP64 tail (stalled, or a polish tolerance), which
+                                 they ran on the single-LP engine from their state */
 } nir_family_stats;
 /* Supplies member `index`: cost (ncol), rowlo and rowup (nrow), and an
  * optional warm start x0 (ncol) / y0 (nrow) flagged in *warm (bit 0: x0,
@@ -267,8 +270,8 @@ typedef int (*nir_family_fill_fn)(void* user, int64_t index, double* cost, doubl
 typedef int (*nir_family_done_fn)(void* user, int64_t index, const double* x, const double* y,
                                   const nir_info* info);
 NIR_API void nir_family_opts_default(nir_family_opts* o);
-/* Uploads and scales the shared matrix (pdlp mode FP64 parameters; warm
- * starts come per member through the fill callback). */
+/* Uploads and scales the shared matrix (pdlp mode parameters, precision FP64
+ * or mixed; warm starts come per member through the fill callback). */
 NIR_API int nir_family_create(nir_ctx* ctx, const nir_lp* base, const nir_params* p, const nir_family_opts* o,
                               nir_family** out);
 /* Solves members 0 .. count-1; should_stop (polled between blocks) ends the

@@ -1,7 +1,8 @@
 // Parametric-family engine: many LPs that share one constraint matrix A and
 // its column bounds and differ in costs and row bounds (the family
 // b(theta), c(theta) of a learned warm start), solved by the restarted PDHG
-// of pdlp mode in FP64, every instance on exactly the single-LP trajectory.
+// of pdlp mode in FP64 or in re-centred mixed precision, every instance on
+// exactly the single-LP trajectory of that precision.
 //
 // Layout: instances occupy lanes. A group of 64 lanes (one wave64) keeps its
 // 64 instances' vectors interleaved, element j of lane q at
@@ -25,6 +26,16 @@
 // (pdlp_prepare_instance), scattered into the lanes, and evaluated at their
 // start point (mode 1) before they iterate. Finished and empty lanes are
 // skipped by every kernel (whole waves return when all 64 lanes are idle).
+//
+// Mixed precision (the single-LP engine's FP32 phase, pdlp.hip): each lane
+// iterates on FP32 deltas around its FP64 anchor (xa / ya), with FP32
+// constants re-based on it (rc = c - A^T Y, [l - X, u - X], [rl - A X,
+// ru - A X], sa = -Y / sigma) and FP32 matrix values; evaluations and
+// decisions are FP64 (T(z) = anchor + delta, A^T T(z)_y = A^T Y + A^T dyp). At
+// a restart the anchor moves to T(z), A X and A^T Y are recomputed in FP64 and
+// the constants re-based. A lane whose FP32 phase ends needing the FP64 tail
+// (stalled, or a polish tolerance) continues on the single-LP engine from its
+// state, which switches to FP64 exactly as it does for one LP.
 //
 // The reference solves one LP per call with a CPU loop
 // (src/gpu/pdhg_solver.hpp:25-107, single-vector products
@@ -87,29 +98,39 @@ struct View {
     dd* part_r;   // [G][ebr][6][kL]
     dd* cert_c;   // [G][ebc][6][kL]
     dd* cert_r;   // [G][ebr][6][kL]
+    // mixed precision: FP32 deltas around the FP64 anchor xa / ya
+    const float* aval32;                  // FP32 copies of the scaled values
+    const float* tval32;
+    float *fx, *fxbar, *fxp, *rc, *dl, *du;   // [G][n][kL]
+    float *fy, *fyp, *drl, *dru, *sa;         // [G][m][kL]
+    double* HA;                               // A X of the anchor [G][m][kL]
+    double* GA;                               // A^T Y of the anchor [G][n][kL]
 };
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
 // Row r of a CSR matrix times the lane's interleaved operand v (already
-// offset to the lane): sum in ascending column order from 0.0, products
-// rounded before the add (no contraction), four operand reads in flight.
-__device__ __forceinline__ double row_sum(const int* __restrict__ rp, const int* __restrict__ col,
-                                          const double* __restrict__ val, const double* __restrict__ v, int r) {
+// offset to the lane): sum in ascending column order from 0.0 in the
+// precision T of the values, operand entries converted to T, products
+// rounded before the add (no contraction), four operand reads in flight -
+// the single-LP stream_rows sum for the same T and operand type.
+template <typename T, typename V>
+__device__ __forceinline__ T row_sum(const int* __restrict__ rp, const int* __restrict__ col,
+                                     const T* __restrict__ val, const V* __restrict__ v, int r) {
     const int p0 = rp[r], p1 = rp[r + 1];
-    double s = 0.0;
+    T s = T(0);
     int p = p0;
     for (; p + 4 <= p1; p += 4) {
         const int c0 = col[p], c1 = col[p + 1], c2 = col[p + 2], c3 = col[p + 3];
-        const double a0 = val[p], a1 = val[p + 1], a2 = val[p + 2], a3 = val[p + 3];
-        const double g0 = v[static_cast<size_t>(c0) * kL], g1 = v[static_cast<size_t>(c1) * kL];
-        const double g2 = v[static_cast<size_t>(c2) * kL], g3 = v[static_cast<size_t>(c3) * kL];
+        const T a0 = val[p], a1 = val[p + 1], a2 = val[p + 2], a3 = val[p + 3];
+        const T g0 = static_cast<T>(v[static_cast<size_t>(c0) * kL]), g1 = static_cast<T>(v[static_cast<size_t>(c1) * kL]);
+        const T g2 = static_cast<T>(v[static_cast<size_t>(c2) * kL]), g3 = static_cast<T>(v[static_cast<size_t>(c3) * kL]);
         s += a0 * g0;
         s += a1 * g1;
         s += a2 * g2;
         s += a3 * g3;
     }
-    for (; p < p1; ++p) s += val[p] * v[static_cast<size_t>(col[p]) * kL];
+    for (; p < p1; ++p) s += val[p] * static_cast<T>(v[static_cast<size_t>(col[p]) * kL]);
     return s;
 }
 
@@ -137,7 +158,7 @@ __global__ __launch_bounds__(256) void k_fam_primal(View V, int k_lo, int save) 
     const int r1 = min(r0 + kRowsPerWave, V.n);
     if (done) return;   // finished lanes issue no loads; their waves carry on for the others
     for (int r = r0; r < r1; ++r) {
-        const double gs = row_sum(V.trp, V.tcol, V.tval, yv, r);
+        const double gs = row_sum<double>(V.trp, V.tcol, V.tval, yv, r);
         const size_t e = xb + static_cast<size_t>(r) * kL;
         const PrimalOut<double> o = primal_point<double>(V.x[e], gs, V.c[e], V.lo[r], V.up[r], V.xa[e], tau, hw);
         V.xbar[e] = o.xb;
@@ -161,11 +182,59 @@ __global__ __launch_bounds__(256) void k_fam_dual(View V, int k_lo, int save) {
     const int r1 = min(r0 + kRowsPerWave, V.m);
     if (done) return;
     for (int r = r0; r < r1; ++r) {
-        const double h = row_sum(V.arp, V.acol, V.aval, xv, r);
+        const double h = row_sum<double>(V.arp, V.acol, V.aval, xv, r);
         const size_t e = yb + static_cast<size_t>(r) * kL;
         const DualOut<double> o = dual_point<double>(V.y[e], h, V.rl[e], V.ru[e], V.ya[e], sigma, hw);
         V.y[e] = o.next;
         if (save) V.yp[e] = o.yp;
+    }
+}
+
+// FP32 phase of mixed precision (pdlp.hip k_primal_pdlp<float>): deltas
+// around the anchor, re-based constants, a zero Halpern anchor.
+__global__ __launch_bounds__(256) void k_fam_primal32(View V, int k_lo, int save) {
+    const int g = blockIdx.y;
+    const int lane = threadIdx.x & (kL - 1);
+    const int li = g * kL + lane;
+    const bool done = V.ldone[li] != 0;
+    if (__all(done)) return;
+    const float tau = static_cast<float>(V.ltau[li]);
+    const Halpern<float> hw = halpern<float>(V.lkb[li] + k_lo);
+    const int r0 = (blockIdx.x * 4 + wave_id()) * kRowsPerWave;
+    const float* yv = V.fy + static_cast<size_t>(g) * V.m * kL + lane;
+    const size_t xb = static_cast<size_t>(g) * V.n * kL + lane;
+    const int r1 = min(r0 + kRowsPerWave, V.n);
+    if (done) return;
+    for (int r = r0; r < r1; ++r) {
+        const float gs = row_sum<float>(V.trp, V.tcol, V.tval32, yv, r);
+        const size_t e = xb + static_cast<size_t>(r) * kL;
+        const PrimalOut<float> o = primal_point<float>(V.fx[e], gs, V.rc[e], V.dl[e], V.du[e], 0.0f, tau, hw);
+        V.fxbar[e] = o.xb;
+        V.fx[e] = o.next;
+        if (save) V.fxp[e] = o.cand;
+    }
+}
+
+// pdlp.hip k_dual_pdlp<float>: sa = -Y / sigma stands in the anchor's place.
+__global__ __launch_bounds__(256) void k_fam_dual32(View V, int k_lo, int save) {
+    const int g = blockIdx.y;
+    const int lane = threadIdx.x & (kL - 1);
+    const int li = g * kL + lane;
+    const bool done = V.ldone[li] != 0;
+    if (__all(done)) return;
+    const float sigma = static_cast<float>(V.lsigma[li]);
+    const Halpern<float> hw = halpern<float>(V.lkb[li] + k_lo);
+    const int r0 = (blockIdx.x * 4 + wave_id()) * kRowsPerWave;
+    const float* xv = V.fxbar + static_cast<size_t>(g) * V.n * kL + lane;
+    const size_t yb = static_cast<size_t>(g) * V.m * kL + lane;
+    const int r1 = min(r0 + kRowsPerWave, V.m);
+    if (done) return;
+    for (int r = r0; r < r1; ++r) {
+        const float h = row_sum<float>(V.arp, V.acol, V.aval32, xv, r);
+        const size_t e = yb + static_cast<size_t>(r) * kL;
+        const DualOut<float> o = dual_point_delta<float>(V.fy[e], h, V.drl[e], V.dru[e], V.sa[e], sigma, hw);
+        V.fy[e] = o.next;
+        if (save) V.fyp[e] = o.yp;
     }
 }
 
@@ -197,8 +266,10 @@ __device__ __forceinline__ void waves_to_part(dd (&acc)[NQ], dd* sh, dd* part, i
 
 // Column evaluation terms (pdlp.hip k_eval_cols): A^T yp per lane. FRESH
 // (a template parameter, so each pass is straight-line code) selects the
-// lanes: refilled ones at their start point, or the running ones.
-template <int FRESH>
+// lanes: refilled ones at their start point, or the running ones. DELTA: the
+// FP32 phase, T(z) = anchor + delta and A^T yp = GA + A^T dyp (FP64 values,
+// FP32 operand converted).
+template <int FRESH, int DELTA>
 __global__ __launch_bounds__(256) void k_fam_eval_cols(View V) {
     __shared__ dd sh[4 * 5 * kL];
     const int g = blockIdx.y;
@@ -207,13 +278,19 @@ __global__ __launch_bounds__(256) void k_fam_eval_cols(View V) {
     const bool serve = eval_lane<FRESH>(V, li);
     if (__all(!serve)) return;
     dd acc[5] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
-    const double* yv = V.yp + static_cast<size_t>(g) * V.m * kL + lane;
+    const size_t yo = static_cast<size_t>(g) * V.m * kL + lane;
     const size_t xb = static_cast<size_t>(g) * V.n * kL + lane;
     for (int r = blockIdx.x * 4 + wave_id(); serve && r < V.n; r += gridDim.x * 4) {
-        const double gs = row_sum(V.trp, V.tcol, V.tval, yv, r);
         const size_t e = xb + static_cast<size_t>(r) * kL;
         dd t[5];
-        col_terms(gs, V.c[e], V.lo[r], V.up[r], V.cscale[r], V.xp[e], V.xa[e], t);
+        if constexpr (DELTA) {
+            const double gs = row_sum<double>(V.trp, V.tcol, V.tval, V.fyp + yo, r);
+            const double X = V.xa[e];
+            col_terms(V.GA[e] + gs, V.c[e], V.lo[r], V.up[r], V.cscale[r], X + static_cast<double>(V.fxp[e]), X, t);
+        } else {
+            const double gs = row_sum<double>(V.trp, V.tcol, V.tval, V.yp + yo, r);
+            col_terms(gs, V.c[e], V.lo[r], V.up[r], V.cscale[r], V.xp[e], V.xa[e], t);
+        }
 #pragma unroll
         for (int q = 0; q < 5; ++q) acc[q] = dd_add(acc[q], t[q]);
     }
@@ -221,7 +298,7 @@ __global__ __launch_bounds__(256) void k_fam_eval_cols(View V) {
 }
 
 // Row evaluation terms (pdlp.hip k_eval_rows): A xp per lane.
-template <int FRESH>
+template <int FRESH, int DELTA>
 __global__ __launch_bounds__(256) void k_fam_eval_rows(View V) {
     __shared__ dd sh[4 * 6 * kL];
     const int g = blockIdx.y;
@@ -230,13 +307,19 @@ __global__ __launch_bounds__(256) void k_fam_eval_rows(View V) {
     const bool serve = eval_lane<FRESH>(V, li);
     if (__all(!serve)) return;
     dd acc[6] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
-    const double* xv = V.xp + static_cast<size_t>(g) * V.n * kL + lane;
+    const size_t xo = static_cast<size_t>(g) * V.n * kL + lane;
     const size_t yb = static_cast<size_t>(g) * V.m * kL + lane;
     for (int r = blockIdx.x * 4 + wave_id(); serve && r < V.m; r += gridDim.x * 4) {
-        const double h = row_sum(V.arp, V.acol, V.aval, xv, r);
         const size_t e = yb + static_cast<size_t>(r) * kL;
         dd t[6];
-        row_terms(h, V.rl[e], V.ru[e], V.rscale[r], V.yp[e], V.ya[e], t);
+        if constexpr (DELTA) {
+            const double h = row_sum<double>(V.arp, V.acol, V.aval, V.fxp + xo, r);
+            const double Y = V.ya[e];
+            row_terms(V.HA[e] + h, V.rl[e], V.ru[e], V.rscale[r], Y + static_cast<double>(V.fyp[e]), Y, t);
+        } else {
+            const double h = row_sum<double>(V.arp, V.acol, V.aval, V.xp + xo, r);
+            row_terms(h, V.rl[e], V.ru[e], V.rscale[r], V.yp[e], V.ya[e], t);
+        }
 #pragma unroll
         for (int q = 0; q < 6; ++q) acc[q] = dd_add(acc[q], t[q]);
     }
@@ -274,7 +357,8 @@ __device__ __forceinline__ bool cert_lane(const View& V, int li) { return !V.ldo
 
 // Restart bookkeeping over the interleaved vectors; blockIdx.y = group.
 // STAGE 0: keep the old anchor (k_cert_save); 1: iterate and anchor := T(z)
-// (k_restart_copy); 2: the anchor step (k_cert_diff). The stage is a
+// (k_restart_copy); 2: the anchor step (k_cert_diff); 3: the FP32 phase's
+// anchor moves to T(z) = anchor + delta (k_anchor_move). The stage is a
 // template parameter, not a kernel argument: one straight-line kernel per
 // stage.
 template <int STAGE>
@@ -295,8 +379,10 @@ __global__ __launch_bounds__(256) void k_fam_restart(View V) {
             const double v = V.xp[e];
             V.x[e] = v;
             V.xa[e] = v;
-        } else {
+        } else if constexpr (STAGE == 2) {
             V.rx[e] = V.xa[e] - V.qx[e];
+        } else {
+            V.xa[e] = V.xa[e] + static_cast<double>(V.fxp[e]);
         }
     }
     for (size_t t = t0; t < ny; t += stride) {
@@ -307,9 +393,80 @@ __global__ __launch_bounds__(256) void k_fam_restart(View V) {
             const double v = V.yp[e];
             V.y[e] = v;
             V.ya[e] = v;
-        } else {
+        } else if constexpr (STAGE == 2) {
             V.ry[e] = V.ya[e] - V.qy[e];
+        } else {
+            V.ya[e] = V.ya[e] + static_cast<double>(V.fyp[e]);
         }
+    }
+}
+
+// Lanes re-centred: refilled lanes at their start point (FORCE, as the
+// single-LP set-up does) or lanes whose decision restarted.
+template <int FORCE>
+__device__ __forceinline__ bool recenter_lane(const View& V, int li) {
+    if constexpr (FORCE) return V.lfresh[li] != 0;
+    else return cert_lane(V, li);
+}
+
+// The anchor's products in FP64 (pdlp.hip k_spmv_on_restart): HA = A X over
+// the rows of A, GA = A^T Y over the rows of A^T (blockIdx.x below the first
+// `split` workgroups), 4 rows per wave.
+template <int FORCE>
+__global__ __launch_bounds__(256) void k_fam_anchor(View V, int split) {
+    const int g = blockIdx.y;
+    const int lane = threadIdx.x & (kL - 1);
+    const int li = g * kL + lane;
+    const bool serve = recenter_lane<FORCE>(V, li);
+    if (__all(!serve)) return;
+    if (!serve) return;
+    const bool rows_of_a = static_cast<int>(blockIdx.x) < split;
+    const int bx = rows_of_a ? blockIdx.x : blockIdx.x - split;
+    const int r0 = (bx * 4 + wave_id()) * kRowsPerWave;
+    if (rows_of_a) {
+        const double* xv = V.xa + static_cast<size_t>(g) * V.n * kL + lane;
+        const size_t yb = static_cast<size_t>(g) * V.m * kL + lane;
+        const int r1 = min(r0 + kRowsPerWave, V.m);
+        for (int r = r0; r < r1; ++r) V.HA[yb + static_cast<size_t>(r) * kL] = row_sum<double>(V.arp, V.acol, V.aval, xv, r);
+    } else {
+        const double* yv = V.ya + static_cast<size_t>(g) * V.m * kL + lane;
+        const size_t xb = static_cast<size_t>(g) * V.n * kL + lane;
+        const int r1 = min(r0 + kRowsPerWave, V.n);
+        for (int r = r0; r < r1; ++r) V.GA[xb + static_cast<size_t>(r) * kL] = row_sum<double>(V.trp, V.tcol, V.tval, yv, r);
+    }
+}
+
+// FP32 constants re-based on the anchor and zero deltas (pdlp.hip
+// k_recenter); sigma is the lane's step after its decision.
+template <int FORCE>
+__global__ __launch_bounds__(256) void k_fam_recenter(View V) {
+    const int g = blockIdx.y;
+    const int lane = threadIdx.x & (kL - 1);
+    const int li = g * kL + lane;
+    if (!recenter_lane<FORCE>(V, li)) return;
+    const double sigma = V.st[li].sigma;
+    const size_t nx = static_cast<size_t>(V.n) * kL, ny = static_cast<size_t>(V.m) * kL;
+    const size_t bx = static_cast<size_t>(g) * nx, by = static_cast<size_t>(g) * ny;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    const size_t t0 = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (size_t t = t0; t < nx; t += stride) {
+        const size_t e = bx + t;
+        const size_t j = t / kL;
+        const double X = V.xa[e];
+        V.rc[e] = static_cast<float>(V.c[e] - V.GA[e]);
+        V.dl[e] = static_cast<float>(V.lo[j] - X);
+        V.du[e] = static_cast<float>(V.up[j] - X);
+        V.fx[e] = 0.0f;
+        V.fxp[e] = 0.0f;
+    }
+    for (size_t t = t0; t < ny; t += stride) {
+        const size_t e = by + t;
+        const double H = V.HA[e];
+        V.drl[e] = static_cast<float>(V.rl[e] - H);
+        V.dru[e] = static_cast<float>(V.ru[e] - H);
+        V.sa[e] = static_cast<float>(-V.ya[e] / sigma);
+        V.fy[e] = 0.0f;
+        V.fyp[e] = 0.0f;
     }
 }
 
@@ -431,6 +588,7 @@ __global__ __launch_bounds__(256) void k_fam_scatter(View V, int count, const in
 
 // Compaction: lane src[f] moves to the free lane dst[f] (every vector an
 // iteration or evaluation block reads, and the state); src is left empty.
+template <int MIXED>
 __global__ __launch_bounds__(256) void k_fam_move(View V, int count, const int* __restrict__ src,
                                                   const int* __restrict__ dst) {
     const int f = blockIdx.y;
@@ -448,6 +606,15 @@ __global__ __launch_bounds__(256) void k_fam_move(View V, int count, const int* 
         V.xa[b] = V.xa[a];
         V.xp[b] = V.xp[a];
         V.c[b] = V.c[a];
+        if constexpr (MIXED) {
+            V.fx[b] = V.fx[a];
+            V.fxbar[b] = V.fxbar[a];
+            V.fxp[b] = V.fxp[a];
+            V.rc[b] = V.rc[a];
+            V.dl[b] = V.dl[a];
+            V.du[b] = V.du[a];
+            V.GA[b] = V.GA[a];
+        }
     }
     for (int i = t0; i < V.m; i += stride) {
         const size_t a = sy + static_cast<size_t>(i) * kL, b = dy + static_cast<size_t>(i) * kL;
@@ -456,6 +623,14 @@ __global__ __launch_bounds__(256) void k_fam_move(View V, int count, const int* 
         V.yp[b] = V.yp[a];
         V.rl[b] = V.rl[a];
         V.ru[b] = V.ru[a];
+        if constexpr (MIXED) {
+            V.fy[b] = V.fy[a];
+            V.fyp[b] = V.fyp[a];
+            V.drl[b] = V.drl[a];
+            V.dru[b] = V.dru[a];
+            V.sa[b] = V.sa[a];
+            V.HA[b] = V.HA[a];
+        }
     }
     if (t0 == 0) {
         V.st[ld] = V.st[ls];
@@ -471,7 +646,9 @@ __global__ __launch_bounds__(256) void k_fam_move(View V, int count, const int* 
     }
 }
 
-// Harvest: T(z) and the scaled cost of lane lanes[f] into slot f.
+// Harvest: T(z) and the scaled cost of lane lanes[f] into slot f (in the FP32
+// phase T(z) = anchor + delta, in FP64).
+template <int MIXED>
 __global__ __launch_bounds__(256) void k_fam_gather(View V, int count, const int* __restrict__ lanes,
                                                     double* __restrict__ ox, double* __restrict__ oc,
                                                     double* __restrict__ oy, PdlpState* __restrict__ ost) {
@@ -484,16 +661,23 @@ __global__ __launch_bounds__(256) void k_fam_gather(View V, int count, const int
     const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
     for (int j = t0; j < V.n; j += stride) {
         const size_t e = bx + static_cast<size_t>(j) * kL;
-        ox[static_cast<size_t>(f) * V.n + j] = V.xp[e];
+        ox[static_cast<size_t>(f) * V.n + j] = MIXED ? V.xa[e] + static_cast<double>(V.fxp[e]) : V.xp[e];
         oc[static_cast<size_t>(f) * V.n + j] = V.c[e];
     }
-    for (int i = t0; i < V.m; i += stride) oy[static_cast<size_t>(f) * V.m + i] = V.yp[by + static_cast<size_t>(i) * kL];
+    for (int i = t0; i < V.m; i += stride) {
+        const size_t e = by + static_cast<size_t>(i) * kL;
+        oy[static_cast<size_t>(f) * V.m + i] = MIXED ? V.ya[e] + static_cast<double>(V.fyp[e]) : V.yp[e];
+    }
     if (t0 == 0) ost[f] = V.st[li];
 }
 
 // Hand-off: the whole state of lane li into contiguous vectors, x, xbar, xa,
-// xp, c (n each) then y, ya, yp, rl, ru (m each); the lane is left empty.
-__global__ __launch_bounds__(256) void k_fam_extract(View V, int li, double* __restrict__ out) {
+// xp, c (n each) then y, ya, yp, rl, ru (m each); with MIXED also GA (n) and
+// HA (m) after those, and the FP32 vectors fx, fxbar, fxp, rc, dl, du (n
+// each) then fy, fyp, drl, dru, sa (m each) into out32.
+template <int MIXED>
+__global__ __launch_bounds__(256) void k_fam_extract(View V, int li, double* __restrict__ out,
+                                                     float* __restrict__ out32) {
     const int g = li / kL, q = li % kL;
     const size_t bx = static_cast<size_t>(g) * V.n * kL + q, by = static_cast<size_t>(g) * V.m * kL + q;
     const size_t n = V.n, m = V.m;
@@ -506,6 +690,15 @@ __global__ __launch_bounds__(256) void k_fam_extract(View V, int li, double* __r
         out[2 * n + j] = V.xa[e];
         out[3 * n + j] = V.xp[e];
         out[4 * n + j] = V.c[e];
+        if constexpr (MIXED) {
+            out[5 * n + 5 * m + j] = V.GA[e];
+            out32[j] = V.fx[e];
+            out32[n + j] = V.fxbar[e];
+            out32[2 * n + j] = V.fxp[e];
+            out32[3 * n + j] = V.rc[e];
+            out32[4 * n + j] = V.dl[e];
+            out32[5 * n + j] = V.du[e];
+        }
     }
     double* oy = out + 5 * n;
     for (int i = t0; i < V.m; i += stride) {
@@ -515,6 +708,15 @@ __global__ __launch_bounds__(256) void k_fam_extract(View V, int li, double* __r
         oy[2 * m + i] = V.yp[e];
         oy[3 * m + i] = V.rl[e];
         oy[4 * m + i] = V.ru[e];
+        if constexpr (MIXED) {
+            out[6 * n + 5 * m + i] = V.HA[e];
+            float* fy = out32 + 6 * n;
+            fy[i] = V.fy[e];
+            fy[m + i] = V.fyp[e];
+            fy[2 * m + i] = V.drl[e];
+            fy[3 * m + i] = V.dru[e];
+            fy[4 * m + i] = V.sa[e];
+        }
     }
 }
 
@@ -525,6 +727,13 @@ __global__ void k_fam_release(View V, int li) {
         V.lrestart[li] = 0;
         V.lfresh[li] = 0;
     }
+}
+
+// FP32 copies of the scaled matrix values (mixed precision).
+__global__ void k_fam_to_float(const double* __restrict__ in, float* __restrict__ out, long long n) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    for (long long i = t; i < n; i += stride) out[i] = static_cast<float>(in[i]);
 }
 
 // Stop request: every running lane ends with status 4 (time limit).
@@ -572,6 +781,10 @@ struct nir_family {
     DevArray<double> s_c, s_x, s_rl, s_ru, s_y;
     DevArray<PdlpState> s_st;
     DevArray<long long> s_src;   // [2][S]: device start row of x / y per slot, -1 = staged
+    // mixed precision
+    bool mixed = false;
+    DevArray<float> aval32, tval32, fx, fxbar, fxp, rc, dl, du, fy, fyp, drl, dru, sa;
+    DevArray<double> HA, GA;
     std::vector<int> h_done;
     // device warm starts (nir_family_set_device_starts)
     const nir_pod* start_x = nullptr;
@@ -619,6 +832,12 @@ struct nir_family {
         V.part_r = part_r.p;
         V.cert_c = cert_c.p;
         V.cert_r = cert_r.p;
+        V.aval32 = aval32.p;
+        V.tval32 = tval32.p;
+        V.fx = fx.p; V.fxbar = fxbar.p; V.fxp = fxp.p; V.rc = rc.p; V.dl = dl.p; V.du = du.p;
+        V.fy = fy.p; V.fyp = fyp.p; V.drl = drl.p; V.dru = dru.p; V.sa = sa.p;
+        V.HA = HA.p;
+        V.GA = GA.p;
         return V;
     }
 };
@@ -629,9 +848,27 @@ int iter_grid(int rows) { return std::max(1, (rows + 4 * kRowsPerWave - 1) / (4 
 
 template <int FRESH>
 void launch_eval(nir_family* F, const View& V) {
-    k_fam_eval_cols<FRESH><<<dim3(F->ebc, F->Gl), 256, 0, F->s>>>(V);
-    k_fam_eval_rows<FRESH><<<dim3(F->ebr, F->Gl), 256, 0, F->s>>>(V);
+    if (F->mixed) {
+        k_fam_eval_cols<FRESH, 1><<<dim3(F->ebc, F->Gl), 256, 0, F->s>>>(V);
+        k_fam_eval_rows<FRESH, 1><<<dim3(F->ebr, F->Gl), 256, 0, F->s>>>(V);
+    } else {
+        k_fam_eval_cols<FRESH, 0><<<dim3(F->ebc, F->Gl), 256, 0, F->s>>>(V);
+        k_fam_eval_rows<FRESH, 0><<<dim3(F->ebr, F->Gl), 256, 0, F->s>>>(V);
+    }
     k_fam_decide<FRESH><<<F->Gl, kL, 0, F->s>>>(V);
+}
+
+int restart_grid(const nir_family* F) { return std::max(1, std::min(1024, (std::max(F->n, F->m) * kL + 255) / 256)); }
+
+// Re-centring of the FP32 phase (pdlp.hip recenter): the anchor's products
+// in FP64, then the re-based constants and zero deltas. FORCE: the refilled
+// lanes at their start point; else the lanes whose decision restarted, after
+// their anchor moved.
+template <int FORCE>
+void launch_recenter(nir_family* F, const View& V) {
+    const int split = iter_grid(F->m);
+    k_fam_anchor<FORCE><<<dim3(split + iter_grid(F->n), F->Gl), 256, 0, F->s>>>(V, split);
+    k_fam_recenter<FORCE><<<dim3(restart_grid(F), F->Gl), 256, 0, F->s>>>(V);
 }
 
 // One evaluation block: eval_freq iterations, the evaluation and decision,
@@ -643,13 +880,23 @@ void enqueue_block(nir_family* F) {
     const dim3 gp(iter_grid(F->n), F->Gl), gd(iter_grid(F->m), F->Gl);
     for (int kl = 0; kl < ef; ++kl) {
         const int save = kl == ef - 1 ? 1 : 0;
-        k_fam_primal<<<gp, 256, 0, F->s>>>(V, kl, save);
-        k_fam_dual<<<gd, 256, 0, F->s>>>(V, kl, save);
+        if (F->mixed) {
+            k_fam_primal32<<<gp, 256, 0, F->s>>>(V, kl, save);
+            k_fam_dual32<<<gd, 256, 0, F->s>>>(V, kl, save);
+        } else {
+            k_fam_primal<<<gp, 256, 0, F->s>>>(V, kl, save);
+            k_fam_dual<<<gd, 256, 0, F->s>>>(V, kl, save);
+        }
     }
     launch_eval<0>(F, V);
-    const int gv = std::max(1, std::min(1024, (std::max(F->n, F->m) * kL + 255) / 256));
+    const int gv = restart_grid(F);
     k_fam_restart<0><<<dim3(gv, F->Gl), 256, 0, F->s>>>(V);
-    k_fam_restart<1><<<dim3(gv, F->Gl), 256, 0, F->s>>>(V);
+    if (F->mixed) {
+        k_fam_restart<3><<<dim3(gv, F->Gl), 256, 0, F->s>>>(V);
+        launch_recenter<0>(F, V);
+    } else {
+        k_fam_restart<1><<<dim3(gv, F->Gl), 256, 0, F->s>>>(V);
+    }
     k_fam_restart<2><<<dim3(gv, F->Gl), 256, 0, F->s>>>(V);
     k_fam_cert_cols<<<dim3(F->ebc, F->Gl), 256, 0, F->s>>>(V);
     k_fam_cert_rows<<<dim3(F->ebr, F->Gl), 256, 0, F->s>>>(V);
@@ -710,7 +957,9 @@ NIR_API int nir_family_create(nir_ctx* ctx, const nir_lp* base, const nir_params
         if (o) F->opts = *o;
         else nir_family_opts_default(&F->opts);
         if (F->prm.mode != NIR_MODE_PDLP) throw std::invalid_argument("the family engine runs pdlp mode only");
-        if (F->prm.precision != NIR_PREC_FP64) throw std::invalid_argument("the family engine runs FP64 only");
+        if (F->prm.precision != NIR_PREC_FP64 && F->prm.precision != NIR_PREC_MIXED)
+            throw std::invalid_argument("unknown precision");
+        F->mixed = F->prm.precision == NIR_PREC_MIXED;
         if (F->prm.x0 || F->prm.y0 || F->prm.unscale_row || F->prm.unscale_col)
             throw std::invalid_argument("family solves take warm starts per instance and no unscaling vectors");
         if (F->opts.lanes <= 0 || F->opts.lanes % kL != 0)
@@ -747,7 +996,7 @@ NIR_API int nir_family_create(nir_ctx* ctx, const nir_lp* base, const nir_params
         {
             // The scaled column bounds are the same for every member.
             PdlpInstance I;
-            pdlp_prepare_instance(base, F->prm, F->R, F->C, F->eta, false, I);
+            pdlp_prepare_instance(base, F->prm, F->R, F->C, F->eta, F->mixed, I);
             F->lo.alloc(n);
             F->lo.upload(I.lo.data(), n, F->s);
             F->up.alloc(n);
@@ -758,6 +1007,18 @@ NIR_API int nir_family_create(nir_ctx* ctx, const nir_lp* base, const nir_params
         const size_t NX = L * n, NY = L * m;
         for (auto* b : {&F->x, &F->xbar, &F->xa, &F->xp, &F->c, &F->qx, &F->rx}) b->alloc(NX);
         for (auto* b : {&F->y, &F->ya, &F->yp, &F->rl, &F->ru, &F->qy, &F->ry}) b->alloc(NY);
+        if (F->mixed) {
+            for (auto* b : {&F->fx, &F->fxbar, &F->fxp, &F->rc, &F->dl, &F->du}) b->alloc(NX);
+            for (auto* b : {&F->fy, &F->fyp, &F->drl, &F->dru, &F->sa}) b->alloc(NY);
+            F->GA.alloc(NX);
+            F->HA.alloc(NY);
+            F->aval32.alloc(static_cast<size_t>(F->nnz));
+            F->tval32.alloc(static_cast<size_t>(F->nnz));
+            const int gconv = static_cast<int>(std::max<long long>(1, std::min<long long>((F->nnz + 255) / 256, 4096)));
+            k_fam_to_float<<<gconv, 256, 0, F->s>>>(F->A.val.p, F->aval32.p, F->nnz);
+            k_fam_to_float<<<gconv, 256, 0, F->s>>>(F->AT.val.p, F->tval32.p, F->nnz);
+            HIP_CHECK(hipGetLastError());
+        }
         F->st.alloc(L);
         F->ltau.alloc(L);
         F->lsigma.alloc(L);
@@ -890,7 +1151,7 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
             p.x0 = ((warm & 1) && !P.dev_x) ? x0.data() : nullptr;
             p.y0 = ((warm & 2) && !P.dev_y) ? y0.data() : nullptr;
             if (weight > 0.0) p.primal_weight = weight;
-            pdlp_prepare_instance(&lp, p, F->R, F->C, F->eta, false, P.inst);
+            pdlp_prepare_instance(&lp, p, F->R, F->C, F->eta, F->mixed, P.inst);
             ready.push_back(std::move(P));
             F->stats.prepare_seconds += secs(tp);
         };
@@ -940,10 +1201,99 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
             }
             if (any) {
                 update_groups();
+                if (F->mixed) launch_recenter<1>(F, F->view());
                 launch_eval<1>(F, F->view());
                 HIP_CHECK(hipGetLastError());
                 ++F->stats.refills;
             }
+        };
+        // Lane l's member continued alone on the single-LP engine from the
+        // lane's state (vectors and PdlpState copied on the device between
+        // blocks, where both engines stand at the same point of the same
+        // iteration), to its end; the result goes to the done callback and
+        // the lane is emptied. Used for the hand-off of the last members and
+        // for an FP32 phase that ends needing the FP64 tail.
+        DevArray<double> ext;
+        DevArray<float> ext32;
+        std::vector<double> hcs, hxs, hys;
+        bool single_stopped = false;
+        auto run_single = [&](int l) {
+            const size_t N = static_cast<size_t>(n), M = static_cast<size_t>(m);
+            if (!ext.p) {
+                ext.alloc(6 * N + 6 * M);
+                if (F->mixed) ext32.alloc(6 * N + 5 * M);
+                hcs.resize(N);
+                hxs.resize(N);
+                hys.resize(M);
+            }
+            const int gx = std::max(1, std::min(256, (std::max(n, m) + 255) / 256));
+            if (F->mixed) k_fam_extract<1><<<gx, 256, 0, F->s>>>(F->view(), l, ext.p, ext32.p);
+            else k_fam_extract<0><<<gx, 256, 0, F->s>>>(F->view(), l, ext.p, nullptr);
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipMemcpyAsync(hcs.data(), ext.p + 4 * N, sizeof(double) * N, hipMemcpyDeviceToHost, F->s));
+            HIP_CHECK(hipStreamSynchronize(F->s));
+            PdlpAdopt a;
+            a.m = m;
+            a.n = n;
+            a.nnz = F->nnz;
+            a.A = &F->A;
+            a.AT = &F->AT;
+            a.R = F->dR.p;
+            a.C = F->dC.p;
+            a.lo = F->lo.p;
+            a.up = F->up.p;
+            const double* ex = ext.p;
+            const double* ey = ext.p + 5 * N;
+            a.x = ex;
+            a.xbar = ex + N;
+            a.xa = ex + 2 * N;
+            a.xp = ex + 3 * N;
+            a.c = ex + 4 * N;
+            a.y = ey;
+            a.ya = ey + M;
+            a.yp = ey + 2 * M;
+            a.rl = ey + 3 * M;
+            a.ru = ey + 4 * M;
+            a.st = F->st.p + l;
+            if (F->mixed) {
+                a.mixed = true;
+                a.GA = ext.p + 5 * N + 5 * M;
+                a.HA = ext.p + 6 * N + 5 * M;
+                const float* fxv = ext32.p;
+                const float* fyv = ext32.p + 6 * N;
+                a.fx = fxv;
+                a.fxbar = fxv + N;
+                a.fxp = fxv + 2 * N;
+                a.rc = fxv + 3 * N;
+                a.dl = fxv + 4 * N;
+                a.du = fxv + 5 * N;
+                a.fy = fyv;
+                a.fyp = fyv + M;
+                a.drl = fyv + 2 * M;
+                a.dru = fyv + 3 * M;
+                a.sa = fyv + 4 * M;
+            }
+            a.h_c.resize(N);
+            for (int j = 0; j < n; ++j) a.h_c[j] = hcs[j] / F->C[j];   // caller units (C: powers of two)
+            a.offset = F->offset;
+            nir_info info;
+            {
+                std::unique_ptr<Pdlp, PdlpDeleter> P(pdlp_create());
+                pdlp_adopt(*P, F->s, F->prm, a);
+                pdlp_run(*P, F->s, F->prm, std::numeric_limits<long long>::max() / 4);
+                pdlp_result(*P, F->s, hxs.data(), hys.data(), &info);
+            }
+            info.setup_seconds = F->stats.setup_seconds;
+            info.solve_seconds = 0.0;
+            if (info.status == 4) single_stopped = true;
+            k_fam_release<<<1, 64, 0, F->s>>>(F->view(), l);
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipStreamSynchronize(F->s));
+            const long long idx = lane_inst[l];
+            F->stats.total_iterations += info.iterations;
+            ++finished;
+            if (done(user, idx, hxs.data(), hys.data(), &info) != 0)
+                throw std::runtime_error("family done callback failed for instance " + std::to_string(idx));
         };
         // Harvest every finished lane: T(z) unscaled, the objective in
         // double-double from the caller's costs (scaled cost / C, exact), the
@@ -954,6 +1304,7 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
             std::vector<long long> index;
             std::vector<double> x, c, y;
             std::vector<PdlpState> st;
+            std::vector<char> taken;   // already handed out (FP64 tail on the single-LP engine)
         };
         Fetched pending;
         long long fetched = 0;
@@ -978,14 +1329,30 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
                 std::copy(fin.begin() + b, fin.begin() + b + k, hl.begin());
                 F->s_lanes.upload(hl.data(), k, F->s);
                 const int gx = std::max(1, std::min(64, (std::max(n, m) + 255) / 256));
-                k_fam_gather<<<dim3(gx, static_cast<unsigned>(k)), 256, 0, F->s>>>(
-                    F->view(), static_cast<int>(k), F->s_lanes.p, F->s_x.p, F->s_c.p, F->s_y.p, F->s_st.p);
+                if (F->mixed)
+                    k_fam_gather<1><<<dim3(gx, static_cast<unsigned>(k)), 256, 0, F->s>>>(
+                        F->view(), static_cast<int>(k), F->s_lanes.p, F->s_x.p, F->s_c.p, F->s_y.p, F->s_st.p);
+                else
+                    k_fam_gather<0><<<dim3(gx, static_cast<unsigned>(k)), 256, 0, F->s>>>(
+                        F->view(), static_cast<int>(k), F->s_lanes.p, F->s_x.p, F->s_c.p, F->s_y.p, F->s_st.p);
                 HIP_CHECK(hipGetLastError());
                 F->s_x.download(out.x.data() + b * n, k * n, F->s);
                 F->s_c.download(out.c.data() + b * n, k * n, F->s);
                 F->s_y.download(out.y.data() + b * m, k * m, F->s);
                 F->s_st.download(out.st.data() + b, k, F->s);
                 HIP_CHECK(hipStreamSynchronize(F->s));
+            }
+            // An FP32 phase that met its tolerance with a polish to do, or
+            // stalled, goes on in FP64: on the single-LP engine, from the
+            // lane's state, before the lane is refilled.
+            out.taken.assign(fin.size(), 0);
+            for (size_t f = 0; f < fin.size(); ++f) {
+                const PdlpState& h = out.st[f];
+                if (F->mixed && ((h.status == 0 && F->prm.polish_tol > 0) || h.status == 2)) {
+                    run_single(fin[f]);
+                    out.taken[f] = 1;
+                    ++F->stats.tails;
+                }
             }
             for (int l : fin) {
                 out.index.push_back(lane_inst[l]);
@@ -1005,6 +1372,7 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
         // nir_session_result reports them, handed to the callback.
         auto process = [&](Fetched& B) {
             for (size_t f = 0; f < B.index.size(); ++f) {
+                if (!B.taken.empty() && B.taken[f]) continue;
                 double* xs = B.x.data() + f * n;
                 double* ys = B.y.data() + f * m;
                 const double* cs = B.c.data() + f * n;
@@ -1057,8 +1425,12 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
                 std::vector<int> hd(dst.begin() + b, dst.begin() + b + k);
                 F->s_dst.upload(hd.data(), k, F->s);
                 const int gx = std::max(1, std::min(64, (std::max(n, m) + 255) / 256));
-                k_fam_move<<<dim3(gx, static_cast<unsigned>(k)), 256, 0, F->s>>>(F->view(), static_cast<int>(k),
-                                                                                   F->s_lanes.p, F->s_dst.p);
+                if (F->mixed)
+                    k_fam_move<1><<<dim3(gx, static_cast<unsigned>(k)), 256, 0, F->s>>>(
+                        F->view(), static_cast<int>(k), F->s_lanes.p, F->s_dst.p);
+                else
+                    k_fam_move<0><<<dim3(gx, static_cast<unsigned>(k)), 256, 0, F->s>>>(
+                        F->view(), static_cast<int>(k), F->s_lanes.p, F->s_dst.p);
                 HIP_CHECK(hipGetLastError());
                 HIP_CHECK(hipStreamSynchronize(F->s));
             }
@@ -1074,80 +1446,26 @@ NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fi
         };
         bool stopped = false;
         // Once no member waits for a lane and at most opts.handoff still run,
-        // each continues alone on the single-LP engine from its lane's state
-        // (vectors and PdlpState copied on the device between blocks, where
-        // both engines stand at the same point of the same iteration), so its
-        // result is the one it would have reached in the family. A lone
-        // member in a 64-lane group walks its rows one lane at a time; the
+        // each continues alone on the single-LP engine (run_single): a lone
+        // member in a 64-lane group walks its rows one lane at a time, the
         // single-LP engine spreads each row's nonzeros over a workgroup.
         auto handoff = [&]() -> bool {
             process(pending);
             const long long before = fetched;
             const auto th = clock::now();
-            DevArray<double> ext;
-            ext.alloc(5 * static_cast<size_t>(n) + 5 * static_cast<size_t>(m));
-            std::vector<double> cs(n), xs(n), ys(m);
-            const int gx = std::max(1, std::min(256, (std::max(n, m) + 255) / 256));
             for (int l = 0; l < F->lanes; ++l) {
                 if (lane_inst[l] < 0) continue;
                 // A stop request ends the remaining lanes in the family (the
                 // main loop's stop path).
                 if (F->prm.should_stop && F->prm.should_stop(F->prm.stop_user)) break;
-                k_fam_extract<<<gx, 256, 0, F->s>>>(F->view(), l, ext.p);
-                HIP_CHECK(hipGetLastError());
-                HIP_CHECK(hipMemcpyAsync(cs.data(), ext.p + 4 * static_cast<size_t>(n), sizeof(double) * n,
-                                         hipMemcpyDeviceToHost, F->s));
-                HIP_CHECK(hipStreamSynchronize(F->s));
-                PdlpAdopt a;
-                a.m = m;
-                a.n = n;
-                a.nnz = F->nnz;
-                a.A = &F->A;
-                a.AT = &F->AT;
-                a.R = F->dR.p;
-                a.C = F->dC.p;
-                a.lo = F->lo.p;
-                a.up = F->up.p;
-                const double* ex = ext.p;
-                const double* ey = ext.p + 5 * static_cast<size_t>(n);
-                a.x = ex;
-                a.xbar = ex + n;
-                a.xa = ex + 2 * static_cast<size_t>(n);
-                a.xp = ex + 3 * static_cast<size_t>(n);
-                a.c = ex + 4 * static_cast<size_t>(n);
-                a.y = ey;
-                a.ya = ey + m;
-                a.yp = ey + 2 * static_cast<size_t>(m);
-                a.rl = ey + 3 * static_cast<size_t>(m);
-                a.ru = ey + 4 * static_cast<size_t>(m);
-                a.st = F->st.p + l;
-                a.h_c.resize(n);
-                for (int j = 0; j < n; ++j) a.h_c[j] = cs[j] / F->C[j];   // caller units (C: powers of two)
-                a.offset = F->offset;
-                nir_info info;
-                {
-                    std::unique_ptr<Pdlp, PdlpDeleter> P(pdlp_create());
-                    pdlp_adopt(*P, F->s, F->prm, a);
-                    pdlp_run(*P, F->s, F->prm, std::numeric_limits<long long>::max() / 4);
-                    pdlp_result(*P, F->s, xs.data(), ys.data(), &info);
-                }
-                info.setup_seconds = F->stats.setup_seconds;
-                info.solve_seconds = 0.0;
-                if (info.status == 4) stopped = true;
-                k_fam_release<<<1, 64, 0, F->s>>>(F->view(), l);
-                HIP_CHECK(hipGetLastError());
-                HIP_CHECK(hipStreamSynchronize(F->s));
-                const long long idx = lane_inst[l];
+                run_single(l);
                 lane_inst[l] = -1;
                 free_lanes.push_back(l);
                 --active;
                 ++fetched;
-                ++finished;
                 ++F->stats.handoffs;
-                F->stats.total_iterations += info.iterations;
-                if (done(user, idx, xs.data(), ys.data(), &info) != 0)
-                    throw std::runtime_error("family done callback failed for instance " + std::to_string(idx));
             }
+            if (single_stopped) stopped = true;
             std::sort(free_lanes.begin(), free_lanes.end(), std::greater<int>());
             update_groups();
             F->stats.handoff_seconds += secs(th);

@@ -985,6 +985,32 @@ void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm,
     P.setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// A finished state: the end of the run, or, for an FP32 phase that met its
+// tolerance with a polish to do or stalled, the switch to the FP64 tail.
+// Returns true when the run goes on (in FP64).
+static bool on_done(Pdlp& P, hipStream_t s, const PdlpState& h) {
+    if (P.mixed && (h.status == 0 || h.status == 2) && !P.polish_done) {
+        const double target = P.polish_tol > 0 ? std::min(P.polish_tol, P.tol) : P.tol;
+        if (h.status == 0 && P.polish_tol <= 0) {
+            P.finished = true;
+            P.status = 0;
+            return false;
+        }
+        P.polish_done = true;
+        switch_to_fp64(P, s, target);
+        PdlpState h2 = read_state(P, s);
+        if (h2.done) {
+            P.finished = true;
+            P.status = sov_status(h2);
+            return false;
+        }
+        return true;
+    }
+    P.finished = true;
+    P.status = sov_status(h);
+    return false;
+}
+
 void pdlp_adopt(Pdlp& P, hipStream_t s, const nir_params& prm, const PdlpAdopt& a) {
     auto t0 = std::chrono::steady_clock::now();
     P.n = a.n;
@@ -995,7 +1021,7 @@ void pdlp_adopt(Pdlp& P, hipStream_t s, const nir_params& prm, const PdlpAdopt& 
     P.eval_freq = std::max(1, prm.eval_freq);
     P.use_graph = prm.use_graph != 0;
     P.tol = prm.tol > 0 ? prm.tol : 1e-6;
-    P.mixed = false;
+    P.mixed = a.mixed;
     P.polish_tol = prm.polish_tol;
     P.offset = a.offset;
     P.should_stop = prm.should_stop;
@@ -1027,11 +1053,33 @@ void pdlp_adopt(Pdlp& P, hipStream_t s, const nir_params& prm, const PdlpAdopt& 
     P.st.alloc(1);
     P.st.from_device(a.st, 1, s);
     alloc_work(P);
-    const PdlpState h = read_state(P, s);
-    if (h.done) {
-        P.finished = true;
-        P.status = sov_status(h);
+    if (P.mixed) {
+        P.a32.val.alloc(P.nnz);
+        P.at32.val.alloc(P.nnz);
+        P.a32.part.alloc(std::max(m, 1));
+        P.at32.part.alloc(std::max(n, 1));
+        k_to_float<<<grid_for(P.nnz), 256, 0, s>>>(P.A->val.p, P.a32.val.p, P.nnz);
+        k_to_float<<<grid_for(P.nnz), 256, 0, s>>>(P.AT->val.p, P.at32.val.p, P.nnz);
+        PD_CHECK(hipGetLastError());
+        P.f.alloc(n, m);
+        P.HA.alloc(m);
+        P.GA.alloc(n);
+        P.HA.from_device(a.HA, m, s);
+        P.GA.from_device(a.GA, n, s);
+        P.f.x.from_device(a.fx, n, s);
+        P.f.xbar.from_device(a.fxbar, n, s);
+        P.f.xp.from_device(a.fxp, n, s);
+        P.f.c.from_device(a.rc, n, s);
+        P.f.lo.from_device(a.dl, n, s);
+        P.f.up.from_device(a.du, n, s);
+        P.f.y.from_device(a.fy, m, s);
+        P.f.yp.from_device(a.fyp, m, s);
+        P.f.rl.from_device(a.drl, m, s);
+        P.f.ru.from_device(a.dru, m, s);
+        P.f.ya.from_device(a.sa, m, s);
     }
+    const PdlpState h = read_state(P, s);
+    if (h.done) on_done(P, s, h);
     P.setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -1060,25 +1108,8 @@ void pdlp_run(Pdlp& P, hipStream_t s, const nir_params& prm, long long budget) {
             if (wait_poll(P, slot ^ 1) || blocks == 0) {
                 PdlpState h = read_state(P, s);
                 if (h.done) {
-                    if (P.mixed && (h.status == 0 || h.status == 2) && !P.polish_done) {
-                        // FP32 phase met its tolerance (or stalled): FP64 tail.
-                        const double target = P.polish_tol > 0 ? std::min(P.polish_tol, P.tol) : P.tol;
-                        if (h.status == 0 && P.polish_tol <= 0) {
-                            P.finished = true;
-                            P.status = 0;
-                            break;
-                        }
-                        P.polish_done = true;
-                        switch_to_fp64(P, s, target);
-                        PdlpState h2 = read_state(P, s);
-                        if (h2.done) {
-                            P.finished = true;
-                            P.status = sov_status(h2);
-                        }
-                        continue;
-                    }
-                    P.finished = true;
-                    P.status = sov_status(h);
+                    // FP32 phase met its tolerance (or stalled): FP64 tail.
+                    if (on_done(P, s, h)) continue;
                     break;
                 }
             }

@@ -67,7 +67,10 @@ void pdlp_prepare_instance(const nir_lp* lp, const nir_params& prm, const std::v
 // A run continued on this engine from a state made elsewhere (a lane of the
 // family engine, between evaluation blocks): the scaled matrices, scaling and
 // column bounds are shared, the member's vectors and state are copied on the
-// device. FP64 only; the next pdlp_run starts with the next block.
+// device; the next pdlp_run starts with the next block. With `mixed` the
+// state is the FP32 phase's (FP64 anchor in xa / ya, its products HA, GA,
+// FP32 deltas and re-based constants); a state whose FP32 phase has ended
+// switches to the FP64 tail here, as pdlp_run does.
 struct PdlpAdopt {
     int m = 0, n = 0;
     long long nnz = 0;
@@ -80,6 +83,10 @@ struct PdlpAdopt {
     const PdlpState* st = nullptr;   // device
     std::vector<double> h_c;         // caller-unit costs (objective of the result)
     double offset = 0.0;
+    bool mixed = false;
+    const double *HA = nullptr, *GA = nullptr;
+    const float *fx = nullptr, *fxbar = nullptr, *fxp = nullptr, *rc = nullptr, *dl = nullptr, *du = nullptr;
+    const float *fy = nullptr, *fyp = nullptr, *drl = nullptr, *dru = nullptr, *sa = nullptr;
 };
 void pdlp_adopt(Pdlp& P, hipStream_t s, const nir_params& prm, const PdlpAdopt& a);
 

@@ -113,7 +113,7 @@ class NirFamilyStats(C.Structure):
                 ("refills", C.c_int64), ("total_iterations", C.c_int64), ("setup_seconds", C.c_double),
                 ("solve_seconds", C.c_double), ("compactions", C.c_int64), ("group_blocks", C.c_int64),
                 ("prepare_seconds", C.c_double), ("wait_seconds", C.c_double), ("handoffs", C.c_int64),
-                ("handoff_seconds", C.c_double)]
+                ("handoff_seconds", C.c_double), ("tails", C.c_int64)]
 
 
 class NirPresolveInfo(C.Structure):

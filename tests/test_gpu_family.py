@@ -188,6 +188,41 @@ def test_family_error_return_then_destroy(engine):
         assert same(res[i], engine.solve(fam.instance(th[i]), params)), f"member {i}"
 
 
+@pytest.mark.parametrize("polish", [0.0, 1e-7])
+def test_family_mixed_precision_members_match_single_lp(engine, polish):
+    """Mixed precision: every member equals the single-LP engine's mixed run
+    bit for bit (re-centred FP32 phase; with a polish tolerance every member
+    then runs its FP64 tail on the single-LP engine from its lane's state),
+    through refills, warm starts and the hand-off; the oracle's or_kkt
+    certifies the tolerance."""
+    from nirnaya_amd.engine import PREC_MIXED
+    fam = ParametricFamily(20264, 400, 1000, 5)
+    th = fam.thetas(90, 23)
+    params = default_params(mode=MODE_PDLP, precision=PREC_MIXED, max_iters=400000, tol=1e-6, polish_tol=polish)
+    rng = np.random.default_rng(8)
+    cold = engine.solve(fam.instance(th[0]), default_params(mode=MODE_PDLP, max_iters=400000, tol=1e-6))
+    starts = {i: (cold.x * (1 + 0.01 * rng.standard_normal(cold.x.size)), None) for i in range(0, len(th), 7)}
+    F = engine.family(fam.base, params, lanes=64)
+    try:
+        res = F.solve(len(th), lambda i: (*members(fam, th)(i), *starts.get(i, (None, None))))
+        st = F.stats()
+    finally:
+        F.close()
+    assert st.instances == len(th) and st.refills >= 2 and st.handoffs >= 1
+    if polish > 0:
+        assert st.tails >= 1
+    for i, t in enumerate(th):
+        x0, y0 = starts.get(i, (None, None))
+        s = engine.solve(fam.instance(t), params, x0=x0, y0=y0)
+        assert same(res[i], s), f"member {i}: it {res[i].iterations}/{s.iterations} st {res[i].status}/{s.status}"
+        assert res[i].status == 0
+    for i in (0, 1, 7):
+        k = Oracle(fam.instance(th[i])).kkt(res[i].x, res[i].y)
+        assert max(k[:3]) <= (polish or 1e-6), (i, k[:3])
+    print(f"mixed, polish {polish}: {len(th)} members equal the single-LP engine; tails {st.tails}, "
+          f"hand-offs {st.handoffs}, blocks {st.blocks}")
+
+
 def test_family_certifies_infeasible_members(engine):
     """Members of an infeasible base (costs perturbed) end with the single-LP
     engine's certificate status and iterate."""
