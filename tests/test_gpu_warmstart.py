@@ -120,3 +120,24 @@ def test_cfg4_full_size_family_warm_start(engine):
           f"POD ranks {pod_x.rank}/{pod_y.rank}; loss {losses[0]:.3e} -> {losses[-1]:.3e}; held-out cold median "
           f"{np.median(ci):.0f} ({t_cold:.1f}s), warm median {np.median(wi):.0f} ({t_warm:.1f}s); saved "
           f"{1 - wi.sum() / ci.sum():+.1%}")
+
+
+def test_full_size_family_mixed_precision(engine):
+    """The specified family at full size on the mixed-precision family
+    engine: 16 members to 1e-6, each certified by the oracle's or_kkt
+    (double-double on the host), one equal to the single-LP mixed engine bit
+    for bit."""
+    from nirnaya_amd.engine import PREC_MIXED
+    from nirnaya_amd.warmstart import family_solve
+    from oracle_lib import Oracle
+    fam = ParametricFamily(20264, 50_000, 120_000, 5)
+    params = default_params(mode=MODE_PDLP, precision=PREC_MIXED, max_iters=400000, tol=1e-6)
+    th = fam.thetas(16, 5)
+    res, dt = family_solve(engine, fam, th, params)
+    for i, t in enumerate(th):
+        assert res[i].status == 0, i
+        k = Oracle(fam.instance(t)).kkt(res[i].x, res[i].y)
+        assert max(k[:3]) <= 1e-6, (i, k[:3])
+    one = engine.solve(fam.instance(th[3]), params)
+    assert np.array_equal(one.x, res[3].x) and np.array_equal(one.y, res[3].y) and one.iterations == res[3].iterations
+    print(f"full size, mixed: 16 members in {dt:.1f}s, iterations median {np.median([r.iterations for r in res]):.0f}")
