@@ -49,6 +49,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -1504,6 +1505,10 @@ NIR_API int nir_batch_solve(nir_batch* B) {
                 HIP_CHECK(hipEventSynchronize(B->poll_ev[prev]));
                 pending[prev] = 0;
                 remaining = B->h_poll[prev];
+                if (B->prm.verbose > 1)
+                    std::fprintf(stderr, "[batch] %.4f s: %d instances unfinished\n",
+                                 std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(),
+                                 remaining);
             }
         }
         HIP_CHECK(hipStreamSynchronize(B->s));
