@@ -835,7 +835,7 @@ def main():
             calibration["gather_rate_G_per_s"][str(length * 8)] = {"gather": round(g, 1), "value_x_gather": round(gv, 1)}
             log(f"[bench] random FP64 gathers, operand {length * 8 / 1e6:.2f} MB: {g:.0f} G/s alone, "
                 f"{gv:.0f} G/s with the value stream")
-    params = default_params(mode=MODE_REFERENCE, max_iters=args.warmup + 2 * args.steps + 2064, check_termination=0,
+    params = default_params(mode=MODE_REFERENCE, max_iters=args.warmup + 2 * args.steps + 6064, check_termination=0,
                             tile_nnz=args.tile, tile_rows=args.tile_rows, seg_width=args.seg,
                             seg_width_t=args.seg_t, seg_fused=args.fused)
     sess = eng.session(lp, params)
@@ -854,6 +854,12 @@ def main():
     # launches; `value` is the production sweep of exactly `steps`.
     kt = t if args.steps >= 1000 else sess.time(1000)
     k_iters = max(args.steps, 1000)
+    # A longer production sweep beside the timed one, so that a short --steps
+    # run still shows the steady rate it samples (reported, not `value`).
+    long_steps = 2000
+    lt = t if args.steps >= long_steps else sess.time(long_steps)
+    long_it_s = max(args.steps, long_steps) / (lt.total_ms / 1e3)
+    log(f"[bench] production sweep of {max(args.steps, long_steps)} iterations: {long_it_s:.0f} iterations/s")
     log(f"[bench] timed {args.steps} iterations: {t.total_ms:.1f} ms; per-dispatch sweep over {k_iters}: A xbar "
         f"{kt.spmv_ms / k_iters * 1e3:.1f} us in {kt.spmv_passes} passes, A^T y {kt.spmtv_ms / k_iters * 1e3:.1f} us "
         f"in {kt.spmtv_passes} passes")
@@ -925,6 +931,7 @@ def main():
                     "spmtv_gbs": t.spmtv_bytes / spmtv_avg_s / 1e9,
                     "iteration_bytes": t.iter_bytes,
                     "iteration_gbs": t.iter_bytes * args.steps / (t.total_ms / 1e3) / 1e9},
+        "production_sweep": {"iterations": max(args.steps, long_steps), "iterations_per_s": round(long_it_s, 2)},
         "time_to_1e-6": ttt,
         "batched_512_planning_lps": batched,
         "calibration": calibration,

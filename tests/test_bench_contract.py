@@ -44,6 +44,10 @@ def test_default_run_prints_the_contract_line():
     for key in ("value", "unit", "cores", "kind", "sample"):
         assert key in cb, key
     assert cb["kind"] in ("reference", "port") and cb["value"] > 0
+    # The longer production sweep beside the timed steps agrees with `value`
+    # to within run-to-run noise.
+    ps = d["production_sweep"]
+    assert ps["iterations"] == 2000 and abs(ps["iterations_per_s"] / d["value"] - 1.0) < 0.2
 
 
 @pytest.mark.gpu
