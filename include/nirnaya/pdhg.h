@@ -131,7 +131,11 @@ NIR_API int nir_session_solve(nir_session* s);
 NIR_API int nir_session_result(nir_session* s, double* x, double* y, nir_info* info);
 NIR_API int nir_session_time(nir_session* s, int64_t iters, nir_timing* out);
 /* Tile shapes in use: {A tile, A rows, A segments, A fused, A^T tile, A^T rows,
- * A^T segments, A^T fused} (fused = all segments walked in one launch). */
+ * A^T segments, A^T fused} (fused = all segments walked in one launch). An A
+ * tile of 0 marks a reference-mode session on the small-LP path: every
+ * iteration in one workgroup with x, xbar and y in LDS (chosen for LPs of at
+ * most 16,384 nonzeros, 2n + m <= 19,200, rows of A and A^T of at most 512
+ * nonzeros, when no tile or segment width is given). */
 NIR_API int nir_session_shape(nir_session* s, int32_t* out8);
 /* Relative KKT error of a reference-mode session's current iterate (x, y) on
  * its LP, evaluated on the device with double-double products and sums:

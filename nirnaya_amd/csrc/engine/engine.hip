@@ -150,6 +150,165 @@ __global__ __launch_bounds__(kThreads) void k_vmax_reduce(const double* __restri
     if (threadIdx.x == 0) st->vslot[k & 1] = static_cast<unsigned long long>(__double_as_longlong(v));
 }
 
+// sum_{p in [p0, p1)} val[p] v[col[p]] from 0.0 in ascending p, products
+// rounded before the add; four operand reads in flight per step.
+__device__ __forceinline__ double seq_dot(int p0, int p1, const int* __restrict__ col, const double* __restrict__ val,
+                                          const double* __restrict__ v) {
+    double s = 0.0;
+    int p = p0;
+    for (; p + 4 <= p1; p += 4) {
+        const int c0 = col[p], c1 = col[p + 1], c2 = col[p + 2], c3 = col[p + 3];
+        const double a0 = val[p], a1 = val[p + 1], a2 = val[p + 2], a3 = val[p + 3];
+        const double g0 = v[c0], g1 = v[c1], g2 = v[c2], g3 = v[c3];
+        s += a0 * g0;
+        s += a1 * g1;
+        s += a2 * g2;
+        s += a3 * g3;
+    }
+    for (; p < p1; ++p) s += val[p] * v[col[p]];
+    return s;
+}
+
+// Small LPs: `count` reference iterations k0, k0 + 1, ... in one workgroup,
+// with x, xbar and y resident in LDS (2n + m doubles) and a barrier between
+// the primal and the dual half instead of a kernel boundary. Lanes own
+// columns and rows by stride; every product is the sequential sum of the
+// multi-kernel path (ascending row order for A^T y, ascending column order
+// for A xbar, from 0.0), and the updates are the same expressions, so the
+// iterates are the same bits. The stop rule is tested after the dual half of
+// iteration k (the reference's `iter > 20 && iter % 32 == 0 && V <= 1e-6`,
+// pdhg_solver.hpp:97-100); vpart[0] receives the last iteration's maximum
+// row violation (the other entries hold 0).
+template <int MAT>
+__global__ __launch_bounds__(1024) void k_ref_small(int m, int n, const int* __restrict__ trp,
+                                                    const int* __restrict__ tcol, const double* __restrict__ tval,
+                                                    const int* __restrict__ arp, const int* __restrict__ acol,
+                                                    const double* __restrict__ aval, double* __restrict__ x,
+                                                    double* __restrict__ xbar, double* __restrict__ y,
+                                                    const double* __restrict__ c, const double* __restrict__ lo,
+                                                    const double* __restrict__ up, const double* __restrict__ rl,
+                                                    const double* __restrict__ ru, double tau, double sigma,
+                                                    RefState* st, double* __restrict__ vpart, int nvpart, int k0,
+                                                    int count, int check) {
+    extern __shared__ double lds[];
+    __shared__ double red[16];
+    __shared__ int stop;
+    if (st->done) return;
+    double* sx = lds;
+    double* sxb = lds + n;
+    double* sy = lds + 2 * static_cast<size_t>(n);
+    const int tid = threadIdx.x, nt = blockDim.x, nw = nt >> 6;
+    for (int j = tid; j < n; j += nt) {
+        sx[j] = x[j];
+        sxb[j] = xbar[j];
+    }
+    for (int i = tid; i < m; i += nt) sy[i] = y[i];
+    const int nz = MAT ? arp[m] : 0;   // from global memory, before arp moves to LDS
+    if (MAT) {
+        // MAT >= 1: both matrices copied into LDS behind the vectors (values,
+        // then row pointers and indices); MAT == 2: the costs and bounds as
+        // well, so nothing in the iteration loop waits on global memory.
+        double* sav = sy + m;
+        double* stv = sav + nz;
+        int* sarp = reinterpret_cast<int*>(stv + nz);
+        int* strp = sarp + (m + 1);
+        int* sac = strp + (n + 1);
+        int* stc = sac + nz;
+        for (int p = tid; p < nz; p += nt) {
+            sav[p] = aval[p];
+            stv[p] = tval[p];
+            sac[p] = acol[p];
+            stc[p] = tcol[p];
+        }
+        for (int i = tid; i <= m; i += nt) sarp[i] = arp[i];
+        for (int j = tid; j <= n; j += nt) strp[j] = trp[j];
+        aval = sav;
+        tval = stv;
+        arp = sarp;
+        trp = strp;
+        acol = sac;
+        tcol = stc;
+    }
+    if (MAT == 2) {
+        // then the costs and bounds (doubles, after the index arrays)
+        const size_t ints = static_cast<size_t>(m) + 1 + n + 1 + 2 * static_cast<size_t>(nz);
+        double* sconst = reinterpret_cast<double*>(const_cast<int*>(arp)) + (ints + 1) / 2;
+        double *sc = sconst, *slo = sc + n, *sup = slo + n, *srl = sup + n, *sru = srl + m;
+        for (int j = tid; j < n; j += nt) {
+            sc[j] = c[j];
+            slo[j] = lo[j];
+            sup[j] = up[j];
+        }
+        for (int i = tid; i < m; i += nt) {
+            srl[i] = rl[i];
+            sru[i] = ru[i];
+        }
+        c = sc;
+        lo = slo;
+        up = sup;
+        rl = srl;
+        ru = sru;
+    }
+    if (tid == 0) stop = 0;
+    __syncthreads();
+    double V = 0.0;
+    int k = k0;
+    for (; k < k0 + count; ++k) {
+        for (int j = tid; j < n; j += nt) {
+            const double g = seq_dot(trp[j], trp[j + 1], tcol, tval, sy);
+            const double xj = sx[j];
+            double cand = xj - tau * (c[j] - g);
+            if (finite_bound(lo[j])) cand = amax(lo[j], cand);
+            if (finite_bound(up[j])) cand = amin(up[j], cand);
+            sxb[j] = 2.0 * cand - xj;
+            sx[j] = cand;
+        }
+        __syncthreads();
+        double vloc = 0.0;
+        for (int i = tid; i < m; i += nt) {
+            const double h = seq_dot(arp[i], arp[i + 1], acol, aval, sxb);
+            const double yi = sy[i];
+            double r = h - yi / sigma;
+            const double l = rl[i], u = ru[i];
+            const bool lf = finite_bound(l), uf = finite_bound(u);
+            if (lf) r = amax(l, r);
+            if (uf) r = amin(u, r);
+            sy[i] = yi + sigma * (r - h);
+            double viol = 0.0;
+            if (lf) viol = amax(viol, l - h);
+            if (uf) viol = amax(viol, h - u);
+            vloc = amax(vloc, viol);
+        }
+        // Maximum violation over the workgroup (every lane gets it); the
+        // barriers also order this dual half before the next primal half.
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) vloc = amax(vloc, __shfl_xor(vloc, o));
+        if ((tid & 63) == 0) red[tid >> 6] = vloc;
+        __syncthreads();
+        V = red[0];
+        for (int w = 1; w < nw; ++w) V = amax(V, red[w]);
+        __syncthreads();
+        if (check && k > 20 && (k & 31) == 0 && V <= 1e-6) {
+            ++k;
+            if (tid == 0) stop = 1;
+            break;
+        }
+    }
+    for (int j = tid; j < n; j += nt) {
+        x[j] = sx[j];
+        xbar[j] = sxb[j];
+    }
+    for (int i = tid; i < m; i += nt) y[i] = sy[i];
+    for (int b = 1 + tid; b < nvpart; b += nt) vpart[b] = 0.0;
+    if (tid == 0) {
+        vpart[0] = V;
+        if (stop) {
+            st->done = 1;
+            st->iters = k;
+        }
+    }
+}
+
 __global__ void k_set_base(RefState* st, int base) { st->base = base; }
 __global__ void k_advance(RefState* st, int by) { st->base += by; }
 
@@ -368,6 +527,11 @@ struct nir_session {
     double tau = 0.0, sigma = 0.0;
     double setup_seconds = 0.0, solve_seconds = 0.0;
 
+    // small LPs: all iterations of a chunk in one workgroup (k_ref_small)
+    int small_threads = 0;   // 0: the multi-kernel path
+    int small_mat = 0;       // 1: the matrices are copied into LDS as well, 2: and costs and bounds
+    size_t small_lds = 0;
+
     // graph of 32 reference iterations
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -474,6 +638,18 @@ void enqueue_ref_block(nir_session* S, long long k, long long count) {
     }
 }
 
+// Iterations [k, k + count) of a small LP in one workgroup.
+void enqueue_ref_small(nir_session* S, long long k, long long count) {
+    const CsrShape& A = S->A.shape;
+    const CsrShape& AT = S->AT.shape;
+    auto kern = S->small_mat == 2 ? k_ref_small<2> : S->small_mat == 1 ? k_ref_small<1> : k_ref_small<0>;
+    kern<<<1, S->small_threads, S->small_lds, S->s>>>(
+        S->m, S->n, AT.rowptr.p, AT.col.p, S->AT.val.p, A.rowptr.p, A.col.p, S->A.val.p, S->x.p, S->xbar.p, S->y.p,
+        S->c.p, S->lo.p, S->up.p, S->rl.p, S->ru.p, S->tau, S->sigma, S->st.p, S->vpart.p,
+        static_cast<int>(S->vpart.n), static_cast<int>(k), static_cast<int>(count), S->prm.check_termination ? 1 : 0);
+    HIP_CHECK(hipGetLastError());
+}
+
 // Copy the state word to pinned slot `slot` and mark it pending.
 void post_poll(nir_session* S, int slot) {
     HIP_CHECK(hipMemcpyAsync(&S->h_poll[slot], S->st.p, sizeof(RefState), hipMemcpyDeviceToHost, S->s));
@@ -512,7 +688,8 @@ void run_reference(nir_session* S, long long budget) {
         }
         const long long room = 256 - (S->k_next % 256);
         const long long chunk = std::min({budget, room, S->max_iters - S->k_next});
-        enqueue_ref_block(S, S->k_next, chunk);
+        if (S->small_threads > 0) enqueue_ref_small(S, S->k_next, chunk);
+        else enqueue_ref_block(S, S->k_next, chunk);
         S->k_next += chunk;
         budget -= chunk;
         const int slot = static_cast<int>(S->poll_serial++ & 1);
@@ -549,6 +726,51 @@ void reference_step_sizes(nir_session* S) {
     const double smax = std::sqrt(eta);
     S->tau = 0.85 / std::max(1e-6, smax);
     S->sigma = 0.85 / std::max(1e-6, smax);
+}
+
+// Small LPs run their iterations in one workgroup (k_ref_small) when the
+// default layout is in use (no explicit tile or segments: those ask for the
+// tiled kernels), x, xbar and y fit its LDS and no row of A or A^T is long
+// enough to hold one lane up; with both matrices (and then the costs and
+// bounds) in LDS as well when they fit beside the vectors, otherwise only up
+// to kSmallGlobalNnz nonzeros (beyond that the tiled kernels' full-chip
+// products are faster, measured on MI355X).
+constexpr long long kSmallGlobalNnz = 3072;
+constexpr int kSmallMaxRow = 512;
+constexpr size_t kSmallLdsBytes = 150 * 1024;
+
+void choose_small_path(nir_session* S, const std::vector<int>& a_ptr, const std::vector<int>& at_ptr, bool tuned) {
+    S->small_threads = 0;
+    S->small_mat = 0;
+    if (tuned || S->prm.tile_nnz != 0 || S->prm.seg_width != 0 || S->prm.seg_width_t > 0) return;
+    if (S->A.shape.nseg != 1 || S->AT.shape.nseg != 1) return;
+    const size_t m = static_cast<size_t>(S->m), n = static_cast<size_t>(S->n), nz = static_cast<size_t>(S->nnz);
+    const size_t vec = (2 * n + m) * sizeof(double);
+    const size_t ints = m + 1 + n + 1 + 2 * nz;
+    const size_t mat = 2 * nz * sizeof(double) + (ints + 1) / 2 * sizeof(double);
+    const size_t consts = (3 * n + 2 * m) * sizeof(double);
+    int use_mat;
+    size_t bytes;
+    if (vec + mat + consts <= kSmallLdsBytes) use_mat = 2, bytes = vec + mat + consts;
+    else if (vec + mat <= kSmallLdsBytes) use_mat = 1, bytes = vec + mat;
+    else if (vec <= kSmallLdsBytes && S->nnz <= kSmallGlobalNnz) use_mat = 0, bytes = vec;
+    else return;
+    for (int i = 0; i < S->m; ++i)
+        if (a_ptr[i + 1] - a_ptr[i] > kSmallMaxRow) return;
+    for (int j = 0; j < S->n; ++j)
+        if (at_ptr[j + 1] - at_ptr[j] > kSmallMaxRow) return;
+    const size_t lds = std::max<size_t>(bytes, 8);
+    const void* fn = use_mat == 2   ? reinterpret_cast<const void*>(&k_ref_small<2>)
+                     : use_mat == 1 ? reinterpret_cast<const void*>(&k_ref_small<1>)
+                                    : reinterpret_cast<const void*>(&k_ref_small<0>);
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+    }
+    const int most = std::max({S->m, S->n, 64});
+    S->small_threads = std::min(1024, (most + 63) / 64 * 64);
+    S->small_mat = use_mat;
+    S->small_lds = lds;
 }
 
 // Nonzeros from which tile_nnz = 0 autotunes the layout.
@@ -729,6 +951,7 @@ NIR_API int nir_session_create(nir_ctx* ctx, const nir_lp* lp, const nir_params*
             for (auto& e : S->poll_ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
             reference_step_sizes(S.get());
             HIP_CHECK(hipStreamSynchronize(S->s));
+            choose_small_path(S.get(), a_ptr, at_ptr, tune);
         }
         S->setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         *out = S.release();
@@ -885,7 +1108,7 @@ NIR_API int nir_session_time(nir_session* S, int64_t iters, nir_timing* out) {
 
 NIR_API int nir_session_shape(nir_session* S, int32_t* out8) {
     if (!S || !out8) return -1;
-    out8[0] = S->A.shape.shape.tile;
+    out8[0] = S->small_threads > 0 ? 0 : S->A.shape.shape.tile;
     out8[1] = S->A.shape.shape.rows;
     out8[2] = S->A.shape.nseg;
     out8[3] = S->A.shape.fused_ok ? 1 : 0;

@@ -96,9 +96,10 @@ def test_interrupt_from_another_thread():
     # deadline (src/gpu/pdhg_solver.hpp:63-65), ends with TimeLimit.
     import threading
     import time
-    from nirnaya_amd.engine import gen_equality_lp
-    m = build(NirnayaModel("intr"), gen_equality_lp(12, 60, 150, 4))
-    m.set_option("pdhg_tol", "1e-300")            # never met: only the interrupt ends the run
+    # An infeasible LP (an equality row repeated with its right-hand side
+    # shifted by 1): the reference loop's row violation never falls to 1e-6,
+    # so only the interrupt ends the run.
+    m = build(NirnayaModel("intr"), lpkit.infeasible_lps()["repeated_row_shifted"][1])
     m.set_option("pdhg_max_iters", "2000000000")
     out = {}
 

@@ -1,0 +1,157 @@
+"""GPU parity of reference mode's small-LP path (one workgroup runs every
+iteration of a chunk, x, xbar and y in LDS) against the oracle and against
+the tiled multi-kernel path on the same LPs.
+
+Both paths sum every product sequentially from 0.0 in the reference's order
+(pdhg_solver.hpp:70-95, csc_matrix.hpp:99-120) with the same update
+expressions, so x, y, the iteration count, the stop decision and the last
+maximum row violation are held to bit equality.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from nirnaya_amd.engine import LP, MODE_REFERENCE, default_params, gen_equality_lp, gen_mixed_lp, gen_staircase_lp
+from oracle_lib import Oracle
+import lpkit
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _crude_blending():
+    lp, _, _ = lpkit.read_mps_lp(os.path.join(HERE, "golden", "MRPL_Crude_Blending_LP.mps"))
+    return lp
+
+
+def _edge_lp():
+    """Free, one-sided and ranged rows, free and one-sided columns, an empty
+    row and an empty column."""
+    rng = np.random.default_rng(21)
+    m, n = 90, 700
+    A = np.zeros((m, n))
+    for j in range(n):
+        rows = rng.choice(m - 2, size=3, replace=False) + 1
+        A[rows, j] = rng.standard_normal(3)
+    A[:, 11] = 0.0
+    A[0, :] = 0.0
+    inf = np.inf
+    l = np.where(rng.random(n) < 0.2, -inf, 0.0)
+    u = np.where(rng.random(n) < 0.3, inf, rng.uniform(1, 5, n))
+    rl = rng.uniform(-1, 0, m)
+    ru = rl + rng.uniform(0, 2, m)
+    rl[rng.random(m) < 0.3] = -inf
+    ru[rng.random(m) < 0.3] = inf
+    rl[7], ru[7] = -inf, inf
+    return LP.from_dense(A, rng.standard_normal(n), l, u, rl, ru)
+
+
+def _wide_sparse_lp():
+    """4,000 x 7,000 with 2,400 nonzeros: the vectors fit the LDS, the matrices
+    beside them do not, so the matrices are read from global memory."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(8)
+    m, n, nz = 4000, 7000, 2400
+    A = sp.csc_matrix((rng.standard_normal(nz), (rng.integers(0, m, nz), rng.integers(0, n, nz))), shape=(m, n))
+    A.sum_duplicates()
+    A.sort_indices()
+    x0 = rng.uniform(0, 1, n)
+    b = A @ x0
+    return LP(m, n, A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data.astype(np.float64),
+              rng.uniform(-1, 1, n), np.zeros(n), np.full(n, 4.0), b - 0.5, b + 0.5)
+
+
+CASES = {
+    "unit_11": lpkit.pdhg_unit_lp,
+    "transport": lpkit.transport_lp,
+    "crude_blending": _crude_blending,
+    "edges": _edge_lp,
+    "equality_100x250": lambda: gen_equality_lp(7, 100, 250, 5),
+    "staircase_T4K5": lambda: gen_staircase_lp(11, 4, 5),
+    "mixed_150x450": lambda: gen_mixed_lp(3, 150, 450, 6.0, 0.01, 1e3),
+    "equality_400x1000": lambda: gen_equality_lp(7, 400, 1000, 5),   # matrices in LDS, bounds in global memory
+    "wide_sparse": _wide_sparse_lp,                                    # matrices in global memory
+}
+
+
+def _params(K, check, **kw):
+    return default_params(mode=MODE_REFERENCE, max_iters=K, check_termination=1 if check else 0, **kw)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_small_path_bit_identical(engine, name):
+    lp = CASES[name]()
+    K = 333   # crosses chunk boundaries at 256 and the stop rule's 32-cadence
+    s = engine.session(lp, _params(K, False))
+    try:
+        assert s.shape()[0] == 0, "expected the small-LP path"
+        s.solve()
+        r = s.result()
+    finally:
+        s.close()
+    xo, yo, tau, it, V = Oracle(lp).pdhg(K, check=False)
+    assert r.iterations == it == K and r.tau == tau
+    assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+    assert r.info.max_primal_viol == V
+    # The tiled multi-kernel path on the same LP (an explicit tile asks for it).
+    t = engine.session(lp, _params(K, False, tile_nnz=1024))
+    try:
+        assert t.shape()[0] == 1024
+        t.solve()
+        rt = t.result()
+    finally:
+        t.close()
+    assert np.array_equal(rt.x, r.x) and np.array_equal(rt.y, r.y)
+    assert rt.info.max_primal_viol == r.info.max_primal_viol
+
+
+@pytest.mark.parametrize("name", ["unit_11", "transport", "crude_blending", "equality_100x250", "staircase_T4K5"])
+def test_small_path_stop_rule(engine, name):
+    # The reference's stop rule (iter > 20, iter % 32 == 0, V <= 1e-6) decided
+    # inside the workgroup: same iteration, iterates and status as the oracle.
+    lp = CASES[name]()
+    r = engine.solve(lp, _params(50000, True))
+    xo, yo, _, it, V = Oracle(lp).pdhg(50000, check=True)
+    assert r.iterations == it and r.status == 0
+    assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+    assert r.info.max_primal_viol == V
+
+
+def test_small_path_runs_in_pieces(engine):
+    # Budgets that split chunks anywhere give the same iterates as one run.
+    lp = gen_equality_lp(7, 100, 250, 5)
+    K = 1000
+    s = engine.session(lp, _params(K, False))
+    try:
+        for b in (37, 100, 300, 1, 562):
+            s.run(b)
+        r = s.result()
+    finally:
+        s.close()
+    xo, yo, _, it, _ = Oracle(lp).pdhg(K, check=False)
+    assert r.iterations == K
+    assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+
+
+def test_small_path_warm_start(engine):
+    lp = gen_staircase_lp(11, 4, 5)
+    rng = np.random.default_rng(4)
+    x0 = rng.uniform(0, 1, lp.n)
+    y0 = rng.standard_normal(lp.m)
+    r = engine.solve(lp, _params(500, False), x0=x0, y0=y0)
+    xo, yo, _, _, _ = Oracle(lp).pdhg(500, check=False, x0=x0, y0=y0)
+    assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+
+
+def test_small_path_not_taken_beyond_its_limits(engine):
+    # Too many nonzeros, a row longer than 512, or matrices that do not fit
+    # the LDS beside the vectors past 3,072 nonzeros: the tiled path.
+    for lp in (gen_equality_lp(20260, 2000, 5000, 5), gen_equality_lp(7, 6, 4000, 5),
+               gen_mixed_lp(3, 600, 1800, 6.0, 0.01, 1e3)):
+        s = engine.session(lp, _params(10, False))
+        try:
+            assert s.shape()[0] != 0
+        finally:
+            s.close()
