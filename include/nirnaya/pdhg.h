@@ -177,7 +177,11 @@ NIR_API int nir_calibrate_copy(nir_ctx* ctx, int64_t bytes, double* ms);
 NIR_API int nir_calibrate_gather(nir_ctx* ctx, int64_t n, int32_t len, int32_t with_values, int32_t blocks,
                                  double* us);
 
-/* One-shot: create, solve, copy out, destroy. */
+/* One-shot: create, solve, copy out, destroy. In pdlp mode an LP of at most
+ * 2,500 nonzeros and 8,192 rows and columns, with no explicit layout and no
+ * unscaling vectors, runs as a batch of one (nir_batch_create_warm, 256
+ * iterations per launch): the same bits as the session path, one persistent
+ * workgroup instead of two launches per iteration. */
 NIR_API int nir_pdhg_solve(nir_ctx* ctx, const nir_lp* lp, const nir_params* p, double* x, double* y,
                            nir_info* info);
 

@@ -497,6 +497,28 @@ class Engine:
     def session(self, lp: LP, params: Optional[NirParams] = None, x0=None, y0=None) -> "Session":
         return Session(self, lp, params, x0, y0)
 
+    def solve_oneshot(self, lp: LP, params: Optional[NirParams] = None, x0=None, y0=None) -> Result:
+        """nir_pdhg_solve: the one-shot entry point the facade calls (small
+        pdlp-mode LPs run there as a batch of one)."""
+        prm = NirParams.from_buffer_copy(params) if params is not None else default_params()
+        keep = []
+        if x0 is not None:
+            x0 = np.ascontiguousarray(x0, dtype=np.float64)
+            keep.append(x0)
+            prm.x0 = _ptr(x0, C.c_double)
+        if y0 is not None:
+            y0 = np.ascontiguousarray(y0, dtype=np.float64)
+            keep.append(y0)
+            prm.y0 = _ptr(y0, C.c_double)
+        lps = lp.struct()
+        x = np.zeros(lp.n)
+        y = np.zeros(lp.m)
+        info = NirInfo()
+        if self.lib.nir_pdhg_solve(self.h, C.byref(lps), C.byref(prm), _ptr(x, C.c_double), _ptr(y, C.c_double),
+                                   C.byref(info)) != 0:
+            raise EngineError("nir_pdhg_solve: " + self.error())
+        return Result(x, y, info.status, info.iterations, info.objective, info.tau, info.sigma, info)
+
     def solve(self, lp: LP, params: Optional[NirParams] = None, x0=None, y0=None) -> Result:
         s = self.session(lp, params, x0, y0)
         try:

@@ -155,3 +155,42 @@ def test_small_path_not_taken_beyond_its_limits(engine):
             assert s.shape()[0] != 0
         finally:
             s.close()
+
+
+# ---- pdlp mode: small LPs through the one-shot entry point run as a batch of one
+
+PDLP_CASES = ["unit_11", "transport", "crude_blending", "edges", "equality_100x250", "staircase_T4K5"]
+
+
+def _same_result(a, b):
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+    for f in ("status", "iterations", "restarts", "polish_iterations", "objective", "tau", "sigma",
+              "primal_weight", "rel_primal", "rel_dual", "rel_gap"):
+        assert getattr(a.info, f) == getattr(b.info, f), f
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("name", PDLP_CASES)
+def test_pdlp_oneshot_small_equals_session(engine, name, prec):
+    # nir_pdhg_solve routes these LPs to the batch engine; the session path
+    # (what nir_pdhg_solve runs for larger LPs) must give the same bits.
+    from nirnaya_amd.engine import MODE_PDLP
+    lp = CASES[name]()
+    p = default_params(mode=MODE_PDLP, precision=prec, max_iters=200000, tol=1e-6,
+                       polish_tol=1e-8 if prec else 0.0)
+    one = engine.solve_oneshot(lp, p)
+    ses = engine.solve(lp, p)
+    _same_result(one, ses)
+    if name in ("unit_11", "transport", "crude_blending", "staircase_T4K5"):
+        assert one.status == 0
+
+
+def test_pdlp_oneshot_small_warm_start_and_cap(engine):
+    from nirnaya_amd.engine import MODE_PDLP
+    lp = gen_staircase_lp(11, 4, 5)
+    rng = np.random.default_rng(9)
+    x0 = rng.uniform(0, 1, lp.n)
+    y0 = rng.standard_normal(lp.m)
+    for cap in (100, 1000, 200000):
+        p = default_params(mode=MODE_PDLP, max_iters=cap, tol=1e-6)
+        _same_result(engine.solve_oneshot(lp, p, x0=x0, y0=y0), engine.solve(lp, p, x0=x0, y0=y0))
