@@ -288,6 +288,9 @@ __global__ __launch_bounds__(1024) void k_ref_small(int m, int n, const int* __r
         V = red[0];
         for (int w = 1; w < nw; ++w) V = amax(V, red[w]);
         __syncthreads();
+        // The violation slot the tiled path's stop rule reads (k_vmax_reduce),
+        // kept current so that a session may continue on either path.
+        if ((k & 31) == 0 && tid == 0) st->vslot[k & 1] = static_cast<unsigned long long>(__double_as_longlong(V));
         if (check && k > 20 && (k & 31) == 0 && V <= 1e-6) {
             ++k;
             if (tid == 0) stop = 1;

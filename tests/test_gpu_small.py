@@ -135,6 +135,25 @@ def test_small_path_runs_in_pieces(engine):
     assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
 
 
+def test_small_path_continues_on_the_tiled_path(engine):
+    # 40 iterations on the small path, then 60 on the tiled kernels
+    # (nir_session_time runs them), stop rule armed: the violation slots the
+    # tiled stop rule reads are the small path's, so the run equals the
+    # oracle's 100 iterations.
+    lp = _crude_blending()
+    s = engine.session(lp, _params(100, True))
+    try:
+        assert s.shape()[0] == 0
+        s.run(40)
+        s.time(30)
+        r = s.result()
+    finally:
+        s.close()
+    xo, yo, _, it, _ = Oracle(lp).pdhg(100, check=True)
+    assert it == 100 and r.iterations == 100
+    assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+
+
 def test_small_path_warm_start(engine):
     lp = gen_staircase_lp(11, 4, 5)
     rng = np.random.default_rng(4)
