@@ -66,6 +66,7 @@
 #include "lpcheck.hpp"
 #include "pdlp.hpp"
 #include "pdlp_core.hpp"
+#include "trace.hpp"
 
 namespace nir {
 namespace gpu {
@@ -1201,6 +1202,7 @@ NIR_API int nir_batch_create(nir_ctx* ctx, const nir_lp* const* lps, int32_t cou
 NIR_API int nir_batch_create_warm(nir_ctx* ctx, const nir_lp* const* lps, int32_t count,
                                   const double* const* x0s, const double* const* y0s, const nir_params* p,
                                   const nir_batch_opts* o, nir_batch** out) {
+    const nir::TraceRange trace("nir_batch_create");
     if (!ctx || !out) return -1;
     *out = nullptr;
     auto t0 = std::chrono::steady_clock::now();
@@ -1464,6 +1466,7 @@ NIR_API int nir_batch_create_warm(nir_ctx* ctx, const nir_lp* const* lps, int32_
 }
 
 NIR_API int nir_batch_solve(nir_batch* B) {
+    const nir::TraceRange trace("nir_batch_solve");
     if (!B) return -1;
     try {
         HIP_CHECK(hipSetDevice(B->ctx->device));

@@ -37,6 +37,7 @@
 #include "ctx.hpp"
 #include "lpcheck.hpp"
 #include "pdlp.hpp"
+#include "trace.hpp"
 #include "verify.hpp"
 
 namespace nir {
@@ -869,6 +870,7 @@ NIR_API int nir_ctx_create(int device, nir_ctx** out) {
             delete c;
             throw HipError(std::string("hipStreamCreate: ") + hipGetErrorString(e));
         }
+        roctxNameHipStream("nirnaya engine stream", c->stream);
         *out = c;
         return 0;
     } catch (const std::exception& ex) {
@@ -887,6 +889,7 @@ NIR_API void nir_ctx_destroy(nir_ctx* ctx) {
 NIR_API const char* nir_ctx_error(const nir_ctx* ctx) { return ctx ? ctx->error.c_str() : g_last_error.c_str(); }
 
 NIR_API int nir_session_create(nir_ctx* ctx, const nir_lp* lp, const nir_params* p, nir_session** out) {
+    const nir::TraceRange trace("nir_session_create");
     if (!ctx || !out) return -1;
     *out = nullptr;
     auto t0 = std::chrono::steady_clock::now();
@@ -987,6 +990,7 @@ NIR_API int nir_session_create(nir_ctx* ctx, const nir_lp* lp, const nir_params*
 }
 
 NIR_API int nir_session_run(nir_session* S, int64_t iters) {
+    const nir::TraceRange trace("nir_session_run");
     if (!S) return -1;
     try {
         HIP_CHECK(hipSetDevice(S->ctx->device));
@@ -1273,6 +1277,7 @@ NIR_API void nir_session_destroy(nir_session* S) {
 
 NIR_API int nir_pdhg_solve(nir_ctx* ctx, const nir_lp* lp, const nir_params* p, double* x, double* y,
                            nir_info* info) {
+    const nir::TraceRange trace("nir_pdhg_solve");
     if (small_pdlp(lp, p)) {
         // A small LP in pdlp mode runs as a batch of one: a persistent
         // workgroup iterates it from LDS-staged operands instead of two

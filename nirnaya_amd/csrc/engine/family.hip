@@ -60,6 +60,7 @@
 #include "pdlp.hpp"
 #include "pdlp_core.hpp"
 #include "pod.hpp"
+#include "trace.hpp"
 
 namespace nir {
 namespace gpu {
@@ -946,6 +947,7 @@ NIR_API void nir_family_opts_default(nir_family_opts* o) {
 
 NIR_API int nir_family_create(nir_ctx* ctx, const nir_lp* base, const nir_params* p, const nir_family_opts* o,
                               nir_family** out) {
+    const nir::TraceRange trace("nir_family_create");
     if (!ctx || !out) return -1;
     *out = nullptr;
     auto t0 = std::chrono::steady_clock::now();
@@ -1075,6 +1077,7 @@ NIR_API int nir_family_create(nir_ctx* ctx, const nir_lp* base, const nir_params
 
 NIR_API int nir_family_solve(nir_family* F, int64_t count, nir_family_fill_fn fill, nir_family_done_fn done,
                              void* user) {
+    const nir::TraceRange trace("nir_family_solve");
     if (!F || !fill || !done || count < 0) return -1;
     try {
         HIP_CHECK(hipSetDevice(F->ctx->device));

@@ -30,6 +30,7 @@
 #include "../../../include/nirnaya/pdhg.h"
 #include "ctx.hpp"
 #include "devmat.hpp"
+#include "trace.hpp"
 
 namespace nir {
 namespace gpu {
@@ -630,6 +631,7 @@ NIR_API int nir_mlp_set_data(nir_mlp* P, const float* x, const float* target, in
 // Adam (beta 0.9 / 0.999, eps 1e-8) over shuffled mini-batches; epoch_loss
 // (optional, `epochs` entries) receives the mean training loss per epoch.
 NIR_API int nir_mlp_train(nir_mlp* P, int32_t epochs, int32_t batch, float lr, uint64_t seed, float* epoch_loss) {
+    const nir::TraceRange trace("nir_mlp_train");
     if (!P || epochs < 0) return -1;
     try {
         HIP_CHECK(hipSetDevice(P->ctx->device));
@@ -734,6 +736,7 @@ NIR_API int nir_mlp_gemm_check(nir_ctx* ctx, int32_t M, int32_t N, int32_t K, co
 // already mean-centred by the caller). nir_pod_gram: G = X^T X (N x N).
 // nir_pod_project: Y = X V for V [N][R] (Y [len][R]).
 NIR_API int nir_pod_gram(nir_ctx* ctx, const double* X, int32_t len, int32_t N, double* G) {
+    const nir::TraceRange trace("nir_pod_gram");
     if (!ctx || !X || !G || len < 0 || N <= 0) return -1;
     rocblas_handle h = nullptr;
     try {
@@ -802,6 +805,7 @@ NIR_API int nir_pod_project(nir_ctx* ctx, const double* X, int32_t len, int32_t 
 
 NIR_API int nir_pod_create(nir_ctx* ctx, const double* X, int32_t len, int32_t N, const double* V, int32_t R,
                            const double* mean, nir_pod** out) {
+    const nir::TraceRange trace("nir_pod_create");
     if (!ctx || !X || !V || !mean || !out || len <= 0 || N <= 0 || R <= 0) return -1;
     *out = nullptr;
     auto P = std::make_unique<nir_pod>();
@@ -852,6 +856,7 @@ NIR_API int nir_pod_modes(nir_pod* P, double* modes) {
 // (len x count) = M^T (len x R) alpha^T (R x count), one DGEMM onto the
 // broadcast mean.
 NIR_API int nir_pod_decode(nir_pod* P, const double* alpha, int32_t count, double* out) {
+    const nir::TraceRange trace("nir_pod_decode");
     if (!P || !alpha || count <= 0) return -1;
     try {
         HIP_CHECK(hipSetDevice(P->ctx->device));

@@ -32,6 +32,7 @@
 #include "lpcheck.hpp"
 #include "pdlp.hpp"
 #include "pdlp_core.hpp"
+#include "trace.hpp"
 #include "verify.hpp"
 
 namespace nir {
@@ -348,6 +349,7 @@ extern "C" {
 
 NIR_API int nir_verify_solution(nir_ctx* ctx, const nir_lp* lp, const double* x, double claimed_obj, double tol,
                                 nir_verify_report* out) {
+    const nir::TraceRange trace("nir_verify_solution");
     if (!ctx || !out) return -1;
     try {
         check_lp(lp);

@@ -6,6 +6,7 @@
 #include <fstream>
 
 #include "../../../include/nirnaya/pdhg.h"
+#include "../engine/trace.hpp"
 
 namespace nir {
 namespace {
@@ -24,6 +25,7 @@ struct EngineRun {
 // call site of the reference's PdhgSolver::solve (solver.hpp:235-244).
 EngineRun run_engine(const Model& lp, const Options& opt, const Deadline& deadline, const Scaling& sc,
                      const std::vector<double>* wx, const std::vector<double>* wy, double primal_weight = 0.0) {
+    const TraceRange trace("sov engine");
     EngineRun r;
     nir_lp v{};
     v.nrow = lp.nrow;
@@ -110,6 +112,7 @@ EngineRun polish_feasibility(const Model& lp, const Options& opt, const Deadline
 constexpr long long kDeviceVerifyNnz = 1ll << 20;
 
 CheckReport certify(const Model& m, const std::vector<double>& x, double claimed, double tol, int device) {
+    const TraceRange trace("sov certify");
     if (m.A.nnz() < kDeviceVerifyNnz || m.num_integers() > 0 || static_cast<Idx>(x.size()) != m.ncol)
         return verify_solution(m, x, claimed, tol);
     CheckReport rep;
@@ -183,7 +186,10 @@ SolveResult Solver::solve(const Model& user_model) {
     log_msg(L, 1, "========================================================================\n");
 
     const Model internal = valid.to_internal_min();
-    Presolved pre = presolve(internal, options.presolve, options.presolve_rounds);
+    Presolved pre = [&] {
+        const TraceRange trace("sov presolve");
+        return presolve(internal, options.presolve, options.presolve_rounds);
+    }();
     const double t_presolve = deadline.elapsed();
     if (options.presolve > 0 && (pre.rows_removed > 0 || pre.cols_removed > 0 || pre.tightenings > 0))
         log_msg(L, 1, "Presolve: %d rows, %d cols remaining (removed %d rows, %d cols, %d tightenings)\n",
@@ -208,7 +214,10 @@ SolveResult Solver::solve(const Model& user_model) {
     // > 3.8x), so there it only runs when pdhg_host_scaling asks for it.
     const bool host_scale = options.scale && (options.pdhg.mode != "pdlp" || options.pdhg.host_scaling != 0);
     Scaling sc;
-    const Model scaled = scale_geometric(pre.reduced, host_scale, sc);
+    const Model scaled = [&] {
+        const TraceRange trace("sov scaling");
+        return scale_geometric(pre.reduced, host_scale, sc);
+    }();
     const double t_scaled = deadline.elapsed();
     double t_engine = t_scaled;
     std::vector<double> x_red(static_cast<size_t>(pre.reduced.ncol), 0.0);
