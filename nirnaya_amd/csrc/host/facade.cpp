@@ -261,6 +261,8 @@ SolveResult Solver::solve(const Model& user_model) {
             out.rel_primal = er.info.rel_primal;
             out.rel_dual = er.info.rel_dual;
             out.rel_gap = er.info.rel_gap;
+            out.restarts = er.info.restarts;
+            out.primal_weight = er.info.primal_weight;
             if (out.status != Status::Optimal) break;
             sc.unscale_primal(er.x, x_red);
             sc.unscale_duals(er.y, y_red);
@@ -354,6 +356,12 @@ SolveResult Solver::solve(const Model& user_model) {
                 out.verification.max_int_viol);
     }
     log_msg(L, 1, "  Iterations: %d | Time: %.4f s\n", out.iterations, out.solve_time_sec);
+    if (out.engine_iterations > 0 && options.pdhg.mode == "pdlp")
+        log_msg(L, 1, "  PDHG:       rel. KKT primal %.2e dual %.2e gap %.2e | restarts %d | primal weight %.4g\n",
+                out.rel_primal, out.rel_dual, out.rel_gap, out.restarts, out.primal_weight);
+    if (out.engine_iterations > 0 && out.engine_seconds > 0.0)
+        log_msg(L, 1, "  Engine:     %lld iterations in %.4f s (%.0f iterations/s, set-up included)\n",
+                out.engine_iterations, out.engine_seconds, out.engine_iterations / out.engine_seconds);
     log_msg(L, 1, "========================================================================\n");
 
     if (!options.sol_path.empty() && out.has_incumbent)
@@ -379,6 +387,9 @@ bool Solver::write_stats_json(const std::string& path, const Model& m, const Sol
     std::fprintf(f, "  \"solve_time_sec\": %.9g,\n", r.solve_time_sec);
     std::fprintf(f, "  \"engine_iterations\": %lld,\n", r.engine_iterations);
     std::fprintf(f, "  \"engine_seconds\": %.9g,\n", r.engine_seconds);
+    std::fprintf(f, "  \"rel_kkt\": [%.9g, %.9g, %.9g],\n", fin(r.rel_primal), fin(r.rel_dual), fin(r.rel_gap));
+    std::fprintf(f, "  \"restarts\": %d,\n", r.restarts);
+    std::fprintf(f, "  \"primal_weight\": %.9g,\n", fin(r.primal_weight));
     std::fprintf(f, "  \"verified\": %s\n", r.verification.passed ? "true" : "false");
     std::fprintf(f, "}\n");
     std::fclose(f);

@@ -36,6 +36,8 @@ struct SolveResult {
     long long engine_iterations = 0;
     double engine_seconds = 0.0;
     double rel_primal = 0.0, rel_dual = 0.0, rel_gap = 0.0;
+    int restarts = 0;             // pdlp mode: restarts of the last engine run
+    double primal_weight = 0.0;   // pdlp mode: its final primal weight
 };
 
 class Solver {

@@ -136,6 +136,24 @@ def test_checker_certifies_returned_solution(tmp_path):
         assert rc in (1, 5)
 
 
+def test_cli_reports_engine_metrics(tmp_path):
+    # sovsolve in pdlp mode: the summary carries the relative KKT terms,
+    # restarts, primal weight and the engine's iteration rate, and the stats
+    # JSON the same numbers.
+    import json
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    st = tmp_path / "stats.json"
+    r = subprocess.run([os.path.join(root, "bin", "sovsolve"), lpkit.CRUDE_MPS, "--method=pdhg", "--pdhg-mode=pdlp",
+                        "--stats-json", str(st)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "PDHG:       rel. KKT primal" in r.stdout and "iterations/s" in r.stdout
+    d = json.loads(st.read_text())
+    assert d["status"] == "optimal"
+    assert len(d["rel_kkt"]) == 3 and max(d["rel_kkt"]) <= 1e-6
+    assert d["restarts"] >= 0 and d["primal_weight"] > 0 and d["engine_iterations"] > 0
+
+
 def test_facade_feasibility_polish_makes_the_certificate_pass(tmp_path):
     # A loose relative KKT tolerance (1e-4, l2 over all rows) leaves single
     # rows violated beyond the certificate's 1e-6; the pdlp path then polishes
