@@ -140,41 +140,69 @@ inline bool partition_fused(const std::vector<int>& rp, int nrows, int nseg, int
     return true;
 }
 
-// Upload a CSR matrix (columns ascending within rows) in the segmented
-// layout. `perm` receives, for each stored slot, the input nonzero it holds.
-inline void upload_shape(CsrShape& S, int nrows, int ncols, const std::vector<int>& rowptr, const int* col,
-                         TileShape shape, std::vector<int>& perm, hipStream_t st) {
-    S.nrows = nrows;
-    S.ncols = ncols;
-    S.nnz = rowptr.back();
-    S.shape = shape;
-    const int W = (shape.seg_width > 0 && shape.seg_width < ncols) ? shape.seg_width : std::max(ncols, 1);
-    S.nseg = std::max(1, (ncols + W - 1) / W);
-    const int nseg = S.nseg;
-    std::vector<int> rp(static_cast<size_t>(nseg) * (nrows + 1));
-    perm.assign(static_cast<size_t>(S.nnz), 0);
-    // Count entries per (segment, row), then lay segments out one after another.
+// Host side of a column-segmented layout of a CSR matrix (columns ascending
+// within rows): per segment row pointers (absolute offsets), the stored
+// slots' column indices, and `perm`, the input nonzero each stored slot holds.
+struct SegHost {
+    int nrows = 0, ncols = 0, nnz = 0, nseg = 1;
+    std::vector<int> rp;              // nseg * (nrows + 1)
+    std::vector<int> perm;            // nnz
+    std::vector<int> col;             // nnz, segment-major
+    std::vector<long long> seg_nz;    // nseg + 1
+};
+
+inline SegHost build_segments(int nrows, int ncols, const std::vector<int>& rowptr, const int* col, int seg_width) {
+    SegHost H;
+    H.nrows = nrows;
+    H.ncols = ncols;
+    H.nnz = rowptr.back();
+    const int W = (seg_width > 0 && seg_width < ncols) ? seg_width : std::max(ncols, 1);
+    H.nseg = std::max(1, (ncols + W - 1) / W);
+    const int nseg = H.nseg;
+    H.rp.assign(static_cast<size_t>(nseg) * (nrows + 1), 0);
+    H.perm.assign(static_cast<size_t>(H.nnz), 0);
+    // Count entries per segment, then lay segments out one after another.
     std::vector<long long> seg_count(nseg, 0);
-    for (int p = 0; p < S.nnz; ++p) ++seg_count[col[p] / W];
-    std::vector<long long> seg_base(nseg + 1, 0);
-    for (int s = 0; s < nseg; ++s) seg_base[s + 1] = seg_base[s] + seg_count[s];
-    S.seg_nz = seg_base;
-    std::vector<long long> fill(seg_base.begin(), seg_base.end() - 1);
-    for (int s = 0; s < nseg; ++s) rp[static_cast<size_t>(s) * (nrows + 1)] = static_cast<int>(seg_base[s]);
+    for (int p = 0; p < H.nnz; ++p) ++seg_count[col[p] / W];
+    H.seg_nz.assign(nseg + 1, 0);
+    for (int s = 0; s < nseg; ++s) H.seg_nz[s + 1] = H.seg_nz[s] + seg_count[s];
+    std::vector<long long> fill(H.seg_nz.begin(), H.seg_nz.end() - 1);
+    for (int s = 0; s < nseg; ++s) H.rp[static_cast<size_t>(s) * (nrows + 1)] = static_cast<int>(H.seg_nz[s]);
     for (int r = 0; r < nrows; ++r) {
         for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) {
             const int s = col[p] / W;
-            perm[fill[s]++] = p;
+            H.perm[fill[s]++] = p;
         }
-        for (int s = 0; s < nseg; ++s) rp[static_cast<size_t>(s) * (nrows + 1) + r + 1] = static_cast<int>(fill[s]);
+        for (int s = 0; s < nseg; ++s) H.rp[static_cast<size_t>(s) * (nrows + 1) + r + 1] = static_cast<int>(fill[s]);
     }
-    std::vector<int> c2(static_cast<size_t>(S.nnz));
-    for (int q = 0; q < S.nnz; ++q) c2[q] = col[perm[q]];
+    H.col.resize(static_cast<size_t>(H.nnz));
+    for (int q = 0; q < H.nnz; ++q) H.col[q] = col[H.perm[q]];
+    return H;
+}
+
+// Row pointers and column indices of a layout on the device.
+inline void upload_structure(CsrShape& S, const SegHost& H, hipStream_t st) {
+    S.nrows = H.nrows;
+    S.ncols = H.ncols;
+    S.nnz = H.nnz;
+    S.nseg = H.nseg;
+    S.seg_nz = H.seg_nz;
+    S.rowptr.alloc(H.rp.size());
+    S.rowptr.upload(H.rp.data(), H.rp.size(), st);
+    S.col.alloc(H.col.size());
+    S.col.upload(H.col.data(), H.col.size(), st);
+}
+
+// Row blocks of a layout for one tile shape (per-segment descriptors, and the
+// fused-segment partition when asked for and possible), on the device.
+inline void set_blocks(CsrShape& S, const SegHost& H, TileShape shape, hipStream_t st) {
+    S.shape = shape;
+    const int nseg = H.nseg, nrows = H.nrows;
     std::vector<int4> desc_all;
     S.blk_off.assign(nseg, 0);
     S.nblk.assign(nseg, 0);
     for (int s = 0; s < nseg; ++s) {
-        const int* srp = rp.data() + static_cast<size_t>(s) * (nrows + 1);
+        const int* srp = H.rp.data() + static_cast<size_t>(s) * (nrows + 1);
         std::vector<int> b = partition_rows(srp, nrows, shape.tile, std::min(shape.rows, kThreads));
         S.blk_off[s] = static_cast<int>(desc_all.size());
         S.nblk[s] = static_cast<int>(b.size()) - 1;
@@ -187,17 +215,15 @@ inline void upload_shape(CsrShape& S, int nrows, int ncols, const std::vector<in
             desc_all.push_back(d);
         }
     }
-    S.rowptr.alloc(rp.size());
-    S.rowptr.upload(rp.data(), rp.size(), st);
-    S.col.alloc(c2.size());
-    S.col.upload(c2.data(), c2.size(), st);
     S.desc.alloc(desc_all.size());
     S.desc.upload(desc_all.data(), desc_all.size(), st);
     S.fused_ok = false;
     S.fnblk = 0;
+    S.frows.release();
+    S.fnz.release();
     if (shape.fused && nseg > 1) {
         std::vector<int2> fr, fz;
-        if (partition_fused(rp, nrows, nseg, shape.tile, std::min(shape.rows, kThreads), fr, fz)) {
+        if (partition_fused(H.rp, nrows, nseg, shape.tile, std::min(shape.rows, kThreads), fr, fz)) {
             S.fused_ok = true;
             S.fnblk = static_cast<int>(fr.size());
             S.frows.alloc(fr.size());
@@ -206,7 +232,17 @@ inline void upload_shape(CsrShape& S, int nrows, int ncols, const std::vector<in
             S.fnz.upload(fz.data(), fz.size(), st);
         }
     }
+}
+
+// Upload a CSR matrix (columns ascending within rows) in the segmented
+// layout. `perm` receives, for each stored slot, the input nonzero it holds.
+inline void upload_shape(CsrShape& S, int nrows, int ncols, const std::vector<int>& rowptr, const int* col,
+                         TileShape shape, std::vector<int>& perm, hipStream_t st) {
+    SegHost H = build_segments(nrows, ncols, rowptr, col, shape.seg_width);
+    upload_structure(S, H, st);
+    set_blocks(S, H, shape, st);
     HIP_CHECK(hipStreamSynchronize(st));
+    perm.swap(H.perm);
 }
 
 // A matrix on the device: segmented structure plus its FP64 values and the

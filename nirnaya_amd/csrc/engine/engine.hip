@@ -24,6 +24,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <future>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -397,18 +398,65 @@ void spmv_store(const DevMatrix& M, const double* v, double* out, hipStream_t s)
     spmv_store<double>(M.shape, M.val.p, M.part.p, v, out, s);
 }
 
-// Time 8 products y = M v in the layout `c`, values in precision T.
+// A segment width's host layout and its values permuted into it, in
+// precision T.
 template <typename T>
-float time_layout(int nrows, int ncols, const std::vector<int>& rowptr, const int* col, const double* val,
-                  const TileShape& c, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+struct WidthLayout {
+    SegHost H;
+    std::vector<T> vals;
+};
+
+template <typename T>
+WidthLayout<T> build_width(int nrows, int ncols, const std::vector<int>& rowptr, const int* col, const double* val,
+                           int width) {
+    WidthLayout<T> L;
+    L.H = build_segments(nrows, ncols, rowptr, col, width);
+    L.vals.resize(L.H.perm.size());
+    for (size_t q = 0; q < L.H.perm.size(); ++q) L.vals[q] = static_cast<T>(val[L.H.perm[q]]);
+    std::vector<int>().swap(L.H.perm);   // not needed for timing
+    return L;
+}
+
+// Time 8 products y = M v for every candidate of one segment width: the
+// segmented structure and the permuted values are uploaded once for the
+// width, only the row blocks change per tile shape. Keeps the fastest in
+// `best` / `best_ms`.
+template <typename T>
+void time_width(const WidthLayout<T>& L, const std::vector<TileShape>& cands, hipStream_t s, hipEvent_t e0,
+                hipEvent_t e1, DevArray<T>& part, DevArray<T>& v, DevArray<T>& out, TileShape& best,
+                float& best_ms) {
     CsrShape S;
-    std::vector<int> perm;
-    upload_shape(S, nrows, ncols, rowptr, col, c, perm, s);
-    std::vector<T> hv(perm.size());
-    for (size_t q = 0; q < perm.size(); ++q) hv[q] = static_cast<T>(val[perm[q]]);
-    DevArray<T> dval, part, v, out;
-    dval.alloc(hv.size());
-    dval.upload(hv.data(), hv.size(), s);
+    upload_structure(S, L.H, s);
+    DevArray<T> dval;
+    dval.alloc(L.vals.size());
+    dval.upload(L.vals.data(), L.vals.size(), s);
+    for (const TileShape& c : cands) {
+        // The previous candidate's products have completed (event synchronised
+        // below), so its block arrays can be replaced.
+        set_blocks(S, L.H, c, s);
+        for (int w = 0; w < 2; ++w) spmv_store<T>(S, dval.p, part.p, v.p, out.p, s);
+        HIP_CHECK(hipEventRecord(e0, s));
+        for (int w = 0; w < 8; ++w) spmv_store<T>(S, dval.p, part.p, v.p, out.p, s);
+        HIP_CHECK(hipEventRecord(e1, s));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+template <typename T>
+TileShape time_candidates(int nrows, int ncols, const std::vector<int>& rowptr, const int* col, const double* val,
+                          const std::vector<int>& widths, const std::vector<std::vector<TileShape>>& per_width,
+                          hipStream_t s) {
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    DevArray<T> part, v, out;
     part.alloc(static_cast<size_t>(std::max(nrows, 1)));
     v.alloc(static_cast<size_t>(std::max(ncols, 1)));
     out.alloc(static_cast<size_t>(std::max(nrows, 1)));
@@ -416,14 +464,29 @@ float time_layout(int nrows, int ncols, const std::vector<int>& rowptr, const in
         std::vector<T> ones(static_cast<size_t>(std::max(ncols, 1)), T(1));
         v.upload(ones.data(), ones.size(), s);
     }
-    for (int w = 0; w < 2; ++w) spmv_store<T>(S, dval.p, part.p, v.p, out.p, s);
-    HIP_CHECK(hipEventRecord(e0, s));
-    for (int w = 0; w < 8; ++w) spmv_store<T>(S, dval.p, part.p, v.p, out.p, s);
-    HIP_CHECK(hipEventRecord(e1, s));
-    HIP_CHECK(hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-    return ms;
+    TileShape best = per_width.front().front();
+    float best_ms = 1e30f;
+    // The host layouts of all widths are built concurrently (O(nnz) each);
+    // the products are then timed one width at a time on the stream.
+    std::vector<std::future<WidthLayout<T>>> layouts;
+    for (int w : widths)
+        layouts.push_back(std::async(std::launch::async, [&, w] { return build_width<T>(nrows, ncols, rowptr, col, val, w); }));
+    try {
+        for (size_t k = 0; k < widths.size(); ++k) {
+            const WidthLayout<T> L = layouts[k].get();
+            time_width<T>(L, per_width[k], s, e0, e1, part, v, out, best, best_ms);
+        }
+    } catch (...) {
+        for (auto& f : layouts)
+            if (f.valid()) f.wait();
+        (void)hipStreamSynchronize(s);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
 }
 
 // Autotuner: time y = M v for every candidate layout (segment width, tile,
@@ -459,36 +522,28 @@ TileShape autotune(int nrows, int ncols, const std::vector<int>& rowptr, const i
         auto it = cache.find(key);
         if (it != cache.end()) return it->second;
     }
-    std::vector<TileShape> cands;
     // Power-of-two segment widths plus the even splits in two, three and
     // four (multiples of 1024 columns): equal segments keep every pass's
     // operand slice the same size (an FP32 operand of a few MB per segment
     // fits one XCD's L2 in two passes).
-    std::vector<int> widths = {0, 1 << 19, 1 << 18, 1 << 17};
+    std::vector<int> all = {0, 1 << 19, 1 << 18, 1 << 17};
     for (int parts = 2; parts <= 4; ++parts) {
         const int w = static_cast<int>((static_cast<long long>(ncols) / parts + 1023) / 1024 * 1024);
-        if (w > 0 && std::find(widths.begin(), widths.end(), w) == widths.end()) widths.push_back(w);
+        if (w > 0 && std::find(all.begin(), all.end(), w) == all.end()) all.push_back(w);
     }
-    for (int w : widths)
-        if (w == 0 || w < ncols)
-            for (int t : {1024, 2048, 4096})
-                for (int f : {0, 1})
-                    if (f == 0 || (w > 0 && allow_fused)) cands.push_back(TileShape{t, 256, w, f});
-    hipEvent_t e0, e1;
-    HIP_CHECK(hipEventCreate(&e0));
-    HIP_CHECK(hipEventCreate(&e1));
-    TileShape best = cands.front();
-    float best_ms = 1e30f;
-    for (const TileShape& c : cands) {
-        const float ms = fp32 ? time_layout<float>(nrows, ncols, rowptr, col, val, c, s, e0, e1)
-                              : time_layout<double>(nrows, ncols, rowptr, col, val, c, s, e0, e1);
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = c;
-        }
+    std::vector<int> widths;
+    std::vector<std::vector<TileShape>> per_width;
+    for (int w : all) {
+        if (w != 0 && w >= ncols) continue;
+        std::vector<TileShape> cands;
+        for (int t : {1024, 2048, 4096})
+            for (int f : {0, 1})
+                if (f == 0 || (w > 0 && allow_fused)) cands.push_back(TileShape{t, 256, w, f});
+        widths.push_back(w);
+        per_width.push_back(std::move(cands));
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    const TileShape best = fp32 ? time_candidates<float>(nrows, ncols, rowptr, col, val, widths, per_width, s)
+                                : time_candidates<double>(nrows, ncols, rowptr, col, val, widths, per_width, s);
     std::lock_guard<std::mutex> lock(mu);
     cache[key] = best;
     return best;
