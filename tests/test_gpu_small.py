@@ -164,6 +164,34 @@ def test_small_path_warm_start(engine):
     assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
 
 
+def _no_rows_lp():
+    rng = np.random.default_rng(2)
+    n = 40
+    return LP(0, n, np.zeros(n + 1, dtype=np.int64), np.zeros(0, dtype=np.int32), np.zeros(0),
+              rng.uniform(-1, 1, n), np.zeros(n), np.full(n, 3.0), np.zeros(0), np.zeros(0))
+
+
+def _empty_matrix_lp():
+    rng = np.random.default_rng(3)
+    m, n = 7, 12
+    return LP(m, n, np.zeros(n + 1, dtype=np.int64), np.zeros(0, dtype=np.int32), np.zeros(0),
+              rng.uniform(-1, 1, n), -np.ones(n), np.ones(n), np.full(m, -1.0), np.full(m, 1.0))
+
+
+@pytest.mark.parametrize("make", [_no_rows_lp, _empty_matrix_lp], ids=["no_rows", "empty_matrix"])
+def test_small_path_degenerate_shapes(engine, make):
+    # No rows at all, or rows and columns without a single nonzero: both paths
+    # and the oracle agree, stop rule included.
+    lp = make()
+    for check, K in ((False, 100), (True, 50000)):
+        r = engine.solve(lp, _params(K, check))
+        rt = engine.solve(lp, _params(K, check, tile_nnz=1024))
+        xo, yo, _, it, V = Oracle(lp).pdhg(K, check=check)
+        assert r.iterations == rt.iterations == it
+        assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+        assert np.array_equal(rt.x, xo) and np.array_equal(rt.y, yo)
+
+
 def test_small_path_not_taken_beyond_its_limits(engine):
     # Too many nonzeros, a row longer than 512, or matrices that do not fit
     # the LDS beside the vectors past 3,072 nonzeros: the tiled path.
