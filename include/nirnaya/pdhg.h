@@ -61,7 +61,9 @@ typedef struct {
                                   -1 = autotune tile, rows and segments per matrix; 0 = autotune
                                   when nnz >= 2^21 and seg_width is 0, else 2048) */
     int32_t tile_rows;         /* SpMV tile: max rows per workgroup (0 = default 256) */
-    int32_t seg_width;         /* column-segment width of the A xbar products (0 = one segment) */
+    int32_t seg_width;         /* column-segment width of the A xbar products (0 = one segment;
+                                  pdlp mode keeps one segment for a matrix with a row of more
+                                  than 4,096 nonzeros, whose products it sums in chunks) */
     int32_t seg_width_t;       /* the same for the A^T y products (-1 = seg_width) */
     const double* x0;          /* optional warm start (ncol), NULL = cold */
     const double* y0;          /* optional warm start (nrow), NULL = cold */

@@ -1239,6 +1239,11 @@ NIR_API int nir_batch_create_warm(nir_ctx* ctx, const nir_lp* const* lps, int32_
             if (lp->nrow >= kMaxEntries || lp->ncol >= kMaxEntries ||
                 (lp->ncol > 0 && lp->col_start[lp->ncol] >= kMaxEntries))
                 throw std::invalid_argument("batch instances are limited to 2^28 rows, columns and nonzeros each");
+            // The persistent kernel sums each row on one lane in sequential
+            // order, which is pdlp mode's order up to kLongRow nonzeros only.
+            if (longest_line(lp) > kLongRow)
+                throw std::invalid_argument("batch instances with a row or column of more than 4096 nonzeros are not "
+                                            "supported: solve them with the single-LP engine");
             Inst& I = B->h_inst[i];
             I.m = lp->nrow;
             I.n = lp->ncol;

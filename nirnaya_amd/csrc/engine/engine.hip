@@ -512,10 +512,11 @@ unsigned long long layout_signature(int nrows, const std::vector<int>& rowptr, c
 }
 
 TileShape autotune(int nrows, int ncols, const std::vector<int>& rowptr, const int* col, const double* val,
-                   hipStream_t s, bool fp32, bool allow_fused) {
+                   hipStream_t s, bool fp32, bool allow_fused, bool allow_segments) {
     static std::mutex mu;
     static std::map<std::tuple<int, int, int, int, unsigned long long>, TileShape> cache;
-    const auto key = std::make_tuple(nrows, ncols, rowptr.back(), (fp32 ? 1 : 0) + (allow_fused ? 2 : 0),
+    const auto key = std::make_tuple(nrows, ncols, rowptr.back(),
+                                     (fp32 ? 1 : 0) + (allow_fused ? 2 : 0) + (allow_segments ? 4 : 0),
                                      layout_signature(nrows, rowptr, col));
     {
         std::lock_guard<std::mutex> lock(mu);
@@ -534,7 +535,7 @@ TileShape autotune(int nrows, int ncols, const std::vector<int>& rowptr, const i
     std::vector<int> widths;
     std::vector<std::vector<TileShape>> per_width;
     for (int w : all) {
-        if (w != 0 && w >= ncols) continue;
+        if (w != 0 && (w >= ncols || !allow_segments)) continue;
         std::vector<TileShape> cands;
         for (int t : {1024, 2048, 4096})
             for (int f : {0, 1})
@@ -787,6 +788,12 @@ void reference_step_sizes(nir_session* S) {
     S->sigma = 0.85 / std::max(1e-6, smax);
 }
 
+int max_row_length(const std::vector<int>& rowptr) {
+    int most = 0;
+    for (size_t r = 0; r + 1 < rowptr.size(); ++r) most = std::max(most, rowptr[r + 1] - rowptr[r]);
+    return most;
+}
+
 // Small LPs run their iterations in one workgroup (k_ref_small) when the
 // default layout is in use (no explicit tile or segments: those ask for the
 // tiled kernels), x, xbar and y fit its LDS and no row of A or A^T is long
@@ -988,8 +995,15 @@ NIR_API int nir_session_create(nir_ctx* ctx, const nir_lp* lp, const nir_params*
         // iterates in FP32 in mixed precision: time what will run.
         const bool pdlp = S->prm.mode == NIR_MODE_PDLP;
         const bool fp32 = pdlp && S->prm.precision == NIR_PREC_MIXED;
-        const TileShape ta = tune ? autotune(m, n, a_ptr, a_col.data(), a_val.data(), S->s, fp32, !pdlp) : base;
-        const TileShape tt = tune ? autotune(n, m, at_ptr, lp->row_index, lp->value, S->s, fp32, !pdlp) : base_t;
+        // pdlp mode sums rows of more than kLongRow nonzeros in the chunked
+        // order (spmv.hpp), which a column-segmented layout would cut; a
+        // matrix with such rows keeps one segment.
+        const bool seg_a = !pdlp || max_row_length(a_ptr) <= kLongRow;
+        const bool seg_t = !pdlp || max_row_length(at_ptr) <= kLongRow;
+        TileShape ta = tune ? autotune(m, n, a_ptr, a_col.data(), a_val.data(), S->s, fp32, !pdlp, seg_a) : base;
+        TileShape tt = tune ? autotune(n, m, at_ptr, lp->row_index, lp->value, S->s, fp32, !pdlp, seg_t) : base_t;
+        if (!seg_a) ta.seg_width = 0, ta.fused = 0;
+        if (!seg_t) tt.seg_width = 0, tt.fused = 0;
         upload_matrix(S->A, m, n, a_ptr, a_col.data(), a_val.data(), ta, S->s);
         upload_matrix(S->AT, n, m, at_ptr, lp->row_index, lp->value, tt, S->s);
 

@@ -974,6 +974,11 @@ NIR_API int nir_family_create(nir_ctx* ctx, const nir_lp* base, const nir_params
         F->n = n;
         F->nnz = n > 0 ? base->col_start[n] : 0;
         if (F->nnz >= (1LL << 31)) throw std::invalid_argument("family matrix has 2^31 or more nonzeros");
+        // Each lane sums a row in sequential order, which is pdlp mode's order
+        // up to kLongRow nonzeros only.
+        if (longest_line(base) > kLongRow)
+            throw std::invalid_argument("family matrices with a row or column of more than 4096 nonzeros are not "
+                                        "supported: solve the members with the single-LP engine");
         F->col_start.assign(base->col_start, base->col_start + n + 1);
         F->row_index.assign(base->row_index, base->row_index + F->nnz);
         F->value.assign(base->value, base->value + F->nnz);

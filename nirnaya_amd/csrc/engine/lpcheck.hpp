@@ -1,6 +1,7 @@
 // Validation of a caller's LP (every index range-checked once on the host:
 // the kernels trust them) and the counting transpose CSC(A) -> CSR(A).
 #pragma once
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <vector>
@@ -24,6 +25,17 @@ inline void check_lp(const nir_lp* lp) {
         if (lp->col_start[j + 1] < lp->col_start[j]) throw std::invalid_argument("col_start not non-decreasing");
     for (int64_t p = 0; p < nnz; ++p)
         if (lp->row_index[p] < 0 || lp->row_index[p] >= lp->nrow) throw std::invalid_argument("row index out of range");
+}
+
+// Most nonzeros in one row and in one column of A (row counts by a pass over
+// the row indices).
+inline int64_t longest_line(const nir_lp* lp) {
+    int64_t most = 0;
+    for (int32_t j = 0; j < lp->ncol; ++j) most = std::max<int64_t>(most, lp->col_start[j + 1] - lp->col_start[j]);
+    std::vector<int64_t> rc(static_cast<size_t>(std::max(lp->nrow, 0)), 0);
+    const int64_t nnz = lp->ncol > 0 ? lp->col_start[lp->ncol] : 0;
+    for (int64_t p = 0; p < nnz; ++p) most = std::max(most, ++rc[lp->row_index[p]]);
+    return most;
 }
 
 // CSR of A from the caller's CSC by a counting transpose; column indices come

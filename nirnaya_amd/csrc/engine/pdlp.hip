@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kThreads) void k_primal_pdlp(Pass<T> P, const T* __
         if (P.last) o = Ops{x[j], c[j], lo[j], up[j], kDelta ? T(0) : xa[j]};
         return o;
     };
-    stream_rows<TILE>(P, y, prod, blockIdx.x, pre, [&](int j, T g, const Ops& o) {
+    stream_rows<TILE, 1>(P, y, prod, blockIdx.x, pre, [&](int j, T g, const Ops& o) {
         const PrimalOut<T> r = primal_point<T>(o.x, g, o.c, o.l, o.u, o.a, tau, hw);
         xbar[j] = r.xb;
         x[j] = r.next;
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(kThreads) void k_dual_pdlp(Pass<T> P, const T* __re
     };
     // FP32 runs the re-centred form; `ya` then holds sa = -Y / sigma.
     constexpr bool kDelta = std::is_same<T, float>::value;
-    stream_rows<TILE>(P, xbar, prod, blockIdx.x, pre, [&](int i, T h, const Ops& o) {
+    stream_rows<TILE, 1>(P, xbar, prod, blockIdx.x, pre, [&](int i, T h, const Ops& o) {
         const DualOut<T> r = kDelta ? dual_point_delta<T>(o.y, h, o.l, o.u, o.a, sigma, hw)
                                     : dual_point<T>(o.y, h, o.l, o.u, o.a, sigma, hw);
         y[i] = r.next;
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(kThreads) void k_eval_cols(Pass<double> P, const TI
     dd acc[5] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
     // Re-centred iterates (anchor != null): T(z) = anchor + delta and
     // A^T yp = base + A^T dyp.
-    stream_rows<TILE>(P, yp, prod, blockIdx.x, [&](int j, double g) {
+    stream_rows<TILE, 1>(P, yp, prod, blockIdx.x, [&](int j, double g) {
         if (anchor)
             col_terms(base[j] + g, c[j], lo[j], up[j], cscale[j], anchor[j] + static_cast<double>(xp[j]), anchor[j],
                       acc);
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(kThreads) void k_eval_rows(Pass<double> P, const TI
     __shared__ dd sh[6 * kThreads];
     if (st->done && !force) return;
     dd acc[6] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
-    stream_rows<TILE>(P, xp, prod, blockIdx.x, [&](int i, double h) {
+    stream_rows<TILE, 1>(P, xp, prod, blockIdx.x, [&](int i, double h) {
         if (anchor)
             row_terms(base[i] + h, rl[i], ru[i], rscale[i], anchor[i] + static_cast<double>(yp[i]), anchor[i], acc);
         else
@@ -314,7 +314,7 @@ template <int TILE>
 __global__ __launch_bounds__(kThreads) void k_spmv_plain(Pass<double> P, const double* __restrict__ v,
                                                          double* __restrict__ out) {
     __shared__ double prod[lds_slots(TILE)];
-    stream_rows<TILE>(P, v, prod, blockIdx.x, [&](int r, double s) { out[r] = s; });
+    stream_rows<TILE, 1>(P, v, prod, blockIdx.x, [&](int r, double s) { out[r] = s; });
 }
 
 // y = M v only when the decision restarted (or `force`): the FP64 anchor
@@ -325,7 +325,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv_on_restart(Pass<double> P, co
                                                               const PdlpState* __restrict__ st, int force) {
     __shared__ double prod[lds_slots(TILE)];
     if (!force && (st->done || !st->restart)) return;
-    stream_rows<TILE>(P, v, prod, blockIdx.x, [&](int r, double s) { out[r] = s; });
+    stream_rows<TILE, 1>(P, v, prod, blockIdx.x, [&](int r, double s) { out[r] = s; });
 }
 
 // Re-centring, step 1: the anchor moves to T(z) = anchor + delta (FP64).

@@ -82,10 +82,17 @@ struct PreNone {
 // and value 0 instead, gather v[0] (the dense operand also has at least one
 // element), and store into LDS slots the row phase never reads; so all loads
 // and gathers of a lane are in flight together.
-template <int TILE, typename T, typename V, typename Pre, typename F>
+// Rows longer than this many nonzeros are summed in the chunked order when
+// ORDER == 1 (see stream_rows).
+constexpr int kLongRow = 4096;
+constexpr int kChunk = 64;        // products per chunk
+constexpr int kGroupChunks = 32;  // chunks per group (2048 products)
+
+template <int TILE, int ORDER = 0, typename T, typename V, typename Pre, typename F>
 __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restrict__ v, T* prod, int b,
                                             Pre&& prefetch, F&& on_row) {
     static_assert(TILE % kThreads == 0, "tile must be a multiple of the workgroup");
+    static_assert(TILE % kChunk == 0 && TILE / kChunk <= 64, "tiles hold whole chunks");
     const Csr<T>& A = P.M;
     const int tid = threadIdx.x;
     const int4 d = A.desc[b];   // one load instead of blk -> rowptr chained loads
@@ -114,6 +121,18 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
                 vi[u] = in ? vl : T(0);
             }
         };
+        // ORDER == 1 and more than kLongRow nonzeros (pdlp mode; such rows are
+        // never split by column segments, so the row starts at p0): the
+        // chunked order. Products are summed from 0.0 in chunks of kChunk
+        // consecutive positions, chunk sums from 0.0 in groups of
+        // kGroupChunks, group sums from 0.0 into the row sum - each level
+        // sequential, so 32 (or 64) lanes sum a tile's chunks at once and
+        // lane 0 adds only one value per chunk. Rows up to kLongRow nonzeros
+        // keep the sequential order in every mode.
+        const bool chunked = ORDER == 1 && cnt > kLongRow;
+        __shared__ T chunk_sum[TILE / kChunk];
+        T group = T(0);     // lane 0: the current group's partial sum
+        int in_group = 0;   // chunks added to it
         load(p0);
         for (int c0 = p0; c0 < p1; c0 += TILE) {
             T g[PER];
@@ -123,13 +142,44 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
             for (int u = 0; u < PER; ++u) prod[lds_at(tid + u * kThreads)] = vi[u] * g[u];
             __syncthreads();
             if (c0 + TILE < p1) load(c0 + TILE);
-            if (tid == 0) {
+            if (chunked) {
                 const int e = p1 - c0 < TILE ? p1 - c0 : TILE;
-                for (int q = 0; q < e; ++q) s += prod[lds_at(q)];
+                const int nch = (e + kChunk - 1) / kChunk;
+                if (tid < nch) {
+                    const int q1 = (tid + 1) * kChunk < e ? (tid + 1) * kChunk : e;
+                    T cs = T(0);
+                    for (int q = tid * kChunk; q < q1; ++q) cs += prod[lds_at(q)];
+                    chunk_sum[tid] = cs;
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    for (int k = 0; k < nch; ++k) {
+                        group += chunk_sum[k];
+                        if (++in_group == kGroupChunks) {
+                            s += group;
+                            group = T(0);
+                            in_group = 0;
+                        }
+                    }
+                }
+            } else if (tid == 0) {
+                // The same sequential sum, eight LDS reads in flight per step
+                // so that the chain of adds, not the reads, sets the pace.
+                const int e = p1 - c0 < TILE ? p1 - c0 : TILE;
+                int q = 0;
+                for (; q + 8 <= e; q += 8) {
+                    T a[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) a[k] = prod[lds_at(q + k)];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) s += a[k];
+                }
+                for (; q < e; ++q) s += prod[lds_at(q)];
             }
             __syncthreads();
         }
         if (tid == 0) {
+            if (chunked && in_group > 0) s += group;
             if (P.last) on_row(r0, s, prefetch(r0));
             else P.part[r0] = s;
         }
@@ -180,9 +230,9 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
 }
 
 // Same without epilogue prefetch.
-template <int TILE, typename T, typename V, typename F>
+template <int TILE, int ORDER = 0, typename T, typename V, typename F>
 __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restrict__ v, T* prod, int b, F&& on_row) {
-    stream_rows<TILE>(P, v, prod, b, PreNone{}, [&](int r, T s, NoPre) { on_row(r, s); });
+    stream_rows<TILE, ORDER>(P, v, prod, b, PreNone{}, [&](int r, T s, NoPre) { on_row(r, s); });
 }
 
 // ------------------------------------------------------------------ fused segments

@@ -388,16 +388,44 @@ typedef struct {
     double* tv;         /* CSC values (scaled) */
 } pmat;
 
-static double row_dot(const pmat* M, int32_t i, const double* v) {
+/* The engine's pdlp-mode order of a product over one row of L nonzeros
+ * (engine/spmv.hpp, stream_rows with ORDER == 1): for L <= 4096 the
+ * sequential sum from 0.0 in ascending position; for L > 4096, products are
+ * summed from 0.0 in chunks of 64 consecutive positions, the chunk sums from
+ * 0.0 in groups of 32 chunks, and the group sums from 0.0 into the row sum.
+ * val/idx point at the row's first entry. */
+#define OR_LONG_ROW 4096
+#define OR_CHUNK 64
+#define OR_GROUP 32
+double or_pdlp_row_sum(const double* val, const int32_t* idx, int64_t len, const double* v) {
     double s = 0.0;
-    for (int32_t p = M->rp[i]; p < M->rp[i + 1]; ++p) s += M->av[p] * v[M->ci[p]];
+    if (len <= OR_LONG_ROW) {
+        for (int64_t p = 0; p < len; ++p) s += val[p] * v[idx[p]];
+        return s;
+    }
+    double group = 0.0;
+    int in_group = 0;
+    for (int64_t c0 = 0; c0 < len; c0 += OR_CHUNK) {
+        const int64_t c1 = c0 + OR_CHUNK < len ? c0 + OR_CHUNK : len;
+        double cs = 0.0;
+        for (int64_t p = c0; p < c1; ++p) cs += val[p] * v[idx[p]];
+        group += cs;
+        if (++in_group == OR_GROUP) {
+            s += group;
+            group = 0.0;
+            in_group = 0;
+        }
+    }
+    if (in_group > 0) s += group;
     return s;
 }
 
+static double row_dot(const pmat* M, int32_t i, const double* v) {
+    return or_pdlp_row_sum(M->av + M->rp[i], M->ci + M->rp[i], (int64_t)M->rp[i + 1] - M->rp[i], v);
+}
+
 static double col_dot(const pmat* M, int32_t j, const double* v) {
-    double s = 0.0;
-    for (int64_t p = M->cs[j]; p < M->cs[j + 1]; ++p) s += M->tv[p] * v[M->ri[p]];
-    return s;
+    return or_pdlp_row_sum(M->tv + M->cs[j], M->ri + M->cs[j], M->cs[j + 1] - M->cs[j], v);
 }
 
 /* Evaluation at (xp, yp) with anchor (xa, ya); mode 1 = start point. */

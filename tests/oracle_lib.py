@@ -63,6 +63,8 @@ def oracle():
         L.or_kkt.argtypes = [C.POINTER(OrLp), _dp, _dp, _dp]
         L.or_scale_pdlp.argtypes = [C.POINTER(OrLp), C.c_int32, C.c_int32, _dp, _dp]
         L.or_pdlp.argtypes = [C.POINTER(OrLp), C.POINTER(OrPdlpParams), _dp, _dp, C.POINTER(OrPdlpInfo)]
+        L.or_pdlp_row_sum.argtypes = [_dp, _i32p, C.c_int64, _dp]
+        L.or_pdlp_row_sum.restype = C.c_double
         _oracle = L
     return _oracle
 

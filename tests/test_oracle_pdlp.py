@@ -100,3 +100,44 @@ def test_pdlp_oracle_no_certificate_on_feasible_lps(name):
     assert _linprog_status(lp) == 0
     _, _, info = Oracle(lp).pdlp(max_iters=400000, tol=1e-9)
     assert info.status == 0
+
+
+def _row_sum(val, idx, v):
+    from oracle_lib import oracle, _p
+    import ctypes as C
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    idx = np.ascontiguousarray(idx, dtype=np.int32)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return oracle().or_pdlp_row_sum(_p(val), _p(idx, C.c_int32), val.size, _p(v))
+
+
+# Row of 1.0 followed by len-1 terms of u = 2^-60. A term or a chunk sum of
+# 64 u = 2^-54 added to 1.0 rounds away (half an ulp of 1.0 is 2^-53), while
+# 32 chunk sums added from 0.0 give 2^-49 exactly. So the sequential order
+# returns 1.0, and the chunked order returns 1 + (groups after the first)
+# * 2^-49: the first group (positions 0..2047) is lost against the 1.0. A
+# chunk or group of another size, chunk sums added straight into the row sum,
+# or the threshold moved by one each give another value.
+@pytest.mark.parametrize("length,expect", [
+    (100, 1.0),
+    (4096, 1.0),                       # the longest sequential row
+    (4097, 1.0 + 2.0 ** -49),          # group 1 full, group 2 = one lost term
+    (6144, 1.0 + 2.0 ** -48),          # groups 1 and 2 full
+    (2048 * 9, 1.0 + 8 * 2.0 ** -49),
+])
+def test_pdlp_row_sum_order_closed_form(length, expect):
+    u = 2.0 ** -60
+    val = np.full(length, u)
+    val[0] = 1.0
+    assert _row_sum(val, np.arange(length), np.ones(length)) == expect
+
+
+def test_pdlp_row_sum_gathers_by_index():
+    # Integer products are exact in any order: the result is the exact dot
+    # product of the gathered entries, short and long rows alike.
+    rng = np.random.default_rng(5)
+    v = rng.integers(-1000, 1000, 20000).astype(np.float64)
+    for length in (0, 1, 63, 4096, 4097, 9000):
+        idx = rng.choice(v.size, length, replace=False)
+        val = rng.integers(-50, 50, length).astype(np.float64)
+        assert _row_sum(val, idx, v) == float(np.dot(val.astype(np.int64), v[idx].astype(np.int64)))
