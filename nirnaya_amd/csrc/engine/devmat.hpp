@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -78,6 +79,8 @@ struct TileShape {
     int rows = 256;
     int seg_width = 0;
     int fused = 0;   // walk all segments in one launch (SegCsr) when the rows allow it
+    int split = 0;   // split rows of more than kLongRow nonzeros over workgroups (one segment
+                     // only; for kernels that sum such rows in the chunked order)
 };
 
 // Structure (row pointers per segment, column indices, row blocks) of a
@@ -94,11 +97,16 @@ struct CsrShape {
     DevArray<int2> frows, fnz;
     int fnblk = 0;
     bool fused_ok = false;
+    // Rows split over workgroups (TileShape::split).
+    DevArray<int4> lrow;     // per split row {first group slot, groups, pieces, row}
+    DevArray<int> lcount;    // per split row: zero between launches
+    DevArray<double> gsum;   // group sums, FP64 or FP32 values
+    int nsplit = 0, ngroups = 0;
 
     template <typename T>
     Csr<T> view(const T* val, int s) const {
         return Csr<T>{nrows, ncols, rowptr.p + static_cast<size_t>(s) * (nrows + 1), col.p, val, nblk[s],
-                      desc.p + blk_off[s]};
+                      desc.p + blk_off[s], lrow.p, lcount.p, reinterpret_cast<T*>(gsum.p)};
     }
     template <typename T>
     Pass<T> pass(const T* val, T* part, int s) const {
@@ -111,6 +119,9 @@ struct CsrShape {
     // Blocks whose per-block results (e.g. violation maxima) the last launch writes.
     int result_blocks() const { return fused_ok ? fnblk : nblk[nseg - 1]; }
 };
+
+// Row length above which the layout's rows are split (none: INT_MAX).
+inline int split_len(const CsrShape& S) { return S.nsplit > 0 ? kLongRow : INT_MAX; }
 
 // Fused-segment partition: consecutive rows, at most `rows_max`, whose
 // nonzeros in every segment fit one tile. Fails (returns false) when a single
@@ -193,30 +204,63 @@ inline void upload_structure(CsrShape& S, const SegHost& H, hipStream_t st) {
     S.col.upload(H.col.data(), H.col.size(), st);
 }
 
+// Pieces of a split row: whole groups of kChunk * kGroupChunks nonzeros,
+// max(tile, one group) nonzeros each, from the row's first nonzero.
+inline int split_piece_len(int tile) { return std::max(tile, kGroupLen); }
+
+// Block descriptors of one segment's row pointers `srp` (appended to desc),
+// with rows of more than kLongRow nonzeros split into pieces when `split`
+// (their {first group slot, groups, pieces, row} appended to lrow,
+// group slots counted in `groups`).
+inline void segment_blocks(const int* srp, int nrows, const TileShape& shape, bool split, std::vector<int4>& desc,
+                           std::vector<int4>& lrow, int& groups) {
+    const int piece_len = split_piece_len(shape.tile);
+    if (split && piece_len > 2 * kGroupLen) throw std::invalid_argument("split rows: tile above 4096 nonzeros");
+    const std::vector<int> b = partition_rows(srp, nrows, shape.tile, std::min(shape.rows, kThreads));
+    for (size_t k = 0; k + 1 < b.size(); ++k) {
+        const int p0 = srp[b[k]], p1 = srp[b[k + 1]];
+        if (split && b[k + 1] == b[k] + 1 && p1 - p0 > kLongRow) {
+            const int id = static_cast<int>(lrow.size());
+            const int ng = (p1 - p0 + kGroupLen - 1) / kGroupLen;
+            const int np = (p1 - p0 + piece_len - 1) / piece_len;
+            lrow.push_back(make_int4(groups, ng, np, b[k]));
+            groups += ng;
+            for (int q = p0; q < p1; q += piece_len) desc.push_back(make_int4(b[k], -(id + 1), q, std::min(p1, q + piece_len)));
+            continue;
+        }
+        desc.push_back(make_int4(b[k], b[k + 1], p0, p1));
+    }
+}
+
 // Row blocks of a layout for one tile shape (per-segment descriptors, and the
 // fused-segment partition when asked for and possible), on the device.
 inline void set_blocks(CsrShape& S, const SegHost& H, TileShape shape, hipStream_t st) {
     S.shape = shape;
     const int nseg = H.nseg, nrows = H.nrows;
-    std::vector<int4> desc_all;
+    std::vector<int4> desc_all, lrow;
+    int groups = 0;
     S.blk_off.assign(nseg, 0);
     S.nblk.assign(nseg, 0);
     for (int s = 0; s < nseg; ++s) {
-        const int* srp = H.rp.data() + static_cast<size_t>(s) * (nrows + 1);
-        std::vector<int> b = partition_rows(srp, nrows, shape.tile, std::min(shape.rows, kThreads));
         S.blk_off[s] = static_cast<int>(desc_all.size());
-        S.nblk[s] = static_cast<int>(b.size()) - 1;
-        for (size_t k = 0; k + 1 < b.size(); ++k) {
-            int4 d;
-            d.x = b[k];
-            d.y = b[k + 1];
-            d.z = srp[b[k]];
-            d.w = srp[b[k + 1]];
-            desc_all.push_back(d);
-        }
+        segment_blocks(H.rp.data() + static_cast<size_t>(s) * (nrows + 1), nrows, shape, shape.split && nseg == 1,
+                       desc_all, lrow, groups);
+        S.nblk[s] = static_cast<int>(desc_all.size()) - S.blk_off[s];
     }
     S.desc.alloc(desc_all.size());
     S.desc.upload(desc_all.data(), desc_all.size(), st);
+    S.nsplit = static_cast<int>(lrow.size());
+    S.ngroups = groups;
+    S.lrow.release();
+    S.lcount.release();
+    S.gsum.release();
+    if (S.nsplit > 0) {
+        S.lrow.alloc(lrow.size());
+        S.lrow.upload(lrow.data(), lrow.size(), st);
+        S.lcount.alloc(lrow.size());
+        HIP_CHECK(hipMemsetAsync(S.lcount.p, 0, lrow.size() * sizeof(int), st));
+        S.gsum.alloc(static_cast<size_t>(groups));
+    }
     S.fused_ok = false;
     S.fnblk = 0;
     S.frows.release();

@@ -1004,6 +1004,9 @@ NIR_API int nir_session_create(nir_ctx* ctx, const nir_lp* lp, const nir_params*
         TileShape tt = tune ? autotune(n, m, at_ptr, lp->row_index, lp->value, S->s, fp32, !pdlp, seg_t) : base_t;
         if (!seg_a) ta.seg_width = 0, ta.fused = 0;
         if (!seg_t) tt.seg_width = 0, tt.fused = 0;
+        // ... and splits each such row over workgroups (pieces of whole
+        // groups; the chunked order is kept, so the split changes no bits).
+        ta.split = tt.split = pdlp ? 1 : 0;
         upload_matrix(S->A, m, n, a_ptr, a_col.data(), a_val.data(), ta, S->s);
         upload_matrix(S->AT, n, m, at_ptr, lp->row_index, lp->value, tt, S->s);
 

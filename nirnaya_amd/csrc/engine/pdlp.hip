@@ -257,18 +257,27 @@ __global__ void k_cert_diff(const PdlpState* __restrict__ st, int n, int m, cons
     for (int i = t; i < m; i += stride) dy[i] = Y[i] - qy[i];
 }
 
+// Lines (rows of A or of A^T) longer than `long_len` nonzeros are split
+// rows of a one-segment layout; the one-lane kernels skip them and the
+// split-row kernels below cover them.
+__device__ __forceinline__ bool split_line(int nseg, const int* __restrict__ rp, int r, int long_len) {
+    return nseg == 1 && rp[r + 1] - rp[r] > long_len;
+}
+
 // Column terms (one lane per column, (A^T dy)_j over CSR(A^T)).
 __global__ __launch_bounds__(kThreads) void k_cert_cols(const PdlpState* __restrict__ st, int n, int nseg,
                                                         const int* __restrict__ rp, const int* __restrict__ col,
                                                         const double* __restrict__ val,
                                                         const double* __restrict__ dy, const double* __restrict__ dx,
                                                         const double* __restrict__ c, const double* __restrict__ lo,
-                                                        const double* __restrict__ up, dd* __restrict__ part) {
+                                                        const double* __restrict__ up, dd* __restrict__ part,
+                                                        int long_len) {
     __shared__ dd sh[6 * kThreads];
     if (!cert_active(st)) return;
     dd acc[6] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
     const int j = blockIdx.x * kThreads + threadIdx.x;
-    if (j < n) cert_col_terms(-dd_val(row_dot_dd(n, nseg, rp, col, val, dy, j)), c[j], lo[j], up[j], dx[j], acc);
+    if (j < n && !split_line(nseg, rp, j, long_len))
+        cert_col_terms(-dd_val(row_dot_dd(n, nseg, rp, col, val, dy, j)), c[j], lo[j], up[j], dx[j], acc);
     block_dd_reduce<6>(acc, sh, part + 6 * blockIdx.x);
 }
 
@@ -278,12 +287,89 @@ __global__ __launch_bounds__(kThreads) void k_cert_rows(const PdlpState* __restr
                                                         const double* __restrict__ val,
                                                         const double* __restrict__ dx, const double* __restrict__ dy,
                                                         const double* __restrict__ rl, const double* __restrict__ ru,
-                                                        dd* __restrict__ part) {
+                                                        dd* __restrict__ part, int long_len) {
     __shared__ dd sh[6 * kThreads];
     if (!cert_active(st)) return;
     dd acc[6] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
     const int i = blockIdx.x * kThreads + threadIdx.x;
-    if (i < m) cert_row_terms(dd_val(row_dot_dd(m, nseg, rp, col, val, dx, i)), rl[i], ru[i], dy[i], acc);
+    if (i < m && !split_line(nseg, rp, i, long_len))
+        cert_row_terms(dd_val(row_dot_dd(m, nseg, rp, col, val, dx, i)), rl[i], ru[i], dy[i], acc);
+    block_dd_reduce<6>(acc, sh, part + 6 * blockIdx.x);
+}
+
+// Double-double products of the split rows, one workgroup per piece (other
+// blocks exit): chunks of kChunk products summed in dd from zero by one lane
+// each, then each group's chunk sums in dd by lane 0 into its slot of `gdd`.
+// A piece holds at most 2 * kGroupLen products (64 chunks).
+__global__ __launch_bounds__(kThreads) void k_cert_pieces(const PdlpState* __restrict__ st, Csr<double> M,
+                                                          const double* __restrict__ v, dd* __restrict__ gdd) {
+    __shared__ dd cs[2 * kGroupChunks];
+    if (!cert_active(st)) return;
+    const int4 d = M.desc[blockIdx.x];
+    if (d.y >= 0) return;
+    const int4 L = M.lrow[-d.y - 1];
+    const int tid = threadIdx.x;
+    const int len = d.w - d.z, nch = (len + kChunk - 1) / kChunk;
+    if (tid < nch) {
+        dd a = dd_zero();
+        const int q1 = min((tid + 1) * kChunk, len);
+        for (int q = d.z + tid * kChunk; q < d.z + q1; ++q) a = dd_add(a, dd_prod(M.val[q], v[M.col[q]]));
+        cs[tid] = a;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int g = L.x + (d.z - M.rowptr[L.w]) / kGroupLen;
+        for (int c0 = 0; c0 < nch; c0 += kGroupChunks, ++g) {
+            dd a = dd_zero();
+            for (int c = c0; c < min(nch, c0 + kGroupChunks); ++c) a = dd_add(a, cs[c]);
+            gdd[g] = a;
+        }
+    }
+}
+
+// A split row's dd product: its group sums added in dd from zero in order.
+__device__ __forceinline__ double split_dot(const int4& L, const dd* __restrict__ gdd) {
+    dd a = dd_zero();
+    for (int g = 0; g < L.y; ++g) a = dd_add(a, gdd[L.x + g]);
+    return dd_val(a);
+}
+
+// Row terms of the split rows of A (one lane per split row).
+__global__ __launch_bounds__(kThreads) void k_cert_split_rows(const PdlpState* __restrict__ st,
+                                                              const int4* __restrict__ lrow, int nsplit,
+                                                              const dd* __restrict__ gdd,
+                                                              const double* __restrict__ dy,
+                                                              const double* __restrict__ rl,
+                                                              const double* __restrict__ ru, dd* __restrict__ part) {
+    __shared__ dd sh[6 * kThreads];
+    if (!cert_active(st)) return;
+    dd acc[6] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
+    const int k = blockIdx.x * kThreads + threadIdx.x;
+    if (k < nsplit) {
+        const int4 L = lrow[k];
+        const int i = L.w;
+        cert_row_terms(split_dot(L, gdd), rl[i], ru[i], dy[i], acc);
+    }
+    block_dd_reduce<6>(acc, sh, part + 6 * blockIdx.x);
+}
+
+// Column terms of the split rows of A^T (one lane per split column).
+__global__ __launch_bounds__(kThreads) void k_cert_split_cols(const PdlpState* __restrict__ st,
+                                                              const int4* __restrict__ lrow, int nsplit,
+                                                              const dd* __restrict__ gdd,
+                                                              const double* __restrict__ dx,
+                                                              const double* __restrict__ c,
+                                                              const double* __restrict__ lo,
+                                                              const double* __restrict__ up, dd* __restrict__ part) {
+    __shared__ dd sh[6 * kThreads];
+    if (!cert_active(st)) return;
+    dd acc[6] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
+    const int k = blockIdx.x * kThreads + threadIdx.x;
+    if (k < nsplit) {
+        const int4 L = lrow[k];
+        const int j = L.w;
+        cert_col_terms(-split_dot(L, gdd), c[j], lo[j], up[j], dx[j], acc);
+    }
     block_dd_reduce<6>(acc, sh, part + 6 * blockIdx.x);
 }
 
@@ -373,12 +459,17 @@ __global__ void k_delta_to_double(const double* anchor, const float* delta, doub
 }
 
 // Row measures over a segmented CSR matrix, segments in column order: max |a|
-// (what = 0) or sum |a| (what = 1).
+// (what = 0) or sum |a| (what = 1). Split rows (more than `long_len`
+// nonzeros) get 0 here; k_piece_measure and k_split_total finish them.
 __global__ void k_row_measure(const int* __restrict__ rowptr, int nseg, int nrows, const double* __restrict__ val,
-                              double* out, int what) {
+                              double* out, int what, int long_len) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrows) return;
     double acc = 0.0;
+    if (split_line(nseg, rowptr, r, long_len)) {
+        out[r] = 0.0;
+        return;
+    }
     for (int s = 0; s < nseg; ++s) {
         const int* rp = rowptr + static_cast<size_t>(s) * (nrows + 1);
         for (int p = rp[r]; p < rp[r + 1]; ++p) {
@@ -397,11 +488,71 @@ __global__ void k_factor(const double* meas, double* f, double* total, int n) {
     total[r] *= g;
 }
 
-// val[p] *= rf[row] * cf[col[p]] over every segment (all factors powers of two).
+// Split rows, one workgroup per piece (other blocks exit). what = 0: the
+// piece's max |a| joins out[row] by an atomic max on the bit patterns
+// (nonnegative doubles order as their bits; out[row] starts at 0). what = 1:
+// sum |a| in the chunked order of the products (chunks of kChunk from 0.0 by
+// one lane each, each group's chunk sums from 0.0 by lane 0) into the row's
+// group slots of M.gsum; k_split_total adds the groups.
+__global__ __launch_bounds__(kThreads) void k_piece_measure(Csr<double> M, double* out, int what) {
+    __shared__ double red[2 * kGroupChunks];
+    const int4 d = M.desc[blockIdx.x];
+    if (d.y >= 0) return;
+    const int4 L = M.lrow[-d.y - 1];
+    const int tid = threadIdx.x;
+    if (what == 0) {
+        double a = 0.0;
+        for (int p = d.z + tid; p < d.w; p += kThreads) a = amax(a, fabs(M.val[p]));
+        a = block_max(a, red);
+        if (tid == 0)
+            atomicMax(reinterpret_cast<unsigned long long*>(out + L.w),
+                      static_cast<unsigned long long>(__double_as_longlong(a)));
+        return;
+    }
+    const int len = d.w - d.z, nch = (len + kChunk - 1) / kChunk;
+    if (tid < nch) {
+        double a = 0.0;
+        const int q1 = min((tid + 1) * kChunk, len);
+        for (int q = d.z + tid * kChunk; q < d.z + q1; ++q) a += fabs(M.val[q]);
+        red[tid] = a;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int g = L.x + (d.z - M.rowptr[L.w]) / kGroupLen;
+        for (int c0 = 0; c0 < nch; c0 += kGroupChunks, ++g) {
+            double a = 0.0;
+            for (int c = c0; c < min(nch, c0 + kGroupChunks); ++c) a += red[c];
+            M.gsum[g] = a;
+        }
+    }
+}
+
+// out[row] of each split row = its group sums added from 0.0 in order.
+__global__ void k_split_total(const int4* __restrict__ lrow, int nsplit, const double* __restrict__ gsum,
+                              double* out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nsplit) return;
+    const int4 L = lrow[k];
+    double a = 0.0;
+    for (int g = 0; g < L.y; ++g) a += gsum[L.x + g];
+    out[L.w] = a;
+}
+
+// The pieces' share of k_scale_csr (one workgroup per piece).
+__global__ __launch_bounds__(kThreads) void k_piece_scale(Csr<double> M, double* val, const double* rf,
+                                                          const double* cf) {
+    const int4 d = M.desc[blockIdx.x];
+    if (d.y >= 0) return;
+    const double a = rf[M.lrow[-d.y - 1].w];
+    for (int p = d.z + threadIdx.x; p < d.w; p += kThreads) val[p] = val[p] * (a * cf[M.col[p]]);
+}
+
+// val[p] *= rf[row] * cf[col[p]] over every segment (all factors powers of
+// two); split rows (more than `long_len` nonzeros) are left to k_piece_scale.
 __global__ void k_scale_csr(const int* __restrict__ rowptr, int nseg, int nrows, const int* __restrict__ col,
-                            double* val, const double* rf, const double* cf) {
+                            double* val, const double* rf, const double* cf, int long_len) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrows) return;
+    if (r >= nrows || split_line(nseg, rowptr, r, long_len)) return;
     const double a = rf[r];
     for (int s = 0; s < nseg; ++s) {
         const int* rp = rowptr + static_cast<size_t>(s) * (nrows + 1);
@@ -493,6 +644,7 @@ struct Pdlp {
     Buf<double> qx, qy;        // certificate: the anchor before the restart
     Buf<double> rx, ry;        // certificate: the step between the two anchors
     Buf<dd> cert_c, cert_r;    // certificate: per-block partial sums
+    Buf<dd> cert_gt, cert_ga;  // certificate: group sums of the split rows of A^T and A
     Buf<dd> part_c, part_r;
     Buf<PdlpState> st;
     PdlpState* h_st = nullptr;   // pinned, two poll slots
@@ -611,10 +763,27 @@ void launch_certificate(Pdlp& P, hipStream_t s) {
     const CsrShape& AT = P.AT->shape;
     const CsrShape& A = P.A->shape;
     k_cert_cols<<<gc, kThreads, 0, s>>>(P.st.p, n, AT.nseg, AT.rowptr.p, AT.col.p, P.AT->val.p, P.ry.p, P.rx.p,
-                                        P.d.c.p, P.d.lo.p, P.d.up.p, P.cert_c.p);
+                                        P.d.c.p, P.d.lo.p, P.d.up.p, P.cert_c.p, split_len(AT));
     k_cert_rows<<<gr, kThreads, 0, s>>>(P.st.p, m, A.nseg, A.rowptr.p, A.col.p, P.A->val.p, P.rx.p, P.ry.p, P.d.rl.p,
-                                        P.d.ru.p, P.cert_r.p);
-    k_cert_decide<<<1, kThreads, 0, s>>>(P.st.p, P.cert_c.p, gc, P.cert_r.p, gr);
+                                        P.d.ru.p, P.cert_r.p, split_len(A));
+    // Split rows: dd products per piece, then their terms into partial slots
+    // after the one-lane kernels' slots.
+    int nbc = gc, nbr = gr;
+    if (AT.nsplit > 0) {
+        k_cert_pieces<<<AT.nblk[0], kThreads, 0, s>>>(P.st.p, AT.view<double>(P.AT->val.p, 0), P.ry.p, P.cert_gt.p);
+        const int gl = (AT.nsplit + kThreads - 1) / kThreads;
+        k_cert_split_cols<<<gl, kThreads, 0, s>>>(P.st.p, AT.lrow.p, AT.nsplit, P.cert_gt.p, P.rx.p, P.d.c.p,
+                                                  P.d.lo.p, P.d.up.p, P.cert_c.p + 6 * static_cast<size_t>(gc));
+        nbc += gl;
+    }
+    if (A.nsplit > 0) {
+        k_cert_pieces<<<A.nblk[0], kThreads, 0, s>>>(P.st.p, A.view<double>(P.A->val.p, 0), P.rx.p, P.cert_ga.p);
+        const int gl = (A.nsplit + kThreads - 1) / kThreads;
+        k_cert_split_rows<<<gl, kThreads, 0, s>>>(P.st.p, A.lrow.p, A.nsplit, P.cert_ga.p, P.ry.p, P.d.rl.p,
+                                                  P.d.ru.p, P.cert_r.p + 6 * static_cast<size_t>(gr));
+        nbr += gl;
+    }
+    k_cert_decide<<<1, kThreads, 0, s>>>(P.st.p, P.cert_c.p, nbc, P.cert_r.p, nbr);
 }
 
 template <typename TI>
@@ -759,20 +928,32 @@ double pdlp_prepare_matrix(hipStream_t s, int m, int n, long long nnz, DevMatrix
     double* a_val = Am->val.p;
     double* at_val = ATm->val.p;
     const int gm = (m + 255) / 256, gn = (n + 255) / 256;
+    // One lane per row, and one workgroup per piece of a split row.
+    auto measure = [&](const CsrShape& S, double* val, int rows, double* out, int what) {
+        k_row_measure<<<(rows + 255) / 256, 256, 0, s>>>(S.rowptr.p, S.nseg, rows, val, out, what, split_len(S));
+        if (S.nsplit > 0) {
+            k_piece_measure<<<S.nblk[0], kThreads, 0, s>>>(S.view<double>(val, 0), out, what);
+            if (what == 1) k_split_total<<<(S.nsplit + 255) / 256, 256, 0, s>>>(S.lrow.p, S.nsplit, S.gsum.p, out);
+        }
+    };
+    auto scale = [&](const CsrShape& S, double* val, int rows, const double* rf, const double* cf) {
+        k_scale_csr<<<(rows + 255) / 256, 256, 0, s>>>(S.rowptr.p, S.nseg, rows, S.col.p, val, rf, cf, split_len(S));
+        if (S.nsplit > 0) k_piece_scale<<<S.nblk[0], kThreads, 0, s>>>(S.view<double>(val, 0), val, rf, cf);
+    };
     for (int pass = 0; pass < 11; ++pass) {
         const int what = pass < 10 ? 0 : 1;   // 10 Ruiz (max |a|) passes, then one Pock-Chambolle (sum |a|)
         if (m > 0) {
-            k_row_measure<<<gm, 256, 0, s>>>(SA.rowptr.p, SA.nseg, m, a_val, mr.p, what);
+            measure(SA, a_val, m, mr.p, what);
             k_factor<<<gm, 256, 0, s>>>(mr.p, fr.p, dR, m);
-            k_scale_csr<<<gm, 256, 0, s>>>(SA.rowptr.p, SA.nseg, m, SA.col.p, a_val, fr.p, onesc.p);
+            scale(SA, a_val, m, fr.p, onesc.p);
         }
         if (n > 0) {
-            k_scale_csr<<<gn, 256, 0, s>>>(ST.rowptr.p, ST.nseg, n, ST.col.p, at_val, onesc.p, fr.p);
-            k_row_measure<<<gn, 256, 0, s>>>(ST.rowptr.p, ST.nseg, n, at_val, mc.p, what);
+            scale(ST, at_val, n, onesc.p, fr.p);
+            measure(ST, at_val, n, mc.p, what);
             k_factor<<<gn, 256, 0, s>>>(mc.p, fc.p, dC, n);
-            k_scale_csr<<<gn, 256, 0, s>>>(ST.rowptr.p, ST.nseg, n, ST.col.p, at_val, fc.p, onesr.p);
+            scale(ST, at_val, n, fc.p, onesr.p);
         }
-        if (m > 0) k_scale_csr<<<gm, 256, 0, s>>>(SA.rowptr.p, SA.nseg, m, SA.col.p, a_val, onesr.p, fc.p);
+        if (m > 0) scale(SA, a_val, m, onesr.p, fc.p);
         PD_CHECK(hipGetLastError());
     }
     R.assign(static_cast<size_t>(m), 0.0);
@@ -904,8 +1085,11 @@ static void alloc_work(Pdlp& P) {
     P.qy.alloc(m);
     P.rx.alloc(n);
     P.ry.alloc(m);
-    P.cert_c.alloc(static_cast<size_t>(6) * std::max(1, (n + kThreads - 1) / kThreads));
-    P.cert_r.alloc(static_cast<size_t>(6) * std::max(1, (m + kThreads - 1) / kThreads));
+    const auto blocks = [](int k) { return (k + kThreads - 1) / kThreads; };
+    P.cert_c.alloc(static_cast<size_t>(6) * (std::max(1, blocks(n)) + blocks(ST.nsplit)));
+    P.cert_r.alloc(static_cast<size_t>(6) * (std::max(1, blocks(m)) + blocks(SA.nsplit)));
+    P.cert_gt.alloc(std::max(1, ST.ngroups));
+    P.cert_ga.alloc(std::max(1, SA.ngroups));
 }
 
 void pdlp_setup(Pdlp& P, hipStream_t s, const nir_lp* lp, const nir_params& prm, DevMatrix* Am, DevMatrix* ATm) {

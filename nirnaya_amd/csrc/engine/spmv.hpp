@@ -39,7 +39,12 @@ struct Csr {
     const int* col;      // nnz
     const T* val;        // nnz
     int nblk;            // number of row blocks (>= 1)
-    const int4* desc;    // per block {first row, end row, first nonzero, end nonzero}
+    const int4* desc;    // per block {first row, end row, first nonzero, end nonzero}; a piece
+                         // of a split row has end row -(split row index + 1)
+    // Rows split over workgroups (pdlp mode only, see stream_rows):
+    const int4* lrow;    // per split row {first group slot, groups, pieces, row}
+    int* lcount;         // per split row: pieces finished in the current launch
+    T* gsum;             // group sums of the split rows
 };
 
 // One pass of a (possibly segmented) product: which segment, and where the
@@ -86,7 +91,8 @@ struct PreNone {
 // ORDER == 1 (see stream_rows).
 constexpr int kLongRow = 4096;
 constexpr int kChunk = 64;        // products per chunk
-constexpr int kGroupChunks = 32;  // chunks per group (2048 products)
+constexpr int kGroupChunks = 32;  // chunks per group
+constexpr int kGroupLen = kChunk * kGroupChunks;   // products per group (2048)
 
 template <int TILE, int ORDER = 0, typename T, typename V, typename Pre, typename F>
 __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restrict__ v, T* prod, int b,
@@ -98,8 +104,11 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
     const int4 d = A.desc[b];   // one load instead of blk -> rowptr chained loads
     const int r0 = d.x, r1 = d.y, p0 = d.z, p1 = d.w;
     const int cnt = p1 - p0;
-    if (r1 == r0) return;   // empty matrix: the one block has no rows
-    if (cnt > TILE) {
+    // ORDER == 1: a row of more than kLongRow nonzeros may be split into
+    // pieces of whole groups (below), one workgroup each.
+    const bool piece = ORDER == 1 && r1 < 0;
+    if (!piece && r1 <= r0) return;   // empty matrix: the one block has no rows
+    if (piece || cnt > TILE) {
         // One row longer than a tile: a chain of tiles. All lanes park one
         // tile of the row's products in LDS, then lane 0 adds them in column
         // order to the running sum, so the row keeps the sequential order and
@@ -129,10 +138,25 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
         // sequential, so 32 (or 64) lanes sum a tile's chunks at once and
         // lane 0 adds only one value per chunk. Rows up to kLongRow nonzeros
         // keep the sequential order in every mode.
-        const bool chunked = ORDER == 1 && cnt > kLongRow;
+        //
+        // A piece of a split row covers whole groups from a group boundary of
+        // its row (the row's last piece may end in a partial group). Lane 0
+        // stores each completed group sum in the row's slots of gsum instead
+        // of adding it; the piece that finishes last (a counter per split
+        // row, back to 0 once all pieces counted) adds the row's group sums
+        // from 0.0 in group order - the same sum whichever piece is last -
+        // and runs the epilogue.
+        const bool chunked = ORDER == 1 && (piece || cnt > kLongRow);
         __shared__ T chunk_sum[TILE / kChunk];
         T group = T(0);     // lane 0: the current group's partial sum
         int in_group = 0;   // chunks added to it
+        int4 L = make_int4(0, 0, 0, 0);
+        int slot_g = 0;     // piece: gsum slot of the current group
+        if (piece) {
+            L = A.lrow[-r1 - 1];
+            slot_g = L.x + (p0 - A.rowptr[L.w]) / kGroupLen;
+            s = T(0);
+        }
         load(p0);
         for (int c0 = p0; c0 < p1; c0 += TILE) {
             T g[PER];
@@ -156,7 +180,8 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
                     for (int k = 0; k < nch; ++k) {
                         group += chunk_sum[k];
                         if (++in_group == kGroupChunks) {
-                            s += group;
+                            if (piece) A.gsum[slot_g++] = group;
+                            else s += group;
                             group = T(0);
                             in_group = 0;
                         }
@@ -178,8 +203,33 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
             }
             __syncthreads();
         }
+        if (piece) {
+            __shared__ int last_piece;
+            int* count = A.lcount + (-r1 - 1);
+            if (tid == 0) {
+                if (in_group > 0) A.gsum[slot_g] = group;
+                // Release this piece's group sums, then count it.
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                const int before = __hip_atomic_fetch_add(count, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                last_piece = before == L.z - 1;
+                if (last_piece) __hip_atomic_store(count, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __syncthreads();
+            if (!last_piece) return;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            // All lanes fetch the group sums into LDS, lane 0 adds them.
+            for (int k0 = 0; k0 < L.y; k0 += kThreads) {
+                const int e = L.y - k0 < kThreads ? L.y - k0 : kThreads;
+                if (tid < e) prod[lds_at(tid)] = A.gsum[L.x + k0 + tid];
+                __syncthreads();
+                if (tid == 0)
+                    for (int q = 0; q < e; ++q) s += prod[lds_at(q)];
+                __syncthreads();
+            }
+        } else if (tid == 0 && chunked && in_group > 0) {
+            s += group;
+        }
         if (tid == 0) {
-            if (chunked && in_group > 0) s += group;
             if (P.last) on_row(r0, s, prefetch(r0));
             else P.part[r0] = s;
         }

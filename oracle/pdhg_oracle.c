@@ -388,19 +388,25 @@ typedef struct {
     double* tv;         /* CSC values (scaled) */
 } pmat;
 
-/* The engine's pdlp-mode order of a product over one row of L nonzeros
- * (engine/spmv.hpp, stream_rows with ORDER == 1): for L <= 4096 the
- * sequential sum from 0.0 in ascending position; for L > 4096, products are
- * summed from 0.0 in chunks of 64 consecutive positions, the chunk sums from
- * 0.0 in groups of 32 chunks, and the group sums from 0.0 into the row sum.
- * val/idx point at the row's first entry. */
+/* The engine's pdlp-mode order of a sum over one row of L nonzeros
+ * (engine/spmv.hpp, stream_rows with ORDER == 1, and the split-row kernels
+ * of engine/pdlp.hip): for L <= 4096 the sequential sum from 0.0 in
+ * ascending position; for L > 4096, terms are summed from 0.0 in chunks of
+ * 64 consecutive positions, the chunk sums from 0.0 in groups of 32 chunks,
+ * and the group sums from 0.0 into the row sum. The term at position p is
+ * val[p] * v[idx[p]], or |val[p]| when v is NULL (the Pock-Chambolle row
+ * measure). val/idx point at the row's first entry. */
 #define OR_LONG_ROW 4096
 #define OR_CHUNK 64
 #define OR_GROUP 32
+static double pdlp_term(const double* val, const int32_t* idx, const double* v, int64_t p) {
+    return v ? val[p] * v[idx[p]] : fabs(val[p]);
+}
+
 double or_pdlp_row_sum(const double* val, const int32_t* idx, int64_t len, const double* v) {
     double s = 0.0;
     if (len <= OR_LONG_ROW) {
-        for (int64_t p = 0; p < len; ++p) s += val[p] * v[idx[p]];
+        for (int64_t p = 0; p < len; ++p) s += pdlp_term(val, idx, v, p);
         return s;
     }
     double group = 0.0;
@@ -408,7 +414,7 @@ double or_pdlp_row_sum(const double* val, const int32_t* idx, int64_t len, const
     for (int64_t c0 = 0; c0 < len; c0 += OR_CHUNK) {
         const int64_t c1 = c0 + OR_CHUNK < len ? c0 + OR_CHUNK : len;
         double cs = 0.0;
-        for (int64_t p = c0; p < c1; ++p) cs += val[p] * v[idx[p]];
+        for (int64_t p = c0; p < c1; ++p) cs += pdlp_term(val, idx, v, p);
         group += cs;
         if (++in_group == OR_GROUP) {
             s += group;
@@ -545,22 +551,44 @@ static void pdlp_eval(const pmat* M, pstate* s, const double* c, const double* l
  *   Primal ray dx (certifies dual infeasibility): violations are the parts of
  *   A dx and dx outside the recession cones of the rows and bounds; -c^T dx
  *   must be positive.
- * A dx and A^T dy are double-double dot products in ascending index order and
+ * A dx and A^T dy are double-double dot products in ascending index order
+ * (chunked as in or_pdlp_row_sum for lines of more than 4096 nonzeros) and
  * every sum is double-double, so an exact zero stays zero to ~1e-32. A ray
  * passes when its objective exceeds 1e-12 of the sum of its terms' magnitudes
  * and ||violations||_2 <= eps * objective. Returns 21 (dual ray passes), 22
  * (primal ray passes) or 0; ratios[0..1] = violation / objective (inf when
  * the objective test fails). */
-static double row_dot_dd(const pmat* M, int32_t i, const double* v) {
+static double dd_dot_line(const double* val, const int32_t* idx, int64_t len, const double* v) {
     dd s = {0, 0};
-    for (int32_t p = M->rp[i]; p < M->rp[i + 1]; ++p) dd_add_prod(&s, M->av[p], v[M->ci[p]]);
+    if (len <= OR_LONG_ROW) {
+        for (int64_t p = 0; p < len; ++p) dd_add_prod(&s, val[p], v[idx[p]]);
+        return s.hi + s.lo;
+    }
+    /* Lines of more than 4096 nonzeros: the chunked order of or_pdlp_row_sum
+     * with every level in double-double. */
+    dd group = {0, 0};
+    int in_group = 0;
+    for (int64_t c0 = 0; c0 < len; c0 += OR_CHUNK) {
+        const int64_t c1 = c0 + OR_CHUNK < len ? c0 + OR_CHUNK : len;
+        dd cs = {0, 0};
+        for (int64_t p = c0; p < c1; ++p) dd_add_prod(&cs, val[p], v[idx[p]]);
+        dd_add(&group, cs.hi, cs.lo);
+        if (++in_group == OR_GROUP) {
+            dd_add(&s, group.hi, group.lo);
+            group.hi = group.lo = 0.0;
+            in_group = 0;
+        }
+    }
+    if (in_group > 0) dd_add(&s, group.hi, group.lo);
     return s.hi + s.lo;
 }
 
+static double row_dot_dd(const pmat* M, int32_t i, const double* v) {
+    return dd_dot_line(M->av + M->rp[i], M->ci + M->rp[i], (int64_t)M->rp[i + 1] - M->rp[i], v);
+}
+
 static double col_dot_dd(const pmat* M, int32_t j, const double* v) {
-    dd s = {0, 0};
-    for (int64_t p = M->cs[j]; p < M->cs[j + 1]; ++p) dd_add_prod(&s, M->tv[p], v[M->ri[p]]);
-    return s.hi + s.lo;
+    return dd_dot_line(M->tv + M->cs[j], M->ri + M->cs[j], M->cs[j + 1] - M->cs[j], v);
 }
 
 static int cert_check(const pmat* M, const double* c, const double* lo, const double* up, const double* rl,
@@ -675,10 +703,10 @@ static void pdlp_scale(pmat* Mt, const or_lp* lp, int ruiz, int pc, double* R, d
         const int sum = pass >= ruiz;
         for (int32_t i = 0; i < m; ++i) {
             double acc = 0.0;
-            for (int32_t p = Mt->rp[i]; p < Mt->rp[i + 1]; ++p) {
-                const double a = fabs(Mt->av[p]);
-                acc = sum ? acc + a : smax(acc, a);
-            }
+            if (sum)
+                acc = or_pdlp_row_sum(Mt->av + Mt->rp[i], NULL, (int64_t)Mt->rp[i + 1] - Mt->rp[i], NULL);
+            else
+                for (int32_t p = Mt->rp[i]; p < Mt->rp[i + 1]; ++p) acc = smax(acc, fabs(Mt->av[p]));
             f[i] = pow2_inv_sqrt(acc);
             R[i] *= f[i];
         }
@@ -687,10 +715,10 @@ static void pdlp_scale(pmat* Mt, const or_lp* lp, int ruiz, int pc, double* R, d
         for (int64_t p = 0; p < nnz; ++p) Mt->tv[p] = Mt->tv[p] * (1.0 * f[lp->index[p]]);
         for (int32_t j = 0; j < n; ++j) {
             double acc = 0.0;
-            for (int64_t p = lp->start[j]; p < lp->start[j + 1]; ++p) {
-                const double a = fabs(Mt->tv[p]);
-                acc = sum ? acc + a : smax(acc, a);
-            }
+            if (sum)
+                acc = or_pdlp_row_sum(Mt->tv + lp->start[j], NULL, lp->start[j + 1] - lp->start[j], NULL);
+            else
+                for (int64_t p = lp->start[j]; p < lp->start[j + 1]; ++p) acc = smax(acc, fabs(Mt->tv[p]));
             f[j] = pow2_inv_sqrt(acc);
             C[j] *= f[j];
         }

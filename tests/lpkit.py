@@ -76,6 +76,60 @@ def infeasible_lps():
     return out
 
 
+def dense_line_lp(seed, m, n, row_len, col_len, per_col=3):
+    """Equality LP (feasible by construction, x in [0, 10]) with row 1 holding
+    `row_len` and column 2 holding `col_len` nonzeros beside `per_col` random
+    ones per column."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    ri = rng.integers(0, m, per_col * n)
+    ci = np.repeat(np.arange(n), per_col)
+    ri = np.concatenate([ri, np.full(row_len, 1), rng.choice(m, col_len, replace=False)])
+    ci = np.concatenate([ci, rng.choice(n, row_len, replace=False), np.full(col_len, 2)])
+    A = sp.csc_matrix((rng.standard_normal(ri.size), (ri, ci)), shape=(m, n))
+    A.sum_duplicates()
+    A.sort_indices()
+    b = A @ rng.uniform(0, 1, n)
+    c = A.T @ rng.standard_normal(m) + rng.uniform(0, 1, n)
+    return LP(m, n, A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data.astype(np.float64), c,
+              np.zeros(n), np.full(n, 10.0), b.copy(), b.copy())
+
+
+def _csc(lp):
+    import scipy.sparse as sp
+    return sp.csc_matrix((lp.value, lp.index, lp.start), shape=(lp.m, lp.n))
+
+
+def _from_csc(A, c, l, u, rl, ru):
+    A = A.tocsc()
+    A.sort_indices()
+    return LP(A.shape[0], A.shape[1], A.indptr.astype(np.int64), A.indices.astype(np.int32),
+              A.data.astype(np.float64), np.asarray(c, float), np.asarray(l, float), np.asarray(u, float),
+              np.asarray(rl, float), np.asarray(ru, float))
+
+
+def long_line_infeasible_lps():
+    """As infeasible_lps, on LPs whose rays run through a row and a column of
+    more than 4,096 nonzeros (both certificates' products then take the long
+    lines' summation order)."""
+    import scipy.sparse as sp
+    out = {}
+    # The dense row (row 1) repeated at right-hand side + 1: y on the pair.
+    base = dense_line_lp(31, 5000, 6000, 5000, 4500)
+    A = _csc(base)
+    out["dense_row_repeated_shifted"] = ("dual", _from_csc(
+        sp.vstack([A, A[1]]), base.c, base.l, base.u, np.append(base.rl, base.rl[1] + 1.0),
+        np.append(base.ru, base.ru[1] + 1.0)))
+    # The dense column (column 2) and its negative appended, unbounded above,
+    # the first with cost -1: their difference is a free direction of descent.
+    base = dense_line_lp(32, 4300, 4600, 4500, 4200, per_col=2)
+    A = _csc(base)
+    out["dense_column_free_direction"] = ("primal", _from_csc(
+        sp.hstack([A, A[:, 2], -A[:, 2]]), np.append(base.c, [-1.0, 0.0]), np.append(base.l, [0.0, 0.0]),
+        np.append(base.u, [INF, INF]), base.rl, base.ru))
+    return out
+
+
 def read_mps_lp(path):
     """Minimal free-MPS reader for the fixtures (N/L/G/E rows, COLUMNS, RHS,
     UP/LO bounds, OBJSENSE MAX). Returns (LP with the file's own costs,

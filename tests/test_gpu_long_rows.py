@@ -1,46 +1,33 @@
 """pdlp mode on LPs with dense rows and columns (more than 4,096 nonzeros):
 such rows are summed in the chunked order (chunks of 64 products, groups of
-32 chunks, each level sequential from 0.0; the oracle's pdlp_row_sum), which
-lets a workgroup sum a tile's chunks in parallel instead of one lane adding
-every product. The device trajectory must equal or_pdlp's exactly; reference
+32 chunks, each level sequential from 0.0; the oracle's or_pdlp_row_sum),
+which lets a workgroup sum a tile's chunks in parallel instead of one lane
+adding every product, and lets a row be split over workgroups in pieces of
+whole groups, the last piece to finish adding the group sums in order. The device trajectory must equal or_pdlp's exactly; reference
 mode keeps the sequential order at every length, and the batch and family
 engines (one lane per row) refuse such rows."""
 import numpy as np
 import pytest
-import scipy.sparse as sp
 
 from nirnaya_amd.engine import LP, MODE_PDLP, MODE_REFERENCE, PREC_MIXED, EngineError, default_params
 from oracle_lib import Oracle
+from lpkit import dense_line_lp, long_line_infeasible_lps
 
 pytestmark = pytest.mark.gpu
-
-
-def dense_line_lp(seed, m, n, row_len, col_len, per_col=3):
-    """Equality LP (feasible by construction) with one row of `row_len` and
-    one column of `col_len` nonzeros beside `per_col` random ones per column."""
-    rng = np.random.default_rng(seed)
-    ri = rng.integers(0, m, per_col * n)
-    ci = np.repeat(np.arange(n), per_col)
-    dense_row, dense_col = 1, 2
-    ri = np.concatenate([ri, np.full(row_len, dense_row), rng.choice(m, col_len, replace=False)])
-    ci = np.concatenate([ci, rng.choice(n, row_len, replace=False), np.full(col_len, dense_col)])
-    A = sp.csc_matrix((rng.standard_normal(ri.size), (ri, ci)), shape=(m, n))
-    A.sum_duplicates()
-    A.sort_indices()
-    b = A @ rng.uniform(0, 1, n)
-    c = A.T @ rng.standard_normal(m) + rng.uniform(0, 1, n)
-    return LP(m, n, A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data.astype(np.float64), c,
-              np.zeros(n), np.full(n, 10.0), b.copy(), b.copy())
 
 
 CASES = {
     "row_5000_col_6000": lambda: dense_line_lp(5, 8000, 9000, 5000, 6000),
     "row_10000": lambda: dense_line_lp(6, 3000, 12000, 10000, 100),
     "col_4097": lambda: dense_line_lp(7, 6000, 5000, 100, 4097),
+    # 265 groups of 2,048: more group sums than one pass of the combining
+    # workgroup fetches.
+    "row_540000": lambda: dense_line_lp(8, 2000, 560000, 540000, 100, per_col=1),
 }
-# Iteration caps: the first LP is slow to converge, so its trajectory is
-# compared over its first 6,000 iterations (status: iteration limit).
-CAPS = {"row_5000_col_6000": 6000, "row_10000": 200000, "col_4097": 200000}
+# Iteration caps: the first and last LPs are slow to converge, so their
+# trajectories are compared over their first iterations (status: iteration
+# limit).
+CAPS = {"row_5000_col_6000": 6000, "row_10000": 200000, "col_4097": 200000, "row_540000": 400}
 
 
 _ORACLE = {}
@@ -100,3 +87,19 @@ def test_batch_and_family_refuse_long_rows(engine):
         engine.batch([lp], p)
     with pytest.raises(EngineError, match="4096"):
         engine.family(lp, p, lanes=64)
+
+
+@pytest.mark.parametrize("name", sorted(long_line_infeasible_lps()))
+def test_pdlp_long_lines_certificates_match_oracle(engine, name):
+    # Rays through a dense row and a dense column: the certificate's dd
+    # products of those lines are split over workgroups in the chunked order,
+    # and the status, iteration and iterates equal or_pdlp's.
+    # Engine statuses: 1 infeasible (the oracle's 21); 2 or 3 unbounded (22),
+    # 2 when the primal residual also meets the tolerance.
+    kind, lp = long_line_infeasible_lps()[name]
+    xo, yo, info = Oracle(lp).pdlp(max_iters=100000, tol=1e-6)
+    r = engine.solve(lp, default_params(mode=MODE_PDLP, max_iters=100000, tol=1e-6))
+    assert info.status == {"dual": 21, "primal": 22}[kind]
+    want = 1 if info.status == 21 else (2 if info.rel_p <= 1e-6 else 3)
+    assert r.status == want and r.iterations == info.iterations and r.info.restarts == info.restarts
+    assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)

@@ -70,11 +70,12 @@ def test_pdlp_oracle_units_follow_unscale_vectors():
 def _linprog_status(lp):
     """scipy's HiGHS on the same LP: 0 optimal, 2 infeasible, 3 unbounded."""
     from scipy.optimize import linprog
-    A = lpkit._dense(lp)
+    import scipy.sparse as sp
+    A = lpkit._csc(lp).tocsr()
     fin = lambda v: np.isfinite(v) & (np.abs(v) < 1e20)
     ub_rows = [i for i in range(lp.m) if fin(lp.ru[i])]
     lb_rows = [i for i in range(lp.m) if fin(lp.rl[i])]
-    A_ub = np.vstack([A[ub_rows], -A[lb_rows]])
+    A_ub = sp.vstack([A[ub_rows], -A[lb_rows]])
     b_ub = np.concatenate([lp.ru[ub_rows], -lp.rl[lb_rows]])
     bounds = [(lp.l[j] if fin(lp.l[j]) else None, lp.u[j] if fin(lp.u[j]) else None) for j in range(lp.n)]
     return linprog(lp.c, A_ub=A_ub, b_ub=b_ub, bounds=bounds, method="highs").status
@@ -91,6 +92,17 @@ def test_pdlp_oracle_certifies_infeasible_and_unbounded(name):
     assert info.status == want[0], (name, info.status, info.iterations)
     assert _linprog_status(lp) == want[1]
     assert info.iterations < 20000
+
+
+@pytest.mark.parametrize("name", sorted(lpkit.long_line_infeasible_lps()))
+def test_pdlp_oracle_certificates_through_long_lines(name):
+    # The same pairing with HiGHS where the rays' products run over a row and
+    # a column of more than 4,096 nonzeros (the chunked dd order).
+    kind, lp = lpkit.long_line_infeasible_lps()[name]
+    x, y, info = Oracle(lp).pdlp(max_iters=100000, tol=1e-6)
+    want = {"dual": (21, 2), "primal": (22, 3)}[kind]
+    assert info.status == want[0], (name, info.status, info.iterations)
+    assert _linprog_status(lp) == want[1]
 
 
 @pytest.mark.parametrize("name", ["transport", "unit11", "staircase"])
