@@ -20,8 +20,9 @@ of the same instance.
 Other workloads: --workload small-lp (the reference's CPU-runnable 2,000 x
 5,000 case, GPU and CPU times to 1e-6), --workload mixed (mixed precision on
 the same instance),
---workload planning512 (512 LPs in the batched persistent engine) and
---workload warmstart (learned warm start on a parametric family).
+--workload planning512 (512 LPs in the batched persistent engine),
+--workload warmstart (learned warm start on a parametric family) and
+--workload dense-row (iteration time on LPs with one dense row).
 """
 import argparse
 import json
@@ -723,6 +724,70 @@ def family_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def dense_row_lp(dense_len, m=200_000, n=600_000, seed=11):
+    """Equality LP (x in [0, 10], feasible by construction) with 3 nonzeros
+    per column at uniform rows plus one row (row 3) of `dense_len` nonzeros
+    at distinct random columns; N(0,1) values."""
+    import numpy as np
+    import scipy.sparse as sp
+    from nirnaya_amd.engine import LP
+    rng = np.random.default_rng(seed)
+    ri = rng.integers(0, m, 3 * n)
+    ci = np.repeat(np.arange(n), 3)
+    if dense_len:
+        ri = np.concatenate([ri, np.full(dense_len, 3)])
+        ci = np.concatenate([ci, rng.choice(n, dense_len, replace=False)])
+    A = sp.csc_matrix((rng.standard_normal(ri.size), (ri, ci)), shape=(m, n))
+    A.sum_duplicates()
+    A.sort_indices()
+    b = A @ rng.uniform(0, 1, n)
+    return LP(m, n, A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data.astype(np.float64),
+              rng.uniform(0, 1, n), np.zeros(n), np.full(n, 10.0), b.copy(), b.copy())
+
+
+def dense_row_workload(args):
+    """Iteration time on LPs with one dense row (200,000 x 600,000, 3
+    nonzeros per column, one row of 0 / 10,000 / 100,000 / 600,000): reference
+    mode sums that row sequentially on one workgroup, as the reference's loop
+    does; pdlp mode splits it over workgroups in its chunked order. Each mode
+    runs `warmup` then `steps` iterations with the stop rule off (pdlp mode
+    still evaluates, restarts and tests certificates every 64 iterations);
+    the line reports pdlp mode's iterations/s on the longest row and both
+    modes' time per iteration for every row length."""
+    from nirnaya_amd.engine import Engine, MODE_PDLP, MODE_REFERENCE, default_params
+    eng = Engine(0)
+    rows = {}
+    for d in args.dense_len:
+        lp = dense_row_lp(d)
+        per = {}
+        for name, mode in (("reference", MODE_REFERENCE), ("pdlp", MODE_PDLP)):
+            p = default_params(mode=mode, max_iters=args.warmup + args.steps + 1, check_termination=0, tol=1e-300)
+            t0 = time.perf_counter()
+            s = eng.session(lp, p)
+            per[name + "_setup_s"] = round(time.perf_counter() - t0, 3)
+            try:
+                s.run(args.warmup)
+                t0 = time.perf_counter()
+                s.run(args.steps)
+                s.result()
+                per[name + "_us_per_iteration"] = round((time.perf_counter() - t0) / args.steps * 1e6, 1)
+            finally:
+                s.close()
+        rows[str(d)] = per
+        log(f"[bench] dense row of {d}: {per}")
+    eng.close()
+    longest = rows[str(args.dense_len[-1])]
+    print(json.dumps({
+        "metric": "PDHG iterations/s on an LP with one dense row (pdlp mode, FP64)",
+        "value": round(1e6 / longest["pdlp_us_per_iteration"], 1), "unit": "iterations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"equality LP m=200000 n=600000, 3 nonzeros per column + one row of "
+                               f"{args.dense_len[-1]} nonzeros, seed 11"},
+        "per_dense_row_length": rows,
+        "note": "wall-clock over the timed iterations; set-up (layout autotuning, upload, scaling, step sizes) "
+                "timed apart; reference mode keeps the reference's sequential order in the dense row"}), flush=True)
+
+
 def gemm_workload(args):
     """The warm start's bf16 MFMA GEMM through nir_mlp_gemm_check (FP32
     inputs rounded to bf16 on the device, C = A B^T in FP32) at a large shape
@@ -770,11 +835,14 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the 256 MiB copy kernel (reference for rocprofv3 byte counters)")
     ap.add_argument("--workload", default="large-lp",
-                    choices=["large-lp", "small-lp", "mixed", "planning512", "warmstart", "family", "gemm"],
+                    choices=["large-lp", "small-lp", "mixed", "planning512", "warmstart", "family", "gemm",
+                             "dense-row"],
                     help="large-lp: the headline line (default); small-lp: the reference's CPU-runnable "
                          "2,000 x 5,000 case; mixed: mixed precision on the large instance; "
                          "planning512: the batched workload; warmstart: the learned warm start on a parametric "
                          "family")
+    ap.add_argument("--dense-len", type=lambda t: [int(v) for v in t.split(",")], default=[0, 10000, 100000, 600000],
+                    help="dense-row: lengths of the dense row, comma-separated (the last one gives the value)")
     ap.add_argument("--batch-only", action="store_true",
                     help="planning512: skip the one-at-a-time sample (batch solves only)")
     ap.add_argument("--ws-m", type=int, default=50000)
@@ -808,6 +876,9 @@ def main():
         return
     if args.workload == "gemm":
         gemm_workload(args)
+        return
+    if args.workload == "dense-row":
+        dense_row_workload(args)
         return
     if args.workload == "small-lp":
         small_lp_workload(args)

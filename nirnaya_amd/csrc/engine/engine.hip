@@ -54,10 +54,13 @@ struct RefState {
     unsigned long long vslot[2];   // max row violation of iteration k in slot k & 1 (bits of a double >= 0)
 };
 
-template <int TILE, typename MV, typename T>
+// y = M v. ORDER as in stream_rows (1: pdlp mode's order with split rows;
+// one-pass layouts only).
+template <int TILE, typename MV, typename T, int ORDER = 0>
 __global__ __launch_bounds__(kThreads) void k_spmv_store(MV P, const T* __restrict__ v, T* __restrict__ out) {
     __shared__ T prod[lds_slots(TILE)];
-    stream_rows<TILE>(P, v, prod, blockIdx.x, [&](int r, T s) { out[r] = s; });
+    if constexpr (ORDER == 1) stream_rows<TILE, 1>(P, v, prod, blockIdx.x, [&](int r, T s) { out[r] = s; });
+    else stream_rows<TILE>(P, v, prod, blockIdx.x, [&](int r, T s) { out[r] = s; });
 }
 
 template <int TILE, typename MV>
@@ -388,7 +391,12 @@ void spmv_store(const CsrShape& S, const T* val, T* part, const T* v, T* out, hi
     } else {
         for (int p = 0; p < S.nseg; ++p) {
             const Pass<T> P = S.pass(val, part, p);
-            TILE_DISPATCH(S.shape.tile, k_spmv_store<TL><<<P.M.nblk, kThreads, 0, s>>>(P, v, out));
+            // A layout built for pdlp mode (split rows) is timed in its order.
+            if (S.shape.split) {
+                TILE_DISPATCH(S.shape.tile, (k_spmv_store<TL, Pass<T>, T, 1><<<P.M.nblk, kThreads, 0, s>>>(P, v, out)));
+            } else {
+                TILE_DISPATCH(S.shape.tile, k_spmv_store<TL><<<P.M.nblk, kThreads, 0, s>>>(P, v, out));
+            }
         }
     }
     HIP_CHECK(hipGetLastError());
@@ -512,11 +520,11 @@ unsigned long long layout_signature(int nrows, const std::vector<int>& rowptr, c
 }
 
 TileShape autotune(int nrows, int ncols, const std::vector<int>& rowptr, const int* col, const double* val,
-                   hipStream_t s, bool fp32, bool allow_fused, bool allow_segments) {
+                   hipStream_t s, bool fp32, bool allow_fused, bool allow_segments, bool split) {
     static std::mutex mu;
     static std::map<std::tuple<int, int, int, int, unsigned long long>, TileShape> cache;
     const auto key = std::make_tuple(nrows, ncols, rowptr.back(),
-                                     (fp32 ? 1 : 0) + (allow_fused ? 2 : 0) + (allow_segments ? 4 : 0),
+                                     (fp32 ? 1 : 0) + (allow_fused ? 2 : 0) + (allow_segments ? 4 : 0) + (split ? 8 : 0),
                                      layout_signature(nrows, rowptr, col));
     {
         std::lock_guard<std::mutex> lock(mu);
@@ -539,7 +547,7 @@ TileShape autotune(int nrows, int ncols, const std::vector<int>& rowptr, const i
         std::vector<TileShape> cands;
         for (int t : {1024, 2048, 4096})
             for (int f : {0, 1})
-                if (f == 0 || (w > 0 && allow_fused)) cands.push_back(TileShape{t, 256, w, f});
+                if (f == 0 || (w > 0 && allow_fused)) cands.push_back(TileShape{t, 256, w, f, split ? 1 : 0});
         widths.push_back(w);
         per_width.push_back(std::move(cands));
     }
@@ -1000,8 +1008,8 @@ NIR_API int nir_session_create(nir_ctx* ctx, const nir_lp* lp, const nir_params*
         // matrix with such rows keeps one segment.
         const bool seg_a = !pdlp || max_row_length(a_ptr) <= kLongRow;
         const bool seg_t = !pdlp || max_row_length(at_ptr) <= kLongRow;
-        TileShape ta = tune ? autotune(m, n, a_ptr, a_col.data(), a_val.data(), S->s, fp32, !pdlp, seg_a) : base;
-        TileShape tt = tune ? autotune(n, m, at_ptr, lp->row_index, lp->value, S->s, fp32, !pdlp, seg_t) : base_t;
+        TileShape ta = tune ? autotune(m, n, a_ptr, a_col.data(), a_val.data(), S->s, fp32, !pdlp, seg_a, pdlp) : base;
+        TileShape tt = tune ? autotune(n, m, at_ptr, lp->row_index, lp->value, S->s, fp32, !pdlp, seg_t, pdlp) : base_t;
         if (!seg_a) ta.seg_width = 0, ta.fused = 0;
         if (!seg_t) tt.seg_width = 0, tt.fused = 0;
         // ... and splits each such row over workgroups (pieces of whole

@@ -54,10 +54,12 @@ def test_default_run_prints_the_contract_line():
 @pytest.mark.parametrize("argv", [
     ["--workload", "mixed", "--steps", "16", "--warmup", "4", "--no-ttt", "--no-cpu-baseline"],
     ["--workload", "family", "--steps", "64", "--fam-lanes", "128", "--ws-m", "5000", "--ws-n", "12000"],
-], ids=["mixed", "family"])
+    ["--workload", "dense-row", "--steps", "64", "--warmup", "8", "--dense-len", "0,20000"],
+], ids=["mixed", "family", "dense-row"])
 def test_other_workloads_print_one_line(argv):
     # The secondary workloads' lines (mixed precision on the large LP, the
-    # family engine on a small family) parse and carry a positive value.
+    # family engine on a small family, the dense-row LPs) parse and carry a
+    # positive value.
     r = subprocess.run([sys.executable, "-u", BENCH] + argv, capture_output=True, text=True, timeout=300, cwd=ROOT)
     assert r.returncode == 0, r.stderr[-2000:]
     lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
