@@ -28,8 +28,10 @@
  *                                                     hand-computed factors; the final
  *                                                     pass = one PC pass on the Ruiz result
  *   or_pdlp_row_sum  build-defined product order      closed forms on rows whose tiny
- *                    (rows > 4096 chunked 64 / 32)    terms survive only per chunk and
- *                                                     per group (test_oracle_pdlp.py)
+ *                    (rows > 4096 chunked 64 / 32;    terms survive only per chunk and
+ *                    also the Pock-Chambolle measure  per group (test_oracle_pdlp.py);
+ *                    and, in dd, the certificate)     HiGHS verdicts on rays through
+ *                                                     long rows and columns
  *   or_pdlp          build-defined restarted PDHG     KKT = 0 at known optima,
  *                                                     scaling invariance, HiGHS optimum;
  *                    + ray certificates at restarts   status 21 / 22 exactly on LPs
@@ -108,10 +110,12 @@ typedef struct {
  * R (m) and C (n) are exact powers of two; scaled A' = diag(R) A diag(C). */
 void or_scale_pdlp(const or_lp* lp, int32_t ruiz_passes, int32_t pc_passes, double* R, double* C);
 
-/* pdlp mode's sum of one row's (or column's) products val[p] * v[idx[p]]:
- * sequential from 0.0 for len <= 4096; above that, chunks of 64 positions
- * summed from 0.0, the chunk sums from 0.0 in groups of 32, the group sums
- * from 0.0 into the result. */
+/* pdlp mode's sum of one row's (or column's) terms val[p] * v[idx[p]]
+ * (|val[p]| when v is NULL: the Pock-Chambolle row measure): sequential from
+ * 0.0 for len <= 4096; above that, chunks of 64 positions summed from 0.0,
+ * the chunk sums from 0.0 in groups of 32, the group sums from 0.0 into the
+ * result. The certificate's double-double products of lines above 4096
+ * nonzeros follow the same chunks and groups in double-double. */
 double or_pdlp_row_sum(const double* val, const int32_t* idx, int64_t len, const double* v);
 
 /* The engine's pdlp mode in FP64 (see pdhg_oracle.c). x (n) and y (m) are
