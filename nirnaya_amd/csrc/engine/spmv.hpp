@@ -188,16 +188,39 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
                     }
                 }
             } else if (tid == 0) {
-                // The same sequential sum, eight LDS reads in flight per step
-                // so that the chain of adds, not the reads, sets the pace.
+                // The same sequential sum, its LDS reads double-buffered: the
+                // next eight products are read while the current eight are
+                // added, so the chain of adds, not the LDS latency, sets the
+                // pace.
                 const int e = p1 - c0 < TILE ? p1 - c0 : TILE;
                 int q = 0;
-                for (; q + 8 <= e; q += 8) {
-                    T a[8];
+                if (e >= 8) {
+                    T a[8], b2[8];
+                    auto read = [&](T(&r)[8], int at) {
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) a[k] = prod[lds_at(q + k)];
+                        for (int k = 0; k < 8; ++k) r[k] = prod[lds_at(at + k)];
+                        __builtin_amdgcn_sched_barrier(0);   // issue the reads before the adds
+                    };
+                    auto add = [&](const T(&r)[8]) {
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) s += a[k];
+                        for (int k = 0; k < 8; ++k) s += r[k];
+                    };
+                    read(a, 0);
+                    for (; q + 24 <= e; q += 16) {   // a holds [q, q + 8)
+                        read(b2, q + 8);
+                        add(a);
+                        read(a, q + 16);
+                        add(b2);
+                    }
+                    if (q + 16 <= e) {
+                        read(b2, q + 8);
+                        add(a);
+                        add(b2);
+                        q += 16;
+                    } else {
+                        add(a);
+                        q += 8;
+                    }
                 }
                 for (; q < e; ++q) s += prod[lds_at(q)];
             }
