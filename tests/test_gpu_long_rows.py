@@ -103,3 +103,29 @@ def test_pdlp_long_lines_certificates_match_oracle(engine, name):
     want = 1 if info.status == 21 else (2 if info.rel_p <= 1e-6 else 3)
     assert r.status == want and r.iterations == info.iterations and r.info.restarts == info.restarts
     assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+
+
+def test_drop_in_boundary_solves_a_dense_row_lp():
+    # The whole sov_* pipeline in pdlp mode (validate, presolve, engine with
+    # the split row, unscale, dd certificate) on a 1,000 x 6,000 LP with a row
+    # of 5,000 nonzeros: Optimal with the certificate passing, at HiGHS's
+    # objective.
+    import scipy.sparse as sp
+    from scipy.optimize import linprog
+    from nirnaya_amd.nirnaya import NirnayaModel
+    lp = dense_line_lp(6, 1000, 6000, 5000, 100)
+    A = sp.csc_matrix((lp.value, lp.index, lp.start), shape=(lp.m, lp.n)).tocsr()
+    m = NirnayaModel("dense_row")
+    for k, v in (("log_level", "0"), ("method", "pdhg"), ("pdhg_mode", "pdlp")):
+        assert m.set_option(k, v) == 0
+    for j in range(lp.n):
+        m.add_var(float(lp.c[j]), float(lp.l[j]), float(lp.u[j]))
+    for i in range(lp.m):
+        a, b = A.indptr[i], A.indptr[i + 1]
+        m.add_constr([int(j) for j in A.indices[a:b]], [float(v) for v in A.data[a:b]], float(lp.rl[i]),
+                     float(lp.ru[i]))
+    assert m.optimize() == 0
+    assert m.Status == 0
+    ref = linprog(lp.c, A_eq=A, b_eq=lp.rl, bounds=list(zip(lp.l, lp.u)), method="highs")
+    assert ref.status == 0
+    assert abs(m.ObjVal - ref.fun) <= 1e-5 * (1 + abs(ref.fun))
