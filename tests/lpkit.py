@@ -196,3 +196,40 @@ def read_sol(path):
 
 CRUDE_MPS = os.path.join(GOLDEN, "MRPL_Crude_Blending_LP.mps")
 CRUDE_SOL = os.path.join(GOLDEN, "MRPL_Crude_Blending_LP.sol")
+
+
+def random_long_line_lp(seed):
+    """Seeded LP of random shape with 1-3 rows and 0-2 columns of 4,097 to
+    ~12,000 nonzeros beside 1-4 random nonzeros per column, and mixed bound
+    types: free, one-sided and boxed columns; equality, one-sided, ranged
+    and free rows. Feasible (rows hold A x0 for x0 inside the bounds)."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(7000 + seed)
+    m = int(rng.integers(4200, 7000))
+    n = int(rng.integers(4500, 13000))
+    per = rng.integers(1, 5, n)
+    ri = [rng.integers(0, m, int(per.sum()))]
+    ci = [np.repeat(np.arange(n), per)]
+    for r in rng.choice(m, int(rng.integers(1, 4)), replace=False):
+        k = int(rng.integers(4097, min(n, 12000) + 1))
+        ri.append(np.full(k, r))
+        ci.append(rng.choice(n, k, replace=False))
+    for c in rng.choice(n, int(rng.integers(0, 3)), replace=False):
+        k = int(rng.integers(4097, m + 1))
+        ri.append(rng.choice(m, k, replace=False))
+        ci.append(np.full(k, c))
+    ri, ci = np.concatenate(ri), np.concatenate(ci)
+    A = sp.csc_matrix((rng.standard_normal(ri.size), (ri, ci)), shape=(m, n))
+    A.sum_duplicates()
+    A.sort_indices()
+    kind = rng.integers(0, 4, n)   # 0 boxed, 1 lower only, 2 upper only, 3 free
+    l = np.where((kind == 0) | (kind == 1), 0.0, -INF)
+    u = np.where(kind == 0, rng.uniform(1, 10, n), np.where(kind == 2, 10.0, INF))
+    x0 = np.clip(rng.uniform(-1, 5, n), np.where(np.isfinite(l), l, -1), np.where(np.isfinite(u), u, 5))
+    b = A @ x0
+    rk = rng.integers(0, 4, m)     # 0 equality, 1 >=, 2 <=, 3 ranged
+    rl = np.where(rk == 2, -INF, b - np.where(rk == 0, 0.0, rng.uniform(0, 1, m)))
+    ru = np.where(rk == 1, INF, b + np.where(rk == 0, 0.0, rng.uniform(0, 1, m)))
+    c = A.T @ rng.standard_normal(m) + np.where(kind == 3, 0.0, rng.uniform(0, 1, n))
+    return LP(m, n, A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data.astype(np.float64), c, l, u,
+              rl, ru)

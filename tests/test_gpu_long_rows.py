@@ -129,3 +129,21 @@ def test_drop_in_boundary_solves_a_dense_row_lp():
     ref = linprog(lp.c, A_eq=A, b_eq=lp.rl, bounds=list(zip(lp.l, lp.u)), method="highs")
     assert ref.status == 0
     assert abs(m.ObjVal - ref.fun) <= 1e-5 * (1 + abs(ref.fun))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_long_line_lps_match_oracle(engine, seed):
+    # Random shapes, 1-3 long rows and 0-2 long columns of random lengths
+    # (random piece counts and ragged last groups), mixed bound types: the
+    # first 3,008 pdlp iterations in FP64 (the cap rounded up to the
+    # 64-iteration evaluation) equal or_pdlp's bit for bit, and the mixed
+    # precision engine runs the same LPs within the cap.
+    from lpkit import random_long_line_lp
+    lp = random_long_line_lp(seed)
+    xo, yo, info = Oracle(lp).pdlp(max_iters=3000, tol=1e-6)
+    r = engine.solve(lp, default_params(mode=MODE_PDLP, max_iters=3000, tol=1e-6))
+    print(f"seed {seed}: {lp.m} x {lp.n}, {lp.nnz} nonzeros, status {r.status}, it {r.iterations}/{info.iterations}")
+    assert r.iterations == info.iterations and r.info.restarts == info.restarts
+    assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+    mx = engine.solve(lp, default_params(mode=MODE_PDLP, precision=PREC_MIXED, max_iters=3000, tol=1e-6))
+    assert mx.iterations <= 3000 + 64
