@@ -139,6 +139,10 @@ NIR_API int nir_session_time(nir_session* s, int64_t iters, nir_timing* out);
  * most 16,384 nonzeros, 2n + m <= 19,200, rows of A and A^T of at most 512
  * nonzeros, when no tile or segment width is given). */
 NIR_API int nir_session_shape(nir_session* s, int32_t* out8);
+/* Rows split over workgroups (pdlp mode, rows of more than 4,096 nonzeros in a
+ * one-segment layout): out4 = {split rows of A, their pieces, split rows of
+ * A^T (columns of A), their pieces}; all 0 in reference mode. */
+NIR_API int nir_session_split(nir_session* s, int32_t* out4);
 /* Relative KKT error of a reference-mode session's current iterate (x, y) on
  * its LP, evaluated on the device with double-double products and sums:
  * out5 = {relative primal, relative dual, relative gap, primal objective, dual

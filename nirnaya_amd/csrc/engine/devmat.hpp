@@ -101,7 +101,7 @@ struct CsrShape {
     DevArray<int4> lrow;     // per split row {first group slot, groups, pieces, row}
     DevArray<int> lcount;    // per split row: zero between launches
     DevArray<double> gsum;   // group sums, FP64 or FP32 values
-    int nsplit = 0, ngroups = 0;
+    int nsplit = 0, ngroups = 0, npieces = 0;
 
     template <typename T>
     Csr<T> view(const T* val, int s) const {
@@ -251,6 +251,8 @@ inline void set_blocks(CsrShape& S, const SegHost& H, TileShape shape, hipStream
     S.desc.upload(desc_all.data(), desc_all.size(), st);
     S.nsplit = static_cast<int>(lrow.size());
     S.ngroups = groups;
+    S.npieces = 0;
+    for (const int4& L : lrow) S.npieces += L.z;
     S.lrow.release();
     S.lcount.release();
     S.gsum.release();

@@ -1223,6 +1223,15 @@ NIR_API int nir_session_shape(nir_session* S, int32_t* out8) {
     return 0;
 }
 
+NIR_API int nir_session_split(nir_session* S, int32_t* out4) {
+    if (!S || !out4) return -1;
+    out4[0] = S->A.shape.nsplit;
+    out4[1] = S->A.shape.npieces;
+    out4[2] = S->AT.shape.nsplit;
+    out4[3] = S->AT.shape.npieces;
+    return 0;
+}
+
 NIR_API int nir_calibrate_copy(nir_ctx* ctx, int64_t bytes, double* ms) {
     if (!ctx || bytes <= 0) return -1;
     try {

@@ -154,6 +154,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.nir_session_time.argtypes = [vp, C.c_int64, C.POINTER(NirTiming)]
     lib.nir_session_destroy.argtypes = [vp]
     lib.nir_session_shape.argtypes = [vp, _i32p]
+    lib.nir_session_split.argtypes = [vp, _i32p]
     lib.nir_session_kkt.argtypes = [vp, _dp]
     lib.nir_session_gather_floor.argtypes = [vp, C.c_int32, C.c_int32, _dp]
     lib.nir_session_scaling.argtypes = [vp, _dp, _dp]
@@ -578,6 +579,12 @@ class Session:
         """(A tile, A rows, A segments, A fused, A^T tile, A^T rows, A^T segments, A^T fused)."""
         out = (C.c_int32 * 8)()
         self.lib.nir_session_shape(self.h, out)
+        return tuple(out)
+
+    def split(self):
+        """(split rows of A, their pieces, split rows of A^T, their pieces)."""
+        out = (C.c_int32 * 4)()
+        self.lib.nir_session_split(self.h, out)
         return tuple(out)
 
     def kkt(self) -> np.ndarray:

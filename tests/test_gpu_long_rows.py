@@ -51,6 +51,23 @@ def test_pdlp_long_rows_match_oracle(engine, name, tile):
     assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
 
 
+@pytest.mark.parametrize("tile,pieces", [(1024, (3, 3)), (2048, (3, 3)), (4096, (2, 2))])
+def test_pdlp_splits_long_rows_into_pieces(engine, tile, pieces):
+    # One row of 5,000 and one column of 6,000 nonzeros: pieces of
+    # max(tile, 2,048) nonzeros each; reference mode splits nothing.
+    lp = CASES["row_5000_col_6000"]()
+    s = engine.session(lp, default_params(mode=MODE_PDLP, max_iters=100, tol=1e-6, tile_nnz=tile))
+    try:
+        assert s.split() == (1, pieces[0], 1, pieces[1])
+    finally:
+        s.close()
+    s = engine.session(lp, default_params(mode=MODE_REFERENCE, max_iters=100, tile_nnz=tile))
+    try:
+        assert s.split() == (0, 0, 0, 0)
+    finally:
+        s.close()
+
+
 def test_pdlp_long_rows_ignore_segments(engine):
     # A segment width would cut the dense row: the session keeps one segment,
     # and the result is the unsegmented one.
