@@ -109,11 +109,13 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
     const bool piece = ORDER == 1 && r1 < 0;
     if (!piece && r1 <= r0) return;   // empty matrix: the one block has no rows
     if (piece || cnt > TILE) {
-        // One row longer than a tile: a chain of tiles. All lanes park one
-        // tile of the row's products in LDS, then lane 0 adds them in column
-        // order to the running sum, so the row keeps the sequential order and
-        // rounding of a CSR dot product at any length. The next tile's index
-        // and value loads are issued before lane 0 sums the current one.
+        // One row longer than a tile, or one piece of a split row: a chain of
+        // tiles. All lanes park one tile of the row's products in LDS, then
+        // lane 0 adds them in column order to the running sum, so the row
+        // keeps the sequential order and rounding of a CSR dot product at any
+        // length (ORDER == 0, and rows up to kLongRow nonzeros); the chunked
+        // order below otherwise. The next tile's index and value loads are
+        // issued before the current one is summed.
         constexpr int PER = TILE / kThreads;
         T s = P.first ? T(0) : P.part[r0];   // lane 0's copy is the one used
         int ci[PER];
@@ -130,9 +132,9 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
                 vi[u] = in ? vl : T(0);
             }
         };
-        // ORDER == 1 and more than kLongRow nonzeros (pdlp mode; such rows are
-        // never split by column segments, so the row starts at p0): the
-        // chunked order. Products are summed from 0.0 in chunks of kChunk
+        // ORDER == 1 and more than kLongRow nonzeros (pdlp mode; such a row
+        // is never cut by column segments, so it starts at p0, or a piece
+        // starts on one of its group boundaries): the chunked order. Products are summed from 0.0 in chunks of kChunk
         // consecutive positions, chunk sums from 0.0 in groups of
         // kGroupChunks, group sums from 0.0 into the row sum - each level
         // sequential, so 32 (or 64) lanes sum a tile's chunks at once and
