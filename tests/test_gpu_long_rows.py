@@ -120,6 +120,10 @@ def test_pdlp_long_lines_certificates_match_oracle(engine, name):
     want = 1 if info.status == 21 else (2 if info.rel_p <= 1e-6 else 3)
     assert r.status == want and r.iterations == info.iterations and r.info.restarts == info.restarts
     assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
+    # Mixed precision (FP32 iterations, FP64 certificate products) reaches
+    # the same verdict.
+    mx = engine.solve(lp, default_params(mode=MODE_PDLP, precision=PREC_MIXED, max_iters=100000, tol=1e-6))
+    assert mx.status == 1 if kind == "dual" else mx.status in (2, 3)
 
 
 def test_drop_in_boundary_solves_a_dense_row_lp():
