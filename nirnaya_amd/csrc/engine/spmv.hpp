@@ -193,7 +193,9 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
                 // The same sequential sum, its LDS reads double-buffered: the
                 // next eight products are read while the current eight are
                 // added, so the chain of adds, not the LDS latency, sets the
-                // pace.
+                // pace. (Batches of 16 would hide more of the latency but
+                // raise the whole kernel's VGPR count - 74 to 118 for the
+                // 2048 tile - and cut the occupancy of the short-row blocks.)
                 const int e = p1 - c0 < TILE ? p1 - c0 : TILE;
                 int q = 0;
                 if (e >= 8) {
