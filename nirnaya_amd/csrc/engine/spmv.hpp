@@ -94,7 +94,11 @@ constexpr int kChunk = 64;        // products per chunk
 constexpr int kGroupChunks = 32;  // chunks per group
 constexpr int kGroupLen = kChunk * kGroupChunks;   // products per group (2048)
 
-template <int TILE, int ORDER = 0, typename T, typename V, typename Pre, typename F>
+// LONG selects the blocks a launch serves: 2 all, 0 only blocks of short rows
+// (the long-row code is then not compiled in, so it adds nothing to the
+// kernel's register count), 1 only blocks of one row longer than a tile and
+// pieces of split rows.
+template <int TILE, int ORDER = 0, int LONG = 2, typename T, typename V, typename Pre, typename F>
 __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restrict__ v, T* prod, int b,
                                             Pre&& prefetch, F&& on_row) {
     static_assert(TILE % kThreads == 0, "tile must be a multiple of the workgroup");
@@ -108,7 +112,10 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
     // pieces of whole groups (below), one workgroup each.
     const bool piece = ORDER == 1 && r1 < 0;
     if (!piece && r1 <= r0) return;   // empty matrix: the one block has no rows
-    if (piece || cnt > TILE) {
+    const bool long_block = piece || cnt > TILE;
+    if constexpr (LONG == 0) {
+        if (long_block) return;   // served by the LONG == 1 launch
+    } else if (long_block) {
         // One row longer than a tile, or one piece of a split row: a chain of
         // tiles. All lanes park one tile of the row's products in LDS, then
         // lane 0 adds them in column order to the running sum, so the row
@@ -262,6 +269,7 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
         }
         return;
     }
+    if constexpr (LONG == 1) return;
     // Row bookkeeping and epilogue operands first: their latency overlaps the tile.
     const int r = r0 + tid;
     const bool has_row = r < r1;
@@ -307,9 +315,9 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
 }
 
 // Same without epilogue prefetch.
-template <int TILE, int ORDER = 0, typename T, typename V, typename F>
+template <int TILE, int ORDER = 0, int LONG = 2, typename T, typename V, typename F>
 __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restrict__ v, T* prod, int b, F&& on_row) {
-    stream_rows<TILE, ORDER>(P, v, prod, b, PreNone{}, [&](int r, T s, NoPre) { on_row(r, s); });
+    stream_rows<TILE, ORDER, LONG>(P, v, prod, b, PreNone{}, [&](int r, T s, NoPre) { on_row(r, s); });
 }
 
 // ------------------------------------------------------------------ fused segments
