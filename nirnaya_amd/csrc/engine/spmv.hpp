@@ -23,6 +23,7 @@
 // ascending column order and each pass continues the same sequential sum, the
 // result is still bit-identical to the single-pass product.
 #pragma once
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 namespace nir {
@@ -281,21 +282,45 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
     constexpr int PER = TILE / kThreads;
     int ci[PER];
     T vi[PER];
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int q = tid + u * kThreads;
-        const bool in = q < cnt;
-        const int slot = in ? p0 + q : 0;
-        const int cl = load_stream(A.col + slot);
-        const T vl = load_stream(A.val + slot);
-        ci[u] = in ? cl : 0;
-        vi[u] = in ? vl : T(0);
-    }
     T g[PER];
+    if constexpr (std::is_same<T, float>::value) {
+        // FP32 products: all gathers issue together once the indices are in
+        // (a scheduling barrier keeps them after the index loads; the values
+        // are masked only at the products). Measured faster for FP32 and
+        // slower for FP64, which keeps the compiler's interleaving below.
 #pragma unroll
-    for (int u = 0; u < PER; ++u) g[u] = static_cast<T>(v[ci[u]]);
+        for (int u = 0; u < PER; ++u) {
+            const int q = tid + u * kThreads;
+            const bool in = q < cnt;
+            const int slot = in ? p0 + q : 0;
+            const int cl = load_stream(A.col + slot);
+            vi[u] = load_stream(A.val + slot);
+            ci[u] = in ? cl : 0;
+        }
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int u = 0; u < PER; ++u) prod[lds_at(tid + u * kThreads)] = vi[u] * g[u];
+        for (int u = 0; u < PER; ++u) g[u] = static_cast<T>(v[ci[u]]);
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const T w = tid + u * kThreads < cnt ? vi[u] : T(0);
+            prod[lds_at(tid + u * kThreads)] = w * g[u];
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int q = tid + u * kThreads;
+            const bool in = q < cnt;
+            const int slot = in ? p0 + q : 0;
+            const int cl = load_stream(A.col + slot);
+            const T vl = load_stream(A.val + slot);
+            ci[u] = in ? cl : 0;
+            vi[u] = in ? vl : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u) g[u] = static_cast<T>(v[ci[u]]);
+#pragma unroll
+        for (int u = 0; u < PER; ++u) prod[lds_at(tid + u * kThreads)] = vi[u] * g[u];
+    }
     __syncthreads();
     if (has_row) {
         // Sequential sum in column order; four LDS reads in flight per step.
