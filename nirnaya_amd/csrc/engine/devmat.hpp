@@ -102,11 +102,9 @@ struct CsrShape {
     DevArray<int> lcount;    // per split row: zero between launches
     DevArray<double> gsum;   // group sums, FP64 or FP32 values
     int nsplit = 0, ngroups = 0, npieces = 0;
-    // Per segment: the blocks of one row longer than a tile and the pieces of
-    // split rows (indices within the segment's blocks), for launches that
-    // serve them apart from the short-row blocks.
-    DevArray<int> longblk;
-    std::vector<int> nlong, long_off;
+    // Per segment: how many blocks hold one row longer than a tile or a piece
+    // of a split row (kernels compiled without the long-row code need none).
+    std::vector<int> nlong;
 
     template <typename T>
     Csr<T> view(const T* val, int s) const {
@@ -254,19 +252,12 @@ inline void set_blocks(CsrShape& S, const SegHost& H, TileShape shape, hipStream
     }
     S.desc.alloc(desc_all.size());
     S.desc.upload(desc_all.data(), desc_all.size(), st);
-    std::vector<int> longs;
     S.nlong.assign(nseg, 0);
-    S.long_off.assign(nseg, 0);
-    for (int s = 0; s < nseg; ++s) {
-        S.long_off[s] = static_cast<int>(longs.size());
+    for (int s = 0; s < nseg; ++s)
         for (int k = 0; k < S.nblk[s]; ++k) {
             const int4 d = desc_all[static_cast<size_t>(S.blk_off[s]) + k];
-            if (d.y < 0 || (d.y == d.x + 1 && d.w - d.z > shape.tile)) longs.push_back(k);
+            if (d.y < 0 || (d.y == d.x + 1 && d.w - d.z > shape.tile)) ++S.nlong[s];
         }
-        S.nlong[s] = static_cast<int>(longs.size()) - S.long_off[s];
-    }
-    S.longblk.alloc(longs.size());
-    S.longblk.upload(longs.data(), longs.size(), st);
     S.nsplit = static_cast<int>(lrow.size());
     S.ngroups = groups;
     S.npieces = 0;

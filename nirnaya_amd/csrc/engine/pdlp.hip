@@ -79,18 +79,17 @@ __device__ __forceinline__ void block_dd_reduce(dd (&acc)[NQ], dd* sh, dd* out) 
 
 // ------------------------------------------------------------------ kernels
 
-// The iteration kernels run in two launches per product when the layout has
-// blocks of long or split rows: LONG = 0 over every block (the long-row code
-// compiled out, so the short-row blocks keep the registers and occupancy they
-// have on their own), then LONG = 1 over the list `blist` of the long-row
-// blocks only.
+// The iteration kernels are compiled twice: LONG = 0 (the long-row code
+// compiled out, for layouts without blocks of long or split rows - the
+// short-row blocks then keep the registers and occupancy they have on their
+// own) and LONG = 2 (every block, for layouts that have such blocks: they run
+// beside the short-row blocks in the same launch).
 template <int TILE, typename T, int LONG>
 __global__ __launch_bounds__(kThreads) void k_primal_pdlp(Pass<T> P, const T* __restrict__ y, T* __restrict__ x,
                                                           T* __restrict__ xbar, const T* __restrict__ xa,
                                                           T* __restrict__ xp_out, const T* __restrict__ c,
                                                           const T* __restrict__ lo, const T* __restrict__ up,
-                                                          const PdlpState* __restrict__ st, int k_lo, int save,
-                                                          const int* __restrict__ blist) {
+                                                          const PdlpState* __restrict__ st, int k_lo, int save) {
     __shared__ T prod[lds_slots(TILE)];
     if (st->done) return;
     const T tau = static_cast<T>(st->tau);
@@ -106,8 +105,7 @@ __global__ __launch_bounds__(kThreads) void k_primal_pdlp(Pass<T> P, const T* __
         if (P.last) o = Ops{x[j], c[j], lo[j], up[j], kDelta ? T(0) : xa[j]};
         return o;
     };
-    const int b = LONG == 1 ? blist[blockIdx.x] : static_cast<int>(blockIdx.x);
-    stream_rows<TILE, 1, LONG>(P, y, prod, b, pre, [&](int j, T g, const Ops& o) {
+    stream_rows<TILE, 1, LONG>(P, y, prod, blockIdx.x, pre, [&](int j, T g, const Ops& o) {
         const PrimalOut<T> r = primal_point<T>(o.x, g, o.c, o.l, o.u, o.a, tau, hw);
         xbar[j] = r.xb;
         x[j] = r.next;
@@ -119,8 +117,7 @@ template <int TILE, typename T, int LONG>
 __global__ __launch_bounds__(kThreads) void k_dual_pdlp(Pass<T> P, const T* __restrict__ xbar, T* __restrict__ y,
                                                         const T* __restrict__ ya, T* __restrict__ yp_out,
                                                         const T* __restrict__ rl, const T* __restrict__ ru,
-                                                        const PdlpState* __restrict__ st, int k_lo, int save,
-                                                        const int* __restrict__ blist) {
+                                                        const PdlpState* __restrict__ st, int k_lo, int save) {
     __shared__ T prod[lds_slots(TILE)];
     if (st->done) return;
     const T sigma = static_cast<T>(st->sigma);
@@ -135,8 +132,7 @@ __global__ __launch_bounds__(kThreads) void k_dual_pdlp(Pass<T> P, const T* __re
     };
     // FP32 runs the re-centred form; `ya` then holds sa = -Y / sigma.
     constexpr bool kDelta = std::is_same<T, float>::value;
-    const int b = LONG == 1 ? blist[blockIdx.x] : static_cast<int>(blockIdx.x);
-    stream_rows<TILE, 1, LONG>(P, xbar, prod, b, pre, [&](int i, T h, const Ops& o) {
+    stream_rows<TILE, 1, LONG>(P, xbar, prod, blockIdx.x, pre, [&](int i, T h, const Ops& o) {
         const DualOut<T> r = kDelta ? dual_point_delta<T>(o.y, h, o.l, o.u, o.a, sigma, hw)
                                     : dual_point<T>(o.y, h, o.l, o.u, o.a, sigma, hw);
         y[i] = r.next;
@@ -711,15 +707,22 @@ void launch_primal(Pdlp& P, hipStream_t s, Iterates<T>& it, int k_lo, int save) 
     const DevMatrix& M = *P.AT;
     for (int q = 0; q < M.shape.nseg; ++q) {
         const Pass<T> ps = pass_of<T>(M, P.at32, q);
-        const int nl = M.shape.nlong[q];
-        const int* bl = M.shape.longblk.p + M.shape.long_off[q];
-        TILE_DISPATCH(M.shape.shape.tile,
-                      (k_primal_pdlp<TL, T, 0><<<ps.M.nblk, kThreads, 0, s>>>(ps, it.y.p, it.x.p, it.xbar.p, it.xa.p,
-                                                                              it.xp.p, it.c.p, it.lo.p, it.up.p,
-                                                                              P.st.p, k_lo, save, nullptr));
-                      if (nl > 0) (k_primal_pdlp<TL, T, 1><<<nl, kThreads, 0, s>>>(
-                          ps, it.y.p, it.x.p, it.xbar.p, it.xa.p, it.xp.p, it.c.p, it.lo.p, it.up.p, P.st.p, k_lo,
-                          save, bl)));
+        // A second launch for the long-row blocks alone would follow the
+        // short-row launch instead of overlapping it: measured slower (a
+        // 4,000-nonzero row: +20 us per iteration; a split row of 600,000:
+        // 67 vs 56 us, a forked stream under graph capture no better), so
+        // such layouts take one LONG = 2 launch.
+        if (M.shape.nlong[q] > 0) {
+            TILE_DISPATCH(M.shape.shape.tile,
+                          (k_primal_pdlp<TL, T, 2><<<ps.M.nblk, kThreads, 0, s>>>(
+                              ps, it.y.p, it.x.p, it.xbar.p, it.xa.p, it.xp.p, it.c.p, it.lo.p, it.up.p, P.st.p, k_lo,
+                              save)));
+        } else {
+            TILE_DISPATCH(M.shape.shape.tile,
+                          (k_primal_pdlp<TL, T, 0><<<ps.M.nblk, kThreads, 0, s>>>(
+                              ps, it.y.p, it.x.p, it.xbar.p, it.xa.p, it.xp.p, it.c.p, it.lo.p, it.up.p, P.st.p, k_lo,
+                              save)));
+        }
     }
 }
 
@@ -728,14 +731,15 @@ void launch_dual(Pdlp& P, hipStream_t s, Iterates<T>& it, int k_lo, int save) {
     const DevMatrix& M = *P.A;
     for (int q = 0; q < M.shape.nseg; ++q) {
         const Pass<T> ps = pass_of<T>(M, P.a32, q);
-        const int nl = M.shape.nlong[q];
-        const int* bl = M.shape.longblk.p + M.shape.long_off[q];
-        TILE_DISPATCH(M.shape.shape.tile,
-                      (k_dual_pdlp<TL, T, 0><<<ps.M.nblk, kThreads, 0, s>>>(ps, it.xbar.p, it.y.p, it.ya.p, it.yp.p,
-                                                                            it.rl.p, it.ru.p, P.st.p, k_lo, save,
-                                                                            nullptr));
-                      if (nl > 0) (k_dual_pdlp<TL, T, 1><<<nl, kThreads, 0, s>>>(
-                          ps, it.xbar.p, it.y.p, it.ya.p, it.yp.p, it.rl.p, it.ru.p, P.st.p, k_lo, save, bl)));
+        if (M.shape.nlong[q] > 0) {
+            TILE_DISPATCH(M.shape.shape.tile,
+                          (k_dual_pdlp<TL, T, 2><<<ps.M.nblk, kThreads, 0, s>>>(
+                              ps, it.xbar.p, it.y.p, it.ya.p, it.yp.p, it.rl.p, it.ru.p, P.st.p, k_lo, save)));
+        } else {
+            TILE_DISPATCH(M.shape.shape.tile,
+                          (k_dual_pdlp<TL, T, 0><<<ps.M.nblk, kThreads, 0, s>>>(
+                              ps, it.xbar.p, it.y.p, it.ya.p, it.yp.p, it.rl.p, it.ru.p, P.st.p, k_lo, save)));
+        }
     }
 }
 

@@ -97,8 +97,8 @@ constexpr int kGroupLen = kChunk * kGroupChunks;   // products per group (2048)
 
 // LONG selects the blocks a launch serves: 2 all, 0 only blocks of short rows
 // (the long-row code is then not compiled in, so it adds nothing to the
-// kernel's register count), 1 only blocks of one row longer than a tile and
-// pieces of split rows.
+// kernel's register count; blocks of one row longer than a tile and pieces of
+// split rows return at once).
 template <int TILE, int ORDER = 0, int LONG = 2, typename T, typename V, typename Pre, typename F>
 __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restrict__ v, T* prod, int b,
                                             Pre&& prefetch, F&& on_row) {
@@ -115,7 +115,7 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
     if (!piece && r1 <= r0) return;   // empty matrix: the one block has no rows
     const bool long_block = piece || cnt > TILE;
     if constexpr (LONG == 0) {
-        if (long_block) return;   // served by the LONG == 1 launch
+        if (long_block) return;   // the layout has none when LONG == 0 is launched
     } else if (long_block) {
         // One row longer than a tile, or one piece of a split row: a chain of
         // tiles. All lanes park one tile of the row's products in LDS, then
@@ -270,7 +270,6 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
         }
         return;
     }
-    if constexpr (LONG == 1) return;
     // Row bookkeeping and epilogue operands first: their latency overlaps the tile.
     const int r = r0 + tid;
     const bool has_row = r < r1;

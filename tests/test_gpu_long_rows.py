@@ -23,11 +23,15 @@ CASES = {
     # 265 groups of 2,048: more group sums than one pass of the combining
     # workgroup fetches.
     "row_540000": lambda: dense_line_lp(8, 2000, 560000, 540000, 100, per_col=1),
+    # Longer than the 1,024 and 2,048 tiles but summed sequentially (at most
+    # 4,096): one-workgroup blocks in the long-row launch, not split.
+    "row_3000_col_2500": lambda: dense_line_lp(9, 3000, 6000, 3000, 2500),
 }
 # Iteration caps: the first and last LPs are slow to converge, so their
 # trajectories are compared over their first iterations (status: iteration
 # limit).
-CAPS = {"row_5000_col_6000": 6000, "row_10000": 200000, "col_4097": 200000, "row_540000": 400}
+CAPS = {"row_5000_col_6000": 6000, "row_10000": 200000, "col_4097": 200000, "row_540000": 400,
+        "row_3000_col_2500": 200000}
 
 
 _ORACLE = {}
@@ -66,6 +70,16 @@ def test_pdlp_splits_long_rows_into_pieces(engine, tile, pieces):
         assert s.split() == (0, 0, 0, 0)
     finally:
         s.close()
+
+
+@pytest.mark.parametrize("name", ["row_3000_col_2500", "row_10000"])
+def test_pdlp_long_rows_without_graphs(engine, name):
+    # The long-row launches run on a second stream forked from the engine's
+    # and joined back; without graph capture the same bits come out.
+    lp, (xo, yo, info) = _oracle(name)
+    r = engine.solve(lp, default_params(mode=MODE_PDLP, max_iters=CAPS[name], tol=1e-6, tile_nnz=1024, use_graph=0))
+    assert r.iterations == info.iterations and r.info.restarts == info.restarts
+    assert np.array_equal(r.x, xo) and np.array_equal(r.y, yo)
 
 
 def test_pdlp_long_rows_ignore_segments(engine):
