@@ -204,34 +204,38 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
                 // pace. (Batches of 16 would hide more of the latency but
                 // raise the whole kernel's VGPR count - 74 to 118 for the
                 // 2048 tile - and cut the occupancy of the short-row blocks.)
+                // pdlp kernels compile this branch only for layouts that have
+                // long rows, so they read 16 ahead; reference mode's kernels
+                // (one compilation for every layout) read 8 ahead.
+                constexpr int B = ORDER == 1 ? 16 : 8;
                 const int e = p1 - c0 < TILE ? p1 - c0 : TILE;
                 int q = 0;
-                if (e >= 8) {
-                    T a[8], b2[8];
-                    auto read = [&](T(&r)[8], int at) {
+                if (e >= B) {
+                    T a[B], b2[B];
+                    auto read = [&](T(&r)[B], int at) {
 #pragma unroll
-                        for (int k = 0; k < 8; ++k) r[k] = prod[lds_at(at + k)];
+                        for (int k = 0; k < B; ++k) r[k] = prod[lds_at(at + k)];
                         __builtin_amdgcn_sched_barrier(0);   // issue the reads before the adds
                     };
-                    auto add = [&](const T(&r)[8]) {
+                    auto add = [&](const T(&r)[B]) {
 #pragma unroll
-                        for (int k = 0; k < 8; ++k) s += r[k];
+                        for (int k = 0; k < B; ++k) s += r[k];
                     };
                     read(a, 0);
-                    for (; q + 24 <= e; q += 16) {   // a holds [q, q + 8)
-                        read(b2, q + 8);
+                    for (; q + 3 * B <= e; q += 2 * B) {   // a holds [q, q + B)
+                        read(b2, q + B);
                         add(a);
-                        read(a, q + 16);
+                        read(a, q + 2 * B);
                         add(b2);
                     }
-                    if (q + 16 <= e) {
-                        read(b2, q + 8);
+                    if (q + 2 * B <= e) {
+                        read(b2, q + B);
                         add(a);
                         add(b2);
-                        q += 16;
+                        q += 2 * B;
                     } else {
                         add(a);
-                        q += 8;
+                        q += B;
                     }
                 }
                 for (; q < e; ++q) s += prod[lds_at(q)];
