@@ -54,12 +54,14 @@ struct RefState {
     unsigned long long vslot[2];   // max row violation of iteration k in slot k & 1 (bits of a double >= 0)
 };
 
-// y = M v. ORDER as in stream_rows (1: pdlp mode's order with split rows;
-// one-pass layouts only).
-template <int TILE, typename MV, typename T, int ORDER = 0>
+// y = M v. ORDER and LONG as in stream_rows (ORDER 1: pdlp mode's order
+// with split rows; LONG 0: the pdlp kernels' compilation for layouts without
+// long-row blocks), so that a pdlp layout candidate is timed with the code
+// its iterations will run.
+template <int TILE, typename MV, typename T, int ORDER = 0, int LONG = 2>
 __global__ __launch_bounds__(kThreads) void k_spmv_store(MV P, const T* __restrict__ v, T* __restrict__ out) {
     __shared__ T prod[lds_slots(TILE)];
-    if constexpr (ORDER == 1) stream_rows<TILE, 1>(P, v, prod, blockIdx.x, [&](int r, T s) { out[r] = s; });
+    if constexpr (ORDER == 1) stream_rows<TILE, 1, LONG>(P, v, prod, blockIdx.x, [&](int r, T s) { out[r] = s; });
     else stream_rows<TILE>(P, v, prod, blockIdx.x, [&](int r, T s) { out[r] = s; });
 }
 
@@ -392,8 +394,12 @@ void spmv_store(const CsrShape& S, const T* val, T* part, const T* v, T* out, hi
         for (int p = 0; p < S.nseg; ++p) {
             const Pass<T> P = S.pass(val, part, p);
             // A layout built for pdlp mode (split rows) is timed in its order.
-            if (S.shape.split) {
-                TILE_DISPATCH(S.shape.tile, (k_spmv_store<TL, Pass<T>, T, 1><<<P.M.nblk, kThreads, 0, s>>>(P, v, out)));
+            if (S.shape.split && S.nlong[p] == 0) {
+                TILE_DISPATCH(S.shape.tile,
+                              (k_spmv_store<TL, Pass<T>, T, 1, 0><<<P.M.nblk, kThreads, 0, s>>>(P, v, out)));
+            } else if (S.shape.split) {
+                TILE_DISPATCH(S.shape.tile,
+                              (k_spmv_store<TL, Pass<T>, T, 1, 2><<<P.M.nblk, kThreads, 0, s>>>(P, v, out)));
             } else {
                 TILE_DISPATCH(S.shape.tile, k_spmv_store<TL><<<P.M.nblk, kThreads, 0, s>>>(P, v, out));
             }
