@@ -65,7 +65,11 @@ __global__ __launch_bounds__(kThreads) void k_spmv_store(MV P, const T* __restri
     else stream_rows<TILE>(P, v, prod, blockIdx.x, [&](int r, T s) { out[r] = s; });
 }
 
-template <int TILE, typename MV>
+// Segment passes (MV = Pass) are compiled twice, as in pdlp mode: LONG = 0
+// without the long-row code for layouts that have no block of one row longer
+// than a tile, LONG = 2 with it otherwise. Fused-segment launches (SegCsr)
+// hold no long rows.
+template <int TILE, typename MV, int LONG = 2>
 __global__ __launch_bounds__(kThreads) void k_primal_ref(MV P, const double* __restrict__ y,
                                                          double* __restrict__ x, double* __restrict__ xbar,
                                                          const double* __restrict__ c,
@@ -96,17 +100,20 @@ __global__ __launch_bounds__(kThreads) void k_primal_ref(MV P, const double* __r
         if (P.last) o = Ops{x[j], c[j], lo[j], up[j]};
         return o;
     };
-    stream_rows<TILE>(P, y, prod, blockIdx.x, pre, [&](int j, double g, const Ops& o) {
+    const auto on_row = [&](int j, double g, const Ops& o) {
         const double xj = o.x;
         double cand = xj - tau * (o.c - g);
         if (finite_bound(o.l)) cand = amax(o.l, cand);
         if (finite_bound(o.u)) cand = amin(o.u, cand);
         xbar[j] = 2.0 * cand - xj;
         x[j] = cand;
-    });
+    };
+    if constexpr (std::is_same<MV, Pass<double>>::value)
+        stream_rows<TILE, 0, LONG, LONG == 2 ? 16 : 8>(P, y, prod, blockIdx.x, pre, on_row);
+    else stream_rows<TILE>(P, y, prod, blockIdx.x, pre, on_row);
 }
 
-template <int TILE, typename MV>
+template <int TILE, typename MV, int LONG = 2>
 __global__ __launch_bounds__(kThreads) void k_dual_ref(MV P, const double* __restrict__ xbar,
                                                        double* __restrict__ y, const double* __restrict__ rl,
                                                        const double* __restrict__ ru, double sigma,
@@ -123,7 +130,7 @@ __global__ __launch_bounds__(kThreads) void k_dual_ref(MV P, const double* __res
         if (P.last) o = Ops{y[i], rl[i], ru[i]};
         return o;
     };
-    stream_rows<TILE>(P, xbar, prod, blockIdx.x, pre, [&](int i, double h, const Ops& o) {
+    const auto on_row = [&](int i, double h, const Ops& o) {
         const double yi = o.y;
         double r = h - yi / sigma;
         const double l = o.l, u = o.u;
@@ -135,7 +142,10 @@ __global__ __launch_bounds__(kThreads) void k_dual_ref(MV P, const double* __res
         if (lf) viol = amax(viol, l - h);
         if (uf) viol = amax(viol, h - u);
         vmax = amax(vmax, viol);
-    });
+    };
+    if constexpr (std::is_same<MV, Pass<double>>::value)
+        stream_rows<TILE, 0, LONG, LONG == 2 ? 16 : 8>(P, xbar, prod, blockIdx.x, pre, on_row);
+    else stream_rows<TILE>(P, xbar, prod, blockIdx.x, pre, on_row);
     if (P.last) {
         // One plain store per block; the maximum over blocks is only formed on
         // the iterations the stop rule reads it (k_vmax_reduce).
@@ -400,6 +410,9 @@ void spmv_store(const CsrShape& S, const T* val, T* part, const T* v, T* out, hi
             } else if (S.shape.split) {
                 TILE_DISPATCH(S.shape.tile,
                               (k_spmv_store<TL, Pass<T>, T, 1, 2><<<P.M.nblk, kThreads, 0, s>>>(P, v, out)));
+            } else if (S.nlong[p] == 0) {
+                TILE_DISPATCH(S.shape.tile,
+                              (k_spmv_store<TL, Pass<T>, T, 0, 0><<<P.M.nblk, kThreads, 0, s>>>(P, v, out)));
             } else {
                 TILE_DISPATCH(S.shape.tile, k_spmv_store<TL><<<P.M.nblk, kThreads, 0, s>>>(P, v, out));
             }
@@ -643,10 +656,15 @@ void launch_primal_ref(nir_session* S, int k_lo) {
     }
     for (int p = 0; p < M.shape.nseg; ++p) {
         const Pass<double> P = M.shape.pass(M.val.p, M.part.p, p);
-        TILE_DISPATCH(M.shape.shape.tile,
-                      k_primal_ref<TL><<<P.M.nblk, kThreads, 0, S->s>>>(P, S->y.p, S->x.p, S->xbar.p, S->c.p,
-                                                                         S->lo.p, S->up.p, S->tau, S->st.p, k_lo,
-                                                                         check));
+        if (M.shape.nlong[p] > 0) {
+            TILE_DISPATCH(M.shape.shape.tile,
+                          (k_primal_ref<TL, Pass<double>, 2><<<P.M.nblk, kThreads, 0, S->s>>>(
+                              P, S->y.p, S->x.p, S->xbar.p, S->c.p, S->lo.p, S->up.p, S->tau, S->st.p, k_lo, check)));
+        } else {
+            TILE_DISPATCH(M.shape.shape.tile,
+                          (k_primal_ref<TL, Pass<double>, 0><<<P.M.nblk, kThreads, 0, S->s>>>(
+                              P, S->y.p, S->x.p, S->xbar.p, S->c.p, S->lo.p, S->up.p, S->tau, S->st.p, k_lo, check)));
+        }
     }
     HIP_CHECK(hipGetLastError());
 }
@@ -662,9 +680,15 @@ void launch_dual_ref(nir_session* S, int k_lo, long long k) {
     } else {
     for (int p = 0; p < M.shape.nseg; ++p) {
         const Pass<double> P = M.shape.pass(M.val.p, M.part.p, p);
-        TILE_DISPATCH(M.shape.shape.tile,
-                      k_dual_ref<TL><<<P.M.nblk, kThreads, 0, S->s>>>(P, S->xbar.p, S->y.p, S->rl.p, S->ru.p,
-                                                                       S->sigma, S->st.p, S->vpart.p));
+        if (M.shape.nlong[p] > 0) {
+            TILE_DISPATCH(M.shape.shape.tile,
+                          (k_dual_ref<TL, Pass<double>, 2><<<P.M.nblk, kThreads, 0, S->s>>>(
+                              P, S->xbar.p, S->y.p, S->rl.p, S->ru.p, S->sigma, S->st.p, S->vpart.p)));
+        } else {
+            TILE_DISPATCH(M.shape.shape.tile,
+                          (k_dual_ref<TL, Pass<double>, 0><<<P.M.nblk, kThreads, 0, S->s>>>(
+                              P, S->xbar.p, S->y.p, S->rl.p, S->ru.p, S->sigma, S->st.p, S->vpart.p)));
+        }
     }
     }
     // The stop rule reads the violation of iterations with k % 32 == 0.

@@ -105,7 +105,7 @@ __global__ __launch_bounds__(kThreads) void k_primal_pdlp(Pass<T> P, const T* __
         if (P.last) o = Ops{x[j], c[j], lo[j], up[j], kDelta ? T(0) : xa[j]};
         return o;
     };
-    stream_rows<TILE, 1, LONG>(P, y, prod, blockIdx.x, pre, [&](int j, T g, const Ops& o) {
+    stream_rows<TILE, 1, LONG, LONG == 2 ? 16 : 8>(P, y, prod, blockIdx.x, pre, [&](int j, T g, const Ops& o) {
         const PrimalOut<T> r = primal_point<T>(o.x, g, o.c, o.l, o.u, o.a, tau, hw);
         xbar[j] = r.xb;
         x[j] = r.next;
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(kThreads) void k_dual_pdlp(Pass<T> P, const T* __re
     };
     // FP32 runs the re-centred form; `ya` then holds sa = -Y / sigma.
     constexpr bool kDelta = std::is_same<T, float>::value;
-    stream_rows<TILE, 1, LONG>(P, xbar, prod, blockIdx.x, pre, [&](int i, T h, const Ops& o) {
+    stream_rows<TILE, 1, LONG, LONG == 2 ? 16 : 8>(P, xbar, prod, blockIdx.x, pre, [&](int i, T h, const Ops& o) {
         const DualOut<T> r = kDelta ? dual_point_delta<T>(o.y, h, o.l, o.u, o.a, sigma, hw)
                                     : dual_point<T>(o.y, h, o.l, o.u, o.a, sigma, hw);
         y[i] = r.next;

@@ -99,7 +99,10 @@ constexpr int kGroupLen = kChunk * kGroupChunks;   // products per group (2048)
 // (the long-row code is then not compiled in, so it adds nothing to the
 // kernel's register count; blocks of one row longer than a tile and pieces of
 // split rows return at once).
-template <int TILE, int ORDER = 0, int LONG = 2, typename T, typename V, typename Pre, typename F>
+// SEQ: products read ahead of the adds in the one-lane sequential sum of a
+// long row (16 in the iteration kernels compiled for layouts with long rows,
+// 8 elsewhere, where the registers would cost every layout).
+template <int TILE, int ORDER = 0, int LONG = 2, int SEQ = 8, typename T, typename V, typename Pre, typename F>
 __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restrict__ v, T* prod, int b,
                                             Pre&& prefetch, F&& on_row) {
     static_assert(TILE % kThreads == 0, "tile must be a multiple of the workgroup");
@@ -204,10 +207,7 @@ __device__ __forceinline__ void stream_rows(const Pass<T>& P, const V* __restric
                 // pace. (Batches of 16 would hide more of the latency but
                 // raise the whole kernel's VGPR count - 74 to 118 for the
                 // 2048 tile - and cut the occupancy of the short-row blocks.)
-                // pdlp kernels compile this branch only for layouts that have
-                // long rows, so they read 16 ahead; reference mode's kernels
-                // (one compilation for every layout) read 8 ahead.
-                constexpr int B = ORDER == 1 ? 16 : 8;
+                constexpr int B = SEQ;
                 const int e = p1 - c0 < TILE ? p1 - c0 : TILE;
                 int q = 0;
                 if (e >= B) {
