@@ -21,7 +21,8 @@ Other workloads: --workload small-lp (the reference's CPU-runnable 2,000 x
 5,000 case, GPU and CPU times to 1e-6), --workload mixed (mixed precision on
 the same instance),
 --workload planning512 (512 LPs in the batched persistent engine),
---workload warmstart (learned warm start on a parametric family) and
+--workload warmstart (learned warm start on a parametric family),
+--workload pdlp-fp64 (the restarted engine in FP64 on the large LP) and
 --workload dense-row (iteration time on LPs with one dense row).
 """
 import argparse
@@ -498,6 +499,50 @@ def mixed_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def pdlp_fp64_workload(args):
+    """Plain FP64 GPU PDHG - the restarted engine in FP64, one instance, cold
+    start, the baseline the headline metric's time-to-tolerance ratios are
+    taken against - on the 400k x 1.2M instance: its iteration rate over
+    `steps` iterations of the production path (graph replays, evaluation
+    blocks included; tolerance out of reach), and the roofline of its FP64
+    A xbar kernel (k_dual_pdlp<double>) from the events sweep."""
+    from nirnaya_amd.engine import Engine, MODE_PDLP, default_params, gen_mixed_lp
+    lp = gen_mixed_lp(SEED, 400_000, 1_200_000, 6.0, 0.01, 1e3)
+    eng = Engine(0)
+    p = default_params(mode=MODE_PDLP, precision=0, max_iters=10 ** 9, tol=1e-14, tile_nnz=args.tile,
+                       tile_rows=args.tile_rows, seg_width=args.seg, seg_width_t=args.seg_t)
+    sess = eng.session(lp, p)
+    shape = sess.shape()
+    log(f"[bench] FP64 pdlp session ready in {sess.result().info.setup_seconds:.2f}s; shape A x: tile {shape[0]} "
+        f"segments {shape[2]}, A^T y: tile {shape[4]} segments {shape[6]}")
+    sess.run(args.warmup)
+    t0 = time.perf_counter()
+    sess.run(args.steps)
+    wall = time.perf_counter() - t0
+    t = sess.time(args.steps)
+    sess.close()
+    eng.close()
+    it_s = args.steps / wall
+    spmv_avg_s = t.spmv_ms / t.iterations / 1e3
+    spmtv_avg_s = t.spmtv_ms / t.iterations / 1e3
+    achieved = t.spmv_bytes / spmv_avg_s / 1e9
+    log(f"[bench] FP64 pdlp: {it_s:.0f} it/s; A xbar {spmv_avg_s * 1e6:.1f} us ({achieved:.0f} GB/s), "
+        f"A^T y {spmtv_avg_s * 1e6:.1f} us")
+    print(json.dumps({
+        "metric": "restarted FP64 PDHG iterations/s (plain FP64 GPU PDHG)", "value": round(it_s, 2),
+        "unit": "PDHG iterations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"single large LP: m=400000, n=1200000, nnz={lp.nnz}, seed {SEED}, pdlp mode FP64",
+                   "shape": list(shape)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "kernel": "k_dual_pdlp<double> (FP64 A xbar + dual step); one launch = its segment passes",
+                     "bytes_per_launch": t.spmv_bytes, "avg_launch_us": spmv_avg_s * 1e6,
+                     "launches_averaged": t.iterations, "dispatches_per_launch": t.spmv_passes},
+        "kernels": {"spmtv_primal_avg_us": spmtv_avg_s * 1e6, "spmtv_bytes_per_launch": t.spmtv_bytes}}),
+        flush=True)
+
+
 def warmstart_workload(args):
     """Learned warm start on the parametric family (fixed A as the 2,000 x
     5,000 equality LP scaled to m x n, 5 nonzeros per column; b = b0 (1 + 0.1
@@ -835,8 +880,8 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the 256 MiB copy kernel (reference for rocprofv3 byte counters)")
     ap.add_argument("--workload", default="large-lp",
-                    choices=["large-lp", "small-lp", "mixed", "planning512", "warmstart", "family", "gemm",
-                             "dense-row"],
+                    choices=["large-lp", "small-lp", "mixed", "pdlp-fp64", "planning512", "warmstart", "family",
+                             "gemm", "dense-row"],
                     help="large-lp: the headline line (default); small-lp: the reference's CPU-runnable "
                          "2,000 x 5,000 case; mixed: mixed precision on the large instance; "
                          "planning512: the batched workload; warmstart: the learned warm start on a parametric "
@@ -879,6 +924,9 @@ def main():
         return
     if args.workload == "dense-row":
         dense_row_workload(args)
+        return
+    if args.workload == "pdlp-fp64":
+        pdlp_fp64_workload(args)
         return
     if args.workload == "small-lp":
         small_lp_workload(args)

@@ -55,7 +55,8 @@ def test_default_run_prints_the_contract_line():
     ["--workload", "mixed", "--steps", "16", "--warmup", "4", "--no-ttt", "--no-cpu-baseline"],
     ["--workload", "family", "--steps", "64", "--fam-lanes", "128", "--ws-m", "5000", "--ws-n", "12000"],
     ["--workload", "dense-row", "--steps", "64", "--warmup", "8", "--dense-len", "0,20000"],
-], ids=["mixed", "family", "dense-row"])
+    ["--workload", "pdlp-fp64", "--steps", "64", "--warmup", "8"],
+], ids=["mixed", "family", "dense-row", "pdlp-fp64"])
 def test_other_workloads_print_one_line(argv):
     # The secondary workloads' lines (mixed precision on the large LP, the
     # family engine on a small family, the dense-row LPs) parse and carry a
